@@ -1,0 +1,308 @@
+/*
+ * atq.h - C ABI of the MI355X AprilTag quad front-end (threshold -> quads).
+ *
+ * Every entry point is `extern "C"`, takes plain device pointers and sizes,
+ * returns an int status (ATQ_OK = 0) and never throws. `stream` is a
+ * hipStream_t passed as `void*` (NULL = the default stream); all work is
+ * ordered on it. Entry points that mirror a reference stage name the
+ * reference declaration they replace (paths relative to the reference repo
+ * carbotaniuman/sycl-apriltag-test).
+ *
+ * Record types are layout-identical to the reference's (static_asserts in
+ * the library). The one widening: cluster indices are uint32_t instead of
+ * uint16_t (the reference's u16 index and its 65535 sentinel cap the frame at
+ * 65535 clusters, src/main.cpp:639-651, include/fit_quad.h:23). `atq_corner`
+ * keeps its 12-byte size because the reference struct already pads the u16
+ * to 4 bytes.
+ */
+#ifndef ATQ_H
+#define ATQ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    ATQ_OK = 0,
+    ATQ_ERR_BAD_DIMS = 1,  /* width/height not multiples of 4, < 8 or > 8192 */
+    ATQ_ERR_CAPACITY = 2,  /* a count exceeded the buffer it must fit in     */
+    ATQ_ERR_HIP = 3,       /* a HIP runtime call failed                      */
+    ATQ_ERR_BAD_ARG = 4,   /* NULL pointer, bad factor, too-small workspace  */
+    ATQ_ERR_NO_DEVICE = 5  /* no gfx950 device visible                       */
+};
+
+/* ---- records (layout-identical to the reference) ------------------------ */
+
+/* sycl::vec<uint8_t,2> of tile {min,max}: include/threshold.h:7-8 */
+typedef struct atq_extrema {
+    uint8_t min, max;
+} atq_extrema;
+
+/* include/boundary_point.h:42-51. packed_x = [half_pixel:2 | x:14],
+ * packed_y = [black_to_white:1 | unused:1 | y:14] */
+typedef struct atq_boundary_point {
+    uint16_t packed_x, packed_y;
+} atq_boundary_point;
+
+/* include/boundary_point.h:100-105 */
+typedef struct atq_cluster_point {
+    uint16_t packed_x, packed_y;
+    float slope;
+} atq_cluster_point;
+
+/* include/cluster_bounds.h:9-12 */
+typedef struct atq_cluster_extents {
+    uint32_t start, count;
+} atq_cluster_extents;
+
+/* include/cluster_bounds.h:14-26 */
+typedef struct atq_cluster_bounds {
+    uint16_t x_min, x_max, y_min, y_max;
+    int32_t gx_sum, gy_sum;
+    int64_t pxgx_pygy_sum;
+    uint32_t start, count;
+} atq_cluster_bounds;
+
+/* include/line_fit_point.h:11-17 (Mx,My doubled; Mxx,Myy,Mxy quadrupled) */
+typedef struct atq_line_fit_point {
+    int64_t Mx, My, Mxx, Myy, Mxy, W, a, b;
+} atq_line_fit_point;
+
+/* include/fit_quad.h:9-12 */
+typedef struct atq_peak_extents {
+    uint32_t start, count;
+} atq_peak_extents;
+
+/* include/fit_quad.h:21-25, cluster_index widened u16 -> u32 (same size) */
+typedef struct atq_corner {
+    uint32_t line_fit_point_index;
+    uint32_t cluster_index;
+    float error;
+} atq_corner;
+
+/* include/fit_quad.h:27-31 */
+typedef struct atq_fitted_quad {
+    atq_line_fit_point moments[4];
+    uint16_t num_in_moments[4];
+    uint16_t indices[4];
+} atq_fitted_quad;
+
+/* include/fit_quad.h:33-37. blob_index (left unset by the reference,
+ * src/main.cpp:1049) carries the peak-segment index the quad came from. */
+typedef struct atq_quad_corners {
+    float corners[4][2];
+    uint8_t reversed_border;
+    uint8_t pad_[3];
+    uint32_t blob_index;
+} atq_quad_corners;
+
+/* Compile-time constants of the reference, in one struct (defaults equal
+ * the reference; only the compat values are supported for parity). */
+typedef struct atq_params {
+    int32_t decimate;            /* A0, point subsample; 1 = none           */
+    int32_t min_white_black_diff;/* src/threshold.cpp:86 (5)                */
+    int32_t min_component_size;  /* src/find_boundaries.cpp:82 (25)         */
+    int32_t min_tag_width;       /* src/main.cpp:1037 (8)                   */
+} atq_params;
+
+/* Per-frame counts written by the pipeline (device memory, 16 x u32). */
+typedef struct atq_counts {
+    uint32_t n_boundary;      /* N_bp: boundary records                    */
+    uint32_t n_pairs;         /* cluster-pair segments (D3)                */
+    uint32_t n_clusters;      /* clusters passing ValidBlobFilter (D4)     */
+    uint32_t n_cluster_points;/* N_fp: points in valid clusters (D5)       */
+    uint32_t n_corners;       /* compacted corners (D11)                   */
+    uint32_t n_segments;      /* peak segments E (D13)                     */
+    uint32_t n_quads;         /* quads surviving the host checks (A21)     */
+    uint32_t overflow;        /* bit flags: capacity exceeded somewhere    */
+    uint32_t oob_reads;       /* Q13 reads past the moment array (counted) */
+    uint32_t reserved[7];
+} atq_counts;
+
+/* ---- library ------------------------------------------------------------ */
+
+/* Checks that device 0 (or the current device) is gfx950 and records it. */
+int atq_init(void);
+/* Human-readable text of a status code. */
+const char *atq_status_string(int status);
+
+/* ---- stage drop-ins ----------------------------------------------------- */
+
+/* A0: point-subsample decimation out[y][x] = in[f*y][f*x],
+ * out dims 1+(w-1)/f x 1+(h-1)/f. No reference counterpart (notes.txt:18-21
+ * says decimation is "Ignored for now"); follows upstream apriltag's
+ * image_u8_decimate for integer factors. */
+int atq_decimate(void *stream, const uint8_t *gray_in, size_t width_in,
+                 size_t height_in, int factor, uint8_t *gray_out);
+
+/* Replaces threshold_image(q, grayscale, extrema, filtered_extrema,
+ * thresholded, width, height, deps)  include/threshold.h:6-10.
+ * Writes all three outputs like the reference (src/threshold.cpp:8-95). */
+int atq_threshold_image(void *stream, const uint8_t *grayscale,
+                        atq_extrema *extrema, atq_extrema *filtered_extrema,
+                        uint8_t *thresholded, size_t width, size_t height);
+
+/* Fused A0+A1..A3: decimate by `factor` (1 = none) and threshold in one
+ * pass. gray_out receives the decimated image when factor > 1 (it feeds the
+ * line-fit weights) and may be NULL when factor == 1. extrema and
+ * filtered_extrema may be NULL. */
+int atq_decimate_threshold(void *stream, const uint8_t *gray_in,
+                           size_t width_in, size_t height_in, int factor,
+                           uint8_t *gray_out, atq_extrema *extrema,
+                           atq_extrema *filtered_extrema,
+                           uint8_t *thresholded);
+
+/* Replaces image_segmentation(q, thresholded, label_scratch, labels, sizes,
+ * sizes_elem, width, height, deps)  include/segmentation.h:11-15.
+ * labels: white = 0x80000000|root, black = root, 127 -> 0 (Q4 of the
+ * reference, src/segmentation.cpp:348-388). sizes (width*height entries) is
+ * written at every label root and at index 0; other entries are not read by
+ * any stage and are left as they were. sizes_elem is accepted for signature
+ * parity and, as in the reference, unused. label_scratch (width*height u32)
+ * is workspace. */
+int atq_image_segmentation(void *stream, const uint8_t *thresholded,
+                           uint32_t *label_scratch, uint32_t *labels,
+                           uint32_t *sizes, size_t sizes_elem, size_t width,
+                           size_t height);
+
+/* Replaces find_boundaries(q, labels, sizes, points, blob_labels, width,
+ * height, deps)  include/find_boundaries.h:8-12. Writes the 4 sparse planes
+ * of 4*width*height entries exactly as the reference leaves them after its
+ * pre-fill (points 0, keys all-ones where nothing is emitted,
+ * src/main.cpp:312-315); the pre-fill is done here. */
+int atq_find_boundaries(void *stream, const uint32_t *labels,
+                        const uint32_t *sizes, atq_boundary_point *points,
+                        uint64_t *blob_labels, size_t width, size_t height);
+
+/* ---- glue between stages (inline oneDPL calls in src/main.cpp) ---------- */
+
+/* Bytes of workspace the glue entry points below need for a frame of at
+ * most width x height (decimated) pixels. */
+size_t atq_glue_workspace_size(size_t width, size_t height);
+
+/* D1+D2 (src/main.cpp:467-477, 515-518): compact the sparse planes (drop
+ * points whose u32 image is 0) and group by cluster-pair key ascending;
+ * within a key the order is plane-major raster, i.e. a stable sort.
+ * *count_dev (device u32) receives N_bp. Outputs hold 4*width*height. */
+int atq_group_boundaries(void *stream, const atq_boundary_point *points,
+                         const uint64_t *blob_labels, size_t width,
+                         size_t height, atq_boundary_point *out_points,
+                         uint64_t *out_keys, uint32_t *count_dev,
+                         void *workspace, size_t workspace_bytes);
+
+/* D3+D4 (src/main.cpp:571-609): per key segment ClusterBounds
+ * (include/cluster_bounds.h:28-83) then ValidBlobFilter(width, height)
+ * (include/cluster_bounds.h:85-120). n_points is read from device memory.
+ * out_bounds receives the filtered bounds, counts->n_pairs/n_clusters set. */
+int atq_cluster_bounds_filter(void *stream, const atq_boundary_point *points,
+                              const uint64_t *keys, const uint32_t *n_points_dev,
+                              size_t width, size_t height,
+                              atq_cluster_bounds *out_bounds,
+                              atq_counts *counts_dev, void *workspace,
+                              size_t workspace_bytes);
+
+/* D5..D8 (src/main.cpp:621-750): points of valid clusters with their
+ * calc_theta slope and cluster index, cluster extents, sorted by (cluster,
+ * slope) stably. */
+int atq_cluster_points_sort(void *stream, const atq_boundary_point *points,
+                            const uint32_t *n_points_dev,
+                            const atq_cluster_bounds *bounds,
+                            atq_counts *counts_dev,
+                            atq_cluster_point *out_points,
+                            uint32_t *out_cluster_index,
+                            atq_cluster_extents *out_extents, void *workspace,
+                            size_t workspace_bytes);
+
+/* D9+D10 (src/main.cpp:768-801): compute_initial_linefit
+ * (include/line_fit_point.h:50-86) and the per-cluster inclusive prefix sum
+ * of the moments (exact int64). */
+int atq_linefit_prefix(void *stream, const atq_cluster_point *points,
+                       const uint32_t *cluster_index,
+                       const atq_counts *counts_dev, const uint8_t *grayscale,
+                       size_t width, size_t height, atq_line_fit_point *out);
+
+/* Replaces fit_lines(q, points, cluster_indices, cluster_extents,
+ * points_size, found_corners, deps)  include/fit_quad.h:46-50.
+ * Blocking like the reference (src/fit_quad.cpp:374). found_corners
+ * (points_size entries) is zero-filled here, then corners are written. */
+int atq_fit_lines(void *stream, const atq_line_fit_point *points,
+                  const uint32_t *cluster_indices,
+                  const atq_cluster_extents *cluster_extents,
+                  size_t points_size, atq_corner *found_corners);
+
+/* D11..D13 (src/main.cpp:832-907): compact corners with error != 0, sort by
+ * (cluster asc, error desc) stably, segment per cluster. */
+int atq_corner_segments(void *stream, const atq_corner *found_corners,
+                        atq_counts *counts_dev, atq_corner *compacted,
+                        atq_peak_extents *segments, void *workspace,
+                        size_t workspace_bytes);
+
+/* Replaces do_indexing(q, extents, extents_count, compacted_corners,
+ * points, cluster_extents, fitted)  include/fit_quad.h:52-55.
+ * Blocking like the reference (src/fit_quad.cpp:534). fitted
+ * (extents_count entries) is zero-filled here first. n_points bounds the
+ * moment array (reads past it, possible through Q13, read zeros). */
+int atq_do_indexing(void *stream, const atq_peak_extents *extents,
+                    size_t extents_count, const atq_corner *compacted_corners,
+                    const atq_line_fit_point *points, size_t n_points,
+                    const atq_cluster_extents *cluster_extents,
+                    atq_fitted_quad *fitted);
+
+/* A21 process_fitted_quads (src/main.cpp:1035-1162) minus tag decoding:
+ * refit, intersect, area/angle/convexity checks; survivors are written in
+ * input order. *n_quads_dev (device u32) receives the count. */
+int atq_quads_from_fits(void *stream, const atq_fitted_quad *fitted,
+                        size_t n_fitted, atq_quad_corners *out,
+                        uint32_t *n_quads_dev);
+
+/* ---- whole pipeline ----------------------------------------------------- */
+
+typedef struct atq_detector atq_detector;
+
+/* Allocates every buffer for frames up to max_width_in x max_height_in
+ * (before decimation) once; atq_detect_quads then never allocates. */
+int atq_detector_create(size_t max_width_in, size_t max_height_in,
+                        const atq_params *params, atq_detector **out);
+int atq_detector_destroy(atq_detector *det);
+
+/* threshold -> quads for one frame already in device memory. Fully
+ * stream-ordered: no host synchronisation. Quads land in the detector
+ * (atq_detector_buffer ATQ_BUF_QUADS) and counts in ATQ_BUF_COUNTS. */
+int atq_detect_quads(atq_detector *det, void *stream, const uint8_t *gray_in,
+                     size_t width_in, size_t height_in);
+
+enum {
+    ATQ_BUF_GRAY = 0,        /* u8  decimated grey (== input when f=1)   */
+    ATQ_BUF_THRESHOLDED = 1, /* u8  W*H                                  */
+    ATQ_BUF_LABELS = 2,      /* u32 W*H                                  */
+    ATQ_BUF_SIZES = 3,       /* u32 W*H                                  */
+    ATQ_BUF_BOUNDARY_POINTS = 4, /* atq_boundary_point, grouped by key   */
+    ATQ_BUF_BOUNDARY_KEYS = 5,   /* u64                                  */
+    ATQ_BUF_CLUSTER_BOUNDS = 6,  /* filtered atq_cluster_bounds          */
+    ATQ_BUF_CLUSTER_POINTS = 7,  /* atq_cluster_point, angle-sorted      */
+    ATQ_BUF_CLUSTER_INDEX = 8,   /* u32                                  */
+    ATQ_BUF_CLUSTER_EXTENTS = 9, /* atq_cluster_extents                  */
+    ATQ_BUF_LINE_FIT = 10,       /* atq_line_fit_point prefix sums       */
+    ATQ_BUF_CORNERS = 11,        /* atq_corner compacted + sorted        */
+    ATQ_BUF_PEAK_EXTENTS = 12,   /* atq_peak_extents                     */
+    ATQ_BUF_FITTED = 13,         /* atq_fitted_quad                      */
+    ATQ_BUF_QUADS = 14,          /* atq_quad_corners                     */
+    ATQ_BUF_COUNTS = 15,         /* atq_counts                           */
+    ATQ_BUF_EXTREMA = 16,        /* atq_extrema (W/4)*(H/4)              */
+    ATQ_BUF_FILTERED_EXTREMA = 17,
+    ATQ_BUF_SPARSE_CORNERS = 18  /* atq_corner, one slot per cluster point */
+};
+/* Device pointer and capacity (in elements) of a detector buffer. */
+int atq_detector_buffer(atq_detector *det, int which, void **ptr,
+                        size_t *capacity);
+/* Decimated dims of the last frame. */
+int atq_detector_dims(atq_detector *det, size_t *width, size_t *height);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ATQ_H */

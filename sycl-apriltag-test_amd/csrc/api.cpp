@@ -1,0 +1,106 @@
+// api.cpp - the C ABI (include/atq.h): argument checks, stage drop-ins.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "kernels.hpp"
+#include "pipeline.hpp"
+
+using namespace atq;
+
+namespace {
+
+bool g_ready = false;
+
+int check_dims(size_t w, size_t h) {
+    if (w < 8 || h < 8 || w > 8192 || h > 8192 || (w & 3) || (h & 3)) return ATQ_ERR_BAD_DIMS;
+    return ATQ_OK;
+}
+
+int ensure_init() {
+    if (g_ready) return ATQ_OK;
+    return atq_init();
+}
+
+hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace
+
+extern "C" {
+
+int atq_init(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return ATQ_ERR_NO_DEVICE;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return ATQ_ERR_NO_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return ATQ_ERR_HIP;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return ATQ_ERR_NO_DEVICE;
+    g_ready = true;
+    return ATQ_OK;
+}
+
+const char *atq_status_string(int status) {
+    switch (status) {
+    case ATQ_OK: return "ok";
+    case ATQ_ERR_BAD_DIMS: return "width/height must be multiples of 4 in [8, 8192]";
+    case ATQ_ERR_CAPACITY: return "a count exceeded its buffer capacity";
+    case ATQ_ERR_HIP: return "HIP runtime error";
+    case ATQ_ERR_BAD_ARG: return "bad argument";
+    case ATQ_ERR_NO_DEVICE: return "no gfx950 device";
+    default: return "unknown status";
+    }
+}
+
+int atq_decimate(void *stream, const uint8_t *gray_in, size_t width_in, size_t height_in,
+                 int factor, uint8_t *gray_out) {
+    if (!gray_in || !gray_out || factor < 1 || width_in == 0 || height_in == 0) return ATQ_ERR_BAD_ARG;
+    if (int r = ensure_init()) return r;
+    return launch_decimate(S(stream), gray_in, width_in, height_in, factor, gray_out);
+}
+
+int atq_threshold_image(void *stream, const uint8_t *grayscale, atq_extrema *extrema,
+                        atq_extrema *filtered_extrema, uint8_t *thresholded, size_t width,
+                        size_t height) {
+    if (!grayscale || !thresholded) return ATQ_ERR_BAD_ARG;
+    if (int r = check_dims(width, height)) return r;
+    if (int r = ensure_init()) return r;
+    return launch_threshold(S(stream), grayscale, width, height, 1, nullptr, extrema,
+                            filtered_extrema, thresholded, 5);
+}
+
+int atq_decimate_threshold(void *stream, const uint8_t *gray_in, size_t width_in,
+                           size_t height_in, int factor, uint8_t *gray_out, atq_extrema *extrema,
+                           atq_extrema *filtered_extrema, uint8_t *thresholded) {
+    if (!gray_in || !thresholded || factor < 1 || width_in == 0 || height_in == 0)
+        return ATQ_ERR_BAD_ARG;
+    if (factor > 1 && !gray_out) return ATQ_ERR_BAD_ARG;
+    size_t w = factor > 1 ? 1 + (width_in - 1) / factor : width_in;
+    size_t h = factor > 1 ? 1 + (height_in - 1) / factor : height_in;
+    if (int r = check_dims(w, h)) return r;
+    if (int r = ensure_init()) return r;
+    return launch_threshold(S(stream), gray_in, width_in, height_in, factor, gray_out, extrema,
+                            filtered_extrema, thresholded, 5);
+}
+
+int atq_image_segmentation(void *stream, const uint8_t *thresholded, uint32_t *label_scratch,
+                           uint32_t *labels, uint32_t *sizes, size_t sizes_elem, size_t width,
+                           size_t height) {
+    (void)sizes_elem;  // unused, as in the reference (src/segmentation.cpp:97)
+    if (!thresholded || !label_scratch || !labels || !sizes) return ATQ_ERR_BAD_ARG;
+    if (int r = check_dims(width, height)) return r;
+    if (int r = ensure_init()) return r;
+    return launch_ccl(S(stream), thresholded, width, height, label_scratch, labels, sizes);
+}
+
+int atq_find_boundaries(void *stream, const uint32_t *labels, const uint32_t *sizes,
+                        atq_boundary_point *points, uint64_t *blob_labels, size_t width,
+                        size_t height) {
+    if (!labels || !sizes || !points || !blob_labels) return ATQ_ERR_BAD_ARG;
+    if (int r = check_dims(width, height)) return r;
+    if (int r = ensure_init()) return r;
+    return launch_find_boundaries_sparse(S(stream), labels, sizes, width, height, points,
+                                         blob_labels, 25);
+}
+
+}  // extern "C"

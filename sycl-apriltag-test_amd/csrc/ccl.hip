@@ -1,0 +1,278 @@
+// ccl.hip - block-based union-find connected-component labelling (rows A4..A6).
+//
+// The graph is the reference's (src/segmentation.cpp:101-306): one node per
+// (2x2 block, colour), colour 255 ("white", node id = the block's top-left
+// pixel index) and colour 0 ("black", id + 1). For each block, the pixels
+// P(-1,-1), Q(-1,0), Q(-1,1), R(-1,2), S(0,-1), S(1,-1) are tested when the
+// block's neighbour mask (built from its non-127 pixels, colour-agnostic,
+// 8-connected for both colours) selects them; a tested pixel of colour c
+// links the block's colour-c node to the colour-c node of the block holding
+// it. Components are labelled by their minimum node id (min-linking).
+//
+// gfx950 organisation (three kernels instead of the reference's five):
+//  1. k_ccl_local: a work-group owns a TBX x TBY tile of blocks, stages its
+//     pixels (+1 px halo) in LDS, unions the tile-internal edges with an LDS
+//     union-find, counts each local root's pixels, and writes for every node
+//     the global id of its local root (FLAG-tagged for non-roots) and, at
+//     each local root, its pixel count into `sizes`.
+//  2. k_ccl_border: edges that cross tiles are merged on the global parent
+//     array with the reference's lock-free atomicMin union
+//     (include/union_find.h:50-77).
+//  3. k_ccl_final: each block resolves its two roots, adds each local root's
+//     count into its global root's size, and writes the four label words
+//     (white = 0x80000000|root, black = root, 127 -> 0).
+#include "kernels.hpp"
+
+namespace atq {
+
+namespace {
+
+constexpr int TBX = 32;  // blocks per tile row    (64 px)
+constexpr int TBY = 16;  // block rows per tile    (32 px)
+constexpr int NODES = 2 * TBX * TBY;
+constexpr uint32_t FLAG = 0x80000000u;  // "not a local root" tag on parent words
+
+constexpr uint8_t C_WHITE = 255, C_BLACK = 0;
+
+// Neighbour mask of a block (bits of the 4x4 grid around it, bit 5 = TL),
+// src/segmentation.cpp:137-186.
+ATQ_DEV uint32_t check_mask(uint8_t tl, uint8_t tr, uint8_t bl, uint32_t x, uint32_t y, uint32_t w,
+                            uint32_t h) {
+    uint32_t m = 0;
+    if (tl == 255 || tl == 0) m |= 0x777u;
+    if (tr == 255 || tr == 0) m |= 0x777u << 1;
+    if (bl == 255 || bl == 0) m |= 0x777u << 4;
+    if (x == 0) m &= 0xEEEEu;
+    else if (x + 2 >= w) m &= 0x7777u;
+    if (y == 0) m &= 0xFFF0u;
+    else if (y + 2 >= h) m &= 0x0FFFu;
+    return m;
+}
+
+// The six tested pixels: (dy, dx) offset from the block's TL pixel, mask
+// bit, and the neighbour block holding the pixel (dby, dbx).
+struct Probe { int8_t dy, dx, bit, dby, dbx; };
+__constant__ Probe PROBES[6] = {
+    {-1, -1, 0, -1, -1},  // P
+    {-1, 0, 1, -1, 0},    // Q
+    {-1, 1, 2, -1, 0},    // Q
+    {-1, 2, 3, -1, 1},    // R
+    {0, -1, 4, 0, -1},    // S
+    {1, -1, 8, 0, -1},    // S
+};
+
+ATQ_DEV uint32_t lds_find(const uint32_t *lp, uint32_t n) {
+    while (true) {
+        uint32_t p = lp[n];
+        if (p == n) return n;
+        n = p;
+    }
+}
+
+ATQ_DEV void lds_union(uint32_t *lp, uint32_t a, uint32_t b) {
+    while (true) {
+        a = lds_find(lp, a);
+        b = lds_find(lp, b);
+        if (a < b) {
+            uint32_t old = atomicMin(&lp[b], a);
+            if (old == b) return;
+            b = old;
+        } else if (b < a) {
+            uint32_t old = atomicMin(&lp[a], b);
+            if (old == a) return;
+            a = old;
+        } else {
+            return;
+        }
+    }
+}
+
+ATQ_DEV uint32_t g_load(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+ATQ_DEV uint32_t g_find(const uint32_t *parent, uint32_t n) {
+    while (true) {
+        uint32_t p = g_load(&parent[n]) & ~FLAG;
+        if (p == n) return n;
+        n = p;
+    }
+}
+
+ATQ_DEV void g_union(uint32_t *parent, uint32_t a, uint32_t b) {
+    while (true) {
+        a = g_find(parent, a);
+        b = g_find(parent, b);
+        if (a < b) {
+            uint32_t old = atomicMin(&parent[b], a);
+            if (old == b) return;
+            b = old;
+        } else if (b < a) {
+            uint32_t old = atomicMin(&parent[a], b);
+            if (old == a) return;
+            a = old;
+        } else {
+            return;
+        }
+    }
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(256)
+k_ccl_local(const uint8_t *__restrict__ thr, uint32_t w, uint32_t h, uint32_t *__restrict__ parent,
+            uint32_t *__restrict__ sizes) {
+    constexpr int PR = 2 * TBY + 1;      // pixel rows incl. the halo row above
+    constexpr int PC = 2 * TBX + 2;      // pixel cols incl. left halo and R column
+    __shared__ uint8_t px[PR][PC + 2];
+    __shared__ uint32_t lp[NODES];
+    __shared__ uint32_t cnt[NODES];
+
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * (2 * TBX), y0 = blockIdx.y * (2 * TBY);
+
+    for (int i = tid; i < PR * PC; i += 256) {
+        int r = i / PC, c = i % PC;
+        int y = y0 - 1 + r, x = x0 - 1 + c;
+        uint8_t v = 127;
+        if (y >= 0 && y < (int)h && x >= 0 && x < (int)w) v = thr[(size_t)y * w + x];
+        px[r][c] = v;
+    }
+    for (int i = tid; i < NODES; i += 256) {
+        lp[i] = i;
+        cnt[i] = 0;
+    }
+    __syncthreads();
+
+    // tile-internal unions
+    for (int b = tid; b < TBX * TBY; b += 256) {
+        int bx = b % TBX, by = b / TBX;
+        int x = x0 + 2 * bx, y = y0 + 2 * by;
+        if (x >= (int)w || y >= (int)h) continue;
+        const int pr = 1 + 2 * by, pc = 1 + 2 * bx;
+        uint32_t m = check_mask(px[pr][pc], px[pr][pc + 1], px[pr + 1][pc], x, y, w, h);
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            Probe q = PROBES[k];
+            if (!(m & (1u << q.bit))) continue;
+            uint8_t v = px[pr + q.dy][pc + q.dx];
+            if (v != C_WHITE && v != C_BLACK) continue;
+            int nbx = bx + q.dbx, nby = by + q.dby;
+            if (nbx < 0 || nbx >= TBX || nby < 0) continue;  // crosses the tile
+            uint32_t c = v == C_WHITE ? 0 : 1;
+            lds_union(lp, 2 * (by * TBX + bx) + c, 2 * (nby * TBX + nbx) + c);
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < NODES; i += 256) lp[i] = lds_find(lp, i);
+    __syncthreads();
+    // pixel counts per local root
+    for (int b = tid; b < TBX * TBY; b += 256) {
+        int bx = b % TBX, by = b / TBX;
+        int x = x0 + 2 * bx, y = y0 + 2 * by;
+        if (x >= (int)w || y >= (int)h) continue;
+        const int pr = 1 + 2 * by, pc = 1 + 2 * bx;
+        uint8_t p4[4] = {px[pr][pc], px[pr][pc + 1], px[pr + 1][pc], px[pr + 1][pc + 1]};
+        uint32_t nw = 0, nb = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            nw += p4[k] == C_WHITE;
+            nb += p4[k] == C_BLACK;
+        }
+        if (nw) atomicAdd(&cnt[lp[2 * b]], nw);
+        if (nb) atomicAdd(&cnt[lp[2 * b + 1]], nb);
+    }
+    __syncthreads();
+    // publish: parent word per node, size per local root
+    for (int n = tid; n < NODES; n += 256) {
+        int b = n >> 1, c = n & 1;
+        int bx = b % TBX, by = b / TBX;
+        int x = x0 + 2 * bx, y = y0 + 2 * by;
+        if (x >= (int)w || y >= (int)h) continue;
+        uint32_t r = lp[n];
+        int rb = r >> 1;
+        uint32_t gr = (uint32_t)(y0 + 2 * (rb / TBX)) * w + (uint32_t)(x0 + 2 * (rb % TBX)) + (r & 1);
+        uint32_t gn = (uint32_t)y * w + (uint32_t)x + c;
+        parent[gn] = r == (uint32_t)n ? gr : (gr | FLAG);
+        if (r == (uint32_t)n) sizes[gn] = cnt[n];
+    }
+}
+
+// One item per (tile, border block); merges the block's tile-crossing edges.
+__global__ void __launch_bounds__(256)
+k_ccl_border(const uint8_t *__restrict__ thr, uint32_t w, uint32_t h, uint32_t tiles_x,
+             uint32_t n_items, uint32_t *__restrict__ parent) {
+    constexpr int PER_TILE = TBX + 2 * (TBY - 1);
+    uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= n_items) return;
+    uint32_t tile = item / PER_TILE, k = item % PER_TILE;
+    int bx, by;
+    if (k < (uint32_t)TBX) { bx = k; by = 0; }
+    else if (k < (uint32_t)(TBX + TBY - 1)) { bx = 0; by = 1 + (k - TBX); }
+    else { bx = TBX - 1; by = 1 + (k - TBX - (TBY - 1)); }
+    int x0 = (tile % tiles_x) * 2 * TBX, y0 = (tile / tiles_x) * 2 * TBY;
+    int x = x0 + 2 * bx, y = y0 + 2 * by;
+    if (x >= (int)w || y >= (int)h) return;
+    const uint8_t *row = thr + (size_t)y * w;
+    uint32_t m = check_mask(row[x], row[x + 1], row[x + w], x, y, w, h);
+    uint32_t id = (uint32_t)y * w + (uint32_t)x;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        Probe q = PROBES[i];
+        if (!(m & (1u << q.bit))) continue;
+        int nbx = bx + q.dbx, nby = by + q.dby;
+        if (nbx >= 0 && nbx < TBX && nby >= 0) continue;  // tile-internal: done locally
+        uint8_t v = thr[(size_t)(y + q.dy) * w + (x + q.dx)];
+        if (v != C_WHITE && v != C_BLACK) continue;
+        uint32_t c = v == C_WHITE ? 0 : 1;
+        uint32_t nid = (uint32_t)(y + 2 * q.dby) * w + (uint32_t)(x + 2 * q.dbx);
+        g_union(parent, id + c, nid + c);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_ccl_final(const uint8_t *__restrict__ thr, uint32_t w, uint32_t h, uint32_t *__restrict__ parent,
+            uint32_t *__restrict__ labels, uint32_t *__restrict__ sizes) {
+    uint32_t bx = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t by = blockIdx.y;
+    if (2 * bx >= w || 2 * by >= h) return;
+    uint32_t x = 2 * bx, y = 2 * by;
+    uint32_t id = y * w + x;
+    uint32_t root[2];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        uint32_t v = parent[id + c];
+        uint32_t r = g_find(parent, v & ~FLAG);
+        root[c] = r;
+        if (!(v & FLAG) && r != id + c) {
+            uint32_t s = sizes[id + c];
+            if (s) atomicAdd(&sizes[r], s);
+        }
+    }
+    uint16_t top = *reinterpret_cast<const uint16_t *>(thr + id);
+    uint16_t bot = *reinterpret_cast<const uint16_t *>(thr + id + w);
+    auto lab = [&](uint32_t v) -> uint32_t {
+        v &= 0xFFu;
+        if (v == C_WHITE) return LABEL_PIXEL_MASK | root[0];
+        if (v == C_BLACK) return root[1];
+        return 0u;
+    };
+    uint2 t2 = make_uint2(lab(top), lab(top >> 8));
+    uint2 b2 = make_uint2(lab(bot), lab(bot >> 8));
+    *reinterpret_cast<uint2 *>(labels + id) = t2;
+    *reinterpret_cast<uint2 *>(labels + id + w) = b2;
+}
+
+int launch_ccl(hipStream_t s, const uint8_t *thr, size_t w, size_t h, uint32_t *parent,
+               uint32_t *labels, uint32_t *sizes) {
+    dim3 tiles(atq_div_up(w, 2 * TBX), atq_div_up(h, 2 * TBY));
+    k_ccl_local<<<tiles, 256, 0, s>>>(thr, (uint32_t)w, (uint32_t)h, parent, sizes);
+    uint32_t n_items = tiles.x * tiles.y * (TBX + 2 * (TBY - 1));
+    k_ccl_border<<<atq_div_up(n_items, 256), 256, 0, s>>>(thr, (uint32_t)w, (uint32_t)h, tiles.x,
+                                                          n_items, parent);
+    dim3 fgrid(atq_div_up(w / 2, 256), (unsigned)(h / 2));
+    k_ccl_final<<<fgrid, 256, 0, s>>>(thr, (uint32_t)w, (uint32_t)h, parent, labels, sizes);
+    return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
+}
+
+}  // namespace atq

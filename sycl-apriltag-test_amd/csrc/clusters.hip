@@ -1,0 +1,224 @@
+// clusters.hip - cluster-pair segments, bounds, filter, angle sort, line-fit
+// moments (rows A10..A16; src/main.cpp:571-801).
+//
+// After the key sort, equal keys form segments (the reference's
+// reduce_by_segment, D3). Each segment's ClusterBounds is reduced by one
+// wave and filtered (ValidBlobFilter, D4); segments too small or too long to
+// pass are rejected from their length alone. Valid segments are numbered in
+// key order (their cluster index) and laid out contiguously (D6/D7 extents).
+// Every record of a valid segment is placed at its final position with its
+// calc_theta slope (D5). Then one work-group per cluster sorts its points by
+// (slope, plane, y, x) - the reference's angle sort made stable (D8) - in
+// LDS (global scratch for clusters larger than LDS), computes each point's
+// line-fit moments from the grey image (D9) and the cluster's inclusive
+// prefix sums in exact int64 (D10).
+#include "clusters.hpp"
+
+namespace atq {
+
+namespace {
+
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr int CS_THREADS = 256;
+constexpr uint32_t LDS_KEYS = 4096;  // 32 KiB of u64 keys
+
+struct HeadFlag {
+    const uint64_t *keys;
+    ATQ_DEV uint32_t operator()(uint32_t i) const { return i == 0 || keys[i] != keys[i - 1]; }
+};
+struct HeadOut {
+    uint32_t *seg_id, *seg_start;
+    ATQ_DEV void operator()(uint32_t i, uint32_t pre, uint32_t v) const {
+        seg_id[i] = pre + v - 1;
+        if (v) seg_start[pre] = i;
+    }
+};
+
+struct ValidFlag {
+    const uint32_t *valid;
+    ATQ_DEV uint32_t operator()(uint32_t s) const { return valid[s]; }
+};
+struct ValidOut {
+    uint32_t *seg_cluster;
+    const atq_cluster_bounds *seg_bounds;
+    atq_cluster_bounds *bounds;
+    ATQ_DEV void operator()(uint32_t s, uint32_t pre, uint32_t v) const {
+        seg_cluster[s] = v ? pre : NONE;
+        if (v) bounds[pre] = seg_bounds[s];
+    }
+};
+
+struct ClusterCount {
+    const atq_cluster_bounds *bounds;
+    ATQ_DEV uint32_t operator()(uint32_t c) const { return bounds[c].count; }
+};
+struct ExtentOut {
+    atq_cluster_extents *ext;
+    ATQ_DEV void operator()(uint32_t c, uint32_t pre, uint32_t v) const { ext[c] = {pre, v}; }
+};
+
+}  // namespace
+
+// seg_start[S] = N (end sentinel)
+__global__ void k_seg_close(const atq_counts *counts, uint32_t cap, uint32_t *seg_start) {
+    uint32_t n = min(counts->n_boundary, cap);
+    seg_start[counts->n_pairs] = n;
+}
+
+// One wave per segment: reduce ClusterBounds and apply ValidBlobFilter.
+__global__ void __launch_bounds__(256)
+k_seg_bounds(const atq_boundary_point *__restrict__ pts, const uint32_t *__restrict__ seg_start,
+             const atq_counts *counts, uint32_t w, uint32_t h, uint32_t *__restrict__ valid,
+             atq_cluster_bounds *__restrict__ seg_bounds) {
+    const uint32_t S = counts->n_pairs;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint64_t max_px = 2ull * (2ull * w + 2ull * h);
+    for (uint32_t s = wave; s < S; s += nwaves) {
+        const uint32_t b = seg_start[s], e = seg_start[s + 1], c = e - b;
+        if (c < 24 || c > max_px) {
+            if (lane == 0) valid[s] = 0;
+            continue;
+        }
+        uint32_t xmin = 0xFFFF, xmax = 0, ymin = 0xFFFF, ymax = 0;
+        int32_t gxs = 0, gys = 0;
+        int64_t pg = 0;
+        for (uint32_t i = b + lane; i < e; i += 64) {
+            atq_boundary_point p = pts[i];
+            uint32_t xx = x_with_dx(p.packed_x), yy = y_with_dy(p.packed_x, p.packed_y);
+            int gx = grad_x(p.packed_x, p.packed_y), gy = grad_y(p.packed_x, p.packed_y);
+            xmin = min(xmin, xx); xmax = max(xmax, xx);
+            ymin = min(ymin, yy); ymax = max(ymax, yy);
+            gxs += gx; gys += gy;
+            pg += (int64_t)xx * gx + (int64_t)yy * gy;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            xmin = min(xmin, (uint32_t)__shfl_xor(xmin, o, 64));
+            xmax = max(xmax, (uint32_t)__shfl_xor(xmax, o, 64));
+            ymin = min(ymin, (uint32_t)__shfl_xor(ymin, o, 64));
+            ymax = max(ymax, (uint32_t)__shfl_xor(ymax, o, 64));
+            gxs += __shfl_xor(gxs, o, 64);
+            gys += __shfl_xor(gys, o, 64);
+            pg += __shfl_xor(pg, o, 64);
+        }
+        if (lane == 0) {
+            atq_cluster_bounds cb;
+            cb.x_min = (uint16_t)xmin; cb.x_max = (uint16_t)xmax;
+            cb.y_min = (uint16_t)ymin; cb.y_max = (uint16_t)ymax;
+            cb.gx_sum = gxs; cb.gy_sum = gys; cb.pxgx_pygy_sum = pg;
+            cb.start = b; cb.count = c;
+            bool ok = valid_blob(cb, w, h);
+            valid[s] = ok;
+            if (ok) seg_bounds[s] = cb;
+        }
+    }
+}
+
+// D5: each record of a valid segment goes to its final slot with its slope.
+__global__ void __launch_bounds__(256)
+k_assign(const atq_boundary_point *__restrict__ pts, const uint32_t *__restrict__ seg_id,
+         const uint32_t *__restrict__ seg_start, const uint32_t *__restrict__ seg_cluster,
+         const atq_cluster_bounds *__restrict__ bounds, const atq_cluster_extents *__restrict__ ext,
+         const atq_counts *counts, uint32_t cap, atq_cluster_point *__restrict__ cpts,
+         uint32_t *__restrict__ cidx) {
+    const uint32_t n = min(counts->n_boundary, cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t s = seg_id[i];
+        uint32_t c = seg_cluster[s];
+        if (c == NONE) continue;
+        uint32_t pos = ext[c].start + (i - seg_start[s]);
+        atq_boundary_point p = pts[i];
+        const atq_cluster_bounds &b = bounds[c];
+        float th = calc_theta(p.packed_x, p.packed_y, bounds_cx(b), bounds_cy(b));
+        cpts[pos] = atq_cluster_point{p.packed_x, p.packed_y, th};
+        cidx[pos] = c;
+    }
+}
+
+// D8 + D9 + D10: one work-group per cluster.
+__global__ void __launch_bounds__(CS_THREADS)
+k_cluster_sort_fit(atq_cluster_point *__restrict__ cpts, const atq_cluster_extents *__restrict__ ext,
+                   const atq_counts *counts, const uint8_t *__restrict__ grey, uint32_t w, uint32_t h,
+                   uint64_t *__restrict__ scratch, atq_line_fit_point *__restrict__ lf) {
+    __shared__ uint64_t keys_lds[LDS_KEYS];
+    __shared__ int64_t lds4[4];
+    const uint32_t ncl = counts->n_clusters;
+    for (uint32_t c = blockIdx.x; c < ncl; c += gridDim.x) {
+        const uint32_t s0 = ext[c].start, n = ext[c].count;
+        const bool in_lds = n <= LDS_KEYS;
+        uint64_t *keys = in_lds ? keys_lds : scratch + s0;
+        for (uint32_t j = threadIdx.x; j < n; j += CS_THREADS) {
+            atq_cluster_point p = cpts[s0 + j];
+            keys[j] = angle_key(p.packed_x, p.packed_y, p.slope);
+        }
+        if (!in_lds) __threadfence_block();
+        __syncthreads();
+        block_bitonic_sort<uint64_t>(keys, n, CS_THREADS, !in_lds);
+        int64_t carry[6] = {0, 0, 0, 0, 0, 0};
+        for (uint32_t base = 0; base < n; base += CS_THREADS) {
+            uint32_t j = base + threadIdx.x;
+            atq_line_fit_point m{0, 0, 0, 0, 0, 0, 0, 0};
+            if (j < n) {
+                atq_cluster_point p;
+                angle_key_decode(keys[j], p);
+                cpts[s0 + j] = p;
+                m = initial_linefit(p.packed_x, p.packed_y, w, h, grey);
+            }
+            int64_t v[6] = {m.Mx, m.My, m.Mxx, m.Myy, m.Mxy, m.W};
+            int64_t out[6];
+#pragma unroll
+            for (int f = 0; f < 6; f++) {
+                int64_t tot;
+                int64_t ex = block_exclusive_scan<int64_t>(v[f], lds4, &tot);
+                out[f] = carry[f] + ex + v[f];
+                carry[f] += tot;
+            }
+            if (j < n) lf[s0 + j] = atq_line_fit_point{out[0], out[1], out[2], out[3], out[4], out[5], 0, 0};
+        }
+        __syncthreads();
+    }
+}
+
+int launch_segments(hipStream_t s, const uint64_t *keys, atq_counts *counts, uint32_t cap,
+                    uint32_t *seg_id, uint32_t *seg_start, uint32_t *scan_scratch) {
+    int r = device_scan(s, &counts->n_boundary, cap, HeadFlag{keys}, HeadOut{seg_id, seg_start},
+                        scan_scratch, &counts->n_pairs);
+    if (r) return r;
+    k_seg_close<<<1, 1, 0, s>>>(counts, cap, seg_start);
+    return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
+}
+
+int launch_bounds_filter(hipStream_t s, const atq_boundary_point *pts, const uint32_t *seg_start,
+                         atq_counts *counts, uint32_t cap, uint32_t w, uint32_t h, uint32_t *valid,
+                         atq_cluster_bounds *seg_bounds, uint32_t *seg_cluster,
+                         atq_cluster_bounds *bounds, uint32_t *scan_scratch) {
+    k_seg_bounds<<<2048, 256, 0, s>>>(pts, seg_start, counts, w, h, valid, seg_bounds);
+    return device_scan(s, &counts->n_pairs, cap, ValidFlag{valid},
+                       ValidOut{seg_cluster, seg_bounds, bounds}, scan_scratch, &counts->n_clusters);
+}
+
+int launch_extents(hipStream_t s, const atq_cluster_bounds *bounds, atq_counts *counts, uint32_t cap,
+                   atq_cluster_extents *ext, uint32_t *scan_scratch) {
+    return device_scan(s, &counts->n_clusters, cap, ClusterCount{bounds}, ExtentOut{ext},
+                       scan_scratch, &counts->n_cluster_points);
+}
+
+int launch_assign(hipStream_t s, const atq_boundary_point *pts, const uint32_t *seg_id,
+                  const uint32_t *seg_start, const uint32_t *seg_cluster,
+                  const atq_cluster_bounds *bounds, const atq_cluster_extents *ext,
+                  const atq_counts *counts, uint32_t cap, atq_cluster_point *cpts, uint32_t *cidx) {
+    k_assign<<<2048, 256, 0, s>>>(pts, seg_id, seg_start, seg_cluster, bounds, ext, counts, cap,
+                                  cpts, cidx);
+    return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
+}
+
+int launch_cluster_sort_fit(hipStream_t s, atq_cluster_point *cpts, const atq_cluster_extents *ext,
+                            const atq_counts *counts, const uint8_t *grey, uint32_t w, uint32_t h,
+                            uint64_t *scratch, atq_line_fit_point *lf) {
+    k_cluster_sort_fit<<<2048, CS_THREADS, 0, s>>>(cpts, ext, counts, grey, w, h, scratch, lf);
+    return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
+}
+
+}  // namespace atq

@@ -1,0 +1,27 @@
+// kernels.hpp - launchers of the gfx950 kernels (one .hip file per stage).
+#pragma once
+#include "common.hpp"
+
+namespace atq {
+
+// threshold.hip
+int launch_decimate(hipStream_t s, const uint8_t *in, size_t w_in, size_t h_in, int f,
+                    uint8_t *out);
+int launch_threshold(hipStream_t s, const uint8_t *in, size_t w_in, size_t h_in, int f,
+                     uint8_t *gray_out, atq_extrema *ext, atq_extrema *fil, uint8_t *thr,
+                     int min_diff);
+
+// ccl.hip
+int launch_ccl(hipStream_t s, const uint8_t *thr, size_t w, size_t h, uint32_t *parent,
+               uint32_t *labels, uint32_t *sizes);
+
+// boundary.hip
+int launch_find_boundaries_sparse(hipStream_t s, const uint32_t *labels, const uint32_t *sizes,
+                                  size_t w, size_t h, atq_boundary_point *points,
+                                  uint64_t *keys, int min_size);
+int launch_find_boundaries_compact(hipStream_t s, const uint32_t *labels, const uint32_t *sizes,
+                                   size_t w, size_t h, atq_boundary_point *points,
+                                   uint64_t *keys, uint32_t capacity, atq_counts *counts,
+                                   int min_size);
+
+}  // namespace atq

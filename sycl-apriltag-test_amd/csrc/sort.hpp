@@ -1,0 +1,58 @@
+// sort.hpp - radix sort launcher and the in-work-group bitonic sort.
+#pragma once
+#include "scan.hpp"
+
+namespace atq {
+
+constexpr unsigned RS_GROUPS = 512;
+
+// Sorts (keys, vals) by the key bits at `shifts` (8-bit digits, LSD first).
+// `hist` holds 256*RS_GROUPS u32. On return the sorted data are in the alt
+// buffers when *result_in_alt is true.
+int radix_sort_pairs(hipStream_t s, uint64_t *keys, uint32_t *vals, uint64_t *keys_alt,
+                     uint32_t *vals_alt, const uint32_t *count, uint32_t cap,
+                     const uint32_t *shifts, int n_shifts, uint32_t *hist, bool *result_in_alt);
+
+// Digit shifts that cover a pair key whose labels are below npx.
+int pair_key_shifts(size_t npx, uint32_t *shifts);
+
+// Ascending sort of a[0..n) by a work-group of `nthreads` threads with the
+// bitonic network in its "all comparators ascending" form, so that the
+// missing elements a[n..P) of the power-of-two size P act as +infinity and
+// are never touched. `a` is LDS or global memory owned by this work-group.
+// All threads of the work-group must call it.
+template <typename T>
+ATQ_DEV void block_bitonic_sort(T *a, uint32_t n, int nthreads, bool global_mem) {
+    if (n < 2) return;
+    uint32_t P = 1;
+    while (P < n) P <<= 1;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        // flip: i in the lower half of each k-block against its mirror
+        for (uint32_t c = t; c < P / 2; c += nthreads) {
+            uint32_t half = k >> 1;
+            uint32_t i = (c / half) * k + (c % half);
+            uint32_t j = i ^ (k - 1);
+            if (j < n) {
+                T x = a[i], y = a[j];
+                if (y < x) { a[i] = y; a[j] = x; }
+            }
+        }
+        if (global_mem) __threadfence_block();
+        __syncthreads();
+        for (uint32_t st = k >> 2; st > 0; st >>= 1) {
+            for (uint32_t c = t; c < P / 2; c += nthreads) {
+                uint32_t i = (c / st) * (2 * st) + (c % st);
+                uint32_t j = i + st;
+                if (j < n) {
+                    T x = a[i], y = a[j];
+                    if (y < x) { a[i] = y; a[j] = x; }
+                }
+            }
+            if (global_mem) __threadfence_block();
+            __syncthreads();
+        }
+    }
+}
+
+}  // namespace atq

@@ -1,0 +1,29 @@
+"""numpy dtypes layout-identical to the C ABI records in include/atq.h
+(which in turn match the reference structs)."""
+import numpy as np
+
+EXTREMA = np.dtype([("min", "u1"), ("max", "u1")])
+BOUNDARY_POINT = np.dtype([("packed_x", "<u2"), ("packed_y", "<u2")])
+CLUSTER_POINT = np.dtype([("packed_x", "<u2"), ("packed_y", "<u2"), ("slope", "<f4")])
+CLUSTER_EXTENTS = np.dtype([("start", "<u4"), ("count", "<u4")])
+CLUSTER_BOUNDS = np.dtype([("x_min", "<u2"), ("x_max", "<u2"), ("y_min", "<u2"),
+                           ("y_max", "<u2"), ("gx_sum", "<i4"), ("gy_sum", "<i4"),
+                           ("pxgx_pygy_sum", "<i8"), ("start", "<u4"), ("count", "<u4")])
+LINE_FIT_POINT = np.dtype([(n, "<i8") for n in ("Mx", "My", "Mxx", "Myy", "Mxy", "W", "a", "b")])
+PEAK_EXTENTS = np.dtype([("start", "<u4"), ("count", "<u4")])
+CORNER = np.dtype([("line_fit_point_index", "<u4"), ("cluster_index", "<u4"), ("error", "<f4")])
+FITTED_QUAD = np.dtype([("moments", LINE_FIT_POINT, (4,)), ("num_in_moments", "<u2", (4,)),
+                        ("indices", "<u2", (4,))])
+QUAD_CORNERS = np.dtype([("corners", "<f4", (4, 2)), ("reversed_border", "u1"),
+                         ("pad_", "u1", (3,)), ("blob_index", "<u4")])
+COUNTS = np.dtype([(n, "<u4") for n in (
+    "n_boundary", "n_pairs", "n_clusters", "n_cluster_points", "n_corners", "n_segments",
+    "n_quads", "overflow", "oob_reads")] + [("reserved", "<u4", (7,))])
+
+SIZES = {
+    "EXTREMA": 2, "BOUNDARY_POINT": 4, "CLUSTER_POINT": 8, "CLUSTER_EXTENTS": 8,
+    "CLUSTER_BOUNDS": 32, "LINE_FIT_POINT": 64, "PEAK_EXTENTS": 8, "CORNER": 12,
+    "FITTED_QUAD": 272, "QUAD_CORNERS": 40, "COUNTS": 64,
+}
+for _name, _size in SIZES.items():
+    assert globals()[_name].itemsize == _size, (_name, globals()[_name].itemsize)
