@@ -219,9 +219,12 @@ int atq_cluster_points_sort(void *stream, const atq_boundary_point *points,
 
 /* D9+D10 (src/main.cpp:768-801): compute_initial_linefit
  * (include/line_fit_point.h:50-86) and the per-cluster inclusive prefix sum
- * of the moments (exact int64). */
+ * of the moments (exact int64). The clusters are the `extents` runs of the
+ * angle-sorted points (the reference keys the segmented scan by the
+ * per-point cluster index, whose runs are these extents);
+ * counts_dev->n_clusters gives their number. */
 int atq_linefit_prefix(void *stream, const atq_cluster_point *points,
-                       const uint32_t *cluster_index,
+                       const atq_cluster_extents *extents,
                        const atq_counts *counts_dev, const uint8_t *grayscale,
                        size_t width, size_t height, atq_line_fit_point *out);
 
@@ -257,7 +260,8 @@ int atq_do_indexing(void *stream, const atq_peak_extents *extents,
  * input order. *n_quads_dev (device u32) receives the count. */
 int atq_quads_from_fits(void *stream, const atq_fitted_quad *fitted,
                         size_t n_fitted, atq_quad_corners *out,
-                        uint32_t *n_quads_dev);
+                        uint32_t *n_quads_dev, void *workspace,
+                        size_t workspace_bytes);
 
 /* ---- whole pipeline ----------------------------------------------------- */
 
@@ -301,6 +305,28 @@ int atq_detector_buffer(atq_detector *det, int which, void **ptr,
                         size_t *capacity);
 /* Decimated dims of the last frame. */
 int atq_detector_dims(atq_detector *det, size_t *width, size_t *height);
+
+/* Per-stage timing, the counterpart of the reference's `prog` prints of
+ * cumulative microseconds after each stage (src/main.cpp:330-335 ...
+ * 972-977). When enabled, atq_detect_quads records a HIP event on its stream
+ * after each stage; atq_detector_stage_times waits for the last frame's
+ * events and returns each stage's milliseconds. */
+enum {
+    ATQ_STAGE_THRESHOLD = 0,   /* decimate + tile extrema + threshold       */
+    ATQ_STAGE_CCL = 1,         /* union-find labelling + sizes              */
+    ATQ_STAGE_BOUNDARY = 2,    /* boundary records                          */
+    ATQ_STAGE_KEY_SORT = 3,    /* radix sort by cluster-pair key            */
+    ATQ_STAGE_CLUSTERS = 4,    /* segments, bounds, filter, extents, slopes */
+    ATQ_STAGE_ANGLE_FIT = 5,   /* angle sort + line-fit moments + prefix    */
+    ATQ_STAGE_FIT_LINES = 6,   /* corner candidates                         */
+    ATQ_STAGE_CORNERS = 7,     /* corner grouping + sort                    */
+    ATQ_STAGE_INDEXING = 8,    /* 4-corner selection                        */
+    ATQ_STAGE_QUADS = 9,       /* corner geometry + checks                  */
+    ATQ_N_STAGES = 10
+};
+int atq_detector_set_timing(atq_detector *det, int enable);
+int atq_detector_stage_times(atq_detector *det, float *ms, int n);
+const char *atq_stage_name(int stage);
 
 #ifdef __cplusplus
 }

@@ -60,8 +60,9 @@ struct ExtentOut {
 }  // namespace
 
 // seg_start[S] = N (end sentinel)
-__global__ void k_seg_close(const atq_counts *counts, uint32_t cap, uint32_t *seg_start) {
-    uint32_t n = min(counts->n_boundary, cap);
+__global__ void k_seg_close(const uint32_t *n_dev, const atq_counts *counts, uint32_t cap,
+                            uint32_t *seg_start) {
+    uint32_t n = min(*n_dev, cap);
     seg_start[counts->n_pairs] = n;
 }
 
@@ -121,9 +122,9 @@ __global__ void __launch_bounds__(256)
 k_assign(const atq_boundary_point *__restrict__ pts, const uint32_t *__restrict__ seg_id,
          const uint32_t *__restrict__ seg_start, const uint32_t *__restrict__ seg_cluster,
          const atq_cluster_bounds *__restrict__ bounds, const atq_cluster_extents *__restrict__ ext,
-         const atq_counts *counts, uint32_t cap, atq_cluster_point *__restrict__ cpts,
+         const uint32_t *n_dev, uint32_t cap, atq_cluster_point *__restrict__ cpts,
          uint32_t *__restrict__ cidx) {
-    const uint32_t n = min(counts->n_boundary, cap);
+    const uint32_t n = min(*n_dev, cap);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint32_t s = seg_id[i];
         uint32_t c = seg_cluster[s];
@@ -137,33 +138,97 @@ k_assign(const atq_boundary_point *__restrict__ pts, const uint32_t *__restrict_
     }
 }
 
+// D5 in the reference's own form: lower_bound over the filtered clusters
+// (src/main.cpp:621-662), for callers that hold no segment table.
+__global__ void __launch_bounds__(256)
+k_assign_search(const atq_boundary_point *__restrict__ pts, const uint32_t *n_dev, uint32_t cap,
+                const atq_cluster_bounds *__restrict__ bounds, const atq_counts *counts,
+                const atq_cluster_extents *__restrict__ ext, atq_cluster_point *__restrict__ cpts,
+                uint32_t *__restrict__ cidx) {
+    const uint32_t n = min(*n_dev, cap);
+    const uint32_t ncl = counts->n_clusters;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = ncl;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (bounds[mid].start + bounds[mid].count <= i) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo == ncl) continue;
+        const atq_cluster_bounds &b = bounds[lo];
+        if (i < b.start || i >= b.start + b.count) continue;
+        atq_boundary_point p = pts[i];
+        uint32_t pos = ext[lo].start + (i - b.start);
+        cpts[pos] = atq_cluster_point{p.packed_x, p.packed_y,
+                                      calc_theta(p.packed_x, p.packed_y, bounds_cx(b), bounds_cy(b))};
+        cidx[pos] = lo;
+    }
+}
+
+// Extents of the runs of equal cluster index (for callers holding only the
+// per-point index array, as the reference's D10 does).
+struct RunHead {
+    const uint32_t *cidx;
+    ATQ_DEV uint32_t operator()(uint32_t i) const { return i == 0 || cidx[i] != cidx[i - 1]; }
+};
+struct RunOut {
+    atq_cluster_extents *ext;
+    const uint32_t *n_dev;
+    uint32_t cap;
+    const uint32_t *cidx;
+    ATQ_DEV void operator()(uint32_t i, uint32_t pre, uint32_t v) const {
+        if (!v) return;
+        const uint32_t n = min(*n_dev, cap);
+        uint32_t e = i + 1;
+        while (e < n && cidx[e] == cidx[i]) e++;
+        ext[pre] = atq_cluster_extents{i, e - i};
+    }
+};
+
 // D8 + D9 + D10: one work-group per cluster.
+// `sort` off: the points are already in angle order; `lf` null: no moments.
 __global__ void __launch_bounds__(CS_THREADS)
 k_cluster_sort_fit(atq_cluster_point *__restrict__ cpts, const atq_cluster_extents *__restrict__ ext,
-                   const atq_counts *counts, const uint8_t *__restrict__ grey, uint32_t w, uint32_t h,
+                   const uint32_t *n_clusters_dev, uint32_t cap_cl, bool sort,
+                   const uint8_t *__restrict__ grey, uint32_t w, uint32_t h,
                    uint64_t *__restrict__ scratch, atq_line_fit_point *__restrict__ lf) {
     __shared__ uint64_t keys_lds[LDS_KEYS];
     __shared__ int64_t lds4[4];
-    const uint32_t ncl = counts->n_clusters;
+    const uint32_t ncl = min(*n_clusters_dev, cap_cl);
     for (uint32_t c = blockIdx.x; c < ncl; c += gridDim.x) {
         const uint32_t s0 = ext[c].start, n = ext[c].count;
         const bool in_lds = n <= LDS_KEYS;
         uint64_t *keys = in_lds ? keys_lds : scratch + s0;
-        for (uint32_t j = threadIdx.x; j < n; j += CS_THREADS) {
-            atq_cluster_point p = cpts[s0 + j];
-            keys[j] = angle_key(p.packed_x, p.packed_y, p.slope);
+        if (sort) {
+            for (uint32_t j = threadIdx.x; j < n; j += CS_THREADS) {
+                atq_cluster_point p = cpts[s0 + j];
+                keys[j] = angle_key(p.packed_x, p.packed_y, p.slope);
+            }
+            if (!in_lds) __threadfence_block();
+            __syncthreads();
+            block_bitonic_sort<uint64_t>(keys, n, CS_THREADS, !in_lds);
+            if (!lf) {
+                for (uint32_t j = threadIdx.x; j < n; j += CS_THREADS) {
+                    atq_cluster_point p;
+                    angle_key_decode(keys[j], p);
+                    cpts[s0 + j] = p;
+                }
+                __syncthreads();
+                continue;
+            }
         }
-        if (!in_lds) __threadfence_block();
-        __syncthreads();
-        block_bitonic_sort<uint64_t>(keys, n, CS_THREADS, !in_lds);
         int64_t carry[6] = {0, 0, 0, 0, 0, 0};
         for (uint32_t base = 0; base < n; base += CS_THREADS) {
             uint32_t j = base + threadIdx.x;
             atq_line_fit_point m{0, 0, 0, 0, 0, 0, 0, 0};
             if (j < n) {
                 atq_cluster_point p;
-                angle_key_decode(keys[j], p);
-                cpts[s0 + j] = p;
+                if (sort) {
+                    angle_key_decode(keys[j], p);
+                    cpts[s0 + j] = p;
+                } else {
+                    p = cpts[s0 + j];
+                }
                 m = initial_linefit(p.packed_x, p.packed_y, w, h, grey);
             }
             int64_t v[6] = {m.Mx, m.My, m.Mxx, m.Myy, m.Mxy, m.W};
@@ -181,12 +246,25 @@ k_cluster_sort_fit(atq_cluster_point *__restrict__ cpts, const atq_cluster_exten
     }
 }
 
-int launch_segments(hipStream_t s, const uint64_t *keys, atq_counts *counts, uint32_t cap,
-                    uint32_t *seg_id, uint32_t *seg_start, uint32_t *scan_scratch) {
-    int r = device_scan(s, &counts->n_boundary, cap, HeadFlag{keys}, HeadOut{seg_id, seg_start},
-                        scan_scratch, &counts->n_pairs);
+int launch_assign_search(hipStream_t s, const atq_boundary_point *pts, const uint32_t *n_dev,
+                         uint32_t cap, const atq_cluster_bounds *bounds, const atq_counts *counts,
+                         const atq_cluster_extents *ext, atq_cluster_point *cpts, uint32_t *cidx) {
+    k_assign_search<<<2048, 256, 0, s>>>(pts, n_dev, cap, bounds, counts, ext, cpts, cidx);
+    return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
+}
+
+int launch_run_extents(hipStream_t s, const uint32_t *cidx, const uint32_t *n_dev, uint32_t cap,
+                       atq_cluster_extents *ext, uint32_t *n_runs_dev, uint32_t *scan_scratch) {
+    return device_scan(s, n_dev, cap, RunHead{cidx}, RunOut{ext, n_dev, cap, cidx}, scan_scratch,
+                       n_runs_dev);
+}
+
+int launch_segments(hipStream_t s, const uint64_t *keys, const uint32_t *n_dev, atq_counts *counts,
+                    uint32_t cap, uint32_t *seg_id, uint32_t *seg_start, uint32_t *scan_scratch) {
+    int r = device_scan(s, n_dev, cap, HeadFlag{keys}, HeadOut{seg_id, seg_start}, scan_scratch,
+                        &counts->n_pairs);
     if (r) return r;
-    k_seg_close<<<1, 1, 0, s>>>(counts, cap, seg_start);
+    k_seg_close<<<1, 1, 0, s>>>(n_dev, counts, cap, seg_start);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
@@ -208,16 +286,18 @@ int launch_extents(hipStream_t s, const atq_cluster_bounds *bounds, atq_counts *
 int launch_assign(hipStream_t s, const atq_boundary_point *pts, const uint32_t *seg_id,
                   const uint32_t *seg_start, const uint32_t *seg_cluster,
                   const atq_cluster_bounds *bounds, const atq_cluster_extents *ext,
-                  const atq_counts *counts, uint32_t cap, atq_cluster_point *cpts, uint32_t *cidx) {
-    k_assign<<<2048, 256, 0, s>>>(pts, seg_id, seg_start, seg_cluster, bounds, ext, counts, cap,
+                  const uint32_t *n_dev, uint32_t cap, atq_cluster_point *cpts, uint32_t *cidx) {
+    k_assign<<<2048, 256, 0, s>>>(pts, seg_id, seg_start, seg_cluster, bounds, ext, n_dev, cap,
                                   cpts, cidx);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
 int launch_cluster_sort_fit(hipStream_t s, atq_cluster_point *cpts, const atq_cluster_extents *ext,
-                            const atq_counts *counts, const uint8_t *grey, uint32_t w, uint32_t h,
-                            uint64_t *scratch, atq_line_fit_point *lf) {
-    k_cluster_sort_fit<<<2048, CS_THREADS, 0, s>>>(cpts, ext, counts, grey, w, h, scratch, lf);
+                            const uint32_t *n_clusters_dev, uint32_t cap_cl, bool sort,
+                            const uint8_t *grey, uint32_t w, uint32_t h, uint64_t *scratch,
+                            atq_line_fit_point *lf) {
+    k_cluster_sort_fit<<<2048, CS_THREADS, 0, s>>>(cpts, ext, n_clusters_dev, cap_cl, sort, grey, w,
+                                                   h, scratch, lf);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 

@@ -1,0 +1,468 @@
+// fitquad.hip - corner candidates, quad selection and corner geometry
+// (rows A17..A21; src/fit_quad.cpp, src/main.cpp:832-1162).
+//
+// fit_lines: the reference's 32-item work-groups with their two-cluster
+// bookkeeping (src/fit_quad.cpp:153-375) compute, for every point x >= 1,
+// exactly a per-cluster cyclic computation: window errors
+// e(p) = n_w * eig_err(moment[p-k, p+k]) with k = min(20, n/12), the 7-tap
+// filter f(p) = sum_t F[t] e(p+t-3) accumulated in order, and a strict local
+// maximum test f(p) > f(p-1), f(p) > f(p+1), all indices modulo the cluster
+// size n; global index 0 is never a candidate (its work-group slot is a
+// halo). Here one thread per point evaluates that definition directly from
+// the prefix-summed moments.
+//
+// Corners (error != 0) are grouped per cluster and sorted by (error desc,
+// index asc) - the reference's D11-D13 with a stable sort - by one
+// work-group per cluster. do_indexing runs one 256-item work-group per peak
+// segment like the reference (src/fit_quad.cpp:392-535), pairing segment e
+// with cluster extents[e] as the reference does. The quad checks of
+// process_fitted_quads run one thread per fitted quad.
+#include "clusters.hpp"
+
+namespace atq {
+
+namespace {
+
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr float MAX_LINE_FIT_MSE = 10.0f;
+// (float)cos(10 deg) and cos(10 deg) as glibc computes 10*M_PI/180
+constexpr float COS10_F = 0x1.f838b8p-1f;
+constexpr double COS10_D = 0x1.f838b8c811c17p-1;
+
+__constant__ float FILTER[7] = {0x1.6c0504p-7f, 0x1.152aaap-3f, 0x1.368b3p-1f, 0x1p+0f,
+                                0x1.368b3p-1f,  0x1.152aaap-3f, 0x1.6c0504p-7f};
+
+ATQ_DEV atq_line_fit_point lf_read(const atq_line_fit_point *p, int64_t i, int64_t bound,
+                                   uint32_t *oob) {
+    if (i < 0 || i >= bound) {
+        if (oob) atomicAdd(oob, 1u);
+        return atq_line_fit_point{0, 0, 0, 0, 0, 0, 0, 0};
+    }
+    return p[i];
+}
+
+ATQ_DEV void lf_sub(atq_line_fit_point &a, const atq_line_fit_point &b) {
+    a.Mx -= b.Mx; a.My -= b.My; a.Mxx -= b.Mxx; a.Myy -= b.Myy; a.Mxy -= b.Mxy; a.W -= b.W;
+}
+ATQ_DEV void lf_add(atq_line_fit_point &a, const atq_line_fit_point &b) {
+    a.Mx += b.Mx; a.My += b.My; a.Mxx += b.Mxx; a.Myy += b.Myy; a.Mxy += b.Mxy; a.W += b.W;
+}
+
+// src/fit_quad.cpp:15-35
+ATQ_DEV atq_line_fit_point get_moment(const atq_line_fit_point *p, uint32_t n, uint32_t i0,
+                                      uint32_t i1, uint32_t *total, int64_t bound, uint32_t *oob) {
+    atq_line_fit_point r{0, 0, 0, 0, 0, 0, 0, 0};
+    *total = 0;
+    if (i0 < i1) {
+        *total = i1 - i0 + 1;
+        r = lf_read(p, i1, bound, oob);
+        if (i0 > 0) lf_sub(r, lf_read(p, (int64_t)i0 - 1, bound, oob));
+    } else if (i1 < i0) {
+        *total = n - i0 + i1 + 1;
+        r = lf_read(p, (int64_t)n - 1, bound, oob);
+        lf_sub(r, lf_read(p, (int64_t)i0 - 1, bound, oob));
+        lf_add(r, lf_read(p, i1, bound, oob));
+    }
+    return r;
+}
+
+// src/fit_quad.cpp:37-105 (int64 products wrap; no FMA contraction)
+ATQ_DEV void fit_line(const atq_line_fit_point &m, uint32_t num, float *line, float *err,
+                      float *mse) {
+    int64_t Csxx = wsub(wmul(m.Mxx, m.W), wmul(m.Mx, m.Mx));
+    int64_t Csxy = wsub(wmul(m.Mxy, m.W), wmul(m.Mx, m.My));
+    int64_t Csyy = wsub(wmul(m.Myy, m.W), wmul(m.My, m.My));
+    int64_t corr = wmul(wmul(m.W, m.W), 4);
+    float sdist = hypot_f((float)wsub(Csxx, Csyy), (float)wmul(2, Csxy));
+    float eig_err = ((float)wadd(Csxx, Csyy) - sdist) / (float)wmul(corr, 2);
+    if (line) {
+        float nx1 = (float)wsub(Csxx, Csyy) - sdist;
+        float ny1 = (float)wmul(2, Csxy);
+        float nx2 = ny1;
+        float ny2 = (float)wsub(Csyy, Csxx) - sdist;
+        float M1 = nx1 * nx1 + ny1 * ny1;
+        float M2 = nx2 * nx2 + ny2 * ny2;
+        float nx, ny;
+        if (M1 > M2) { nx = nx1; ny = ny1; } else { nx = nx2; ny = ny2; }
+        line[0] = (float)m.Mx / (float)wmul(m.W, 2);
+        line[1] = (float)m.My / (float)wmul(m.W, 2);
+        float length = hypot_f(nx, ny);
+        if ((double)fabsf(length) < 1e-12) {
+            line[2] = 0.0f;
+            line[3] = 0.0f;
+        } else {
+            line[2] = nx / length;
+            line[3] = ny / length;
+        }
+    }
+    if (err) *err = (float)num * eig_err;
+    if (mse) *mse = eig_err;
+}
+
+ATQ_DEV float window_error(const atq_line_fit_point *p, uint32_t n, uint32_t k, uint32_t pos) {
+    uint32_t i0 = (pos + n - k) % n, i1 = (pos + k) % n, total;
+    atq_line_fit_point m = get_moment(p, n, i0, i1, &total, n, nullptr);
+    float e;
+    fit_line(m, total, nullptr, &e, nullptr);
+    return e;
+}
+
+// Binomials for the combinatorial number system decode (n <= 10, k <= 4);
+// for these the reference's uint8_t table (include/combinadics.h) is exact.
+ATQ_DEV uint32_t binom(uint32_t n, uint32_t k) {
+    uint32_t r = 1;
+    if (k > n) return 0;
+    for (uint32_t i = 1; i <= k; i++) r = r * (n - k + i) / i;
+    return r;
+}
+
+// include/combinadics.h:43-59: digits c0 > c1 > c2 > c3 with
+// num = C(c0,4) + C(c1,3) + C(c2,2) + C(c3,1).
+ATQ_DEV void combo_decode(uint32_t num, uint32_t out[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        uint32_t kk = 4 - i, c = kk - 1;
+        while (binom(c + 1, kk) <= num) c++;
+        out[i] = c;
+        num -= binom(c, kk);
+    }
+}
+
+struct CornerCount {
+    const uint32_t *ccount;
+    ATQ_DEV uint32_t operator()(uint32_t c) const { return ccount[c]; }
+};
+struct CornerStartOut {
+    uint32_t *cstart;
+    ATQ_DEV void operator()(uint32_t c, uint32_t pre, uint32_t) const { cstart[c] = pre; }
+};
+struct HasCorner {
+    const uint32_t *ccount;
+    ATQ_DEV uint32_t operator()(uint32_t c) const { return ccount[c] != 0; }
+};
+struct PeakOut {
+    const uint32_t *ccount, *cstart;
+    atq_peak_extents *peaks;
+    ATQ_DEV void operator()(uint32_t c, uint32_t pre, uint32_t v) const {
+        if (v) peaks[pre] = atq_peak_extents{cstart[c], ccount[c]};
+    }
+};
+struct QuadFlag {
+    const uint32_t *flags;
+    ATQ_DEV uint32_t operator()(uint32_t e) const { return flags[e]; }
+};
+struct QuadOut {
+    const atq_quad_corners *tmp;
+    atq_quad_corners *out;
+    ATQ_DEV void operator()(uint32_t e, uint32_t pre, uint32_t v) const {
+        if (v) out[pre] = tmp[e];
+    }
+};
+
+}  // namespace
+
+__global__ void k_set_u32(uint32_t *p, uint32_t v) { *p = v; }
+
+__global__ void k_zero_u32(uint32_t *p, const uint32_t *n_dev, uint32_t cap) {
+    uint32_t n = n_dev ? min(*n_dev, cap) : cap;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// One thread per cluster point: the cyclic window / filter / maximum.
+__global__ void __launch_bounds__(256)
+k_fit_lines(const atq_line_fit_point *__restrict__ lf, const uint32_t *__restrict__ cidx,
+            const atq_cluster_extents *__restrict__ ext, const uint32_t *n_dev, uint32_t cap,
+            atq_corner *__restrict__ sparse, uint32_t *__restrict__ ccount) {
+    const uint32_t N = n_dev ? min(*n_dev, cap) : cap;
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N; x += gridDim.x * blockDim.x) {
+        const uint32_t c = cidx[x];
+        const atq_cluster_extents e = ext[c];
+        const uint32_t s = e.start, n = e.count, j = x - s;
+        const uint32_t k = min(20u, n / 12u);
+        atq_corner out{0, 0, 0.0f};
+        if (x >= 1 && k >= 2) {
+            const atq_line_fit_point *p = lf + s;
+            float er[9];
+#pragma unroll
+            for (int d = 0; d < 9; d++) er[d] = window_error(p, n, k, (j + n + d - 4) % n);
+            float f[3];
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int t = 0; t < 7; t++) acc = acc + er[q + t] * FILTER[t];
+                f[q] = acc;
+            }
+            if (f[1] > f[0] && f[1] > f[2]) {
+                out = atq_corner{j, c, f[1]};
+                if (f[1] != 0.0f && ccount) atomicAdd(&ccount[c], 1u);
+            }
+        }
+        sparse[x] = out;
+    }
+}
+
+// Per-cluster corner counts from a sparse corner array (error != 0 only).
+__global__ void __launch_bounds__(256)
+k_count_corners(const atq_corner *__restrict__ sparse, const uint32_t *n_dev, uint32_t cap,
+                uint32_t *__restrict__ ccount) {
+    const uint32_t N = n_dev ? min(*n_dev, cap) : cap;
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N; x += gridDim.x * blockDim.x) {
+        atq_corner c = sparse[x];
+        if (c.error != 0.0f) atomicAdd(&ccount[c.cluster_index], 1u);
+    }
+}
+
+// Drops every corner (error != 0) into its cluster's slot range, in any order.
+__global__ void __launch_bounds__(256)
+k_corner_scatter(const atq_corner *__restrict__ sparse, const uint32_t *n_dev, uint32_t cap,
+                 const uint32_t *__restrict__ cstart, uint32_t *__restrict__ cursor,
+                 atq_corner *__restrict__ out) {
+    const uint32_t N = n_dev ? min(*n_dev, cap) : cap;
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N; x += gridDim.x * blockDim.x) {
+        atq_corner c = sparse[x];
+        if (c.error != 0.0f) out[cstart[c.cluster_index] + atomicAdd(&cursor[c.cluster_index], 1u)] = c;
+    }
+}
+
+// One work-group per cluster with corners: sort its slot range by
+// (error desc, index asc) - the reference's stable D12 order, since the
+// index is unique within a cluster.
+__global__ void __launch_bounds__(256)
+k_corner_sort(const uint32_t *__restrict__ ccount, const uint32_t *__restrict__ cstart,
+              const atq_counts *counts, atq_corner *__restrict__ out, uint64_t *__restrict__ scratch) {
+    constexpr uint32_t LDS_KEYS = 2048;
+    __shared__ uint64_t keys_lds[LDS_KEYS];
+    const uint32_t ncl = counts->n_clusters;
+    for (uint32_t c = blockIdx.x; c < ncl; c += gridDim.x) {
+        const uint32_t m = ccount[c];
+        if (m == 0) continue;
+        const uint32_t cs = cstart[c];
+        const bool in_lds = m <= LDS_KEYS;
+        uint64_t *keys = in_lds ? keys_lds : scratch + cs;
+        for (uint32_t r = threadIdx.x; r < m; r += 256) {
+            atq_corner cr = out[cs + r];
+            keys[r] = ((uint64_t)~float_key(cr.error) << 32) | cr.line_fit_point_index;
+        }
+        if (!in_lds) __threadfence_block();
+        __syncthreads();
+        block_bitonic_sort<uint64_t>(keys, m, 256, !in_lds);
+        for (uint32_t r = threadIdx.x; r < m; r += 256) {
+            uint64_t k = keys[r];
+            out[cs + r] = atq_corner{(uint32_t)k, c, float_from_key(~(uint32_t)(k >> 32))};
+        }
+        __syncthreads();
+    }
+}
+
+// do_indexing: one 256-item work-group per peak segment.
+__global__ void __launch_bounds__(256)
+k_do_indexing(const atq_peak_extents *__restrict__ peaks, const uint32_t *n_seg_dev, uint32_t cap,
+              const atq_corner *__restrict__ corners, const atq_line_fit_point *__restrict__ lf,
+              const uint32_t *n_points_dev, uint32_t n_points_host,
+              const atq_cluster_extents *__restrict__ ext, atq_fitted_quad *__restrict__ fitted,
+              uint32_t *oob) {
+    __shared__ float red_f[4];
+    __shared__ uint32_t red_u[4];
+    const uint32_t E = n_seg_dev ? min(*n_seg_dev, cap) : cap;
+    const uint32_t NP = n_points_dev ? *n_points_dev : n_points_host;
+    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    for (uint32_t e = blockIdx.x; e < E; e += gridDim.x) {
+        const atq_peak_extents pk = peaks[e];
+        const atq_cluster_extents ce = ext[e];  // paired by position, as the reference does
+        const int64_t bound = (int64_t)NP - (int64_t)ce.start;
+        const atq_line_fit_point *p = lf + ce.start;
+        const uint32_t ncomb = binom(min(pk.count, 10u), 4);
+        const bool valid = t < ncomb;
+        bool valid_fit = valid;
+        float total_err = 0.0f;
+        atq_fitted_quad q;
+        uint32_t li[4] = {0, 0, 0, 0};
+        if (valid) {
+            uint32_t ind[4];
+            combo_decode(t, ind);
+#pragma unroll
+            for (int k = 0; k < 4; k++) li[k] = corners[pk.start + ind[k]].line_fit_point_index;
+            auto sw = [&](int a, int b) {
+                if (!(li[a] < li[b])) { uint32_t tmp = li[a]; li[a] = li[b]; li[b] = tmp; }
+            };
+            sw(0, 1); sw(2, 3); sw(0, 2); sw(1, 3); sw(1, 2);  // src/fit_quad.cpp:377-389
+            float p01[4], p12[4];
+#pragma unroll
+            for (int sgi = 0; sgi < 4; sgi++) {
+                uint32_t total;
+                atq_line_fit_point m = get_moment(p, ce.count, li[sgi], li[(sgi + 1) & 3], &total, bound, oob);
+                float err, mse;
+                fit_line(m, total, sgi == 0 ? p01 : (sgi == 1 ? p12 : nullptr), &err, &mse);
+                valid_fit = valid_fit && (mse <= MAX_LINE_FIT_MSE);
+                q.moments[sgi] = m;
+                q.num_in_moments[sgi] = (uint16_t)total;
+                total_err = total_err + err;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) q.indices[k] = (uint16_t)li[k];
+            float dot = p01[2] * p12[2] + p01[3] * p12[3];
+            valid_fit = valid_fit && (fabsf(dot) <= COS10_F);
+        }
+        const float cur = valid_fit ? total_err : __builtin_inff();
+        // work-group minimum
+        float mn = cur;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) mn = fminf(mn, __shfl_xor(mn, o, 64));
+        if (lane == 0) red_f[wave] = mn;
+        __syncthreads();
+        const float lowest = fminf(fminf(red_f[0], red_f[1]), fminf(red_f[2], red_f[3]));
+        // smallest combination number within 0.1 of the minimum
+        uint32_t cand = ((double)(cur - lowest) < 0.1) ? t : NONE;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) cand = min(cand, (uint32_t)__shfl_xor(cand, o, 64));
+        if (lane == 0) red_u[wave] = cand;
+        __syncthreads();
+        const uint32_t chosen = min(min(red_u[0], red_u[1]), min(red_u[2], red_u[3]));
+        if (lowest == __builtin_inff()) {
+            if (t == 0) {
+                atq_fitted_quad z;
+                for (int k = 0; k < 4; k++) {
+                    z.moments[k] = atq_line_fit_point{0, 0, 0, 0, 0, 0, 0, 0};
+                    z.num_in_moments[k] = 0;
+                    z.indices[k] = 0;
+                }
+                fitted[e] = z;
+            }
+        } else if (t == chosen) {
+            for (int k = 0; k < 4; k++) {
+                q.moments[k].a = 0;
+                q.moments[k].b = 0;
+            }
+            fitted[e] = q;
+        }
+        __syncthreads();
+    }
+}
+
+// process_fitted_quads (src/main.cpp:1035-1162) minus decoding.
+__global__ void __launch_bounds__(256)
+k_quads(const atq_fitted_quad *__restrict__ fitted, const uint32_t *n_dev, uint32_t cap,
+        atq_quad_corners *__restrict__ tmp, uint32_t *__restrict__ flags, int min_tag_width) {
+    const uint32_t E = n_dev ? min(*n_dev, cap) : cap;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        const atq_fitted_quad &q = fitted[e];
+        uint32_t ok = 0;
+        atq_quad_corners c;
+        c.reversed_border = 0;
+        c.pad_[0] = c.pad_[1] = c.pad_[2] = 0;
+        c.blob_index = e;
+        for (int i = 0; i < 4; i++) c.corners[i][0] = c.corners[i][1] = 0.0f;
+        if (q.indices[0] | q.indices[1] | q.indices[2] | q.indices[3]) {
+            ok = 1;
+            float lines[4][4];
+            for (int i = 0; i < 4; i++) {
+                float err, mse;
+                fit_line(q.moments[i], q.num_in_moments[i], lines[i], &err, &mse);
+            }
+            for (int i = 0; i < 4 && ok; i++) {
+                const int i1 = (i + 1) & 3;
+                double A00 = lines[i][3], A01 = -lines[i1][3];
+                double A10 = -lines[i][2], A11 = lines[i1][2];
+                double B0 = -lines[i][0] + lines[i1][0];
+                double B1 = -lines[i][1] + lines[i1][1];
+                double det = A00 * A11 - A10 * A01;
+                double W00 = A11 / det, W01 = -A01 / det;
+                if (fabs(det) < 0.001) { ok = 0; break; }
+                double L0 = W00 * B0 + W01 * B1;
+                c.corners[i][0] = (float)((double)lines[i][0] + L0 * A00);
+                c.corners[i][1] = (float)((double)lines[i][1] + L0 * A10);
+            }
+            if (ok) {
+                float area = 0.0f, len[3], pp;
+                for (int i = 0; i < 3; i++) {
+                    int a = i, b = (i + 1) % 3;
+                    len[i] = hypot_f(c.corners[b][0] - c.corners[a][0], c.corners[b][1] - c.corners[a][1]);
+                }
+                pp = (len[0] + len[1] + len[2]) / 2;
+                area += sqrtf(pp * (pp - len[0]) * (pp - len[1]) * (pp - len[2]));
+                const int idxs[4] = {2, 3, 0, 2};
+                for (int i = 0; i < 3; i++) {
+                    int a = idxs[i], b = idxs[i + 1];
+                    len[i] = hypot_f(c.corners[b][0] - c.corners[a][0], c.corners[b][1] - c.corners[a][1]);
+                }
+                pp = (len[0] + len[1] + len[2]) / 2;
+                area += sqrtf(pp * (pp - len[0]) * (pp - len[1]) * (pp - len[2]));
+                if ((double)area < 0.95 * min_tag_width * min_tag_width) ok = 0;
+            }
+            for (int i = 0; i < 4 && ok; i++) {
+                int i0 = i, i1 = (i + 1) & 3, i2 = (i + 2) & 3;
+                float dx1 = c.corners[i1][0] - c.corners[i0][0];
+                float dy1 = c.corners[i1][1] - c.corners[i0][1];
+                float dx2 = c.corners[i2][0] - c.corners[i1][0];
+                float dy2 = c.corners[i2][1] - c.corners[i1][1];
+                float cosd = (dx1 * dx2 + dy1 * dy2) / sqrtf((dx1 * dx1 + dy1 * dy1) * (dx2 * dx2 + dy2 * dy2));
+                if ((double)fabsf(cosd) > COS10_D || dx1 * dy2 < dy1 * dx2) ok = 0;
+            }
+        }
+        tmp[e] = c;
+        flags[e] = ok;
+    }
+}
+
+int launch_set_u32(hipStream_t s, uint32_t *p, uint32_t v) {
+    k_set_u32<<<1, 1, 0, s>>>(p, v);
+    return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
+}
+
+int launch_zero_u32(hipStream_t s, uint32_t *p, const uint32_t *n_dev, uint32_t cap) {
+    k_zero_u32<<<min(1024u, atq_div_up(cap, 256) + 1), 256, 0, s>>>(p, n_dev, cap);
+    return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
+}
+
+int launch_fit_lines(hipStream_t s, const atq_line_fit_point *lf, const uint32_t *cidx,
+                     const atq_cluster_extents *ext, const uint32_t *n_points_dev, uint32_t cap,
+                     atq_corner *sparse, uint32_t *ccount) {
+    unsigned grid = min(4096u, atq_div_up(cap, 256) + 1);
+    k_fit_lines<<<grid, 256, 0, s>>>(lf, cidx, ext, n_points_dev, cap, sparse, ccount);
+    return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
+}
+
+int launch_corner_segments(hipStream_t s, const atq_corner *sparse, const uint32_t *n_sparse_dev,
+                           uint32_t cap, bool count_first, uint32_t *ccount, uint32_t *cursor,
+                           atq_counts *counts, uint32_t cap_cl, uint32_t *cstart,
+                           atq_peak_extents *peaks, atq_corner *compacted, uint64_t *scratch,
+                           uint32_t *scan_scratch) {
+    unsigned grid = min(4096u, atq_div_up(cap, 256) + 1);
+    unsigned gcl = min(1024u, atq_div_up(cap_cl, 256) + 1);
+    if (count_first) {
+        k_zero_u32<<<gcl, 256, 0, s>>>(ccount, &counts->n_clusters, cap_cl);
+        k_count_corners<<<grid, 256, 0, s>>>(sparse, n_sparse_dev, cap, ccount);
+    }
+    int r = device_scan(s, &counts->n_clusters, cap_cl, CornerCount{ccount}, CornerStartOut{cstart},
+                        scan_scratch, &counts->n_corners);
+    if (r) return r;
+    r = device_scan(s, &counts->n_clusters, cap_cl, HasCorner{ccount}, PeakOut{ccount, cstart, peaks},
+                    scan_scratch, &counts->n_segments);
+    if (r) return r;
+    k_zero_u32<<<gcl, 256, 0, s>>>(cursor, &counts->n_clusters, cap_cl);
+    k_corner_scatter<<<grid, 256, 0, s>>>(sparse, n_sparse_dev, cap, cstart, cursor, compacted);
+    k_corner_sort<<<2048, 256, 0, s>>>(ccount, cstart, counts, compacted, scratch);
+    return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
+}
+
+int launch_do_indexing(hipStream_t s, const atq_peak_extents *peaks, const uint32_t *n_seg_dev,
+                       uint32_t cap, const atq_corner *corners, const atq_line_fit_point *lf,
+                       const uint32_t *n_points_dev, uint32_t n_points_host,
+                       const atq_cluster_extents *ext, atq_fitted_quad *fitted,
+                       uint32_t *oob_counter) {
+    k_do_indexing<<<2048, 256, 0, s>>>(peaks, n_seg_dev, cap, corners, lf, n_points_dev,
+                                       n_points_host, ext, fitted, oob_counter);
+    return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
+}
+
+int launch_quads(hipStream_t s, const atq_fitted_quad *fitted, const uint32_t *n_fitted_dev,
+                 uint32_t cap, atq_quad_corners *tmp, uint32_t *flags, atq_quad_corners *out,
+                 uint32_t *n_out_dev, uint32_t *scan_scratch, int min_tag_width) {
+    k_quads<<<min(1024u, atq_div_up(cap, 256) + 1), 256, 0, s>>>(fitted, n_fitted_dev, cap, tmp,
+                                                                  flags, min_tag_width);
+    return device_scan(s, n_fitted_dev, cap, QuadFlag{flags}, QuadOut{tmp, out}, scan_scratch,
+                       n_out_dev);
+}
+
+}  // namespace atq

@@ -1,0 +1,461 @@
+// pipeline.cpp - the whole-frame detector and the glue entry points.
+//
+// The detector runs threshold -> quads as one stream-ordered chain of
+// kernels. Every count (boundary records, segments, clusters, points,
+// corners, quads) stays in device memory and later kernels read it there,
+// so the host never waits between stages (the reference synchronises after
+// every oneDPL call that returns a count, src/main.cpp:470-907). Buffers are
+// sized once, at creation, for the largest frame.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+
+#include "pipeline.hpp"
+
+using namespace atq;
+
+namespace atq {
+
+namespace {
+
+template <typename T>
+T *carve_one(char *base, size_t &off, size_t count) {
+    off = (off + 255) & ~(size_t)255;
+    T *p = base ? reinterpret_cast<T *>(base + off) : nullptr;
+    off += sizeof(T) * (count ? count : 1);
+    return p;
+}
+
+struct SparseFlag {
+    const uint32_t *pts;
+    ATQ_DEV uint32_t operator()(uint32_t i) const { return pts[i] != 0u; }
+};
+struct SparseOut {
+    const uint32_t *pts;
+    const uint64_t *keys;
+    uint32_t *out_pts;
+    uint64_t *out_keys;
+    ATQ_DEV void operator()(uint32_t i, uint32_t pre, uint32_t v) const {
+        if (v) {
+            out_pts[pre] = pts[i];
+            out_keys[pre] = keys[i];
+        }
+    }
+};
+
+}  // namespace
+
+size_t Workspace::carve(char *base, const Caps &c) {
+    size_t off = 0;
+    keys_alt = carve_one<uint64_t>(base, off, c.cap);
+    vals_alt = carve_one<uint32_t>(base, off, c.cap);
+    seg_id = carve_one<uint32_t>(base, off, c.cap);
+    seg_start = carve_one<uint32_t>(base, off, (size_t)c.cap + 1);
+    valid = carve_one<uint32_t>(base, off, c.cap);
+    seg_cluster = carve_one<uint32_t>(base, off, c.cap);
+    seg_bounds = carve_one<atq_cluster_bounds>(base, off, c.cap);
+    scratch64 = carve_one<uint64_t>(base, off, c.cap);
+    ccount = carve_one<uint32_t>(base, off, c.cap_cl);
+    cstart = carve_one<uint32_t>(base, off, c.cap_cl);
+    cursor = carve_one<uint32_t>(base, off, c.cap_cl);
+    qtmp = carve_one<atq_quad_corners>(base, off, c.cap_cl);
+    qflags = carve_one<uint32_t>(base, off, c.cap_cl);
+    scan = carve_one<uint32_t>(base, off, SCAN_GROUPS + 1);
+    hist = carve_one<uint32_t>(base, off, 256 * RS_GROUPS);
+    counts = carve_one<atq_counts>(base, off, 1);
+    return (off + 255) & ~(size_t)255;
+}
+
+}  // namespace atq
+
+namespace {
+
+hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+int check_dims(size_t w, size_t h) {
+    if (w < 8 || h < 8 || w > 8192 || h > 8192 || (w & 3) || (h & 3)) return ATQ_ERR_BAD_DIMS;
+    return ATQ_OK;
+}
+
+size_t decimated(size_t n, int f) { return f > 1 ? 1 + (n - 1) / f : n; }
+
+// The largest frame (in pixels) a workspace of `bytes` serves.
+Caps caps_from_workspace(size_t bytes) {
+    Workspace ws;
+    size_t lo = 8, hi = (size_t)8192 * 8192;
+    while (lo < hi) {
+        size_t mid = lo + (hi - lo + 1) / 2;
+        if (ws.carve(nullptr, Caps::for_pixels(mid)) <= bytes) lo = mid;
+        else hi = mid - 1;
+    }
+    return Caps::for_pixels(lo);
+}
+
+int take_workspace(Workspace &ws, void *workspace, size_t bytes, size_t w, size_t h) {
+    if (!workspace) return ATQ_ERR_BAD_ARG;
+    Caps c = Caps::for_pixels(w * h);
+    if (ws.carve(nullptr, c) > bytes) return ATQ_ERR_BAD_ARG;
+    ws.carve(static_cast<char *>(workspace), c);
+    return ATQ_OK;
+}
+
+// Detector buffers, carved from one arena.
+size_t carve_detector(atq_detector *d, char *base) {
+    const Caps &c = d->caps;
+    size_t off = 0;
+    d->gray = carve_one<uint8_t>(base, off, c.npx);
+    d->thr = carve_one<uint8_t>(base, off, c.npx);
+    d->ext = carve_one<atq_extrema>(base, off, c.npx / 16 + 1);
+    d->fil = carve_one<atq_extrema>(base, off, c.npx / 16 + 1);
+    d->parent = carve_one<uint32_t>(base, off, c.npx);
+    d->labels = carve_one<uint32_t>(base, off, c.npx);
+    d->sizes = carve_one<uint32_t>(base, off, c.npx);
+    d->keys = carve_one<uint64_t>(base, off, c.cap);
+    d->vals = carve_one<uint32_t>(base, off, c.cap);
+    d->bounds = carve_one<atq_cluster_bounds>(base, off, c.cap_cl);
+    d->cext = carve_one<atq_cluster_extents>(base, off, c.cap_cl);
+    d->cpts = carve_one<atq_cluster_point>(base, off, c.cap);
+    d->cidx = carve_one<uint32_t>(base, off, c.cap);
+    d->lf = carve_one<atq_line_fit_point>(base, off, c.cap);
+    d->sparse = carve_one<atq_corner>(base, off, c.cap);
+    d->corners = carve_one<atq_corner>(base, off, c.cap);
+    d->peaks = carve_one<atq_peak_extents>(base, off, c.cap_cl);
+    d->fitted = carve_one<atq_fitted_quad>(base, off, c.cap_cl);
+    d->quads = carve_one<atq_quad_corners>(base, off, c.cap_cl);
+    d->counts = carve_one<atq_counts>(base, off, 1);
+    off = (off + 255) & ~(size_t)255;
+    size_t ws_bytes = d->ws.carve(base ? base + off : nullptr, c);
+    return off + ws_bytes;
+}
+
+#define TRY(x)                      \
+    do {                            \
+        int r_ = (x);               \
+        if (r_ != ATQ_OK) return r_; \
+    } while (0)
+
+// threshold -> quads, all on `s`.
+int run_pipeline(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t w_in, size_t h_in) {
+    const int f = d->params.decimate;
+    const size_t w = decimated(w_in, f), h = decimated(h_in, f);
+    TRY(check_dims(w, h));
+    if (w_in > d->max_w_in || h_in > d->max_h_in || w * h > d->caps.npx) return ATQ_ERR_CAPACITY;
+    d->w = w;
+    d->h = h;
+    const Caps c = Caps::for_pixels(w * h);  // this frame's own bounds
+    Workspace &ws = d->ws;
+    atq_counts *cnt = d->counts;
+
+    auto mark = [&](int i) -> int {
+        if (d->timing && hipEventRecord(d->ev[i], s) != hipSuccess) return ATQ_ERR_HIP;
+        return ATQ_OK;
+    };
+    TRY(launch_zero_u32(s, reinterpret_cast<uint32_t *>(cnt), nullptr, sizeof(atq_counts) / 4));
+    TRY(mark(0));
+    TRY(launch_threshold(s, gray_in, w_in, h_in, f, d->gray, d->ext, d->fil, d->thr,
+                         d->params.min_white_black_diff));
+    TRY(mark(1 + ATQ_STAGE_THRESHOLD));
+    const uint8_t *grey = f > 1 ? d->gray : gray_in;
+    TRY(launch_ccl(s, d->thr, w, h, d->parent, d->labels, d->sizes));
+    TRY(mark(1 + ATQ_STAGE_CCL));
+    TRY(launch_find_boundaries_compact(s, d->labels, d->sizes, w, h,
+                                       reinterpret_cast<atq_boundary_point *>(d->vals), d->keys,
+                                       c.cap, cnt, d->params.min_component_size));
+    TRY(mark(1 + ATQ_STAGE_BOUNDARY));
+    uint32_t shifts[8];
+    int nsh = pair_key_shifts(w * h, shifts);
+    bool alt = false;
+    TRY(radix_sort_pairs(s, d->keys, d->vals, ws.keys_alt, ws.vals_alt, &cnt->n_boundary, c.cap,
+                         shifts, nsh, ws.hist, &alt));
+    TRY(mark(1 + ATQ_STAGE_KEY_SORT));
+    d->sorted_in_alt = alt;
+    const uint64_t *K = alt ? ws.keys_alt : d->keys;
+    const atq_boundary_point *V =
+        reinterpret_cast<const atq_boundary_point *>(alt ? ws.vals_alt : d->vals);
+    TRY(launch_segments(s, K, &cnt->n_boundary, cnt, c.cap, ws.seg_id, ws.seg_start, ws.scan));
+    TRY(launch_bounds_filter(s, V, ws.seg_start, cnt, c.cap, (uint32_t)w, (uint32_t)h, ws.valid,
+                             ws.seg_bounds, ws.seg_cluster, d->bounds, ws.scan));
+    TRY(launch_extents(s, d->bounds, cnt, c.cap_cl, d->cext, ws.scan));
+    TRY(launch_assign(s, V, ws.seg_id, ws.seg_start, ws.seg_cluster, d->bounds, d->cext,
+                      &cnt->n_boundary, c.cap, d->cpts, d->cidx));
+    TRY(mark(1 + ATQ_STAGE_CLUSTERS));
+    TRY(launch_cluster_sort_fit(s, d->cpts, d->cext, &cnt->n_clusters, c.cap_cl, true, grey,
+                                (uint32_t)w, (uint32_t)h, ws.scratch64, d->lf));
+    TRY(mark(1 + ATQ_STAGE_ANGLE_FIT));
+    TRY(launch_zero_u32(s, ws.ccount, &cnt->n_clusters, c.cap_cl));
+    TRY(launch_fit_lines(s, d->lf, d->cidx, d->cext, &cnt->n_cluster_points, c.cap, d->sparse,
+                         ws.ccount));
+    TRY(mark(1 + ATQ_STAGE_FIT_LINES));
+    TRY(launch_corner_segments(s, d->sparse, &cnt->n_cluster_points, c.cap, false, ws.ccount,
+                               ws.cursor, cnt, c.cap_cl, ws.cstart, d->peaks, d->corners,
+                               ws.scratch64, ws.scan));
+    TRY(mark(1 + ATQ_STAGE_CORNERS));
+    TRY(launch_do_indexing(s, d->peaks, &cnt->n_segments, c.cap_cl, d->corners, d->lf,
+                           &cnt->n_cluster_points, 0, d->cext, d->fitted, &cnt->oob_reads));
+    TRY(mark(1 + ATQ_STAGE_INDEXING));
+    TRY(launch_quads(s, d->fitted, &cnt->n_segments, c.cap_cl, ws.qtmp, ws.qflags, d->quads,
+                     &cnt->n_quads, ws.scan, d->params.min_tag_width));
+    TRY(mark(1 + ATQ_STAGE_QUADS));
+    return ATQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t atq_glue_workspace_size(size_t width, size_t height) {
+    Workspace ws;
+    return ws.carve(nullptr, Caps::for_pixels(width * height));
+}
+
+int atq_group_boundaries(void *stream, const atq_boundary_point *points, const uint64_t *blob_labels,
+                         size_t width, size_t height, atq_boundary_point *out_points,
+                         uint64_t *out_keys, uint32_t *count_dev, void *workspace,
+                         size_t workspace_bytes) {
+    if (!points || !blob_labels || !out_points || !out_keys || !count_dev) return ATQ_ERR_BAD_ARG;
+    TRY(check_dims(width, height));
+    Workspace ws;
+    TRY(take_workspace(ws, workspace, workspace_bytes, width, height));
+    hipStream_t s = S(stream);
+    const uint32_t n = (uint32_t)(4 * width * height);
+    // the element count of the sparse planes, as a device scalar
+    uint32_t *n_dev = &ws.counts->reserved[0];
+    TRY(launch_set_u32(s, n_dev, n));
+    const uint32_t *pts = reinterpret_cast<const uint32_t *>(points);
+    uint32_t *opts = reinterpret_cast<uint32_t *>(out_points);
+    TRY(device_scan(s, n_dev, n, SparseFlag{pts}, SparseOut{pts, blob_labels, opts, out_keys},
+                    ws.scan, count_dev));
+    uint32_t shifts[8];
+    int nsh = pair_key_shifts(width * height, shifts);
+    bool alt = false;
+    TRY(radix_sort_pairs(s, out_keys, opts, ws.keys_alt, ws.vals_alt, count_dev, n, shifts, nsh,
+                         ws.hist, &alt));
+    if (alt) {
+        ATQ_HIP_CHECK(hipMemcpyAsync(out_keys, ws.keys_alt, sizeof(uint64_t) * n,
+                                     hipMemcpyDeviceToDevice, s));
+        ATQ_HIP_CHECK(hipMemcpyAsync(opts, ws.vals_alt, sizeof(uint32_t) * n,
+                                     hipMemcpyDeviceToDevice, s));
+    }
+    // Stream-ordered copy of the count back is not needed: it is on device.
+    return ATQ_OK;
+}
+
+int atq_cluster_bounds_filter(void *stream, const atq_boundary_point *points, const uint64_t *keys,
+                              const uint32_t *n_points_dev, size_t width, size_t height,
+                              atq_cluster_bounds *out_bounds, atq_counts *counts_dev,
+                              void *workspace, size_t workspace_bytes) {
+    if (!points || !keys || !n_points_dev || !out_bounds || !counts_dev) return ATQ_ERR_BAD_ARG;
+    TRY(check_dims(width, height));
+    Workspace ws;
+    TRY(take_workspace(ws, workspace, workspace_bytes, width, height));
+    hipStream_t s = S(stream);
+    const Caps c = Caps::for_pixels(width * height);
+    TRY(launch_segments(s, keys, n_points_dev, counts_dev, c.cap, ws.seg_id, ws.seg_start, ws.scan));
+    return launch_bounds_filter(s, points, ws.seg_start, counts_dev, c.cap, (uint32_t)width,
+                                (uint32_t)height, ws.valid, ws.seg_bounds, ws.seg_cluster,
+                                out_bounds, ws.scan);
+}
+
+int atq_cluster_points_sort(void *stream, const atq_boundary_point *points,
+                            const uint32_t *n_points_dev, const atq_cluster_bounds *bounds,
+                            atq_counts *counts_dev, atq_cluster_point *out_points,
+                            uint32_t *out_cluster_index, atq_cluster_extents *out_extents,
+                            void *workspace, size_t workspace_bytes) {
+    if (!points || !n_points_dev || !bounds || !counts_dev || !out_points || !out_cluster_index ||
+        !out_extents || !workspace)
+        return ATQ_ERR_BAD_ARG;
+    hipStream_t s = S(stream);
+    Workspace ws;
+    const Caps c = caps_from_workspace(workspace_bytes);
+    ws.carve(static_cast<char *>(workspace), c);
+    TRY(launch_extents(s, bounds, counts_dev, c.cap_cl, out_extents, ws.scan));
+    TRY(launch_assign_search(s, points, n_points_dev, c.cap, bounds, counts_dev, out_extents,
+                             out_points, out_cluster_index));
+    return launch_cluster_sort_fit(s, out_points, out_extents, &counts_dev->n_clusters, c.cap_cl,
+                                   true, nullptr, 0, 0, ws.scratch64, nullptr);
+}
+
+int atq_linefit_prefix(void *stream, const atq_cluster_point *points,
+                       const atq_cluster_extents *extents, const atq_counts *counts_dev,
+                       const uint8_t *grayscale, size_t width, size_t height,
+                       atq_line_fit_point *out) {
+    if (!points || !extents || !counts_dev || !grayscale || !out) return ATQ_ERR_BAD_ARG;
+    TRY(check_dims(width, height));
+    const Caps c = Caps::for_pixels(width * height);
+    // sort == false: the points are read, never written.
+    return launch_cluster_sort_fit(S(stream), const_cast<atq_cluster_point *>(points), extents,
+                                   &counts_dev->n_clusters, c.cap_cl, false, grayscale,
+                                   (uint32_t)width, (uint32_t)height, nullptr, out);
+}
+
+int atq_fit_lines(void *stream, const atq_line_fit_point *points, const uint32_t *cluster_indices,
+                  const atq_cluster_extents *cluster_extents, size_t points_size,
+                  atq_corner *found_corners) {
+    if (!points || !cluster_indices || !cluster_extents || !found_corners) return ATQ_ERR_BAD_ARG;
+    if (points_size > 0xFFFFFFFFu) return ATQ_ERR_CAPACITY;
+    hipStream_t s = S(stream);
+    if (points_size) {
+        TRY(launch_fit_lines(s, points, cluster_indices, cluster_extents, nullptr,
+                             (uint32_t)points_size, found_corners, nullptr));
+    }
+    ATQ_HIP_CHECK(hipStreamSynchronize(s));  // blocking, src/fit_quad.cpp:374
+    return ATQ_OK;
+}
+
+int atq_corner_segments(void *stream, const atq_corner *found_corners, atq_counts *counts_dev,
+                        atq_corner *compacted, atq_peak_extents *segments, void *workspace,
+                        size_t workspace_bytes) {
+    if (!found_corners || !counts_dev || !compacted || !segments || !workspace)
+        return ATQ_ERR_BAD_ARG;
+    hipStream_t s = S(stream);
+    Workspace ws;
+    const Caps c = caps_from_workspace(workspace_bytes);
+    ws.carve(static_cast<char *>(workspace), c);
+    return launch_corner_segments(s, found_corners, &counts_dev->n_cluster_points, c.cap, true,
+                                  ws.ccount, ws.cursor, counts_dev, c.cap_cl, ws.cstart, segments,
+                                  compacted, ws.scratch64, ws.scan);
+}
+
+int atq_do_indexing(void *stream, const atq_peak_extents *extents, size_t extents_count,
+                    const atq_corner *compacted_corners, const atq_line_fit_point *points,
+                    size_t n_points, const atq_cluster_extents *cluster_extents,
+                    atq_fitted_quad *fitted) {
+    if (!extents || !compacted_corners || !points || !cluster_extents || !fitted)
+        return ATQ_ERR_BAD_ARG;
+    if (extents_count > 0xFFFFFFFFu || n_points > 0xFFFFFFFFu) return ATQ_ERR_CAPACITY;
+    hipStream_t s = S(stream);
+    if (extents_count) {
+        TRY(launch_do_indexing(s, extents, nullptr, (uint32_t)extents_count, compacted_corners,
+                               points, nullptr, (uint32_t)n_points, cluster_extents, fitted,
+                               nullptr));
+    }
+    ATQ_HIP_CHECK(hipStreamSynchronize(s));  // blocking, src/fit_quad.cpp:534
+    return ATQ_OK;
+}
+
+int atq_quads_from_fits(void *stream, const atq_fitted_quad *fitted, size_t n_fitted,
+                        atq_quad_corners *out, uint32_t *n_quads_dev, void *workspace,
+                        size_t workspace_bytes) {
+    if (!fitted || !out || !n_quads_dev || !workspace) return ATQ_ERR_BAD_ARG;
+    hipStream_t s = S(stream);
+    Workspace ws;
+    const Caps c = caps_from_workspace(workspace_bytes);
+    if (n_fitted > c.cap_cl) return ATQ_ERR_CAPACITY;
+    ws.carve(static_cast<char *>(workspace), c);
+    uint32_t *n_dev = &ws.counts->reserved[1];
+    TRY(launch_set_u32(s, n_dev, (uint32_t)n_fitted));
+    return launch_quads(s, fitted, n_dev, c.cap_cl, ws.qtmp, ws.qflags, out, n_quads_dev, ws.scan,
+                        8);
+}
+
+int atq_detector_create(size_t max_width_in, size_t max_height_in, const atq_params *params,
+                        atq_detector **out) {
+    if (!out) return ATQ_ERR_BAD_ARG;
+    *out = nullptr;
+    atq_params p{1, 5, 25, 8};
+    if (params) p = *params;
+    if (p.decimate < 1) return ATQ_ERR_BAD_ARG;
+    const size_t w = decimated(max_width_in, p.decimate), h = decimated(max_height_in, p.decimate);
+    TRY(check_dims(w, h));
+    if (int r = atq_init()) return r;
+    atq_detector *d = new (std::nothrow) atq_detector();
+    if (!d) return ATQ_ERR_BAD_ARG;
+    d->max_w_in = max_width_in;
+    d->max_h_in = max_height_in;
+    d->max_w = w;
+    d->max_h = h;
+    d->params = p;
+    d->caps = Caps::for_pixels(w * h);
+    const size_t bytes = carve_detector(d, nullptr);
+    if (hipMalloc(&d->arena, bytes) != hipSuccess) {
+        delete d;
+        return ATQ_ERR_HIP;
+    }
+    carve_detector(d, d->arena);
+    d->w = w;
+    d->h = h;
+    *out = d;
+    return ATQ_OK;
+}
+
+int atq_detector_destroy(atq_detector *det) {
+    if (!det) return ATQ_ERR_BAD_ARG;
+    if (det->timing)
+        for (int i = 0; i <= ATQ_N_STAGES; i++) (void)hipEventDestroy(det->ev[i]);
+    if (det->arena) (void)hipFree(det->arena);
+    delete det;
+    return ATQ_OK;
+}
+
+int atq_detect_quads(atq_detector *det, void *stream, const uint8_t *gray_in, size_t width_in,
+                     size_t height_in) {
+    if (!det || !gray_in) return ATQ_ERR_BAD_ARG;
+    return run_pipeline(det, S(stream), gray_in, width_in, height_in);
+}
+
+int atq_detector_buffer(atq_detector *d, int which, void **ptr, size_t *capacity) {
+    if (!d || !ptr || !capacity) return ATQ_ERR_BAD_ARG;
+    const Caps &c = d->caps;
+    const bool alt = d->sorted_in_alt;
+    switch (which) {
+    case ATQ_BUF_GRAY: *ptr = d->gray; *capacity = c.npx; break;
+    case ATQ_BUF_THRESHOLDED: *ptr = d->thr; *capacity = c.npx; break;
+    case ATQ_BUF_LABELS: *ptr = d->labels; *capacity = c.npx; break;
+    case ATQ_BUF_SIZES: *ptr = d->sizes; *capacity = c.npx; break;
+    case ATQ_BUF_BOUNDARY_POINTS: *ptr = alt ? (void *)d->ws.vals_alt : (void *)d->vals; *capacity = c.cap; break;
+    case ATQ_BUF_BOUNDARY_KEYS: *ptr = alt ? (void *)d->ws.keys_alt : (void *)d->keys; *capacity = c.cap; break;
+    case ATQ_BUF_CLUSTER_BOUNDS: *ptr = d->bounds; *capacity = c.cap_cl; break;
+    case ATQ_BUF_CLUSTER_POINTS: *ptr = d->cpts; *capacity = c.cap; break;
+    case ATQ_BUF_CLUSTER_INDEX: *ptr = d->cidx; *capacity = c.cap; break;
+    case ATQ_BUF_CLUSTER_EXTENTS: *ptr = d->cext; *capacity = c.cap_cl; break;
+    case ATQ_BUF_LINE_FIT: *ptr = d->lf; *capacity = c.cap; break;
+    case ATQ_BUF_CORNERS: *ptr = d->corners; *capacity = c.cap; break;
+    case ATQ_BUF_PEAK_EXTENTS: *ptr = d->peaks; *capacity = c.cap_cl; break;
+    case ATQ_BUF_FITTED: *ptr = d->fitted; *capacity = c.cap_cl; break;
+    case ATQ_BUF_QUADS: *ptr = d->quads; *capacity = c.cap_cl; break;
+    case ATQ_BUF_COUNTS: *ptr = d->counts; *capacity = 1; break;
+    case ATQ_BUF_EXTREMA: *ptr = d->ext; *capacity = c.npx / 16; break;
+    case ATQ_BUF_FILTERED_EXTREMA: *ptr = d->fil; *capacity = c.npx / 16; break;
+    case ATQ_BUF_SPARSE_CORNERS: *ptr = d->sparse; *capacity = c.cap; break;
+    default: return ATQ_ERR_BAD_ARG;
+    }
+    return ATQ_OK;
+}
+
+int atq_detector_set_timing(atq_detector *det, int enable) {
+    if (!det) return ATQ_ERR_BAD_ARG;
+    if (enable && !det->timing) {
+        for (int i = 0; i <= ATQ_N_STAGES; i++)
+            if (hipEventCreate(&det->ev[i]) != hipSuccess) return ATQ_ERR_HIP;
+        det->timing = true;
+    } else if (!enable && det->timing) {
+        for (int i = 0; i <= ATQ_N_STAGES; i++) (void)hipEventDestroy(det->ev[i]);
+        det->timing = false;
+    }
+    return ATQ_OK;
+}
+
+int atq_detector_stage_times(atq_detector *det, float *ms, int n) {
+    if (!det || !ms || n < ATQ_N_STAGES || !det->timing) return ATQ_ERR_BAD_ARG;
+    ATQ_HIP_CHECK(hipEventSynchronize(det->ev[ATQ_N_STAGES]));
+    for (int i = 0; i < ATQ_N_STAGES; i++)
+        ATQ_HIP_CHECK(hipEventElapsedTime(&ms[i], det->ev[i], det->ev[i + 1]));
+    return ATQ_OK;
+}
+
+const char *atq_stage_name(int stage) {
+    static const char *names[ATQ_N_STAGES] = {"threshold",  "ccl",       "boundary", "key_sort",
+                                              "clusters",   "angle_fit", "fit_lines", "corners",
+                                              "indexing",   "quads"};
+    return (stage >= 0 && stage < ATQ_N_STAGES) ? names[stage] : "?";
+}
+
+int atq_detector_dims(atq_detector *det, size_t *width, size_t *height) {
+    if (!det || !width || !height) return ATQ_ERR_BAD_ARG;
+    *width = det->w;
+    *height = det->h;
+    return ATQ_OK;
+}
+
+}  // extern "C"

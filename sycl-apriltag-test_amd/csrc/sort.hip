@@ -71,7 +71,7 @@ k_rs_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
              uint32_t cap, uint32_t shift, const uint32_t *__restrict__ hist) {
     __shared__ uint32_t wc[4][RS_BUCKETS];   // per-wave digit counts -> prefixes
     __shared__ uint32_t base_off[RS_BUCKETS];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int t = threadIdx.x, wave = t >> 6;
     base_off[t] = hist[t * gridDim.x + blockIdx.x];
     uint32_t n = min(*count, cap);
     uint32_t b, e;

@@ -40,13 +40,17 @@ SIGNATURES = {
     "atq_fit_lines": (C.c_int, [_P, _P, _P, _P, _S, _P]),
     "atq_corner_segments": (C.c_int, [_P, _P, _P, _P, _P, _P, _S]),
     "atq_do_indexing": (C.c_int, [_P, _P, _S, _P, _P, _S, _P, _P]),
-    "atq_quads_from_fits": (C.c_int, [_P, _P, _S, _P, _P]),
+    "atq_quads_from_fits": (C.c_int, [_P, _P, _S, _P, _P, _P, _S]),
     "atq_detector_create": (C.c_int, [_S, _S, _P, C.POINTER(_P)]),
     "atq_detector_destroy": (C.c_int, [_P]),
     "atq_detect_quads": (C.c_int, [_P, _P, _P, _S, _S]),
     "atq_detector_buffer": (C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(_S)]),
     "atq_detector_dims": (C.c_int, [_P, C.POINTER(_S), C.POINTER(_S)]),
+    "atq_detector_set_timing": (C.c_int, [_P, C.c_int]),
+    "atq_detector_stage_times": (C.c_int, [_P, C.POINTER(C.c_float), C.c_int]),
+    "atq_stage_name": (C.c_char_p, [C.c_int]),
 }
+N_STAGES = 10
 
 BUF = dict(GRAY=0, THRESHOLDED=1, LABELS=2, SIZES=3, BOUNDARY_POINTS=4, BOUNDARY_KEYS=5,
            CLUSTER_BOUNDS=6, CLUSTER_POINTS=7, CLUSTER_INDEX=8, CLUSTER_EXTENTS=9, LINE_FIT=10,
@@ -329,6 +333,15 @@ class Detector:
         cap = C.c_size_t()
         check(lib().atq_detector_buffer(self.handle, BUF[which], C.byref(ptr), C.byref(cap)))
         return ptr.value, cap.value
+
+    def set_timing(self, on=True):
+        check(lib().atq_detector_set_timing(self.handle, 1 if on else 0))
+
+    def stage_times(self):
+        """Milliseconds per stage of the last frame (needs set_timing(True))."""
+        arr = (C.c_float * N_STAGES)()
+        check(lib().atq_detector_stage_times(self.handle, arr, N_STAGES))
+        return {lib().atq_stage_name(i).decode(): arr[i] for i in range(N_STAGES)}
 
     def dims(self):
         w, h = C.c_size_t(), C.c_size_t()
