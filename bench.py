@@ -1,0 +1,232 @@
+#!/usr/bin/env python3
+"""Throughput of the threshold -> quads front-end on one MI355X.
+
+Workload (the headline): plain per-frame pipeline on 1920x1080 uint8 frames,
+30 tags + 200 rectangles/ellipses + 100 lines, blur 1.0, noise 6, seeds
+2000..2099 (100 frames, generated here and uploaded to HBM before timing),
+decimate 1. A step = one frame through threshold -> quads (all stages, the
+detector's buffers allocated once beforehand). K steps are timed between two
+device synchronisations; value = input Mpix/s = K * 1920*1080 / wall seconds.
+
+Also reported: decimate-2 throughput on the same frames, per-stage times from
+HIP events on the launch stream, the 4096x3072 stress frame (threshold and
+CCL stage rooflines), and the CPU oracle (the C++ restatement of the
+reference, OpenMP over the host cores) on a bounded sample of the same frames.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "sycl-apriltag-test_amd")
+sys.path.insert(0, PKG)
+
+HBM_PEAK_GBS = 8000.0
+W1080, H1080 = 1920, 1080
+METRIC = "Mpix/s & frames/s threshold→quads (1080p, 12MP); batched vs plain; HBM GB/s %"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def stage_bytes(stage, f, npx, counts, n_shifts):
+    """Algorithmic HBM bytes of one stage for one frame (npx = decimated px)."""
+    nbp = int(counts["n_boundary"])
+    if stage == "threshold":
+        # f input rows touched per output row, thresholded write, decimated grey write
+        return f * npx + npx + (npx if f > 1 else 0)
+    if stage == "ccl":
+        return 5 * npx  # thresholded read + u32 labels write
+    if stage == "boundary":
+        return 4 * npx + 12 * nbp
+    if stage == "key_sort":
+        return 24 * nbp * n_shifts
+    return None
+
+
+def n_sort_passes(npx):
+    lb = 1
+    while (1 << lb) < npx:
+        lb += 1
+    return 2 * ((lb + 7) // 8)
+
+
+def time_frames(atq, det, devs, w, h, steps, warmup, stream):
+    for i in range(warmup):
+        det.detect(devs[i % len(devs)], w, h, stream.handle)
+    stream.sync()
+    e0, e1 = atq.Event(), atq.Event()
+    t0 = time.perf_counter()
+    e0.record(stream.handle)
+    for i in range(steps):
+        det.detect(devs[i % len(devs)], w, h, stream.handle)
+    e1.record(stream.handle)
+    e1.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, e0.elapsed_ms(e1) / 1e3
+
+
+def stage_profile(atq, det, devs, w, h, n, stream):
+    det.set_timing(True)
+    acc = None
+    for i in range(n):
+        det.detect(devs[i % len(devs)], w, h, stream.handle)
+        st = det.stage_times()
+        acc = st if acc is None else {k: acc[k] + v for k, v in st.items()}
+    det.set_timing(False)
+    return {k: v / n for k, v in acc.items()}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--frames", type=int, default=100)
+    ap.add_argument("--cpu-frames", type=int, default=4)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-12mp", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs on exactly one GPU (frames are independent; "
+              "a batch would shard by frame across GPUs). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import pyatq as atq
+    from pyatq import synth
+
+    atq.init()
+    t = time.time()
+    frames = [synth.plain_1080p(2000 + i).image for i in range(args.frames)]
+    log(f"generated {len(frames)} frames in {time.time() - t:.1f}s")
+    devs = [atq.dev_from_np(f) for f in frames]
+    stream = atq.Stream()
+
+    res = {}
+    for f in (1, 2):
+        det = atq.Detector(W1080, H1080, decimate=f)
+        wall, ev = time_frames(atq, det, devs, W1080, H1080, args.steps, args.warmup, stream)
+        stages = stage_profile(atq, det, devs, W1080, H1080, 20, stream)
+        det.detect(devs[0], W1080, H1080, stream.handle)
+        stream.sync()
+        counts = det.counts()
+        res[f] = dict(wall=wall, ev=ev, stages=stages, counts=counts)
+        log(f"decimate {f}: {args.steps} frames in {wall * 1e3:.1f} ms wall "
+            f"({ev * 1e3:.1f} ms events), quads/frame {int(counts['n_quads'])}")
+        log("  stages ms: " + ", ".join(f"{k} {v:.4f}" for k, v in stages.items()))
+        det.close()
+
+    steps = args.steps
+    wall1 = res[1]["wall"]
+    mpix = steps * W1080 * H1080 / wall1 / 1e6
+    fps = steps / wall1
+    mpix2 = steps * W1080 * H1080 / res[2]["wall"] / 1e6
+
+    # dominant stage at the headline workload and its roofline
+    npx = W1080 * H1080
+    st1 = res[1]["stages"]
+    dom = max(st1, key=st1.get)
+    roof = None
+    alg = stage_bytes(dom, 1, npx, res[1]["counts"], n_sort_passes(npx))
+    if alg is not None:
+        gbs = alg / (st1[dom] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None, "stage": dom,
+                "algorithmic_bytes": int(alg), "avg_ms": round(st1[dom], 5)}
+    else:
+        roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": None, "traffic": None, "stage": dom, "avg_ms": round(st1[dom], 5)}
+
+    stage_roof = {}
+    for st in ("threshold", "ccl", "boundary", "key_sort"):
+        b = stage_bytes(st, 1, npx, res[1]["counts"], n_sort_passes(npx))
+        stage_roof[st] = {"ms": round(st1[st], 5), "GB/s": round(b / (st1[st] * 1e-3) / 1e9, 1),
+                          "frac": round(b / (st1[st] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    extra = {
+        "frames_per_s": round(fps, 2),
+        "mpix_per_s_decimate2": round(mpix2, 2),
+        "frames_per_s_decimate2": round(steps / res[2]["wall"], 2),
+        "event_ms_per_frame": round(res[1]["ev"] / steps * 1e3, 4),
+        "stage_ms_1080p_f1": {k: round(v, 5) for k, v in st1.items()},
+        "stage_ms_1080p_f2": {k: round(v, 5) for k, v in res[2]["stages"].items()},
+        "stage_roofline_1080p_f1": stage_roof,
+        "counts_1080p_f1": {k: int(res[1]["counts"][k]) for k in
+                            ("n_boundary", "n_pairs", "n_clusters", "n_cluster_points",
+                             "n_corners", "n_segments", "n_quads")},
+    }
+
+    if not args.skip_12mp:
+        t = time.time()
+        big = synth.stress_12mp().image
+        log(f"generated 12MP frame in {time.time() - t:.1f}s")
+        dbig = [atq.dev_from_np(big)]
+        det = atq.Detector(4096, 3072, decimate=1)
+        wall, ev = time_frames(atq, det, dbig, 4096, 3072, 10, 2, stream)
+        stages = stage_profile(atq, det, dbig, 4096, 3072, 5, stream)
+        det.detect(dbig[0], 4096, 3072, stream.handle)
+        stream.sync()
+        c = det.counts()
+        npx12 = 4096 * 3072
+        roof12 = {}
+        for st in ("threshold", "ccl", "boundary", "key_sort"):
+            b = stage_bytes(st, 1, npx12, c, n_sort_passes(npx12))
+            roof12[st] = {"ms": round(stages[st], 4), "GB/s": round(b / (stages[st] * 1e-3) / 1e9, 1),
+                          "frac": round(b / (stages[st] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        extra["stress_12mp"] = {
+            "ms_per_frame": round(wall / 10 * 1e3, 3),
+            "mpix_per_s": round(10 * npx12 / wall / 1e6, 1),
+            "stage_ms": {k: round(v, 4) for k, v in stages.items()},
+            "stage_roofline": roof12,
+            "counts": {k: int(c[k]) for k in ("n_boundary", "n_pairs", "n_clusters",
+                                              "n_cluster_points", "n_corners", "n_segments",
+                                              "n_quads")},
+        }
+        log(f"12MP: {wall / 10 * 1e3:.2f} ms/frame; stages {stages}")
+        det.close()
+
+    cpu = None
+    if not args.skip_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        orc = pyoracle.Oracle()
+        nthreads = min(args.cpu_threads, orc.max_threads())
+        sample = frames[:args.cpu_frames]
+        orc.detect(sample[0], 1, nthreads)  # warm-up
+        t = time.perf_counter()
+        for fr in sample:
+            orc.detect(fr, 1, nthreads)
+        dt = time.perf_counter() - t
+        cpu = {"value": round(len(sample) * npx / dt / 1e6, 3), "unit": "Mpix/s",
+               "cores": nthreads, "kind": "port",
+               "sample": f"{len(sample)} of the same 1080p frames (seeds 2000..), decimate 1, "
+                         f"C++ restatement of the reference with OpenMP, {dt:.2f}s"}
+        log(f"cpu oracle: {cpu}")
+
+    line = {
+        "metric": METRIC, "value": round(mpix, 2), "unit": "Mpix/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": round(wall1 / steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": "plain per-frame threshold->quads, 1920x1080 u8, 30 tags + 200 "
+                               "shapes + 100 lines, blur 1.0 noise 6, 100 seeded frames "
+                               "(2000-2099), decimate 1",
+                   "global_batch": 1, "frames": args.frames, "parallelism": "single-gpu"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -112,14 +112,14 @@ typedef struct atq_params {
 /* Per-frame counts written by the pipeline (device memory, 16 x u32). */
 typedef struct atq_counts {
     uint32_t n_boundary;      /* N_bp: boundary records                    */
-    uint32_t n_pairs;         /* cluster-pair segments (D3)                */
-    uint32_t n_clusters;      /* clusters passing ValidBlobFilter (D4)     */
-    uint32_t n_cluster_points;/* N_fp: points in valid clusters (D5)       */
-    uint32_t n_corners;       /* compacted corners (D11)                   */
-    uint32_t n_segments;      /* peak segments E (D13)                     */
+    uint32_t n_pairs;         /* cluster-pair segments                     */
+    uint32_t n_clusters;      /* clusters passing ValidBlobFilter          */
+    uint32_t n_cluster_points;/* points in those clusters                  */
+    uint32_t n_corners;       /* corners with error != 0                   */
+    uint32_t n_segments;      /* peak segments E                           */
     uint32_t n_quads;         /* quads surviving the host checks (A21)     */
     uint32_t overflow;        /* bit flags: capacity exceeded somewhere    */
-    uint32_t oob_reads;       /* Q13 reads past the moment array (counted) */
+    uint32_t oob_reads;       /* do_indexing reads past the moment array   */
     uint32_t reserved[7];
 } atq_counts;
 
@@ -158,8 +158,8 @@ int atq_decimate_threshold(void *stream, const uint8_t *gray_in,
 
 /* Replaces image_segmentation(q, thresholded, label_scratch, labels, sizes,
  * sizes_elem, width, height, deps)  include/segmentation.h:11-15.
- * labels: white = 0x80000000|root, black = root, 127 -> 0 (Q4 of the
- * reference, src/segmentation.cpp:348-388). sizes (width*height entries) is
+ * labels: white = 0x80000000|root, black = root, 127 -> 0, root = the
+ * component's minimum node id (src/segmentation.cpp:348-388). sizes (width*height entries) is
  * written at every label root and at index 0; other entries are not read by
  * any stage and are left as they were. sizes_elem is accepted for signature
  * parity and, as in the reference, unused. label_scratch (width*height u32)
@@ -184,7 +184,7 @@ int atq_find_boundaries(void *stream, const uint32_t *labels,
  * most width x height (decimated) pixels. */
 size_t atq_glue_workspace_size(size_t width, size_t height);
 
-/* D1+D2 (src/main.cpp:467-477, 515-518): compact the sparse planes (drop
+/* Compaction + key sort (src/main.cpp:467-477, 515-518): compact the sparse planes (drop
  * points whose u32 image is 0) and group by cluster-pair key ascending;
  * within a key the order is plane-major raster, i.e. a stable sort.
  * *count_dev (device u32) receives N_bp. Outputs hold 4*width*height. */
@@ -194,7 +194,7 @@ int atq_group_boundaries(void *stream, const atq_boundary_point *points,
                          uint64_t *out_keys, uint32_t *count_dev,
                          void *workspace, size_t workspace_bytes);
 
-/* D3+D4 (src/main.cpp:571-609): per key segment ClusterBounds
+/* Cluster bounds + filter (src/main.cpp:571-609): per key segment ClusterBounds
  * (include/cluster_bounds.h:28-83) then ValidBlobFilter(width, height)
  * (include/cluster_bounds.h:85-120). n_points is read from device memory.
  * out_bounds receives the filtered bounds, counts->n_pairs/n_clusters set. */
@@ -205,7 +205,7 @@ int atq_cluster_bounds_filter(void *stream, const atq_boundary_point *points,
                               atq_counts *counts_dev, void *workspace,
                               size_t workspace_bytes);
 
-/* D5..D8 (src/main.cpp:621-750): points of valid clusters with their
+/* Cluster assignment, extents, angle sort (src/main.cpp:621-750): points of valid clusters with their
  * calc_theta slope and cluster index, cluster extents, sorted by (cluster,
  * slope) stably. */
 int atq_cluster_points_sort(void *stream, const atq_boundary_point *points,
@@ -217,7 +217,7 @@ int atq_cluster_points_sort(void *stream, const atq_boundary_point *points,
                             atq_cluster_extents *out_extents, void *workspace,
                             size_t workspace_bytes);
 
-/* D9+D10 (src/main.cpp:768-801): compute_initial_linefit
+/* Line-fit points + moment prefix (src/main.cpp:768-801): compute_initial_linefit
  * (include/line_fit_point.h:50-86) and the per-cluster inclusive prefix sum
  * of the moments (exact int64). The clusters are the `extents` runs of the
  * angle-sorted points (the reference keys the segmented scan by the
@@ -237,7 +237,7 @@ int atq_fit_lines(void *stream, const atq_line_fit_point *points,
                   const atq_cluster_extents *cluster_extents,
                   size_t points_size, atq_corner *found_corners);
 
-/* D11..D13 (src/main.cpp:832-907): compact corners with error != 0, sort by
+/* Corner compaction, sort, segments (src/main.cpp:832-907): compact corners with error != 0, sort by
  * (cluster asc, error desc) stably, segment per cluster. */
 int atq_corner_segments(void *stream, const atq_corner *found_corners,
                         atq_counts *counts_dev, atq_corner *compacted,
@@ -248,7 +248,8 @@ int atq_corner_segments(void *stream, const atq_corner *found_corners,
  * points, cluster_extents, fitted)  include/fit_quad.h:52-55.
  * Blocking like the reference (src/fit_quad.cpp:534). fitted
  * (extents_count entries) is zero-filled here first. n_points bounds the
- * moment array (reads past it, possible through Q13, read zeros). */
+ * moment array (reads past it, possible because segment e is paired with
+ * cluster_extents[e] as in the reference, read zeros). */
 int atq_do_indexing(void *stream, const atq_peak_extents *extents,
                     size_t extents_count, const atq_corner *compacted_corners,
                     const atq_line_fit_point *points, size_t n_points,

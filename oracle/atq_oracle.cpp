@@ -13,16 +13,18 @@
 //
 // Canonical choices where the reference is not deterministic (each is
 // exercised by tests/test_oracle_*.py):
-//   * every oneDPL sort is stable (D2, D8, D12); compaction keeps input order;
+//   * every oneDPL sort is stable (src/main.cpp:515, 737, 857); compaction
+//     keeps input order;
 //   * floating point is IEEE binary32/64 with no FMA contraction (built with
 //     -ffp-contract=off); hypotf(a,b) is (float)sqrt((double)a*a+(double)b*b),
 //     which is what glibc's hypotf computes;
 //   * moment prefix sums are exact int64 (the reference adds int64 bit
-//     patterns as doubles, exact while sums stay below 2^53, Q10);
-//   * fit_line's int64 products wrap modulo 2^64 (Q11);
-//   * cluster indices are u32 (the reference's u16 caps at 65535, Q18);
+//     patterns as doubles, exact while sums stay below 2^53);
+//   * fit_line's int64 products wrap modulo 2^64;
+//   * cluster indices are u32 (the reference's u16 caps at 65535);
 //   * do_indexing reads of the moment array past its end (possible only
-//     through the Q13 misalignment) read zeros and are counted.
+//     because do_indexing pairs peak segment e with cluster extents[e]) read
+//     zeros and are counted.
 //
 // Parity status per function is recorded in the header table of
 // oracle/README.md.
@@ -46,7 +48,7 @@ struct Extrema { uint8_t mn, mx; };
 
 // include/boundary_point.h:8-36
 enum HalfPixel : uint16_t { RIGHT = 0, BOTTOM_LEFT = 1, BOTTOM = 2, BOTTOM_RIGHT = 3 };
-// (1<<15) & (1<<14) == 0: the half-pixel is never extracted (Q1).
+// (1<<15) & (1<<14) == 0: the half-pixel is never extracted.
 constexpr uint16_t COORDINATE_HALF_PIXEL_MASK = (1u << 15) & (1u << 14);
 constexpr uint16_t COORDINATE_COLOR_DIRECTION_MASK = 1u << 15;
 constexpr uint16_t COORDINATE_VALUE_MASK = (1u << 14) - 1;
@@ -188,7 +190,7 @@ inline float hypot_f(float a, float b) {
     return (float)std::sqrt(da * da + db * db);
 }
 
-// include/line_fit_point.h:50-86 (with the `y = rx / 2` typo, Q2)
+// include/line_fit_point.h:50-86 (with its `y = rx / 2`)
 inline LineFitPoint compute_initial_linefit(ClusterPoint p, uint32_t width, uint32_t height,
                                             const uint8_t *grey) {
     BoundaryPoint bp{p.packed_x, p.packed_y};
@@ -340,14 +342,14 @@ extern "C" {
 
 struct orc_stats {
     int64_t width, height;
-    int64_t n_sparse_points;   // D1 output (N_bp)
-    int64_t n_pairs;           // D3 segments
-    int64_t n_clusters;        // D4
-    int64_t n_cluster_points;  // D5 (N_fp)
-    int64_t n_corners;         // D11
-    int64_t n_segments;        // D13 (E)
+    int64_t n_sparse_points;   // compacted boundary records
+    int64_t n_pairs;           // cluster-pair segments
+    int64_t n_clusters;        // clusters passing ValidBlobFilter
+    int64_t n_cluster_points;  // points of those clusters
+    int64_t n_corners;         // corners with error != 0
+    int64_t n_segments;        // peak segments
     int64_t n_quads;           // A21 survivors
-    int64_t oob_reads;         // Q13 reads past the moment array
+    int64_t oob_reads;         // do_indexing reads past the moment array
     int64_t n_fitted;          // do_indexing outputs with a chosen combo
 };
 
@@ -622,7 +624,7 @@ void orc_find_boundaries(const uint32_t *labels, const uint32_t *sizes, int64_t 
     }
 }
 
-// D1 + D2: src/main.cpp:467-477 (copy_if on the point's u32 image != 0) and
+// Compaction + key sort: src/main.cpp:467-477 (copy_if on the point's u32 image != 0) and
 // src/main.cpp:515-518 (sort_by_key; stable here). Returns N_bp.
 int64_t orc_compact_sort(const uint32_t *points_sparse, const uint64_t *keys_sparse,
                          int64_t n_sparse, uint32_t *out_points, uint64_t *out_keys) {
@@ -639,7 +641,7 @@ int64_t orc_compact_sort(const uint32_t *points_sparse, const uint64_t *keys_spa
     return (int64_t)idx.size();
 }
 
-// D3: src/main.cpp:571-585 (reduce_by_segment of inital_from_point(p, i)).
+// Cluster bounds: src/main.cpp:571-585 (reduce_by_segment of inital_from_point(p, i)).
 int64_t orc_cluster_bounds(const uint32_t *pts_u32, const uint64_t *keys, int64_t n,
                            void *out_v) {
     const BoundaryPoint *pts = reinterpret_cast<const BoundaryPoint *>(pts_u32);
@@ -653,7 +655,7 @@ int64_t orc_cluster_bounds(const uint32_t *pts_u32, const uint64_t *keys, int64_
     return m;
 }
 
-// D4: src/main.cpp:604-609 (copy_if with ValidBlobFilter(width, height)).
+// Filter: src/main.cpp:604-609 (copy_if with ValidBlobFilter(width, height)).
 int64_t orc_filter_bounds(const void *in_v, int64_t n, int64_t w, int64_t h, void *out_v) {
     const ClusterBounds *in = reinterpret_cast<const ClusterBounds *>(in_v);
     ClusterBounds *out = reinterpret_cast<ClusterBounds *>(out_v);
@@ -664,7 +666,7 @@ int64_t orc_filter_bounds(const void *in_v, int64_t n, int64_t w, int64_t h, voi
     return m;
 }
 
-// D5: src/main.cpp:621-675 (lower_bound over the filtered clusters, the
+// Cluster assignment: src/main.cpp:621-675 (lower_bound over the filtered clusters, the
 // theta of each point, copy_if on the index sentinel). Returns N_fp.
 int64_t orc_cluster_points(const uint32_t *pts_u32, int64_t n, const void *filt_v, int64_t ncl,
                            void *out_pts_v, uint32_t *out_idx) {
@@ -685,7 +687,7 @@ int64_t orc_cluster_points(const uint32_t *pts_u32, int64_t n, const void *filt_
     return m;
 }
 
-// D6/D7: src/main.cpp:700-717
+// Extents: src/main.cpp:700-717
 void orc_cluster_extents(const void *filt_v, int64_t ncl, void *out_v) {
     const ClusterBounds *filt = reinterpret_cast<const ClusterBounds *>(filt_v);
     ClusterExtents *out = reinterpret_cast<ClusterExtents *>(out_v);
@@ -696,7 +698,7 @@ void orc_cluster_extents(const void *filt_v, int64_t ncl, void *out_v) {
     }
 }
 
-// D8: src/main.cpp:737-750 (sort by (cluster index, slope); stable here).
+// Angle sort: src/main.cpp:737-750 (sort by (cluster index, slope); stable here).
 void orc_angle_sort(void *pts_v, uint32_t *idx, int64_t n) {
     ClusterPoint *pts = reinterpret_cast<ClusterPoint *>(pts_v);
     std::vector<int64_t> order(n);
@@ -712,7 +714,7 @@ void orc_angle_sort(void *pts_v, uint32_t *idx, int64_t n) {
     std::memcpy(idx, i2.data(), sizeof(uint32_t) * n);
 }
 
-// D9: src/main.cpp:768-775
+// Line-fit points: src/main.cpp:768-775
 void orc_linefit(const void *pts_v, int64_t n, const uint8_t *grey, int64_t w, int64_t h,
                  void *out_v) {
     const ClusterPoint *pts = reinterpret_cast<const ClusterPoint *>(pts_v);
@@ -722,8 +724,8 @@ void orc_linefit(const void *pts_v, int64_t n, const uint8_t *grey, int64_t w, i
         out[i] = compute_initial_linefit(pts[i], (uint32_t)w, (uint32_t)h, grey);
 }
 
-// D10: src/main.cpp:793-801 (inclusive_scan_by_segment keyed by the
-// cluster index; exact int64 here, Q10).
+// Moment prefix: src/main.cpp:793-801 (inclusive_scan_by_segment keyed by
+// the cluster index; exact int64 here).
 void orc_linefit_prefix(const uint32_t *idx, const void *in_v, int64_t n, void *out_v) {
     const LineFitPoint *in = reinterpret_cast<const LineFitPoint *>(in_v);
     LineFitPoint *out = reinterpret_cast<LineFitPoint *>(out_v);
@@ -823,7 +825,8 @@ void orc_fit_lines(const void *pts_v, const uint32_t *cidx, const void *ext_v, i
     }
 }
 
-// D11-D13: src/main.cpp:832-907. Returns E; *n_corner receives D11's count.
+// Corner segments: src/main.cpp:832-907. Returns the segment count;
+// *n_corner receives the compacted corner count.
 int64_t orc_corner_segments(const void *corners_v, int64_t n, void *compacted_v, void *seg_v,
                             int64_t *n_corner) {
     const Corner *c = reinterpret_cast<const Corner *>(corners_v);
@@ -853,7 +856,7 @@ static void sort_4(size_t a[4]) {
 
 // K11 do_indexing: src/fit_quad.cpp:392-535. One emulated work-group of 256
 // combination numbers per peak segment; segment e is paired with
-// cluster_extents[e] (Q13). `fitted` (E entries) is zeroed here as
+// cluster_extents[e], as the reference does. `fitted` (E entries) is zeroed here as
 // src/main.cpp:318-319 does. n_points bounds reads of `points`.
 int64_t orc_do_indexing(const void *ext_v, int64_t E, const void *corners_v, const void *pts_v,
                         int64_t n_points, const void *cext_v, void *fitted_v) {

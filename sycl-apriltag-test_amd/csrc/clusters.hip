@@ -2,16 +2,19 @@
 // moments (rows A10..A16; src/main.cpp:571-801).
 //
 // After the key sort, equal keys form segments (the reference's
-// reduce_by_segment, D3). Each segment's ClusterBounds is reduced by one
-// wave and filtered (ValidBlobFilter, D4); segments too small or too long to
+// reduce_by_segment, src/main.cpp:571-585). Each segment's ClusterBounds is
+// reduced by one wave and filtered (ValidBlobFilter, src/main.cpp:604-609);
+// segments too small or too long to
 // pass are rejected from their length alone. Valid segments are numbered in
-// key order (their cluster index) and laid out contiguously (D6/D7 extents).
+// key order (their cluster index) and laid out contiguously (the extents of
+// src/main.cpp:700-717).
 // Every record of a valid segment is placed at its final position with its
-// calc_theta slope (D5). Then one work-group per cluster sorts its points by
-// (slope, plane, y, x) - the reference's angle sort made stable (D8) - in
+// calc_theta slope (src/main.cpp:621-675). Then one work-group per cluster
+// sorts its points by (slope, plane, y, x) - the reference's angle sort
+// (src/main.cpp:737-750) made stable - in
 // LDS (global scratch for clusters larger than LDS), computes each point's
-// line-fit moments from the grey image (D9) and the cluster's inclusive
-// prefix sums in exact int64 (D10).
+// line-fit moments from the grey image (src/main.cpp:768-775) and the
+// cluster's inclusive prefix sums in exact int64 (src/main.cpp:793-801).
 #include "clusters.hpp"
 
 namespace atq {
@@ -117,7 +120,7 @@ k_seg_bounds(const atq_boundary_point *__restrict__ pts, const uint32_t *__restr
     }
 }
 
-// D5: each record of a valid segment goes to its final slot with its slope.
+// Each record of a valid segment goes to its final slot with its slope.
 __global__ void __launch_bounds__(256)
 k_assign(const atq_boundary_point *__restrict__ pts, const uint32_t *__restrict__ seg_id,
          const uint32_t *__restrict__ seg_start, const uint32_t *__restrict__ seg_cluster,
@@ -138,7 +141,7 @@ k_assign(const atq_boundary_point *__restrict__ pts, const uint32_t *__restrict_
     }
 }
 
-// D5 in the reference's own form: lower_bound over the filtered clusters
+// Cluster assignment in the reference's own form: lower_bound over the filtered clusters
 // (src/main.cpp:621-662), for callers that hold no segment table.
 __global__ void __launch_bounds__(256)
 k_assign_search(const atq_boundary_point *__restrict__ pts, const uint32_t *n_dev, uint32_t cap,
@@ -166,7 +169,7 @@ k_assign_search(const atq_boundary_point *__restrict__ pts, const uint32_t *n_de
 }
 
 // Extents of the runs of equal cluster index (for callers holding only the
-// per-point index array, as the reference's D10 does).
+// per-point index array, as the reference's segmented scan does).
 struct RunHead {
     const uint32_t *cidx;
     ATQ_DEV uint32_t operator()(uint32_t i) const { return i == 0 || cidx[i] != cidx[i - 1]; }
@@ -185,7 +188,7 @@ struct RunOut {
     }
 };
 
-// D8 + D9 + D10: one work-group per cluster.
+// Angle sort + line-fit moments + prefix sums: one work-group per cluster.
 // `sort` off: the points are already in angle order; `lf` null: no moments.
 __global__ void __launch_bounds__(CS_THREADS)
 k_cluster_sort_fit(atq_cluster_point *__restrict__ cpts, const atq_cluster_extents *__restrict__ ext,

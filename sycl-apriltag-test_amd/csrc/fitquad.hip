@@ -12,7 +12,7 @@
 // the prefix-summed moments.
 //
 // Corners (error != 0) are grouped per cluster and sorted by (error desc,
-// index asc) - the reference's D11-D13 with a stable sort - by one
+// index asc) - src/main.cpp:832-907 with a stable sort - by one
 // work-group per cluster. do_indexing runs one 256-item work-group per peak
 // segment like the reference (src/fit_quad.cpp:392-535), pairing segment e
 // with cluster extents[e] as the reference does. The quad checks of
@@ -226,7 +226,7 @@ k_corner_scatter(const atq_corner *__restrict__ sparse, const uint32_t *n_dev, u
 }
 
 // One work-group per cluster with corners: sort its slot range by
-// (error desc, index asc) - the reference's stable D12 order, since the
+// (error desc, index asc) - the reference's corner order made stable, since the
 // index is unique within a cluster.
 __global__ void __launch_bounds__(256)
 k_corner_sort(const uint32_t *__restrict__ ccount, const uint32_t *__restrict__ cstart,

@@ -1,6 +1,6 @@
 // sort.hip - LSD radix sort of (u64 key, u32 value) records (row A9).
 //
-// Groups the boundary records by cluster-pair key (the reference's D2
+// Groups the boundary records by cluster-pair key (the reference's
 // sort_by_key, src/main.cpp:515-518). Only the key bits that can be set are
 // sorted: a pair key is min_label << 32 | max_label with labels below W*H,
 // so the passes cover bits [0, lb) and [32, 32 + lb), lb = bit length of

@@ -325,7 +325,7 @@ class Detector:
             pass
 
     def detect(self, gray_dev, w_in, h_in, stream_handle=None):
-        s = stream_handle if stream_handle is not None else stream()
+        s = C.c_void_p(stream_handle) if stream_handle is not None else stream()
         check(lib().atq_detect_quads(self.handle, s, P(gray_dev), w_in, h_in))
 
     def buffer(self, which):

@@ -1,0 +1,87 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports
+every entry point include/atq.h declares, record layouts, the synthetic
+workloads are reproducible, and the ctypes table matches the header."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "atq.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(atq_\w+)\s*\(", txt)))
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    for must in ("atq_threshold_image", "atq_image_segmentation", "atq_find_boundaries",
+                 "atq_fit_lines", "atq_do_indexing", "atq_detect_quads"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    import ctypes
+    import pyatq
+    lib = pyatq.lib()  # loads; no compute call is made here
+    missing = [n for n in _declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_ctypes_table_covers_header():
+    import pyatq
+    assert set(_declared()) == set(pyatq.SIGNATURES)
+
+
+def test_record_layouts():
+    from pyatq import records as R
+    assert R.FITTED_QUAD.itemsize == 272 and R.QUAD_CORNERS.itemsize == 40
+    assert R.CORNER.itemsize == 12 and R.CLUSTER_BOUNDS.itemsize == 32
+
+
+def test_status_strings():
+    import pyatq
+    for code in range(6):
+        assert pyatq.lib().atq_status_string(code)
+
+
+def test_workspace_size_grows_with_frame():
+    import pyatq
+    a = pyatq.lib().atq_glue_workspace_size(640, 480)
+    b = pyatq.lib().atq_glue_workspace_size(1920, 1080)
+    assert 0 < a < b
+
+
+def test_synthetic_frames_are_reproducible():
+    from pyatq import synth
+    a, b = synth.plain_small(1000), synth.plain_small(1000)
+    assert np.array_equal(a.image, b.image) and a.image.shape == (480, 640)
+    assert not np.array_equal(a.image, synth.plain_small(1001).image)
+    fr = synth.plain_1080p(2000)
+    assert fr.image.shape == (1080, 1920) and len(fr.corners) == 30
+
+
+def test_batch_mixed_draws_listed_sizes():
+    from pyatq import synth
+    import numpy as _np
+    rng = _np.random.Generator(_np.random.PCG64(4000))
+    # only the size/decimate draws are checked (rendering 16 frames is slow)
+    for _ in range(16):
+        w, h = synth.BATCH_SIZES[int(rng.integers(len(synth.BATCH_SIZES)))]
+        tags = int(rng.integers(5, 151))
+        dec = int(rng.integers(1, 3))
+        dw, dh = synth.decimated_dims(w, h, dec)
+        assert dw % 4 == 0 and dh % 4 == 0 and 5 <= tags <= 150
+
+
+def test_product_path_does_not_touch_oracle():
+    """Nothing under the package or the C sources loads the oracle."""
+    pkg = os.path.join(ROOT, "sycl-apriltag-test_amd")
+    for dp, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".cpp", ".hpp")):
+                txt = open(os.path.join(dp, f), errors="ignore").read()
+                assert "pyoracle" not in txt and "liboracle" not in txt, f
