@@ -120,7 +120,8 @@ typedef struct atq_counts {
     uint32_t n_quads;         /* quads surviving the host checks (A21)     */
     uint32_t overflow;        /* bit flags: capacity exceeded somewhere    */
     uint32_t oob_reads;       /* do_indexing reads past the moment array   */
-    uint32_t reserved[7];
+    uint32_t key_label_bits;  /* lb of the detector's packed pair keys     */
+    uint32_t reserved[6];
 } atq_counts;
 
 /* ---- library ------------------------------------------------------------ */
@@ -264,6 +265,44 @@ int atq_quads_from_fits(void *stream, const atq_fitted_quad *fitted,
                         uint32_t *n_quads_dev, void *workspace,
                         size_t workspace_bytes);
 
+/* ---- differentiable line-fit / corner stage ----------------------------- */
+/* No reference counterpart. The reference's line-fit weight is an integer
+ * read at the wrong row (include/line_fit_point.h:57); this form weights
+ * point i by w_i = s_i * sqrt(1 + gx^2 + gy^2), (gx, gy) the central
+ * differences of a float image at the point's pixel (x+1, y), s_i a per-point
+ * boundary weight, so the corners are differentiable in the pixels and in s.
+ * A quad is four lines fitted (weighted centroid + major axis, float32 with
+ * compensated sums) to the cyclic point ranges [idx0,idx1], [idx1,idx2],
+ * [idx2,idx3], [idx3,idx0] of one angle-sorted cluster - the ranges
+ * do_indexing forms - and corner k intersects lines k and k+1 mod 4, as
+ * process_fitted_quads does (src/main.cpp:1059-1096). */
+typedef struct atq_diff_quad {
+    uint32_t cluster_start, cluster_count; /* the cluster's extents in points */
+    uint32_t idx[4];                       /* ascending indices in the cluster */
+} atq_diff_quad;
+
+/* corners: n_quads x 4 x 2 floats. boundary_weights (n_points) may be NULL
+ * (all 1). w_out (n_points) may be NULL; else receives w_i of every point of
+ * every quad's cluster. */
+int atq_linefit_corners_forward(void *stream, const float *image, size_t width,
+                                size_t height, const atq_cluster_point *points,
+                                size_t n_points, const float *boundary_weights,
+                                const atq_diff_quad *quads, size_t n_quads,
+                                float *corners, float *w_out);
+
+/* Vector-Jacobian product of the forward: given grad_corners = dL/dcorners
+ * (n_quads x 8), ADDS dL/d(boundary weight) into grad_weights (n_points) and
+ * dL/d(pixel) into grad_image (width*height); either may be NULL. Callers
+ * zero them first. */
+int atq_linefit_corners_backward(void *stream, const float *image,
+                                 size_t width, size_t height,
+                                 const atq_cluster_point *points,
+                                 size_t n_points,
+                                 const float *boundary_weights,
+                                 const atq_diff_quad *quads, size_t n_quads,
+                                 const float *grad_corners,
+                                 float *grad_weights, float *grad_image);
+
 /* ---- whole pipeline ----------------------------------------------------- */
 
 typedef struct atq_detector atq_detector;
@@ -286,7 +325,7 @@ enum {
     ATQ_BUF_LABELS = 2,      /* u32 W*H                                  */
     ATQ_BUF_SIZES = 3,       /* u32 W*H                                  */
     ATQ_BUF_BOUNDARY_POINTS = 4, /* atq_boundary_point, grouped by key   */
-    ATQ_BUF_BOUNDARY_KEYS = 5,   /* u64                                  */
+    ATQ_BUF_BOUNDARY_KEYS = 5,   /* u64 packed min<<lb|max, lb = counts.key_label_bits */
     ATQ_BUF_CLUSTER_BOUNDS = 6,  /* filtered atq_cluster_bounds          */
     ATQ_BUF_CLUSTER_POINTS = 7,  /* atq_cluster_point, angle-sorted      */
     ATQ_BUF_CLUSTER_INDEX = 8,   /* u32                                  */
@@ -328,6 +367,26 @@ enum {
 int atq_detector_set_timing(atq_detector *det, int enable);
 int atq_detector_stage_times(atq_detector *det, float *ms, int n);
 const char *atq_stage_name(int stage);
+
+/* ---- batched form --------------------------------------------------------- */
+/* No reference counterpart. Many frames of uneven sizes and decimation
+ * factors through one launch: each frame has its own detector (sized for it)
+ * and input buffer; the frames' pipelines are captured once into a single
+ * HIP graph as parallel branches over internal streams, so a batch is one
+ * graph launch. Results per frame are bit-identical to atq_detect_quads on
+ * that frame and are read through atq_batch_frame's detector. */
+typedef struct atq_batch atq_batch;
+int atq_batch_create(size_t n_frames, const size_t *widths_in,
+                     const size_t *heights_in, const int32_t *decimate,
+                     atq_batch **out);
+int atq_batch_destroy(atq_batch *batch);
+/* Device input buffer (width_in*height_in bytes) of frame i. */
+int atq_batch_input(atq_batch *batch, size_t i, uint8_t **gray_in);
+/* Runs every frame. use_graph != 0: launches the captured graph (capturing
+ * it on the first call); 0: the same branches launched eagerly. Ordered on
+ * `stream` (which must not be the NULL stream when use_graph != 0). */
+int atq_batch_detect(atq_batch *batch, void *stream, int use_graph);
+int atq_batch_frame(atq_batch *batch, size_t i, atq_detector **det);
 
 #ifdef __cplusplus
 }

@@ -103,4 +103,29 @@ int atq_find_boundaries(void *stream, const uint32_t *labels, const uint32_t *si
                                          blob_labels, 25);
 }
 
+int atq_linefit_corners_forward(void *stream, const float *image, size_t width, size_t height,
+                                const atq_cluster_point *points, size_t n_points,
+                                const float *boundary_weights, const atq_diff_quad *quads,
+                                size_t n_quads, float *corners, float *w_out) {
+    if (!image || !points || !quads || !corners) return ATQ_ERR_BAD_ARG;
+    if (width < 3 || height < 3 || n_points > 0xFFFFFFFFu || n_quads > 0xFFFFFFFFu)
+        return ATQ_ERR_BAD_DIMS;
+    if (int r = ensure_init()) return r;
+    return launch_diff_forward(S(stream), image, width, height, points, boundary_weights, quads,
+                               n_quads, corners, w_out);
+}
+
+int atq_linefit_corners_backward(void *stream, const float *image, size_t width, size_t height,
+                                 const atq_cluster_point *points, size_t n_points,
+                                 const float *boundary_weights, const atq_diff_quad *quads,
+                                 size_t n_quads, const float *grad_corners, float *grad_weights,
+                                 float *grad_image) {
+    if (!image || !points || !quads || !grad_corners) return ATQ_ERR_BAD_ARG;
+    if (width < 3 || height < 3 || n_points > 0xFFFFFFFFu || n_quads > 0xFFFFFFFFu)
+        return ATQ_ERR_BAD_DIMS;
+    if (int r = ensure_init()) return r;
+    return launch_diff_backward(S(stream), image, width, height, points, boundary_weights, quads,
+                                n_quads, grad_corners, grad_weights, grad_image);
+}
+
 }  // extern "C"

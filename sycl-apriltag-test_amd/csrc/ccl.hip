@@ -199,34 +199,59 @@ k_ccl_local(const uint8_t *__restrict__ thr, uint32_t w, uint32_t h, uint32_t *_
 }
 
 // One item per (tile, border block); merges the block's tile-crossing edges.
+// Each edge becomes a union of the two local roots (parent words of
+// non-roots point straight at their local root); neighbouring border blocks
+// nearly always join the same two local roots, so a union repeated by the
+// previous lane (same probe) or by an earlier probe of the same block is
+// skipped before it reaches the global atomics.
 __global__ void __launch_bounds__(256)
 k_ccl_border(const uint8_t *__restrict__ thr, uint32_t w, uint32_t h, uint32_t tiles_x,
              uint32_t n_items, uint32_t *__restrict__ parent) {
     constexpr int PER_TILE = TBX + 2 * (TBY - 1);
-    uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
-    if (item >= n_items) return;
-    uint32_t tile = item / PER_TILE, k = item % PER_TILE;
-    int bx, by;
-    if (k < (uint32_t)TBX) { bx = k; by = 0; }
-    else if (k < (uint32_t)(TBX + TBY - 1)) { bx = 0; by = 1 + (k - TBX); }
-    else { bx = TBX - 1; by = 1 + (k - TBX - (TBY - 1)); }
-    int x0 = (tile % tiles_x) * 2 * TBX, y0 = (tile / tiles_x) * 2 * TBY;
-    int x = x0 + 2 * bx, y = y0 + 2 * by;
-    if (x >= (int)w || y >= (int)h) return;
-    const uint8_t *row = thr + (size_t)y * w;
-    uint32_t m = check_mask(row[x], row[x + 1], row[x + w], x, y, w, h);
-    uint32_t id = (uint32_t)y * w + (uint32_t)x;
+    constexpr uint32_t NONE = 0xFFFFFFFFu;
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    bool live = item < n_items;
+    int bx = 0, by = 0, x = 0, y = 0;
+    if (live) {
+        const uint32_t tile = item / PER_TILE, k = item % PER_TILE;
+        if (k < (uint32_t)TBX) { bx = k; by = 0; }
+        else if (k < (uint32_t)(TBX + TBY - 1)) { bx = 0; by = 1 + (k - TBX); }
+        else { bx = TBX - 1; by = 1 + (k - TBX - (TBY - 1)); }
+        const int x0 = (tile % tiles_x) * 2 * TBX, y0 = (tile / tiles_x) * 2 * TBY;
+        x = x0 + 2 * bx;
+        y = y0 + 2 * by;
+        live = x < (int)w && y < (int)h;
+    }
+    uint32_t m = 0, id = 0;
+    if (live) {
+        const uint8_t *row = thr + (size_t)y * w;
+        m = check_mask(row[x], row[x + 1], row[x + w], x, y, w, h);
+        id = (uint32_t)y * w + (uint32_t)x;
+    }
+    uint32_t ea[6], eb[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) {
-        Probe q = PROBES[i];
-        if (!(m & (1u << q.bit))) continue;
-        int nbx = bx + q.dbx, nby = by + q.dby;
-        if (nbx >= 0 && nbx < TBX && nby >= 0) continue;  // tile-internal: done locally
-        uint8_t v = thr[(size_t)(y + q.dy) * w + (x + q.dx)];
-        if (v != C_WHITE && v != C_BLACK) continue;
-        uint32_t c = v == C_WHITE ? 0 : 1;
-        uint32_t nid = (uint32_t)(y + 2 * q.dby) * w + (uint32_t)(x + 2 * q.dbx);
-        g_union(parent, id + c, nid + c);
+        ea[i] = NONE;
+        eb[i] = NONE;
+        const Probe q = PROBES[i];
+        const int nbx = bx + q.dbx, nby = by + q.dby;
+        if (live && (m & (1u << q.bit)) && !(nbx >= 0 && nbx < TBX && nby >= 0)) {
+            const uint8_t v = thr[(size_t)(y + q.dy) * w + (x + q.dx)];
+            if (v == C_WHITE || v == C_BLACK) {
+                const uint32_t c = v == C_WHITE ? 0 : 1;
+                const uint32_t nid = (uint32_t)(y + 2 * q.dby) * w + (uint32_t)(x + 2 * q.dbx);
+                ea[i] = g_load(&parent[id + c]) & ~FLAG;
+                eb[i] = g_load(&parent[nid + c]) & ~FLAG;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const uint32_t pa = __shfl_up(ea[i], 1, 64), pb = __shfl_up(eb[i], 1, 64);
+        bool dup = __lane_id() > 0 && pa == ea[i] && pb == eb[i];
+#pragma unroll
+        for (int j = 0; j < i; j++) dup = dup || (ea[j] == ea[i] && eb[j] == eb[i]);
+        if (ea[i] != NONE && !dup) g_union(parent, ea[i], eb[i]);
     }
 }
 

@@ -22,12 +22,15 @@ namespace atq {
 namespace {
 
 constexpr uint32_t NONE = 0xFFFFFFFFu;
-constexpr int CS_THREADS = 256;
-constexpr uint32_t LDS_KEYS = 4096;  // 32 KiB of u64 keys
 
+// Segment heads: keys differ above `shift` (the detector's keys carry the
+// plane in their 2 low bits).
 struct HeadFlag {
     const uint64_t *keys;
-    ATQ_DEV uint32_t operator()(uint32_t i) const { return i == 0 || keys[i] != keys[i - 1]; }
+    uint32_t shift;
+    ATQ_DEV uint32_t operator()(uint32_t i) const {
+        return i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift);
+    }
 };
 struct HeadOut {
     uint32_t *seg_id, *seg_start;
@@ -188,62 +191,55 @@ struct RunOut {
     }
 };
 
-// Angle sort + line-fit moments + prefix sums: one work-group per cluster.
-// `sort` off: the points are already in angle order; `lf` null: no moments.
+// Angle sort + line-fit moments + prefix sums: one 1024-thread work-group
+// per cluster. Keys are slope << 32 | position in the cluster, so equal
+// slopes keep the input order (the reference's sort made stable); they are
+// sorted in LDS (dynamic, up to 128 KiB) or, for larger clusters, in global
+// scratch. `sort` off: the input is already in angle order. `out` null: no
+// points written. `lf` null: no moments.
+constexpr int CS_THREADS = 1024;
+
 __global__ void __launch_bounds__(CS_THREADS)
-k_cluster_sort_fit(atq_cluster_point *__restrict__ cpts, const atq_cluster_extents *__restrict__ ext,
-                   const uint32_t *n_clusters_dev, uint32_t cap_cl, bool sort,
-                   const uint8_t *__restrict__ grey, uint32_t w, uint32_t h,
-                   uint64_t *__restrict__ scratch, atq_line_fit_point *__restrict__ lf) {
-    __shared__ uint64_t keys_lds[LDS_KEYS];
-    __shared__ int64_t lds4[4];
+k_cluster_sort_fit(const atq_cluster_point *__restrict__ in, atq_cluster_point *__restrict__ out,
+                   const atq_cluster_extents *__restrict__ ext, const uint32_t *n_clusters_dev,
+                   uint32_t cap_cl, bool sort, uint32_t lds_keys, const uint8_t *__restrict__ grey,
+                   uint32_t w, uint32_t h, uint64_t *__restrict__ scratch,
+                   atq_line_fit_point *__restrict__ lf) {
+    extern __shared__ uint64_t keys_lds[];
+    __shared__ int64_t lds16[16];
     const uint32_t ncl = min(*n_clusters_dev, cap_cl);
     for (uint32_t c = blockIdx.x; c < ncl; c += gridDim.x) {
         const uint32_t s0 = ext[c].start, n = ext[c].count;
-        const bool in_lds = n <= LDS_KEYS;
+        const bool in_lds = n <= lds_keys;
         uint64_t *keys = in_lds ? keys_lds : scratch + s0;
         if (sort) {
-            for (uint32_t j = threadIdx.x; j < n; j += CS_THREADS) {
-                atq_cluster_point p = cpts[s0 + j];
-                keys[j] = angle_key(p.packed_x, p.packed_y, p.slope);
-            }
+            for (uint32_t j = threadIdx.x; j < n; j += CS_THREADS)
+                keys[j] = ((uint64_t)float_key(in[s0 + j].slope) << 32) | j;
             if (!in_lds) __threadfence_block();
             __syncthreads();
             block_bitonic_sort<uint64_t>(keys, n, CS_THREADS, !in_lds);
-            if (!lf) {
-                for (uint32_t j = threadIdx.x; j < n; j += CS_THREADS) {
-                    atq_cluster_point p;
-                    angle_key_decode(keys[j], p);
-                    cpts[s0 + j] = p;
-                }
-                __syncthreads();
-                continue;
-            }
         }
         int64_t carry[6] = {0, 0, 0, 0, 0, 0};
         for (uint32_t base = 0; base < n; base += CS_THREADS) {
-            uint32_t j = base + threadIdx.x;
+            const uint32_t j = base + threadIdx.x;
             atq_line_fit_point m{0, 0, 0, 0, 0, 0, 0, 0};
             if (j < n) {
-                atq_cluster_point p;
-                if (sort) {
-                    angle_key_decode(keys[j], p);
-                    cpts[s0 + j] = p;
-                } else {
-                    p = cpts[s0 + j];
-                }
-                m = initial_linefit(p.packed_x, p.packed_y, w, h, grey);
+                const uint32_t src = sort ? (uint32_t)keys[j] : j;
+                const atq_cluster_point p = in[s0 + src];
+                if (out) out[s0 + j] = p;
+                if (lf) m = initial_linefit(p.packed_x, p.packed_y, w, h, grey);
             }
+            if (!lf) continue;
             int64_t v[6] = {m.Mx, m.My, m.Mxx, m.Myy, m.Mxy, m.W};
-            int64_t out[6];
+            int64_t o6[6];
 #pragma unroll
             for (int f = 0; f < 6; f++) {
                 int64_t tot;
-                int64_t ex = block_exclusive_scan<int64_t>(v[f], lds4, &tot);
-                out[f] = carry[f] + ex + v[f];
+                int64_t ex = block_exclusive_scan<int64_t, CS_THREADS>(v[f], lds16, &tot);
+                o6[f] = carry[f] + ex + v[f];
                 carry[f] += tot;
             }
-            if (j < n) lf[s0 + j] = atq_line_fit_point{out[0], out[1], out[2], out[3], out[4], out[5], 0, 0};
+            if (j < n) lf[s0 + j] = atq_line_fit_point{o6[0], o6[1], o6[2], o6[3], o6[4], o6[5], 0, 0};
         }
         __syncthreads();
     }
@@ -262,10 +258,11 @@ int launch_run_extents(hipStream_t s, const uint32_t *cidx, const uint32_t *n_de
                        n_runs_dev);
 }
 
-int launch_segments(hipStream_t s, const uint64_t *keys, const uint32_t *n_dev, atq_counts *counts,
-                    uint32_t cap, uint32_t *seg_id, uint32_t *seg_start, uint32_t *scan_scratch) {
-    int r = device_scan(s, n_dev, cap, HeadFlag{keys}, HeadOut{seg_id, seg_start}, scan_scratch,
-                        &counts->n_pairs);
+int launch_segments(hipStream_t s, const uint64_t *keys, uint32_t key_shift, const uint32_t *n_dev,
+                    atq_counts *counts, uint32_t cap, uint32_t *seg_id, uint32_t *seg_start,
+                    uint32_t *scan_scratch) {
+    int r = device_scan(s, n_dev, cap, HeadFlag{keys, key_shift}, HeadOut{seg_id, seg_start},
+                        scan_scratch, &counts->n_pairs);
     if (r) return r;
     k_seg_close<<<1, 1, 0, s>>>(n_dev, counts, cap, seg_start);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
@@ -295,12 +292,29 @@ int launch_assign(hipStream_t s, const atq_boundary_point *pts, const uint32_t *
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
-int launch_cluster_sort_fit(hipStream_t s, atq_cluster_point *cpts, const atq_cluster_extents *ext,
-                            const uint32_t *n_clusters_dev, uint32_t cap_cl, bool sort,
-                            const uint8_t *grey, uint32_t w, uint32_t h, uint64_t *scratch,
-                            atq_line_fit_point *lf) {
-    k_cluster_sort_fit<<<2048, CS_THREADS, 0, s>>>(cpts, ext, n_clusters_dev, cap_cl, sort, grey, w,
-                                                   h, scratch, lf);
+// Keys that fit in one work-group's LDS next to the static allocations.
+static uint32_t angle_sort_lds_keys() {
+    static uint32_t keys = 0;
+    if (keys) return keys;
+    int dev = 0, max_lds = 65536;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    uint32_t k = 16384;
+    while (k > 1024 && (size_t)k * 8 + 1024 > (size_t)max_lds) k >>= 1;
+    if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_cluster_sort_fit),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)(k * 8)) != hipSuccess)
+        k = 4096;  // within the default dynamic allowance
+    keys = k;
+    return keys;
+}
+
+int launch_cluster_sort_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out,
+                            const atq_cluster_extents *ext, const uint32_t *n_clusters_dev,
+                            uint32_t cap_cl, bool sort, const uint8_t *grey, uint32_t w, uint32_t h,
+                            uint64_t *scratch, atq_line_fit_point *lf) {
+    const uint32_t keys = angle_sort_lds_keys();
+    k_cluster_sort_fit<<<1024, CS_THREADS, keys * 8, s>>>(in, out, ext, n_clusters_dev, cap_cl, sort,
+                                                          keys, grey, w, h, scratch, lf);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 

@@ -6,8 +6,9 @@
 namespace atq {
 
 // clusters.hip
-int launch_segments(hipStream_t s, const uint64_t *keys, const uint32_t *n_dev, atq_counts *counts,
-                    uint32_t cap, uint32_t *seg_id, uint32_t *seg_start, uint32_t *scan_scratch);
+int launch_segments(hipStream_t s, const uint64_t *keys, uint32_t key_shift, const uint32_t *n_dev,
+                    atq_counts *counts, uint32_t cap, uint32_t *seg_id, uint32_t *seg_start,
+                    uint32_t *scan_scratch);
 int launch_bounds_filter(hipStream_t s, const atq_boundary_point *pts, const uint32_t *seg_start,
                          atq_counts *counts, uint32_t cap, uint32_t w, uint32_t h, uint32_t *valid,
                          atq_cluster_bounds *seg_bounds, uint32_t *seg_cluster,
@@ -18,10 +19,10 @@ int launch_assign(hipStream_t s, const atq_boundary_point *pts, const uint32_t *
                   const uint32_t *seg_start, const uint32_t *seg_cluster,
                   const atq_cluster_bounds *bounds, const atq_cluster_extents *ext,
                   const uint32_t *n_dev, uint32_t cap, atq_cluster_point *cpts, uint32_t *cidx);
-int launch_cluster_sort_fit(hipStream_t s, atq_cluster_point *cpts, const atq_cluster_extents *ext,
-                            const uint32_t *n_clusters_dev, uint32_t cap_cl, bool sort,
-                            const uint8_t *grey, uint32_t w, uint32_t h, uint64_t *scratch,
-                            atq_line_fit_point *lf);
+int launch_cluster_sort_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out,
+                            const atq_cluster_extents *ext, const uint32_t *n_clusters_dev,
+                            uint32_t cap_cl, bool sort, const uint8_t *grey, uint32_t w, uint32_t h,
+                            uint64_t *scratch, atq_line_fit_point *lf);
 int launch_assign_search(hipStream_t s, const atq_boundary_point *pts, const uint32_t *n_dev,
                          uint32_t cap, const atq_cluster_bounds *bounds, const atq_counts *counts,
                          const atq_cluster_extents *ext, atq_cluster_point *cpts, uint32_t *cidx);
@@ -35,8 +36,7 @@ int launch_fit_lines(hipStream_t s, const atq_line_fit_point *lf, const uint32_t
 int launch_set_u32(hipStream_t s, uint32_t *p, uint32_t v);
 int launch_zero_u32(hipStream_t s, uint32_t *p, const uint32_t *n_dev, uint32_t cap);
 int launch_corner_segments(hipStream_t s, const atq_corner *sparse, const uint32_t *n_sparse_dev,
-                           uint32_t cap, bool count_first, uint32_t *ccount, uint32_t *cursor,
-                           atq_counts *counts, uint32_t cap_cl, uint32_t *cstart,
+                           uint32_t cap, atq_counts *counts, uint32_t cap_cl,
                            atq_peak_extents *peaks, atq_corner *compacted, uint64_t *scratch,
                            uint32_t *scan_scratch);
 int launch_do_indexing(hipStream_t s, const atq_peak_extents *peaks, const uint32_t *n_seg_dev,

@@ -11,9 +11,8 @@
 // halo). Here one thread per point evaluates that definition directly from
 // the prefix-summed moments.
 //
-// Corners (error != 0) are grouped per cluster and sorted by (error desc,
-// index asc) - src/main.cpp:832-907 with a stable sort - by one
-// work-group per cluster. do_indexing runs one 256-item work-group per peak
+// Corners (error != 0) are compacted, grouped per cluster and sorted by
+// (error desc, index asc) - src/main.cpp:832-907 with a stable sort. do_indexing runs one 256-item work-group per peak
 // segment like the reference (src/fit_quad.cpp:392-535), pairing segment e
 // with cluster extents[e] as the reference does. The quad checks of
 // process_fitted_quads run one thread per fitted quad.
@@ -128,25 +127,6 @@ ATQ_DEV void combo_decode(uint32_t num, uint32_t out[4]) {
     }
 }
 
-struct CornerCount {
-    const uint32_t *ccount;
-    ATQ_DEV uint32_t operator()(uint32_t c) const { return ccount[c]; }
-};
-struct CornerStartOut {
-    uint32_t *cstart;
-    ATQ_DEV void operator()(uint32_t c, uint32_t pre, uint32_t) const { cstart[c] = pre; }
-};
-struct HasCorner {
-    const uint32_t *ccount;
-    ATQ_DEV uint32_t operator()(uint32_t c) const { return ccount[c] != 0; }
-};
-struct PeakOut {
-    const uint32_t *ccount, *cstart;
-    atq_peak_extents *peaks;
-    ATQ_DEV void operator()(uint32_t c, uint32_t pre, uint32_t v) const {
-        if (v) peaks[pre] = atq_peak_extents{cstart[c], ccount[c]};
-    }
-};
 struct QuadFlag {
     const uint32_t *flags;
     ATQ_DEV uint32_t operator()(uint32_t e) const { return flags[e]; }
@@ -202,42 +182,54 @@ k_fit_lines(const atq_line_fit_point *__restrict__ lf, const uint32_t *__restric
     }
 }
 
-// Per-cluster corner counts from a sparse corner array (error != 0 only).
-__global__ void __launch_bounds__(256)
-k_count_corners(const atq_corner *__restrict__ sparse, const uint32_t *n_dev, uint32_t cap,
-                uint32_t *__restrict__ ccount) {
-    const uint32_t N = n_dev ? min(*n_dev, cap) : cap;
-    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N; x += gridDim.x * blockDim.x) {
-        atq_corner c = sparse[x];
-        if (c.error != 0.0f) atomicAdd(&ccount[c.cluster_index], 1u);
+// Corner grouping without atomics: the sparse corners (one slot per cluster
+// point, in cluster order) are compacted by a scan on error != 0, the runs
+// of equal cluster index become the peak segments, and each segment is
+// sorted by (error desc, index asc) - the reference's order made stable,
+// since the index is unique within a cluster.
+namespace {
+struct CornerFlag {
+    const atq_corner *sparse;
+    ATQ_DEV uint32_t operator()(uint32_t i) const { return sparse[i].error != 0.0f; }
+};
+struct CornerOut {
+    const atq_corner *sparse;
+    atq_corner *out;
+    ATQ_DEV void operator()(uint32_t i, uint32_t pre, uint32_t v) const {
+        if (v) out[pre] = sparse[i];
+    }
+};
+struct SegHead {
+    const atq_corner *c;
+    ATQ_DEV uint32_t operator()(uint32_t i) const {
+        return i == 0 || c[i].cluster_index != c[i - 1].cluster_index;
+    }
+};
+struct SegStart {
+    atq_peak_extents *peaks;
+    ATQ_DEV void operator()(uint32_t i, uint32_t pre, uint32_t v) const {
+        if (v) peaks[pre].start = i;
+    }
+};
+}  // namespace
+
+__global__ void k_peak_close(const atq_counts *counts, uint32_t cap, atq_peak_extents *peaks) {
+    const uint32_t E = min(counts->n_segments, cap), nc = counts->n_corners;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        const uint32_t end = e + 1 < E ? peaks[e + 1].start : nc;
+        peaks[e].count = end - peaks[e].start;
     }
 }
 
-// Drops every corner (error != 0) into its cluster's slot range, in any order.
 __global__ void __launch_bounds__(256)
-k_corner_scatter(const atq_corner *__restrict__ sparse, const uint32_t *n_dev, uint32_t cap,
-                 const uint32_t *__restrict__ cstart, uint32_t *__restrict__ cursor,
-                 atq_corner *__restrict__ out) {
-    const uint32_t N = n_dev ? min(*n_dev, cap) : cap;
-    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N; x += gridDim.x * blockDim.x) {
-        atq_corner c = sparse[x];
-        if (c.error != 0.0f) out[cstart[c.cluster_index] + atomicAdd(&cursor[c.cluster_index], 1u)] = c;
-    }
-}
-
-// One work-group per cluster with corners: sort its slot range by
-// (error desc, index asc) - the reference's corner order made stable, since the
-// index is unique within a cluster.
-__global__ void __launch_bounds__(256)
-k_corner_sort(const uint32_t *__restrict__ ccount, const uint32_t *__restrict__ cstart,
-              const atq_counts *counts, atq_corner *__restrict__ out, uint64_t *__restrict__ scratch) {
+k_corner_sort(const atq_peak_extents *__restrict__ peaks, const atq_counts *counts, uint32_t cap,
+              atq_corner *__restrict__ out, uint64_t *__restrict__ scratch) {
     constexpr uint32_t LDS_KEYS = 2048;
     __shared__ uint64_t keys_lds[LDS_KEYS];
-    const uint32_t ncl = counts->n_clusters;
-    for (uint32_t c = blockIdx.x; c < ncl; c += gridDim.x) {
-        const uint32_t m = ccount[c];
-        if (m == 0) continue;
-        const uint32_t cs = cstart[c];
+    const uint32_t E = min(counts->n_segments, cap);
+    for (uint32_t e = blockIdx.x; e < E; e += gridDim.x) {
+        const uint32_t cs = peaks[e].start, m = peaks[e].count;
+        const uint32_t c = out[cs].cluster_index;
         const bool in_lds = m <= LDS_KEYS;
         uint64_t *keys = in_lds ? keys_lds : scratch + cs;
         for (uint32_t r = threadIdx.x; r < m; r += 256) {
@@ -424,25 +416,17 @@ int launch_fit_lines(hipStream_t s, const atq_line_fit_point *lf, const uint32_t
 }
 
 int launch_corner_segments(hipStream_t s, const atq_corner *sparse, const uint32_t *n_sparse_dev,
-                           uint32_t cap, bool count_first, uint32_t *ccount, uint32_t *cursor,
-                           atq_counts *counts, uint32_t cap_cl, uint32_t *cstart,
+                           uint32_t cap, atq_counts *counts, uint32_t cap_cl,
                            atq_peak_extents *peaks, atq_corner *compacted, uint64_t *scratch,
                            uint32_t *scan_scratch) {
-    unsigned grid = min(4096u, atq_div_up(cap, 256) + 1);
-    unsigned gcl = min(1024u, atq_div_up(cap_cl, 256) + 1);
-    if (count_first) {
-        k_zero_u32<<<gcl, 256, 0, s>>>(ccount, &counts->n_clusters, cap_cl);
-        k_count_corners<<<grid, 256, 0, s>>>(sparse, n_sparse_dev, cap, ccount);
-    }
-    int r = device_scan(s, &counts->n_clusters, cap_cl, CornerCount{ccount}, CornerStartOut{cstart},
+    int r = device_scan(s, n_sparse_dev, cap, CornerFlag{sparse}, CornerOut{sparse, compacted},
                         scan_scratch, &counts->n_corners);
     if (r) return r;
-    r = device_scan(s, &counts->n_clusters, cap_cl, HasCorner{ccount}, PeakOut{ccount, cstart, peaks},
-                    scan_scratch, &counts->n_segments);
+    r = device_scan(s, &counts->n_corners, cap, SegHead{compacted}, SegStart{peaks}, scan_scratch,
+                    &counts->n_segments);
     if (r) return r;
-    k_zero_u32<<<gcl, 256, 0, s>>>(cursor, &counts->n_clusters, cap_cl);
-    k_corner_scatter<<<grid, 256, 0, s>>>(sparse, n_sparse_dev, cap, cstart, cursor, compacted);
-    k_corner_sort<<<2048, 256, 0, s>>>(ccount, cstart, counts, compacted, scratch);
+    k_peak_close<<<min(1024u, atq_div_up(cap_cl, 256) + 1), 256, 0, s>>>(counts, cap_cl, peaks);
+    k_corner_sort<<<2048, 256, 0, s>>>(peaks, counts, cap_cl, compacted, scratch);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 

@@ -22,6 +22,16 @@ int launch_find_boundaries_sparse(hipStream_t s, const uint32_t *labels, const u
 int launch_find_boundaries_compact(hipStream_t s, const uint32_t *labels, const uint32_t *sizes,
                                    size_t w, size_t h, atq_boundary_point *points,
                                    uint64_t *keys, uint32_t capacity, atq_counts *counts,
-                                   int min_size);
+                                   int min_size, uint32_t label_bits, uint32_t *tile_scratch,
+                                   uint32_t *scan_scratch);
+constexpr unsigned BOUNDARY_TILE_PX = 1024;  // pixels per compaction tile
+
+// diff.hip
+int launch_diff_forward(hipStream_t s, const float *img, size_t w, size_t h,
+                        const atq_cluster_point *pts, const float *sw, const atq_diff_quad *quads,
+                        size_t nq, float *corners, float *w_out);
+int launch_diff_backward(hipStream_t s, const float *img, size_t w, size_t h,
+                         const atq_cluster_point *pts, const float *sw, const atq_diff_quad *quads,
+                         size_t nq, const float *gcorners, float *gw, float *gimg);
 
 }  // namespace atq

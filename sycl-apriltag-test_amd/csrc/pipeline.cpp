@@ -10,6 +10,7 @@
 
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "pipeline.hpp"
 
@@ -56,13 +57,13 @@ size_t Workspace::carve(char *base, const Caps &c) {
     seg_cluster = carve_one<uint32_t>(base, off, c.cap);
     seg_bounds = carve_one<atq_cluster_bounds>(base, off, c.cap);
     scratch64 = carve_one<uint64_t>(base, off, c.cap);
-    ccount = carve_one<uint32_t>(base, off, c.cap_cl);
-    cstart = carve_one<uint32_t>(base, off, c.cap_cl);
-    cursor = carve_one<uint32_t>(base, off, c.cap_cl);
+    cpts_tmp = carve_one<atq_cluster_point>(base, off, c.cap);
+    tiles = carve_one<uint32_t>(base, off, 2 * ((size_t)c.npx / BOUNDARY_TILE_PX + 1));
     qtmp = carve_one<atq_quad_corners>(base, off, c.cap_cl);
     qflags = carve_one<uint32_t>(base, off, c.cap_cl);
     scan = carve_one<uint32_t>(base, off, SCAN_GROUPS + 1);
-    hist = carve_one<uint32_t>(base, off, 256 * RS_GROUPS);
+    hist = carve_one<uint32_t>(base, off, 2048 * RS_GROUPS);
+    digit_base = carve_one<uint32_t>(base, off, 2048);
     counts = carve_one<atq_counts>(base, off, 1);
     return (off + 255) & ~(size_t)255;
 }
@@ -159,37 +160,39 @@ int run_pipeline(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t 
     const uint8_t *grey = f > 1 ? d->gray : gray_in;
     TRY(launch_ccl(s, d->thr, w, h, d->parent, d->labels, d->sizes));
     TRY(mark(1 + ATQ_STAGE_CCL));
+    // Boundary records in the reference's order, keyed by the packed pair key
+    // min<<lb|max with the plane in 2 low bits; the stable sort then yields
+    // the reference's (pair, plane, y, x) order.
+    const RadixPlan plan = radix_plan(w * h, 2);
+    TRY(launch_set_u32(s, &cnt->key_label_bits, (uint32_t)plan.label_bits));
     TRY(launch_find_boundaries_compact(s, d->labels, d->sizes, w, h,
                                        reinterpret_cast<atq_boundary_point *>(d->vals), d->keys,
-                                       c.cap, cnt, d->params.min_component_size));
+                                       c.cap, cnt, d->params.min_component_size,
+                                       (uint32_t)plan.label_bits, ws.tiles, ws.scan));
     TRY(mark(1 + ATQ_STAGE_BOUNDARY));
-    uint32_t shifts[8];
-    int nsh = pair_key_shifts(w * h, shifts);
     bool alt = false;
-    TRY(radix_sort_pairs(s, d->keys, d->vals, ws.keys_alt, ws.vals_alt, &cnt->n_boundary, c.cap,
-                         shifts, nsh, ws.hist, &alt));
+    TRY(radix_sort_packed(s, plan, d->keys, d->vals, ws.keys_alt, ws.vals_alt, &cnt->n_boundary,
+                          c.cap, ws.hist, ws.digit_base, &alt));
     TRY(mark(1 + ATQ_STAGE_KEY_SORT));
     d->sorted_in_alt = alt;
     const uint64_t *K = alt ? ws.keys_alt : d->keys;
     const atq_boundary_point *V =
         reinterpret_cast<const atq_boundary_point *>(alt ? ws.vals_alt : d->vals);
-    TRY(launch_segments(s, K, &cnt->n_boundary, cnt, c.cap, ws.seg_id, ws.seg_start, ws.scan));
+    TRY(launch_segments(s, K, 2, &cnt->n_boundary, cnt, c.cap, ws.seg_id, ws.seg_start, ws.scan));
     TRY(launch_bounds_filter(s, V, ws.seg_start, cnt, c.cap, (uint32_t)w, (uint32_t)h, ws.valid,
                              ws.seg_bounds, ws.seg_cluster, d->bounds, ws.scan));
     TRY(launch_extents(s, d->bounds, cnt, c.cap_cl, d->cext, ws.scan));
     TRY(launch_assign(s, V, ws.seg_id, ws.seg_start, ws.seg_cluster, d->bounds, d->cext,
-                      &cnt->n_boundary, c.cap, d->cpts, d->cidx));
+                      &cnt->n_boundary, c.cap, ws.cpts_tmp, d->cidx));
     TRY(mark(1 + ATQ_STAGE_CLUSTERS));
-    TRY(launch_cluster_sort_fit(s, d->cpts, d->cext, &cnt->n_clusters, c.cap_cl, true, grey,
-                                (uint32_t)w, (uint32_t)h, ws.scratch64, d->lf));
+    TRY(launch_cluster_sort_fit(s, ws.cpts_tmp, d->cpts, d->cext, &cnt->n_clusters, c.cap_cl, true,
+                                grey, (uint32_t)w, (uint32_t)h, ws.scratch64, d->lf));
     TRY(mark(1 + ATQ_STAGE_ANGLE_FIT));
-    TRY(launch_zero_u32(s, ws.ccount, &cnt->n_clusters, c.cap_cl));
     TRY(launch_fit_lines(s, d->lf, d->cidx, d->cext, &cnt->n_cluster_points, c.cap, d->sparse,
-                         ws.ccount));
+                         nullptr));
     TRY(mark(1 + ATQ_STAGE_FIT_LINES));
-    TRY(launch_corner_segments(s, d->sparse, &cnt->n_cluster_points, c.cap, false, ws.ccount,
-                               ws.cursor, cnt, c.cap_cl, ws.cstart, d->peaks, d->corners,
-                               ws.scratch64, ws.scan));
+    TRY(launch_corner_segments(s, d->sparse, &cnt->n_cluster_points, c.cap, cnt, c.cap_cl, d->peaks,
+                               d->corners, ws.scratch64, ws.scan));
     TRY(mark(1 + ATQ_STAGE_CORNERS));
     TRY(launch_do_indexing(s, d->peaks, &cnt->n_segments, c.cap_cl, d->corners, d->lf,
                            &cnt->n_cluster_points, 0, d->cext, d->fitted, &cnt->oob_reads));
@@ -226,17 +229,17 @@ int atq_group_boundaries(void *stream, const atq_boundary_point *points, const u
     uint32_t *opts = reinterpret_cast<uint32_t *>(out_points);
     TRY(device_scan(s, n_dev, n, SparseFlag{pts}, SparseOut{pts, blob_labels, opts, out_keys},
                     ws.scan, count_dev));
-    uint32_t shifts[8];
-    int nsh = pair_key_shifts(width * height, shifts);
+    const RadixPlan plan = radix_plan(width * height, 0);
+    const uint32_t lb = (uint32_t)plan.label_bits;
+    TRY(launch_pack_keys(s, out_keys, out_keys, count_dev, n, lb, false));
     bool alt = false;
-    TRY(radix_sort_pairs(s, out_keys, opts, ws.keys_alt, ws.vals_alt, count_dev, n, shifts, nsh,
-                         ws.hist, &alt));
-    if (alt) {
-        ATQ_HIP_CHECK(hipMemcpyAsync(out_keys, ws.keys_alt, sizeof(uint64_t) * n,
-                                     hipMemcpyDeviceToDevice, s));
+    TRY(radix_sort_packed(s, plan, out_keys, opts, ws.keys_alt, ws.vals_alt, count_dev, n, ws.hist,
+                          ws.digit_base, &alt));
+    // unpack into the caller's buffer (from wherever the last pass wrote)
+    TRY(launch_pack_keys(s, alt ? ws.keys_alt : out_keys, out_keys, count_dev, n, lb, true));
+    if (alt)
         ATQ_HIP_CHECK(hipMemcpyAsync(opts, ws.vals_alt, sizeof(uint32_t) * n,
                                      hipMemcpyDeviceToDevice, s));
-    }
     // Stream-ordered copy of the count back is not needed: it is on device.
     return ATQ_OK;
 }
@@ -251,7 +254,8 @@ int atq_cluster_bounds_filter(void *stream, const atq_boundary_point *points, co
     TRY(take_workspace(ws, workspace, workspace_bytes, width, height));
     hipStream_t s = S(stream);
     const Caps c = Caps::for_pixels(width * height);
-    TRY(launch_segments(s, keys, n_points_dev, counts_dev, c.cap, ws.seg_id, ws.seg_start, ws.scan));
+    TRY(launch_segments(s, keys, 0, n_points_dev, counts_dev, c.cap, ws.seg_id, ws.seg_start,
+                        ws.scan));
     return launch_bounds_filter(s, points, ws.seg_start, counts_dev, c.cap, (uint32_t)width,
                                 (uint32_t)height, ws.valid, ws.seg_bounds, ws.seg_cluster,
                                 out_bounds, ws.scan);
@@ -271,9 +275,9 @@ int atq_cluster_points_sort(void *stream, const atq_boundary_point *points,
     ws.carve(static_cast<char *>(workspace), c);
     TRY(launch_extents(s, bounds, counts_dev, c.cap_cl, out_extents, ws.scan));
     TRY(launch_assign_search(s, points, n_points_dev, c.cap, bounds, counts_dev, out_extents,
-                             out_points, out_cluster_index));
-    return launch_cluster_sort_fit(s, out_points, out_extents, &counts_dev->n_clusters, c.cap_cl,
-                                   true, nullptr, 0, 0, ws.scratch64, nullptr);
+                             ws.cpts_tmp, out_cluster_index));
+    return launch_cluster_sort_fit(s, ws.cpts_tmp, out_points, out_extents, &counts_dev->n_clusters,
+                                   c.cap_cl, true, nullptr, 0, 0, ws.scratch64, nullptr);
 }
 
 int atq_linefit_prefix(void *stream, const atq_cluster_point *points,
@@ -283,10 +287,9 @@ int atq_linefit_prefix(void *stream, const atq_cluster_point *points,
     if (!points || !extents || !counts_dev || !grayscale || !out) return ATQ_ERR_BAD_ARG;
     TRY(check_dims(width, height));
     const Caps c = Caps::for_pixels(width * height);
-    // sort == false: the points are read, never written.
-    return launch_cluster_sort_fit(S(stream), const_cast<atq_cluster_point *>(points), extents,
-                                   &counts_dev->n_clusters, c.cap_cl, false, grayscale,
-                                   (uint32_t)width, (uint32_t)height, nullptr, out);
+    return launch_cluster_sort_fit(S(stream), points, nullptr, extents, &counts_dev->n_clusters,
+                                   c.cap_cl, false, grayscale, (uint32_t)width, (uint32_t)height,
+                                   nullptr, out);
 }
 
 int atq_fit_lines(void *stream, const atq_line_fit_point *points, const uint32_t *cluster_indices,
@@ -312,9 +315,8 @@ int atq_corner_segments(void *stream, const atq_corner *found_corners, atq_count
     Workspace ws;
     const Caps c = caps_from_workspace(workspace_bytes);
     ws.carve(static_cast<char *>(workspace), c);
-    return launch_corner_segments(s, found_corners, &counts_dev->n_cluster_points, c.cap, true,
-                                  ws.ccount, ws.cursor, counts_dev, c.cap_cl, ws.cstart, segments,
-                                  compacted, ws.scratch64, ws.scan);
+    return launch_corner_segments(s, found_corners, &counts_dev->n_cluster_points, c.cap, counts_dev,
+                                  c.cap_cl, segments, compacted, ws.scratch64, ws.scan);
 }
 
 int atq_do_indexing(void *stream, const atq_peak_extents *extents, size_t extents_count,
@@ -455,6 +457,129 @@ int atq_detector_dims(atq_detector *det, size_t *width, size_t *height) {
     if (!det || !width || !height) return ATQ_ERR_BAD_ARG;
     *width = det->w;
     *height = det->h;
+    return ATQ_OK;
+}
+
+}  // extern "C"
+
+// ---- batched form ---------------------------------------------------------
+
+struct atq_batch {
+    std::vector<atq_detector *> dets;
+    std::vector<uint8_t *> inputs;
+    std::vector<size_t> w_in, h_in;
+    static constexpr int N_STREAMS = 4;
+    hipStream_t streams[N_STREAMS] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t joins[N_STREAMS] = {};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+// Every frame's pipeline on internal streams, forked from and joined to `s`.
+int batch_branches(atq_batch *b, hipStream_t s) {
+    ATQ_HIP_CHECK(hipEventRecord(b->fork, s));
+    const int ns = (int)std::min<size_t>(atq_batch::N_STREAMS, b->dets.size());
+    for (int k = 0; k < ns; k++) ATQ_HIP_CHECK(hipStreamWaitEvent(b->streams[k], b->fork, 0));
+    for (size_t i = 0; i < b->dets.size(); i++)
+        TRY(run_pipeline(b->dets[i], b->streams[i % ns], b->inputs[i], b->w_in[i], b->h_in[i]));
+    for (int k = 0; k < ns; k++) {
+        ATQ_HIP_CHECK(hipEventRecord(b->joins[k], b->streams[k]));
+        ATQ_HIP_CHECK(hipStreamWaitEvent(s, b->joins[k], 0));
+    }
+    return ATQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int atq_batch_destroy(atq_batch *b) {
+    if (!b) return ATQ_ERR_BAD_ARG;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (auto *d : b->dets)
+        if (d) atq_detector_destroy(d);
+    for (auto *p : b->inputs)
+        if (p) (void)hipFree(p);
+    for (int k = 0; k < atq_batch::N_STREAMS; k++) {
+        if (b->streams[k]) (void)hipStreamDestroy(b->streams[k]);
+        if (b->joins[k]) (void)hipEventDestroy(b->joins[k]);
+    }
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    delete b;
+    return ATQ_OK;
+}
+
+int atq_batch_create(size_t n_frames, const size_t *widths_in, const size_t *heights_in,
+                     const int32_t *decimate, atq_batch **out) {
+    if (!out || !widths_in || !heights_in || n_frames == 0) return ATQ_ERR_BAD_ARG;
+    *out = nullptr;
+    if (int r = atq_init()) return r;
+    atq_batch *b = new (std::nothrow) atq_batch();
+    if (!b) return ATQ_ERR_BAD_ARG;
+    int rc = ATQ_OK;
+    for (size_t i = 0; i < n_frames && rc == ATQ_OK; i++) {
+        atq_params p{decimate ? decimate[i] : 1, 5, 25, 8};
+        atq_detector *d = nullptr;
+        rc = atq_detector_create(widths_in[i], heights_in[i], &p, &d);
+        b->dets.push_back(d);
+        uint8_t *in = nullptr;
+        if (rc == ATQ_OK && hipMalloc(&in, widths_in[i] * heights_in[i]) != hipSuccess) rc = ATQ_ERR_HIP;
+        b->inputs.push_back(in);
+        b->w_in.push_back(widths_in[i]);
+        b->h_in.push_back(heights_in[i]);
+    }
+    for (int k = 0; k < atq_batch::N_STREAMS && rc == ATQ_OK; k++) {
+        if (hipStreamCreateWithFlags(&b->streams[k], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&b->joins[k], hipEventDisableTiming) != hipSuccess)
+            rc = ATQ_ERR_HIP;
+    }
+    if (rc == ATQ_OK && hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess)
+        rc = ATQ_ERR_HIP;
+    if (rc != ATQ_OK) {
+        atq_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return ATQ_OK;
+}
+
+int atq_batch_input(atq_batch *b, size_t i, uint8_t **gray_in) {
+    if (!b || !gray_in || i >= b->inputs.size()) return ATQ_ERR_BAD_ARG;
+    *gray_in = b->inputs[i];
+    return ATQ_OK;
+}
+
+int atq_batch_frame(atq_batch *b, size_t i, atq_detector **det) {
+    if (!b || !det || i >= b->dets.size()) return ATQ_ERR_BAD_ARG;
+    *det = b->dets[i];
+    return ATQ_OK;
+}
+
+int atq_batch_detect(atq_batch *b, void *stream, int use_graph) {
+    if (!b) return ATQ_ERR_BAD_ARG;
+    hipStream_t s = S(stream);
+    if (!use_graph) return batch_branches(b, s);
+    if (!s) return ATQ_ERR_BAD_ARG;
+    if (!b->exec) {
+        for (auto *d : b->dets)
+            if (d->timing) return ATQ_ERR_BAD_ARG;  // no timing events inside a graph
+        ATQ_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = batch_branches(b, s);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc != ATQ_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return ATQ_ERR_HIP;
+        b->graph = g;
+        ATQ_HIP_CHECK(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    }
+    ATQ_HIP_CHECK(hipGraphLaunch(b->exec, s));
     return ATQ_OK;
 }
 

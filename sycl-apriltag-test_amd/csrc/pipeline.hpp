@@ -26,11 +26,12 @@ struct Workspace {
     uint32_t *seg_id, *seg_start, *valid, *seg_cluster;
     atq_cluster_bounds *seg_bounds;
     uint64_t *scratch64;
-    uint32_t *ccount, *cstart, *cursor;
+    atq_cluster_point *cpts_tmp;
+    uint32_t *tiles;
     atq_quad_corners *qtmp;
     uint32_t *qflags;
     uint32_t *scan;
-    uint32_t *hist;
+    uint32_t *hist, *digit_base;
     atq_counts *counts;
 
     // Carves the layout from `base` (nullptr: only measures). Returns bytes.

@@ -84,20 +84,4 @@ ATQ_DEV atq_line_fit_point initial_linefit(uint16_t px, uint16_t py, uint32_t w,
     return r;
 }
 
-// Angle-sort key: slope, then the reference's compaction order
-// (plane, y, x), then the colour-direction bit (never needed to break ties:
-// (plane, y, x) is unique per record). 63 bits.
-ATQ_DEV uint64_t angle_key(uint16_t px, uint16_t py, float slope) {
-    uint64_t lo = ((uint64_t)(px >> 14) << 29) | ((uint64_t)(py & COORD_VALUE_MASK) << 15) |
-                  ((uint64_t)(px & COORD_VALUE_MASK) << 1) | (uint64_t)(py >> 15);
-    return ((uint64_t)float_key(slope) << 32) | lo;
-}
-ATQ_DEV void angle_key_decode(uint64_t k, atq_cluster_point &p) {
-    uint32_t lo = (uint32_t)k;
-    uint16_t hp = (lo >> 29) & 3u, y = (lo >> 15) & COORD_VALUE_MASK, x = (lo >> 1) & COORD_VALUE_MASK;
-    p.packed_x = (uint16_t)(x | (hp << 14));
-    p.packed_y = (uint16_t)(y | ((lo & 1u) << 15));
-    p.slope = float_from_key((uint32_t)(k >> 32));
-}
-
 }  // namespace atq

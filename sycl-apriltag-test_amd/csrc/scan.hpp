@@ -13,9 +13,11 @@ namespace atq {
 constexpr int SCAN_THREADS = 256;
 constexpr unsigned SCAN_GROUPS = 1024;
 
-// Exclusive scan over the 256 threads of a work-group; *total = group sum.
-template <typename T>
-ATQ_DEV T block_exclusive_scan(T v, T *lds4, T *total) {
+// Exclusive scan over the NT threads of a work-group (lds holds NT/64
+// entries); *total = group sum.
+template <typename T, int NT = 256>
+ATQ_DEV T block_exclusive_scan(T v, T *lds, T *total) {
+    constexpr int NW = NT / 64;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     T x = v;
 #pragma unroll
@@ -23,12 +25,12 @@ ATQ_DEV T block_exclusive_scan(T v, T *lds4, T *total) {
         T y = __shfl_up(x, o, 64);
         if (lane >= o) x += y;
     }
-    if (lane == 63) lds4[wave] = x;
+    if (lane == 63) lds[wave] = x;
     __syncthreads();
     T before = 0, sum = 0;
 #pragma unroll
-    for (int w = 0; w < 4; w++) {
-        T t = lds4[w];
+    for (int w = 0; w < NW; w++) {
+        T t = lds[w];
         if (w < wave) before += t;
         sum += t;
     }
@@ -57,7 +59,7 @@ template <typename F>
 __global__ void __launch_bounds__(SCAN_THREADS)
 k_scan_partials(const uint32_t *count, uint32_t count_cap, F f, uint32_t *partials) {
     __shared__ uint32_t lds4[4];
-    uint32_t n = min(*count, count_cap);
+    uint32_t n = count ? min(*count, count_cap) : count_cap;
     uint32_t b, e;
     chunk_range(n, blockIdx.x, gridDim.x, &b, &e);
     uint32_t acc = 0;
@@ -98,7 +100,7 @@ template <typename F, typename O>
 __global__ void __launch_bounds__(SCAN_THREADS)
 k_scan_apply(const uint32_t *count, uint32_t count_cap, F f, const uint32_t *partials, O o) {
     __shared__ uint32_t lds4[4];
-    uint32_t n = min(*count, count_cap);
+    uint32_t n = count ? min(*count, count_cap) : count_cap;
     uint32_t b, e;
     chunk_range(n, blockIdx.x, gridDim.x, &b, &e);
     uint32_t carry = partials[blockIdx.x];
@@ -112,8 +114,9 @@ k_scan_apply(const uint32_t *count, uint32_t count_cap, F f, const uint32_t *par
     }
 }
 
-// Launches the three scan kernels. `scratch` holds SCAN_GROUPS + 1 u32;
-// *total receives the sum (device memory).
+// Launches the three scan kernels over min(*count, count_cap) elements
+// (count == nullptr: exactly count_cap). `scratch` holds SCAN_GROUPS + 1
+// u32; *total receives the sum (device memory).
 template <typename F, typename O>
 int device_scan(hipStream_t s, const uint32_t *count, uint32_t count_cap, F f, O o,
                 uint32_t *scratch, uint32_t *total) {

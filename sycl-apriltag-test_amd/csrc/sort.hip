@@ -1,49 +1,78 @@
 // sort.hip - LSD radix sort of (u64 key, u32 value) records (row A9).
 //
 // Groups the boundary records by cluster-pair key (the reference's
-// sort_by_key, src/main.cpp:515-518). Only the key bits that can be set are
-// sorted: a pair key is min_label << 32 | max_label with labels below W*H,
-// so the passes cover bits [0, lb) and [32, 32 + lb), lb = bit length of
-// W*H - 1. Each 8-bit pass is three kernels over a fixed grid of G
-// work-groups owning contiguous chunks (the record count is read on the
-// device): per-chunk digit histograms, one scan of the digit-major
-// histogram table, and a stable scatter that ranks equal digits inside each
-// wave with ballots (a 64-lane multisplit) and across the 4 waves in LDS.
-// The sort is stable, so equal keys keep their input order.
+// sort_by_key, src/main.cpp:515-518). A pair key is min_label << 32 |
+// max_label with labels below W*H; it is packed to min_label << lb |
+// max_label (lb = bit length of W*H - 1), which keeps the order, so only
+// 2*lb bits are sorted: 4 passes of 10-11 bit digits at 1080p, 5 at 12 MP
+// (the reference's key has 64).
+//
+// One pass = four kernels over a fixed grid of G work-groups, each owning a
+// contiguous chunk (the record count is read on the device):
+//   k_rs_hist       per-chunk digit histogram (LDS atomics)
+//   k_rs_scan_rows  one work-group per digit: exclusive scan of its G counts
+//   k_rs_scan_base  one work-group: exclusive scan of the digit totals
+//   k_rs_scatter    rounds of 2048 records (8 per lane): each wave ranks its
+//                   512 records digit by digit with ballots (a 64-lane
+//                   multisplit), the 4 waves' counts are scanned in LDS, and
+//                   every record is written at
+//                   digit base + chunk offset + wave offset + rank.
+// Records keep their input order within a digit, so the sort is stable.
 #include "sort.hpp"
 
 namespace atq {
 
 namespace {
 constexpr int RS_THREADS = 256;
-constexpr int RS_BUCKETS = 256;
+constexpr int RS_ITEMS = 8;                       // records per lane per round
+constexpr int RS_ROUND = RS_THREADS * RS_ITEMS;   // 2048
+constexpr int RS_WAVE_SPAN = 64 * RS_ITEMS;       // 512 records per wave per round
 }  // namespace
 
+template <int BITS>
 __global__ void __launch_bounds__(RS_THREADS)
 k_rs_hist(const uint64_t *__restrict__ keys, const uint32_t *count, uint32_t cap, uint32_t shift,
           uint32_t *__restrict__ hist) {
-    __shared__ uint32_t h[RS_BUCKETS];
-    h[threadIdx.x] = 0;
+    constexpr int B = 1 << BITS;
+    __shared__ uint32_t h[B];
+    for (int i = threadIdx.x; i < B; i += RS_THREADS) h[i] = 0;
     __syncthreads();
     uint32_t n = min(*count, cap);
     uint32_t b, e;
     chunk_range(n, blockIdx.x, gridDim.x, &b, &e);
     for (uint32_t i = b + threadIdx.x; i < e; i += RS_THREADS)
-        atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 0xFFu], 1u);
+        atomicAdd(&h[(uint32_t)(keys[i] >> shift) & (B - 1)], 1u);
     __syncthreads();
-    hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+    for (int d = threadIdx.x; d < B; d += RS_THREADS) hist[(size_t)d * gridDim.x + blockIdx.x] = h[d];
 }
 
-// Exclusive scan of the 256*G digit-major histogram table by one work-group.
-__global__ void __launch_bounds__(1024) k_rs_scan(uint32_t *hist, uint32_t total_entries) {
+// One work-group per digit row: exclusive scan over the G chunks in place.
+__global__ void __launch_bounds__(RS_THREADS)
+k_rs_scan_rows(uint32_t *__restrict__ hist, uint32_t groups, uint32_t *__restrict__ row_total) {
+    __shared__ uint32_t lds4[4];
+    uint32_t *row = hist + (size_t)blockIdx.x * groups;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < groups; base += RS_THREADS) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t v = i < groups ? row[i] : 0u;
+        uint32_t tot;
+        uint32_t ex = block_exclusive_scan<uint32_t>(v, lds4, &tot);
+        if (i < groups) row[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) row_total[blockIdx.x] = carry;
+}
+
+// One work-group: exclusive scan of the digit totals in place.
+__global__ void __launch_bounds__(1024) k_rs_scan_base(uint32_t *row_total, uint32_t nbuckets) {
     __shared__ uint32_t wsum[16];
     __shared__ uint32_t carry_s;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     if (t == 0) carry_s = 0;
     __syncthreads();
-    for (uint32_t base = 0; base < total_entries; base += 1024) {
+    for (uint32_t base = 0; base < nbuckets; base += 1024) {
         uint32_t i = base + t;
-        uint32_t v = i < total_entries ? hist[i] : 0u;
+        uint32_t v = i < nbuckets ? row_total[i] : 0u;
         uint32_t x = v;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
@@ -58,87 +87,158 @@ __global__ void __launch_bounds__(1024) k_rs_scan(uint32_t *hist, uint32_t total
             sum += wsum[w];
         }
         uint32_t carry = carry_s;
-        if (i < total_entries) hist[i] = carry + before + x - v;
+        if (i < nbuckets) row_total[i] = carry + before + x - v;
         __syncthreads();
         if (t == 0) carry_s = carry + sum;
         __syncthreads();
     }
 }
 
+template <int BITS>
 __global__ void __launch_bounds__(RS_THREADS)
 k_rs_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
              uint64_t *__restrict__ kout, uint32_t *__restrict__ vout, const uint32_t *count,
-             uint32_t cap, uint32_t shift, const uint32_t *__restrict__ hist) {
-    __shared__ uint32_t wc[4][RS_BUCKETS];   // per-wave digit counts -> prefixes
-    __shared__ uint32_t base_off[RS_BUCKETS];
-    const int t = threadIdx.x, wave = t >> 6;
-    base_off[t] = hist[t * gridDim.x + blockIdx.x];
+             uint32_t cap, uint32_t shift, const uint32_t *__restrict__ hist,
+             const uint32_t *__restrict__ digit_base) {
+    constexpr int B = 1 << BITS;
+    __shared__ uint32_t wc[4][B];      // per-wave digit counts, then wave offsets
+    __shared__ uint32_t base_off[B];   // running output offset of each digit
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    for (int d = t; d < B; d += RS_THREADS)
+        base_off[d] = digit_base[d] + hist[(size_t)d * gridDim.x + blockIdx.x];
     uint32_t n = min(*count, cap);
     uint32_t b, e;
     chunk_range(n, blockIdx.x, gridDim.x, &b, &e);
     const uint64_t lt = lanemask_lt();
-    for (uint32_t r = b; r < e; r += RS_THREADS) {
-#pragma unroll
-        for (int w = 0; w < 4; w++) wc[w][t] = 0;
-        __syncthreads();
-        uint32_t i = r + t;
-        bool valid = i < e;
-        uint64_t k = valid ? kin[i] : 0ull;
-        uint32_t v = valid ? vin[i] : 0u;
-        uint32_t d = (uint32_t)(k >> shift) & 0xFFu;
-        uint64_t m = __ballot(valid);
-#pragma unroll
-        for (int bit = 0; bit < 8; bit++) {
-            uint64_t bb = __ballot((d >> bit) & 1u);
-            m &= ((d >> bit) & 1u) ? bb : ~bb;
+    for (uint32_t r = b; r < e; r += RS_ROUND) {
+        for (int d = t; d < B; d += RS_THREADS) {
+            wc[0][d] = 0;
+            wc[1][d] = 0;
+            wc[2][d] = 0;
+            wc[3][d] = 0;
         }
-        uint32_t rank = (uint32_t)__popcll(m & lt);
-        if (valid && rank == 0) wc[wave][d] = (uint32_t)__popcll(m);
         __syncthreads();
-        {
-            uint32_t run = base_off[t];
+        uint64_t k[RS_ITEMS];
+        uint32_t v[RS_ITEMS], dg[RS_ITEMS], rank[RS_ITEMS];
+        const uint32_t wbase = r + wave * RS_WAVE_SPAN;
+#pragma unroll
+        for (int j = 0; j < RS_ITEMS; j++) {
+            const uint32_t i = wbase + j * 64 + lane;
+            const bool valid = i < e;
+            k[j] = valid ? kin[i] : 0ull;
+            v[j] = valid ? vin[i] : 0u;
+            const uint32_t d = (uint32_t)(k[j] >> shift) & (B - 1);
+            dg[j] = valid ? d : 0xFFFFFFFFu;
+            uint64_t m = __ballot(valid);
+#pragma unroll
+            for (int bit = 0; bit < BITS; bit++) {
+                uint64_t bb = __ballot((d >> bit) & 1u);
+                m &= ((d >> bit) & 1u) ? bb : ~bb;
+            }
+            // The lowest lane of each digit group reads and advances the
+            // wave's running count of that digit and shares the old value.
+            const int leader = valid ? __ffsll((unsigned long long)m) - 1 : lane;
+            uint32_t before = 0;
+            if (valid && lane == leader) {
+                before = wc[wave][d];
+                wc[wave][d] = before + (uint32_t)__popcll(m);
+            }
+            before = __shfl(before, leader, 64);
+            rank[j] = before + (uint32_t)__popcll(m & lt);
+        }
+        __syncthreads();
+        for (int d = t; d < B; d += RS_THREADS) {
+            uint32_t run = base_off[d];
 #pragma unroll
             for (int w = 0; w < 4; w++) {
-                uint32_t c = wc[w][t];
-                wc[w][t] = run;
+                uint32_t c = wc[w][d];
+                wc[w][d] = run;
                 run += c;
             }
-            base_off[t] = run;
+            base_off[d] = run;
         }
         __syncthreads();
-        if (valid) {
-            uint32_t pos = wc[wave][d] + rank;
-            kout[pos] = k;
-            vout[pos] = v;
+#pragma unroll
+        for (int j = 0; j < RS_ITEMS; j++) {
+            if (dg[j] != 0xFFFFFFFFu) {
+                uint32_t pos = wc[wave][dg[j]] + rank[j];
+                kout[pos] = k[j];
+                vout[pos] = v[j];
+            }
         }
         __syncthreads();
     }
 }
 
-int radix_sort_pairs(hipStream_t s, uint64_t *keys, uint32_t *vals, uint64_t *keys_alt,
-                     uint32_t *vals_alt, const uint32_t *count, uint32_t cap,
-                     const uint32_t *shifts, int n_shifts, uint32_t *hist, bool *result_in_alt) {
-    const unsigned G = RS_GROUPS;
+// Pair key min<<32|max <-> min<<lb|max.
+__global__ void k_pack_keys(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
+                            const uint32_t *count, uint32_t cap, uint32_t lb) {
+    uint32_t n = min(*count, cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint64_t k = in[i];
+        out[i] = ((k >> 32) << lb) | (k & 0xFFFFFFFFull);
+    }
+}
+__global__ void k_unpack_keys(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
+                              const uint32_t *count, uint32_t cap, uint32_t lb) {
+    uint32_t n = min(*count, cap);
+    const uint64_t lo = (1ull << lb) - 1;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint64_t k = in[i];
+        out[i] = ((k >> lb) << 32) | (k & lo);
+    }
+}
+
+RadixPlan radix_plan(size_t npx, int extra_bits) {
+    RadixPlan p;
+    int lb = 1;
+    while (lb < 31 && ((size_t)1 << lb) < npx) lb++;
+    p.label_bits = lb;
+    const int total = 2 * lb + extra_bits;
+    p.passes = (total + 10) / 11;
+    const int per = (total + p.passes - 1) / p.passes;
+    p.digit_bits = per <= 8 ? 8 : (per <= 10 ? 10 : 11);
+    size_t groups = npx / 4096;
+    p.groups = (uint32_t)(groups < 32 ? 32 : (groups > RS_GROUPS ? RS_GROUPS : groups));
+    return p;
+}
+
+template <int BITS>
+static void rs_pass(hipStream_t s, const RadixPlan &p, uint32_t shift, const uint64_t *ki,
+                    const uint32_t *vi, uint64_t *ko, uint32_t *vo, const uint32_t *count,
+                    uint32_t cap, uint32_t *hist, uint32_t *digit_base) {
+    const unsigned G = p.groups;
+    k_rs_hist<BITS><<<G, RS_THREADS, 0, s>>>(ki, count, cap, shift, hist);
+    k_rs_scan_rows<<<1u << BITS, RS_THREADS, 0, s>>>(hist, G, digit_base);
+    k_rs_scan_base<<<1, 1024, 0, s>>>(digit_base, 1u << BITS);
+    k_rs_scatter<BITS><<<G, RS_THREADS, 0, s>>>(ki, vi, ko, vo, count, cap, shift, hist, digit_base);
+}
+
+int radix_sort_packed(hipStream_t s, const RadixPlan &p, uint64_t *keys, uint32_t *vals,
+                      uint64_t *keys_alt, uint32_t *vals_alt, const uint32_t *count, uint32_t cap,
+                      uint32_t *hist, uint32_t *digit_base, bool *result_in_alt) {
     uint64_t *ki = keys, *ko = keys_alt;
     uint32_t *vi = vals, *vo = vals_alt;
-    for (int p = 0; p < n_shifts; p++) {
-        k_rs_hist<<<G, RS_THREADS, 0, s>>>(ki, count, cap, shifts[p], hist);
-        k_rs_scan<<<1, 1024, 0, s>>>(hist, RS_BUCKETS * G);
-        k_rs_scatter<<<G, RS_THREADS, 0, s>>>(ki, vi, ko, vo, count, cap, shifts[p], hist);
+    for (int pass = 0; pass < p.passes; pass++) {
+        const uint32_t shift = (uint32_t)(pass * p.digit_bits);
+        switch (p.digit_bits) {
+        case 8: rs_pass<8>(s, p, shift, ki, vi, ko, vo, count, cap, hist, digit_base); break;
+        case 10: rs_pass<10>(s, p, shift, ki, vi, ko, vo, count, cap, hist, digit_base); break;
+        default: rs_pass<11>(s, p, shift, ki, vi, ko, vo, count, cap, hist, digit_base); break;
+        }
         uint64_t *tk = ki; ki = ko; ko = tk;
         uint32_t *tv = vi; vi = vo; vo = tv;
     }
-    *result_in_alt = (n_shifts & 1) != 0;
+    *result_in_alt = (p.passes & 1) != 0;
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
-int pair_key_shifts(size_t npx, uint32_t *shifts) {
-    int lb = 1;
-    while (lb < 32 && ((size_t)1 << lb) < npx) lb++;
-    int n = 0;
-    for (int sh = 0; sh < lb; sh += 8) shifts[n++] = (uint32_t)sh;
-    for (int sh = 0; sh < lb; sh += 8) shifts[n++] = (uint32_t)(32 + sh);
-    return n;
+int launch_pack_keys(hipStream_t s, const uint64_t *in, uint64_t *out, const uint32_t *count,
+                     uint32_t cap, uint32_t lb, bool unpack) {
+    unsigned grid = min(4096u, atq_div_up(cap, 256) + 1);
+    if (unpack) k_unpack_keys<<<grid, 256, 0, s>>>(in, out, count, cap, lb);
+    else k_pack_keys<<<grid, 256, 0, s>>>(in, out, count, cap, lb);
+    return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
 }  // namespace atq

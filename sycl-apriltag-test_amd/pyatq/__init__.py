@@ -49,6 +49,14 @@ SIGNATURES = {
     "atq_detector_set_timing": (C.c_int, [_P, C.c_int]),
     "atq_detector_stage_times": (C.c_int, [_P, C.POINTER(C.c_float), C.c_int]),
     "atq_stage_name": (C.c_char_p, [C.c_int]),
+    "atq_batch_create": (C.c_int, [_S, C.POINTER(_S), C.POINTER(_S), C.POINTER(C.c_int32),
+                                   C.POINTER(_P)]),
+    "atq_batch_destroy": (C.c_int, [_P]),
+    "atq_batch_input": (C.c_int, [_P, _S, C.POINTER(_P)]),
+    "atq_batch_detect": (C.c_int, [_P, _P, C.c_int]),
+    "atq_batch_frame": (C.c_int, [_P, _S, C.POINTER(_P)]),
+    "atq_linefit_corners_forward": (C.c_int, [_P, _P, _S, _S, _P, _S, _P, _P, _S, _P, _P]),
+    "atq_linefit_corners_backward": (C.c_int, [_P, _P, _S, _S, _P, _S, _P, _P, _S, _P, _P, _P]),
 }
 N_STAGES = 10
 
@@ -94,6 +102,49 @@ def check(status):
 
 
 # ---- HIP plumbing (the same libamdhip64 instance libatq.so links) ---------
+
+class Batch:
+    """Frames of uneven sizes through one graph launch (atq_batch_*)."""
+
+    def __init__(self, sizes, decimates):
+        init()
+        n = len(sizes)
+        ws = (C.c_size_t * n)(*[w for w, _ in sizes])
+        hs = (C.c_size_t * n)(*[h for _, h in sizes])
+        ds = (C.c_int32 * n)(*decimates)
+        h = C.c_void_p()
+        check(lib().atq_batch_create(n, ws, hs, ds, C.byref(h)))
+        self.handle, self.sizes = h, list(sizes)
+
+    def upload(self, i, img):
+        p = C.c_void_p()
+        check(lib().atq_batch_input(self.handle, i, C.byref(p)))
+        a = np.ascontiguousarray(img)
+        hip_check(_hip().hipMemcpy(p, C.c_void_p(a.ctypes.data), a.nbytes, 1), "hipMemcpy H2D")
+
+    def detect(self, stream_handle, use_graph=True):
+        check(lib().atq_batch_detect(self.handle, C.c_void_p(stream_handle), 1 if use_graph else 0))
+
+    def frame(self, i):
+        """A Detector view (not owning) of frame i's buffers."""
+        p = C.c_void_p()
+        check(lib().atq_batch_frame(self.handle, i, C.byref(p)))
+        d = Detector.__new__(Detector)
+        d.handle = p
+        d._borrowed = True
+        return d
+
+    def close(self):
+        if self.handle:
+            lib().atq_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
 
 _hiplib = None
 
@@ -314,9 +365,9 @@ class Detector:
         self.handle = h
 
     def close(self):
-        if self.handle:
+        if self.handle and not getattr(self, "_borrowed", False):
             lib().atq_detector_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:

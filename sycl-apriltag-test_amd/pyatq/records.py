@@ -16,14 +16,15 @@ FITTED_QUAD = np.dtype([("moments", LINE_FIT_POINT, (4,)), ("num_in_moments", "<
                         ("indices", "<u2", (4,))])
 QUAD_CORNERS = np.dtype([("corners", "<f4", (4, 2)), ("reversed_border", "u1"),
                          ("pad_", "u1", (3,)), ("blob_index", "<u4")])
+DIFF_QUAD = np.dtype([("cluster_start", "<u4"), ("cluster_count", "<u4"), ("idx", "<u4", (4,))])
 COUNTS = np.dtype([(n, "<u4") for n in (
     "n_boundary", "n_pairs", "n_clusters", "n_cluster_points", "n_corners", "n_segments",
-    "n_quads", "overflow", "oob_reads")] + [("reserved", "<u4", (7,))])
+    "n_quads", "overflow", "oob_reads", "key_label_bits")] + [("reserved", "<u4", (6,))])
 
 SIZES = {
     "EXTREMA": 2, "BOUNDARY_POINT": 4, "CLUSTER_POINT": 8, "CLUSTER_EXTENTS": 8,
     "CLUSTER_BOUNDS": 32, "LINE_FIT_POINT": 64, "PEAK_EXTENTS": 8, "CORNER": 12,
-    "FITTED_QUAD": 272, "QUAD_CORNERS": 40, "COUNTS": 64,
+    "FITTED_QUAD": 272, "QUAD_CORNERS": 40, "COUNTS": 64, "DIFF_QUAD": 24,
 }
 for _name, _size in SIZES.items():
     assert globals()[_name].itemsize == _size, (_name, globals()[_name].itemsize)
