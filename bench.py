@@ -50,10 +50,12 @@ def stage_bytes(stage, f, npx, counts, n_shifts):
 
 
 def n_sort_passes(npx):
+    """LSD passes of the boundary key sort: 2 label fields + 2 plane bits in
+    digits of at most 11 bits."""
     lb = 1
-    while (1 << lb) < npx:
+    while lb < 31 and (1 << lb) < npx:
         lb += 1
-    return 2 * ((lb + 7) // 8)
+    return (2 * lb + 2 + 10) // 11
 
 
 def time_frames(atq, det, devs, w, h, steps, warmup, stream):
@@ -82,6 +84,85 @@ def stage_profile(atq, det, devs, w, h, n, stream):
     return {k: v / n for k, v in acc.items()}
 
 
+def bench_12mp(atq, synth, stream, steps=10):
+    """The 4096x3072 stress frame (maze, dense clutter), decimate 1."""
+    t = time.time()
+    big = synth.stress_12mp().image
+    log(f"generated 12MP frame in {time.time() - t:.1f}s")
+    dbig = [atq.dev_from_np(big)]
+    det = atq.Detector(4096, 3072, decimate=1)
+    wall, ev = time_frames(atq, det, dbig, 4096, 3072, steps, 2, stream)
+    stages = stage_profile(atq, det, dbig, 4096, 3072, 5, stream)
+    det.detect(dbig[0], 4096, 3072, stream.handle)
+    stream.sync()
+    c = det.counts()
+    npx12 = 4096 * 3072
+    roof12 = {}
+    for st in ("threshold", "ccl", "boundary", "key_sort"):
+        b = stage_bytes(st, 1, npx12, c, n_sort_passes(npx12))
+        roof12[st] = {"ms": round(stages[st], 4), "GB/s": round(b / (stages[st] * 1e-3) / 1e9, 1),
+                      "frac": round(b / (stages[st] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    out = {
+        "ms_per_frame": round(wall / steps * 1e3, 3),
+        "mpix_per_s": round(steps * npx12 / wall / 1e6, 1),
+        "stage_ms": {k: round(v, 4) for k, v in stages.items()},
+        "stage_roofline": roof12,
+        "counts": {k: int(c[k]) for k in ("n_boundary", "n_pairs", "n_clusters",
+                                          "n_cluster_points", "n_corners", "n_segments",
+                                          "n_quads")},
+    }
+    log(f"12MP: {wall / steps * 1e3:.2f} ms/frame; stages {stages}")
+    det.close()
+    return out
+
+
+def bench_batch(atq, synth, stream, rounds):
+    """16 frames of mixed sizes and decimation: one detector per frame run one
+    after another on one stream (plain) vs the batch's single graph launch."""
+    frames = synth.batch_mixed(4000, n=16)
+    sizes = [(f.image.shape[1], f.image.shape[0]) for f in frames]
+    decs = [f.decimate for f in frames]
+    devs = [atq.dev_from_np(f.image) for f in frames]
+    dets = [atq.Detector(w, h, decimate=d) for (w, h), d in zip(sizes, decs)]
+
+    def plain_round():
+        for (w, h), det, dv in zip(sizes, dets, devs):
+            det.detect(dv, w, h, stream.handle)
+
+    b = atq.Batch(sizes, decs)
+    for i, f in enumerate(frames):
+        b.upload(i, f.image)
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        stream.sync()
+        t0 = time.perf_counter()
+        for _ in range(rounds):
+            fn()
+        stream.sync()
+        return time.perf_counter() - t0
+
+    t_plain = timed(plain_round)
+    t_graph = timed(lambda: b.detect(stream.handle, use_graph=True))
+    t_eager = timed(lambda: b.detect(stream.handle, use_graph=False))
+    same = all(b.frame(i).quads().tobytes() == dets[i].quads().tobytes() for i in range(16))
+    mpix = sum(w * h for w, h in sizes) / 1e6
+    out = {"frames": [f"{w}x{h}/d{d}" for (w, h), d in zip(sizes, decs)],
+           "rounds": rounds,
+           "plain_frames_per_s": round(16 * rounds / t_plain, 1),
+           "batched_graph_frames_per_s": round(16 * rounds / t_graph, 1),
+           "batched_eager_frames_per_s": round(16 * rounds / t_eager, 1),
+           "batched_graph_mpix_per_s": round(mpix * rounds / t_graph, 1),
+           "graph_vs_plain": round(t_plain / t_graph, 3),
+           "per_frame_quads_identical": bool(same)}
+    log(f"batch: {out}")
+    b.close()
+    for det in dets:
+        det.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +173,11 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-12mp", action="store_true")
+    ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--skip-f2", action="store_true", help="no decimate-2 leg")
+    ap.add_argument("--only-12mp", action="store_true",
+                    help="only the 4096x3072 leg, --steps frames (for profiling)")
+    ap.add_argument("--batch-rounds", type=int, default=20)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -105,6 +191,9 @@ def main():
     from pyatq import synth
 
     atq.init()
+    if args.only_12mp:
+        print(json.dumps({"stress_12mp": bench_12mp(atq, synth, atq.Stream(), args.steps)}))
+        return
     t = time.time()
     frames = [synth.plain_1080p(2000 + i).image for i in range(args.frames)]
     log(f"generated {len(frames)} frames in {time.time() - t:.1f}s")
@@ -112,7 +201,7 @@ def main():
     stream = atq.Stream()
 
     res = {}
-    for f in (1, 2):
+    for f in ((1,) if args.skip_f2 else (1, 2)):
         det = atq.Detector(W1080, H1080, decimate=f)
         wall, ev = time_frames(atq, det, devs, W1080, H1080, args.steps, args.warmup, stream)
         stages = stage_profile(atq, det, devs, W1080, H1080, 20, stream)
@@ -129,7 +218,6 @@ def main():
     wall1 = res[1]["wall"]
     mpix = steps * W1080 * H1080 / wall1 / 1e6
     fps = steps / wall1
-    mpix2 = steps * W1080 * H1080 / res[2]["wall"] / 1e6
 
     # dominant stage at the headline workload and its roofline
     npx = W1080 * H1080
@@ -154,45 +242,24 @@ def main():
 
     extra = {
         "frames_per_s": round(fps, 2),
-        "mpix_per_s_decimate2": round(mpix2, 2),
-        "frames_per_s_decimate2": round(steps / res[2]["wall"], 2),
         "event_ms_per_frame": round(res[1]["ev"] / steps * 1e3, 4),
         "stage_ms_1080p_f1": {k: round(v, 5) for k, v in st1.items()},
-        "stage_ms_1080p_f2": {k: round(v, 5) for k, v in res[2]["stages"].items()},
         "stage_roofline_1080p_f1": stage_roof,
         "counts_1080p_f1": {k: int(res[1]["counts"][k]) for k in
                             ("n_boundary", "n_pairs", "n_clusters", "n_cluster_points",
                              "n_corners", "n_segments", "n_quads")},
     }
 
+    if 2 in res:
+        extra["mpix_per_s_decimate2"] = round(steps * W1080 * H1080 / res[2]["wall"] / 1e6, 2)
+        extra["frames_per_s_decimate2"] = round(steps / res[2]["wall"], 2)
+        extra["stage_ms_1080p_f2"] = {k: round(v, 5) for k, v in res[2]["stages"].items()}
+
     if not args.skip_12mp:
-        t = time.time()
-        big = synth.stress_12mp().image
-        log(f"generated 12MP frame in {time.time() - t:.1f}s")
-        dbig = [atq.dev_from_np(big)]
-        det = atq.Detector(4096, 3072, decimate=1)
-        wall, ev = time_frames(atq, det, dbig, 4096, 3072, 10, 2, stream)
-        stages = stage_profile(atq, det, dbig, 4096, 3072, 5, stream)
-        det.detect(dbig[0], 4096, 3072, stream.handle)
-        stream.sync()
-        c = det.counts()
-        npx12 = 4096 * 3072
-        roof12 = {}
-        for st in ("threshold", "ccl", "boundary", "key_sort"):
-            b = stage_bytes(st, 1, npx12, c, n_sort_passes(npx12))
-            roof12[st] = {"ms": round(stages[st], 4), "GB/s": round(b / (stages[st] * 1e-3) / 1e9, 1),
-                          "frac": round(b / (stages[st] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-        extra["stress_12mp"] = {
-            "ms_per_frame": round(wall / 10 * 1e3, 3),
-            "mpix_per_s": round(10 * npx12 / wall / 1e6, 1),
-            "stage_ms": {k: round(v, 4) for k, v in stages.items()},
-            "stage_roofline": roof12,
-            "counts": {k: int(c[k]) for k in ("n_boundary", "n_pairs", "n_clusters",
-                                              "n_cluster_points", "n_corners", "n_segments",
-                                              "n_quads")},
-        }
-        log(f"12MP: {wall / 10 * 1e3:.2f} ms/frame; stages {stages}")
-        det.close()
+        extra["stress_12mp"] = bench_12mp(atq, synth, stream)
+
+    if not args.skip_batch:
+        extra["batched_16_mixed"] = bench_batch(atq, synth, stream, args.batch_rounds)
 
     cpu = None
     if not args.skip_cpu:

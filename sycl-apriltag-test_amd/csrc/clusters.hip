@@ -191,57 +191,166 @@ struct RunOut {
     }
 };
 
-// Angle sort + line-fit moments + prefix sums: one 1024-thread work-group
-// per cluster. Keys are slope << 32 | position in the cluster, so equal
-// slopes keep the input order (the reference's sort made stable); they are
-// sorted in LDS (dynamic, up to 128 KiB) or, for larger clusters, in global
-// scratch. `sort` off: the input is already in angle order. `out` null: no
-// points written. `lf` null: no moments.
-constexpr int CS_THREADS = 1024;
+// Angle sort: keys are slope << 32 | position in the cluster, so equal
+// slopes keep the input order (the reference's sort, made stable). Every
+// cluster is cut into chunks of ANGLE_CHUNK points, each sorted in LDS by one
+// work-group; clusters longer than a chunk are then finished by merge passes
+// in which every key finds its rank in the partner run by binary search
+// (runs double each pass; a cluster needing q passes ends in buffer q % 2).
+constexpr uint32_t ANGLE_CHUNK = 4096;
 
-__global__ void __launch_bounds__(CS_THREADS)
-k_cluster_sort_fit(const atq_cluster_point *__restrict__ in, atq_cluster_point *__restrict__ out,
-                   const atq_cluster_extents *__restrict__ ext, const uint32_t *n_clusters_dev,
-                   uint32_t cap_cl, bool sort, uint32_t lds_keys, const uint8_t *__restrict__ grey,
-                   uint32_t w, uint32_t h, uint64_t *__restrict__ scratch,
-                   atq_line_fit_point *__restrict__ lf) {
-    extern __shared__ uint64_t keys_lds[];
-    __shared__ int64_t lds16[16];
+__global__ void __launch_bounds__(256)
+k_angle_chunks(const atq_cluster_point *__restrict__ in, const atq_cluster_extents *__restrict__ ext,
+               const uint32_t *n_clusters_dev, uint32_t cap_cl, uint32_t max_chunks,
+               uint64_t *__restrict__ keys_out) {
+    __shared__ uint64_t keys[ANGLE_CHUNK];
     const uint32_t ncl = min(*n_clusters_dev, cap_cl);
+    const uint64_t items = (uint64_t)ncl * max_chunks;
+    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
+        const uint32_t c = (uint32_t)(it / max_chunks), k = (uint32_t)(it % max_chunks);
+        const uint32_t s0 = ext[c].start, n = ext[c].count;
+        const uint32_t b = k * ANGLE_CHUNK;
+        if (b >= n) continue;
+        const uint32_t m = min(ANGLE_CHUNK, n - b);
+        for (uint32_t j = threadIdx.x; j < m; j += 256)
+            keys[j] = ((uint64_t)float_key(in[s0 + b + j].slope) << 32) | (b + j);
+        __syncthreads();
+        block_bitonic_sort<uint64_t>(keys, m, 256, false);
+        for (uint32_t j = threadIdx.x; j < m; j += 256) keys_out[s0 + b + j] = keys[j];
+        __syncthreads();
+    }
+}
+
+ATQ_DEV uint32_t lower_bound_u64(const uint64_t *a, uint32_t n, uint64_t key) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// One merge pass over runs of length `run` (clusters longer than `run`).
+__global__ void __launch_bounds__(256)
+k_angle_merge(const uint64_t *__restrict__ kin, uint64_t *__restrict__ kout,
+              const uint32_t *__restrict__ cidx, const atq_cluster_extents *__restrict__ ext,
+              const uint32_t *n_points_dev, uint32_t cap, uint32_t run) {
+    const uint32_t N = min(*n_points_dev, cap);
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N; x += gridDim.x * blockDim.x) {
+        const atq_cluster_extents e = ext[cidx[x]];
+        if (e.count <= run) continue;
+        const uint32_t j = x - e.start;
+        const uint32_t base = j / (2 * run) * (2 * run);
+        const uint64_t *c = kin + e.start;
+        const uint64_t key = c[j];
+        uint32_t pos;
+        if (j - base < run) {  // left run: count the right run's smaller keys
+            const uint32_t rb = min(base + run, e.count), re = min(base + 2 * run, e.count);
+            pos = j + lower_bound_u64(c + rb, re - rb, key);
+        } else {               // right run: count the left run's smaller keys
+            pos = base + (j - base - run) + lower_bound_u64(c + base, run, key);
+        }
+        kout[e.start + pos] = key;
+    }
+}
+
+ATQ_DEV uint32_t merge_passes(uint32_t n) {
+    uint32_t q = 0;
+    for (uint32_t r = ANGLE_CHUNK; r < n; r <<= 1) q++;
+    return q;
+}
+
+// Line-fit points + per-cluster inclusive moment prefix, one work-group per
+// cluster, reading the points in angle order. `keys0/keys1` null: the input
+// is already in order. `out` null: no points written. `lf` null: no moments.
+// Each thread takes FIT_ITEMS consecutive points, so one round of the
+// work-group covers FIT_ITEMS * 256 points with one scan of the six sums.
+constexpr int FIT_ITEMS = 8;
+
+__global__ void __launch_bounds__(256)
+k_cluster_fit(const atq_cluster_point *__restrict__ in, atq_cluster_point *__restrict__ out,
+              const uint64_t *__restrict__ keys0, const uint64_t *__restrict__ keys1,
+              const atq_cluster_extents *__restrict__ ext, const uint32_t *n_clusters_dev,
+              uint32_t cap_cl, const uint8_t *__restrict__ grey, uint32_t w, uint32_t h,
+              atq_line_fit_point *__restrict__ lf) {
+    __shared__ int64_t wsum[4][6];
+    const uint32_t ncl = min(*n_clusters_dev, cap_cl);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint32_t c = blockIdx.x; c < ncl; c += gridDim.x) {
         const uint32_t s0 = ext[c].start, n = ext[c].count;
-        const bool in_lds = n <= lds_keys;
-        uint64_t *keys = in_lds ? keys_lds : scratch + s0;
-        if (sort) {
-            for (uint32_t j = threadIdx.x; j < n; j += CS_THREADS)
-                keys[j] = ((uint64_t)float_key(in[s0 + j].slope) << 32) | j;
-            if (!in_lds) __threadfence_block();
-            __syncthreads();
-            block_bitonic_sort<uint64_t>(keys, n, CS_THREADS, !in_lds);
-        }
+        const uint64_t *keys = keys0 ? ((merge_passes(n) & 1) ? keys1 : keys0) + s0 : nullptr;
         int64_t carry[6] = {0, 0, 0, 0, 0, 0};
-        for (uint32_t base = 0; base < n; base += CS_THREADS) {
-            const uint32_t j = base + threadIdx.x;
-            atq_line_fit_point m{0, 0, 0, 0, 0, 0, 0, 0};
-            if (j < n) {
-                const uint32_t src = sort ? (uint32_t)keys[j] : j;
-                const atq_cluster_point p = in[s0 + src];
-                if (out) out[s0 + j] = p;
-                if (lf) m = initial_linefit(p.packed_x, p.packed_y, w, h, grey);
+        for (uint32_t base = 0; base < n; base += FIT_ITEMS * 256) {
+            const uint32_t j0 = base + threadIdx.x * FIT_ITEMS;
+            uint32_t W[FIT_ITEMS], RX[FIT_ITEMS], RY[FIT_ITEMS];
+            int64_t v[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < FIT_ITEMS; k++) {
+                W[k] = 0;
+                RX[k] = RY[k] = 0;
+                const uint32_t j = j0 + k;
+                if (j < n) {
+                    const uint32_t src = keys ? (uint32_t)keys[j] : j;
+                    const atq_cluster_point p = in[s0 + src];
+                    if (out) out[s0 + j] = p;
+                    if (lf) {
+                        linefit_terms(p.packed_x, p.packed_y, w, h, grey, &W[k], &RX[k], &RY[k]);
+                        const atq_line_fit_point m = linefit_from_terms(W[k], RX[k], RY[k]);
+                        v[0] += m.Mx;
+                        v[1] += m.My;
+                        v[2] += m.Mxx;
+                        v[3] += m.Myy;
+                        v[4] += m.Mxy;
+                        v[5] += m.W;
+                    }
+                }
             }
             if (!lf) continue;
-            int64_t v[6] = {m.Mx, m.My, m.Mxx, m.Myy, m.Mxy, m.W};
-            int64_t o6[6];
+            // inclusive wave scan of the thread totals, then across the 4 waves
+            int64_t x[6];
 #pragma unroll
             for (int f = 0; f < 6; f++) {
-                int64_t tot;
-                int64_t ex = block_exclusive_scan<int64_t, CS_THREADS>(v[f], lds16, &tot);
-                o6[f] = carry[f] + ex + v[f];
+                x[f] = v[f];
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    int64_t y = __shfl_up(x[f], o, 64);
+                    if (lane >= o) x[f] += y;
+                }
+            }
+            if (lane == 63)
+#pragma unroll
+                for (int f = 0; f < 6; f++) wsum[wave][f] = x[f];
+            __syncthreads();
+            int64_t acc[6];
+#pragma unroll
+            for (int f = 0; f < 6; f++) {
+                int64_t before = 0, tot = 0;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int64_t t = wsum[q][f];
+                    if (q < wave) before += t;
+                    tot += t;
+                }
+                acc[f] = carry[f] + before + x[f] - v[f];
                 carry[f] += tot;
             }
-            if (j < n) lf[s0 + j] = atq_line_fit_point{o6[0], o6[1], o6[2], o6[3], o6[4], o6[5], 0, 0};
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < FIT_ITEMS; k++) {
+                const uint32_t j = j0 + k;
+                if (j < n) {
+                    const atq_line_fit_point m = linefit_from_terms(W[k], RX[k], RY[k]);
+                    acc[0] += m.Mx;
+                    acc[1] += m.My;
+                    acc[2] += m.Mxx;
+                    acc[3] += m.Myy;
+                    acc[4] += m.Mxy;
+                    acc[5] += m.W;
+                    lf[s0 + j] = atq_line_fit_point{acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], 0, 0};
+                }
+            }
         }
-        __syncthreads();
     }
 }
 
@@ -292,29 +401,25 @@ int launch_assign(hipStream_t s, const atq_boundary_point *pts, const uint32_t *
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
-// Keys that fit in one work-group's LDS next to the static allocations.
-static uint32_t angle_sort_lds_keys() {
-    static uint32_t keys = 0;
-    if (keys) return keys;
-    int dev = 0, max_lds = 65536;
-    if (hipGetDevice(&dev) == hipSuccess)
-        (void)hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-    uint32_t k = 16384;
-    while (k > 1024 && (size_t)k * 8 + 1024 > (size_t)max_lds) k >>= 1;
-    if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_cluster_sort_fit),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)(k * 8)) != hipSuccess)
-        k = 4096;  // within the default dynamic allowance
-    keys = k;
-    return keys;
+int launch_angle_sort(hipStream_t s, const atq_cluster_point *in, const atq_cluster_extents *ext,
+                      const uint32_t *cidx, const atq_counts *counts, uint32_t cap, uint32_t cap_cl,
+                      uint32_t max_cluster, uint64_t *keys0, uint64_t *keys1) {
+    const uint32_t max_chunks = (max_cluster + ANGLE_CHUNK - 1) / ANGLE_CHUNK;
+    k_angle_chunks<<<2048, 256, 0, s>>>(in, ext, &counts->n_clusters, cap_cl, max_chunks, keys0);
+    const unsigned grid = min(4096u, atq_div_up(cap, 256) + 1);
+    int pass = 0;
+    for (uint32_t run = ANGLE_CHUNK; run < max_cluster; run <<= 1, pass++)
+        k_angle_merge<<<grid, 256, 0, s>>>((pass & 1) ? keys1 : keys0, (pass & 1) ? keys0 : keys1,
+                                           cidx, ext, &counts->n_cluster_points, cap, run);
+    return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
-int launch_cluster_sort_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out,
-                            const atq_cluster_extents *ext, const uint32_t *n_clusters_dev,
-                            uint32_t cap_cl, bool sort, const uint8_t *grey, uint32_t w, uint32_t h,
-                            uint64_t *scratch, atq_line_fit_point *lf) {
-    const uint32_t keys = angle_sort_lds_keys();
-    k_cluster_sort_fit<<<1024, CS_THREADS, keys * 8, s>>>(in, out, ext, n_clusters_dev, cap_cl, sort,
-                                                          keys, grey, w, h, scratch, lf);
+int launch_cluster_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out,
+                       const uint64_t *keys0, const uint64_t *keys1, const atq_cluster_extents *ext,
+                       const uint32_t *n_clusters_dev, uint32_t cap_cl, const uint8_t *grey,
+                       uint32_t w, uint32_t h, atq_line_fit_point *lf) {
+    k_cluster_fit<<<2048, 256, 0, s>>>(in, out, keys0, keys1, ext, n_clusters_dev, cap_cl, grey, w,
+                                       h, lf);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 

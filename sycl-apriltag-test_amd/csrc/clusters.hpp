@@ -19,10 +19,18 @@ int launch_assign(hipStream_t s, const atq_boundary_point *pts, const uint32_t *
                   const uint32_t *seg_start, const uint32_t *seg_cluster,
                   const atq_cluster_bounds *bounds, const atq_cluster_extents *ext,
                   const uint32_t *n_dev, uint32_t cap, atq_cluster_point *cpts, uint32_t *cidx);
-int launch_cluster_sort_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out,
-                            const atq_cluster_extents *ext, const uint32_t *n_clusters_dev,
-                            uint32_t cap_cl, bool sort, const uint8_t *grey, uint32_t w, uint32_t h,
-                            uint64_t *scratch, atq_line_fit_point *lf);
+// Angle sort of every cluster's points (keys0/keys1: ping-pong key buffers
+// of cap entries; a cluster of n points ends in keys1 when it needed an odd
+// number of merge passes). max_cluster bounds the cluster length.
+int launch_angle_sort(hipStream_t s, const atq_cluster_point *in, const atq_cluster_extents *ext,
+                      const uint32_t *cidx, const atq_counts *counts, uint32_t cap, uint32_t cap_cl,
+                      uint32_t max_cluster, uint64_t *keys0, uint64_t *keys1);
+// Points in angle order (via the keys; keys0 null: input already in order)
+// to `out` (may be null) and their line-fit prefix sums to `lf` (may be null).
+int launch_cluster_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out,
+                       const uint64_t *keys0, const uint64_t *keys1, const atq_cluster_extents *ext,
+                       const uint32_t *n_clusters_dev, uint32_t cap_cl, const uint8_t *grey,
+                       uint32_t w, uint32_t h, atq_line_fit_point *lf);
 int launch_assign_search(hipStream_t s, const atq_boundary_point *pts, const uint32_t *n_dev,
                          uint32_t cap, const atq_cluster_bounds *bounds, const atq_counts *counts,
                          const atq_cluster_extents *ext, atq_cluster_point *cpts, uint32_t *cidx);

@@ -8,6 +8,7 @@
 // sized once, at creation, for the largest frame.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -185,8 +186,13 @@ int run_pipeline(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t 
     TRY(launch_assign(s, V, ws.seg_id, ws.seg_start, ws.seg_cluster, d->bounds, d->cext,
                       &cnt->n_boundary, c.cap, ws.cpts_tmp, d->cidx));
     TRY(mark(1 + ATQ_STAGE_CLUSTERS));
-    TRY(launch_cluster_sort_fit(s, ws.cpts_tmp, d->cpts, d->cext, &cnt->n_clusters, c.cap_cl, true,
-                                grey, (uint32_t)w, (uint32_t)h, ws.scratch64, d->lf));
+    // a valid cluster holds at most 2 * (2w + 2h) points (valid_blob)
+    const uint32_t max_cluster = (uint32_t)std::min<size_t>(c.cap, 4 * (w + h));
+    uint64_t *ak = ws.angle_keys();
+    TRY(launch_angle_sort(s, ws.cpts_tmp, d->cext, d->cidx, cnt, c.cap, c.cap_cl, max_cluster, ak,
+                          ak + c.cap));
+    TRY(launch_cluster_fit(s, ws.cpts_tmp, d->cpts, ak, ak + c.cap, d->cext, &cnt->n_clusters,
+                           c.cap_cl, grey, (uint32_t)w, (uint32_t)h, d->lf));
     TRY(mark(1 + ATQ_STAGE_ANGLE_FIT));
     TRY(launch_fit_lines(s, d->lf, d->cidx, d->cext, &cnt->n_cluster_points, c.cap, d->sparse,
                          nullptr));
@@ -276,8 +282,11 @@ int atq_cluster_points_sort(void *stream, const atq_boundary_point *points,
     TRY(launch_extents(s, bounds, counts_dev, c.cap_cl, out_extents, ws.scan));
     TRY(launch_assign_search(s, points, n_points_dev, c.cap, bounds, counts_dev, out_extents,
                              ws.cpts_tmp, out_cluster_index));
-    return launch_cluster_sort_fit(s, ws.cpts_tmp, out_points, out_extents, &counts_dev->n_clusters,
-                                   c.cap_cl, true, nullptr, 0, 0, ws.scratch64, nullptr);
+    uint64_t *ak = ws.angle_keys();
+    TRY(launch_angle_sort(s, ws.cpts_tmp, out_extents, out_cluster_index, counts_dev, c.cap,
+                          c.cap_cl, c.cap, ak, ak + c.cap));
+    return launch_cluster_fit(s, ws.cpts_tmp, out_points, ak, ak + c.cap, out_extents,
+                              &counts_dev->n_clusters, c.cap_cl, nullptr, 0, 0, nullptr);
 }
 
 int atq_linefit_prefix(void *stream, const atq_cluster_point *points,
@@ -287,9 +296,9 @@ int atq_linefit_prefix(void *stream, const atq_cluster_point *points,
     if (!points || !extents || !counts_dev || !grayscale || !out) return ATQ_ERR_BAD_ARG;
     TRY(check_dims(width, height));
     const Caps c = Caps::for_pixels(width * height);
-    return launch_cluster_sort_fit(S(stream), points, nullptr, extents, &counts_dev->n_clusters,
-                                   c.cap_cl, false, grayscale, (uint32_t)width, (uint32_t)height,
-                                   nullptr, out);
+    return launch_cluster_fit(S(stream), points, nullptr, nullptr, nullptr, extents,
+                              &counts_dev->n_clusters, c.cap_cl, grayscale, (uint32_t)width,
+                              (uint32_t)height, out);
 }
 
 int atq_fit_lines(void *stream, const atq_line_fit_point *points, const uint32_t *cluster_indices,

@@ -34,6 +34,10 @@ struct Workspace {
     uint32_t *hist, *digit_base;
     atq_counts *counts;
 
+    // The angle sort's two key buffers (2 * cap entries) reuse seg_bounds
+    // (cap entries of 32 bytes), which is dead once points are assigned.
+    uint64_t *angle_keys() const { return reinterpret_cast<uint64_t *>(seg_bounds); }
+
     // Carves the layout from `base` (nullptr: only measures). Returns bytes.
     size_t carve(char *base, const Caps &c);
 };

@@ -59,29 +59,36 @@ ATQ_DEV float calc_theta(uint16_t px, uint16_t py, float cx, float cy) {
 }
 
 // include/line_fit_point.h:50-86, including the row index taken from rx
-// (`uint32_t y = rx / 2`).
-ATQ_DEV atq_line_fit_point initial_linefit(uint16_t px, uint16_t py, uint32_t w, uint32_t h,
-                                           const uint8_t *__restrict__ grey) {
-    int64_t rx = (int64_t)x_with_dx(px) + 1;
-    int64_t ry = (int64_t)y_with_dy(px, py) + 1;
-    uint32_t x = (uint32_t)(rx / 2);
-    uint32_t y = (uint32_t)(rx / 2);
-    int64_t W = 1;
+// (`uint32_t y = rx / 2`): the weight W and the doubled coordinates + 1 that
+// every moment of the point is built from.
+ATQ_DEV void linefit_terms(uint16_t px, uint16_t py, uint32_t w, uint32_t h,
+                           const uint8_t *__restrict__ grey, uint32_t *W_out, uint32_t *rx_out,
+                           uint32_t *ry_out) {
+    const uint32_t rx = x_with_dx(px) + 1;
+    const uint32_t ry = y_with_dy(px, py) + 1;
+    const uint32_t x = rx / 2;
+    const uint32_t y = rx / 2;
+    uint32_t W = 1;
     if (x > 0 && x + 1 < w && y > 0 && y + 1 < h) {
         int gx = (int)grey[(size_t)y * w + x + 1] - (int)grey[(size_t)y * w + x - 1];
         int gy = (int)grey[(size_t)(y + 1) * w + x] - (int)grey[(size_t)(y - 1) * w + x];
-        W += (int64_t)hypot_f((float)gx, (float)gy);
+        W += (uint32_t)hypot_f((float)gx, (float)gy);
     }
-    atq_line_fit_point r;
-    r.Mx = W * rx;
-    r.My = W * ry;
-    r.Mxx = W * rx * rx;
-    r.Myy = W * ry * ry;
-    r.Mxy = W * rx * ry;
-    r.W = W;
-    r.a = 0;
-    r.b = 0;
-    return r;
+    *W_out = W;
+    *rx_out = rx;
+    *ry_out = ry;
+}
+
+ATQ_DEV atq_line_fit_point linefit_from_terms(uint32_t W, uint32_t rx, uint32_t ry) {
+    const int64_t w = W, x = rx, y = ry;
+    return atq_line_fit_point{w * x, w * y, w * x * x, w * y * y, w * x * y, w, 0, 0};
+}
+
+ATQ_DEV atq_line_fit_point initial_linefit(uint16_t px, uint16_t py, uint32_t w, uint32_t h,
+                                           const uint8_t *__restrict__ grey) {
+    uint32_t W, rx, ry;
+    linefit_terms(px, py, w, h, grey, &W, &rx, &ry);
+    return linefit_from_terms(W, rx, ry);
 }
 
 }  // namespace atq

@@ -38,11 +38,11 @@ ATQ_DEV void block_bitonic_sort(T *a, uint32_t n, int nthreads, bool global_mem)
     uint32_t P = 1;
     while (P < n) P <<= 1;
     const uint32_t t = threadIdx.x;
-    for (uint32_t k = 2; k <= P; k <<= 1) {
+    for (uint32_t lk = 1; (1u << lk) <= P; lk++) {
+        const uint32_t k = 1u << lk;
         // flip: i in the lower half of each k-block against its mirror
         for (uint32_t c = t; c < P / 2; c += nthreads) {
-            uint32_t half = k >> 1;
-            uint32_t i = (c / half) * k + (c % half);
+            uint32_t i = ((c >> (lk - 1)) << lk) | (c & ((k >> 1) - 1));
             uint32_t j = i ^ (k - 1);
             if (j < n) {
                 T x = a[i], y = a[j];
@@ -51,9 +51,10 @@ ATQ_DEV void block_bitonic_sort(T *a, uint32_t n, int nthreads, bool global_mem)
         }
         if (global_mem) __threadfence_block();
         __syncthreads();
-        for (uint32_t st = k >> 2; st > 0; st >>= 1) {
+        for (int ls = (int)lk - 2; ls >= 0; ls--) {
+            const uint32_t st = 1u << ls;
             for (uint32_t c = t; c < P / 2; c += nthreads) {
-                uint32_t i = (c / st) * (2 * st) + (c % st);
+                uint32_t i = ((c >> ls) << (ls + 1)) | (c & (st - 1));
                 uint32_t j = i + st;
                 if (j < n) {
                     T x = a[i], y = a[j];
