@@ -107,6 +107,10 @@ typedef struct atq_params {
     int32_t min_white_black_diff;/* src/threshold.cpp:86 (5)                */
     int32_t min_component_size;  /* src/find_boundaries.cpp:82 (25)         */
     int32_t min_tag_width;       /* src/main.cpp:1037 (8)                   */
+    int32_t pair_table_log2;     /* detector only: its cluster-pair table has
+                                    2^n slots, n in [8, 24]; 0 = automatic
+                                    (about one slot per 64 pixels, 2^14 to
+                                    2^20). No counterpart in the reference. */
 } atq_params;
 
 /* Per-frame counts written by the pipeline (device memory, 16 x u32). */
@@ -121,7 +125,12 @@ typedef struct atq_counts {
     uint32_t overflow;        /* bit flags: capacity exceeded somewhere    */
     uint32_t oob_reads;       /* do_indexing reads past the moment array   */
     uint32_t key_label_bits;  /* lb of the detector's packed pair keys     */
-    uint32_t reserved[6];
+    uint32_t pair_filter_off; /* 1: the detector kept every boundary record
+                                 (its pair table overflowed); 0: it dropped
+                                 the records of pairs whose record count
+                                 ValidBlobFilter rejects (< 24 or
+                                 > 2 * (2w + 2h))                          */
+    uint32_t reserved[5];
 } atq_counts;
 
 /* ---- library ------------------------------------------------------------ */
@@ -326,7 +335,10 @@ enum {
     ATQ_BUF_SIZES = 3,       /* u32 W*H                                  */
     ATQ_BUF_BOUNDARY_POINTS = 4, /* atq_boundary_point, grouped by key   */
     ATQ_BUF_BOUNDARY_KEYS = 5,   /* u64 packed min<<lb|max, lb = counts.key_label_bits */
-    ATQ_BUF_CLUSTER_BOUNDS = 6,  /* filtered atq_cluster_bounds          */
+    ATQ_BUF_CLUSTER_BOUNDS = 6,  /* filtered atq_cluster_bounds; start
+                                    indexes BOUNDARY_POINTS, which holds
+                                    only the records of pairs whose count
+                                    can pass (see pair_filter_off)      */
     ATQ_BUF_CLUSTER_POINTS = 7,  /* atq_cluster_point, angle-sorted      */
     ATQ_BUF_CLUSTER_INDEX = 8,   /* u32                                  */
     ATQ_BUF_CLUSTER_EXTENTS = 9, /* atq_cluster_extents                  */
@@ -338,7 +350,10 @@ enum {
     ATQ_BUF_COUNTS = 15,         /* atq_counts                           */
     ATQ_BUF_EXTREMA = 16,        /* atq_extrema (W/4)*(H/4)              */
     ATQ_BUF_FILTERED_EXTREMA = 17,
-    ATQ_BUF_SPARSE_CORNERS = 18  /* atq_corner, one slot per cluster point */
+    ATQ_BUF_SPARSE_CORNERS = 18, /* atq_corner, one slot per cluster point */
+    ATQ_BUF_PAIR_KEYS = 19,      /* u64 ~(min<<32|max) per pair-table slot, 0 = empty */
+    ATQ_BUF_PAIR_COUNTS = 20     /* u32 record count per slot (exact up to
+                                    2 * (2w + 2h), above it only "greater") */
 };
 /* Device pointer and capacity (in elements) of a detector buffer. */
 int atq_detector_buffer(atq_detector *det, int which, void **ptr,

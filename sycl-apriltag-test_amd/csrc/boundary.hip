@@ -11,7 +11,8 @@
 // local not) << 15; key = min(label values) << 32 | max(label values).
 //
 // Two forms: the drop-in writes the reference's 4 sparse planes in full;
-// the pipeline form writes the records densely in the reference's order.
+// the detector's form writes, densely and in the reference's order, the
+// records of the cluster pairs whose record count can pass ValidBlobFilter.
 #include "kernels.hpp"
 #include "scan.hpp"
 
@@ -19,11 +20,13 @@ namespace atq {
 
 namespace {
 
+// The records of one pixel, one slot per plane (the planes are tested in
+// plane order, so slot order is the reference's record order).
 struct PixelRecords {
-    uint32_t n;
+    uint32_t mask;  // bit p: plane p holds a record
     atq_boundary_point p[4];
     uint64_t k[4];
-    uint8_t plane[4];
+    ATQ_DEV uint32_t n() const { return __popc(mask); }
 };
 
 ATQ_DEV uint32_t label_at(const uint32_t *labels, uint32_t w, uint32_t h, uint32_t x, uint32_t y) {
@@ -34,7 +37,7 @@ ATQ_DEV uint32_t label_at(const uint32_t *labels, uint32_t w, uint32_t h, uint32
 ATQ_DEV void pixel_records(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes,
                            uint32_t w, uint32_t h, uint32_t x, uint32_t y, uint32_t min_size,
                            uint32_t key_shift, PixelRecords &out) {
-    out.n = 0;
+    out.mask = 0;
     if (x == 0 || x >= w || y >= h) return;
     const uint32_t L = labels[(size_t)y * w + x];
     const uint32_t lv = L & LABEL_VALUE_MASK, wl = L & LABEL_PIXEL_MASK;
@@ -43,16 +46,15 @@ ATQ_DEV void pixel_records(const uint32_t *__restrict__ labels, const uint32_t *
     const uint32_t down = label_at(labels, w, h, x, y + 1);
     const uint32_t down_left = label_at(labels, w, h, x - 1, y + 1);
     const uint32_t down_right = label_at(labels, w, h, x + 1, y + 1);
-    auto test = [&](uint32_t T, uint32_t plane) {
+    auto test = [&](uint32_t T, int plane) {
         const uint32_t wt = T & LABEL_PIXEL_MASK;
         if (wt == wl) return;
         const uint32_t tv = T & LABEL_VALUE_MASK;
         if (!local_big || sizes[tv] < min_size) return;
-        uint32_t i = out.n++;
-        out.p[i].packed_x = (uint16_t)(x | (plane << 14));
-        out.p[i].packed_y = (uint16_t)(wt > wl ? (COLOR_DIRECTION_MASK | y) : y);
-        out.k[i] = ((uint64_t)min(lv, tv) << key_shift) | (uint64_t)max(lv, tv);
-        out.plane[i] = (uint8_t)plane;
+        out.mask |= 1u << plane;
+        out.p[plane].packed_x = (uint16_t)(x | ((uint32_t)plane << 14));
+        out.p[plane].packed_y = (uint16_t)(wt > wl ? (COLOR_DIRECTION_MASK | y) : y);
+        out.k[plane] = ((uint64_t)min(lv, tv) << key_shift) | (uint64_t)max(lv, tv);
     };
     test(right, 0);
     {
@@ -78,78 +80,218 @@ k_boundaries_sparse(const uint32_t *__restrict__ labels, const uint32_t *__restr
     PixelRecords r;
     pixel_records(labels, sizes, w, h, x, y, min_size, 32, r);
     const size_t n = (size_t)w * h, lin = (size_t)y * w + x;
-    atq_boundary_point pp[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    uint64_t kk[4] = {~0ull, ~0ull, ~0ull, ~0ull};
-    for (uint32_t i = 0; i < r.n; i++) {
-        pp[r.plane[i]] = r.p[i];
-        kk[r.plane[i]] = r.k[i];
-    }
 #pragma unroll
     for (int p = 0; p < 4; p++) {
-        points[p * n + lin] = pp[p];
-        keys[p * n + lin] = kk[p];
+        const bool has = (r.mask >> p) & 1u;
+        points[p * n + lin] = has ? r.p[p] : atq_boundary_point{0, 0};
+        keys[p * n + lin] = has ? r.k[p] : ~0ull;
     }
 }
 
-// Compact form, in the reference's stable order without atomics: tiles of
-// 1024 consecutive pixels (4 per thread) count their records, the counts are
-// scanned, and each tile writes its records at its offset in raster order.
-// The key is the packed pair key (min << lb | max) with the plane in its two
-// low bits, so a stable sort by it yields (pair, plane, y, x): the
-// reference's plane-major compaction followed by its sort by pair key.
+// Filtered compact form (the detector's). A cluster pair whose record count
+// is below 24 or above 2 * (2w + 2h) is rejected by ValidBlobFilter
+// (include/cluster_bounds.h:85-120) whatever else it holds, so its records
+// never reach a cluster; here they are dropped before the key sort:
+//  1. k_pair_count: tiles of 1024 consecutive pixels (4 per thread) merge
+//     their records per pair in an LDS table, then add the per-pair counts
+//     into a global open-addressing table (key min<<32|max). A count already
+//     above the upper limit is
+//     not added to any more (the pairs that span the frame would otherwise
+//     queue on one address), so a count at or below the limit is exact.
+//  2. k_bnd_filter: each tile looks its pairs up, writes its surviving
+//     records densely into its own region (4 per pixel) in raster order and
+//     plane order within a pixel, and its count.
+//  3. the tile counts are scanned and k_bnd_gather moves the regions
+//     together: the records of the surviving pairs in the reference's order.
+// If the global table runs out of probes, counts->pair_filter_off is set and
+// every record is kept.
 constexpr int BND_PX_PER_THREAD = 4;
 constexpr int BND_TILE = 256 * BND_PX_PER_THREAD;
+constexpr int TILE_SLOTS = 512;   // LDS pair table of a tile
+constexpr int PAIR_PROBES = 64;   // global table probe limit
+constexpr uint32_t NO_SLOT = 0xFFFFFFFFu;
 
-__global__ void __launch_bounds__(256)
-k_bnd_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes, uint32_t w,
-            uint32_t h, uint32_t min_size, uint32_t *__restrict__ tile_counts) {
-    __shared__ uint32_t lds4[4];
-    const size_t n = (size_t)w * h;
-    const size_t p0 = (size_t)blockIdx.x * BND_TILE + threadIdx.x * BND_PX_PER_THREAD;
-    uint32_t c = 0;
-    if (p0 < n) {
-        const uint32_t y = (uint32_t)(p0 / w), x0 = (uint32_t)(p0 % w);
-        for (int k = 0; k < BND_PX_PER_THREAD; k++) {
-            PixelRecords r;
-            pixel_records(labels, sizes, w, h, x0 + k, y, min_size, 0, r);
-            c += r.n;
+ATQ_DEV uint32_t pair_hash(uint64_t key) {
+    return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 32);
+}
+
+// Slot of `key` in the tile's LDS table (inserted when absent and `insert`),
+// or NO_SLOT. Slots hold ~key, 0 = empty.
+ATQ_DEV uint32_t tile_slot(uint64_t *lk, uint64_t key, bool insert) {
+    const uint64_t nk = ~key;
+    uint32_t h = pair_hash(key) & (TILE_SLOTS - 1);
+    for (int p = 0; p < TILE_SLOTS; p++, h = (h + 1) & (TILE_SLOTS - 1)) {
+        const uint64_t k = lk[h];
+        if (k == nk) return h;
+        if (k == 0) {
+            if (!insert) return NO_SLOT;
+            const uint64_t old = atomicCAS((unsigned long long *)&lk[h], 0ull,
+                                           (unsigned long long)nk);
+            if (old == 0 || old == nk) return h;
         }
     }
-    uint32_t t = block_reduce_sum<uint32_t>(c, lds4);
-    if (threadIdx.x == 0) tile_counts[blockIdx.x] = t;
+    return NO_SLOT;
+}
+
+// Slot of `key` in the global table (inserted when absent and `insert`), or
+// NO_SLOT when PAIR_PROBES slots were tried. Slots hold ~key, so that a
+// zero-filled table is empty (key 0 is a real pair: label 0, the unknown
+// pixels, against the white component whose root is node 0).
+ATQ_DEV uint32_t global_slot(const PairTable &t, uint64_t key, bool insert) {
+    const uint64_t nk = ~key;
+    uint32_t h = pair_hash(key) & t.mask;
+    for (int p = 0; p < PAIR_PROBES; p++, h = (h + 1) & t.mask) {
+        const uint64_t k = __hip_atomic_load(&t.keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (k == nk) return h;
+        if (k == 0) {
+            if (!insert) return NO_SLOT;
+            const uint64_t old = atomicCAS((unsigned long long *)&t.keys[h], 0ull,
+                                           (unsigned long long)nk);
+            if (old == 0 || old == nk) return h;
+        }
+    }
+    return NO_SLOT;
+}
+
+ATQ_DEV void global_add(const PairTable &t, uint64_t key, uint32_t c, uint32_t max_count,
+                        atq_counts *counts) {
+    const uint32_t h = global_slot(t, key, true);
+    if (h == NO_SLOT) {
+        counts->pair_filter_off = 1u;
+        return;
+    }
+    if (__hip_atomic_load(&t.cnt[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= max_count)
+        atomicAdd(&t.cnt[h], c);
+}
+
+// Walks a thread's records (4 pixels, planes in order) with a one-entry
+// cache, so runs of one pair reach the LDS table once.
+template <typename F>
+ATQ_DEV void for_each_pair_run(const PixelRecords (&r)[BND_PX_PER_THREAD], F f) {
+    uint64_t cur = 0;
+    uint32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < BND_PX_PER_THREAD; k++)
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            if (!((r[k].mask >> p) & 1u)) continue;
+            if (r[k].k[p] != cur) {
+                if (c) f(cur, c);
+                cur = r[k].k[p];
+                c = 0;
+            }
+            c++;
+        }
+    if (c) f(cur, c);
+}
+
+ATQ_DEV void tile_records(const uint32_t *labels, const uint32_t *sizes, uint32_t w, uint32_t h,
+                          uint32_t min_size, PixelRecords (&r)[BND_PX_PER_THREAD]) {
+    const size_t n = (size_t)w * h;
+    const size_t p0 = (size_t)blockIdx.x * BND_TILE + threadIdx.x * BND_PX_PER_THREAD;
+#pragma unroll
+    for (int k = 0; k < BND_PX_PER_THREAD; k++) r[k].mask = 0;
+    if (p0 >= n) return;
+    const uint32_t y = (uint32_t)(p0 / w), x0 = (uint32_t)(p0 % w);
+#pragma unroll
+    for (int k = 0; k < BND_PX_PER_THREAD; k++)
+        pixel_records(labels, sizes, w, h, x0 + k, y, min_size, 32, r[k]);
 }
 
 __global__ void __launch_bounds__(256)
-k_bnd_emit(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes, uint32_t w,
-           uint32_t h, uint32_t min_size, uint32_t lb, const uint32_t *__restrict__ tile_off,
-           atq_boundary_point *__restrict__ points, uint64_t *__restrict__ keys, uint32_t capacity,
-           atq_counts *counts) {
+k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes, uint32_t w,
+             uint32_t h, uint32_t min_size, PairTable table, uint32_t max_count,
+             atq_counts *counts) {
+    __shared__ uint64_t lk[TILE_SLOTS];
+    __shared__ uint32_t lc[TILE_SLOTS];
+    for (int i = threadIdx.x; i < TILE_SLOTS; i += 256) {
+        lk[i] = 0;
+        lc[i] = 0;
+    }
+    __syncthreads();
+    PixelRecords r[BND_PX_PER_THREAD];
+    tile_records(labels, sizes, w, h, min_size, r);
+    for_each_pair_run(r, [&](uint64_t key, uint32_t c) {
+        const uint32_t sl = tile_slot(lk, key, true);
+        if (sl != NO_SLOT) atomicAdd(&lc[sl], c);
+        else global_add(table, key, c, max_count, counts);
+    });
+    __syncthreads();
+    for (int i = threadIdx.x; i < TILE_SLOTS; i += 256)
+        if (lk[i]) global_add(table, ~lk[i], lc[i], max_count, counts);
+}
+
+__global__ void __launch_bounds__(256)
+k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes, uint32_t w,
+             uint32_t h, uint32_t min_size, uint32_t lb, PairTable table, uint32_t max_count,
+             const atq_counts *counts, uint64_t *__restrict__ tile_keys,
+             atq_boundary_point *__restrict__ tile_points, uint32_t *__restrict__ tile_counts) {
+    __shared__ uint64_t lk[TILE_SLOTS];
+    __shared__ uint32_t lc[TILE_SLOTS];  // 1: the pair survives
     __shared__ uint32_t lds4[4];
-    const size_t n = (size_t)w * h;
-    const size_t p0 = (size_t)blockIdx.x * BND_TILE + threadIdx.x * BND_PX_PER_THREAD;
-    const bool live = p0 < n;
-    const uint32_t y = live ? (uint32_t)(p0 / w) : 0, x0 = live ? (uint32_t)(p0 % w) : 0;
-    uint32_t c = 0;
-    if (live)
-        for (int k = 0; k < BND_PX_PER_THREAD; k++) {
-            PixelRecords r;
-            pixel_records(labels, sizes, w, h, x0 + k, y, min_size, 0, r);
-            c += r.n;
+    for (int i = threadIdx.x; i < TILE_SLOTS; i += 256) {
+        lk[i] = 0;
+        lc[i] = 0;
+    }
+    const bool keep_all = counts->pair_filter_off != 0;
+    __syncthreads();
+    PixelRecords r[BND_PX_PER_THREAD];
+    tile_records(labels, sizes, w, h, min_size, r);
+    auto survives = [&](uint64_t key) -> bool {
+        const uint32_t g = global_slot(table, key, false);
+        if (g == NO_SLOT) return true;
+        const uint32_t c = __hip_atomic_load(&table.cnt[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return c >= 24 && c <= max_count;
+    };
+    if (!keep_all) {
+        for_each_pair_run(r, [&](uint64_t key, uint32_t) { tile_slot(lk, key, true); });
+        __syncthreads();
+        for (int i = threadIdx.x; i < TILE_SLOTS; i += 256)
+            if (lk[i]) lc[i] = survives(~lk[i]) ? 1u : 0u;
+        __syncthreads();
+    }
+    uint32_t keep = 0, c = 0;  // keep: bit 4k+p for record (pixel k, plane p)
+#pragma unroll
+    for (int k = 0; k < BND_PX_PER_THREAD; k++)
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            if (!((r[k].mask >> p) & 1u)) continue;
+            bool ok = true;
+            if (!keep_all) {
+                const uint32_t sl = tile_slot(lk, r[k].k[p], false);
+                ok = sl != NO_SLOT ? lc[sl] != 0 : survives(r[k].k[p]);
+            }
+            if (ok) {
+                keep |= 1u << (4 * k + p);
+                c++;
+            }
         }
     uint32_t tot;
-    uint32_t pos = tile_off[blockIdx.x] + block_exclusive_scan<uint32_t>(c, lds4, &tot);
-    if (!live || c == 0) return;
-    if (pos + c > capacity) {
-        atomicOr(&counts->overflow, OVF_BOUNDARY);
-        return;
-    }
-    for (int k = 0; k < BND_PX_PER_THREAD; k++) {
-        PixelRecords r;
-        pixel_records(labels, sizes, w, h, x0 + k, y, min_size, lb, r);
-        for (uint32_t i = 0; i < r.n; i++, pos++) {
-            points[pos] = r.p[i];
-            keys[pos] = (r.k[i] << 2) | r.plane[i];
+    uint32_t pos = block_exclusive_scan<uint32_t>(c, lds4, &tot);
+    const size_t base = (size_t)blockIdx.x * (4 * BND_TILE);
+#pragma unroll
+    for (int k = 0; k < BND_PX_PER_THREAD; k++)
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            if (!((keep >> (4 * k + p)) & 1u)) continue;
+            const uint64_t key = r[k].k[p];
+            const uint64_t packed = ((key >> 32) << lb) | (key & 0xFFFFFFFFull);
+            tile_keys[base + pos] = (packed << 2) | (uint64_t)p;
+            tile_points[base + pos] = r[k].p[p];
+            pos++;
         }
+    if (threadIdx.x == 0) tile_counts[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(256)
+k_bnd_gather(const uint64_t *__restrict__ tile_keys, const atq_boundary_point *__restrict__ tile_points,
+             const uint32_t *__restrict__ tile_counts, const uint32_t *__restrict__ tile_off,
+             uint64_t *__restrict__ keys, atq_boundary_point *__restrict__ points) {
+    const uint32_t n = tile_counts[blockIdx.x], off = tile_off[blockIdx.x];
+    const size_t base = (size_t)blockIdx.x * (4 * BND_TILE);
+    for (uint32_t i = threadIdx.x; i < n; i += 256) {
+        keys[off + i] = tile_keys[base + i];
+        points[off + i] = tile_points[base + i];
     }
 }
 
@@ -173,20 +315,24 @@ int launch_find_boundaries_sparse(hipStream_t s, const uint32_t *labels, const u
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
-int launch_find_boundaries_compact(hipStream_t s, const uint32_t *labels, const uint32_t *sizes,
-                                   size_t w, size_t h, atq_boundary_point *points, uint64_t *keys,
-                                   uint32_t capacity, atq_counts *counts, int min_size,
-                                   uint32_t label_bits, uint32_t *tile_scratch,
-                                   uint32_t *scan_scratch) {
+int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const uint32_t *sizes,
+                                    size_t w, size_t h, int min_size, uint32_t label_bits,
+                                    const PairTable &table, atq_counts *counts,
+                                    uint64_t *tile_keys, atq_boundary_point *tile_points,
+                                    uint32_t *tile_scratch, uint64_t *keys,
+                                    atq_boundary_point *points, uint32_t *scan_scratch) {
     const unsigned tiles = atq_div_up(w * h, BND_TILE);
+    const uint32_t max_count = (uint32_t)(2 * (2 * w + 2 * h));
     uint32_t *tile_counts = tile_scratch, *tile_off = tile_scratch + tiles;
-    k_bnd_count<<<tiles, 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size,
-                                      tile_counts);
+    k_pair_count<<<tiles, 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size,
+                                       table, max_count, counts);
+    k_bnd_filter<<<tiles, 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size,
+                                       label_bits, table, max_count, counts, tile_keys, tile_points,
+                                       tile_counts);
     int r = device_scan(s, nullptr, tiles, TileCount{tile_counts}, TileOffset{tile_off},
                         scan_scratch, &counts->n_boundary);
     if (r) return r;
-    k_bnd_emit<<<tiles, 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size,
-                                     label_bits, tile_off, points, keys, capacity, counts);
+    k_bnd_gather<<<tiles, 256, 0, s>>>(tile_keys, tile_points, tile_counts, tile_off, keys, points);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 

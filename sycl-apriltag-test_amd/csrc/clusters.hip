@@ -16,6 +16,7 @@
 // line-fit moments from the grey image (src/main.cpp:768-775) and the
 // cluster's inclusive prefix sums in exact int64 (src/main.cpp:793-801).
 #include "clusters.hpp"
+#include "segsort.hpp"
 
 namespace atq {
 
@@ -191,165 +192,160 @@ struct RunOut {
     }
 };
 
-// Angle sort: keys are slope << 32 | position in the cluster, so equal
-// slopes keep the input order (the reference's sort, made stable). Every
-// cluster is cut into chunks of ANGLE_CHUNK points, each sorted in LDS by one
-// work-group; clusters longer than a chunk are then finished by merge passes
-// in which every key finds its rank in the partner run by binary search
-// (runs double each pass; a cluster needing q passes ends in buffer q % 2).
-constexpr uint32_t ANGLE_CHUNK = 4096;
+// Angle sort keys: slope << 32 | position in the cluster, so equal slopes
+// keep the input order (the reference's sort, made stable).
+struct AngleKey {
+    const atq_cluster_point *in;
+    ATQ_DEV uint64_t operator()(uint32_t, uint32_t start, uint32_t j) const {
+        return ((uint64_t)float_key(in[start + j].slope) << 32) | j;
+    }
+};
 
+// One chunk of up to SEG_CHUNK points of a cluster in angle order (src[i]:
+// the point at chunk position e0 + i, or NONE past the end): writes the
+// points to `out` (if non-null) and their line-fit moments
+// (include/line_fit_point.h:50-86), prefix-summed from `carry` over the
+// chunk, to `lf` (if non-null). Called by all SEG_NT threads; tot gets the
+// chunk's sums. Each point keeps only its weight and doubled coordinates in
+// registers; the moments are rebuilt from them when written.
+ATQ_DEV void fit_chunk(const atq_cluster_point *in, atq_cluster_point *out, uint32_t start,
+                       uint32_t e0, const uint32_t (&src)[SEG_IPT], const uint8_t *grey, uint32_t w,
+                       uint32_t h, atq_line_fit_point *lf, const int64_t (&carry)[6],
+                       int64_t (&tot)[6], int64_t *lds) {
+    uint32_t W[SEG_IPT], RX[SEG_IPT], RY[SEG_IPT];
+    int64_t sum[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < SEG_IPT; i++) {
+        W[i] = RX[i] = RY[i] = 0;
+        if (src[i] == NONE) continue;
+        const atq_cluster_point p = in[start + src[i]];
+        if (out) out[start + e0 + i] = p;
+        if (lf) {
+            linefit_terms(p.packed_x, p.packed_y, w, h, grey, &W[i], &RX[i], &RY[i]);
+            const atq_line_fit_point m = linefit_from_terms(W[i], RX[i], RY[i]);
+            sum[0] += m.Mx;
+            sum[1] += m.My;
+            sum[2] += m.Mxx;
+            sum[3] += m.Myy;
+            sum[4] += m.Mxy;
+            sum[5] += m.W;
+        }
+    }
+    if (!lf) return;
+    int64_t acc[6];
+    seg_block_scan6(sum, acc, tot, lds);
+#pragma unroll
+    for (int q = 0; q < 6; q++) acc[q] += carry[q] - sum[q];
+#pragma unroll
+    for (int i = 0; i < SEG_IPT; i++) {
+        if (src[i] == NONE) continue;
+        const atq_line_fit_point m = linefit_from_terms(W[i], RX[i], RY[i]);
+        acc[0] += m.Mx;
+        acc[1] += m.My;
+        acc[2] += m.Mxx;
+        acc[3] += m.Myy;
+        acc[4] += m.Mxy;
+        acc[5] += m.W;
+        lf[start + e0 + i] = atq_line_fit_point{acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], 0, 0};
+    }
+}
+
+// Clusters of at most SEG_CHUNK points are finished inside the sort, from
+// the sorted keys in registers.
+struct AngleFinish {
+    const atq_cluster_point *in;
+    atq_cluster_point *out;
+    const uint8_t *grey;
+    uint32_t w, h;
+    atq_line_fit_point *lf;
+    ATQ_DEV void operator()(uint32_t, uint32_t start, uint32_t n, const uint64_t (&v)[SEG_IPT],
+                            uint64_t *lds) const {
+        const uint32_t e0 = threadIdx.x * SEG_IPT;
+        uint32_t src[SEG_IPT];
+#pragma unroll
+        for (int i = 0; i < SEG_IPT; i++) src[i] = e0 + i < n ? (uint32_t)v[i] : NONE;
+        const int64_t zero[6] = {0, 0, 0, 0, 0, 0};
+        int64_t tot[6];
+        fit_chunk(in, out, start, e0, src, grey, w, h, lf, zero, tot,
+                  reinterpret_cast<int64_t *>(lds));
+    }
+};
+
+// Slot of (cluster c, chunk k) in the chunk-sum table: c + (start + k *
+// SEG_CHUNK) / SEG_CHUNK is strictly increasing over the chunks of
+// successive clusters, so the table needs n_clusters + n_points / SEG_CHUNK
+// + 1 slots.
+ATQ_DEV uint32_t fit_slot(uint32_t c, uint32_t start, uint32_t k) {
+    return c + (start + k * SEG_CHUNK) / SEG_CHUNK;
+}
+
+// Chunk-parallel form for clusters longer than `min_count` points: every
+// chunk writes its points and moments prefix-summed within the chunk, and
+// its sums to the slot table (k_fit_carry then adds the earlier chunks).
+// slot_sums null: one work-group per cluster (max_chunks 1) walks its chunks
+// in order and carries the sums itself. `keys0/keys1` null: the input is in
+// order.
+__global__ void __launch_bounds__(SEG_NT)
+k_fit_chunks(const atq_cluster_point *__restrict__ in, atq_cluster_point *__restrict__ out,
+             const uint64_t *__restrict__ keys0, const uint64_t *__restrict__ keys1,
+             const atq_cluster_extents *__restrict__ ext, const uint32_t *n_clusters_dev,
+             uint32_t cap_cl, uint32_t max_chunks, uint32_t min_count,
+             const uint8_t *__restrict__ grey, uint32_t w, uint32_t h,
+             atq_line_fit_point *__restrict__ lf, int64_t *__restrict__ slot_sums) {
+    __shared__ int64_t lds[6 * SEG_NT / 64 + 6];
+    const uint32_t ncl = min(*n_clusters_dev, cap_cl);
+    const uint64_t items = (uint64_t)ncl * max_chunks;
+    const uint32_t e0 = threadIdx.x * SEG_IPT;
+    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
+        const uint32_t c = (uint32_t)(it / max_chunks), k = (uint32_t)(it % max_chunks);
+        const uint32_t start = ext[c].start, n = ext[c].count, b = k * SEG_CHUNK;
+        if (n <= min_count || b >= n) continue;
+        const uint64_t *keys =
+            keys0 ? ((seg_merge_passes(n) & 1) ? keys1 : keys0) + start : nullptr;
+        int64_t carry[6] = {0, 0, 0, 0, 0, 0}, tot[6];
+        // no slot table: this work-group takes the cluster's chunks in turn
+        const uint32_t b_end = slot_sums ? b + 1 : n;
+        for (uint32_t bb = b; bb < b_end; bb += SEG_CHUNK) {
+            uint32_t src[SEG_IPT];
+#pragma unroll
+            for (int i = 0; i < SEG_IPT; i++) {
+                const uint32_t j = bb + e0 + i;
+                src[i] = j < n ? (keys ? (uint32_t)keys[j] : j) : NONE;
+            }
+            fit_chunk(in, out, start, bb + e0, src, grey, w, h, lf, carry, tot, lds);
+#pragma unroll
+            for (int q = 0; q < 6; q++) carry[q] += tot[q];
+        }
+        if (slot_sums && lf && threadIdx.x == 0)
+#pragma unroll
+            for (int q = 0; q < 6; q++) slot_sums[6 * (size_t)fit_slot(c, start, k) + q] = tot[q];
+    }
+}
+
+// Adds to every chunk k >= 1 of the clusters longer than max(min_count,
+// SEG_CHUNK) the sums of the cluster's earlier chunks.
 __global__ void __launch_bounds__(256)
-k_angle_chunks(const atq_cluster_point *__restrict__ in, const atq_cluster_extents *__restrict__ ext,
-               const uint32_t *n_clusters_dev, uint32_t cap_cl, uint32_t max_chunks,
-               uint64_t *__restrict__ keys_out) {
-    __shared__ uint64_t keys[ANGLE_CHUNK];
+k_fit_carry(const atq_cluster_extents *__restrict__ ext, const uint32_t *n_clusters_dev,
+            uint32_t cap_cl, uint32_t max_chunks, uint32_t min_count,
+            const int64_t *__restrict__ slot_sums, atq_line_fit_point *__restrict__ lf) {
     const uint32_t ncl = min(*n_clusters_dev, cap_cl);
     const uint64_t items = (uint64_t)ncl * max_chunks;
     for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
         const uint32_t c = (uint32_t)(it / max_chunks), k = (uint32_t)(it % max_chunks);
-        const uint32_t s0 = ext[c].start, n = ext[c].count;
-        const uint32_t b = k * ANGLE_CHUNK;
-        if (b >= n) continue;
-        const uint32_t m = min(ANGLE_CHUNK, n - b);
-        for (uint32_t j = threadIdx.x; j < m; j += 256)
-            keys[j] = ((uint64_t)float_key(in[s0 + b + j].slope) << 32) | (b + j);
-        __syncthreads();
-        block_bitonic_sort<uint64_t>(keys, m, 256, false);
-        for (uint32_t j = threadIdx.x; j < m; j += 256) keys_out[s0 + b + j] = keys[j];
-        __syncthreads();
-    }
-}
-
-ATQ_DEV uint32_t lower_bound_u64(const uint64_t *a, uint32_t n, uint64_t key) {
-    uint32_t lo = 0, hi = n;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (a[mid] < key) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// One merge pass over runs of length `run` (clusters longer than `run`).
-__global__ void __launch_bounds__(256)
-k_angle_merge(const uint64_t *__restrict__ kin, uint64_t *__restrict__ kout,
-              const uint32_t *__restrict__ cidx, const atq_cluster_extents *__restrict__ ext,
-              const uint32_t *n_points_dev, uint32_t cap, uint32_t run) {
-    const uint32_t N = min(*n_points_dev, cap);
-    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N; x += gridDim.x * blockDim.x) {
-        const atq_cluster_extents e = ext[cidx[x]];
-        if (e.count <= run) continue;
-        const uint32_t j = x - e.start;
-        const uint32_t base = j / (2 * run) * (2 * run);
-        const uint64_t *c = kin + e.start;
-        const uint64_t key = c[j];
-        uint32_t pos;
-        if (j - base < run) {  // left run: count the right run's smaller keys
-            const uint32_t rb = min(base + run, e.count), re = min(base + 2 * run, e.count);
-            pos = j + lower_bound_u64(c + rb, re - rb, key);
-        } else {               // right run: count the left run's smaller keys
-            pos = base + (j - base - run) + lower_bound_u64(c + base, run, key);
-        }
-        kout[e.start + pos] = key;
-    }
-}
-
-ATQ_DEV uint32_t merge_passes(uint32_t n) {
-    uint32_t q = 0;
-    for (uint32_t r = ANGLE_CHUNK; r < n; r <<= 1) q++;
-    return q;
-}
-
-// Line-fit points + per-cluster inclusive moment prefix, one work-group per
-// cluster, reading the points in angle order. `keys0/keys1` null: the input
-// is already in order. `out` null: no points written. `lf` null: no moments.
-// Each thread takes FIT_ITEMS consecutive points, so one round of the
-// work-group covers FIT_ITEMS * 256 points with one scan of the six sums.
-constexpr int FIT_ITEMS = 8;
-
-__global__ void __launch_bounds__(256)
-k_cluster_fit(const atq_cluster_point *__restrict__ in, atq_cluster_point *__restrict__ out,
-              const uint64_t *__restrict__ keys0, const uint64_t *__restrict__ keys1,
-              const atq_cluster_extents *__restrict__ ext, const uint32_t *n_clusters_dev,
-              uint32_t cap_cl, const uint8_t *__restrict__ grey, uint32_t w, uint32_t h,
-              atq_line_fit_point *__restrict__ lf) {
-    __shared__ int64_t wsum[4][6];
-    const uint32_t ncl = min(*n_clusters_dev, cap_cl);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint32_t c = blockIdx.x; c < ncl; c += gridDim.x) {
-        const uint32_t s0 = ext[c].start, n = ext[c].count;
-        const uint64_t *keys = keys0 ? ((merge_passes(n) & 1) ? keys1 : keys0) + s0 : nullptr;
+        const uint32_t start = ext[c].start, n = ext[c].count, b = k * SEG_CHUNK;
+        if (n <= min_count || k == 0 || b >= n) continue;
         int64_t carry[6] = {0, 0, 0, 0, 0, 0};
-        for (uint32_t base = 0; base < n; base += FIT_ITEMS * 256) {
-            const uint32_t j0 = base + threadIdx.x * FIT_ITEMS;
-            uint32_t W[FIT_ITEMS], RX[FIT_ITEMS], RY[FIT_ITEMS];
-            int64_t v[6] = {0, 0, 0, 0, 0, 0};
+        for (uint32_t q = 0; q < k; q++)
 #pragma unroll
-            for (int k = 0; k < FIT_ITEMS; k++) {
-                W[k] = 0;
-                RX[k] = RY[k] = 0;
-                const uint32_t j = j0 + k;
-                if (j < n) {
-                    const uint32_t src = keys ? (uint32_t)keys[j] : j;
-                    const atq_cluster_point p = in[s0 + src];
-                    if (out) out[s0 + j] = p;
-                    if (lf) {
-                        linefit_terms(p.packed_x, p.packed_y, w, h, grey, &W[k], &RX[k], &RY[k]);
-                        const atq_line_fit_point m = linefit_from_terms(W[k], RX[k], RY[k]);
-                        v[0] += m.Mx;
-                        v[1] += m.My;
-                        v[2] += m.Mxx;
-                        v[3] += m.Myy;
-                        v[4] += m.Mxy;
-                        v[5] += m.W;
-                    }
-                }
-            }
-            if (!lf) continue;
-            // inclusive wave scan of the thread totals, then across the 4 waves
-            int64_t x[6];
-#pragma unroll
-            for (int f = 0; f < 6; f++) {
-                x[f] = v[f];
-#pragma unroll
-                for (int o = 1; o < 64; o <<= 1) {
-                    int64_t y = __shfl_up(x[f], o, 64);
-                    if (lane >= o) x[f] += y;
-                }
-            }
-            if (lane == 63)
-#pragma unroll
-                for (int f = 0; f < 6; f++) wsum[wave][f] = x[f];
-            __syncthreads();
-            int64_t acc[6];
-#pragma unroll
-            for (int f = 0; f < 6; f++) {
-                int64_t before = 0, tot = 0;
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int64_t t = wsum[q][f];
-                    if (q < wave) before += t;
-                    tot += t;
-                }
-                acc[f] = carry[f] + before + x[f] - v[f];
-                carry[f] += tot;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < FIT_ITEMS; k++) {
-                const uint32_t j = j0 + k;
-                if (j < n) {
-                    const atq_line_fit_point m = linefit_from_terms(W[k], RX[k], RY[k]);
-                    acc[0] += m.Mx;
-                    acc[1] += m.My;
-                    acc[2] += m.Mxx;
-                    acc[3] += m.Myy;
-                    acc[4] += m.Mxy;
-                    acc[5] += m.W;
-                    lf[s0 + j] = atq_line_fit_point{acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], 0, 0};
-                }
-            }
+            for (int f = 0; f < 6; f++) carry[f] += slot_sums[6 * (size_t)fit_slot(c, start, q) + f];
+        for (uint32_t j = b + threadIdx.x; j < min(n, b + SEG_CHUNK); j += 256) {
+            atq_line_fit_point &p = lf[start + j];
+            p.Mx += carry[0];
+            p.My += carry[1];
+            p.Mxx += carry[2];
+            p.Myy += carry[3];
+            p.Mxy += carry[4];
+            p.W += carry[5];
         }
     }
 }
@@ -401,25 +397,29 @@ int launch_assign(hipStream_t s, const atq_boundary_point *pts, const uint32_t *
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
-int launch_angle_sort(hipStream_t s, const atq_cluster_point *in, const atq_cluster_extents *ext,
-                      const uint32_t *cidx, const atq_counts *counts, uint32_t cap, uint32_t cap_cl,
-                      uint32_t max_cluster, uint64_t *keys0, uint64_t *keys1) {
-    const uint32_t max_chunks = (max_cluster + ANGLE_CHUNK - 1) / ANGLE_CHUNK;
-    k_angle_chunks<<<2048, 256, 0, s>>>(in, ext, &counts->n_clusters, cap_cl, max_chunks, keys0);
-    const unsigned grid = min(4096u, atq_div_up(cap, 256) + 1);
-    int pass = 0;
-    for (uint32_t run = ANGLE_CHUNK; run < max_cluster; run <<= 1, pass++)
-        k_angle_merge<<<grid, 256, 0, s>>>((pass & 1) ? keys1 : keys0, (pass & 1) ? keys0 : keys1,
-                                           cidx, ext, &counts->n_cluster_points, cap, run);
+int launch_angle_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out,
+                     const atq_cluster_extents *ext, const uint32_t *cidx, const atq_counts *counts,
+                     uint32_t cap, uint32_t cap_cl, uint32_t max_cluster, const uint8_t *grey,
+                     uint32_t w, uint32_t h, atq_line_fit_point *lf, uint64_t *keys0,
+                     uint64_t *keys1, int64_t *slot_sums) {
+    int r = seg_sort(s, AngleKey{in}, AngleFinish{in, out, grey, w, h, lf}, ext,
+                     &counts->n_clusters, cap_cl, cidx, &counts->n_cluster_points, cap, max_cluster,
+                     keys0, keys1);
+    if (r || max_cluster <= SEG_CHUNK) return r;
+    const uint32_t max_chunks = seg_max_chunks(max_cluster);
+    k_fit_chunks<<<1024, SEG_NT, 0, s>>>(in, out, keys0, keys1, ext, &counts->n_clusters, cap_cl,
+                                         max_chunks, SEG_CHUNK, grey, w, h, lf, slot_sums);
+    if (lf)
+        k_fit_carry<<<1024, 256, 0, s>>>(ext, &counts->n_clusters, cap_cl, max_chunks, SEG_CHUNK,
+                                         slot_sums, lf);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
-int launch_cluster_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out,
-                       const uint64_t *keys0, const uint64_t *keys1, const atq_cluster_extents *ext,
+int launch_cluster_fit(hipStream_t s, const atq_cluster_point *in, const atq_cluster_extents *ext,
                        const uint32_t *n_clusters_dev, uint32_t cap_cl, const uint8_t *grey,
                        uint32_t w, uint32_t h, atq_line_fit_point *lf) {
-    k_cluster_fit<<<2048, 256, 0, s>>>(in, out, keys0, keys1, ext, n_clusters_dev, cap_cl, grey, w,
-                                       h, lf);
+    k_fit_chunks<<<1024, SEG_NT, 0, s>>>(in, nullptr, nullptr, nullptr, ext, n_clusters_dev, cap_cl,
+                                         1, 0, grey, w, h, lf, nullptr);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 

@@ -19,16 +19,19 @@ int launch_assign(hipStream_t s, const atq_boundary_point *pts, const uint32_t *
                   const uint32_t *seg_start, const uint32_t *seg_cluster,
                   const atq_cluster_bounds *bounds, const atq_cluster_extents *ext,
                   const uint32_t *n_dev, uint32_t cap, atq_cluster_point *cpts, uint32_t *cidx);
-// Angle sort of every cluster's points (keys0/keys1: ping-pong key buffers
-// of cap entries; a cluster of n points ends in keys1 when it needed an odd
-// number of merge passes). max_cluster bounds the cluster length.
-int launch_angle_sort(hipStream_t s, const atq_cluster_point *in, const atq_cluster_extents *ext,
-                      const uint32_t *cidx, const atq_counts *counts, uint32_t cap, uint32_t cap_cl,
-                      uint32_t max_cluster, uint64_t *keys0, uint64_t *keys1);
-// Points in angle order (via the keys; keys0 null: input already in order)
-// to `out` (may be null) and their line-fit prefix sums to `lf` (may be null).
-int launch_cluster_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out,
-                       const uint64_t *keys0, const uint64_t *keys1, const atq_cluster_extents *ext,
+// Angle sort of every cluster's points (src/main.cpp:737-750), the points
+// in that order to `out` and their line-fit prefix sums to `lf` (may be
+// null; src/main.cpp:768-801). cidx: cluster of every point; max_cluster
+// bounds the cluster length. keys0/keys1: two key buffers of cap entries;
+// slot_sums: 6 * (cap_cl + cap / SEG_CHUNK + 2) int64.
+int launch_angle_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out,
+                     const atq_cluster_extents *ext, const uint32_t *cidx, const atq_counts *counts,
+                     uint32_t cap, uint32_t cap_cl, uint32_t max_cluster, const uint8_t *grey,
+                     uint32_t w, uint32_t h, atq_line_fit_point *lf, uint64_t *keys0,
+                     uint64_t *keys1, int64_t *slot_sums);
+// Line-fit prefix sums of points already in angle order (one work-group per
+// cluster).
+int launch_cluster_fit(hipStream_t s, const atq_cluster_point *in, const atq_cluster_extents *ext,
                        const uint32_t *n_clusters_dev, uint32_t cap_cl, const uint8_t *grey,
                        uint32_t w, uint32_t h, atq_line_fit_point *lf);
 int launch_assign_search(hipStream_t s, const atq_boundary_point *pts, const uint32_t *n_dev,
@@ -42,11 +45,14 @@ int launch_fit_lines(hipStream_t s, const atq_line_fit_point *lf, const uint32_t
                      const atq_cluster_extents *ext, const uint32_t *n_points_dev, uint32_t cap,
                      atq_corner *sparse, uint32_t *ccount);
 int launch_set_u32(hipStream_t s, uint32_t *p, uint32_t v);
-int launch_zero_u32(hipStream_t s, uint32_t *p, const uint32_t *n_dev, uint32_t cap);
+// Zeroes p[0 .. min(*n_dev, cap)) (n_dev null: cap words), except that
+// p[set_idx] gets set_val.
+int launch_zero_u32(hipStream_t s, uint32_t *p, const uint32_t *n_dev, uint32_t cap,
+                    uint32_t set_idx = 0xFFFFFFFFu, uint32_t set_val = 0);
 int launch_corner_segments(hipStream_t s, const atq_corner *sparse, const uint32_t *n_sparse_dev,
-                           uint32_t cap, atq_counts *counts, uint32_t cap_cl,
-                           atq_peak_extents *peaks, atq_corner *compacted, uint64_t *scratch,
-                           uint32_t *scan_scratch);
+                           uint32_t cap, atq_counts *counts, uint32_t cap_cl, uint32_t max_seg,
+                           atq_peak_extents *peaks, atq_corner *compacted, uint32_t *seg_of,
+                           uint64_t *scratch, uint32_t *scan_scratch);
 int launch_do_indexing(hipStream_t s, const atq_peak_extents *peaks, const uint32_t *n_seg_dev,
                        uint32_t cap, const atq_corner *corners, const atq_line_fit_point *lf,
                        const uint32_t *n_points_dev, uint32_t n_points_host,

@@ -17,6 +17,7 @@
 // with cluster extents[e] as the reference does. The quad checks of
 // process_fitted_quads run one thread per fitted quad.
 #include "clusters.hpp"
+#include "segsort.hpp"
 
 namespace atq {
 
@@ -143,9 +144,11 @@ struct QuadOut {
 
 __global__ void k_set_u32(uint32_t *p, uint32_t v) { *p = v; }
 
-__global__ void k_zero_u32(uint32_t *p, const uint32_t *n_dev, uint32_t cap) {
+__global__ void k_zero_u32(uint32_t *p, const uint32_t *n_dev, uint32_t cap, uint32_t set_idx,
+                           uint32_t set_val) {
     uint32_t n = n_dev ? min(*n_dev, cap) : cap;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        p[i] = i == set_idx ? set_val : 0u;
 }
 
 // One thread per cluster point: the cyclic window / filter / maximum.
@@ -207,8 +210,10 @@ struct SegHead {
 };
 struct SegStart {
     atq_peak_extents *peaks;
+    uint32_t *seg_of;
     ATQ_DEV void operator()(uint32_t i, uint32_t pre, uint32_t v) const {
         if (v) peaks[pre].start = i;
+        seg_of[i] = pre + v - 1;
     }
 };
 }  // namespace
@@ -221,29 +226,52 @@ __global__ void k_peak_close(const atq_counts *counts, uint32_t cap, atq_peak_ex
     }
 }
 
+// Peak order inside a segment: error descending, then line-fit index
+// ascending (the reference's stable sort by error, include/fit_quad.h).
+struct CornerKey {
+    const atq_corner *c;
+    ATQ_DEV uint64_t operator()(uint32_t, uint32_t start, uint32_t j) const {
+        const atq_corner cr = c[start + j];
+        return ((uint64_t)~float_key(cr.error) << 32) | cr.line_fit_point_index;
+    }
+};
+
+// Segments of at most SEG_CHUNK corners are written back in key order
+// inside the sort, from the registers.
+struct CornerFinish {
+    atq_corner *out;
+    ATQ_DEV void operator()(uint32_t, uint32_t start, uint32_t n, const uint64_t (&v)[SEG_IPT],
+                            uint64_t *) const {
+        const uint32_t c = out[start].cluster_index;
+        __syncthreads();
+        const uint32_t e0 = threadIdx.x * SEG_IPT;
+#pragma unroll
+        for (int i = 0; i < SEG_IPT; i++)
+            if (e0 + i < n)
+                out[start + e0 + i] =
+                    atq_corner{(uint32_t)v[i], c, float_from_key(~(uint32_t)(v[i] >> 32))};
+    }
+};
+
+// Writes the corners of segments longer than SEG_CHUNK back in key order.
+// The cluster index is read from the segment's first corner, which every
+// writer of that slot rewrites with the same cluster index.
 __global__ void __launch_bounds__(256)
-k_corner_sort(const atq_peak_extents *__restrict__ peaks, const atq_counts *counts, uint32_t cap,
-              atq_corner *__restrict__ out, uint64_t *__restrict__ scratch) {
-    constexpr uint32_t LDS_KEYS = 2048;
-    __shared__ uint64_t keys_lds[LDS_KEYS];
+k_corner_emit(const atq_peak_extents *__restrict__ peaks, const atq_counts *counts, uint32_t cap,
+              uint32_t max_chunks, const uint64_t *__restrict__ keys0,
+              const uint64_t *__restrict__ keys1, atq_corner *out) {
     const uint32_t E = min(counts->n_segments, cap);
-    for (uint32_t e = blockIdx.x; e < E; e += gridDim.x) {
+    const uint64_t items = (uint64_t)E * max_chunks;
+    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
+        const uint32_t e = (uint32_t)(it / max_chunks), ch = (uint32_t)(it % max_chunks);
         const uint32_t cs = peaks[e].start, m = peaks[e].count;
+        if (m <= SEG_CHUNK || ch * SEG_CHUNK >= m) continue;
         const uint32_t c = out[cs].cluster_index;
-        const bool in_lds = m <= LDS_KEYS;
-        uint64_t *keys = in_lds ? keys_lds : scratch + cs;
-        for (uint32_t r = threadIdx.x; r < m; r += 256) {
-            atq_corner cr = out[cs + r];
-            keys[r] = ((uint64_t)~float_key(cr.error) << 32) | cr.line_fit_point_index;
+        const uint64_t *k = ((seg_merge_passes(m) & 1) ? keys1 : keys0) + cs;
+        for (uint32_t r = ch * SEG_CHUNK + threadIdx.x; r < min(m, (ch + 1) * SEG_CHUNK); r += 256) {
+            const uint64_t kk = k[r];
+            out[cs + r] = atq_corner{(uint32_t)kk, c, float_from_key(~(uint32_t)(kk >> 32))};
         }
-        if (!in_lds) __threadfence_block();
-        __syncthreads();
-        block_bitonic_sort<uint64_t>(keys, m, 256, !in_lds);
-        for (uint32_t r = threadIdx.x; r < m; r += 256) {
-            uint64_t k = keys[r];
-            out[cs + r] = atq_corner{(uint32_t)k, c, float_from_key(~(uint32_t)(k >> 32))};
-        }
-        __syncthreads();
     }
 }
 
@@ -402,8 +430,9 @@ int launch_set_u32(hipStream_t s, uint32_t *p, uint32_t v) {
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
-int launch_zero_u32(hipStream_t s, uint32_t *p, const uint32_t *n_dev, uint32_t cap) {
-    k_zero_u32<<<min(1024u, atq_div_up(cap, 256) + 1), 256, 0, s>>>(p, n_dev, cap);
+int launch_zero_u32(hipStream_t s, uint32_t *p, const uint32_t *n_dev, uint32_t cap,
+                    uint32_t set_idx, uint32_t set_val) {
+    k_zero_u32<<<min(1024u, atq_div_up(cap, 256) + 1), 256, 0, s>>>(p, n_dev, cap, set_idx, set_val);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
@@ -416,17 +445,22 @@ int launch_fit_lines(hipStream_t s, const atq_line_fit_point *lf, const uint32_t
 }
 
 int launch_corner_segments(hipStream_t s, const atq_corner *sparse, const uint32_t *n_sparse_dev,
-                           uint32_t cap, atq_counts *counts, uint32_t cap_cl,
-                           atq_peak_extents *peaks, atq_corner *compacted, uint64_t *scratch,
-                           uint32_t *scan_scratch) {
+                           uint32_t cap, atq_counts *counts, uint32_t cap_cl, uint32_t max_seg,
+                           atq_peak_extents *peaks, atq_corner *compacted, uint32_t *seg_of,
+                           uint64_t *scratch, uint32_t *scan_scratch) {
     int r = device_scan(s, n_sparse_dev, cap, CornerFlag{sparse}, CornerOut{sparse, compacted},
                         scan_scratch, &counts->n_corners);
     if (r) return r;
-    r = device_scan(s, &counts->n_corners, cap, SegHead{compacted}, SegStart{peaks}, scan_scratch,
+    r = device_scan(s, &counts->n_corners, cap, SegHead{compacted}, SegStart{peaks, seg_of}, scan_scratch,
                     &counts->n_segments);
     if (r) return r;
     k_peak_close<<<min(1024u, atq_div_up(cap_cl, 256) + 1), 256, 0, s>>>(counts, cap_cl, peaks);
-    k_corner_sort<<<2048, 256, 0, s>>>(peaks, counts, cap_cl, compacted, scratch);
+    r = seg_sort(s, CornerKey{compacted}, CornerFinish{compacted}, peaks, &counts->n_segments,
+                 cap_cl, seg_of, &counts->n_corners, cap, max_seg, scratch, scratch + cap);
+    if (r || max_seg <= SEG_CHUNK) return r;
+    const uint32_t max_chunks = seg_max_chunks(max_seg);
+    k_corner_emit<<<2048, 256, 0, s>>>(peaks, counts, cap_cl, max_chunks, scratch, scratch + cap,
+                                       compacted);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 

@@ -19,11 +19,22 @@ int launch_ccl(hipStream_t s, const uint8_t *thr, size_t w, size_t h, uint32_t *
 int launch_find_boundaries_sparse(hipStream_t s, const uint32_t *labels, const uint32_t *sizes,
                                   size_t w, size_t h, atq_boundary_point *points,
                                   uint64_t *keys, int min_size);
-int launch_find_boundaries_compact(hipStream_t s, const uint32_t *labels, const uint32_t *sizes,
-                                   size_t w, size_t h, atq_boundary_point *points,
-                                   uint64_t *keys, uint32_t capacity, atq_counts *counts,
-                                   int min_size, uint32_t label_bits, uint32_t *tile_scratch,
-                                   uint32_t *scan_scratch);
+// The detector's open-addressing table of cluster pairs and record counts
+// (slots = mask + 1, a power of two; zeroed before each frame).
+struct PairTable {
+    uint64_t *keys;
+    uint32_t *cnt;
+    uint32_t mask;
+};
+// Records of the pairs whose count lies in [24, 2 * (2w + 2h)], packed keys
+// ((min << lb | max) << 2 | plane), in the reference's order; tile_keys /
+// tile_points hold 4 * w * h entries, tile_scratch 2 * tiles + 2.
+int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const uint32_t *sizes,
+                                    size_t w, size_t h, int min_size, uint32_t label_bits,
+                                    const PairTable &table, atq_counts *counts,
+                                    uint64_t *tile_keys, atq_boundary_point *tile_points,
+                                    uint32_t *tile_scratch, uint64_t *keys,
+                                    atq_boundary_point *points, uint32_t *scan_scratch);
 constexpr unsigned BOUNDARY_TILE_PX = 1024;  // pixels per compaction tile
 
 // diff.hip

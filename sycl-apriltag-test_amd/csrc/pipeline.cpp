@@ -57,8 +57,8 @@ size_t Workspace::carve(char *base, const Caps &c) {
     valid = carve_one<uint32_t>(base, off, c.cap);
     seg_cluster = carve_one<uint32_t>(base, off, c.cap);
     seg_bounds = carve_one<atq_cluster_bounds>(base, off, c.cap);
-    scratch64 = carve_one<uint64_t>(base, off, c.cap);
     cpts_tmp = carve_one<atq_cluster_point>(base, off, c.cap);
+    fit_sums = carve_one<int64_t>(base, off, 6 * ((size_t)c.cap_cl + c.cap / SEG_CHUNK + 2));
     tiles = carve_one<uint32_t>(base, off, 2 * ((size_t)c.npx / BOUNDARY_TILE_PX + 1));
     qtmp = carve_one<atq_quad_corners>(base, off, c.cap_cl);
     qflags = carve_one<uint32_t>(base, off, c.cap_cl);
@@ -125,6 +125,8 @@ size_t carve_detector(atq_detector *d, char *base) {
     d->peaks = carve_one<atq_peak_extents>(base, off, c.cap_cl);
     d->fitted = carve_one<atq_fitted_quad>(base, off, c.cap_cl);
     d->quads = carve_one<atq_quad_corners>(base, off, c.cap_cl);
+    d->pair_keys = carve_one<uint64_t>(base, off, c.pair_slots);
+    d->pair_cnt = carve_one<uint32_t>(base, off, c.pair_slots);
     d->counts = carve_one<atq_counts>(base, off, 1);
     off = (off + 255) & ~(size_t)255;
     size_t ws_bytes = d->ws.carve(base ? base + off : nullptr, c);
@@ -153,7 +155,13 @@ int run_pipeline(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t 
         if (d->timing && hipEventRecord(d->ev[i], s) != hipSuccess) return ATQ_ERR_HIP;
         return ATQ_OK;
     };
-    TRY(launch_zero_u32(s, reinterpret_cast<uint32_t *>(cnt), nullptr, sizeof(atq_counts) / 4));
+    // one fill: the pair table and the counts (with the key width set)
+    const RadixPlan plan = radix_plan(w * h, 2);
+    uint32_t *zero_begin = reinterpret_cast<uint32_t *>(d->pair_keys);
+    const uint32_t zero_words = (uint32_t)((reinterpret_cast<char *>(cnt + 1) -
+                                            reinterpret_cast<char *>(zero_begin)) / 4);
+    const uint32_t lb_word = (uint32_t)(reinterpret_cast<uint32_t *>(&cnt->key_label_bits) - zero_begin);
+    TRY(launch_zero_u32(s, zero_begin, nullptr, zero_words, lb_word, (uint32_t)plan.label_bits));
     TRY(mark(0));
     TRY(launch_threshold(s, gray_in, w_in, h_in, f, d->gray, d->ext, d->fil, d->thr,
                          d->params.min_white_black_diff));
@@ -161,15 +169,16 @@ int run_pipeline(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t 
     const uint8_t *grey = f > 1 ? d->gray : gray_in;
     TRY(launch_ccl(s, d->thr, w, h, d->parent, d->labels, d->sizes));
     TRY(mark(1 + ATQ_STAGE_CCL));
-    // Boundary records in the reference's order, keyed by the packed pair key
-    // min<<lb|max with the plane in 2 low bits; the stable sort then yields
-    // the reference's (pair, plane, y, x) order.
-    const RadixPlan plan = radix_plan(w * h, 2);
-    TRY(launch_set_u32(s, &cnt->key_label_bits, (uint32_t)plan.label_bits));
-    TRY(launch_find_boundaries_compact(s, d->labels, d->sizes, w, h,
-                                       reinterpret_cast<atq_boundary_point *>(d->vals), d->keys,
-                                       c.cap, cnt, d->params.min_component_size,
-                                       (uint32_t)plan.label_bits, ws.tiles, ws.scan));
+    // Boundary records of the pairs that can pass ValidBlobFilter, in the
+    // reference's order, keyed by the packed pair key min<<lb|max with the
+    // plane in 2 low bits; the stable sort then yields the reference's
+    // (pair, plane, y, x) order.
+    const PairTable table{d->pair_keys, d->pair_cnt, d->caps.pair_slots - 1};
+    TRY(launch_find_boundaries_filtered(s, d->labels, d->sizes, w, h, d->params.min_component_size,
+                                        (uint32_t)plan.label_bits, table, cnt, ws.keys_alt,
+                                        reinterpret_cast<atq_boundary_point *>(ws.vals_alt),
+                                        ws.tiles, d->keys,
+                                        reinterpret_cast<atq_boundary_point *>(d->vals), ws.scan));
     TRY(mark(1 + ATQ_STAGE_BOUNDARY));
     bool alt = false;
     TRY(radix_sort_packed(s, plan, d->keys, d->vals, ws.keys_alt, ws.vals_alt, &cnt->n_boundary,
@@ -189,16 +198,16 @@ int run_pipeline(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t 
     // a valid cluster holds at most 2 * (2w + 2h) points (valid_blob)
     const uint32_t max_cluster = (uint32_t)std::min<size_t>(c.cap, 4 * (w + h));
     uint64_t *ak = ws.angle_keys();
-    TRY(launch_angle_sort(s, ws.cpts_tmp, d->cext, d->cidx, cnt, c.cap, c.cap_cl, max_cluster, ak,
-                          ak + c.cap));
-    TRY(launch_cluster_fit(s, ws.cpts_tmp, d->cpts, ak, ak + c.cap, d->cext, &cnt->n_clusters,
-                           c.cap_cl, grey, (uint32_t)w, (uint32_t)h, d->lf));
+    TRY(launch_angle_fit(s, ws.cpts_tmp, d->cpts, d->cext, d->cidx, cnt, c.cap, c.cap_cl,
+                         max_cluster, grey, (uint32_t)w, (uint32_t)h, d->lf, ak, ak + c.cap,
+                         ws.fit_sums));
     TRY(mark(1 + ATQ_STAGE_ANGLE_FIT));
     TRY(launch_fit_lines(s, d->lf, d->cidx, d->cext, &cnt->n_cluster_points, c.cap, d->sparse,
                          nullptr));
     TRY(mark(1 + ATQ_STAGE_FIT_LINES));
-    TRY(launch_corner_segments(s, d->sparse, &cnt->n_cluster_points, c.cap, cnt, c.cap_cl, d->peaks,
-                               d->corners, ws.scratch64, ws.scan));
+    TRY(launch_corner_segments(s, d->sparse, &cnt->n_cluster_points, c.cap, cnt, c.cap_cl,
+                               max_cluster, d->peaks, d->corners, ws.seg_id, ws.angle_keys(),
+                               ws.scan));
     TRY(mark(1 + ATQ_STAGE_CORNERS));
     TRY(launch_do_indexing(s, d->peaks, &cnt->n_segments, c.cap_cl, d->corners, d->lf,
                            &cnt->n_cluster_points, 0, d->cext, d->fitted, &cnt->oob_reads));
@@ -283,10 +292,12 @@ int atq_cluster_points_sort(void *stream, const atq_boundary_point *points,
     TRY(launch_assign_search(s, points, n_points_dev, c.cap, bounds, counts_dev, out_extents,
                              ws.cpts_tmp, out_cluster_index));
     uint64_t *ak = ws.angle_keys();
-    TRY(launch_angle_sort(s, ws.cpts_tmp, out_extents, out_cluster_index, counts_dev, c.cap,
-                          c.cap_cl, c.cap, ak, ak + c.cap));
-    return launch_cluster_fit(s, ws.cpts_tmp, out_points, ak, ak + c.cap, out_extents,
-                              &counts_dev->n_clusters, c.cap_cl, nullptr, 0, 0, nullptr);
+    // a valid cluster holds at most 2 * (2w + 2h) points (valid_blob), and
+    // w + h <= npx / 8 + 8 for frames of at least 8 pixels a side
+    const uint32_t max_cluster = std::min<uint32_t>(c.cap, c.npx / 2 + 32);
+    return launch_angle_fit(s, ws.cpts_tmp, out_points, out_extents, out_cluster_index, counts_dev,
+                            c.cap, c.cap_cl, max_cluster, nullptr, 0, 0, nullptr, ak, ak + c.cap,
+                            ws.fit_sums);
 }
 
 int atq_linefit_prefix(void *stream, const atq_cluster_point *points,
@@ -296,9 +307,8 @@ int atq_linefit_prefix(void *stream, const atq_cluster_point *points,
     if (!points || !extents || !counts_dev || !grayscale || !out) return ATQ_ERR_BAD_ARG;
     TRY(check_dims(width, height));
     const Caps c = Caps::for_pixels(width * height);
-    return launch_cluster_fit(S(stream), points, nullptr, nullptr, nullptr, extents,
-                              &counts_dev->n_clusters, c.cap_cl, grayscale, (uint32_t)width,
-                              (uint32_t)height, out);
+    return launch_cluster_fit(S(stream), points, extents, &counts_dev->n_clusters, c.cap_cl,
+                              grayscale, (uint32_t)width, (uint32_t)height, out);
 }
 
 int atq_fit_lines(void *stream, const atq_line_fit_point *points, const uint32_t *cluster_indices,
@@ -325,7 +335,9 @@ int atq_corner_segments(void *stream, const atq_corner *found_corners, atq_count
     const Caps c = caps_from_workspace(workspace_bytes);
     ws.carve(static_cast<char *>(workspace), c);
     return launch_corner_segments(s, found_corners, &counts_dev->n_cluster_points, c.cap, counts_dev,
-                                  c.cap_cl, segments, compacted, ws.scratch64, ws.scan);
+                                  c.cap_cl, std::min<uint32_t>(c.cap, c.npx / 2 + 32), segments,
+                                  compacted, ws.seg_id,
+                                  ws.angle_keys(), ws.scan);
 }
 
 int atq_do_indexing(void *stream, const atq_peak_extents *extents, size_t extents_count,
@@ -364,9 +376,11 @@ int atq_detector_create(size_t max_width_in, size_t max_height_in, const atq_par
                         atq_detector **out) {
     if (!out) return ATQ_ERR_BAD_ARG;
     *out = nullptr;
-    atq_params p{1, 5, 25, 8};
+    atq_params p{1, 5, 25, 8, 0};
     if (params) p = *params;
     if (p.decimate < 1) return ATQ_ERR_BAD_ARG;
+    if (p.pair_table_log2 != 0 && (p.pair_table_log2 < 8 || p.pair_table_log2 > 24))
+        return ATQ_ERR_BAD_ARG;
     const size_t w = decimated(max_width_in, p.decimate), h = decimated(max_height_in, p.decimate);
     TRY(check_dims(w, h));
     if (int r = atq_init()) return r;
@@ -378,6 +392,7 @@ int atq_detector_create(size_t max_width_in, size_t max_height_in, const atq_par
     d->max_h = h;
     d->params = p;
     d->caps = Caps::for_pixels(w * h);
+    if (p.pair_table_log2) d->caps.pair_slots = 1u << p.pair_table_log2;
     const size_t bytes = carve_detector(d, nullptr);
     if (hipMalloc(&d->arena, bytes) != hipSuccess) {
         delete d;
@@ -429,6 +444,8 @@ int atq_detector_buffer(atq_detector *d, int which, void **ptr, size_t *capacity
     case ATQ_BUF_EXTREMA: *ptr = d->ext; *capacity = c.npx / 16; break;
     case ATQ_BUF_FILTERED_EXTREMA: *ptr = d->fil; *capacity = c.npx / 16; break;
     case ATQ_BUF_SPARSE_CORNERS: *ptr = d->sparse; *capacity = c.cap; break;
+    case ATQ_BUF_PAIR_KEYS: *ptr = d->pair_keys; *capacity = c.pair_slots; break;
+    case ATQ_BUF_PAIR_COUNTS: *ptr = d->pair_cnt; *capacity = c.pair_slots; break;
     default: return ATQ_ERR_BAD_ARG;
     }
     return ATQ_OK;
@@ -531,7 +548,7 @@ int atq_batch_create(size_t n_frames, const size_t *widths_in, const size_t *hei
     if (!b) return ATQ_ERR_BAD_ARG;
     int rc = ATQ_OK;
     for (size_t i = 0; i < n_frames && rc == ATQ_OK; i++) {
-        atq_params p{decimate ? decimate[i] : 1, 5, 25, 8};
+        atq_params p{decimate ? decimate[i] : 1, 5, 25, 8, 0};
         atq_detector *d = nullptr;
         rc = atq_detector_create(widths_in[i], heights_in[i], &p, &d);
         b->dets.push_back(d);

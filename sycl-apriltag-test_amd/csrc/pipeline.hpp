@@ -2,19 +2,24 @@
 #pragma once
 #include "clusters.hpp"
 #include "kernels.hpp"
+#include "segsort.hpp"
 
 namespace atq {
 
 // Capacities for a frame of npx decimated pixels: at most 4 boundary records
 // per pixel (one per tested direction) and, since a valid cluster has at
 // least 24 points, at most cap/24 + 1 clusters.
+// The detector's pair table has pair_slots (a power of two in [2^14, 2^20],
+// about one per 64 pixels) slots.
 struct Caps {
-    uint32_t npx, cap, cap_cl;
+    uint32_t npx, cap, cap_cl, pair_slots;
     static Caps for_pixels(size_t npx) {
         Caps c;
         c.npx = (uint32_t)npx;
         c.cap = (uint32_t)(4 * npx);
         c.cap_cl = c.cap / 24 + 1;
+        c.pair_slots = 1u << 14;
+        while (c.pair_slots < (1u << 20) && c.pair_slots < npx / 64) c.pair_slots <<= 1;
         return c;
     }
 };
@@ -25,8 +30,8 @@ struct Workspace {
     uint32_t *vals_alt;
     uint32_t *seg_id, *seg_start, *valid, *seg_cluster;
     atq_cluster_bounds *seg_bounds;
-    uint64_t *scratch64;
     atq_cluster_point *cpts_tmp;
+    int64_t *fit_sums;  // per (cluster, chunk) line-fit sums of long clusters
     uint32_t *tiles;
     atq_quad_corners *qtmp;
     uint32_t *qflags;
@@ -34,8 +39,9 @@ struct Workspace {
     uint32_t *hist, *digit_base;
     atq_counts *counts;
 
-    // The angle sort's two key buffers (2 * cap entries) reuse seg_bounds
-    // (cap entries of 32 bytes), which is dead once points are assigned.
+    // Two key buffers of cap entries each for the angle and corner sorts;
+    // they reuse seg_bounds (cap entries of 32 bytes), which is dead once the
+    // points are assigned to clusters.
     uint64_t *angle_keys() const { return reinterpret_cast<uint64_t *>(seg_bounds); }
 
     // Carves the layout from `base` (nullptr: only measures). Returns bytes.
@@ -63,6 +69,8 @@ struct atq_detector {
     atq_peak_extents *peaks;
     atq_fitted_quad *fitted;
     atq_quad_corners *quads;
+    uint64_t *pair_keys;  // pair table, zeroed with counts before each frame
+    uint32_t *pair_cnt;
     atq_counts *counts;
     atq::Workspace ws;
     size_t w, h;

@@ -63,7 +63,7 @@ N_STAGES = 10
 BUF = dict(GRAY=0, THRESHOLDED=1, LABELS=2, SIZES=3, BOUNDARY_POINTS=4, BOUNDARY_KEYS=5,
            CLUSTER_BOUNDS=6, CLUSTER_POINTS=7, CLUSTER_INDEX=8, CLUSTER_EXTENTS=9, LINE_FIT=10,
            CORNERS=11, PEAK_EXTENTS=12, FITTED=13, QUADS=14, COUNTS=15, EXTREMA=16,
-           FILTERED_EXTREMA=17, SPARSE_CORNERS=18)
+           FILTERED_EXTREMA=17, SPARSE_CORNERS=18, PAIR_KEYS=19, PAIR_COUNTS=20)
 
 
 class AtqError(RuntimeError):
@@ -72,7 +72,8 @@ class AtqError(RuntimeError):
 
 class AtqParams(C.Structure):
     _fields_ = [("decimate", C.c_int32), ("min_white_black_diff", C.c_int32),
-                ("min_component_size", C.c_int32), ("min_tag_width", C.c_int32)]
+                ("min_component_size", C.c_int32), ("min_tag_width", C.c_int32),
+                ("pair_table_log2", C.c_int32)]
 
 
 _lib = None
@@ -357,8 +358,8 @@ def find_boundaries(labels_dev, sizes_dev, w, h):
 class Detector:
     """Owns the per-frame buffers (allocated once) of the whole pipeline."""
 
-    def __init__(self, max_w_in, max_h_in, decimate=1):
-        self.params = AtqParams(decimate, 5, 25, 8)
+    def __init__(self, max_w_in, max_h_in, decimate=1, pair_table_log2=0):
+        self.params = AtqParams(decimate, 5, 25, 8, pair_table_log2)
         h = C.c_void_p()
         init()
         check(lib().atq_detector_create(max_w_in, max_h_in, C.byref(self.params), C.byref(h)))

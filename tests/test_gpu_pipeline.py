@@ -3,7 +3,9 @@ quads against the oracle (rows A8-A21), and the glue drop-ins chained.
 
 Bit-exact everywhere, element by element: the GPU writes boundary records
 in the reference's plane-major raster order and every sort is stable, as the
-oracle's are.
+oracle's are. The detector drops, before its key sort, the records of pairs
+whose record count ValidBlobFilter rejects; its grouped records are compared
+with the oracle's restricted to the other pairs.
 """
 import numpy as np
 import pytest
@@ -34,15 +36,15 @@ def _frames():
 FRAMES = _frames()
 
 
-def run_detector(atq, img, f):
+def run_detector(atq, img, f, **kw):
     from pyatq import records as R
     h_in, w_in = img.shape
-    det = atq.Detector(w_in, h_in, decimate=f)
+    det = atq.Detector(w_in, h_in, decimate=f, **kw)
     dev = atq.dev_from_np(img)
     det.detect(dev, w_in, h_in)
     atq.sync()
     c = det.counts()
-    out = {"counts": c}
+    out = {"counts": c, "dims": det.dims()}
     rd = det.read
     out["thresholded"] = rd("THRESHOLDED", np.uint8, det.dims()[0] * det.dims()[1])
     nb = int(c["n_boundary"])
@@ -63,6 +65,9 @@ def run_detector(atq, img, f):
     out["peak_extents"] = rd("PEAK_EXTENTS", R.PEAK_EXTENTS, ne)
     out["fitted"] = rd("FITTED", R.FITTED_QUAD, ne)
     out["quads"] = rd("QUADS", R.QUAD_CORNERS, int(c["n_quads"]))
+    slots = det.buffer("PAIR_KEYS")[1]
+    out["pair_keys"] = rd("PAIR_KEYS", np.uint64, slots)
+    out["pair_counts"] = rd("PAIR_COUNTS", np.uint32, slots)
     det.close()
     return out
 
@@ -71,11 +76,50 @@ def _bytes(a):
     return np.ascontiguousarray(a).view(np.uint8)
 
 
+def surviving_records(o, w, h):
+    """The oracle's grouped records restricted to the pairs whose record
+    count ValidBlobFilter can accept ([24, 2 * (2w + 2h)]): what the detector
+    keeps when its pair table did not overflow. Also the position of every
+    oracle record in that restricted array."""
+    keys, points = o["keys"], o["points"]
+    uniq, inv, cnt = np.unique(keys, return_inverse=True, return_counts=True)
+    ok_pair = (cnt >= 24) & (cnt <= 2 * (2 * w + 2 * h))
+    keep = ok_pair[inv]
+    new_pos = np.cumsum(keep) - keep
+    return keys[keep], points[keep], int(ok_pair.sum()), new_pos
+
+
+def check_pair_table(g, o):
+    """Every pair of the frame in the table, with its exact record count up
+    to the rejection limit and a count above it otherwise."""
+    w, h = g["dims"]
+    limit = 2 * (2 * w + 2 * h)
+    uniq, cnt = np.unique(o["keys"], return_counts=True)
+    used = g["pair_keys"] != 0          # slots hold ~key
+    keys = ~g["pair_keys"][used]
+    order = np.argsort(keys)
+    gk, gc = keys[order], g["pair_counts"][used][order]
+    np.testing.assert_array_equal(gk, uniq)
+    np.testing.assert_array_equal(np.minimum(gc, limit + 1), np.minimum(cnt, limit + 1))
+
+
 def check_against_oracle(g, o):
     c = g["counts"]
     assert int(c["overflow"]) == 0
-    assert int(c["n_boundary"]) == len(o["points"])
-    assert int(c["n_pairs"]) == len(o["bounds_all"])
+    w, h = g["dims"]
+    if not int(c["pair_filter_off"]):
+        check_pair_table(g, o)
+    o_bounds = o["bounds"]
+    if int(c["pair_filter_off"]):
+        keys, points, n_pairs = o["keys"], o["points"], len(o["bounds_all"])
+    else:
+        keys, points, n_pairs, new_pos = surviving_records(o, w, h)
+        # a cluster's start indexes the detector's own (restricted) records
+        o_bounds = o_bounds.copy()
+        if len(o_bounds):
+            o_bounds["start"] = new_pos[o_bounds["start"]]
+    assert int(c["n_boundary"]) == len(points)
+    assert int(c["n_pairs"]) == n_pairs
     assert int(c["n_clusters"]) == len(o["bounds"])
     assert int(c["n_cluster_points"]) == len(o["cluster_points"])
     assert int(c["n_corners"]) == len(o["corners"])
@@ -83,9 +127,10 @@ def check_against_oracle(g, o):
     assert int(c["oob_reads"]) == int(o["oob_reads"])
     np.testing.assert_array_equal(g["thresholded"], o["thresholded"].reshape(-1))
     # grouped records: the reference's stable order, element by element
-    np.testing.assert_array_equal(g["keys"], o["keys"])
-    np.testing.assert_array_equal(g["points"], o["points"])
-    for name in ("bounds", "extents", "cluster_points", "cluster_index", "line_fit",
+    np.testing.assert_array_equal(g["keys"], keys)
+    np.testing.assert_array_equal(g["points"], points)
+    np.testing.assert_array_equal(_bytes(g["bounds"]), _bytes(o_bounds), err_msg="bounds")
+    for name in ("extents", "cluster_points", "cluster_index", "line_fit",
                  "sparse_corners", "corners", "peak_extents", "fitted"):
         np.testing.assert_array_equal(_bytes(g[name]), _bytes(o[name]), err_msg=name)
     assert int(c["n_quads"]) == len(o["quads"])
@@ -115,3 +160,20 @@ def test_detector_reuse_shrinking_frames(atq, oracle):
         want, _ = oracle.detect(img, 1)
         np.testing.assert_array_equal(_bytes(got), _bytes(want))
     det.close()
+
+
+def test_detector_pair_table_overflow(atq, oracle):
+    """5x5 squares on a 2-pixel grid make thousands of cluster pairs; a
+    detector with a 256-slot pair table must notice the overflow, keep every
+    boundary record, and still match the oracle element by element."""
+    yy, xx = np.mgrid[0:480, 0:640]
+    img = np.where(((yy % 7) < 5) & ((xx % 7) < 5), 220, 30).astype(np.uint8)
+    o = oracle.pipeline(img, 1)
+    assert len(np.unique(o["keys"])) > 1000
+    g = run_detector(atq, img, 1, pair_table_log2=8)
+    assert int(g["counts"]["pair_filter_off"]) == 1
+    check_against_oracle(g, o)
+    # the default table holds them all: filtered, and the same downstream
+    g = run_detector(atq, img, 1)
+    assert int(g["counts"]["pair_filter_off"]) == 0
+    check_against_oracle(g, o)
