@@ -29,43 +29,78 @@ struct PixelRecords {
     ATQ_DEV uint32_t n() const { return __popc(mask); }
 };
 
-ATQ_DEV uint32_t label_at(const uint32_t *labels, uint32_t w, uint32_t h, uint32_t x, uint32_t y) {
-    return (x < w && y < h) ? labels[(size_t)y * w + x] : 0u;
+// The labels that 4 pixels (x0 .. x0+3 of row y, x0 a multiple of 4) test,
+// loaded together: columns x0-1 .. x0+4 of rows y and y+1 (0 outside the
+// image), and for each one whether its component holds at
+// least min_size pixels, all in one round of independent loads.
+struct QuadLabels {
+    uint32_t a[6], b[6];    // rows y and y+1
+    uint32_t big_a, big_b;  // bit i: sizes[value] >= min_size
+};
+
+ATQ_DEV void load_quad(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes,
+                       uint32_t w, uint32_t h, uint32_t min_size, uint32_t x0, uint32_t y,
+                       QuadLabels &q) {
+    const uint32_t *r0 = labels + (size_t)y * w;
+    const uint4 m0 = *reinterpret_cast<const uint4 *>(r0 + x0);
+    q.a[0] = x0 > 0 ? r0[x0 - 1] : 0u;
+    q.a[1] = m0.x;
+    q.a[2] = m0.y;
+    q.a[3] = m0.z;
+    q.a[4] = m0.w;
+    q.a[5] = x0 + 4 < w ? r0[x0 + 4] : 0u;
+    if (y + 1 < h) {
+        const uint32_t *r1 = r0 + w;
+        const uint4 m1 = *reinterpret_cast<const uint4 *>(r1 + x0);
+        q.b[0] = x0 > 0 ? r1[x0 - 1] : 0u;
+        q.b[1] = m1.x;
+        q.b[2] = m1.y;
+        q.b[3] = m1.z;
+        q.b[4] = m1.w;
+        q.b[5] = x0 + 4 < w ? r1[x0 + 4] : 0u;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 6; i++) q.b[i] = 0u;
+    }
+    q.big_a = q.big_b = 0;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        if (sizes[q.a[i] & LABEL_VALUE_MASK] >= min_size) q.big_a |= 1u << i;
+        if (sizes[q.b[i] & LABEL_VALUE_MASK] >= min_size) q.big_b |= 1u << i;
+    }
 }
 
-// key_shift: 32 for the reference's min<<32|max, lb for the packed form.
-ATQ_DEV void pixel_records(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes,
-                           uint32_t w, uint32_t h, uint32_t x, uint32_t y, uint32_t min_size,
-                           uint32_t key_shift, PixelRecords &out) {
+// The records of pixel x = x0 + k (x >= 1; column 0 is never a local pixel):
+// RIGHT, BOTTOM_LEFT (unless left and down differ in white bit and are both
+// large), BOTTOM, BOTTOM_RIGHT, each when the white bits differ and both
+// components are large (src/find_boundaries.cpp:94-170).
+ATQ_DEV void quad_pixel_records(const QuadLabels &q, uint32_t x0, uint32_t y, int k,
+                                uint32_t key_shift, PixelRecords &out) {
     out.mask = 0;
-    if (x == 0 || x >= w || y >= h) return;
-    const uint32_t L = labels[(size_t)y * w + x];
+    const uint32_t x = x0 + k;
+    if (x == 0) return;
+    const uint32_t L = q.a[k + 1];
     const uint32_t lv = L & LABEL_VALUE_MASK, wl = L & LABEL_PIXEL_MASK;
-    const bool local_big = sizes[lv] >= min_size;
-    const uint32_t right = label_at(labels, w, h, x + 1, y);
-    const uint32_t down = label_at(labels, w, h, x, y + 1);
-    const uint32_t down_left = label_at(labels, w, h, x - 1, y + 1);
-    const uint32_t down_right = label_at(labels, w, h, x + 1, y + 1);
-    auto test = [&](uint32_t T, int plane) {
+    const bool local_big = (q.big_a >> (k + 1)) & 1u;
+    auto test = [&](uint32_t T, bool t_big, int plane) {
         const uint32_t wt = T & LABEL_PIXEL_MASK;
         if (wt == wl) return;
+        if (!local_big || !t_big) return;
         const uint32_t tv = T & LABEL_VALUE_MASK;
-        if (!local_big || sizes[tv] < min_size) return;
         out.mask |= 1u << plane;
         out.p[plane].packed_x = (uint16_t)(x | ((uint32_t)plane << 14));
         out.p[plane].packed_y = (uint16_t)(wt > wl ? (COLOR_DIRECTION_MASK | y) : y);
         out.k[plane] = ((uint64_t)min(lv, tv) << key_shift) | (uint64_t)max(lv, tv);
     };
-    test(right, 0);
+    test(q.a[k + 2], (q.big_a >> (k + 2)) & 1u, 0);  // right
     {
-        const uint32_t left = labels[(size_t)y * w + x - 1];
-        bool dup = (left & LABEL_PIXEL_MASK) != (down & LABEL_PIXEL_MASK) &&
-                   sizes[left & LABEL_VALUE_MASK] >= min_size &&
-                   sizes[down & LABEL_VALUE_MASK] >= min_size;
-        if (!dup) test(down_left, 1);
+        const uint32_t left = q.a[k], down = q.b[k + 1];
+        const bool dup = (left & LABEL_PIXEL_MASK) != (down & LABEL_PIXEL_MASK) &&
+                         ((q.big_a >> k) & 1u) && ((q.big_b >> (k + 1)) & 1u);
+        if (!dup) test(q.b[k], (q.big_b >> k) & 1u, 1);  // bottom-left
     }
-    test(down, 2);
-    test(down_right, 3);
+    test(q.b[k + 1], (q.big_b >> (k + 1)) & 1u, 2);  // bottom
+    test(q.b[k + 2], (q.big_b >> (k + 2)) & 1u, 3);  // bottom-right
 }
 
 }  // namespace
@@ -74,17 +109,22 @@ __global__ void __launch_bounds__(256)
 k_boundaries_sparse(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes,
                     uint32_t w, uint32_t h, atq_boundary_point *__restrict__ points,
                     uint64_t *__restrict__ keys, uint32_t min_size) {
-    uint32_t x = blockIdx.x * 64 + (threadIdx.x & 63);
-    uint32_t y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= w || y >= h) return;
-    PixelRecords r;
-    pixel_records(labels, sizes, w, h, x, y, min_size, 32, r);
-    const size_t n = (size_t)w * h, lin = (size_t)y * w + x;
+    const size_t n = (size_t)w * h;
+    const size_t p0 = ((size_t)blockIdx.x * 256 + threadIdx.x) * 4;
+    if (p0 >= n) return;
+    const uint32_t y = (uint32_t)(p0 / w), x0 = (uint32_t)(p0 % w);
+    QuadLabels q;
+    load_quad(labels, sizes, w, h, min_size, x0, y, q);
 #pragma unroll
-    for (int p = 0; p < 4; p++) {
-        const bool has = (r.mask >> p) & 1u;
-        points[p * n + lin] = has ? r.p[p] : atq_boundary_point{0, 0};
-        keys[p * n + lin] = has ? r.k[p] : ~0ull;
+    for (int k = 0; k < 4; k++) {
+        PixelRecords r;
+        quad_pixel_records(q, x0, y, k, 32, r);
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const bool has = (r.mask >> p) & 1u;
+            points[p * n + p0 + k] = has ? r.p[p] : atq_boundary_point{0, 0};
+            keys[p * n + p0 + k] = has ? r.k[p] : ~0ull;
+        }
     }
 }
 
@@ -185,22 +225,28 @@ ATQ_DEV void for_each_pair_run(const PixelRecords (&r)[BND_PX_PER_THREAD], F f) 
     if (c) f(cur, c);
 }
 
+// Records of the 4 pixels of this thread in tile `tile` (w is a multiple
+// of 4, so the 4 pixels share a row).
 ATQ_DEV void tile_records(const uint32_t *labels, const uint32_t *sizes, uint32_t w, uint32_t h,
-                          uint32_t min_size, PixelRecords (&r)[BND_PX_PER_THREAD]) {
+                          uint32_t min_size, uint32_t tile, PixelRecords (&r)[BND_PX_PER_THREAD]) {
     const size_t n = (size_t)w * h;
-    const size_t p0 = (size_t)blockIdx.x * BND_TILE + threadIdx.x * BND_PX_PER_THREAD;
+    const size_t p0 = (size_t)tile * BND_TILE + threadIdx.x * BND_PX_PER_THREAD;
 #pragma unroll
     for (int k = 0; k < BND_PX_PER_THREAD; k++) r[k].mask = 0;
     if (p0 >= n) return;
     const uint32_t y = (uint32_t)(p0 / w), x0 = (uint32_t)(p0 % w);
+    QuadLabels q;
+    load_quad(labels, sizes, w, h, min_size, x0, y, q);
 #pragma unroll
-    for (int k = 0; k < BND_PX_PER_THREAD; k++)
-        pixel_records(labels, sizes, w, h, x0 + k, y, min_size, 32, r[k]);
+    for (int k = 0; k < BND_PX_PER_THREAD; k++) quad_pixel_records(q, x0, y, k, 32, r[k]);
 }
 
+// Each work-group takes a contiguous run of tiles and merges all their
+// records in its LDS table before touching the global one, so a pair that
+// spans the frame costs one global add per work-group, not per tile.
 __global__ void __launch_bounds__(256)
 k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes, uint32_t w,
-             uint32_t h, uint32_t min_size, PairTable table, uint32_t max_count,
+             uint32_t h, uint32_t min_size, uint32_t tiles, PairTable table, uint32_t max_count,
              atq_counts *counts) {
     __shared__ uint64_t lk[TILE_SLOTS];
     __shared__ uint32_t lc[TILE_SLOTS];
@@ -209,13 +255,17 @@ k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
         lc[i] = 0;
     }
     __syncthreads();
-    PixelRecords r[BND_PX_PER_THREAD];
-    tile_records(labels, sizes, w, h, min_size, r);
-    for_each_pair_run(r, [&](uint64_t key, uint32_t c) {
-        const uint32_t sl = tile_slot(lk, key, true);
-        if (sl != NO_SLOT) atomicAdd(&lc[sl], c);
-        else global_add(table, key, c, max_count, counts);
-    });
+    const uint32_t per = (tiles + gridDim.x - 1) / gridDim.x;
+    const uint32_t t_end = min(tiles, (blockIdx.x + 1) * per);
+    for (uint32_t tile = blockIdx.x * per; tile < t_end; tile++) {
+        PixelRecords r[BND_PX_PER_THREAD];
+        tile_records(labels, sizes, w, h, min_size, tile, r);
+        for_each_pair_run(r, [&](uint64_t key, uint32_t c) {
+            const uint32_t sl = tile_slot(lk, key, true);
+            if (sl != NO_SLOT) atomicAdd(&lc[sl], c);
+            else global_add(table, key, c, max_count, counts);
+        });
+    }
     __syncthreads();
     for (int i = threadIdx.x; i < TILE_SLOTS; i += 256)
         if (lk[i]) global_add(table, ~lk[i], lc[i], max_count, counts);
@@ -236,7 +286,7 @@ k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
     const bool keep_all = counts->pair_filter_off != 0;
     __syncthreads();
     PixelRecords r[BND_PX_PER_THREAD];
-    tile_records(labels, sizes, w, h, min_size, r);
+    tile_records(labels, sizes, w, h, min_size, blockIdx.x, r);
     auto survives = [&](uint64_t key) -> bool {
         const uint32_t g = global_slot(table, key, false);
         if (g == NO_SLOT) return true;
@@ -309,8 +359,7 @@ struct TileOffset {
 int launch_find_boundaries_sparse(hipStream_t s, const uint32_t *labels, const uint32_t *sizes,
                                   size_t w, size_t h, atq_boundary_point *points, uint64_t *keys,
                                   int min_size) {
-    dim3 grid(atq_div_up(w, 64), atq_div_up(h, 4));
-    k_boundaries_sparse<<<grid, 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h, points, keys,
+    k_boundaries_sparse<<<atq_div_up(w * h, 1024), 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h, points, keys,
                                              (uint32_t)min_size);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
@@ -324,8 +373,9 @@ int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const
     const unsigned tiles = atq_div_up(w * h, BND_TILE);
     const uint32_t max_count = (uint32_t)(2 * (2 * w + 2 * h));
     uint32_t *tile_counts = tile_scratch, *tile_off = tile_scratch + tiles;
-    k_pair_count<<<tiles, 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size,
-                                       table, max_count, counts);
+    k_pair_count<<<min(tiles, 512u), 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h,
+                                                   (uint32_t)min_size, tiles, table, max_count,
+                                                   counts);
     k_bnd_filter<<<tiles, 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size,
                                        label_bits, table, max_count, counts, tile_keys, tile_points,
                                        tile_counts);

@@ -73,54 +73,102 @@ __global__ void k_seg_close(const uint32_t *n_dev, const atq_counts *counts, uin
     seg_start[counts->n_pairs] = n;
 }
 
-// One wave per segment: reduce ClusterBounds and apply ValidBlobFilter.
+// One work-group per segment: reduce ClusterBounds and apply
+// ValidBlobFilter. Each thread loads 4 records per round before using them.
+struct BoundsAcc {
+    uint32_t xmin, xmax, ymin, ymax;
+    int32_t gxs, gys;
+    int64_t pg;
+    ATQ_DEV void init() {
+        xmin = ymin = 0xFFFF;
+        xmax = ymax = 0;
+        gxs = gys = 0;
+        pg = 0;
+    }
+    ATQ_DEV void add(atq_boundary_point p) {
+        const uint32_t xx = x_with_dx(p.packed_x), yy = y_with_dy(p.packed_x, p.packed_y);
+        const int gx = grad_x(p.packed_x, p.packed_y), gy = grad_y(p.packed_x, p.packed_y);
+        xmin = min(xmin, xx);
+        xmax = max(xmax, xx);
+        ymin = min(ymin, yy);
+        ymax = max(ymax, yy);
+        gxs += gx;
+        gys += gy;
+        pg += (int64_t)xx * gx + (int64_t)yy * gy;
+    }
+    ATQ_DEV void merge(const BoundsAcc &o) {
+        xmin = min(xmin, o.xmin);
+        xmax = max(xmax, o.xmax);
+        ymin = min(ymin, o.ymin);
+        ymax = max(ymax, o.ymax);
+        gxs += o.gxs;
+        gys += o.gys;
+        pg += o.pg;
+    }
+    ATQ_DEV void wave_reduce() {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            BoundsAcc t;
+            t.xmin = __shfl_xor(xmin, o, 64);
+            t.xmax = __shfl_xor(xmax, o, 64);
+            t.ymin = __shfl_xor(ymin, o, 64);
+            t.ymax = __shfl_xor(ymax, o, 64);
+            t.gxs = __shfl_xor(gxs, o, 64);
+            t.gys = __shfl_xor(gys, o, 64);
+            t.pg = __shfl_xor(pg, o, 64);
+            merge(t);
+        }
+    }
+};
+
 __global__ void __launch_bounds__(256)
 k_seg_bounds(const atq_boundary_point *__restrict__ pts, const uint32_t *__restrict__ seg_start,
              const atq_counts *counts, uint32_t w, uint32_t h, uint32_t *__restrict__ valid,
              atq_cluster_bounds *__restrict__ seg_bounds) {
+    __shared__ BoundsAcc part[4];
     const uint32_t S = counts->n_pairs;
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t max_px = 2ull * (2ull * w + 2ull * h);
-    for (uint32_t s = wave; s < S; s += nwaves) {
+    for (uint32_t s = blockIdx.x; s < S; s += gridDim.x) {
         const uint32_t b = seg_start[s], e = seg_start[s + 1], c = e - b;
         if (c < 24 || c > max_px) {
-            if (lane == 0) valid[s] = 0;
+            if (threadIdx.x == 0) valid[s] = 0;
             continue;
         }
-        uint32_t xmin = 0xFFFF, xmax = 0, ymin = 0xFFFF, ymax = 0;
-        int32_t gxs = 0, gys = 0;
-        int64_t pg = 0;
-        for (uint32_t i = b + lane; i < e; i += 64) {
-            atq_boundary_point p = pts[i];
-            uint32_t xx = x_with_dx(p.packed_x), yy = y_with_dy(p.packed_x, p.packed_y);
-            int gx = grad_x(p.packed_x, p.packed_y), gy = grad_y(p.packed_x, p.packed_y);
-            xmin = min(xmin, xx); xmax = max(xmax, xx);
-            ymin = min(ymin, yy); ymax = max(ymax, yy);
-            gxs += gx; gys += gy;
-            pg += (int64_t)xx * gx + (int64_t)yy * gy;
-        }
+        BoundsAcc acc;
+        acc.init();
+        for (uint32_t i0 = b + threadIdx.x; i0 < e; i0 += 4 * 256) {
+            atq_boundary_point q[4];
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            xmin = min(xmin, (uint32_t)__shfl_xor(xmin, o, 64));
-            xmax = max(xmax, (uint32_t)__shfl_xor(xmax, o, 64));
-            ymin = min(ymin, (uint32_t)__shfl_xor(ymin, o, 64));
-            ymax = max(ymax, (uint32_t)__shfl_xor(ymax, o, 64));
-            gxs += __shfl_xor(gxs, o, 64);
-            gys += __shfl_xor(gys, o, 64);
-            pg += __shfl_xor(pg, o, 64);
+            for (int k = 0; k < 4; k++) {
+                const uint32_t i = i0 + 256 * k;
+                q[k] = i < e ? pts[i] : atq_boundary_point{0, 0};
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (i0 + 256 * k < e) acc.add(q[k]);
         }
-        if (lane == 0) {
+        acc.wave_reduce();
+        if (lane == 0) part[wave] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int q = 1; q < 4; q++) acc.merge(part[q]);
             atq_cluster_bounds cb;
-            cb.x_min = (uint16_t)xmin; cb.x_max = (uint16_t)xmax;
-            cb.y_min = (uint16_t)ymin; cb.y_max = (uint16_t)ymax;
-            cb.gx_sum = gxs; cb.gy_sum = gys; cb.pxgx_pygy_sum = pg;
-            cb.start = b; cb.count = c;
-            bool ok = valid_blob(cb, w, h);
+            cb.x_min = (uint16_t)acc.xmin;
+            cb.x_max = (uint16_t)acc.xmax;
+            cb.y_min = (uint16_t)acc.ymin;
+            cb.y_max = (uint16_t)acc.ymax;
+            cb.gx_sum = acc.gxs;
+            cb.gy_sum = acc.gys;
+            cb.pxgx_pygy_sum = acc.pg;
+            cb.start = b;
+            cb.count = c;
+            const bool ok = valid_blob(cb, w, h);
             valid[s] = ok;
             if (ok) seg_bounds[s] = cb;
         }
+        __syncthreads();
     }
 }
 
