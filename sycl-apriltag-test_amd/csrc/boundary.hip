@@ -369,7 +369,7 @@ int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const
                                     const PairTable &table, atq_counts *counts,
                                     uint64_t *tile_keys, atq_boundary_point *tile_points,
                                     uint32_t *tile_scratch, uint64_t *keys,
-                                    atq_boundary_point *points, uint32_t *scan_scratch) {
+                                    atq_boundary_point *points, ScanCtx &sc) {
     const unsigned tiles = atq_div_up(w * h, BND_TILE);
     const uint32_t max_count = (uint32_t)(2 * (2 * w + 2 * h));
     uint32_t *tile_counts = tile_scratch, *tile_off = tile_scratch + tiles;
@@ -380,7 +380,7 @@ int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const
                                        label_bits, table, max_count, counts, tile_keys, tile_points,
                                        tile_counts);
     int r = device_scan(s, nullptr, tiles, TileCount{tile_counts}, TileOffset{tile_off},
-                        scan_scratch, &counts->n_boundary);
+                        sc, &counts->n_boundary);
     if (r) return r;
     k_bnd_gather<<<tiles, 256, 0, s>>>(tile_keys, tile_points, tile_counts, tile_off, keys, points);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;

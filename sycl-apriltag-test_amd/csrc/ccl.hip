@@ -9,7 +9,7 @@
 // links the block's colour-c node to the colour-c node of the block holding
 // it. Components are labelled by their minimum node id (min-linking).
 //
-// gfx950 organisation (three kernels instead of the reference's five):
+// gfx950 organisation (four kernels, the reference has five):
 //  1. k_ccl_local: a work-group owns a TBX x TBY tile of blocks, stages its
 //     pixels (+1 px halo) in LDS, unions the tile-internal edges with an LDS
 //     union-find, counts each local root's pixels, and writes for every node
@@ -18,9 +18,11 @@
 //  2. k_ccl_border: edges that cross tiles are merged on the global parent
 //     array with the reference's lock-free atomicMin union
 //     (include/union_find.h:50-77).
-//  3. k_ccl_final: each block resolves its two roots, adds each local root's
-//     count into its global root's size, and writes the four label words
-//     (white = 0x80000000|root, black = root, 127 -> 0).
+//  3. k_ccl_flatten: every local root is pointed straight at its final
+//     root and adds its count into the root's size.
+//  4. k_ccl_final: each block reads its two roots (at most two loads) and
+//     writes the four label words (white = 0x80000000|root, black = root,
+//     127 -> 0).
 #include "kernels.hpp"
 
 namespace atq {
@@ -99,10 +101,25 @@ ATQ_DEV uint32_t g_find(const uint32_t *parent, uint32_t n) {
     }
 }
 
+// Root of local root n (a FLAG-free word), halving the path on the way:
+// every visited word is lowered to its grandparent. Parents only ever
+// decrease towards the root (min-linking), so atomicMin keeps each word an
+// ancestor in the same tree while unions run concurrently.
+ATQ_DEV uint32_t g_find_halving(uint32_t *parent, uint32_t n) {
+    while (true) {
+        const uint32_t p = g_load(&parent[n]);
+        if (p == n) return n;
+        const uint32_t gp = g_load(&parent[p]);
+        if (gp == p) return p;
+        atomicMin(&parent[n], gp);
+        n = gp;
+    }
+}
+
 ATQ_DEV void g_union(uint32_t *parent, uint32_t a, uint32_t b) {
     while (true) {
-        a = g_find(parent, a);
-        b = g_find(parent, b);
+        a = g_find_halving(parent, a);
+        b = g_find_halving(parent, b);
         if (a < b) {
             uint32_t old = atomicMin(&parent[b], a);
             if (old == b) return;
@@ -255,31 +272,48 @@ k_ccl_border(const uint8_t *__restrict__ thr, uint32_t w, uint32_t h, uint32_t t
     }
 }
 
+// After the border merge: every local root gets its final root (path
+// halving, then the root itself), and adds its pixel count into the root's
+// size. Once the root's size reaches size_cap further adds are skipped (the
+// detector only asks whether a size reaches min_component_size; the drop-in
+// passes UINT32_MAX for exact sizes).
 __global__ void __launch_bounds__(256)
-k_ccl_final(const uint8_t *__restrict__ thr, uint32_t w, uint32_t h, uint32_t *__restrict__ parent,
-            uint32_t *__restrict__ labels, uint32_t *__restrict__ sizes) {
+k_ccl_flatten(uint32_t w, uint32_t h, uint32_t *__restrict__ parent, uint32_t *__restrict__ sizes,
+              uint32_t size_cap) {
+    const uint32_t bx = blockIdx.x * blockDim.x + threadIdx.x, by = blockIdx.y;
+    if (2 * bx >= w || 2 * by >= h) return;
+    const uint32_t id = 2 * by * w + 2 * bx;
+#pragma unroll
+    for (uint32_t c = 0; c < 2; c++) {
+        const uint32_t n = id + c;
+        const uint32_t v = parent[n];
+        if ((v & FLAG) || v == n) continue;  // not a local root, or a final root
+        const uint32_t r = g_find_halving(parent, v);
+        atomicMin(&parent[n], r);
+        const uint32_t s = sizes[n];
+        if (s && g_load(&sizes[r]) < size_cap) atomicAdd(&sizes[r], s);
+    }
+}
+
+// Labels: white = 0x80000000 | root, black = root, 127 -> 0. A node's word
+// is its final root (a local root, flattened) or FLAG | its local root.
+__global__ void __launch_bounds__(256)
+k_ccl_final(const uint8_t *__restrict__ thr, uint32_t w, uint32_t h,
+            const uint32_t *__restrict__ parent, uint32_t *__restrict__ labels) {
     uint32_t bx = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t by = blockIdx.y;
     if (2 * bx >= w || 2 * by >= h) return;
     uint32_t x = 2 * bx, y = 2 * by;
     uint32_t id = y * w + x;
-    uint32_t root[2];
-#pragma unroll
-    for (int c = 0; c < 2; c++) {
-        uint32_t v = parent[id + c];
-        uint32_t r = g_find(parent, v & ~FLAG);
-        root[c] = r;
-        if (!(v & FLAG) && r != id + c) {
-            uint32_t s = sizes[id + c];
-            if (s) atomicAdd(&sizes[r], s);
-        }
-    }
+    const uint2 v = *reinterpret_cast<const uint2 *>(parent + id);
+    const uint32_t r0 = (v.x & FLAG) ? parent[v.x & ~FLAG] : v.x;
+    const uint32_t r1 = (v.y & FLAG) ? parent[v.y & ~FLAG] : v.y;
     uint16_t top = *reinterpret_cast<const uint16_t *>(thr + id);
     uint16_t bot = *reinterpret_cast<const uint16_t *>(thr + id + w);
-    auto lab = [&](uint32_t v) -> uint32_t {
-        v &= 0xFFu;
-        if (v == C_WHITE) return LABEL_PIXEL_MASK | root[0];
-        if (v == C_BLACK) return root[1];
+    auto lab = [&](uint32_t p) -> uint32_t {
+        p &= 0xFFu;
+        if (p == C_WHITE) return LABEL_PIXEL_MASK | r0;
+        if (p == C_BLACK) return r1;
         return 0u;
     };
     uint2 t2 = make_uint2(lab(top), lab(top >> 8));
@@ -289,14 +323,15 @@ k_ccl_final(const uint8_t *__restrict__ thr, uint32_t w, uint32_t h, uint32_t *_
 }
 
 int launch_ccl(hipStream_t s, const uint8_t *thr, size_t w, size_t h, uint32_t *parent,
-               uint32_t *labels, uint32_t *sizes) {
+               uint32_t *labels, uint32_t *sizes, uint32_t size_cap) {
     dim3 tiles(atq_div_up(w, 2 * TBX), atq_div_up(h, 2 * TBY));
     k_ccl_local<<<tiles, 256, 0, s>>>(thr, (uint32_t)w, (uint32_t)h, parent, sizes);
     uint32_t n_items = tiles.x * tiles.y * (TBX + 2 * (TBY - 1));
     k_ccl_border<<<atq_div_up(n_items, 256), 256, 0, s>>>(thr, (uint32_t)w, (uint32_t)h, tiles.x,
                                                           n_items, parent);
     dim3 fgrid(atq_div_up(w / 2, 256), (unsigned)(h / 2));
-    k_ccl_final<<<fgrid, 256, 0, s>>>(thr, (uint32_t)w, (uint32_t)h, parent, labels, sizes);
+    k_ccl_flatten<<<fgrid, 256, 0, s>>>((uint32_t)w, (uint32_t)h, parent, sizes, size_cap);
+    k_ccl_final<<<fgrid, 256, 0, s>>>(thr, (uint32_t)w, (uint32_t)h, parent, labels);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 

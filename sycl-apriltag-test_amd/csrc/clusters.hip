@@ -406,16 +406,16 @@ int launch_assign_search(hipStream_t s, const atq_boundary_point *pts, const uin
 }
 
 int launch_run_extents(hipStream_t s, const uint32_t *cidx, const uint32_t *n_dev, uint32_t cap,
-                       atq_cluster_extents *ext, uint32_t *n_runs_dev, uint32_t *scan_scratch) {
-    return device_scan(s, n_dev, cap, RunHead{cidx}, RunOut{ext, n_dev, cap, cidx}, scan_scratch,
+                       atq_cluster_extents *ext, uint32_t *n_runs_dev, ScanCtx &sc) {
+    return device_scan(s, n_dev, cap, RunHead{cidx}, RunOut{ext, n_dev, cap, cidx}, sc,
                        n_runs_dev);
 }
 
 int launch_segments(hipStream_t s, const uint64_t *keys, uint32_t key_shift, const uint32_t *n_dev,
                     atq_counts *counts, uint32_t cap, uint32_t *seg_id, uint32_t *seg_start,
-                    uint32_t *scan_scratch) {
+                    ScanCtx &sc) {
     int r = device_scan(s, n_dev, cap, HeadFlag{keys, key_shift}, HeadOut{seg_id, seg_start},
-                        scan_scratch, &counts->n_pairs);
+                        sc, &counts->n_pairs);
     if (r) return r;
     k_seg_close<<<1, 1, 0, s>>>(n_dev, counts, cap, seg_start);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
@@ -424,16 +424,16 @@ int launch_segments(hipStream_t s, const uint64_t *keys, uint32_t key_shift, con
 int launch_bounds_filter(hipStream_t s, const atq_boundary_point *pts, const uint32_t *seg_start,
                          atq_counts *counts, uint32_t cap, uint32_t w, uint32_t h, uint32_t *valid,
                          atq_cluster_bounds *seg_bounds, uint32_t *seg_cluster,
-                         atq_cluster_bounds *bounds, uint32_t *scan_scratch) {
+                         atq_cluster_bounds *bounds, ScanCtx &sc) {
     k_seg_bounds<<<2048, 256, 0, s>>>(pts, seg_start, counts, w, h, valid, seg_bounds);
     return device_scan(s, &counts->n_pairs, cap, ValidFlag{valid},
-                       ValidOut{seg_cluster, seg_bounds, bounds}, scan_scratch, &counts->n_clusters);
+                       ValidOut{seg_cluster, seg_bounds, bounds}, sc, &counts->n_clusters);
 }
 
 int launch_extents(hipStream_t s, const atq_cluster_bounds *bounds, atq_counts *counts, uint32_t cap,
-                   atq_cluster_extents *ext, uint32_t *scan_scratch) {
+                   atq_cluster_extents *ext, ScanCtx &sc) {
     return device_scan(s, &counts->n_clusters, cap, ClusterCount{bounds}, ExtentOut{ext},
-                       scan_scratch, &counts->n_cluster_points);
+                       sc, &counts->n_cluster_points);
 }
 
 int launch_assign(hipStream_t s, const atq_boundary_point *pts, const uint32_t *seg_id,

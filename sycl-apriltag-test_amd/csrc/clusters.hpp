@@ -5,16 +5,18 @@
 
 namespace atq {
 
+struct ScanCtx;  // scan.hpp
+
 // clusters.hip
 int launch_segments(hipStream_t s, const uint64_t *keys, uint32_t key_shift, const uint32_t *n_dev,
                     atq_counts *counts, uint32_t cap, uint32_t *seg_id, uint32_t *seg_start,
-                    uint32_t *scan_scratch);
+                    ScanCtx &sc);
 int launch_bounds_filter(hipStream_t s, const atq_boundary_point *pts, const uint32_t *seg_start,
                          atq_counts *counts, uint32_t cap, uint32_t w, uint32_t h, uint32_t *valid,
                          atq_cluster_bounds *seg_bounds, uint32_t *seg_cluster,
-                         atq_cluster_bounds *bounds, uint32_t *scan_scratch);
+                         atq_cluster_bounds *bounds, ScanCtx &sc);
 int launch_extents(hipStream_t s, const atq_cluster_bounds *bounds, atq_counts *counts, uint32_t cap,
-                   atq_cluster_extents *ext, uint32_t *scan_scratch);
+                   atq_cluster_extents *ext, ScanCtx &sc);
 int launch_assign(hipStream_t s, const atq_boundary_point *pts, const uint32_t *seg_id,
                   const uint32_t *seg_start, const uint32_t *seg_cluster,
                   const atq_cluster_bounds *bounds, const atq_cluster_extents *ext,
@@ -38,7 +40,7 @@ int launch_assign_search(hipStream_t s, const atq_boundary_point *pts, const uin
                          uint32_t cap, const atq_cluster_bounds *bounds, const atq_counts *counts,
                          const atq_cluster_extents *ext, atq_cluster_point *cpts, uint32_t *cidx);
 int launch_run_extents(hipStream_t s, const uint32_t *cidx, const uint32_t *n_dev, uint32_t cap,
-                       atq_cluster_extents *ext, uint32_t *n_runs_dev, uint32_t *scan_scratch);
+                       atq_cluster_extents *ext, uint32_t *n_runs_dev, ScanCtx &sc);
 
 // fitquad.hip
 int launch_fit_lines(hipStream_t s, const atq_line_fit_point *lf, const uint32_t *cidx,
@@ -52,7 +54,7 @@ int launch_zero_u32(hipStream_t s, uint32_t *p, const uint32_t *n_dev, uint32_t 
 int launch_corner_segments(hipStream_t s, const atq_corner *sparse, const uint32_t *n_sparse_dev,
                            uint32_t cap, atq_counts *counts, uint32_t cap_cl, uint32_t max_seg,
                            atq_peak_extents *peaks, atq_corner *compacted, uint32_t *seg_of,
-                           uint64_t *scratch, uint32_t *scan_scratch);
+                           uint64_t *scratch, ScanCtx &sc);
 int launch_do_indexing(hipStream_t s, const atq_peak_extents *peaks, const uint32_t *n_seg_dev,
                        uint32_t cap, const atq_corner *corners, const atq_line_fit_point *lf,
                        const uint32_t *n_points_dev, uint32_t n_points_host,
@@ -60,6 +62,6 @@ int launch_do_indexing(hipStream_t s, const atq_peak_extents *peaks, const uint3
                        uint32_t *oob_counter);
 int launch_quads(hipStream_t s, const atq_fitted_quad *fitted, const uint32_t *n_fitted_dev,
                  uint32_t cap, atq_quad_corners *tmp, uint32_t *flags, atq_quad_corners *out,
-                 uint32_t *n_out_dev, uint32_t *scan_scratch, int min_tag_width);
+                 uint32_t *n_out_dev, ScanCtx &sc, int min_tag_width);
 
 }  // namespace atq

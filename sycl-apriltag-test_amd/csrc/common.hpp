@@ -28,6 +28,7 @@ constexpr int WAVE = 64;
 // overflow flag bits in atq_counts::overflow
 constexpr uint32_t OVF_BOUNDARY = 1u << 0;
 constexpr uint32_t OVF_CORNERS = 1u << 1;
+constexpr uint32_t OVF_SCAN = 1u << 2;  // a look-back wait gave up: results void
 
 ATQ_DEV int half_pixel_dx(uint16_t packed_x) {
     switch ((packed_x & HALF_PIXEL_MASK) >> 14) {

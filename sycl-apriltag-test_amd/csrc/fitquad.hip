@@ -447,11 +447,11 @@ int launch_fit_lines(hipStream_t s, const atq_line_fit_point *lf, const uint32_t
 int launch_corner_segments(hipStream_t s, const atq_corner *sparse, const uint32_t *n_sparse_dev,
                            uint32_t cap, atq_counts *counts, uint32_t cap_cl, uint32_t max_seg,
                            atq_peak_extents *peaks, atq_corner *compacted, uint32_t *seg_of,
-                           uint64_t *scratch, uint32_t *scan_scratch) {
+                           uint64_t *scratch, ScanCtx &sc) {
     int r = device_scan(s, n_sparse_dev, cap, CornerFlag{sparse}, CornerOut{sparse, compacted},
-                        scan_scratch, &counts->n_corners);
+                        sc, &counts->n_corners);
     if (r) return r;
-    r = device_scan(s, &counts->n_corners, cap, SegHead{compacted}, SegStart{peaks, seg_of}, scan_scratch,
+    r = device_scan(s, &counts->n_corners, cap, SegHead{compacted}, SegStart{peaks, seg_of}, sc,
                     &counts->n_segments);
     if (r) return r;
     k_peak_close<<<min(1024u, atq_div_up(cap_cl, 256) + 1), 256, 0, s>>>(counts, cap_cl, peaks);
@@ -476,10 +476,10 @@ int launch_do_indexing(hipStream_t s, const atq_peak_extents *peaks, const uint3
 
 int launch_quads(hipStream_t s, const atq_fitted_quad *fitted, const uint32_t *n_fitted_dev,
                  uint32_t cap, atq_quad_corners *tmp, uint32_t *flags, atq_quad_corners *out,
-                 uint32_t *n_out_dev, uint32_t *scan_scratch, int min_tag_width) {
+                 uint32_t *n_out_dev, ScanCtx &sc, int min_tag_width) {
     k_quads<<<min(1024u, atq_div_up(cap, 256) + 1), 256, 0, s>>>(fitted, n_fitted_dev, cap, tmp,
                                                                   flags, min_tag_width);
-    return device_scan(s, n_fitted_dev, cap, QuadFlag{flags}, QuadOut{tmp, out}, scan_scratch,
+    return device_scan(s, n_fitted_dev, cap, QuadFlag{flags}, QuadOut{tmp, out}, sc,
                        n_out_dev);
 }
 

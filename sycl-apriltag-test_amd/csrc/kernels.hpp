@@ -4,6 +4,8 @@
 
 namespace atq {
 
+struct ScanCtx;  // scan.hpp
+
 // threshold.hip
 int launch_decimate(hipStream_t s, const uint8_t *in, size_t w_in, size_t h_in, int f,
                     uint8_t *out);
@@ -13,7 +15,7 @@ int launch_threshold(hipStream_t s, const uint8_t *in, size_t w_in, size_t h_in,
 
 // ccl.hip
 int launch_ccl(hipStream_t s, const uint8_t *thr, size_t w, size_t h, uint32_t *parent,
-               uint32_t *labels, uint32_t *sizes);
+               uint32_t *labels, uint32_t *sizes, uint32_t size_cap = 0xFFFFFFFFu);
 
 // boundary.hip
 int launch_find_boundaries_sparse(hipStream_t s, const uint32_t *labels, const uint32_t *sizes,
@@ -34,7 +36,7 @@ int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const
                                     const PairTable &table, atq_counts *counts,
                                     uint64_t *tile_keys, atq_boundary_point *tile_points,
                                     uint32_t *tile_scratch, uint64_t *keys,
-                                    atq_boundary_point *points, uint32_t *scan_scratch);
+                                    atq_boundary_point *points, ScanCtx &sc);
 constexpr unsigned BOUNDARY_TILE_PX = 1024;  // pixels per compaction tile
 
 // diff.hip

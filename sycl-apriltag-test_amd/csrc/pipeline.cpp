@@ -102,6 +102,10 @@ int take_workspace(Workspace &ws, void *workspace, size_t bytes, size_t w, size_
     return ATQ_OK;
 }
 
+// Look-back flags per scan array: enough tiles for the longest scan (at
+// most cap elements).
+uint32_t scan_flag_count(const Caps &c) { return c.cap / LB_TILE + 2; }
+
 // Detector buffers, carved from one arena.
 size_t carve_detector(atq_detector *d, char *base) {
     const Caps &c = d->caps;
@@ -125,6 +129,7 @@ size_t carve_detector(atq_detector *d, char *base) {
     d->peaks = carve_one<atq_peak_extents>(base, off, c.cap_cl);
     d->fitted = carve_one<atq_fitted_quad>(base, off, c.cap_cl);
     d->quads = carve_one<atq_quad_corners>(base, off, c.cap_cl);
+    d->scan_flags = carve_one<uint64_t>(base, off, 2 * (size_t)scan_flag_count(c));
     d->pair_keys = carve_one<uint64_t>(base, off, c.pair_slots);
     d->pair_cnt = carve_one<uint32_t>(base, off, c.pair_slots);
     d->counts = carve_one<atq_counts>(base, off, 1);
@@ -150,14 +155,19 @@ int run_pipeline(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t 
     const Caps c = Caps::for_pixels(w * h);  // this frame's own bounds
     Workspace &ws = d->ws;
     atq_counts *cnt = d->counts;
+    ScanCtx sc;
+    sc.flags = d->scan_flags;
+    sc.flag_count = scan_flag_count(d->caps);
+    sc.error = &cnt->overflow;
 
     auto mark = [&](int i) -> int {
         if (d->timing && hipEventRecord(d->ev[i], s) != hipSuccess) return ATQ_ERR_HIP;
         return ATQ_OK;
     };
-    // one fill: the pair table and the counts (with the key width set)
+    // one fill: the scan flags, the pair table and the counts (with the key
+    // width set)
     const RadixPlan plan = radix_plan(w * h, 2);
-    uint32_t *zero_begin = reinterpret_cast<uint32_t *>(d->pair_keys);
+    uint32_t *zero_begin = reinterpret_cast<uint32_t *>(d->scan_flags);
     const uint32_t zero_words = (uint32_t)((reinterpret_cast<char *>(cnt + 1) -
                                             reinterpret_cast<char *>(zero_begin)) / 4);
     const uint32_t lb_word = (uint32_t)(reinterpret_cast<uint32_t *>(&cnt->key_label_bits) - zero_begin);
@@ -167,7 +177,9 @@ int run_pipeline(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t 
                          d->params.min_white_black_diff));
     TRY(mark(1 + ATQ_STAGE_THRESHOLD));
     const uint8_t *grey = f > 1 ? d->gray : gray_in;
-    TRY(launch_ccl(s, d->thr, w, h, d->parent, d->labels, d->sizes));
+    // sizes only decide "at least min_component_size" here
+    TRY(launch_ccl(s, d->thr, w, h, d->parent, d->labels, d->sizes,
+                   (uint32_t)std::max(d->params.min_component_size, 1)));
     TRY(mark(1 + ATQ_STAGE_CCL));
     // Boundary records of the pairs that can pass ValidBlobFilter, in the
     // reference's order, keyed by the packed pair key min<<lb|max with the
@@ -178,7 +190,7 @@ int run_pipeline(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t 
                                         (uint32_t)plan.label_bits, table, cnt, ws.keys_alt,
                                         reinterpret_cast<atq_boundary_point *>(ws.vals_alt),
                                         ws.tiles, d->keys,
-                                        reinterpret_cast<atq_boundary_point *>(d->vals), ws.scan));
+                                        reinterpret_cast<atq_boundary_point *>(d->vals), sc));
     TRY(mark(1 + ATQ_STAGE_BOUNDARY));
     bool alt = false;
     TRY(radix_sort_packed(s, plan, d->keys, d->vals, ws.keys_alt, ws.vals_alt, &cnt->n_boundary,
@@ -188,10 +200,10 @@ int run_pipeline(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t 
     const uint64_t *K = alt ? ws.keys_alt : d->keys;
     const atq_boundary_point *V =
         reinterpret_cast<const atq_boundary_point *>(alt ? ws.vals_alt : d->vals);
-    TRY(launch_segments(s, K, 2, &cnt->n_boundary, cnt, c.cap, ws.seg_id, ws.seg_start, ws.scan));
+    TRY(launch_segments(s, K, 2, &cnt->n_boundary, cnt, c.cap, ws.seg_id, ws.seg_start, sc));
     TRY(launch_bounds_filter(s, V, ws.seg_start, cnt, c.cap, (uint32_t)w, (uint32_t)h, ws.valid,
-                             ws.seg_bounds, ws.seg_cluster, d->bounds, ws.scan));
-    TRY(launch_extents(s, d->bounds, cnt, c.cap_cl, d->cext, ws.scan));
+                             ws.seg_bounds, ws.seg_cluster, d->bounds, sc));
+    TRY(launch_extents(s, d->bounds, cnt, c.cap_cl, d->cext, sc));
     TRY(launch_assign(s, V, ws.seg_id, ws.seg_start, ws.seg_cluster, d->bounds, d->cext,
                       &cnt->n_boundary, c.cap, ws.cpts_tmp, d->cidx));
     TRY(mark(1 + ATQ_STAGE_CLUSTERS));
@@ -207,13 +219,13 @@ int run_pipeline(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t 
     TRY(mark(1 + ATQ_STAGE_FIT_LINES));
     TRY(launch_corner_segments(s, d->sparse, &cnt->n_cluster_points, c.cap, cnt, c.cap_cl,
                                max_cluster, d->peaks, d->corners, ws.seg_id, ws.angle_keys(),
-                               ws.scan));
+                               sc));
     TRY(mark(1 + ATQ_STAGE_CORNERS));
     TRY(launch_do_indexing(s, d->peaks, &cnt->n_segments, c.cap_cl, d->corners, d->lf,
                            &cnt->n_cluster_points, 0, d->cext, d->fitted, &cnt->oob_reads));
     TRY(mark(1 + ATQ_STAGE_INDEXING));
     TRY(launch_quads(s, d->fitted, &cnt->n_segments, c.cap_cl, ws.qtmp, ws.qflags, d->quads,
-                     &cnt->n_quads, ws.scan, d->params.min_tag_width));
+                     &cnt->n_quads, sc, d->params.min_tag_width));
     TRY(mark(1 + ATQ_STAGE_QUADS));
     return ATQ_OK;
 }
@@ -235,6 +247,8 @@ int atq_group_boundaries(void *stream, const atq_boundary_point *points, const u
     TRY(check_dims(width, height));
     Workspace ws;
     TRY(take_workspace(ws, workspace, workspace_bytes, width, height));
+    ScanCtx sc;
+    sc.scratch = ws.scan;
     hipStream_t s = S(stream);
     const uint32_t n = (uint32_t)(4 * width * height);
     // the element count of the sparse planes, as a device scalar
@@ -243,7 +257,7 @@ int atq_group_boundaries(void *stream, const atq_boundary_point *points, const u
     const uint32_t *pts = reinterpret_cast<const uint32_t *>(points);
     uint32_t *opts = reinterpret_cast<uint32_t *>(out_points);
     TRY(device_scan(s, n_dev, n, SparseFlag{pts}, SparseOut{pts, blob_labels, opts, out_keys},
-                    ws.scan, count_dev));
+                    sc, count_dev));
     const RadixPlan plan = radix_plan(width * height, 0);
     const uint32_t lb = (uint32_t)plan.label_bits;
     TRY(launch_pack_keys(s, out_keys, out_keys, count_dev, n, lb, false));
@@ -267,13 +281,15 @@ int atq_cluster_bounds_filter(void *stream, const atq_boundary_point *points, co
     TRY(check_dims(width, height));
     Workspace ws;
     TRY(take_workspace(ws, workspace, workspace_bytes, width, height));
+    ScanCtx sc;
+    sc.scratch = ws.scan;
     hipStream_t s = S(stream);
     const Caps c = Caps::for_pixels(width * height);
     TRY(launch_segments(s, keys, 0, n_points_dev, counts_dev, c.cap, ws.seg_id, ws.seg_start,
-                        ws.scan));
+                        sc));
     return launch_bounds_filter(s, points, ws.seg_start, counts_dev, c.cap, (uint32_t)width,
                                 (uint32_t)height, ws.valid, ws.seg_bounds, ws.seg_cluster,
-                                out_bounds, ws.scan);
+                                out_bounds, sc);
 }
 
 int atq_cluster_points_sort(void *stream, const atq_boundary_point *points,
@@ -288,7 +304,9 @@ int atq_cluster_points_sort(void *stream, const atq_boundary_point *points,
     Workspace ws;
     const Caps c = caps_from_workspace(workspace_bytes);
     ws.carve(static_cast<char *>(workspace), c);
-    TRY(launch_extents(s, bounds, counts_dev, c.cap_cl, out_extents, ws.scan));
+    ScanCtx sc;
+    sc.scratch = ws.scan;
+    TRY(launch_extents(s, bounds, counts_dev, c.cap_cl, out_extents, sc));
     TRY(launch_assign_search(s, points, n_points_dev, c.cap, bounds, counts_dev, out_extents,
                              ws.cpts_tmp, out_cluster_index));
     uint64_t *ak = ws.angle_keys();
@@ -334,10 +352,12 @@ int atq_corner_segments(void *stream, const atq_corner *found_corners, atq_count
     Workspace ws;
     const Caps c = caps_from_workspace(workspace_bytes);
     ws.carve(static_cast<char *>(workspace), c);
+    ScanCtx sc;
+    sc.scratch = ws.scan;
     return launch_corner_segments(s, found_corners, &counts_dev->n_cluster_points, c.cap, counts_dev,
                                   c.cap_cl, std::min<uint32_t>(c.cap, c.npx / 2 + 32), segments,
                                   compacted, ws.seg_id,
-                                  ws.angle_keys(), ws.scan);
+                                  ws.angle_keys(), sc);
 }
 
 int atq_do_indexing(void *stream, const atq_peak_extents *extents, size_t extents_count,
@@ -366,9 +386,11 @@ int atq_quads_from_fits(void *stream, const atq_fitted_quad *fitted, size_t n_fi
     const Caps c = caps_from_workspace(workspace_bytes);
     if (n_fitted > c.cap_cl) return ATQ_ERR_CAPACITY;
     ws.carve(static_cast<char *>(workspace), c);
+    ScanCtx sc;
+    sc.scratch = ws.scan;
     uint32_t *n_dev = &ws.counts->reserved[1];
     TRY(launch_set_u32(s, n_dev, (uint32_t)n_fitted));
-    return launch_quads(s, fitted, n_dev, c.cap_cl, ws.qtmp, ws.qflags, out, n_quads_dev, ws.scan,
+    return launch_quads(s, fitted, n_dev, c.cap_cl, ws.qtmp, ws.qflags, out, n_quads_dev, sc,
                         8);
 }
 

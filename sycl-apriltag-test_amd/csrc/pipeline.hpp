@@ -2,6 +2,7 @@
 #pragma once
 #include "clusters.hpp"
 #include "kernels.hpp"
+#include "scan.hpp"
 #include "segsort.hpp"
 
 namespace atq {
@@ -69,7 +70,8 @@ struct atq_detector {
     atq_peak_extents *peaks;
     atq_fitted_quad *fitted;
     atq_quad_corners *quads;
-    uint64_t *pair_keys;  // pair table, zeroed with counts before each frame
+    uint64_t *scan_flags;  // two look-back flag arrays, zeroed before each frame
+    uint64_t *pair_keys;   // pair table, zeroed with counts before each frame
     uint32_t *pair_cnt;
     atq_counts *counts;
     atq::Workspace ws;

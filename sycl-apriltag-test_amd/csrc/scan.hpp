@@ -114,15 +114,124 @@ k_scan_apply(const uint32_t *count, uint32_t count_cap, F f, const uint32_t *par
     }
 }
 
-// Launches the three scan kernels over min(*count, count_cap) elements
-// (count == nullptr: exactly count_cap). `scratch` holds SCAN_GROUPS + 1
-// u32; *total receives the sum (device memory).
+// Single-pass form (decoupled look-back): work-group p scans its tile of
+// LB_TILE elements, publishes the tile's sum, then walks back over the
+// flags of the tiles before it (64 at a time, one per lane) until it meets
+// an inclusive prefix, and publishes its own. Flag words: status in bits
+// 62-63 (1: tile sum, 2: inclusive prefix, 0: not yet), value in bits 0-31.
+// Work-groups are dispatched in index order, so every tile waited on is
+// running or done; the wait is still bounded (LB_SPIN_LIMIT reads), after
+// which *error gets OVF_SCAN and the scan's results are void.
+// Each call uses one of two flag arrays and clears the other for the next
+// call (both must be zero before the first call of a sequence).
+constexpr int LB_ITEMS = 8;
+constexpr uint32_t LB_TILE = SCAN_THREADS * LB_ITEMS;
+constexpr uint64_t LB_SUM = 1ull << 62, LB_PREFIX = 2ull << 62;
+constexpr uint32_t LB_SPIN_LIMIT = 1u << 22;
+
+ATQ_DEV uint64_t lb_load(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+ATQ_DEV void lb_store(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <typename F, typename O>
-int device_scan(hipStream_t s, const uint32_t *count, uint32_t count_cap, F f, O o,
-                uint32_t *scratch, uint32_t *total) {
-    k_scan_partials<F><<<SCAN_GROUPS, SCAN_THREADS, 0, s>>>(count, count_cap, f, scratch);
-    k_scan_top<0><<<1, 1024, 0, s>>>(scratch, SCAN_GROUPS, total);
-    k_scan_apply<F, O><<<SCAN_GROUPS, SCAN_THREADS, 0, s>>>(count, count_cap, f, scratch, o);
+__global__ void __launch_bounds__(SCAN_THREADS)
+k_scan_lookback(const uint32_t *count, uint32_t count_cap, F f, O o, uint64_t *flags,
+                uint64_t *next_flags, uint32_t n_flags, uint32_t *total, uint32_t *error) {
+    __shared__ uint32_t lds4[4];
+    __shared__ uint32_t s_prefix;
+    for (uint32_t i = blockIdx.x * SCAN_THREADS + threadIdx.x; i < n_flags;
+         i += gridDim.x * SCAN_THREADS)
+        next_flags[i] = 0;
+    const uint32_t n = count ? min(*count, count_cap) : count_cap;
+    const uint32_t p = blockIdx.x, b = p * LB_TILE;
+    if (b >= n) {
+        if (p == 0 && threadIdx.x == 0) *total = 0;
+        return;
+    }
+    const uint32_t i0 = b + threadIdx.x * LB_ITEMS;
+    uint32_t v[LB_ITEMS], sum = 0;
+#pragma unroll
+    for (int k = 0; k < LB_ITEMS; k++) {
+        v[k] = i0 + k < n ? f(i0 + k) : 0u;
+        sum += v[k];
+    }
+    uint32_t tile_sum;
+    const uint32_t ex = block_exclusive_scan<uint32_t>(sum, lds4, &tile_sum);
+    if (threadIdx.x < 64) {
+        const uint32_t lane = threadIdx.x;
+        uint32_t excl = 0;
+        if (p == 0) {
+            if (lane == 0) lb_store(&flags[0], LB_PREFIX | tile_sum);
+        } else {
+            if (lane == 0) lb_store(&flags[p], LB_SUM | tile_sum);
+            int64_t q = (int64_t)p - 1;
+            uint32_t spins = 0;
+            while (true) {
+                const int64_t idx = q - (int64_t)lane;
+                const uint64_t fl = idx >= 0 ? lb_load(&flags[idx]) : LB_PREFIX;
+                const uint64_t st = fl >> 62;
+                const uint64_t pre = __ballot(st == 2), zero = __ballot(st == 0);
+                const int first_pre = pre ? __ffsll((long long)pre) - 1 : 64;
+                const uint64_t before = first_pre == 64 ? ~0ull : ((1ull << first_pre) - 1);
+                if ((zero & before) && ++spins < LB_SPIN_LIMIT) continue;
+                uint32_t c = (int)lane <= first_pre ? (uint32_t)fl : 0u;
+#pragma unroll
+                for (int o2 = 32; o2 > 0; o2 >>= 1) c += __shfl_xor(c, o2, 64);
+                excl += c;
+                if (first_pre < 64 || spins >= LB_SPIN_LIMIT) break;
+                q -= 64;
+            }
+            if (spins >= LB_SPIN_LIMIT && lane == 0) atomicOr(error, OVF_SCAN);
+            if (lane == 0) lb_store(&flags[p], LB_PREFIX | (uint32_t)(excl + tile_sum));
+        }
+        if (lane == 0) s_prefix = excl;
+    }
+    __syncthreads();
+    const uint32_t prefix = s_prefix;
+    if (threadIdx.x == 0 && b + LB_TILE >= n) *total = prefix + tile_sum;
+    uint32_t run = prefix + ex;
+#pragma unroll
+    for (int k = 0; k < LB_ITEMS; k++) {
+        if (i0 + k < n) o(i0 + k, run, v[k]);
+        run += v[k];
+    }
+}
+
+// Where a device scan keeps its state. flags null: the three-kernel form
+// (scratch: SCAN_GROUPS + 1 u32). Otherwise the look-back form: flags holds
+// two arrays of flag_count u64 (zero before the first scan), `calls`
+// counts the scans launched so far (it picks the array), error receives
+// the look-back failure bit.
+struct ScanCtx {
+    uint32_t *scratch = nullptr;
+    uint64_t *flags = nullptr;
+    uint32_t flag_count = 0;
+    uint32_t calls = 0;
+    uint32_t *error = nullptr;
+};
+
+// Scans min(*count, count_cap) elements (count == nullptr: exactly
+// count_cap): o(i, exclusive prefix, value) for every element; *total
+// receives the sum (device memory).
+template <typename F, typename O>
+int device_scan(hipStream_t s, const uint32_t *count, uint32_t count_cap, F f, O o, ScanCtx &sc,
+                uint32_t *total) {
+    if (sc.flags) {
+        const uint32_t tiles = count_cap / LB_TILE + 1;
+        if (tiles > sc.flag_count) return ATQ_ERR_CAPACITY;
+        uint64_t *cur = sc.flags + (size_t)(sc.calls & 1) * sc.flag_count;
+        uint64_t *next = sc.flags + (size_t)((sc.calls + 1) & 1) * sc.flag_count;
+        sc.calls++;
+        k_scan_lookback<F, O><<<tiles, SCAN_THREADS, 0, s>>>(count, count_cap, f, o, cur, next,
+                                                             sc.flag_count, total, sc.error);
+        return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
+    }
+    k_scan_partials<F><<<SCAN_GROUPS, SCAN_THREADS, 0, s>>>(count, count_cap, f, sc.scratch);
+    k_scan_top<0><<<1, 1024, 0, s>>>(sc.scratch, SCAN_GROUPS, total);
+    k_scan_apply<F, O><<<SCAN_GROUPS, SCAN_THREADS, 0, s>>>(count, count_cap, f, sc.scratch, o);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
