@@ -63,8 +63,8 @@ size_t Workspace::carve(char *base, const Caps &c) {
     qtmp = carve_one<atq_quad_corners>(base, off, c.cap_cl);
     qflags = carve_one<uint32_t>(base, off, c.cap_cl);
     scan = carve_one<uint32_t>(base, off, SCAN_GROUPS + 1);
-    hist = carve_one<uint32_t>(base, off, 2048 * RS_GROUPS);
-    digit_base = carve_one<uint32_t>(base, off, 2048);
+    hist = carve_one<uint32_t>(base, off, (size_t)RS_MAX_DIGITS * RS_GROUPS);
+    digit_totals = carve_one<uint32_t>(base, off, RS_MAX_DIGITS);
     counts = carve_one<atq_counts>(base, off, 1);
     return (off + 255) & ~(size_t)255;
 }
@@ -194,7 +194,7 @@ int run_pipeline(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t 
     TRY(mark(1 + ATQ_STAGE_BOUNDARY));
     bool alt = false;
     TRY(radix_sort_packed(s, plan, d->keys, d->vals, ws.keys_alt, ws.vals_alt, &cnt->n_boundary,
-                          c.cap, ws.hist, ws.digit_base, &alt));
+                          c.cap, ws.hist, ws.digit_totals, &alt));
     TRY(mark(1 + ATQ_STAGE_KEY_SORT));
     d->sorted_in_alt = alt;
     const uint64_t *K = alt ? ws.keys_alt : d->keys;
@@ -263,7 +263,7 @@ int atq_group_boundaries(void *stream, const atq_boundary_point *points, const u
     TRY(launch_pack_keys(s, out_keys, out_keys, count_dev, n, lb, false));
     bool alt = false;
     TRY(radix_sort_packed(s, plan, out_keys, opts, ws.keys_alt, ws.vals_alt, count_dev, n, ws.hist,
-                          ws.digit_base, &alt));
+                          ws.digit_totals, &alt));
     // unpack into the caller's buffer (from wherever the last pass wrote)
     TRY(launch_pack_keys(s, alt ? ws.keys_alt : out_keys, out_keys, count_dev, n, lb, true));
     if (alt)

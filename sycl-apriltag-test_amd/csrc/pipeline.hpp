@@ -37,7 +37,7 @@ struct Workspace {
     atq_quad_corners *qtmp;
     uint32_t *qflags;
     uint32_t *scan;
-    uint32_t *hist, *digit_base;
+    uint32_t *hist, *digit_totals;
     atq_counts *counts;
 
     // Two key buffers of cap entries each for the angle and corner sorts;

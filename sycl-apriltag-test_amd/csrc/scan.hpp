@@ -136,6 +136,38 @@ ATQ_DEV void lb_store(uint64_t *p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Called by the 64 lanes of one wave of work-group p: publishes p's sum,
+// walks back to an inclusive prefix, publishes p's own and returns the sum
+// of all tiles before p (every lane gets it).
+ATQ_DEV uint32_t lb_lookback(uint64_t *flags, uint32_t p, uint32_t tile_sum, uint32_t *error) {
+    const uint32_t lane = threadIdx.x & 63;
+    if (p == 0) {
+        if (lane == 0) lb_store(&flags[0], LB_PREFIX | tile_sum);
+        return 0;
+    }
+    if (lane == 0) lb_store(&flags[p], LB_SUM | tile_sum);
+    uint32_t excl = 0, spins = 0;
+    int64_t q = (int64_t)p - 1;
+    while (true) {
+        const int64_t idx = q - (int64_t)lane;
+        const uint64_t fl = idx >= 0 ? lb_load(&flags[idx]) : LB_PREFIX;
+        const uint64_t st = fl >> 62;
+        const uint64_t pre = __ballot(st == 2), zero = __ballot(st == 0);
+        const int first_pre = pre ? __ffsll((long long)pre) - 1 : 64;
+        const uint64_t before = first_pre == 64 ? ~0ull : ((1ull << first_pre) - 1);
+        if ((zero & before) && ++spins < LB_SPIN_LIMIT) continue;
+        uint32_t c = (int)lane <= first_pre ? (uint32_t)fl : 0u;
+#pragma unroll
+        for (int o2 = 32; o2 > 0; o2 >>= 1) c += __shfl_xor(c, o2, 64);
+        excl += c;
+        if (first_pre < 64 || spins >= LB_SPIN_LIMIT) break;
+        q -= 64;
+    }
+    if (spins >= LB_SPIN_LIMIT && lane == 0 && error) atomicOr(error, OVF_SCAN);
+    if (lane == 0) lb_store(&flags[p], LB_PREFIX | (uint32_t)(excl + tile_sum));
+    return excl;
+}
+
 template <typename F, typename O>
 __global__ void __launch_bounds__(SCAN_THREADS)
 k_scan_lookback(const uint32_t *count, uint32_t count_cap, F f, O o, uint64_t *flags,
@@ -161,33 +193,8 @@ k_scan_lookback(const uint32_t *count, uint32_t count_cap, F f, O o, uint64_t *f
     uint32_t tile_sum;
     const uint32_t ex = block_exclusive_scan<uint32_t>(sum, lds4, &tile_sum);
     if (threadIdx.x < 64) {
-        const uint32_t lane = threadIdx.x;
-        uint32_t excl = 0;
-        if (p == 0) {
-            if (lane == 0) lb_store(&flags[0], LB_PREFIX | tile_sum);
-        } else {
-            if (lane == 0) lb_store(&flags[p], LB_SUM | tile_sum);
-            int64_t q = (int64_t)p - 1;
-            uint32_t spins = 0;
-            while (true) {
-                const int64_t idx = q - (int64_t)lane;
-                const uint64_t fl = idx >= 0 ? lb_load(&flags[idx]) : LB_PREFIX;
-                const uint64_t st = fl >> 62;
-                const uint64_t pre = __ballot(st == 2), zero = __ballot(st == 0);
-                const int first_pre = pre ? __ffsll((long long)pre) - 1 : 64;
-                const uint64_t before = first_pre == 64 ? ~0ull : ((1ull << first_pre) - 1);
-                if ((zero & before) && ++spins < LB_SPIN_LIMIT) continue;
-                uint32_t c = (int)lane <= first_pre ? (uint32_t)fl : 0u;
-#pragma unroll
-                for (int o2 = 32; o2 > 0; o2 >>= 1) c += __shfl_xor(c, o2, 64);
-                excl += c;
-                if (first_pre < 64 || spins >= LB_SPIN_LIMIT) break;
-                q -= 64;
-            }
-            if (spins >= LB_SPIN_LIMIT && lane == 0) atomicOr(error, OVF_SCAN);
-            if (lane == 0) lb_store(&flags[p], LB_PREFIX | (uint32_t)(excl + tile_sum));
-        }
-        if (lane == 0) s_prefix = excl;
+        const uint32_t excl = lb_lookback(flags, p, tile_sum, error);
+        if (threadIdx.x == 0) s_prefix = excl;
     }
     __syncthreads();
     const uint32_t prefix = s_prefix;

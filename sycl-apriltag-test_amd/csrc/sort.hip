@@ -7,16 +7,18 @@
 // 2*lb bits are sorted: 4 passes of 10-11 bit digits at 1080p, 5 at 12 MP
 // (the reference's key has 64).
 //
-// One pass = four kernels over a fixed grid of G work-groups, each owning a
-// contiguous chunk (the record count is read on the device):
-//   k_rs_hist       per-chunk digit histogram (LDS atomics)
-//   k_rs_scan_rows  one work-group per digit: exclusive scan of its G counts
-//   k_rs_scan_base  one work-group: exclusive scan of the digit totals
-//   k_rs_scatter    rounds of 2048 records (8 per lane): each wave ranks its
-//                   512 records digit by digit with ballots (a 64-lane
-//                   multisplit), the 4 waves' counts are scanned in LDS, and
-//                   every record is written at
-//                   digit base + chunk offset + wave offset + rank.
+// The records are cut into G contiguous chunks of a multiple of 2048
+// records, G <= RS_GROUPS (both derived from the record count on the
+// device). One pass = three kernels:
+//   k_rs_hist     per-chunk digit histogram (LDS atomics) -> hist[digit][g]
+//   k_rs_scan     one work-group per digit: exclusive scan of its G counts
+//                 in place, and the digit's total
+//   k_rs_scatter  each work-group first turns the digit totals into digit
+//                 bases (a 2048-entry work-group scan), then in rounds of
+//                 2048 records (8 per lane) each wave ranks its 512 records
+//                 digit by digit with ballots (a 64-lane multisplit), the 4
+//                 waves' counts are scanned in LDS, and every record is
+//                 written at digit base + chunk offset + wave offset + rank.
 // Records keep their input order within a digit, so the sort is stable.
 #include "sort.hpp"
 
@@ -29,69 +31,53 @@ constexpr int RS_ROUND = RS_THREADS * RS_ITEMS;   // 2048
 constexpr int RS_WAVE_SPAN = 64 * RS_ITEMS;       // 512 records per wave per round
 }  // namespace
 
+// Chunk size and count for n records.
+ATQ_DEV void rs_chunks(uint32_t n, uint32_t *chunk, uint32_t *groups) {
+    uint32_t c = max((uint32_t)RS_ROUND, (n + RS_GROUPS - 1) / RS_GROUPS);
+    c = (c + RS_ROUND - 1) / RS_ROUND * RS_ROUND;
+    *chunk = c;
+    *groups = (n + c - 1) / c;
+}
+
 template <int BITS>
 __global__ void __launch_bounds__(RS_THREADS)
 k_rs_hist(const uint64_t *__restrict__ keys, const uint32_t *count, uint32_t cap, uint32_t shift,
           uint32_t *__restrict__ hist) {
     constexpr int B = 1 << BITS;
     __shared__ uint32_t h[B];
+    const uint32_t n = min(*count, cap);
+    uint32_t chunk, G;
+    rs_chunks(n, &chunk, &G);
+    if (blockIdx.x >= G) return;
     for (int i = threadIdx.x; i < B; i += RS_THREADS) h[i] = 0;
     __syncthreads();
-    uint32_t n = min(*count, cap);
-    uint32_t b, e;
-    chunk_range(n, blockIdx.x, gridDim.x, &b, &e);
+    const uint32_t b = blockIdx.x * chunk, e = min(n, b + chunk);
     for (uint32_t i = b + threadIdx.x; i < e; i += RS_THREADS)
         atomicAdd(&h[(uint32_t)(keys[i] >> shift) & (B - 1)], 1u);
     __syncthreads();
-    for (int d = threadIdx.x; d < B; d += RS_THREADS) hist[(size_t)d * gridDim.x + blockIdx.x] = h[d];
+    for (int d = threadIdx.x; d < B; d += RS_THREADS) hist[(size_t)d * G + blockIdx.x] = h[d];
 }
 
-// One work-group per digit row: exclusive scan over the G chunks in place.
+// One work-group per digit d: exclusive scan of hist[d][0..G) in place;
+// the digit's total to totals[d].
 __global__ void __launch_bounds__(RS_THREADS)
-k_rs_scan_rows(uint32_t *__restrict__ hist, uint32_t groups, uint32_t *__restrict__ row_total) {
+k_rs_scan(uint32_t *__restrict__ hist, const uint32_t *count, uint32_t cap,
+          uint32_t *__restrict__ totals) {
     __shared__ uint32_t lds4[4];
-    uint32_t *row = hist + (size_t)blockIdx.x * groups;
+    const uint32_t n = min(*count, cap);
+    uint32_t chunk, G;
+    rs_chunks(n, &chunk, &G);
+    uint32_t *row = hist + (size_t)blockIdx.x * G;
     uint32_t carry = 0;
-    for (uint32_t base = 0; base < groups; base += RS_THREADS) {
-        uint32_t i = base + threadIdx.x;
-        uint32_t v = i < groups ? row[i] : 0u;
+    for (uint32_t base = 0; base < G; base += RS_THREADS) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < G ? row[i] : 0u;
         uint32_t tot;
-        uint32_t ex = block_exclusive_scan<uint32_t>(v, lds4, &tot);
-        if (i < groups) row[i] = carry + ex;
+        const uint32_t ex = block_exclusive_scan<uint32_t>(v, lds4, &tot);
+        if (i < G) row[i] = carry + ex;
         carry += tot;
     }
-    if (threadIdx.x == 0) row_total[blockIdx.x] = carry;
-}
-
-// One work-group: exclusive scan of the digit totals in place.
-__global__ void __launch_bounds__(1024) k_rs_scan_base(uint32_t *row_total, uint32_t nbuckets) {
-    __shared__ uint32_t wsum[16];
-    __shared__ uint32_t carry_s;
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    if (t == 0) carry_s = 0;
-    __syncthreads();
-    for (uint32_t base = 0; base < nbuckets; base += 1024) {
-        uint32_t i = base + t;
-        uint32_t v = i < nbuckets ? row_total[i] : 0u;
-        uint32_t x = v;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            uint32_t y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
-        }
-        if (lane == 63) wsum[wave] = x;
-        __syncthreads();
-        uint32_t before = 0, sum = 0;
-        for (int w = 0; w < 16; w++) {
-            if (w < wave) before += wsum[w];
-            sum += wsum[w];
-        }
-        uint32_t carry = carry_s;
-        if (i < nbuckets) row_total[i] = carry + before + x - v;
-        __syncthreads();
-        if (t == 0) carry_s = carry + sum;
-        __syncthreads();
-    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
 template <int BITS>
@@ -99,16 +85,34 @@ __global__ void __launch_bounds__(RS_THREADS)
 k_rs_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
              uint64_t *__restrict__ kout, uint32_t *__restrict__ vout, const uint32_t *count,
              uint32_t cap, uint32_t shift, const uint32_t *__restrict__ hist,
-             const uint32_t *__restrict__ digit_base) {
+             const uint32_t *__restrict__ totals) {
     constexpr int B = 1 << BITS;
+    constexpr int PER = B / RS_THREADS;  // digits per thread in the base scan
     __shared__ uint32_t wc[4][B];      // per-wave digit counts, then wave offsets
     __shared__ uint32_t base_off[B];   // running output offset of each digit
+    __shared__ uint32_t lds4[4];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    for (int d = t; d < B; d += RS_THREADS)
-        base_off[d] = digit_base[d] + hist[(size_t)d * gridDim.x + blockIdx.x];
-    uint32_t n = min(*count, cap);
-    uint32_t b, e;
-    chunk_range(n, blockIdx.x, gridDim.x, &b, &e);
+    const uint32_t n = min(*count, cap);
+    uint32_t chunk, G;
+    rs_chunks(n, &chunk, &G);
+    if (blockIdx.x >= G) return;
+    {   // digit bases: exclusive scan of the totals, PER consecutive digits per thread
+        uint32_t tv[PER], sum = 0;
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            tv[k] = totals[t * PER + k];
+            sum += tv[k];
+        }
+        uint32_t all;
+        uint32_t run = block_exclusive_scan<uint32_t>(sum, lds4, &all);
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int d = t * PER + k;
+            base_off[d] = run + hist[(size_t)d * G + blockIdx.x];
+            run += tv[k];
+        }
+    }
+    const uint32_t b = blockIdx.x * chunk, e = min(n, b + chunk);
     const uint64_t lt = lanemask_lt();
     for (uint32_t r = b; r < e; r += RS_ROUND) {
         for (int d = t; d < B; d += RS_THREADS) {
@@ -198,33 +202,31 @@ RadixPlan radix_plan(size_t npx, int extra_bits) {
     p.passes = (total + 10) / 11;
     const int per = (total + p.passes - 1) / p.passes;
     p.digit_bits = per <= 8 ? 8 : (per <= 10 ? 10 : 11);
-    size_t groups = npx / 4096;
-    p.groups = (uint32_t)(groups < 32 ? 32 : (groups > RS_GROUPS ? RS_GROUPS : groups));
     return p;
 }
 
 template <int BITS>
-static void rs_pass(hipStream_t s, const RadixPlan &p, uint32_t shift, const uint64_t *ki,
-                    const uint32_t *vi, uint64_t *ko, uint32_t *vo, const uint32_t *count,
-                    uint32_t cap, uint32_t *hist, uint32_t *digit_base) {
-    const unsigned G = p.groups;
-    k_rs_hist<BITS><<<G, RS_THREADS, 0, s>>>(ki, count, cap, shift, hist);
-    k_rs_scan_rows<<<1u << BITS, RS_THREADS, 0, s>>>(hist, G, digit_base);
-    k_rs_scan_base<<<1, 1024, 0, s>>>(digit_base, 1u << BITS);
-    k_rs_scatter<BITS><<<G, RS_THREADS, 0, s>>>(ki, vi, ko, vo, count, cap, shift, hist, digit_base);
+static void rs_pass(hipStream_t s, uint32_t shift, const uint64_t *ki, const uint32_t *vi,
+                    uint64_t *ko, uint32_t *vo, const uint32_t *count, uint32_t cap,
+                    uint32_t *hist, uint32_t *totals) {
+    constexpr uint32_t B = 1u << BITS;
+    k_rs_hist<BITS><<<RS_GROUPS, RS_THREADS, 0, s>>>(ki, count, cap, shift, hist);
+    k_rs_scan<<<B, RS_THREADS, 0, s>>>(hist, count, cap, totals);
+    k_rs_scatter<BITS><<<RS_GROUPS, RS_THREADS, 0, s>>>(ki, vi, ko, vo, count, cap, shift, hist,
+                                                       totals);
 }
 
 int radix_sort_packed(hipStream_t s, const RadixPlan &p, uint64_t *keys, uint32_t *vals,
                       uint64_t *keys_alt, uint32_t *vals_alt, const uint32_t *count, uint32_t cap,
-                      uint32_t *hist, uint32_t *digit_base, bool *result_in_alt) {
+                      uint32_t *hist, uint32_t *totals, bool *result_in_alt) {
     uint64_t *ki = keys, *ko = keys_alt;
     uint32_t *vi = vals, *vo = vals_alt;
     for (int pass = 0; pass < p.passes; pass++) {
         const uint32_t shift = (uint32_t)(pass * p.digit_bits);
         switch (p.digit_bits) {
-        case 8: rs_pass<8>(s, p, shift, ki, vi, ko, vo, count, cap, hist, digit_base); break;
-        case 10: rs_pass<10>(s, p, shift, ki, vi, ko, vo, count, cap, hist, digit_base); break;
-        default: rs_pass<11>(s, p, shift, ki, vi, ko, vo, count, cap, hist, digit_base); break;
+        case 8: rs_pass<8>(s, shift, ki, vi, ko, vo, count, cap, hist, totals); break;
+        case 10: rs_pass<10>(s, shift, ki, vi, ko, vo, count, cap, hist, totals); break;
+        default: rs_pass<11>(s, shift, ki, vi, ko, vo, count, cap, hist, totals); break;
         }
         uint64_t *tk = ki; ki = ko; ko = tk;
         uint32_t *tv = vi; vi = vo; vo = tv;

@@ -4,24 +4,24 @@
 
 namespace atq {
 
-constexpr unsigned RS_GROUPS = 512;
+constexpr unsigned RS_GROUPS = 512;      // most chunks per pass
+constexpr unsigned RS_MAX_DIGITS = 2048; // 11-bit digits
 
 // How a frame of npx pixels sorts its packed pair keys.
 struct RadixPlan {
     int label_bits;   // lb: labels are below 2^lb
     int passes;       // over the 2*lb (+ extra) packed key bits
     int digit_bits;   // 8, 10 or 11
-    uint32_t groups;  // work-groups (contiguous chunks) per pass
 };
 // Keys of 2*lb + extra_bits bits (labels below npx).
 RadixPlan radix_plan(size_t npx, int extra_bits);
 
 // Sorts (keys, vals) by the low passes*digit_bits key bits, stably. `hist`
-// holds 2^11 * RS_GROUPS u32, `digit_base` 2^11 u32. On return the sorted
-// data are in the alt buffers when *result_in_alt is true.
+// holds RS_MAX_DIGITS * RS_GROUPS u32, `totals` RS_MAX_DIGITS u32. On
+// return the sorted data are in the alt buffers when *result_in_alt is true.
 int radix_sort_packed(hipStream_t s, const RadixPlan &p, uint64_t *keys, uint32_t *vals,
                       uint64_t *keys_alt, uint32_t *vals_alt, const uint32_t *count, uint32_t cap,
-                      uint32_t *hist, uint32_t *digit_base, bool *result_in_alt);
+                      uint32_t *hist, uint32_t *totals, bool *result_in_alt);
 
 // min<<32|max <-> min<<lb|max over count keys (out may alias in).
 int launch_pack_keys(hipStream_t s, const uint64_t *in, uint64_t *out, const uint32_t *count,
