@@ -43,9 +43,11 @@ int launch_run_extents(hipStream_t s, const uint32_t *cidx, const uint32_t *n_de
                        atq_cluster_extents *ext, uint32_t *n_runs_dev, ScanCtx &sc);
 
 // fitquad.hip
+// fit_lines (src/fit_quad.cpp:153-375, per point). errs (cap
+// floats, may be null) holds the window errors between two kernels.
 int launch_fit_lines(hipStream_t s, const atq_line_fit_point *lf, const uint32_t *cidx,
                      const atq_cluster_extents *ext, const uint32_t *n_points_dev, uint32_t cap,
-                     atq_corner *sparse, uint32_t *ccount);
+                     atq_corner *sparse, uint32_t *ccount, float *errs = nullptr);
 int launch_set_u32(hipStream_t s, uint32_t *p, uint32_t v);
 // Zeroes p[0 .. min(*n_dev, cap)) (n_dev null: cap words), except that
 // p[set_idx] gets set_val.

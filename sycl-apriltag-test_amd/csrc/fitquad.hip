@@ -152,10 +152,29 @@ __global__ void k_zero_u32(uint32_t *p, const uint32_t *n_dev, uint32_t cap, uin
 }
 
 // One thread per cluster point: the cyclic window / filter / maximum.
+// Error of the window of 2k+1 points centred on every cluster point
+// (k = min(20, n/12); 0 when k < 2).
+__global__ void __launch_bounds__(256)
+k_window_err(const atq_line_fit_point *__restrict__ lf, const uint32_t *__restrict__ cidx,
+             const atq_cluster_extents *__restrict__ ext, const uint32_t *n_dev, uint32_t cap,
+             float *__restrict__ errs) {
+    const uint32_t N = n_dev ? min(*n_dev, cap) : cap;
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N; x += gridDim.x * blockDim.x) {
+        const atq_cluster_extents e = ext[cidx[x]];
+        const uint32_t n = e.count, k = min(20u, n / 12u);
+        errs[x] = k >= 2 ? window_error(lf + e.start, n, k, x - e.start) : 0.0f;
+    }
+}
+
+// Per point: the filtered errors at positions j-1, j, j+1 (cyclic in the
+// cluster, 7-tap filter over 9 window errors) and a corner where the middle
+// one is a strict maximum. Window errors come from `errs` (k_window_err) or,
+// when it is null, are computed here.
 __global__ void __launch_bounds__(256)
 k_fit_lines(const atq_line_fit_point *__restrict__ lf, const uint32_t *__restrict__ cidx,
             const atq_cluster_extents *__restrict__ ext, const uint32_t *n_dev, uint32_t cap,
-            atq_corner *__restrict__ sparse, uint32_t *__restrict__ ccount) {
+            const float *__restrict__ errs, atq_corner *__restrict__ sparse,
+            uint32_t *__restrict__ ccount) {
     const uint32_t N = n_dev ? min(*n_dev, cap) : cap;
     for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N; x += gridDim.x * blockDim.x) {
         const uint32_t c = cidx[x];
@@ -167,7 +186,10 @@ k_fit_lines(const atq_line_fit_point *__restrict__ lf, const uint32_t *__restric
             const atq_line_fit_point *p = lf + s;
             float er[9];
 #pragma unroll
-            for (int d = 0; d < 9; d++) er[d] = window_error(p, n, k, (j + n + d - 4) % n);
+            for (int d = 0; d < 9; d++) {
+                const uint32_t pos = (j + n + d - 4) % n;
+                er[d] = errs ? errs[s + pos] : window_error(p, n, k, pos);
+            }
             float f[3];
 #pragma unroll
             for (int q = 0; q < 3; q++) {
@@ -438,9 +460,10 @@ int launch_zero_u32(hipStream_t s, uint32_t *p, const uint32_t *n_dev, uint32_t 
 
 int launch_fit_lines(hipStream_t s, const atq_line_fit_point *lf, const uint32_t *cidx,
                      const atq_cluster_extents *ext, const uint32_t *n_points_dev, uint32_t cap,
-                     atq_corner *sparse, uint32_t *ccount) {
+                     atq_corner *sparse, uint32_t *ccount, float *errs) {
     unsigned grid = min(4096u, atq_div_up(cap, 256) + 1);
-    k_fit_lines<<<grid, 256, 0, s>>>(lf, cidx, ext, n_points_dev, cap, sparse, ccount);
+    if (errs) k_window_err<<<grid, 256, 0, s>>>(lf, cidx, ext, n_points_dev, cap, errs);
+    k_fit_lines<<<grid, 256, 0, s>>>(lf, cidx, ext, n_points_dev, cap, errs, sparse, ccount);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 

@@ -215,7 +215,7 @@ int run_pipeline(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t 
                          ws.fit_sums));
     TRY(mark(1 + ATQ_STAGE_ANGLE_FIT));
     TRY(launch_fit_lines(s, d->lf, d->cidx, d->cext, &cnt->n_cluster_points, c.cap, d->sparse,
-                         nullptr));
+                         nullptr, reinterpret_cast<float *>(ws.valid)));
     TRY(mark(1 + ATQ_STAGE_FIT_LINES));
     TRY(launch_corner_segments(s, d->sparse, &cnt->n_cluster_points, c.cap, cnt, c.cap_cl,
                                max_cluster, d->peaks, d->corners, ws.seg_id, ws.angle_keys(),

@@ -2,18 +2,19 @@
 //
 // A segment is a {start, count} run of one array (clusters, peak segments).
 // Every segment is cut into chunks of SEG_CHUNK keys; one 1024-thread
-// work-group sorts a chunk with a bitonic network whose keys stay in
-// registers (4 per thread): exchanges within a thread are register moves,
-// within a wave lane shuffles, and only the exchanges across the 16 waves go
-// through LDS. A segment that fits one chunk is finished right there, from
-// the registers, by the caller's Finish functor. Longer segments are written
-// to keys0 and completed by merge passes in which every key finds its rank in
-// the partner run by binary search; runs double each pass, so a segment of n
-// keys needs seg_merge_passes(n) passes and ends in keys1 when that number is
-// odd, else in keys0. Keys must be distinct (callers put the element's
-// position in the low bits), so the result is a stable sort.
+// work-group sorts a chunk in LDS with a stable LSD radix sort over the
+// keys' high 32 bits (4 passes of 8 bits; each wave ranks its elements with
+// ballots, the 16 waves' digit counts are scanned in the work-group). The
+// low 32 bits ride along: callers put there a number that increases with the
+// element's position (its index), so sorting by the high word alone orders
+// the full 64-bit keys, which are distinct. A segment that fits one chunk is
+// finished right there by the caller's Finish functor. Longer segments are
+// written to keys0 and completed by merge passes in which every key finds
+// its rank in the partner run by binary search; runs double each pass, so a
+// segment of n keys needs seg_merge_passes(n) passes and ends in keys1 when
+// that number is odd, else in keys0.
 #pragma once
-#include "common.hpp"
+#include "scan.hpp"
 
 namespace atq {
 
@@ -74,58 +75,81 @@ ATQ_DEV void seg_block_scan6(const int64_t (&v)[6], int64_t (&incl)[6], int64_t 
     __syncthreads();
 }
 
-ATQ_DEV void seg_cas(uint64_t &a, uint64_t &b, bool ascending) {
-    const uint64_t lo = a < b ? a : b, hi = a < b ? b : a;
-    a = ascending ? lo : hi;
-    b = ascending ? hi : lo;
-}
+// LDS of a chunk sort: the keys' high and low words, and the digit
+// counters (256 digits x 16 waves, digit-major).
+constexpr int SEG_WAVES = SEG_NT / 64;
+constexpr uint32_t SEG_WAVE_SPAN = 64 * SEG_IPT;  // elements per wave
+struct SegLds {
+    uint32_t hi[SEG_CHUNK];
+    uint32_t lo[SEG_CHUNK];
+    uint32_t cnt[256 * SEG_WAVES];
+    uint32_t wsum[SEG_WAVES];
+};
+static_assert(256 * SEG_WAVES == 4 * SEG_NT, "four counters per thread");
 
-template <int J>
-ATQ_DEV void seg_regs_step(uint64_t (&v)[SEG_IPT], uint32_t k, uint32_t t) {
+// Stable LSD radix sort of L.hi/L.lo[0..m) by the high word. Wave w holds
+// elements w*SEG_WAVE_SPAN + i*64 + lane (i < SEG_IPT), so ranks taken in i
+// order, then by lane, follow the element order.
+ATQ_DEV void seg_lds_radix(SegLds &L, uint32_t m) {
+    constexpr int NW = SEG_WAVES;
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t wb = w * SEG_WAVE_SPAN;
+    const uint64_t lt = lanemask_lt();
+    for (int pass = 0; pass < 4; pass++) {
+        const uint32_t shift = 8 * pass;
+        uint32_t hi[SEG_IPT], lo[SEG_IPT], d[SEG_IPT], rank[SEG_IPT];
 #pragma unroll
-    for (int i = 0; i < SEG_IPT; i++) {
-        if (i & J) continue;
-        const uint32_t e = t * SEG_IPT + i;
-        seg_cas(v[i], v[i + J], (e & k) == 0);
-    }
-}
-
-ATQ_DEV uint32_t seg_lds_index(uint32_t e) { return e + (e >> 4); }
-
-// Bitonic sort (ascending) of the first P keys (P a power of two <= SEG_CHUNK)
-// held blocked in registers: thread t holds keys t*SEG_IPT .. t*SEG_IPT+3.
-// Keys past P must be SEG_PAD. All SEG_NT threads call it with the same P;
-// lds holds SEG_CHUNK + SEG_CHUNK/16 keys.
-ATQ_DEV void seg_block_sort(uint64_t (&v)[SEG_IPT], uint32_t P, uint64_t *lds) {
-    const uint32_t t = threadIdx.x, lane = t & 63, e0 = t * SEG_IPT;
-    for (uint32_t k = 2; k <= P; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            if (j < (uint32_t)SEG_IPT) {
-                if (j == 2) seg_regs_step<2>(v, k, t);
-                else seg_regs_step<1>(v, k, t);
-            } else if (j < (uint32_t)SEG_IPT * 64) {
-                // partner = same register of lane ^ m; the element keeps the
-                // minimum iff (it is the lower of the pair) == (ascending)
-                const uint32_t m = j / SEG_IPT;
-                const bool keep_min = ((lane & m) == 0) == ((e0 & k) == 0);
-#pragma unroll
-                for (int i = 0; i < SEG_IPT; i++) {
-                    const uint64_t o = __shfl_xor(v[i], (int)m, 64);
-                    v[i] = keep_min ? (o < v[i] ? o : v[i]) : (o > v[i] ? o : v[i]);
-                }
-            } else {
-#pragma unroll
-                for (int i = 0; i < SEG_IPT; i++) lds[seg_lds_index(e0 + i)] = v[i];
-                __syncthreads();
-                const bool keep_min = ((e0 & j) == 0) == ((e0 & k) == 0);
-#pragma unroll
-                for (int i = 0; i < SEG_IPT; i++) {
-                    const uint64_t o = lds[seg_lds_index((e0 + i) ^ j)];
-                    v[i] = keep_min ? (o < v[i] ? o : v[i]) : (o > v[i] ? o : v[i]);
-                }
-                __syncthreads();
-            }
+        for (int i = 0; i < SEG_IPT; i++) {
+            const uint32_t e = wb + i * 64 + lane;
+            hi[i] = e < m ? L.hi[e] : 0u;
+            lo[i] = e < m ? L.lo[e] : 0u;
+            d[i] = (hi[i] >> shift) & 255u;
         }
+#pragma unroll
+        for (int k = 0; k < 4; k++) L.cnt[4 * t + k] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < SEG_IPT; i++) {
+            if (wb + i * 64 >= m) break;  // the rest of this wave is empty
+            const bool valid = wb + i * 64 + lane < m;
+            uint64_t mk = __ballot(valid);
+#pragma unroll
+            for (int bit = 0; bit < 8; bit++) {
+                const uint64_t bb = __ballot((d[i] >> bit) & 1u);
+                mk &= ((d[i] >> bit) & 1u) ? bb : ~bb;
+            }
+            const int leader = valid ? __ffsll((unsigned long long)mk) - 1 : (int)lane;
+            uint32_t before = 0;
+            if (valid && (int)lane == leader) {
+                before = L.cnt[d[i] * NW + w];
+                L.cnt[d[i] * NW + w] = before + (uint32_t)__popcll(mk);
+            }
+            before = __shfl(before, leader, 64);
+            rank[i] = before + (uint32_t)__popcll(mk & lt);
+        }
+        __syncthreads();
+        uint32_t c4[4], sum = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            c4[k] = L.cnt[4 * t + k];
+            sum += c4[k];
+        }
+        uint32_t all;
+        uint32_t run = block_exclusive_scan<uint32_t, SEG_NT>(sum, L.wsum, &all);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            L.cnt[4 * t + k] = run;
+            run += c4[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < SEG_IPT; i++) {
+            if (wb + i * 64 + lane >= m) continue;
+            const uint32_t pos = L.cnt[d[i] * NW + w] + rank[i];
+            L.hi[pos] = hi[i];
+            L.lo[pos] = lo[i];
+        }
+        __syncthreads();
     }
 }
 
@@ -134,14 +158,14 @@ ATQ_DEV void seg_block_sort(uint64_t (&v)[SEG_IPT], uint32_t P, uint64_t *lds) {
 // Finish: void operator()(uint32_t seg, uint32_t start, uint32_t n,
 // const uint64_t (&v)[SEG_IPT], uint64_t *lds) const, called by every thread
 // of the work-group with a whole sorted segment of n <= SEG_CHUNK keys in
-// registers (key e0 + i in v[i], e0 = threadIdx.x * SEG_IPT; lds is free).
-// Keys move between global memory and registers through LDS, so that both
-// the global loads and the stores are coalesced.
+// registers (key e0 + i in v[i], e0 = threadIdx.x * SEG_IPT) and lds, 2048
+// u64 of scratch.
+// Keys move between global memory and LDS coalesced.
 template <typename KeyGen, typename Finish, typename Ext>
 __global__ void __launch_bounds__(SEG_NT)
 k_seg_chunks(KeyGen gen, Finish fin, const Ext *__restrict__ segs, const uint32_t *n_segs_dev,
              uint32_t cap_segs, uint32_t max_chunks, uint64_t *__restrict__ keys0) {
-    __shared__ uint64_t lds[SEG_CHUNK + SEG_CHUNK / 16];
+    __shared__ SegLds L;
     const uint32_t nseg = min(*n_segs_dev, cap_segs);
     const uint64_t items = (uint64_t)nseg * max_chunks;
     const uint32_t t = threadIdx.x, e0 = t * SEG_IPT;
@@ -151,25 +175,25 @@ k_seg_chunks(KeyGen gen, Finish fin, const Ext *__restrict__ segs, const uint32_
         const uint32_t b = ch * SEG_CHUNK;
         if (b >= n) continue;
         const uint32_t m = min(SEG_CHUNK, n - b);
-        uint32_t P = 1;
-        while (P < m) P <<= 1;
-        for (uint32_t j = t; j < m; j += SEG_NT) lds[seg_lds_index(j)] = gen(sg, start, b + j);
+        for (uint32_t j = t; j < m; j += SEG_NT) {
+            const uint64_t k = gen(sg, start, b + j);
+            L.hi[j] = (uint32_t)(k >> 32);
+            L.lo[j] = (uint32_t)k;
+        }
         __syncthreads();
-        uint64_t v[SEG_IPT];
-#pragma unroll
-        for (int i = 0; i < SEG_IPT; i++) v[i] = e0 + i < m ? lds[seg_lds_index(e0 + i)] : SEG_PAD;
-        __syncthreads();
-        seg_block_sort(v, P, lds);
+        seg_lds_radix(L, m);
         if (n <= SEG_CHUNK) {
-            fin(sg, start, n, v, lds);
+            uint64_t v[SEG_IPT];
+#pragma unroll
+            for (int i = 0; i < SEG_IPT; i++)
+                v[i] = e0 + i < m ? ((uint64_t)L.hi[e0 + i] << 32) | L.lo[e0 + i] : SEG_PAD;
+            __syncthreads();
+            fin(sg, start, n, v, reinterpret_cast<uint64_t *>(L.cnt));
             __syncthreads();
             continue;
         }
-#pragma unroll
-        for (int i = 0; i < SEG_IPT; i++)
-            if (e0 + i < m) lds[seg_lds_index(e0 + i)] = v[i];
-        __syncthreads();
-        for (uint32_t j = t; j < m; j += SEG_NT) keys0[start + b + j] = lds[seg_lds_index(j)];
+        for (uint32_t j = t; j < m; j += SEG_NT)
+            keys0[start + b + j] = ((uint64_t)L.hi[j] << 32) | L.lo[j];
         __syncthreads();
     }
 }

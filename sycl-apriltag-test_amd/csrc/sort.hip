@@ -52,8 +52,18 @@ k_rs_hist(const uint64_t *__restrict__ keys, const uint32_t *count, uint32_t cap
     for (int i = threadIdx.x; i < B; i += RS_THREADS) h[i] = 0;
     __syncthreads();
     const uint32_t b = blockIdx.x * chunk, e = min(n, b + chunk);
-    for (uint32_t i = b + threadIdx.x; i < e; i += RS_THREADS)
-        atomicAdd(&h[(uint32_t)(keys[i] >> shift) & (B - 1)], 1u);
+    // RS_ITEMS independent loads per lane before the LDS atomics
+    for (uint32_t r = b; r < e; r += RS_ROUND) {
+        uint32_t d[RS_ITEMS];
+#pragma unroll
+        for (int j = 0; j < RS_ITEMS; j++) {
+            const uint32_t i = r + j * RS_THREADS + threadIdx.x;
+            d[j] = i < e ? (uint32_t)(keys[i] >> shift) & (B - 1) : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (int j = 0; j < RS_ITEMS; j++)
+            if (d[j] != 0xFFFFFFFFu) atomicAdd(&h[d[j]], 1u);
+    }
     __syncthreads();
     for (int d = threadIdx.x; d < B; d += RS_THREADS) hist[(size_t)d * G + blockIdx.x] = h[d];
 }

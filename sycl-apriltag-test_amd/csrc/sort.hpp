@@ -4,7 +4,7 @@
 
 namespace atq {
 
-constexpr unsigned RS_GROUPS = 512;      // most chunks per pass
+constexpr unsigned RS_GROUPS = 1024;     // most chunks per pass
 constexpr unsigned RS_MAX_DIGITS = 2048; // 11-bit digits
 
 // How a frame of npx pixels sorts its packed pair keys.
