@@ -175,6 +175,9 @@ def main():
     ap.add_argument("--skip-12mp", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-f2", action="store_true", help="no decimate-2 leg")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="first copy this many MiB device to device (a counter reference "
+                         "for profiler runs)")
     ap.add_argument("--only-12mp", action="store_true",
                     help="only the 4096x3072 leg, --steps frames (for profiling)")
     ap.add_argument("--batch-rounds", type=int, default=20)
@@ -191,6 +194,14 @@ def main():
     from pyatq import synth
 
     atq.init()
+    if args.copy_mb:
+        nb = args.copy_mb << 20
+        src, dst = atq.DevBuf(nb), atq.DevBuf(nb)
+        atq.hip_check(atq._hip().hipMemcpy(atq.C.c_void_p(dst.ptr), atq.C.c_void_p(src.ptr), nb, 3),
+                      "hipMemcpy D2D")
+        log(f"copied {args.copy_mb} MiB device to device")
+        src.free()
+        dst.free()
     if args.only_12mp:
         print(json.dumps({"stress_12mp": bench_12mp(atq, synth, atq.Stream(), args.steps)}))
         return

@@ -10,9 +10,9 @@
 // The records are cut into G contiguous chunks of a multiple of 2048
 // records, G <= RS_GROUPS (both derived from the record count on the
 // device). One pass = three kernels:
-//   k_rs_hist     per-chunk digit histogram (LDS atomics) -> hist[digit][g]
-//   k_rs_scan     one work-group per digit: exclusive scan of its G counts
-//                 in place, and the digit's total
+//   k_rs_hist     per-chunk digit histogram (LDS atomics) -> hist[g][digit]
+//   k_rs_scan     per digit: exclusive scan of its G counts in place (16
+//                 waves share a digit's rows), and the digit's total
 //   k_rs_scatter  each work-group first turns the digit totals into digit
 //                 bases (a 2048-entry work-group scan), then in rounds of
 //                 2048 records (8 per lane) each wave ranks its 512 records
@@ -65,29 +65,55 @@ k_rs_hist(const uint64_t *__restrict__ keys, const uint32_t *count, uint32_t cap
             if (d[j] != 0xFFFFFFFFu) atomicAdd(&h[d[j]], 1u);
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < B; d += RS_THREADS) hist[(size_t)d * G + blockIdx.x] = h[d];
+    for (int d = threadIdx.x; d < B; d += RS_THREADS) hist[(size_t)blockIdx.x * B + d] = h[d];
 }
 
-// One work-group per digit d: exclusive scan of hist[d][0..G) in place;
-// the digit's total to totals[d].
-__global__ void __launch_bounds__(RS_THREADS)
-k_rs_scan(uint32_t *__restrict__ hist, const uint32_t *count, uint32_t cap,
+// Exclusive scan of every digit's column hist[0..G)[d] in place, and the
+// digit's total to totals[d]. A work-group of 16 waves takes 64 digits (one
+// per lane, so each wave reads 256 contiguous bytes of a row); wave q takes
+// the q-th sixteenth of the rows: it sums them, the 16 partial sums are
+// scanned in LDS, and it rescans them from its offset.
+constexpr int RS_SCAN_WAVES = 16;
+__global__ void __launch_bounds__(64 * RS_SCAN_WAVES)
+k_rs_scan(uint32_t *__restrict__ hist, uint32_t nbuckets, const uint32_t *count, uint32_t cap,
           uint32_t *__restrict__ totals) {
-    __shared__ uint32_t lds4[4];
+    __shared__ uint32_t part[RS_SCAN_WAVES][64];
     const uint32_t n = min(*count, cap);
     uint32_t chunk, G;
     rs_chunks(n, &chunk, &G);
-    uint32_t *row = hist + (size_t)blockIdx.x * G;
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < G; base += RS_THREADS) {
-        const uint32_t i = base + threadIdx.x;
-        const uint32_t v = i < G ? row[i] : 0u;
-        uint32_t tot;
-        const uint32_t ex = block_exclusive_scan<uint32_t>(v, lds4, &tot);
-        if (i < G) row[i] = carry + ex;
-        carry += tot;
+    const uint32_t lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const uint32_t d = blockIdx.x * 64 + lane;
+    const uint32_t per = (G + RS_SCAN_WAVES - 1) / RS_SCAN_WAVES;
+    const uint32_t g0 = min(G, q * per), g1 = min(G, g0 + per);
+    uint32_t *col = hist + d;
+    uint32_t sum = 0;
+    for (uint32_t g = g0; g < g1; g += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = g + k < g1 ? col[(size_t)(g + k) * nbuckets] : 0u;
+#pragma unroll
+        for (int k = 0; k < 8; k++) sum += v[k];
     }
-    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+    part[q][lane] = sum;
+    __syncthreads();
+    uint32_t run = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < RS_SCAN_WAVES; k++) {
+        const uint32_t p = part[k][lane];
+        if (k < (int)q) run += p;
+        total += p;
+    }
+    for (uint32_t g = g0; g < g1; g += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = g + k < g1 ? col[(size_t)(g + k) * nbuckets] : 0u;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            if (g + k < g1) col[(size_t)(g + k) * nbuckets] = run;
+            run += v[k];
+        }
+    }
+    if (q == 0) totals[d] = total;
 }
 
 template <int BITS>
@@ -118,7 +144,7 @@ k_rs_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
 #pragma unroll
         for (int k = 0; k < PER; k++) {
             const int d = t * PER + k;
-            base_off[d] = run + hist[(size_t)d * G + blockIdx.x];
+            base_off[d] = run + hist[(size_t)blockIdx.x * B + d];
             run += tv[k];
         }
     }
@@ -221,7 +247,7 @@ static void rs_pass(hipStream_t s, uint32_t shift, const uint64_t *ki, const uin
                     uint32_t *hist, uint32_t *totals) {
     constexpr uint32_t B = 1u << BITS;
     k_rs_hist<BITS><<<RS_GROUPS, RS_THREADS, 0, s>>>(ki, count, cap, shift, hist);
-    k_rs_scan<<<B, RS_THREADS, 0, s>>>(hist, count, cap, totals);
+    k_rs_scan<<<B / 64, 64 * RS_SCAN_WAVES, 0, s>>>(hist, B, count, cap, totals);
     k_rs_scatter<BITS><<<RS_GROUPS, RS_THREADS, 0, s>>>(ki, vi, ko, vo, count, cap, shift, hist,
                                                        totals);
 }
