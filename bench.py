@@ -35,18 +35,43 @@ def log(*a):
 
 
 def stage_bytes(stage, f, npx, counts, n_shifts):
-    """Algorithmic HBM bytes of one stage for one frame (npx = decimated px)."""
+    """Algorithmic HBM bytes of one stage for one frame (npx = decimated px):
+    what the stage must read and write at least, with every intermediate in
+    HBM once."""
     nbp = int(counts["n_boundary"])
+    ncp = int(counts["n_cluster_points"])
+    nc = int(counts["n_corners"])
     if stage == "threshold":
         # f input rows touched per output row, thresholded write, decimated grey write
         return f * npx + npx + (npx if f > 1 else 0)
     if stage == "ccl":
         return 5 * npx  # thresholded read + u32 labels write
     if stage == "boundary":
-        return 4 * npx + 12 * nbp
+        return 4 * npx + 12 * nbp  # labels read + (u64 key, u32 record) written
     if stage == "key_sort":
-        return 24 * nbp * n_shifts
+        return 24 * nbp * n_shifts  # per pass: key + record read and written
+    if stage == "clusters":
+        return 12 * nbp + 12 * ncp  # grouped records read, cluster point + index written
+    if stage == "angle_fit":
+        return 80 * ncp  # point read, sorted point (8 B) and line-fit prefix (64 B) written
+    if stage == "fit_lines":
+        return 76 * ncp  # line-fit prefix read, corner slot (12 B) written
+    if stage == "corners":
+        return 12 * ncp + 24 * nc  # corner slots read, corners compacted and sorted
     return None
+
+
+def stage_traffic(stage):
+    """HBM bytes per frame of `stage` at the headline workload, from the two
+    counter runs (FETCH_SIZE, WRITE_SIZE, scaled by a device copy) summarised
+    in profiles/pipeline_1080p_f1_hbm_traffic.json; None if absent."""
+    path = os.path.join(ROOT, "profiles", "pipeline_1080p_f1_hbm_traffic.json")
+    try:
+        with open(path) as f:
+            st = json.load(f)["stages"].get(stage)
+    except (OSError, ValueError, KeyError):
+        return None
+    return None if st is None else int(st["fetch_bytes"] + st["write_bytes"])
 
 
 def n_sort_passes(npx):
@@ -98,7 +123,8 @@ def bench_12mp(atq, synth, stream, steps=10):
     c = det.counts()
     npx12 = 4096 * 3072
     roof12 = {}
-    for st in ("threshold", "ccl", "boundary", "key_sort"):
+    for st in ("threshold", "ccl", "boundary", "key_sort", "clusters", "angle_fit", "fit_lines",
+               "corners"):
         b = stage_bytes(st, 1, npx12, c, n_sort_passes(npx12))
         roof12[st] = {"ms": round(stages[st], 4), "GB/s": round(b / (stages[st] * 1e-3) / 1e9, 1),
                       "frac": round(b / (stages[st] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
@@ -236,17 +262,19 @@ def main():
     dom = max(st1, key=st1.get)
     roof = None
     alg = stage_bytes(dom, 1, npx, res[1]["counts"], n_sort_passes(npx))
+    traffic = stage_traffic(dom)
     if alg is not None:
         gbs = alg / (st1[dom] * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None, "stage": dom,
+                "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": traffic, "stage": dom,
                 "algorithmic_bytes": int(alg), "avg_ms": round(st1[dom], 5)}
     else:
         roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": None, "traffic": None, "stage": dom, "avg_ms": round(st1[dom], 5)}
+                "frac": None, "traffic": traffic, "stage": dom, "avg_ms": round(st1[dom], 5)}
 
     stage_roof = {}
-    for st in ("threshold", "ccl", "boundary", "key_sort"):
+    for st in ("threshold", "ccl", "boundary", "key_sort", "clusters", "angle_fit", "fit_lines",
+               "corners"):
         b = stage_bytes(st, 1, npx, res[1]["counts"], n_sort_passes(npx))
         stage_roof[st] = {"ms": round(st1[st], 5), "GB/s": round(b / (st1[st] * 1e-3) / 1e9, 1),
                           "frac": round(b / (st1[st] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
