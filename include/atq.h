@@ -103,7 +103,7 @@ typedef struct atq_quad_corners {
 /* Compile-time constants of the reference, in one struct (defaults equal
  * the reference; only the compat values are supported for parity). */
 typedef struct atq_params {
-    int32_t decimate;            /* A0, point subsample; 1 = none           */
+    int32_t decimate;            /* point subsample factor; 1 = none        */
     int32_t min_white_black_diff;/* src/threshold.cpp:86 (5)                */
     int32_t min_component_size;  /* src/find_boundaries.cpp:82 (25)         */
     int32_t min_tag_width;       /* src/main.cpp:1037 (8)                   */
@@ -121,7 +121,7 @@ typedef struct atq_counts {
     uint32_t n_cluster_points;/* points in those clusters                  */
     uint32_t n_corners;       /* corners with error != 0                   */
     uint32_t n_segments;      /* peak segments E                           */
-    uint32_t n_quads;         /* quads surviving the host checks (A21)     */
+    uint32_t n_quads;         /* quads surviving process_fitted_quads      */
     uint32_t overflow;        /* bit flags: capacity exceeded somewhere    */
     uint32_t oob_reads;       /* do_indexing reads past the moment array   */
     uint32_t key_label_bits;  /* lb of the detector's packed pair keys     */
@@ -142,7 +142,7 @@ const char *atq_status_string(int status);
 
 /* ---- stage drop-ins ----------------------------------------------------- */
 
-/* A0: point-subsample decimation out[y][x] = in[f*y][f*x],
+/* Point-subsample decimation out[y][x] = in[f*y][f*x],
  * out dims 1+(w-1)/f x 1+(h-1)/f. No reference counterpart (notes.txt:18-21
  * says decimation is "Ignored for now"); follows upstream apriltag's
  * image_u8_decimate for integer factors. */
@@ -156,7 +156,7 @@ int atq_threshold_image(void *stream, const uint8_t *grayscale,
                         atq_extrema *extrema, atq_extrema *filtered_extrema,
                         uint8_t *thresholded, size_t width, size_t height);
 
-/* Fused A0+A1..A3: decimate by `factor` (1 = none) and threshold in one
+/* Fused decimation + threshold: decimate by `factor` (1 = none) and threshold in one
  * pass. gray_out receives the decimated image when factor > 1 (it feeds the
  * line-fit weights) and may be NULL when factor == 1. extrema and
  * filtered_extrema may be NULL. */
@@ -266,7 +266,7 @@ int atq_do_indexing(void *stream, const atq_peak_extents *extents,
                     const atq_cluster_extents *cluster_extents,
                     atq_fitted_quad *fitted);
 
-/* A21 process_fitted_quads (src/main.cpp:1035-1162) minus tag decoding:
+/* process_fitted_quads (src/main.cpp:1035-1162) minus tag decoding:
  * refit, intersect, area/angle/convexity checks; survivors are written in
  * input order. *n_quads_dev (device u32) receives the count. */
 int atq_quads_from_fits(void *stream, const atq_fitted_quad *fitted,

@@ -348,12 +348,12 @@ struct orc_stats {
     int64_t n_cluster_points;  // points of those clusters
     int64_t n_corners;         // corners with error != 0
     int64_t n_segments;        // peak segments
-    int64_t n_quads;           // A21 survivors
+    int64_t n_quads;           // process_fitted_quads survivors
     int64_t oob_reads;         // do_indexing reads past the moment array
     int64_t n_fitted;          // do_indexing outputs with a chosen combo
 };
 
-// A0: upstream apriltag image_u8_decimate for integer factors: point
+// Decimation: upstream apriltag image_u8_decimate for integer factors: point
 // subsample, output 1+(w-1)/f by 1+(h-1)/f (not in the reference, notes.txt:18-21).
 void orc_decimate(const uint8_t *in, int64_t w_in, int64_t h_in, int f, uint8_t *out) {
     int64_t w = 1 + (w_in - 1) / f, h = 1 + (h_in - 1) / f;
@@ -925,7 +925,7 @@ int64_t orc_do_indexing(const void *ext_v, int64_t E, const void *corners_v, con
     return (int64_t)oob_total;
 }
 
-// A21 process_fitted_quads: src/main.cpp:1035-1162, minus decode_quads.
+// process_fitted_quads: src/main.cpp:1035-1162, minus decode_quads.
 // Quads whose indices are all zero (never written) are skipped
 // (src/main.cpp:996-1018, 1044-1047). Returns the number written.
 int64_t orc_process_quads(const void *fq_v, int64_t E, void *out_v) {

@@ -1,4 +1,4 @@
-// boundary.hip - boundary-point extraction (row A7, src/find_boundaries.cpp).
+// boundary.hip - boundary-point extraction (src/find_boundaries.cpp).
 //
 // For a pixel (x >= 1; column 0 is never a local pixel in the reference's
 // 8x32 work-group tiling, src/find_boundaries.cpp:50-92) with label L, the

@@ -1,4 +1,4 @@
-// ccl.hip - block-based union-find connected-component labelling (rows A4..A6).
+// ccl.hip - block-based union-find connected-component labelling (src/segmentation.cpp).
 //
 // The graph is the reference's (src/segmentation.cpp:101-306): one node per
 // (2x2 block, colour), colour 255 ("white", node id = the block's top-left

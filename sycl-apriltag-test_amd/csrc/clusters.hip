@@ -1,5 +1,5 @@
 // clusters.hip - cluster-pair segments, bounds, filter, angle sort, line-fit
-// moments (rows A10..A16; src/main.cpp:571-801).
+// moments (src/main.cpp:571-801).
 //
 // After the key sort, equal keys form segments (the reference's
 // reduce_by_segment, src/main.cpp:571-585). Each segment's ClusterBounds is

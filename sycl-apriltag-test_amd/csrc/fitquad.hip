@@ -1,5 +1,5 @@
 // fitquad.hip - corner candidates, quad selection and corner geometry
-// (rows A17..A21; src/fit_quad.cpp, src/main.cpp:832-1162).
+// (src/fit_quad.cpp, src/main.cpp:832-1162).
 //
 // fit_lines: the reference's 32-item work-groups with their two-cluster
 // bookkeeping (src/fit_quad.cpp:153-375) compute, for every point x >= 1,

@@ -1,4 +1,4 @@
-// sort.hip - LSD radix sort of (u64 key, u32 value) records (row A9).
+// sort.hip - LSD radix sort of (u64 key, u32 value) records (src/main.cpp:515-518).
 //
 // Groups the boundary records by cluster-pair key (the reference's
 // sort_by_key, src/main.cpp:515-518). A pair key is min_label << 32 |

@@ -1,4 +1,4 @@
-// threshold.hip - decimation + adaptive tile threshold (rows A0..A3).
+// threshold.hip - decimation + adaptive tile threshold (src/threshold.cpp).
 //
 // One kernel does what the reference does in three passes
 // (src/threshold.cpp:8-95): a work-group stages a region of RX x RY 4x4

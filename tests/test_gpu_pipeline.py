@@ -1,5 +1,6 @@
 """Whole-pipeline parity on the GPU: the detector's every intermediate and its
-quads against the oracle (rows A8-A21), and the glue drop-ins chained.
+quads against the oracle (grouping through process_fitted_quads), and the
+glue drop-ins chained.
 
 Bit-exact everywhere, element by element: the GPU writes boundary records
 in the reference's plane-major raster order and every sort is stable, as the

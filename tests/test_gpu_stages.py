@@ -1,6 +1,6 @@
 """Stage parity on the GPU: each C-ABI drop-in against the oracle, bit-exact.
 
-Rows A0-A7: decimate, threshold (extrema, filtered extrema, thresholded),
+Decimate, threshold (extrema, filtered extrema, thresholded),
 segmentation (labels, sizes at every label root and at index 0), boundary
 sparse planes.
 """
