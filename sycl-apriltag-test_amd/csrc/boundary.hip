@@ -274,8 +274,10 @@ k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
 __global__ void __launch_bounds__(256)
 k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes, uint32_t w,
              uint32_t h, uint32_t min_size, uint32_t lb, PairTable table, uint32_t max_count,
-             const atq_counts *counts, uint64_t *__restrict__ tile_keys,
+             const atq_counts *counts, uint32_t n_tiles, uint64_t *__restrict__ tile_keys,
              atq_boundary_point *__restrict__ tile_points, uint32_t *__restrict__ tile_counts) {
+    const uint32_t tile = xcd_tile(blockIdx.x, n_tiles);
+    if (tile >= n_tiles) return;
     __shared__ uint64_t lk[TILE_SLOTS];
     __shared__ uint32_t lc[TILE_SLOTS];  // 1: the pair survives
     __shared__ uint32_t lds4[4];
@@ -286,7 +288,7 @@ k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
     const bool keep_all = counts->pair_filter_off != 0;
     __syncthreads();
     PixelRecords r[BND_PX_PER_THREAD];
-    tile_records(labels, sizes, w, h, min_size, blockIdx.x, r);
+    tile_records(labels, sizes, w, h, min_size, tile, r);
     auto survives = [&](uint64_t key) -> bool {
         const uint32_t g = global_slot(table, key, false);
         if (g == NO_SLOT) return true;
@@ -318,7 +320,7 @@ k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
         }
     uint32_t tot;
     uint32_t pos = block_exclusive_scan<uint32_t>(c, lds4, &tot);
-    const size_t base = (size_t)blockIdx.x * (4 * BND_TILE);
+    const size_t base = (size_t)tile * (4 * BND_TILE);
 #pragma unroll
     for (int k = 0; k < BND_PX_PER_THREAD; k++)
 #pragma unroll
@@ -330,7 +332,7 @@ k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
             tile_points[base + pos] = r[k].p[p];
             pos++;
         }
-    if (threadIdx.x == 0) tile_counts[blockIdx.x] = tot;
+    if (threadIdx.x == 0) tile_counts[tile] = tot;
 }
 
 __global__ void __launch_bounds__(256)
@@ -376,9 +378,9 @@ int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const
     k_pair_count<<<min(tiles, 512u), 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h,
                                                    (uint32_t)min_size, tiles, table, max_count,
                                                    counts);
-    k_bnd_filter<<<tiles, 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size,
-                                       label_bits, table, max_count, counts, tile_keys, tile_points,
-                                       tile_counts);
+    k_bnd_filter<<<xcd_grid(tiles), 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h,
+                                                 (uint32_t)min_size, label_bits, table, max_count,
+                                                 counts, tiles, tile_keys, tile_points, tile_counts);
     int r = device_scan(s, nullptr, tiles, TileCount{tile_counts}, TileOffset{tile_off},
                         sc, &counts->n_boundary);
     if (r) return r;

@@ -137,23 +137,34 @@ ATQ_DEV void g_union(uint32_t *parent, uint32_t a, uint32_t b) {
 }  // namespace
 
 __global__ void __launch_bounds__(256)
-k_ccl_local(const uint8_t *__restrict__ thr, uint32_t w, uint32_t h, uint32_t *__restrict__ parent,
-            uint32_t *__restrict__ sizes) {
+k_ccl_local(const uint8_t *__restrict__ thr, uint32_t w, uint32_t h, uint32_t tiles_x,
+            uint32_t n_tiles, uint32_t *__restrict__ parent, uint32_t *__restrict__ sizes) {
     constexpr int PR = 2 * TBY + 1;      // pixel rows incl. the halo row above
     constexpr int PC = 2 * TBX + 2;      // pixel cols incl. left halo and R column
     __shared__ uint8_t px[PR][PC + 2];
     __shared__ uint32_t lp[NODES];
     __shared__ uint32_t cnt[NODES];
 
+    const uint32_t tile = xcd_tile(blockIdx.x, n_tiles);
+    if (tile >= n_tiles) return;
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * (2 * TBX), y0 = blockIdx.y * (2 * TBY);
+    const int x0 = (tile % tiles_x) * (2 * TBX), y0 = (tile / tiles_x) * (2 * TBY);
 
-    for (int i = tid; i < PR * PC; i += 256) {
-        int r = i / PC, c = i % PC;
-        int y = y0 - 1 + r, x = x0 - 1 + c;
-        uint8_t v = 127;
-        if (y >= 0 && y < (int)h && x >= 0 && x < (int)w) v = thr[(size_t)y * w + x];
-        px[r][c] = v;
+    // every load of a thread is issued before the first LDS store
+    constexpr int PER = (PR * PC + 255) / 256;
+    uint8_t vals[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = tid + 256 * k;
+        const int r = i / PC, c = i % PC;
+        const int y = y0 - 1 + r, x = x0 - 1 + c;
+        vals[k] = (i < PR * PC && y >= 0 && y < (int)h && x >= 0 && x < (int)w)
+                      ? thr[(size_t)y * w + x] : (uint8_t)127;
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = tid + 256 * k;
+        if (i < PR * PC) px[i / PC][i % PC] = vals[k];
     }
     for (int i = tid; i < NODES; i += 256) {
         lp[i] = i;
@@ -161,23 +172,67 @@ k_ccl_local(const uint8_t *__restrict__ thr, uint32_t w, uint32_t h, uint32_t *_
     }
     __syncthreads();
 
-    // tile-internal unions
-    for (int b = tid; b < TBX * TBY; b += 256) {
-        int bx = b % TBX, by = b / TBX;
-        int x = x0 + 2 * bx, y = y0 + 2 * by;
-        if (x >= (int)w || y >= (int)h) continue;
-        const int pr = 1 + 2 * by, pc = 1 + 2 * bx;
-        uint32_t m = check_mask(px[pr][pc], px[pr][pc + 1], px[pr + 1][pc], x, y, w, h);
+    // Tile-internal unions in two steps. (a) Each block row lives in one
+    // half-wave (32 lanes = TBX blocks): a block whose S probes reach its
+    // left neighbour in colour c continues that neighbour's run, so with one
+    // ballot every node points straight at the first node of its run (the
+    // run's minimum id). (b) The P, Q, Q, R probes join runs to the row
+    // above through the LDS union-find; a union that repeats the previous
+    // lane's (run, run above) pair is skipped.
+    static_assert(TBX == 32 && TBY % 8 == 0, "a block row is a half-wave");
+    const int lane = tid & 63;
+    const uint64_t half = lane < 32 ? 0x00000000FFFFFFFFull : 0xFFFFFFFF00000000ull;
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+    uint32_t pv[TBY / 8];  // probe pixels of this thread's blocks: 2 bits per probe
+                           // (0: not tested or 127, 1: white, 2: black)
 #pragma unroll
-        for (int k = 0; k < 6; k++) {
-            Probe q = PROBES[k];
-            if (!(m & (1u << q.bit))) continue;
-            uint8_t v = px[pr + q.dy][pc + q.dx];
-            if (v != C_WHITE && v != C_BLACK) continue;
-            int nbx = bx + q.dbx, nby = by + q.dby;
-            if (nbx < 0 || nbx >= TBX || nby < 0) continue;  // crosses the tile
-            uint32_t c = v == C_WHITE ? 0 : 1;
-            lds_union(lp, 2 * (by * TBX + bx) + c, 2 * (nby * TBX + nbx) + c);
+    for (int pass = 0; pass < TBY / 8; pass++) {
+        const int b = tid + 256 * pass, bx = b % TBX, by = b / TBX;
+        const int x = x0 + 2 * bx, y = y0 + 2 * by;
+        const bool in_img = x < (int)w && y < (int)h;
+        const int pr = 1 + 2 * by, pc = 1 + 2 * bx;
+        uint32_t m = 0, v2 = 0;
+        if (in_img) {
+            m = check_mask(px[pr][pc], px[pr][pc + 1], px[pr + 1][pc], x, y, w, h);
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+                const Probe q = PROBES[k];
+                if (!(m & (1u << q.bit))) continue;
+                const uint8_t v = px[pr + q.dy][pc + q.dx];
+                if (v == C_WHITE) v2 |= 1u << (2 * k);
+                else if (v == C_BLACK) v2 |= 2u << (2 * k);
+            }
+        }
+        pv[pass] = v2;
+#pragma unroll
+        for (uint32_t c = 0; c < 2; c++) {
+            // S probes are k = 4, 5; both reach the left neighbour (0, -1)
+            const uint32_t code = c + 1;
+            const bool left = bx > 0 && (((v2 >> 8) & 3u) == code || ((v2 >> 10) & 3u) == code);
+            const uint64_t starts = __ballot(!left) & half & upto;
+            const int run0 = (63 - __clzll((long long)starts)) & 31;
+            lp[2 * (by * TBX + bx) + c] = 2 * (by * TBX + run0) + c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int pass = 0; pass < TBY / 8; pass++) {
+        const int b = tid + 256 * pass, bx = b % TBX, by = b / TBX;
+        const uint32_t v2 = pv[pass];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {  // P, Q, Q, R: the row above
+            const Probe q = PROBES[k];
+            const uint32_t code = (v2 >> (2 * k)) & 3u;
+            const int nbx = bx + q.dbx, nby = by + q.dby;
+            uint32_t ra = 0xFFFFFFFFu, rb = 0xFFFFFFFFu;
+            if (code && nbx >= 0 && nbx < TBX && nby >= 0) {
+                const uint32_t c = code - 1;
+                ra = lp[2 * (by * TBX + bx) + c];
+                rb = lp[2 * (nby * TBX + nbx) + c];
+            }
+            const uint32_t pa = __shfl_up(ra, 1, 64), pb = __shfl_up(rb, 1, 64);
+            const bool dup = (lane & 31) > 0 && pa == ra && pb == rb;
+            if (ra != 0xFFFFFFFFu && !dup) lds_union(lp, ra, rb);
         }
     }
     __syncthreads();
@@ -325,7 +380,8 @@ k_ccl_final(const uint8_t *__restrict__ thr, uint32_t w, uint32_t h,
 int launch_ccl(hipStream_t s, const uint8_t *thr, size_t w, size_t h, uint32_t *parent,
                uint32_t *labels, uint32_t *sizes, uint32_t size_cap) {
     dim3 tiles(atq_div_up(w, 2 * TBX), atq_div_up(h, 2 * TBY));
-    k_ccl_local<<<tiles, 256, 0, s>>>(thr, (uint32_t)w, (uint32_t)h, parent, sizes);
+    k_ccl_local<<<xcd_grid(tiles.x * tiles.y), 256, 0, s>>>(thr, (uint32_t)w, (uint32_t)h, tiles.x,
+                                                           tiles.x * tiles.y, parent, sizes);
     uint32_t n_items = tiles.x * tiles.y * (TBX + 2 * (TBY - 1));
     k_ccl_border<<<atq_div_up(n_items, 256), 256, 0, s>>>(thr, (uint32_t)w, (uint32_t)h, tiles.x,
                                                           n_items, parent);

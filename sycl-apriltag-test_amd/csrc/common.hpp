@@ -73,6 +73,19 @@ ATQ_DEV uint32_t wave_exclusive_sum(uint32_t v, uint32_t *total) {
     return x - v;
 }
 
+// Work-groups are handed to the 8 XCDs (each with its own L2) round-robin in
+// launch order. For a grid of 8 * ceil(n / 8) work-groups this maps work-group
+// b to tile xcd_tile(b, n), so that each XCD takes one contiguous run of
+// tiles and neighbouring tiles share that XCD's L2 (halo rows and columns are
+// then fetched from HBM once). Returns n (no tile) for the padding groups.
+constexpr uint32_t N_XCD = 8;
+ATQ_DEV uint32_t xcd_tile(uint32_t b, uint32_t n) {
+    const uint32_t per = (n + N_XCD - 1) / N_XCD;
+    const uint32_t t = (b % N_XCD) * per + b / N_XCD;
+    return t < n ? t : n;
+}
+inline uint32_t xcd_grid(uint32_t n) { return (n + N_XCD - 1) / N_XCD * N_XCD; }
+
 // Orderable image of a float (no NaN is ever sorted here): ascending u32
 // order equals ascending float order.
 ATQ_DEV uint32_t float_key(float f) {

@@ -7,7 +7,8 @@
 // neighbourhood (edge-clipped), and thresholds the interior pixels:
 //   127 if max-min < min_diff, else 255 if v > min+(max-min)/2, else 0.
 // HBM traffic: f*Npx read (rows touched) + Npx written (+ Npx decimated grey
-// when f > 1) + the halo re-reads.
+// when f > 1); regions go to the XCDs in contiguous runs, so the halo lines a
+// region shares with its neighbours mostly come from the same L2.
 #include "kernels.hpp"
 
 namespace atq {
@@ -25,8 +26,9 @@ __global__ void k_decimate(const uint8_t *__restrict__ in, uint32_t w_in, uint32
 template <int RX, int RY>
 __global__ void __launch_bounds__(256)
 k_threshold(const uint8_t *__restrict__ in, uint32_t w_in, uint32_t f, uint32_t w, uint32_t h,
-            uint8_t *__restrict__ gray_out, atq_extrema *__restrict__ ext_out,
-            atq_extrema *__restrict__ fil_out, uint8_t *__restrict__ thr_out, int min_diff) {
+            uint32_t regions_x, uint32_t n_regions, uint8_t *__restrict__ gray_out,
+            atq_extrema *__restrict__ ext_out, atq_extrema *__restrict__ fil_out,
+            uint8_t *__restrict__ thr_out, int min_diff) {
     constexpr int PW = (RX + 2) * 4;  // region width in pixels incl. halo
     constexpr int PH = (RY + 2) * 4;
     constexpr int TW = RX + 2, TH = RY + 2;
@@ -34,18 +36,26 @@ k_threshold(const uint8_t *__restrict__ in, uint32_t w_in, uint32_t f, uint32_t 
     __shared__ uint8_t emin[TH][TW], emax[TH][TW];
     __shared__ uint8_t fmin[RY][RX], fmax[RY][RX];
 
+    const uint32_t region = xcd_tile(blockIdx.x, n_regions);
+    if (region >= n_regions) return;
     const int tw = (int)(w / 4), th = (int)(h / 4);
-    const int tx0 = (int)blockIdx.x * RX - 1;  // first (halo) tile of the region
-    const int ty0 = (int)blockIdx.y * RY - 1;
+    const int tx0 = (int)(region % regions_x) * RX - 1;  // first (halo) tile of the region
+    const int ty0 = (int)(region / regions_x) * RY - 1;
     const int px0 = tx0 * 4, py0 = ty0 * 4;
     const int tid = threadIdx.x;
 
-    // 1. stage decimated pixels of the region (words of 4 pixels)
-    for (int i = tid; i < PH * (PW / 4); i += 256) {
-        int r = i / (PW / 4), c = i % (PW / 4);
-        int y = py0 + r, x = px0 + c * 4;
+    // 1. stage decimated pixels of the region (words of 4 pixels); every
+    // load of a thread is issued before the first LDS store
+    constexpr int WORDS = PH * (PW / 4);
+    constexpr int PER = (WORDS + 255) / 256;
+    uint32_t words[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = tid + 256 * k;
+        const int r = i / (PW / 4), c = i % (PW / 4);
+        const int y = py0 + r, x = px0 + c * 4;
         uint32_t word = 0;
-        if (y >= 0 && y < (int)h && x >= 0 && x < (int)w) {
+        if (i < WORDS && y >= 0 && y < (int)h && x >= 0 && x < (int)w) {
             if (f == 1) {
                 word = *reinterpret_cast<const uint32_t *>(in + (size_t)y * w_in + x);
             } else if (f == 2 && (w_in & 3) == 0) {
@@ -55,10 +65,15 @@ k_threshold(const uint8_t *__restrict__ in, uint32_t w_in, uint32_t f, uint32_t 
             } else {
                 const uint8_t *row = in + (size_t)(y * f) * w_in;
 #pragma unroll
-                for (int k = 0; k < 4; k++) word |= (uint32_t)row[(size_t)(x + k) * f] << (8 * k);
+                for (int q = 0; q < 4; q++) word |= (uint32_t)row[(size_t)(x + q) * f] << (8 * q);
             }
         }
-        pix[r][c] = word;
+        words[k] = word;
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = tid + 256 * k;
+        if (i < WORDS) pix[i / (PW / 4)][i % (PW / 4)] = words[k];
     }
     __syncthreads();
 
@@ -143,10 +158,10 @@ int launch_threshold(hipStream_t s, const uint8_t *in, size_t w_in, size_t h_in,
     constexpr int RX = 32, RY = 16;
     size_t w = f > 1 ? 1 + (w_in - 1) / f : w_in;
     size_t h = f > 1 ? 1 + (h_in - 1) / f : h_in;
-    dim3 grid(atq_div_up(w / 4, RX), atq_div_up(h / 4, RY));
-    k_threshold<RX, RY><<<grid, 256, 0, s>>>(in, (uint32_t)w_in, (uint32_t)f, (uint32_t)w,
-                                             (uint32_t)h, f > 1 ? gray_out : nullptr, ext, fil,
-                                             thr, min_diff);
+    const uint32_t rx = atq_div_up(w / 4, RX), ry = atq_div_up(h / 4, RY);
+    k_threshold<RX, RY><<<xcd_grid(rx * ry), 256, 0, s>>>(
+        in, (uint32_t)w_in, (uint32_t)f, (uint32_t)w, (uint32_t)h, rx, rx * ry,
+        f > 1 ? gray_out : nullptr, ext, fil, thr, min_diff);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
