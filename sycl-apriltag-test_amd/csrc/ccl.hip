@@ -15,9 +15,10 @@
 //     union-find, counts each local root's pixels, and writes for every node
 //     the global id of its local root (FLAG-tagged for non-roots) and, at
 //     each local root, its pixel count into `sizes`.
-//  2. k_ccl_border: edges that cross tiles are merged on the global parent
+//  2. k_ccl_border: one wave per tile collects the distinct pairs of local
+//     roots its border edges join and merges each once on the global parent
 //     array with the reference's lock-free atomicMin union
-//     (include/union_find.h:50-77).
+//     (include/union_find.h:50-77), halving paths as it walks.
 //  3. k_ccl_flatten: every local root is pointed straight at its final
 //     root and adds its count into the root's size.
 //  4. k_ccl_final: each block reads its two roots (at most two loads) and
@@ -270,22 +271,31 @@ k_ccl_local(const uint8_t *__restrict__ thr, uint32_t w, uint32_t h, uint32_t ti
     }
 }
 
-// One item per (tile, border block); merges the block's tile-crossing edges.
-// Each edge becomes a union of the two local roots (parent words of
-// non-roots point straight at their local root); neighbouring border blocks
-// nearly always join the same two local roots, so a union repeated by the
-// previous lane (same probe) or by an earlier probe of the same block is
-// skipped before it reaches the global atomics.
+// One wave per tile; lane k < 62 takes border block k (top row, then the
+// left and right columns) and derives the edges that leave the tile. Each
+// edge is a pair of local roots (a parent word of a non-root points straight
+// at its local root); the wave keeps the distinct pairs in an LDS hash set
+// and only the lane that inserts a pair merges it on the global parent array
+// (lock-free atomicMin union, include/union_find.h:50-77). A tile border is
+// crossed by few components, so a tile issues a handful of global unions.
+constexpr int BORDER_ITEMS = TBX + 2 * (TBY - 1);
+constexpr int BORDER_SLOTS = 512;
+static_assert(BORDER_ITEMS <= 64, "one border block per lane");
+
 __global__ void __launch_bounds__(256)
 k_ccl_border(const uint8_t *__restrict__ thr, uint32_t w, uint32_t h, uint32_t tiles_x,
-             uint32_t n_items, uint32_t *__restrict__ parent) {
-    constexpr int PER_TILE = TBX + 2 * (TBY - 1);
+             uint32_t n_tiles, uint32_t *__restrict__ parent) {
     constexpr uint32_t NONE = 0xFFFFFFFFu;
-    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
-    bool live = item < n_items;
+    constexpr uint64_t EMPTY = ~0ull;
+    __shared__ uint64_t set[4][BORDER_SLOTS];
+    const uint32_t wave = threadIdx.x >> 6, k = threadIdx.x & 63;
+    const uint32_t tile = blockIdx.x * 4 + wave;
+    uint64_t *hs = set[wave];
+    for (int i = k; i < BORDER_SLOTS; i += 64) hs[i] = EMPTY;
+    __syncthreads();
+    bool live = tile < n_tiles && k < (uint32_t)BORDER_ITEMS;
     int bx = 0, by = 0, x = 0, y = 0;
     if (live) {
-        const uint32_t tile = item / PER_TILE, k = item % PER_TILE;
         if (k < (uint32_t)TBX) { bx = k; by = 0; }
         else if (k < (uint32_t)(TBX + TBY - 1)) { bx = 0; by = 1 + (k - TBX); }
         else { bx = TBX - 1; by = 1 + (k - TBX - (TBY - 1)); }
@@ -312,18 +322,29 @@ k_ccl_border(const uint8_t *__restrict__ thr, uint32_t w, uint32_t h, uint32_t t
             if (v == C_WHITE || v == C_BLACK) {
                 const uint32_t c = v == C_WHITE ? 0 : 1;
                 const uint32_t nid = (uint32_t)(y + 2 * q.dby) * w + (uint32_t)(x + 2 * q.dbx);
-                ea[i] = g_load(&parent[id + c]) & ~FLAG;
-                eb[i] = g_load(&parent[nid + c]) & ~FLAG;
+                const uint32_t a = g_load(&parent[id + c]) & ~FLAG;
+                const uint32_t b = g_load(&parent[nid + c]) & ~FLAG;
+                ea[i] = min(a, b);
+                eb[i] = max(a, b);
             }
         }
     }
 #pragma unroll
     for (int i = 0; i < 6; i++) {
-        const uint32_t pa = __shfl_up(ea[i], 1, 64), pb = __shfl_up(eb[i], 1, 64);
-        bool dup = __lane_id() > 0 && pa == ea[i] && pb == eb[i];
-#pragma unroll
-        for (int j = 0; j < i; j++) dup = dup || (ea[j] == ea[i] && eb[j] == eb[i]);
-        if (ea[i] != NONE && !dup) g_union(parent, ea[i], eb[i]);
+        if (ea[i] == NONE || ea[i] == eb[i]) continue;
+        const uint64_t key = ((uint64_t)ea[i] << 32) | eb[i];
+        uint32_t hsl = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 32) & (BORDER_SLOTS - 1);
+        bool mine = true;  // the set is full: merge it here
+        for (int p = 0; p < BORDER_SLOTS; p++, hsl = (hsl + 1) & (BORDER_SLOTS - 1)) {
+            const uint64_t old = atomicCAS((unsigned long long *)&hs[hsl], (unsigned long long)EMPTY,
+                                           (unsigned long long)key);
+            if (old == EMPTY) break;
+            if (old == key) {
+                mine = false;
+                break;
+            }
+        }
+        if (mine) g_union(parent, ea[i], eb[i]);
     }
 }
 
@@ -382,9 +403,9 @@ int launch_ccl(hipStream_t s, const uint8_t *thr, size_t w, size_t h, uint32_t *
     dim3 tiles(atq_div_up(w, 2 * TBX), atq_div_up(h, 2 * TBY));
     k_ccl_local<<<xcd_grid(tiles.x * tiles.y), 256, 0, s>>>(thr, (uint32_t)w, (uint32_t)h, tiles.x,
                                                            tiles.x * tiles.y, parent, sizes);
-    uint32_t n_items = tiles.x * tiles.y * (TBX + 2 * (TBY - 1));
-    k_ccl_border<<<atq_div_up(n_items, 256), 256, 0, s>>>(thr, (uint32_t)w, (uint32_t)h, tiles.x,
-                                                          n_items, parent);
+    const uint32_t n_tiles = tiles.x * tiles.y;
+    k_ccl_border<<<atq_div_up(n_tiles, 4), 256, 0, s>>>(thr, (uint32_t)w, (uint32_t)h, tiles.x,
+                                                       n_tiles, parent);
     dim3 fgrid(atq_div_up(w / 2, 256), (unsigned)(h / 2));
     k_ccl_flatten<<<fgrid, 256, 0, s>>>((uint32_t)w, (uint32_t)h, parent, sizes, size_cap);
     k_ccl_final<<<fgrid, 256, 0, s>>>(thr, (uint32_t)w, (uint32_t)h, parent, labels);
