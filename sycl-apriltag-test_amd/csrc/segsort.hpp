@@ -2,17 +2,16 @@
 //
 // A segment is a {start, count} run of one array (clusters, peak segments).
 // Every segment is cut into chunks of SEG_CHUNK keys; one 1024-thread
-// work-group sorts a chunk in LDS with a stable LSD radix sort over the
-// keys' high 32 bits (4 passes of 8 bits; each wave ranks its elements with
-// ballots, the 16 waves' digit counts are scanned in the work-group). The
-// low 32 bits ride along: callers put there a number that increases with the
-// element's position (its index), so sorting by the high word alone orders
-// the full 64-bit keys, which are distinct. A segment that fits one chunk is
-// finished right there by the caller's Finish functor. Longer segments are
-// written to keys0 and completed by merge passes in which every key finds
-// its rank in the partner run by binary search; runs double each pass, so a
-// segment of n keys needs seg_merge_passes(n) passes and ends in keys1 when
-// that number is odd, else in keys0.
+// work-group sorts a chunk in LDS: each thread sorts 4 keys in registers, then
+// runs are merged pairwise (merge path: every thread finds where its 4
+// outputs start by binary search on its diagonal and merges them), 10
+// levels for a full chunk. Keys must be distinct (callers put the element's
+// position in the low bits), so the result is a stable sort. A segment that
+// fits one chunk is finished right there by the caller's Finish functor.
+// Longer segments are written to keys0 and completed by merge passes in
+// which every key finds its rank in the partner run by binary search; runs
+// double each pass, so a segment of n keys needs seg_merge_passes(n) passes
+// and ends in keys1 when that number is odd, else in keys0.
 #pragma once
 #include "scan.hpp"
 
@@ -75,80 +74,60 @@ ATQ_DEV void seg_block_scan6(const int64_t (&v)[6], int64_t (&incl)[6], int64_t 
     __syncthreads();
 }
 
-// LDS of a chunk sort: the keys' high and low words, and the digit
-// counters (256 digits x 16 waves, digit-major).
-constexpr int SEG_WAVES = SEG_NT / 64;
-constexpr uint32_t SEG_WAVE_SPAN = 64 * SEG_IPT;  // elements per wave
-struct SegLds {
-    uint32_t hi[SEG_CHUNK];
-    uint32_t lo[SEG_CHUNK];
-    uint32_t cnt[256 * SEG_WAVES];
-    uint32_t wsum[SEG_WAVES];
-};
-static_assert(256 * SEG_WAVES == 4 * SEG_NT, "four counters per thread");
+ATQ_DEV void seg_cas(uint64_t &a, uint64_t &b) {
+    const uint64_t lo = a < b ? a : b, hi = a < b ? b : a;
+    a = lo;
+    b = hi;
+}
 
-// Stable LSD radix sort of L.hi/L.lo[0..m) by the high word. Wave w holds
-// elements w*SEG_WAVE_SPAN + i*64 + lane (i < SEG_IPT), so ranks taken in i
-// order, then by lane, follow the element order.
-ATQ_DEV void seg_lds_radix(SegLds &L, uint32_t m) {
-    constexpr int NW = SEG_WAVES;
-    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const uint32_t wb = w * SEG_WAVE_SPAN;
-    const uint64_t lt = lanemask_lt();
-    for (int pass = 0; pass < 4; pass++) {
-        const uint32_t shift = 8 * pass;
-        uint32_t hi[SEG_IPT], lo[SEG_IPT], d[SEG_IPT], rank[SEG_IPT];
+// Sorts a[0..m) (m <= SEG_CHUNK; a[m..SEG_CHUNK) must hold SEG_PAD) by merge
+// path; b is a second SEG_CHUNK buffer. The result is in a. All SEG_NT
+// threads call it.
+ATQ_DEV void seg_merge_sort(uint64_t *a, uint64_t *b, uint32_t m) {
+    static_assert(SEG_IPT == 4, "four keys per thread");
+    const uint32_t t = threadIdx.x, d0 = t * SEG_IPT;
+    uint32_t P = SEG_IPT;
+    while (P < m) P <<= 1;
+    uint64_t v[SEG_IPT];
 #pragma unroll
-        for (int i = 0; i < SEG_IPT; i++) {
-            const uint32_t e = wb + i * 64 + lane;
-            hi[i] = e < m ? L.hi[e] : 0u;
-            lo[i] = e < m ? L.lo[e] : 0u;
-            d[i] = (hi[i] >> shift) & 255u;
-        }
+    for (int i = 0; i < SEG_IPT; i++) v[i] = a[d0 + i];
+    seg_cas(v[0], v[1]);
+    seg_cas(v[2], v[3]);
+    seg_cas(v[0], v[2]);
+    seg_cas(v[1], v[3]);
+    seg_cas(v[1], v[2]);
 #pragma unroll
-        for (int k = 0; k < 4; k++) L.cnt[4 * t + k] = 0;
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < SEG_IPT; i++) {
-            if (wb + i * 64 >= m) break;  // the rest of this wave is empty
-            const bool valid = wb + i * 64 + lane < m;
-            uint64_t mk = __ballot(valid);
-#pragma unroll
-            for (int bit = 0; bit < 8; bit++) {
-                const uint64_t bb = __ballot((d[i] >> bit) & 1u);
-                mk &= ((d[i] >> bit) & 1u) ? bb : ~bb;
+    for (int i = 0; i < SEG_IPT; i++) a[d0 + i] = v[i];
+    __syncthreads();
+    uint64_t *src = a, *dst = b;
+    for (uint32_t r = SEG_IPT; r < P; r <<= 1) {
+        if (d0 < P) {
+            const uint32_t base = d0 / (2 * r) * (2 * r), d = d0 - base;
+            const uint64_t *A = src + base, *B = src + base + r;
+            // i = number of A keys among the first d outputs
+            uint32_t lo = d > r ? d - r : 0, hi = d < r ? d : r;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (A[mid] < B[d - mid - 1]) lo = mid + 1;
+                else hi = mid;
             }
-            const int leader = valid ? __ffsll((unsigned long long)mk) - 1 : (int)lane;
-            uint32_t before = 0;
-            if (valid && (int)lane == leader) {
-                before = L.cnt[d[i] * NW + w];
-                L.cnt[d[i] * NW + w] = before + (uint32_t)__popcll(mk);
+            uint32_t i = lo, j = d - lo;
+#pragma unroll
+            for (int k = 0; k < SEG_IPT; k++) {
+                const bool take_a = j >= r || (i < r && A[i] < B[j]);
+                dst[base + d + k] = take_a ? A[i] : B[j];
+                if (take_a) i++;
+                else j++;
             }
-            before = __shfl(before, leader, 64);
-            rank[i] = before + (uint32_t)__popcll(mk & lt);
         }
         __syncthreads();
-        uint32_t c4[4], sum = 0;
+        uint64_t *tmp = src;
+        src = dst;
+        dst = tmp;
+    }
+    if (src != a) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            c4[k] = L.cnt[4 * t + k];
-            sum += c4[k];
-        }
-        uint32_t all;
-        uint32_t run = block_exclusive_scan<uint32_t, SEG_NT>(sum, L.wsum, &all);
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            L.cnt[4 * t + k] = run;
-            run += c4[k];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < SEG_IPT; i++) {
-            if (wb + i * 64 + lane >= m) continue;
-            const uint32_t pos = L.cnt[d[i] * NW + w] + rank[i];
-            L.hi[pos] = hi[i];
-            L.lo[pos] = lo[i];
-        }
+        for (int i = 0; i < SEG_IPT; i++) a[d0 + i] = src[d0 + i];
         __syncthreads();
     }
 }
@@ -158,14 +137,13 @@ ATQ_DEV void seg_lds_radix(SegLds &L, uint32_t m) {
 // Finish: void operator()(uint32_t seg, uint32_t start, uint32_t n,
 // const uint64_t (&v)[SEG_IPT], uint64_t *lds) const, called by every thread
 // of the work-group with a whole sorted segment of n <= SEG_CHUNK keys in
-// registers (key e0 + i in v[i], e0 = threadIdx.x * SEG_IPT) and lds, 2048
-// u64 of scratch.
-// Keys move between global memory and LDS coalesced.
+// registers (key e0 + i in v[i], e0 = threadIdx.x * SEG_IPT) and lds, 4096
+// u64 of scratch. Keys move between global memory and LDS coalesced.
 template <typename KeyGen, typename Finish, typename Ext>
 __global__ void __launch_bounds__(SEG_NT)
 k_seg_chunks(KeyGen gen, Finish fin, const Ext *__restrict__ segs, const uint32_t *n_segs_dev,
              uint32_t cap_segs, uint32_t max_chunks, uint64_t *__restrict__ keys0) {
-    __shared__ SegLds L;
+    __shared__ uint64_t ka[SEG_CHUNK], kb[SEG_CHUNK];
     const uint32_t nseg = min(*n_segs_dev, cap_segs);
     const uint64_t items = (uint64_t)nseg * max_chunks;
     const uint32_t t = threadIdx.x, e0 = t * SEG_IPT;
@@ -175,25 +153,19 @@ k_seg_chunks(KeyGen gen, Finish fin, const Ext *__restrict__ segs, const uint32_
         const uint32_t b = ch * SEG_CHUNK;
         if (b >= n) continue;
         const uint32_t m = min(SEG_CHUNK, n - b);
-        for (uint32_t j = t; j < m; j += SEG_NT) {
-            const uint64_t k = gen(sg, start, b + j);
-            L.hi[j] = (uint32_t)(k >> 32);
-            L.lo[j] = (uint32_t)k;
-        }
+        for (uint32_t j = t; j < SEG_CHUNK; j += SEG_NT)
+            ka[j] = j < m ? gen(sg, start, b + j) : SEG_PAD;
         __syncthreads();
-        seg_lds_radix(L, m);
+        seg_merge_sort(ka, kb, m);
         if (n <= SEG_CHUNK) {
             uint64_t v[SEG_IPT];
 #pragma unroll
-            for (int i = 0; i < SEG_IPT; i++)
-                v[i] = e0 + i < m ? ((uint64_t)L.hi[e0 + i] << 32) | L.lo[e0 + i] : SEG_PAD;
-            __syncthreads();
-            fin(sg, start, n, v, reinterpret_cast<uint64_t *>(L.cnt));
+            for (int i = 0; i < SEG_IPT; i++) v[i] = ka[e0 + i];
+            fin(sg, start, n, v, kb);
             __syncthreads();
             continue;
         }
-        for (uint32_t j = t; j < m; j += SEG_NT)
-            keys0[start + b + j] = ((uint64_t)L.hi[j] << 32) | L.lo[j];
+        for (uint32_t j = t; j < m; j += SEG_NT) keys0[start + b + j] = ka[j];
         __syncthreads();
     }
 }
