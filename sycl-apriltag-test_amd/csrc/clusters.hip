@@ -249,52 +249,108 @@ struct AngleKey {
     }
 };
 
-// One chunk of up to SEG_CHUNK points of a cluster in angle order (src[i]:
-// the point at chunk position e0 + i, or NONE past the end): writes the
-// points to `out` (if non-null) and their line-fit moments
-// (include/line_fit_point.h:50-86), prefix-summed from `carry` over the
-// chunk, to `lf` (if non-null). Called by all SEG_NT threads; tot gets the
-// chunk's sums. Each point keeps only its weight and doubled coordinates in
-// registers; the moments are rebuilt from them when written.
-ATQ_DEV void fit_chunk(const atq_cluster_point *in, atq_cluster_point *out, uint32_t start,
-                       uint32_t e0, const uint32_t (&src)[SEG_IPT], const uint8_t *grey, uint32_t w,
-                       uint32_t h, atq_line_fit_point *lf, const int64_t (&carry)[6],
-                       int64_t (&tot)[6], int64_t *lds) {
-    uint32_t W[SEG_IPT], RX[SEG_IPT], RY[SEG_IPT];
-    int64_t sum[6] = {0, 0, 0, 0, 0, 0};
+// Prefix records staged per round of FIT_STAGE positions: the owners put
+// their moments in LDS, then consecutive lanes write consecutive 16-byte
+// quarters of the 64-byte records (a thread's own four records lie 256 bytes
+// from its neighbours', which leaves every store instruction scattered).
+constexpr uint32_t FIT_STAGE = 1024;
+constexpr uint32_t FIT_STAGE_WORDS = 6 * FIT_STAGE;
+
+// Loads the points of chunk positions e0 + i < m (src[i]: their positions in
+// the cluster) and their line-fit terms: weight W and doubled coordinates
+// RX, RY (zero past m); sum gets the thread's moment sums. Two rounds of
+// independent loads (points, then their grey neighbourhoods) instead of a
+// dependent chain per point.
+ATQ_DEV void fit_terms(const atq_cluster_point *in, uint32_t start, uint32_t e0, uint32_t m,
+                       const uint32_t (&src)[SEG_IPT], const uint8_t *grey, uint32_t w,
+                       uint32_t h, atq_cluster_point (&p)[SEG_IPT], uint32_t (&W)[SEG_IPT],
+                       uint32_t (&RX)[SEG_IPT], uint32_t (&RY)[SEG_IPT], int64_t (&sum)[6]) {
+#pragma unroll
+    for (int i = 0; i < SEG_IPT; i++) p[i] = in[start + (e0 + i < m ? src[i] : 0u)];
+    uint32_t g4[SEG_IPT][4];
+#pragma unroll
+    for (int i = 0; i < SEG_IPT; i++) linefit_gather(p[i].packed_x, p[i].packed_y, w, h, grey, g4[i]);
+#pragma unroll
+    for (int q = 0; q < 6; q++) sum[q] = 0;
 #pragma unroll
     for (int i = 0; i < SEG_IPT; i++) {
         W[i] = RX[i] = RY[i] = 0;
-        if (src[i] == NONE) continue;
-        const atq_cluster_point p = in[start + src[i]];
-        if (out) out[start + e0 + i] = p;
-        if (lf) {
-            linefit_terms(p.packed_x, p.packed_y, w, h, grey, &W[i], &RX[i], &RY[i]);
-            const atq_line_fit_point m = linefit_from_terms(W[i], RX[i], RY[i]);
-            sum[0] += m.Mx;
-            sum[1] += m.My;
-            sum[2] += m.Mxx;
-            sum[3] += m.Myy;
-            sum[4] += m.Mxy;
-            sum[5] += m.W;
-        }
+        if (e0 + i >= m) continue;
+        linefit_terms_from(p[i].packed_x, p[i].packed_y, w, h, g4[i], &W[i], &RX[i], &RY[i]);
+        const atq_line_fit_point f = linefit_from_terms(W[i], RX[i], RY[i]);
+        sum[0] += f.Mx;
+        sum[1] += f.My;
+        sum[2] += f.Mxx;
+        sum[3] += f.Myy;
+        sum[4] += f.Mxy;
+        sum[5] += f.W;
     }
-    if (!lf) return;
+}
+
+// One chunk of up to SEG_CHUNK points of a cluster in angle order, chunk
+// positions [base, base + m) (src[i]: the point at chunk position e0 + i,
+// e0 = threadIdx.x * SEG_IPT, for e0 + i < m): writes the points to `out`
+// (if non-null) and their line-fit moments (include/line_fit_point.h:50-86),
+// prefix-summed from `carry` over the chunk, to `lf` (if non-null). Called by
+// all SEG_NT threads; tot gets the chunk's sums; lds holds the scan's
+// scratch, stage FIT_STAGE_WORDS int64 for the store exchange. Each point
+// keeps only its weight and doubled coordinates in registers; the moments
+// are rebuilt from them when written.
+ATQ_DEV void fit_chunk(const atq_cluster_point *in, atq_cluster_point *out, uint32_t start,
+                       uint32_t base, uint32_t m, const uint32_t (&src)[SEG_IPT],
+                       const uint8_t *grey, uint32_t w, uint32_t h, atq_line_fit_point *lf,
+                       const int64_t (&carry)[6], int64_t (&tot)[6], int64_t *lds,
+                       int64_t *stage) {
+    const uint32_t t = threadIdx.x, e0 = t * SEG_IPT;
+    atq_cluster_point p[SEG_IPT];
+    uint32_t W[SEG_IPT], RX[SEG_IPT], RY[SEG_IPT];
+    int64_t sum[6];
+    if (!lf) {
+        if (out)
+#pragma unroll
+            for (int i = 0; i < SEG_IPT; i++)
+                if (e0 + i < m) out[start + base + e0 + i] = in[start + src[i]];
+        return;
+    }
+    fit_terms(in, start, e0, m, src, grey, w, h, p, W, RX, RY, sum);
+    if (out) {
+#pragma unroll
+        for (int i = 0; i < SEG_IPT; i++)
+            if (e0 + i < m) out[start + base + e0 + i] = p[i];
+    }
     int64_t acc[6];
     seg_block_scan6(sum, acc, tot, lds);
 #pragma unroll
     for (int q = 0; q < 6; q++) acc[q] += carry[q] - sum[q];
+    int64_t *lfw = reinterpret_cast<int64_t *>(lf + start + base);
+    for (uint32_t r = 0; r * FIT_STAGE < m; r++) {
+        if (e0 / FIT_STAGE == r) {
+            int64_t *d = stage + 6 * (e0 - r * FIT_STAGE);
 #pragma unroll
-    for (int i = 0; i < SEG_IPT; i++) {
-        if (src[i] == NONE) continue;
-        const atq_line_fit_point m = linefit_from_terms(W[i], RX[i], RY[i]);
-        acc[0] += m.Mx;
-        acc[1] += m.My;
-        acc[2] += m.Mxx;
-        acc[3] += m.Myy;
-        acc[4] += m.Mxy;
-        acc[5] += m.W;
-        lf[start + e0 + i] = atq_line_fit_point{acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], 0, 0};
+            for (int i = 0; i < SEG_IPT; i++) {
+                const atq_line_fit_point f = linefit_from_terms(W[i], RX[i], RY[i]);
+                acc[0] += f.Mx;
+                acc[1] += f.My;
+                acc[2] += f.Mxx;
+                acc[3] += f.Myy;
+                acc[4] += f.Mxy;
+                acc[5] += f.W;
+#pragma unroll
+                for (int q = 0; q < 6; q++) d[6 * i + q] = acc[q];
+            }
+        }
+        __syncthreads();
+        const uint32_t nr = min(FIT_STAGE, m - r * FIT_STAGE);
+        int64_t *dst = lfw + 8 * (size_t)(r * FIT_STAGE);
+#pragma unroll
+        for (uint32_t k = 0; k < 4 * FIT_STAGE / SEG_NT; k++) {
+            const uint32_t piece = t + SEG_NT * k, rc = piece >> 2, qt = piece & 3;
+            if (rc >= nr) continue;
+            longlong2 v = {0, 0};
+            if (qt < 3) v = longlong2{stage[6 * rc + 2 * qt], stage[6 * rc + 2 * qt + 1]};
+            *reinterpret_cast<longlong2 *>(dst + 8 * rc + 2 * qt) = v;
+        }
+        __syncthreads();
     }
 }
 
@@ -308,14 +364,16 @@ struct AngleFinish {
     atq_line_fit_point *lf;
     ATQ_DEV void operator()(uint32_t, uint32_t start, uint32_t n, const uint64_t (&v)[SEG_IPT],
                             uint64_t *lds) const {
-        const uint32_t e0 = threadIdx.x * SEG_IPT;
         uint32_t src[SEG_IPT];
 #pragma unroll
-        for (int i = 0; i < SEG_IPT; i++) src[i] = e0 + i < n ? (uint32_t)v[i] : NONE;
+        for (int i = 0; i < SEG_IPT; i++) src[i] = (uint32_t)v[i];
         const int64_t zero[6] = {0, 0, 0, 0, 0, 0};
         int64_t tot[6];
-        fit_chunk(in, out, start, e0, src, grey, w, h, lf, zero, tot,
-                  reinterpret_cast<int64_t *>(lds));
+        // lds: the keys' half is free once the scan's barriers have passed;
+        // the scan's scratch sits at the far end of the other half
+        int64_t *l = reinterpret_cast<int64_t *>(lds);
+        fit_chunk(in, out, start, 0, n, src, grey, w, h, lf, zero, tot,
+                  l + 2 * SEG_CHUNK - (6 * SEG_NT / 64 + 6), l);
     }
 };
 
@@ -327,20 +385,73 @@ ATQ_DEV uint32_t fit_slot(uint32_t c, uint32_t start, uint32_t k) {
     return c + (start + k * SEG_CHUNK) / SEG_CHUNK;
 }
 
+// Sums of the line-fit moments of every chunk but the last of the clusters
+// longer than `min_count` points, into the slot table (the carries the
+// chunk-parallel fit adds).
+__global__ void __launch_bounds__(SEG_NT)
+k_fit_reduce(const atq_cluster_point *__restrict__ in, const uint64_t *__restrict__ keys0,
+             const uint64_t *__restrict__ keys1, const atq_cluster_extents *__restrict__ ext,
+             const uint32_t *n_clusters_dev, uint32_t cap_cl, uint32_t max_chunks,
+             uint32_t min_count, const uint8_t *__restrict__ grey, uint32_t w, uint32_t h,
+             int64_t *__restrict__ slot_sums) {
+    constexpr int NW = SEG_NT / 64;
+    __shared__ int64_t part[6][NW];
+    const uint32_t ncl = min(*n_clusters_dev, cap_cl);
+    const uint64_t items = (uint64_t)ncl * max_chunks;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t e0 = threadIdx.x * SEG_IPT;
+    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
+        const uint32_t c = (uint32_t)(it / max_chunks), k = (uint32_t)(it % max_chunks);
+        const uint32_t start = ext[c].start, n = ext[c].count, b = k * SEG_CHUNK;
+        if (n <= min_count || b + SEG_CHUNK >= n) continue;  // last chunks carry nothing on
+        const uint64_t *keys = ((seg_merge_passes(n) & 1) ? keys1 : keys0) + start;
+        uint32_t src[SEG_IPT];
+#pragma unroll
+        for (int i = 0; i < SEG_IPT; i++) src[i] = (uint32_t)keys[b + e0 + i];
+        atq_cluster_point p[SEG_IPT];
+        uint32_t W[SEG_IPT], RX[SEG_IPT], RY[SEG_IPT];
+        int64_t sum[6];
+        fit_terms(in, start, e0, SEG_CHUNK, src, grey, w, h, p, W, RX, RY, sum);
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            int64_t x = sum[q];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+            if (lane == 0) part[q][wave] = x;
+        }
+        __syncthreads();
+        if (threadIdx.x < 6) {
+            int64_t x = 0;
+#pragma unroll
+            for (int v = 0; v < NW; v++) x += part[threadIdx.x][v];
+            slot_sums[6 * (size_t)fit_slot(c, start, k) + threadIdx.x] = x;
+        }
+        __syncthreads();
+    }
+}
+
+// A work-group-uniform value kept in scalar registers.
+ATQ_DEV int64_t uniform64(int64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 // Chunk-parallel form for clusters longer than `min_count` points: every
-// chunk writes its points and moments prefix-summed within the chunk, and
-// its sums to the slot table (k_fit_carry then adds the earlier chunks).
-// slot_sums null: one work-group per cluster (max_chunks 1) walks its chunks
-// in order and carries the sums itself. `keys0/keys1` null: the input is in
-// order.
+// chunk writes its points and its moments prefix-summed from the sums of the
+// cluster's earlier chunks (slot table, k_fit_reduce). SEQ: one work-group
+// per cluster (max_chunks 1) walks its chunks in order and carries the sums
+// itself. `keys0/keys1` null: the input is in order.
+template <bool SEQ>
 __global__ void __launch_bounds__(SEG_NT)
 k_fit_chunks(const atq_cluster_point *__restrict__ in, atq_cluster_point *__restrict__ out,
              const uint64_t *__restrict__ keys0, const uint64_t *__restrict__ keys1,
              const atq_cluster_extents *__restrict__ ext, const uint32_t *n_clusters_dev,
              uint32_t cap_cl, uint32_t max_chunks, uint32_t min_count,
              const uint8_t *__restrict__ grey, uint32_t w, uint32_t h,
-             atq_line_fit_point *__restrict__ lf, int64_t *__restrict__ slot_sums) {
+             atq_line_fit_point *__restrict__ lf, const int64_t *__restrict__ slot_sums) {
     __shared__ int64_t lds[6 * SEG_NT / 64 + 6];
+    __shared__ int64_t stage[FIT_STAGE_WORDS];
     const uint32_t ncl = min(*n_clusters_dev, cap_cl);
     const uint64_t items = (uint64_t)ncl * max_chunks;
     const uint32_t e0 = threadIdx.x * SEG_IPT;
@@ -351,49 +462,22 @@ k_fit_chunks(const atq_cluster_point *__restrict__ in, atq_cluster_point *__rest
         const uint64_t *keys =
             keys0 ? ((seg_merge_passes(n) & 1) ? keys1 : keys0) + start : nullptr;
         int64_t carry[6] = {0, 0, 0, 0, 0, 0}, tot[6];
-        // no slot table: this work-group takes the cluster's chunks in turn
-        const uint32_t b_end = slot_sums ? b + 1 : n;
-        for (uint32_t bb = b; bb < b_end; bb += SEG_CHUNK) {
+        if (!SEQ && lf)
+            for (uint32_t q = 0; q < k; q++)
+#pragma unroll
+                for (int f = 0; f < 6; f++) carry[f] += slot_sums[6 * (size_t)fit_slot(c, start, q) + f];
+        for (uint32_t bb = b; bb < (SEQ ? n : b + 1); bb += SEG_CHUNK) {
             uint32_t src[SEG_IPT];
 #pragma unroll
             for (int i = 0; i < SEG_IPT; i++) {
                 const uint32_t j = bb + e0 + i;
-                src[i] = j < n ? (keys ? (uint32_t)keys[j] : j) : NONE;
+                src[i] = j < n ? (keys ? (uint32_t)keys[j] : j) : 0u;
             }
-            fit_chunk(in, out, start, bb + e0, src, grey, w, h, lf, carry, tot, lds);
+            fit_chunk(in, out, start, bb, min(SEG_CHUNK, n - bb), src, grey, w, h, lf, carry, tot,
+                      lds, stage);
+            if (SEQ)
 #pragma unroll
-            for (int q = 0; q < 6; q++) carry[q] += tot[q];
-        }
-        if (slot_sums && lf && threadIdx.x == 0)
-#pragma unroll
-            for (int q = 0; q < 6; q++) slot_sums[6 * (size_t)fit_slot(c, start, k) + q] = tot[q];
-    }
-}
-
-// Adds to every chunk k >= 1 of the clusters longer than max(min_count,
-// SEG_CHUNK) the sums of the cluster's earlier chunks.
-__global__ void __launch_bounds__(256)
-k_fit_carry(const atq_cluster_extents *__restrict__ ext, const uint32_t *n_clusters_dev,
-            uint32_t cap_cl, uint32_t max_chunks, uint32_t min_count,
-            const int64_t *__restrict__ slot_sums, atq_line_fit_point *__restrict__ lf) {
-    const uint32_t ncl = min(*n_clusters_dev, cap_cl);
-    const uint64_t items = (uint64_t)ncl * max_chunks;
-    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
-        const uint32_t c = (uint32_t)(it / max_chunks), k = (uint32_t)(it % max_chunks);
-        const uint32_t start = ext[c].start, n = ext[c].count, b = k * SEG_CHUNK;
-        if (n <= min_count || k == 0 || b >= n) continue;
-        int64_t carry[6] = {0, 0, 0, 0, 0, 0};
-        for (uint32_t q = 0; q < k; q++)
-#pragma unroll
-            for (int f = 0; f < 6; f++) carry[f] += slot_sums[6 * (size_t)fit_slot(c, start, q) + f];
-        for (uint32_t j = b + threadIdx.x; j < min(n, b + SEG_CHUNK); j += 256) {
-            atq_line_fit_point &p = lf[start + j];
-            p.Mx += carry[0];
-            p.My += carry[1];
-            p.Mxx += carry[2];
-            p.Myy += carry[3];
-            p.Mxy += carry[4];
-            p.W += carry[5];
+                for (int q = 0; q < 6; q++) carry[q] += uniform64(tot[q]);
         }
     }
 }
@@ -455,19 +539,20 @@ int launch_angle_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_poi
                      keys0, keys1);
     if (r || max_cluster <= SEG_CHUNK) return r;
     const uint32_t max_chunks = seg_max_chunks(max_cluster);
-    k_fit_chunks<<<1024, SEG_NT, 0, s>>>(in, out, keys0, keys1, ext, &counts->n_clusters, cap_cl,
-                                         max_chunks, SEG_CHUNK, grey, w, h, lf, slot_sums);
     if (lf)
-        k_fit_carry<<<1024, 256, 0, s>>>(ext, &counts->n_clusters, cap_cl, max_chunks, SEG_CHUNK,
-                                         slot_sums, lf);
+        k_fit_reduce<<<1024, SEG_NT, 0, s>>>(in, keys0, keys1, ext, &counts->n_clusters, cap_cl,
+                                             max_chunks, SEG_CHUNK, grey, w, h, slot_sums);
+    k_fit_chunks<false><<<1024, SEG_NT, 0, s>>>(in, out, keys0, keys1, ext, &counts->n_clusters,
+                                                cap_cl, max_chunks, SEG_CHUNK, grey, w, h, lf,
+                                                slot_sums);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
 int launch_cluster_fit(hipStream_t s, const atq_cluster_point *in, const atq_cluster_extents *ext,
                        const uint32_t *n_clusters_dev, uint32_t cap_cl, const uint8_t *grey,
                        uint32_t w, uint32_t h, atq_line_fit_point *lf) {
-    k_fit_chunks<<<1024, SEG_NT, 0, s>>>(in, nullptr, nullptr, nullptr, ext, n_clusters_dev, cap_cl,
-                                         1, 0, grey, w, h, lf, nullptr);
+    k_fit_chunks<true><<<1024, SEG_NT, 0, s>>>(in, nullptr, nullptr, nullptr, ext, n_clusters_dev,
+                                               cap_cl, 1, 0, grey, w, h, lf, nullptr);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 

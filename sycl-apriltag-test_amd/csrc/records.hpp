@@ -79,9 +79,50 @@ ATQ_DEV void linefit_terms(uint16_t px, uint16_t py, uint32_t w, uint32_t h,
     *ry_out = ry;
 }
 
+// linefit_terms in two steps: the four grey pixels it may read (0 where it
+// would not read), then the terms from them.
+ATQ_DEV void linefit_gather(uint16_t px, uint16_t py, uint32_t w, uint32_t h,
+                            const uint8_t *__restrict__ grey, uint32_t (&g)[4]) {
+    const uint32_t rx = x_with_dx(px) + 1;
+    const uint32_t x = rx / 2, y = rx / 2;
+    const bool in = x > 0 && x + 1 < w && y > 0 && y + 1 < h;
+    const size_t c = in ? (size_t)y * w + x : (size_t)w + 1;  // any in-image pixel when unused
+    g[0] = in ? grey[c + 1] : 0u;
+    g[1] = in ? grey[c - 1] : 0u;
+    g[2] = in ? grey[c + w] : 0u;
+    g[3] = in ? grey[c - w] : 0u;
+    (void)py;
+}
+ATQ_DEV void linefit_terms_from(uint16_t px, uint16_t py, uint32_t w, uint32_t h,
+                                const uint32_t (&g)[4], uint32_t *W_out, uint32_t *rx_out,
+                                uint32_t *ry_out) {
+    const uint32_t rx = x_with_dx(px) + 1;
+    const uint32_t ry = y_with_dy(px, py) + 1;
+    const uint32_t x = rx / 2;
+    const uint32_t y = rx / 2;
+    uint32_t W = 1;
+    if (x > 0 && x + 1 < w && y > 0 && y + 1 < h) {
+        const int gx = (int)g[0] - (int)g[1];
+        const int gy = (int)g[2] - (int)g[3];
+        W += (uint32_t)hypot_f((float)gx, (float)gy);
+    }
+    *W_out = W;
+    *rx_out = rx;
+    *ry_out = ry;
+}
+
+// W <= 1 + hypot(255, 255) = 361 and rx, ry <= 2 * 8192 + 2, so W*rx and
+// W*ry fit 32 bits and each second-order moment is one 32x32->64 product.
 ATQ_DEV atq_line_fit_point linefit_from_terms(uint32_t W, uint32_t rx, uint32_t ry) {
-    const int64_t w = W, x = rx, y = ry;
-    return atq_line_fit_point{w * x, w * y, w * x * x, w * y * y, w * x * y, w, 0, 0};
+    const uint32_t wx = W * rx, wy = W * ry;
+    return atq_line_fit_point{(int64_t)wx,
+                              (int64_t)wy,
+                              (int64_t)((uint64_t)wx * rx),
+                              (int64_t)((uint64_t)wy * ry),
+                              (int64_t)((uint64_t)wx * ry),
+                              (int64_t)W,
+                              0,
+                              0};
 }
 
 ATQ_DEV atq_line_fit_point initial_linefit(uint16_t px, uint16_t py, uint32_t w, uint32_t h,

@@ -137,13 +137,16 @@ ATQ_DEV void seg_merge_sort(uint64_t *a, uint64_t *b, uint32_t m) {
 // Finish: void operator()(uint32_t seg, uint32_t start, uint32_t n,
 // const uint64_t (&v)[SEG_IPT], uint64_t *lds) const, called by every thread
 // of the work-group with a whole sorted segment of n <= SEG_CHUNK keys in
-// registers (key e0 + i in v[i], e0 = threadIdx.x * SEG_IPT) and lds, 4096
-// u64 of scratch. Keys move between global memory and LDS coalesced.
+// registers (key e0 + i in v[i], e0 = threadIdx.x * SEG_IPT) and lds,
+// 2 * SEG_CHUNK u64 of scratch: the second half is free on entry, the first
+// half (where the keys were read from) once a barrier has passed. Keys move
+// between global memory and LDS coalesced.
 template <typename KeyGen, typename Finish, typename Ext>
 __global__ void __launch_bounds__(SEG_NT)
 k_seg_chunks(KeyGen gen, Finish fin, const Ext *__restrict__ segs, const uint32_t *n_segs_dev,
              uint32_t cap_segs, uint32_t max_chunks, uint64_t *__restrict__ keys0) {
-    __shared__ uint64_t ka[SEG_CHUNK], kb[SEG_CHUNK];
+    __shared__ uint64_t kab[2 * SEG_CHUNK];
+    uint64_t *const ka = kab, *const kb = kab + SEG_CHUNK;
     const uint32_t nseg = min(*n_segs_dev, cap_segs);
     const uint64_t items = (uint64_t)nseg * max_chunks;
     const uint32_t t = threadIdx.x, e0 = t * SEG_IPT;
@@ -161,7 +164,7 @@ k_seg_chunks(KeyGen gen, Finish fin, const Ext *__restrict__ segs, const uint32_
             uint64_t v[SEG_IPT];
 #pragma unroll
             for (int i = 0; i < SEG_IPT; i++) v[i] = ka[e0 + i];
-            fin(sg, start, n, v, kb);
+            fin(sg, start, n, v, kab);
             __syncthreads();
             continue;
         }
