@@ -341,9 +341,25 @@ k_bnd_gather(const uint64_t *__restrict__ tile_keys, const atq_boundary_point *_
              uint64_t *__restrict__ keys, atq_boundary_point *__restrict__ points) {
     const uint32_t n = tile_counts[blockIdx.x], off = tile_off[blockIdx.x];
     const size_t base = (size_t)blockIdx.x * (4 * BND_TILE);
-    for (uint32_t i = threadIdx.x; i < n; i += 256) {
-        keys[off + i] = tile_keys[base + i];
-        points[off + i] = tile_points[base + i];
+    for (uint32_t i0 = threadIdx.x; i0 < n; i0 += 4 * 256) {  // 4 records' loads in flight
+        uint64_t k[4];
+        atq_boundary_point p[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t i = i0 + 256 * q;
+            if (i < n) {
+                k[q] = tile_keys[base + i];
+                p[q] = tile_points[base + i];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t i = i0 + 256 * q;
+            if (i < n) {
+                keys[off + i] = k[q];
+                points[off + i] = p[q];
+            }
+        }
     }
 }
 

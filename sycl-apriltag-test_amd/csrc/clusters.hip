@@ -33,11 +33,16 @@ struct HeadFlag {
         return i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift);
     }
 };
+// Segment of every record, first record of every segment, and the end
+// sentinel seg_start[S] = N written by the last record.
 struct HeadOut {
     uint32_t *seg_id, *seg_start;
+    const uint32_t *n_dev;
+    uint32_t cap;
     ATQ_DEV void operator()(uint32_t i, uint32_t pre, uint32_t v) const {
         seg_id[i] = pre + v - 1;
         if (v) seg_start[pre] = i;
+        if (i + 1 == min(*n_dev, cap)) seg_start[pre + v] = i + 1;
     }
 };
 
@@ -65,13 +70,6 @@ struct ExtentOut {
 };
 
 }  // namespace
-
-// seg_start[S] = N (end sentinel)
-__global__ void k_seg_close(const uint32_t *n_dev, const atq_counts *counts, uint32_t cap,
-                            uint32_t *seg_start) {
-    uint32_t n = min(*n_dev, cap);
-    seg_start[counts->n_pairs] = n;
-}
 
 // One work-group per segment: reduce ClusterBounds and apply
 // ValidBlobFilter. Each thread loads 4 records per round before using them.
@@ -121,11 +119,14 @@ struct BoundsAcc {
     }
 };
 
-__global__ void __launch_bounds__(256)
+constexpr int SB_NT = 1024, SB_IPT = 8;  // k_seg_bounds: a segment's records per round
+
+__global__ void __launch_bounds__(SB_NT)
 k_seg_bounds(const atq_boundary_point *__restrict__ pts, const uint32_t *__restrict__ seg_start,
              const atq_counts *counts, uint32_t w, uint32_t h, uint32_t *__restrict__ valid,
              atq_cluster_bounds *__restrict__ seg_bounds) {
-    __shared__ BoundsAcc part[4];
+    constexpr int NW = SB_NT / 64;
+    __shared__ BoundsAcc part[NW];
     const uint32_t S = counts->n_pairs;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t max_px = 2ull * (2ull * w + 2ull * h);
@@ -137,23 +138,24 @@ k_seg_bounds(const atq_boundary_point *__restrict__ pts, const uint32_t *__restr
         }
         BoundsAcc acc;
         acc.init();
-        for (uint32_t i0 = b + threadIdx.x; i0 < e; i0 += 4 * 256) {
-            atq_boundary_point q[4];
+        // every load of a round in flight before the first is used
+        for (uint32_t i0 = b + threadIdx.x; i0 < e; i0 += SB_IPT * SB_NT) {
+            atq_boundary_point q[SB_IPT];
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const uint32_t i = i0 + 256 * k;
+            for (int k = 0; k < SB_IPT; k++) {
+                const uint32_t i = i0 + SB_NT * k;
                 q[k] = i < e ? pts[i] : atq_boundary_point{0, 0};
             }
 #pragma unroll
-            for (int k = 0; k < 4; k++)
-                if (i0 + 256 * k < e) acc.add(q[k]);
+            for (int k = 0; k < SB_IPT; k++)
+                if (i0 + SB_NT * k < e) acc.add(q[k]);
         }
         acc.wave_reduce();
         if (lane == 0) part[wave] = acc;
         __syncthreads();
         if (threadIdx.x == 0) {
 #pragma unroll
-            for (int q = 1; q < 4; q++) acc.merge(part[q]);
+            for (int q = 1; q < NW; q++) acc.merge(part[q]);
             atq_cluster_bounds cb;
             cb.x_min = (uint16_t)acc.xmin;
             cb.x_max = (uint16_t)acc.xmax;
@@ -385,26 +387,24 @@ ATQ_DEV uint32_t fit_slot(uint32_t c, uint32_t start, uint32_t k) {
     return c + (start + k * SEG_CHUNK) / SEG_CHUNK;
 }
 
-// Sums of the line-fit moments of every chunk but the last of the clusters
-// longer than `min_count` points, into the slot table (the carries the
-// chunk-parallel fit adds).
+// Sums of the line-fit moments of every chunk but the last of the listed
+// long clusters, into the slot table (the carries the chunk-parallel fit
+// adds).
 __global__ void __launch_bounds__(SEG_NT)
 k_fit_reduce(const atq_cluster_point *__restrict__ in, const uint64_t *__restrict__ keys0,
              const uint64_t *__restrict__ keys1, const atq_cluster_extents *__restrict__ ext,
-             const uint32_t *n_clusters_dev, uint32_t cap_cl, uint32_t max_chunks,
-             uint32_t min_count, const uint8_t *__restrict__ grey, uint32_t w, uint32_t h,
-             int64_t *__restrict__ slot_sums) {
+             SegLong lng, uint32_t max_chunks, const uint8_t *__restrict__ grey, uint32_t w,
+             uint32_t h, int64_t *__restrict__ slot_sums) {
     constexpr int NW = SEG_NT / 64;
     __shared__ int64_t part[6][NW];
-    const uint32_t ncl = min(*n_clusters_dev, cap_cl);
-    const uint64_t items = (uint64_t)ncl * max_chunks;
+    const uint64_t items = (uint64_t)min(*lng.n, lng.cap) * max_chunks;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t e0 = threadIdx.x * SEG_IPT;
     for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
-        const uint32_t c = (uint32_t)(it / max_chunks), k = (uint32_t)(it % max_chunks);
+        const uint32_t c = lng.list[it / max_chunks], k = (uint32_t)(it % max_chunks);
         const uint32_t start = ext[c].start, n = ext[c].count, b = k * SEG_CHUNK;
-        if (n <= min_count || b + SEG_CHUNK >= n) continue;  // last chunks carry nothing on
-        const uint64_t *keys = ((seg_merge_passes(n) & 1) ? keys1 : keys0) + start;
+        if (b + SEG_CHUNK >= n) continue;  // last chunks carry nothing on
+        const uint64_t *keys = seg_sorted_keys(n, keys0, keys1) + start;
         uint32_t src[SEG_IPT];
 #pragma unroll
         for (int i = 0; i < SEG_IPT; i++) src[i] = (uint32_t)keys[b + e0 + i];
@@ -437,30 +437,30 @@ ATQ_DEV int64_t uniform64(int64_t v) {
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-// Chunk-parallel form for clusters longer than `min_count` points: every
-// chunk writes its points and its moments prefix-summed from the sums of the
-// cluster's earlier chunks (slot table, k_fit_reduce). SEQ: one work-group
-// per cluster (max_chunks 1) walks its chunks in order and carries the sums
-// itself. `keys0/keys1` null: the input is in order.
+// Chunk-parallel form for the listed long clusters (their sorted keys in
+// keys0/keys1): every chunk writes its points and its moments prefix-summed
+// from the sums of the cluster's earlier chunks (slot table, k_fit_reduce).
+// SEQ: every cluster (n_clusters_dev) in input order, one work-group per
+// cluster walking its chunks and carrying the sums itself.
 template <bool SEQ>
 __global__ void __launch_bounds__(SEG_NT)
 k_fit_chunks(const atq_cluster_point *__restrict__ in, atq_cluster_point *__restrict__ out,
              const uint64_t *__restrict__ keys0, const uint64_t *__restrict__ keys1,
              const atq_cluster_extents *__restrict__ ext, const uint32_t *n_clusters_dev,
-             uint32_t cap_cl, uint32_t max_chunks, uint32_t min_count,
-             const uint8_t *__restrict__ grey, uint32_t w, uint32_t h,
-             atq_line_fit_point *__restrict__ lf, const int64_t *__restrict__ slot_sums) {
+             uint32_t cap_cl, SegLong lng, uint32_t max_chunks, const uint8_t *__restrict__ grey,
+             uint32_t w, uint32_t h, atq_line_fit_point *__restrict__ lf,
+             const int64_t *__restrict__ slot_sums) {
     __shared__ int64_t lds[6 * SEG_NT / 64 + 6];
     __shared__ int64_t stage[FIT_STAGE_WORDS];
-    const uint32_t ncl = min(*n_clusters_dev, cap_cl);
-    const uint64_t items = (uint64_t)ncl * max_chunks;
+    const uint64_t items = SEQ ? (uint64_t)min(*n_clusters_dev, cap_cl)
+                               : (uint64_t)min(*lng.n, lng.cap) * max_chunks;
     const uint32_t e0 = threadIdx.x * SEG_IPT;
     for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
-        const uint32_t c = (uint32_t)(it / max_chunks), k = (uint32_t)(it % max_chunks);
+        const uint32_t c = SEQ ? (uint32_t)it : lng.list[it / max_chunks];
+        const uint32_t k = SEQ ? 0u : (uint32_t)(it % max_chunks);
         const uint32_t start = ext[c].start, n = ext[c].count, b = k * SEG_CHUNK;
-        if (n <= min_count || b >= n) continue;
-        const uint64_t *keys =
-            keys0 ? ((seg_merge_passes(n) & 1) ? keys1 : keys0) + start : nullptr;
+        if (b >= n) continue;
+        const uint64_t *keys = SEQ ? nullptr : seg_sorted_keys(n, keys0, keys1) + start;
         int64_t carry[6] = {0, 0, 0, 0, 0, 0}, tot[6];
         if (!SEQ && lf)
             for (uint32_t q = 0; q < k; q++)
@@ -498,18 +498,15 @@ int launch_run_extents(hipStream_t s, const uint32_t *cidx, const uint32_t *n_de
 int launch_segments(hipStream_t s, const uint64_t *keys, uint32_t key_shift, const uint32_t *n_dev,
                     atq_counts *counts, uint32_t cap, uint32_t *seg_id, uint32_t *seg_start,
                     ScanCtx &sc) {
-    int r = device_scan(s, n_dev, cap, HeadFlag{keys, key_shift}, HeadOut{seg_id, seg_start},
-                        sc, &counts->n_pairs);
-    if (r) return r;
-    k_seg_close<<<1, 1, 0, s>>>(n_dev, counts, cap, seg_start);
-    return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
+    return device_scan(s, n_dev, cap, HeadFlag{keys, key_shift},
+                       HeadOut{seg_id, seg_start, n_dev, cap}, sc, &counts->n_pairs);
 }
 
 int launch_bounds_filter(hipStream_t s, const atq_boundary_point *pts, const uint32_t *seg_start,
                          atq_counts *counts, uint32_t cap, uint32_t w, uint32_t h, uint32_t *valid,
                          atq_cluster_bounds *seg_bounds, uint32_t *seg_cluster,
                          atq_cluster_bounds *bounds, ScanCtx &sc) {
-    k_seg_bounds<<<2048, 256, 0, s>>>(pts, seg_start, counts, w, h, valid, seg_bounds);
+    k_seg_bounds<<<1024, SB_NT, 0, s>>>(pts, seg_start, counts, w, h, valid, seg_bounds);
     return device_scan(s, &counts->n_pairs, cap, ValidFlag{valid},
                        ValidOut{seg_cluster, seg_bounds, bounds}, sc, &counts->n_clusters);
 }
@@ -530,21 +527,19 @@ int launch_assign(hipStream_t s, const atq_boundary_point *pts, const uint32_t *
 }
 
 int launch_angle_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out,
-                     const atq_cluster_extents *ext, const uint32_t *cidx, const atq_counts *counts,
-                     uint32_t cap, uint32_t cap_cl, uint32_t max_cluster, const uint8_t *grey,
-                     uint32_t w, uint32_t h, atq_line_fit_point *lf, uint64_t *keys0,
-                     uint64_t *keys1, int64_t *slot_sums) {
+                     const atq_cluster_extents *ext, const atq_counts *counts, uint32_t cap_cl,
+                     uint32_t max_cluster, const uint8_t *grey, uint32_t w, uint32_t h,
+                     atq_line_fit_point *lf, uint64_t *keys0, uint64_t *keys1, int64_t *slot_sums,
+                     SegLong lng) {
     int r = seg_sort(s, AngleKey{in}, AngleFinish{in, out, grey, w, h, lf}, ext,
-                     &counts->n_clusters, cap_cl, cidx, &counts->n_cluster_points, cap, max_cluster,
-                     keys0, keys1);
+                     &counts->n_clusters, cap_cl, max_cluster, keys0, keys1, lng);
     if (r || max_cluster <= SEG_CHUNK) return r;
     const uint32_t max_chunks = seg_max_chunks(max_cluster);
     if (lf)
-        k_fit_reduce<<<1024, SEG_NT, 0, s>>>(in, keys0, keys1, ext, &counts->n_clusters, cap_cl,
-                                             max_chunks, SEG_CHUNK, grey, w, h, slot_sums);
-    k_fit_chunks<false><<<1024, SEG_NT, 0, s>>>(in, out, keys0, keys1, ext, &counts->n_clusters,
-                                                cap_cl, max_chunks, SEG_CHUNK, grey, w, h, lf,
-                                                slot_sums);
+        k_fit_reduce<<<512, SEG_NT, 0, s>>>(in, keys0, keys1, ext, lng, max_chunks, grey, w, h,
+                                            slot_sums);
+    k_fit_chunks<false><<<512, SEG_NT, 0, s>>>(in, out, keys0, keys1, ext, &counts->n_clusters,
+                                               cap_cl, lng, max_chunks, grey, w, h, lf, slot_sums);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
@@ -552,7 +547,7 @@ int launch_cluster_fit(hipStream_t s, const atq_cluster_point *in, const atq_clu
                        const uint32_t *n_clusters_dev, uint32_t cap_cl, const uint8_t *grey,
                        uint32_t w, uint32_t h, atq_line_fit_point *lf) {
     k_fit_chunks<true><<<1024, SEG_NT, 0, s>>>(in, nullptr, nullptr, nullptr, ext, n_clusters_dev,
-                                               cap_cl, 1, 0, grey, w, h, lf, nullptr);
+                                               cap_cl, SegLong{}, 1, grey, w, h, lf, nullptr);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 

@@ -1,6 +1,7 @@
 // clusters.hpp - launchers of the cluster and fit_quad stages.
 #pragma once
 #include "records.hpp"
+#include "segsort.hpp"
 #include "sort.hpp"
 
 namespace atq {
@@ -23,14 +24,15 @@ int launch_assign(hipStream_t s, const atq_boundary_point *pts, const uint32_t *
                   const uint32_t *n_dev, uint32_t cap, atq_cluster_point *cpts, uint32_t *cidx);
 // Angle sort of every cluster's points (src/main.cpp:737-750), the points
 // in that order to `out` and their line-fit prefix sums to `lf` (may be
-// null; src/main.cpp:768-801). cidx: cluster of every point; max_cluster
-// bounds the cluster length. keys0/keys1: two key buffers of cap entries;
-// slot_sums: 6 * (cap_cl + cap / SEG_CHUNK + 2) int64.
+// null; src/main.cpp:768-801). max_cluster bounds the cluster length.
+// keys0/keys1: two key buffers of cap entries; slot_sums: 6 * (cap_cl + cap
+// / SEG_CHUNK + 2) int64; lng: a long-segment list of seg_long_cap(cap)
+// entries whose count is zero.
 int launch_angle_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out,
-                     const atq_cluster_extents *ext, const uint32_t *cidx, const atq_counts *counts,
-                     uint32_t cap, uint32_t cap_cl, uint32_t max_cluster, const uint8_t *grey,
-                     uint32_t w, uint32_t h, atq_line_fit_point *lf, uint64_t *keys0,
-                     uint64_t *keys1, int64_t *slot_sums);
+                     const atq_cluster_extents *ext, const atq_counts *counts, uint32_t cap_cl,
+                     uint32_t max_cluster, const uint8_t *grey, uint32_t w, uint32_t h,
+                     atq_line_fit_point *lf, uint64_t *keys0, uint64_t *keys1, int64_t *slot_sums,
+                     SegLong lng);
 // Line-fit prefix sums of points already in angle order (one work-group per
 // cluster).
 int launch_cluster_fit(hipStream_t s, const atq_cluster_point *in, const atq_cluster_extents *ext,
@@ -53,10 +55,13 @@ int launch_set_u32(hipStream_t s, uint32_t *p, uint32_t v);
 // p[set_idx] gets set_val.
 int launch_zero_u32(hipStream_t s, uint32_t *p, const uint32_t *n_dev, uint32_t cap,
                     uint32_t set_idx = 0xFFFFFFFFu, uint32_t set_val = 0);
+// Corners compacted and grouped into per-cluster peak segments, each in
+// (error descending, index) order; scratch: 2 * cap u64; lng as for
+// launch_angle_fit.
 int launch_corner_segments(hipStream_t s, const atq_corner *sparse, const uint32_t *n_sparse_dev,
                            uint32_t cap, atq_counts *counts, uint32_t cap_cl, uint32_t max_seg,
-                           atq_peak_extents *peaks, atq_corner *compacted, uint32_t *seg_of,
-                           uint64_t *scratch, ScanCtx &sc);
+                           atq_peak_extents *peaks, atq_corner *compacted, uint64_t *scratch,
+                           SegLong lng, ScanCtx &sc);
 int launch_do_indexing(hipStream_t s, const atq_peak_extents *peaks, const uint32_t *n_seg_dev,
                        uint32_t cap, const atq_corner *corners, const atq_line_fit_point *lf,
                        const uint32_t *n_points_dev, uint32_t n_points_host,

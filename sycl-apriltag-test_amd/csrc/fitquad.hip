@@ -232,10 +232,8 @@ struct SegHead {
 };
 struct SegStart {
     atq_peak_extents *peaks;
-    uint32_t *seg_of;
     ATQ_DEV void operator()(uint32_t i, uint32_t pre, uint32_t v) const {
         if (v) peaks[pre].start = i;
-        seg_of[i] = pre + v - 1;
     }
 };
 }  // namespace
@@ -275,24 +273,33 @@ struct CornerFinish {
     }
 };
 
-// Writes the corners of segments longer than SEG_CHUNK back in key order.
-// The cluster index is read from the segment's first corner, which every
-// writer of that slot rewrites with the same cluster index.
+// Writes the corners of the listed long segments back in key order. The
+// cluster index is read from the segment's first corner, which every writer
+// of that slot rewrites with the same cluster index.
 __global__ void __launch_bounds__(256)
-k_corner_emit(const atq_peak_extents *__restrict__ peaks, const atq_counts *counts, uint32_t cap,
-              uint32_t max_chunks, const uint64_t *__restrict__ keys0,
-              const uint64_t *__restrict__ keys1, atq_corner *out) {
-    const uint32_t E = min(counts->n_segments, cap);
-    const uint64_t items = (uint64_t)E * max_chunks;
+k_corner_emit(const atq_peak_extents *__restrict__ peaks, SegLong lng, uint32_t max_chunks,
+              const uint64_t *__restrict__ keys0, const uint64_t *__restrict__ keys1,
+              atq_corner *out) {
+    const uint64_t items = (uint64_t)min(*lng.n, lng.cap) * max_chunks;
     for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
-        const uint32_t e = (uint32_t)(it / max_chunks), ch = (uint32_t)(it % max_chunks);
+        const uint32_t e = lng.list[it / max_chunks], ch = (uint32_t)(it % max_chunks);
         const uint32_t cs = peaks[e].start, m = peaks[e].count;
-        if (m <= SEG_CHUNK || ch * SEG_CHUNK >= m) continue;
+        if (ch * SEG_CHUNK >= m) continue;
         const uint32_t c = out[cs].cluster_index;
-        const uint64_t *k = ((seg_merge_passes(m) & 1) ? keys1 : keys0) + cs;
-        for (uint32_t r = ch * SEG_CHUNK + threadIdx.x; r < min(m, (ch + 1) * SEG_CHUNK); r += 256) {
-            const uint64_t kk = k[r];
-            out[cs + r] = atq_corner{(uint32_t)kk, c, float_from_key(~(uint32_t)(kk >> 32))};
+        const uint64_t *k = seg_sorted_keys(m, keys0, keys1) + cs;
+        const uint32_t end = min(m, (ch + 1) * SEG_CHUNK);
+        constexpr int PER = SEG_CHUNK / 256;
+        uint64_t kk[PER];
+#pragma unroll
+        for (int q = 0; q < PER; q++) {  // all loads in flight before the stores
+            const uint32_t r = ch * SEG_CHUNK + threadIdx.x + 256 * q;
+            kk[q] = r < end ? k[r] : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const uint32_t r = ch * SEG_CHUNK + threadIdx.x + 256 * q;
+            if (r < end)
+                out[cs + r] = atq_corner{(uint32_t)kk[q], c, float_from_key(~(uint32_t)(kk[q] >> 32))};
         }
     }
 }
@@ -469,21 +476,20 @@ int launch_fit_lines(hipStream_t s, const atq_line_fit_point *lf, const uint32_t
 
 int launch_corner_segments(hipStream_t s, const atq_corner *sparse, const uint32_t *n_sparse_dev,
                            uint32_t cap, atq_counts *counts, uint32_t cap_cl, uint32_t max_seg,
-                           atq_peak_extents *peaks, atq_corner *compacted, uint32_t *seg_of,
-                           uint64_t *scratch, ScanCtx &sc) {
+                           atq_peak_extents *peaks, atq_corner *compacted, uint64_t *scratch,
+                           SegLong lng, ScanCtx &sc) {
     int r = device_scan(s, n_sparse_dev, cap, CornerFlag{sparse}, CornerOut{sparse, compacted},
                         sc, &counts->n_corners);
     if (r) return r;
-    r = device_scan(s, &counts->n_corners, cap, SegHead{compacted}, SegStart{peaks, seg_of}, sc,
+    r = device_scan(s, &counts->n_corners, cap, SegHead{compacted}, SegStart{peaks}, sc,
                     &counts->n_segments);
     if (r) return r;
-    k_peak_close<<<min(1024u, atq_div_up(cap_cl, 256) + 1), 256, 0, s>>>(counts, cap_cl, peaks);
+    k_peak_close<<<min(64u, atq_div_up(cap_cl, 256) + 1), 256, 0, s>>>(counts, cap_cl, peaks);
     r = seg_sort(s, CornerKey{compacted}, CornerFinish{compacted}, peaks, &counts->n_segments,
-                 cap_cl, seg_of, &counts->n_corners, cap, max_seg, scratch, scratch + cap);
+                 cap_cl, max_seg, scratch, scratch + cap, lng);
     if (r || max_seg <= SEG_CHUNK) return r;
-    const uint32_t max_chunks = seg_max_chunks(max_seg);
-    k_corner_emit<<<2048, 256, 0, s>>>(peaks, counts, cap_cl, max_chunks, scratch, scratch + cap,
-                                       compacted);
+    k_corner_emit<<<256, 256, 0, s>>>(peaks, lng, seg_max_chunks(max_seg), scratch, scratch + cap,
+                                      compacted);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
@@ -492,7 +498,7 @@ int launch_do_indexing(hipStream_t s, const atq_peak_extents *peaks, const uint3
                        const uint32_t *n_points_dev, uint32_t n_points_host,
                        const atq_cluster_extents *ext, atq_fitted_quad *fitted,
                        uint32_t *oob_counter) {
-    k_do_indexing<<<2048, 256, 0, s>>>(peaks, n_seg_dev, cap, corners, lf, n_points_dev,
+    k_do_indexing<<<1024, 256, 0, s>>>(peaks, n_seg_dev, cap, corners, lf, n_points_dev,
                                        n_points_host, ext, fitted, oob_counter);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
@@ -500,7 +506,7 @@ int launch_do_indexing(hipStream_t s, const atq_peak_extents *peaks, const uint3
 int launch_quads(hipStream_t s, const atq_fitted_quad *fitted, const uint32_t *n_fitted_dev,
                  uint32_t cap, atq_quad_corners *tmp, uint32_t *flags, atq_quad_corners *out,
                  uint32_t *n_out_dev, ScanCtx &sc, int min_tag_width) {
-    k_quads<<<min(1024u, atq_div_up(cap, 256) + 1), 256, 0, s>>>(fitted, n_fitted_dev, cap, tmp,
+    k_quads<<<min(256u, atq_div_up(cap, 256) + 1), 256, 0, s>>>(fitted, n_fitted_dev, cap, tmp,
                                                                   flags, min_tag_width);
     return device_scan(s, n_fitted_dev, cap, QuadFlag{flags}, QuadOut{tmp, out}, sc,
                        n_out_dev);

@@ -59,6 +59,7 @@ size_t Workspace::carve(char *base, const Caps &c) {
     seg_bounds = carve_one<atq_cluster_bounds>(base, off, c.cap);
     cpts_tmp = carve_one<atq_cluster_point>(base, off, c.cap);
     fit_sums = carve_one<int64_t>(base, off, 6 * ((size_t)c.cap_cl + c.cap / SEG_CHUNK + 2));
+    long_list = carve_one<uint32_t>(base, off, 2 * (size_t)seg_long_cap(c.cap));
     tiles = carve_one<uint32_t>(base, off, 2 * ((size_t)c.npx / BOUNDARY_TILE_PX + 1));
     qtmp = carve_one<atq_quad_corners>(base, off, c.cap_cl);
     qflags = carve_one<uint32_t>(base, off, c.cap_cl);
@@ -132,6 +133,7 @@ size_t carve_detector(atq_detector *d, char *base) {
     d->scan_flags = carve_one<uint64_t>(base, off, 2 * (size_t)scan_flag_count(c));
     d->pair_keys = carve_one<uint64_t>(base, off, c.pair_slots);
     d->pair_cnt = carve_one<uint32_t>(base, off, c.pair_slots);
+    d->long_n = carve_one<uint32_t>(base, off, 2);
     d->counts = carve_one<atq_counts>(base, off, 1);
     off = (off + 255) & ~(size_t)255;
     size_t ws_bytes = d->ws.carve(base ? base + off : nullptr, c);
@@ -210,16 +212,16 @@ int run_pipeline(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t 
     // a valid cluster holds at most 2 * (2w + 2h) points (valid_blob)
     const uint32_t max_cluster = (uint32_t)std::min<size_t>(c.cap, 4 * (w + h));
     uint64_t *ak = ws.angle_keys();
-    TRY(launch_angle_fit(s, ws.cpts_tmp, d->cpts, d->cext, d->cidx, cnt, c.cap, c.cap_cl,
-                         max_cluster, grey, (uint32_t)w, (uint32_t)h, d->lf, ak, ak + c.cap,
-                         ws.fit_sums));
+    TRY(launch_angle_fit(s, ws.cpts_tmp, d->cpts, d->cext, cnt, c.cap_cl, max_cluster, grey,
+                         (uint32_t)w, (uint32_t)h, d->lf, ak, ak + c.cap, ws.fit_sums,
+                         ws.long_segs(0, &d->long_n[0], c)));
     TRY(mark(1 + ATQ_STAGE_ANGLE_FIT));
     TRY(launch_fit_lines(s, d->lf, d->cidx, d->cext, &cnt->n_cluster_points, c.cap, d->sparse,
                          nullptr, reinterpret_cast<float *>(ws.valid)));
     TRY(mark(1 + ATQ_STAGE_FIT_LINES));
     TRY(launch_corner_segments(s, d->sparse, &cnt->n_cluster_points, c.cap, cnt, c.cap_cl,
-                               max_cluster, d->peaks, d->corners, ws.seg_id, ws.angle_keys(),
-                               sc));
+                               max_cluster, d->peaks, d->corners, ws.angle_keys(),
+                               ws.long_segs(1, &d->long_n[1], c), sc));
     TRY(mark(1 + ATQ_STAGE_CORNERS));
     TRY(launch_do_indexing(s, d->peaks, &cnt->n_segments, c.cap_cl, d->corners, d->lf,
                            &cnt->n_cluster_points, 0, d->cext, d->fitted, &cnt->oob_reads));
@@ -313,9 +315,11 @@ int atq_cluster_points_sort(void *stream, const atq_boundary_point *points,
     // a valid cluster holds at most 2 * (2w + 2h) points (valid_blob), and
     // w + h <= npx / 8 + 8 for frames of at least 8 pixels a side
     const uint32_t max_cluster = std::min<uint32_t>(c.cap, c.npx / 2 + 32);
-    return launch_angle_fit(s, ws.cpts_tmp, out_points, out_extents, out_cluster_index, counts_dev,
-                            c.cap, c.cap_cl, max_cluster, nullptr, 0, 0, nullptr, ak, ak + c.cap,
-                            ws.fit_sums);
+    uint32_t *n_long = &ws.counts->reserved[1];
+    TRY(launch_set_u32(s, n_long, 0));
+    return launch_angle_fit(s, ws.cpts_tmp, out_points, out_extents, counts_dev, c.cap_cl,
+                            max_cluster, nullptr, 0, 0, nullptr, ak, ak + c.cap, ws.fit_sums,
+                            ws.long_segs(0, n_long, c));
 }
 
 int atq_linefit_prefix(void *stream, const atq_cluster_point *points,
@@ -354,10 +358,11 @@ int atq_corner_segments(void *stream, const atq_corner *found_corners, atq_count
     ws.carve(static_cast<char *>(workspace), c);
     ScanCtx sc;
     sc.scratch = ws.scan;
+    uint32_t *n_long = &ws.counts->reserved[1];
+    TRY(launch_set_u32(s, n_long, 0));
     return launch_corner_segments(s, found_corners, &counts_dev->n_cluster_points, c.cap, counts_dev,
                                   c.cap_cl, std::min<uint32_t>(c.cap, c.npx / 2 + 32), segments,
-                                  compacted, ws.seg_id,
-                                  ws.angle_keys(), sc);
+                                  compacted, ws.angle_keys(), ws.long_segs(1, n_long, c), sc);
 }
 
 int atq_do_indexing(void *stream, const atq_peak_extents *extents, size_t extents_count,

@@ -33,6 +33,7 @@ struct Workspace {
     atq_cluster_bounds *seg_bounds;
     atq_cluster_point *cpts_tmp;
     int64_t *fit_sums;  // per (cluster, chunk) line-fit sums of long clusters
+    uint32_t *long_list;  // long segments of the angle and corner sorts (2 lists)
     uint32_t *tiles;
     atq_quad_corners *qtmp;
     uint32_t *qflags;
@@ -44,6 +45,12 @@ struct Workspace {
     // they reuse seg_bounds (cap entries of 32 bytes), which is dead once the
     // points are assigned to clusters.
     uint64_t *angle_keys() const { return reinterpret_cast<uint64_t *>(seg_bounds); }
+
+    // Long-segment list k (0: angle sort, 1: corner sort) counted at n.
+    SegLong long_segs(int k, uint32_t *n, const Caps &c) const {
+        const uint32_t cap = seg_long_cap(c.cap);
+        return SegLong{n, long_list + (size_t)k * cap, cap};
+    }
 
     // Carves the layout from `base` (nullptr: only measures). Returns bytes.
     size_t carve(char *base, const Caps &c);
@@ -73,6 +80,7 @@ struct atq_detector {
     uint64_t *scan_flags;  // two look-back flag arrays, zeroed before each frame
     uint64_t *pair_keys;   // pair table, zeroed with counts before each frame
     uint32_t *pair_cnt;
+    uint32_t *long_n;      // long-segment counts of the two segment sorts, zeroed likewise
     atq_counts *counts;
     atq::Workspace ws;
     size_t w, h;

@@ -124,8 +124,9 @@ k_scan_apply(const uint32_t *count, uint32_t count_cap, F f, const uint32_t *par
 // which *error gets OVF_SCAN and the scan's results are void.
 // Each call uses one of two flag arrays and clears the other for the next
 // call (both must be zero before the first call of a sequence).
+constexpr int LB_THREADS = 256;
 constexpr int LB_ITEMS = 8;
-constexpr uint32_t LB_TILE = SCAN_THREADS * LB_ITEMS;
+constexpr uint32_t LB_TILE = LB_THREADS * LB_ITEMS;
 constexpr uint64_t LB_SUM = 1ull << 62, LB_PREFIX = 2ull << 62;
 constexpr uint32_t LB_SPIN_LIMIT = 1u << 22;
 
@@ -169,13 +170,13 @@ ATQ_DEV uint32_t lb_lookback(uint64_t *flags, uint32_t p, uint32_t tile_sum, uin
 }
 
 template <typename F, typename O>
-__global__ void __launch_bounds__(SCAN_THREADS)
+__global__ void __launch_bounds__(LB_THREADS)
 k_scan_lookback(const uint32_t *count, uint32_t count_cap, F f, O o, uint64_t *flags,
                 uint64_t *next_flags, uint32_t n_flags, uint32_t *total, uint32_t *error) {
-    __shared__ uint32_t lds4[4];
+    __shared__ uint32_t lds16[LB_THREADS / 64];
     __shared__ uint32_t s_prefix;
-    for (uint32_t i = blockIdx.x * SCAN_THREADS + threadIdx.x; i < n_flags;
-         i += gridDim.x * SCAN_THREADS)
+    for (uint32_t i = blockIdx.x * LB_THREADS + threadIdx.x; i < n_flags;
+         i += gridDim.x * LB_THREADS)
         next_flags[i] = 0;
     const uint32_t n = count ? min(*count, count_cap) : count_cap;
     const uint32_t p = blockIdx.x, b = p * LB_TILE;
@@ -191,7 +192,7 @@ k_scan_lookback(const uint32_t *count, uint32_t count_cap, F f, O o, uint64_t *f
         sum += v[k];
     }
     uint32_t tile_sum;
-    const uint32_t ex = block_exclusive_scan<uint32_t>(sum, lds4, &tile_sum);
+    const uint32_t ex = block_exclusive_scan<uint32_t, LB_THREADS>(sum, lds16, &tile_sum);
     if (threadIdx.x < 64) {
         const uint32_t excl = lb_lookback(flags, p, tile_sum, error);
         if (threadIdx.x == 0) s_prefix = excl;
@@ -232,8 +233,8 @@ int device_scan(hipStream_t s, const uint32_t *count, uint32_t count_cap, F f, O
         uint64_t *cur = sc.flags + (size_t)(sc.calls & 1) * sc.flag_count;
         uint64_t *next = sc.flags + (size_t)((sc.calls + 1) & 1) * sc.flag_count;
         sc.calls++;
-        k_scan_lookback<F, O><<<tiles, SCAN_THREADS, 0, s>>>(count, count_cap, f, o, cur, next,
-                                                             sc.flag_count, total, sc.error);
+        k_scan_lookback<F, O><<<tiles, LB_THREADS, 0, s>>>(count, count_cap, f, o, cur, next,
+                                                           sc.flag_count, total, sc.error);
         return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
     }
     k_scan_partials<F><<<SCAN_GROUPS, SCAN_THREADS, 0, s>>>(count, count_cap, f, sc.scratch);

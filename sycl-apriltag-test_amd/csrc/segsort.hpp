@@ -8,10 +8,12 @@
 // levels for a full chunk. Keys must be distinct (callers put the element's
 // position in the low bits), so the result is a stable sort. A segment that
 // fits one chunk is finished right there by the caller's Finish functor.
-// Longer segments are written to keys0 and completed by merge passes in
-// which every key finds its rank in the partner run by binary search; runs
-// double each pass, so a segment of n keys needs seg_merge_passes(n) passes
-// and ends in keys1 when that number is odd, else in keys0.
+// Longer segments are written to keys0 and named in a device-side list of
+// long segments. One work-group then merges each SEG_LDS_KEYS block of a
+// listed segment in LDS (keys0 -> keys1); segments longer than that take
+// global merge passes, in which every key finds its rank in the partner run
+// by binary search, alternating keys1 -> keys0 -> keys1 ...;
+// seg_sorted_keys(n, ...) names the buffer a segment of n keys ends in.
 #pragma once
 #include "scan.hpp"
 
@@ -22,15 +24,38 @@ constexpr int SEG_IPT = 4;
 constexpr uint32_t SEG_CHUNK = SEG_NT * SEG_IPT;
 constexpr uint64_t SEG_PAD = ~0ull;  // sorts after every real key
 
-ATQ_DEV uint32_t seg_merge_passes(uint32_t n) {
+constexpr uint32_t SEG_LDS_KEYS = 16384;  // keys one work-group merges in LDS
+constexpr uint32_t SEG_LDS_IPT = SEG_LDS_KEYS / SEG_NT;
+constexpr uint32_t SEG_MERGE_NT = 256;    // global passes: keys per work-group item
+constexpr uint32_t SEG_MERGE_ITEM = 4 * SEG_MERGE_NT;
+
+// Global merge passes a segment of n > SEG_CHUNK keys takes.
+ATQ_DEV uint32_t seg_global_passes(uint32_t n) {
     uint32_t q = 0;
-    for (uint32_t r = SEG_CHUNK; r < n; r <<= 1) q++;
+    for (uint32_t r = SEG_LDS_KEYS; r < n; r <<= 1) q++;
     return q;
+}
+
+// Where the sorted keys of a long segment (n > SEG_CHUNK) end.
+template <typename T>
+ATQ_DEV T *seg_sorted_keys(uint32_t n, T *keys0, T *keys1) {
+    return (seg_global_passes(n) & 1) ? keys0 : keys1;
 }
 
 inline uint32_t seg_max_chunks(uint32_t max_seg) {
     return max_seg ? (max_seg + SEG_CHUNK - 1) / SEG_CHUNK : 1;
 }
+
+// Entries a long-segment list needs for segments tiling cap_items elements.
+inline uint32_t seg_long_cap(uint32_t cap_items) { return cap_items / (SEG_CHUNK + 1) + 1; }
+
+// The segments longer than SEG_CHUNK, in no particular order; n is zero
+// before the sort that fills it.
+struct SegLong {
+    uint32_t *n;
+    uint32_t *list;
+    uint32_t cap;
+};
 
 // Inclusive scan of six int64 sums over the SEG_NT threads of a work-group
 // (lds: at least 6 * SEG_NT / 64 + 6 entries); tot gets the work-group
@@ -80,9 +105,47 @@ ATQ_DEV void seg_cas(uint64_t &a, uint64_t &b) {
     b = hi;
 }
 
-// Sorts a[0..m) (m <= SEG_CHUNK; a[m..SEG_CHUNK) must hold SEG_PAD) by merge
-// path; b is a second SEG_CHUNK buffer. The result is in a. All SEG_NT
-// threads call it.
+// LDS key buffers carry one pad word per 32 keys: merge-path threads read
+// keys a power-of-two stride apart, which would otherwise fall into a few
+// LDS banks.
+ATQ_DEV uint32_t seg_ix(uint32_t i) { return i + (i >> 5); }
+constexpr uint32_t seg_padded(uint32_t n) { return n + n / 32; }
+
+// Merges the sorted runs src[base, base + r) and src[base + r, base + 2r)
+// (padded indices) from output diagonal d: the first `per` outputs from d
+// on, into v. Each step loads only the key of the run that advanced.
+template <int N>
+ATQ_DEV void seg_merge_at(const uint64_t *src, uint32_t base, uint32_t r, uint32_t d,
+                          uint32_t per, uint64_t (&v)[N]) {
+    // i = number of left-run keys among the first d outputs
+    uint32_t lo = d > r ? d - r : 0, hi = d < r ? d : r;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (src[seg_ix(base + mid)] < src[seg_ix(base + r + d - mid - 1)]) lo = mid + 1;
+        else hi = mid;
+    }
+    uint32_t i = lo, j = d - lo;
+    uint64_t av = i < r ? src[seg_ix(base + i)] : SEG_PAD;
+    uint64_t bv = j < r ? src[seg_ix(base + r + j)] : SEG_PAD;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        if ((uint32_t)k >= per) break;
+        const bool take_a = j >= r || (i < r && av < bv);
+        v[k] = take_a ? av : bv;
+        if (take_a) i++;
+        else j++;
+        if (k + 1 < N) {
+            const bool more = take_a ? i < r : j < r;
+            const uint64_t nv = more ? src[seg_ix(take_a ? base + i : base + r + j)] : SEG_PAD;
+            if (take_a) av = nv;
+            else bv = nv;
+        }
+    }
+}
+
+// Sorts a[0..m) (m <= SEG_CHUNK; a[m..SEG_CHUNK) must hold SEG_PAD; indices
+// padded by seg_ix) by merge path; b is a second buffer of the same size. The
+// result is in a. All SEG_NT threads call it.
 ATQ_DEV void seg_merge_sort(uint64_t *a, uint64_t *b, uint32_t m) {
     static_assert(SEG_IPT == 4, "four keys per thread");
     const uint32_t t = threadIdx.x, d0 = t * SEG_IPT;
@@ -90,35 +153,22 @@ ATQ_DEV void seg_merge_sort(uint64_t *a, uint64_t *b, uint32_t m) {
     while (P < m) P <<= 1;
     uint64_t v[SEG_IPT];
 #pragma unroll
-    for (int i = 0; i < SEG_IPT; i++) v[i] = a[d0 + i];
+    for (int i = 0; i < SEG_IPT; i++) v[i] = a[seg_ix(d0 + i)];
     seg_cas(v[0], v[1]);
     seg_cas(v[2], v[3]);
     seg_cas(v[0], v[2]);
     seg_cas(v[1], v[3]);
     seg_cas(v[1], v[2]);
 #pragma unroll
-    for (int i = 0; i < SEG_IPT; i++) a[d0 + i] = v[i];
+    for (int i = 0; i < SEG_IPT; i++) a[seg_ix(d0 + i)] = v[i];
     __syncthreads();
     uint64_t *src = a, *dst = b;
     for (uint32_t r = SEG_IPT; r < P; r <<= 1) {
         if (d0 < P) {
-            const uint32_t base = d0 / (2 * r) * (2 * r), d = d0 - base;
-            const uint64_t *A = src + base, *B = src + base + r;
-            // i = number of A keys among the first d outputs
-            uint32_t lo = d > r ? d - r : 0, hi = d < r ? d : r;
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (A[mid] < B[d - mid - 1]) lo = mid + 1;
-                else hi = mid;
-            }
-            uint32_t i = lo, j = d - lo;
+            const uint32_t base = d0 / (2 * r) * (2 * r);
+            seg_merge_at(src, base, r, d0 - base, SEG_IPT, v);
 #pragma unroll
-            for (int k = 0; k < SEG_IPT; k++) {
-                const bool take_a = j >= r || (i < r && A[i] < B[j]);
-                dst[base + d + k] = take_a ? A[i] : B[j];
-                if (take_a) i++;
-                else j++;
-            }
+            for (int k = 0; k < SEG_IPT; k++) dst[seg_ix(d0 + k)] = v[k];
         }
         __syncthreads();
         uint64_t *tmp = src;
@@ -127,7 +177,7 @@ ATQ_DEV void seg_merge_sort(uint64_t *a, uint64_t *b, uint32_t m) {
     }
     if (src != a) {
 #pragma unroll
-        for (int i = 0; i < SEG_IPT; i++) a[d0 + i] = src[d0 + i];
+        for (int i = 0; i < SEG_IPT; i++) a[seg_ix(d0 + i)] = src[seg_ix(d0 + i)];
         __syncthreads();
     }
 }
@@ -138,15 +188,16 @@ ATQ_DEV void seg_merge_sort(uint64_t *a, uint64_t *b, uint32_t m) {
 // const uint64_t (&v)[SEG_IPT], uint64_t *lds) const, called by every thread
 // of the work-group with a whole sorted segment of n <= SEG_CHUNK keys in
 // registers (key e0 + i in v[i], e0 = threadIdx.x * SEG_IPT) and lds,
-// 2 * SEG_CHUNK u64 of scratch: the second half is free on entry, the first
-// half (where the keys were read from) once a barrier has passed. Keys move
+// 2 * seg_padded(SEG_CHUNK) u64 of scratch: the second half is free on
+// entry, the first half (where the keys were read from) once a barrier has
+// passed. Keys move
 // between global memory and LDS coalesced.
 template <typename KeyGen, typename Finish, typename Ext>
 __global__ void __launch_bounds__(SEG_NT)
 k_seg_chunks(KeyGen gen, Finish fin, const Ext *__restrict__ segs, const uint32_t *n_segs_dev,
-             uint32_t cap_segs, uint32_t max_chunks, uint64_t *__restrict__ keys0) {
-    __shared__ uint64_t kab[2 * SEG_CHUNK];
-    uint64_t *const ka = kab, *const kb = kab + SEG_CHUNK;
+             uint32_t cap_segs, uint32_t max_chunks, uint64_t *__restrict__ keys0, SegLong lng) {
+    __shared__ uint64_t kab[2 * seg_padded(SEG_CHUNK)];
+    uint64_t *const ka = kab, *const kb = kab + seg_padded(SEG_CHUNK);
     const uint32_t nseg = min(*n_segs_dev, cap_segs);
     const uint64_t items = (uint64_t)nseg * max_chunks;
     const uint32_t t = threadIdx.x, e0 = t * SEG_IPT;
@@ -156,76 +207,168 @@ k_seg_chunks(KeyGen gen, Finish fin, const Ext *__restrict__ segs, const uint32_
         const uint32_t b = ch * SEG_CHUNK;
         if (b >= n) continue;
         const uint32_t m = min(SEG_CHUNK, n - b);
-        for (uint32_t j = t; j < SEG_CHUNK; j += SEG_NT)
-            ka[j] = j < m ? gen(sg, start, b + j) : SEG_PAD;
+        {  // every load of a thread in flight before its first LDS store
+            uint64_t g[SEG_IPT];
+#pragma unroll
+            for (int i = 0; i < SEG_IPT; i++) {
+                const uint32_t j = t + SEG_NT * i;
+                g[i] = j < m ? gen(sg, start, b + j) : SEG_PAD;
+            }
+#pragma unroll
+            for (int i = 0; i < SEG_IPT; i++) ka[seg_ix(t + SEG_NT * i)] = g[i];
+        }
         __syncthreads();
         seg_merge_sort(ka, kb, m);
         if (n <= SEG_CHUNK) {
             uint64_t v[SEG_IPT];
 #pragma unroll
-            for (int i = 0; i < SEG_IPT; i++) v[i] = ka[e0 + i];
+            for (int i = 0; i < SEG_IPT; i++) v[i] = ka[seg_ix(e0 + i)];
             fin(sg, start, n, v, kab);
             __syncthreads();
             continue;
         }
-        for (uint32_t j = t; j < m; j += SEG_NT) keys0[start + b + j] = ka[j];
+        if (ch == 0 && t == 0) {
+            const uint32_t slot = atomicAdd(lng.n, 1u);
+            if (slot < lng.cap) lng.list[slot] = sg;
+        }
+        for (uint32_t j = t; j < m; j += SEG_NT) keys0[start + b + j] = ka[seg_ix(j)];
         __syncthreads();
     }
 }
 
-ATQ_DEV uint32_t seg_lower_bound(const uint64_t *a, uint32_t n, uint64_t key) {
-    uint32_t lo = 0, hi = n;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (a[mid] < key) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// One merge pass, one thread per element: sorted runs of length `run` pair
-// up into runs of 2*run, for every segment longer than `run`. seg_of[x] is
-// the segment of element x.
+// Merges the SEG_CHUNK runs of every SEG_LDS_KEYS block of the listed long
+// segments in LDS: kin (runs of SEG_CHUNK) -> kout (runs of SEG_LDS_KEYS,
+// or the whole segment). Each level, every thread finds the start of its
+// outputs on its merge-path diagonal and merges them into registers.
 template <typename Ext>
 __global__ void __launch_bounds__(SEG_NT)
-k_seg_merge(const Ext *__restrict__ segs, const uint32_t *__restrict__ seg_of,
-            const uint32_t *n_items_dev, uint32_t cap_items, uint32_t run,
-            const uint64_t *__restrict__ kin, uint64_t *__restrict__ kout) {
-    const uint32_t N = min(*n_items_dev, cap_items);
-    for (uint32_t x = blockIdx.x * SEG_NT + threadIdx.x; x < N; x += gridDim.x * SEG_NT) {
-        const Ext e = segs[seg_of[x]];
-        if (e.count <= run) continue;
-        const uint32_t n = e.count, j = x - e.start;
-        const uint64_t *c = kin + e.start;
-        const uint32_t base = j / (2 * run) * (2 * run);
-        const uint64_t key = c[j];
-        uint32_t pos;
-        if (j - base < run) {  // left run: add the right run's smaller keys
-            const uint32_t rb = min(base + run, n), re = min(base + 2 * run, n);
-            pos = j + seg_lower_bound(c + rb, re - rb, key);
-        } else {               // right run: add the left run's smaller keys
-            pos = j - run + seg_lower_bound(c + base, run, key);
+k_seg_merge_lds(const Ext *__restrict__ segs, SegLong lng, uint32_t max_blocks,
+                const uint64_t *__restrict__ kin, uint64_t *__restrict__ kout) {
+    __shared__ uint64_t buf[seg_padded(SEG_LDS_KEYS)];
+    const uint64_t items = (uint64_t)min(*lng.n, lng.cap) * max_blocks;
+    const uint32_t t = threadIdx.x;
+    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
+        const Ext e = segs[lng.list[it / max_blocks]];
+        const uint32_t b = (uint32_t)(it % max_blocks) * SEG_LDS_KEYS;
+        if (b >= e.count) continue;
+        const uint32_t m = min(SEG_LDS_KEYS, e.count - b);
+        uint32_t P = SEG_CHUNK;
+        while (P < m) P <<= 1;
+        const uint32_t per = P / SEG_NT, d0 = t * per;
+        uint64_t v[SEG_LDS_IPT];
+#pragma unroll
+        for (uint32_t k = 0; k < SEG_LDS_IPT; k++) {  // all loads before the first LDS store
+            const uint32_t j = t + SEG_NT * k;
+            v[k] = j < m ? kin[e.start + b + j] : SEG_PAD;
         }
-        kout[e.start + pos] = key;
+#pragma unroll
+        for (uint32_t k = 0; k < SEG_LDS_IPT; k++)
+            if (k < per) buf[seg_ix(t + SEG_NT * k)] = v[k];
+        __syncthreads();
+        for (uint32_t run = SEG_CHUNK; run < P; run <<= 1) {
+            const uint32_t base = d0 / (2 * run) * (2 * run);
+            seg_merge_at(buf, base, run, d0 - base, per, v);
+            __syncthreads();
+#pragma unroll
+            for (uint32_t k = 0; k < SEG_LDS_IPT; k++)
+                if (k < per) buf[seg_ix(d0 + k)] = v[k];
+            __syncthreads();
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < SEG_LDS_IPT; k++) {
+            const uint32_t j = t + SEG_NT * k;
+            if (j < m) v[k] = buf[seg_ix(j)];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < SEG_LDS_IPT; k++) {
+            const uint32_t j = t + SEG_NT * k;
+            if (j < m) kout[e.start + b + j] = v[k];
+        }
+        __syncthreads();
     }
 }
 
-// Sorts every segment. The segments tile elements [0, *n_items_dev) and
-// seg_of maps each element to its segment; max_seg bounds the segment length
-// (longer segments are left partly sorted: callers bound it by construction).
+// One global merge pass over the listed long segments, SEG_MERGE_ITEM keys
+// per item: sorted runs of length `run` pair up into runs of 2*run.
+template <typename Ext>
+__global__ void __launch_bounds__(SEG_MERGE_NT)
+k_seg_merge(const Ext *__restrict__ segs, SegLong lng, uint32_t max_items, uint32_t run,
+            const uint64_t *__restrict__ kin, uint64_t *__restrict__ kout) {
+    constexpr int IPT = SEG_MERGE_ITEM / SEG_MERGE_NT;
+    const uint64_t items = (uint64_t)min(*lng.n, lng.cap) * max_items;
+    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
+        const Ext e = segs[lng.list[it / max_items]];
+        const uint32_t n = e.count, b = (uint32_t)(it % max_items) * SEG_MERGE_ITEM;
+        if (n <= run || b >= n) continue;
+        const uint64_t *c = kin + e.start;
+        // IPT keys per thread, their binary searches run in lockstep so the
+        // loads of one step are in flight together
+        uint32_t lo[IPT], len[IPT], j[IPT];
+        const uint64_t *arr[IPT];
+        uint64_t key[IPT];
+#pragma unroll
+        for (int k = 0; k < IPT; k++) {
+            j[k] = b + threadIdx.x + SEG_MERGE_NT * k;
+            key[k] = j[k] < n ? c[j[k]] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < IPT; k++) {
+            const uint32_t base = j[k] / (2 * run) * (2 * run);
+            lo[k] = 0;
+            len[k] = 0;
+            arr[k] = c;
+            if (j[k] >= n) continue;
+            if (j[k] - base < run) {  // left run: count the right run's smaller keys
+                const uint32_t rb = min(base + run, n), re = min(base + 2 * run, n);
+                arr[k] = c + rb;
+                len[k] = re - rb;
+            } else {                  // right run: count the left run's smaller keys
+                arr[k] = c + base;
+                len[k] = run;
+            }
+        }
+        for (uint32_t r = run; r; r >>= 1) {  // log2(run) + 1 halvings end every search
+#pragma unroll
+            for (int k = 0; k < IPT; k++) {
+                if (!len[k]) continue;
+                const uint32_t half = len[k] >> 1;
+                if (arr[k][lo[k] + half] < key[k]) {
+                    lo[k] += half + 1;
+                    len[k] -= half + 1;
+                } else {
+                    len[k] = half;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < IPT; k++) {
+            if (j[k] >= n) continue;
+            const uint32_t base = j[k] / (2 * run) * (2 * run);
+            const uint32_t pos = j[k] - base < run ? j[k] + lo[k] : j[k] - run + lo[k];
+            kout[e.start + pos] = key[k];
+        }
+    }
+}
+
+// Sorts every segment; max_seg bounds the segment length (longer segments
+// are left partly sorted: callers bound it by construction). lng.n must be
+// zero on entry; on return the list names the segments longer than
+// SEG_CHUNK.
 template <typename KeyGen, typename Finish, typename Ext>
 int seg_sort(hipStream_t s, KeyGen gen, Finish fin, const Ext *segs, const uint32_t *n_segs_dev,
-             uint32_t cap_segs, const uint32_t *seg_of, const uint32_t *n_items_dev,
-             uint32_t cap_items, uint32_t max_seg, uint64_t *keys0, uint64_t *keys1) {
+             uint32_t cap_segs, uint32_t max_seg, uint64_t *keys0, uint64_t *keys1, SegLong lng) {
     const uint32_t max_chunks = seg_max_chunks(max_seg);
     k_seg_chunks<KeyGen, Finish, Ext><<<1024, SEG_NT, 0, s>>>(gen, fin, segs, n_segs_dev, cap_segs,
-                                                               max_chunks, keys0);
-    const unsigned grid = min(8192u, (cap_items + SEG_NT - 1) / SEG_NT + 1);
+                                                               max_chunks, keys0, lng);
+    if (max_seg > SEG_CHUNK)
+        k_seg_merge_lds<Ext><<<256, SEG_NT, 0, s>>>(
+            segs, lng, (max_seg + SEG_LDS_KEYS - 1) / SEG_LDS_KEYS, keys0, keys1);
+    const uint32_t max_items = (max_seg + SEG_MERGE_ITEM - 1) / SEG_MERGE_ITEM;
     int pass = 0;
-    for (uint32_t run = SEG_CHUNK; run < max_seg; run <<= 1, pass++)
-        k_seg_merge<Ext><<<grid, SEG_NT, 0, s>>>(segs, seg_of, n_items_dev, cap_items, run,
-                                                  (pass & 1) ? keys1 : keys0,
-                                                  (pass & 1) ? keys0 : keys1);
+    for (uint32_t run = SEG_LDS_KEYS; run < max_seg; run <<= 1, pass++)
+        k_seg_merge<Ext><<<512, SEG_MERGE_NT, 0, s>>>(segs, lng, max_items, run,
+                                                        (pass & 1) ? keys0 : keys1,
+                                                        (pass & 1) ? keys1 : keys0);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 

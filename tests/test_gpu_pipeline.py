@@ -178,3 +178,15 @@ def test_detector_pair_table_overflow(atq, oracle):
     g = run_detector(atq, img, 1)
     assert int(g["counts"]["pair_filter_off"]) == 0
     check_against_oracle(g, o)
+
+
+def test_detector_stress_12mp(atq, oracle):
+    """The 4096x3072 stress frame (maze, blobs, 200 tags), element by element:
+    its longest clusters exceed one work-group's LDS merge (16384 points), so
+    the global merge passes of the segment sort run too."""
+    from pyatq import synth
+    img = synth.stress_12mp().image
+    o = oracle.pipeline(img, 1)
+    assert int(o["extents"]["count"].max()) > 16384
+    g = run_detector(atq, img, 1)
+    check_against_oracle(g, o)
