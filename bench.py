@@ -116,6 +116,7 @@ def bench_12mp(atq, synth, stream, steps=10):
     log(f"generated 12MP frame in {time.time() - t:.1f}s")
     dbig = [atq.dev_from_np(big)]
     det = atq.Detector(4096, 3072, decimate=1)
+    shapes = det.autotune(dbig[0], 4096, 3072, stream.handle)
     wall, ev = time_frames(atq, det, dbig, 4096, 3072, steps, 2, stream)
     stages = stage_profile(atq, det, dbig, 4096, 3072, 5, stream)
     det.detect(dbig[0], 4096, 3072, stream.handle)
@@ -129,6 +130,7 @@ def bench_12mp(atq, synth, stream, steps=10):
         roof12[st] = {"ms": round(stages[st], 4), "GB/s": round(b / (stages[st] * 1e-3) / 1e9, 1),
                       "frac": round(b / (stages[st] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     out = {
+        "autotuned_shapes": list(shapes),
         "ms_per_frame": round(wall / steps * 1e3, 3),
         "mpix_per_s": round(steps * npx12 / wall / 1e6, 1),
         "stage_ms": {k: round(v, 4) for k, v in stages.items()},
@@ -150,6 +152,8 @@ def bench_batch(atq, synth, stream, rounds):
     decs = [f.decimate for f in frames]
     devs = [atq.dev_from_np(f.image) for f in frames]
     dets = [atq.Detector(w, h, decimate=d) for (w, h), d in zip(sizes, decs)]
+    for (w, h), det, dv in zip(sizes, dets, devs):  # the same tuned shapes as the batch
+        det.autotune(dv, w, h, stream.handle)
 
     def plain_round():
         for (w, h), det, dv in zip(sizes, dets, devs):
@@ -181,7 +185,9 @@ def bench_batch(atq, synth, stream, rounds):
            "batched_eager_frames_per_s": round(16 * rounds / t_eager, 1),
            "batched_graph_mpix_per_s": round(mpix * rounds / t_graph, 1),
            "graph_vs_plain": round(t_plain / t_graph, 3),
-           "per_frame_quads_identical": bool(same)}
+           "per_frame_quads_identical": bool(same),
+           "autotuned_shapes": {f"{w}x{h}/d{d}": list(b.frame(i).shapes())
+                                for i, ((w, h), d) in enumerate(zip(sizes, decs))}}
     log(f"batch: {out}")
     b.close()
     for det in dets:
@@ -240,12 +246,13 @@ def main():
     res = {}
     for f in ((1,) if args.skip_f2 else (1, 2)):
         det = atq.Detector(W1080, H1080, decimate=f)
+        shapes = det.autotune(devs[0], W1080, H1080, stream.handle)
         wall, ev = time_frames(atq, det, devs, W1080, H1080, args.steps, args.warmup, stream)
         stages = stage_profile(atq, det, devs, W1080, H1080, 20, stream)
         det.detect(devs[0], W1080, H1080, stream.handle)
         stream.sync()
         counts = det.counts()
-        res[f] = dict(wall=wall, ev=ev, stages=stages, counts=counts)
+        res[f] = dict(wall=wall, ev=ev, stages=stages, counts=counts, shapes=shapes)
         log(f"decimate {f}: {args.steps} frames in {wall * 1e3:.1f} ms wall "
             f"({ev * 1e3:.1f} ms events), quads/frame {int(counts['n_quads'])}")
         log("  stages ms: " + ", ".join(f"{k} {v:.4f}" for k, v in stages.items()))
@@ -282,6 +289,7 @@ def main():
     extra = {
         "frames_per_s": round(fps, 2),
         "event_ms_per_frame": round(res[1]["ev"] / steps * 1e3, 4),
+        "autotuned_shapes_1080p_f1": list(res[1]["shapes"]),
         "stage_ms_1080p_f1": {k: round(v, 5) for k, v in st1.items()},
         "stage_roofline_1080p_f1": stage_roof,
         "counts_1080p_f1": {k: int(res[1]["counts"][k]) for k in

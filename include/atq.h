@@ -382,16 +382,39 @@ enum {
     ATQ_N_STAGES = 10
 };
 int atq_detector_set_timing(atq_detector *det, int enable);
+
+/* ---- front-end shapes and the autotuner ----------------------------------- */
+/* No reference counterpart (the reference fixes its work-group shapes).
+ * Threshold regions of threshold_tiles_x x threshold_tiles_y 4x4-pixel tiles
+ * (32x16, 64x8, 32x8 or 16x16) and CCL tiles of 32 x ccl_block_rows 2x2-pixel
+ * blocks (8 or 16). Results do not depend on the shapes; the default is
+ * 32x16 and 16. */
+typedef struct atq_shapes {
+    int32_t threshold_tiles_x, threshold_tiles_y, ccl_block_rows;
+} atq_shapes;
+int atq_detector_set_shapes(atq_detector *det, const atq_shapes *shapes);
+int atq_detector_get_shapes(atq_detector *det, atq_shapes *shapes);
+/* Times every supported shape on gray_in (a width_in x height_in frame in
+ * device memory) with det's buffers (their contents are overwritten), keeps
+ * the fastest threshold and CCL shapes in det and in the library's table for
+ * that (width_in, height_in, decimate), and returns them in *best (may be
+ * NULL). Blocks until the timing runs end. A resolution already in the table
+ * takes its shapes from there without timing. */
+int atq_autotune(atq_detector *det, void *stream, const uint8_t *gray_in,
+                 size_t width_in, size_t height_in, atq_shapes *best);
 int atq_detector_stage_times(atq_detector *det, float *ms, int n);
 const char *atq_stage_name(int stage);
 
 /* ---- batched form --------------------------------------------------------- */
 /* No reference counterpart. Many frames of uneven sizes and decimation
- * factors through one launch: each frame has its own detector (sized for it)
- * and input buffer; the frames' pipelines are captured once into a single
- * HIP graph as parallel branches over internal streams, so a batch is one
- * graph launch. Results per frame are bit-identical to atq_detect_quads on
- * that frame and are read through atq_batch_frame's detector. */
+ * factors through one launch chain: each frame has its own detector (sized
+ * for it) and input buffer. The thresholds and CCLs of all frames run as
+ * one persistent launch per kernel over a table of frames (one chain per
+ * autotuned shape), the remaining stages of the frames as parallel branches
+ * over internal streams; the whole chain is captured once into a HIP graph,
+ * so a batch is one graph launch. Results per frame are bit-identical to
+ * atq_detect_quads on that frame and are read through atq_batch_frame's
+ * detector. */
 typedef struct atq_batch atq_batch;
 int atq_batch_create(size_t n_frames, const size_t *widths_in,
                      const size_t *heights_in, const int32_t *decimate,
@@ -403,6 +426,10 @@ int atq_batch_input(atq_batch *batch, size_t i, uint8_t **gray_in);
  * it on the first call); 0: the same branches launched eagerly. Ordered on
  * `stream` (which must not be the NULL stream when use_graph != 0). */
 int atq_batch_detect(atq_batch *batch, void *stream, int use_graph);
+/* Autotunes every frame's shapes on its current input (atq_autotune; each
+ * resolution is timed once). atq_batch_detect does this on its first call
+ * unless it was done before. */
+int atq_batch_autotune(atq_batch *batch, void *stream);
 int atq_batch_frame(atq_batch *batch, size_t i, atq_detector **det);
 
 #ifdef __cplusplus

@@ -9,8 +9,10 @@
 // links the block's colour-c node to the colour-c node of the block holding
 // it. Components are labelled by their minimum node id (min-linking).
 //
-// gfx950 organisation (four kernels, the reference has five):
-//  1. k_ccl_local: a work-group owns a TBX x TBY tile of blocks, stages its
+// gfx950 organisation (four kernels, the reference has five; each takes a
+// table of frames and walks them with a persistent grid):
+//  1. k_ccl_local: a work-group owns a TBX x TBY tile of blocks (TBY = 8 or
+//     16, chosen per resolution by the autotuner), stages its
 //     pixels (+1 px halo) in LDS, unions the tile-internal edges with an LDS
 //     union-find, counts each local root's pixels, and writes for every node
 //     the global id of its local root (FLAG-tagged for non-roots) and, at
@@ -24,15 +26,15 @@
 //  4. k_ccl_final: each block reads its two roots (at most two loads) and
 //     writes the four label words (white = 0x80000000|root, black = root,
 //     127 -> 0).
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace atq {
 
 namespace {
 
-constexpr int TBX = 32;  // blocks per tile row    (64 px)
-constexpr int TBY = 16;  // block rows per tile    (32 px)
-constexpr int NODES = 2 * TBX * TBY;
+constexpr int TBX = 32;  // blocks per tile row (64 px); block rows per tile: TBY
 constexpr uint32_t FLAG = 0x80000000u;  // "not a local root" tag on parent words
 
 constexpr uint8_t C_WHITE = 255, C_BLACK = 0;
@@ -137,19 +139,29 @@ ATQ_DEV void g_union(uint32_t *parent, uint32_t a, uint32_t b) {
 
 }  // namespace
 
-__global__ void __launch_bounds__(256)
-k_ccl_local(const uint8_t *__restrict__ thr, uint32_t w, uint32_t h, uint32_t tiles_x,
-            uint32_t n_tiles, uint32_t *__restrict__ parent, uint32_t *__restrict__ sizes) {
-    constexpr int PR = 2 * TBY + 1;      // pixel rows incl. the halo row above
-    constexpr int PC = 2 * TBX + 2;      // pixel cols incl. left halo and R column
-    __shared__ uint8_t px[PR][PC + 2];
-    __shared__ uint32_t lp[NODES];
-    __shared__ uint32_t cnt[NODES];
+template <int TBY>
+struct LocalLds {
+    static constexpr int PR = 2 * TBY + 1;  // pixel rows incl. the halo row above
+    static constexpr int PC = 2 * TBX + 2;  // pixel cols incl. left halo and R column
+    static constexpr int NODES = 2 * TBX * TBY;
+    uint8_t px[PR][PC + 2];
+    uint32_t lp[NODES];
+    uint32_t cnt[NODES];
+};
 
-    const uint32_t tile = xcd_tile(blockIdx.x, n_tiles);
-    if (tile >= n_tiles) return;
+// One tile of frame F (all 256 threads).
+template <int TBY>
+ATQ_DEV void ccl_local_tile(LocalLds<TBY> &L, const FrontFrame &F, uint32_t tile, uint32_t tiles_x) {
+    using Lds = LocalLds<TBY>;
+    const uint8_t *__restrict__ thr = F.thr;
+    const uint32_t w = F.w, h = F.h;
+    uint32_t *__restrict__ parent = F.parent;
     const int tid = threadIdx.x;
     const int x0 = (tile % tiles_x) * (2 * TBX), y0 = (tile / tiles_x) * (2 * TBY);
+    constexpr int PR = Lds::PR, PC = Lds::PC, NODES = Lds::NODES;
+    auto &px = L.px;
+    auto &lp = L.lp;
+    auto &cnt = L.cnt;
 
     // every load of a thread is issued before the first LDS store
     constexpr int PER = (PR * PC + 255) / 256;
@@ -267,7 +279,23 @@ k_ccl_local(const uint8_t *__restrict__ thr, uint32_t w, uint32_t h, uint32_t ti
         uint32_t gr = (uint32_t)(y0 + 2 * (rb / TBX)) * w + (uint32_t)(x0 + 2 * (rb % TBX)) + (r & 1);
         uint32_t gn = (uint32_t)y * w + (uint32_t)x + c;
         parent[gn] = r == (uint32_t)n ? gr : (gr | FLAG);
-        if (r == (uint32_t)n) sizes[gn] = cnt[n];
+        if (r == (uint32_t)n) F.sizes[gn] = cnt[n];
+    }
+    __syncthreads();  // the LDS is reused by the next tile
+}
+
+template <int TBY>
+__global__ void __launch_bounds__(256)
+k_ccl_local(FrontTable tab) {
+    __shared__ LocalLds<TBY> L;
+    for (uint32_t fi = 0; fi < tab.n; fi++) {
+        const FrontFrame &F = tab.fr[fi];
+        const uint32_t tiles_x = (F.w + 2 * TBX - 1) / (2 * TBX);
+        const uint32_t n_tiles = tiles_x * ((F.h + 2 * TBY - 1) / (2 * TBY));
+        for (uint32_t it = blockIdx.x; it < xcd_grid(n_tiles); it += gridDim.x) {
+            const uint32_t tile = xcd_tile(it, n_tiles);
+            if (tile < n_tiles) ccl_local_tile<TBY>(L, F, tile, tiles_x);
+        }
     }
 }
 
@@ -278,22 +306,23 @@ k_ccl_local(const uint8_t *__restrict__ thr, uint32_t w, uint32_t h, uint32_t ti
 // and only the lane that inserts a pair merges it on the global parent array
 // (lock-free atomicMin union, include/union_find.h:50-77). A tile border is
 // crossed by few components, so a tile issues a handful of global unions.
-constexpr int BORDER_ITEMS = TBX + 2 * (TBY - 1);
+template <int TBY>
+constexpr int border_items() { return TBX + 2 * (TBY - 1); }
 constexpr int BORDER_SLOTS = 512;
-static_assert(BORDER_ITEMS <= 64, "one border block per lane");
 
-__global__ void __launch_bounds__(256)
-k_ccl_border(const uint8_t *__restrict__ thr, uint32_t w, uint32_t h, uint32_t tiles_x,
-             uint32_t n_tiles, uint32_t *__restrict__ parent) {
+// The border edges of one tile of frame F (one wave; lane k).
+template <int TBY>
+ATQ_DEV void ccl_border_tile(uint64_t *hs, const FrontFrame &F, uint32_t tile, uint32_t tiles_x,
+                             uint32_t n_tiles) {
+    constexpr int ITEMS = border_items<TBY>();
+    static_assert(ITEMS <= 64, "one border block per lane");
     constexpr uint32_t NONE = 0xFFFFFFFFu;
     constexpr uint64_t EMPTY = ~0ull;
-    __shared__ uint64_t set[4][BORDER_SLOTS];
-    const uint32_t wave = threadIdx.x >> 6, k = threadIdx.x & 63;
-    const uint32_t tile = blockIdx.x * 4 + wave;
-    uint64_t *hs = set[wave];
-    for (int i = k; i < BORDER_SLOTS; i += 64) hs[i] = EMPTY;
-    __syncthreads();
-    bool live = tile < n_tiles && k < (uint32_t)BORDER_ITEMS;
+    const uint8_t *__restrict__ thr = F.thr;
+    const uint32_t w = F.w, h = F.h;
+    uint32_t *__restrict__ parent = F.parent;
+    const uint32_t k = threadIdx.x & 63;
+    bool live = tile < n_tiles && k < (uint32_t)ITEMS;
     int bx = 0, by = 0, x = 0, y = 0;
     if (live) {
         if (k < (uint32_t)TBX) { bx = k; by = 0; }
@@ -348,68 +377,133 @@ k_ccl_border(const uint8_t *__restrict__ thr, uint32_t w, uint32_t h, uint32_t t
     }
 }
 
+// Four tiles per work-group round, one per wave.
+template <int TBY>
+__global__ void __launch_bounds__(256)
+k_ccl_border(FrontTable tab) {
+    __shared__ uint64_t set[4][BORDER_SLOTS];
+    const uint32_t wave = threadIdx.x >> 6, k = threadIdx.x & 63;
+    uint64_t *hs = set[wave];
+    for (uint32_t fi = 0; fi < tab.n; fi++) {
+        const FrontFrame &F = tab.fr[fi];
+        const uint32_t tiles_x = (F.w + 2 * TBX - 1) / (2 * TBX);
+        const uint32_t n_tiles = tiles_x * ((F.h + 2 * TBY - 1) / (2 * TBY));
+        for (uint32_t g = blockIdx.x; g < (n_tiles + 3) / 4; g += gridDim.x) {
+            for (int i = k; i < BORDER_SLOTS; i += 64) hs[i] = ~0ull;
+            __syncthreads();
+            ccl_border_tile<TBY>(hs, F, 4 * g + wave, tiles_x, n_tiles);
+            __syncthreads();
+        }
+    }
+}
+
 // After the border merge: every local root gets its final root (path
 // halving, then the root itself), and adds its pixel count into the root's
 // size. Once the root's size reaches size_cap further adds are skipped (the
 // detector only asks whether a size reaches min_component_size; the drop-in
 // passes UINT32_MAX for exact sizes).
+// Work items of flatten / final: one 256-block run of a block row.
+ATQ_DEV uint32_t block_runs_x(uint32_t w) { return ((w / 2) + 255) / 256; }
+
 __global__ void __launch_bounds__(256)
-k_ccl_flatten(uint32_t w, uint32_t h, uint32_t *__restrict__ parent, uint32_t *__restrict__ sizes,
-              uint32_t size_cap) {
-    const uint32_t bx = blockIdx.x * blockDim.x + threadIdx.x, by = blockIdx.y;
-    if (2 * bx >= w || 2 * by >= h) return;
-    const uint32_t id = 2 * by * w + 2 * bx;
+k_ccl_flatten(FrontTable tab) {
+    for (uint32_t fi = 0; fi < tab.n; fi++) {
+        const FrontFrame &F = tab.fr[fi];
+        uint32_t *__restrict__ parent = F.parent;
+        uint32_t *__restrict__ sizes = F.sizes;
+        const uint32_t w = F.w, h = F.h, runs_x = block_runs_x(w), size_cap = F.size_cap;
+        for (uint32_t it = blockIdx.x; it < runs_x * (h / 2); it += gridDim.x) {
+            const uint32_t bx = (it % runs_x) * 256 + threadIdx.x, by = it / runs_x;
+            if (2 * bx >= w) continue;
+            const uint32_t id = 2 * by * w + 2 * bx;
 #pragma unroll
-    for (uint32_t c = 0; c < 2; c++) {
-        const uint32_t n = id + c;
-        const uint32_t v = parent[n];
-        if ((v & FLAG) || v == n) continue;  // not a local root, or a final root
-        const uint32_t r = g_find_halving(parent, v);
-        atomicMin(&parent[n], r);
-        const uint32_t s = sizes[n];
-        if (s && g_load(&sizes[r]) < size_cap) atomicAdd(&sizes[r], s);
+            for (uint32_t c = 0; c < 2; c++) {
+                const uint32_t n = id + c;
+                const uint32_t v = parent[n];
+                if ((v & FLAG) || v == n) continue;  // not a local root, or a final root
+                const uint32_t r = g_find_halving(parent, v);
+                atomicMin(&parent[n], r);
+                const uint32_t sz = sizes[n];
+                if (sz && g_load(&sizes[r]) < size_cap) atomicAdd(&sizes[r], sz);
+            }
+        }
     }
 }
 
 // Labels: white = 0x80000000 | root, black = root, 127 -> 0. A node's word
 // is its final root (a local root, flattened) or FLAG | its local root.
 __global__ void __launch_bounds__(256)
-k_ccl_final(const uint8_t *__restrict__ thr, uint32_t w, uint32_t h,
-            const uint32_t *__restrict__ parent, uint32_t *__restrict__ labels) {
-    uint32_t bx = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t by = blockIdx.y;
-    if (2 * bx >= w || 2 * by >= h) return;
-    uint32_t x = 2 * bx, y = 2 * by;
-    uint32_t id = y * w + x;
-    const uint2 v = *reinterpret_cast<const uint2 *>(parent + id);
-    const uint32_t r0 = (v.x & FLAG) ? parent[v.x & ~FLAG] : v.x;
-    const uint32_t r1 = (v.y & FLAG) ? parent[v.y & ~FLAG] : v.y;
-    uint16_t top = *reinterpret_cast<const uint16_t *>(thr + id);
-    uint16_t bot = *reinterpret_cast<const uint16_t *>(thr + id + w);
-    auto lab = [&](uint32_t p) -> uint32_t {
-        p &= 0xFFu;
-        if (p == C_WHITE) return LABEL_PIXEL_MASK | r0;
-        if (p == C_BLACK) return r1;
-        return 0u;
-    };
-    uint2 t2 = make_uint2(lab(top), lab(top >> 8));
-    uint2 b2 = make_uint2(lab(bot), lab(bot >> 8));
-    *reinterpret_cast<uint2 *>(labels + id) = t2;
-    *reinterpret_cast<uint2 *>(labels + id + w) = b2;
+k_ccl_final(FrontTable tab) {
+    for (uint32_t fi = 0; fi < tab.n; fi++) {
+        const FrontFrame &F = tab.fr[fi];
+        const uint32_t *__restrict__ parent = F.parent;
+        const uint8_t *__restrict__ thr = F.thr;
+        uint32_t *__restrict__ labels = F.labels;
+        const uint32_t w = F.w, h = F.h, runs_x = block_runs_x(w);
+        for (uint32_t it = blockIdx.x; it < runs_x * (h / 2); it += gridDim.x) {
+            const uint32_t bx = (it % runs_x) * 256 + threadIdx.x, by = it / runs_x;
+            if (2 * bx >= w) continue;
+            const uint32_t id = 2 * by * w + 2 * bx;
+            const uint2 v = *reinterpret_cast<const uint2 *>(parent + id);
+            const uint32_t r0 = (v.x & FLAG) ? parent[v.x & ~FLAG] : v.x;
+            const uint32_t r1 = (v.y & FLAG) ? parent[v.y & ~FLAG] : v.y;
+            const uint16_t top = *reinterpret_cast<const uint16_t *>(thr + id);
+            const uint16_t bot = *reinterpret_cast<const uint16_t *>(thr + id + w);
+            auto lab = [&](uint32_t p) -> uint32_t {
+                p &= 0xFFu;
+                if (p == C_WHITE) return LABEL_PIXEL_MASK | r0;
+                if (p == C_BLACK) return r1;
+                return 0u;
+            };
+            *reinterpret_cast<uint2 *>(labels + id) = make_uint2(lab(top), lab(top >> 8));
+            *reinterpret_cast<uint2 *>(labels + id + w) = make_uint2(lab(bot), lab(bot >> 8));
+        }
+    }
+}
+
+namespace {
+template <int TBY>
+void ccl_launch(hipStream_t s, const FrontTable &tab) {
+    uint32_t tiles = 0, runs = 0;
+    for (uint32_t i = 0; i < tab.n; i++) {
+        const uint32_t w = tab.fr[i].w, h = tab.fr[i].h;
+        const uint32_t nt = atq_div_up(w, 2 * TBX) * atq_div_up(h, 2 * TBY);
+        tiles = std::max(tiles, nt);
+        runs = std::max(runs, (uint32_t)atq_div_up(w / 2, 256) * (h / 2));
+    }
+    // persistent grids: the largest frame's items, capped well above what
+    // the device holds at once (frames up to 8192 x 8192 loop a few times)
+    const uint32_t lg = std::min<uint32_t>(xcd_grid(tiles), 8192u);
+    const uint32_t bg = std::min<uint32_t>(atq_div_up(tiles, 4), 4096u);
+    const uint32_t fg = std::min<uint32_t>(runs, 16384u);
+    k_ccl_local<TBY><<<lg ? lg : 8, 256, 0, s>>>(tab);
+    k_ccl_border<TBY><<<bg ? bg : 1, 256, 0, s>>>(tab);
+    k_ccl_flatten<<<fg ? fg : 1, 256, 0, s>>>(tab);
+    k_ccl_final<<<fg ? fg : 1, 256, 0, s>>>(tab);
+}
+}  // namespace
+
+int launch_ccl_frames(hipStream_t s, const FrontFrame *frames, int n, const FrontShapes &sh) {
+    if (n < 1 || n > FRONT_MAX_FRAMES || !front_shapes_supported(sh)) return ATQ_ERR_BAD_ARG;
+    FrontTable tab{};
+    for (int i = 0; i < n; i++) tab.fr[i] = frames[i];
+    tab.n = (uint32_t)n;
+    if (sh.ccl_rows == 8) ccl_launch<8>(s, tab);
+    else ccl_launch<16>(s, tab);
+    return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
 int launch_ccl(hipStream_t s, const uint8_t *thr, size_t w, size_t h, uint32_t *parent,
                uint32_t *labels, uint32_t *sizes, uint32_t size_cap) {
-    dim3 tiles(atq_div_up(w, 2 * TBX), atq_div_up(h, 2 * TBY));
-    k_ccl_local<<<xcd_grid(tiles.x * tiles.y), 256, 0, s>>>(thr, (uint32_t)w, (uint32_t)h, tiles.x,
-                                                           tiles.x * tiles.y, parent, sizes);
-    const uint32_t n_tiles = tiles.x * tiles.y;
-    k_ccl_border<<<atq_div_up(n_tiles, 4), 256, 0, s>>>(thr, (uint32_t)w, (uint32_t)h, tiles.x,
-                                                       n_tiles, parent);
-    dim3 fgrid(atq_div_up(w / 2, 256), (unsigned)(h / 2));
-    k_ccl_flatten<<<fgrid, 256, 0, s>>>((uint32_t)w, (uint32_t)h, parent, sizes, size_cap);
-    k_ccl_final<<<fgrid, 256, 0, s>>>(thr, (uint32_t)w, (uint32_t)h, parent, labels);
-    return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
+    FrontFrame fr{};
+    fr.thr = const_cast<uint8_t *>(thr);
+    fr.parent = parent;
+    fr.labels = labels;
+    fr.sizes = sizes;
+    fr.w = (uint32_t)w;
+    fr.h = (uint32_t)h;
+    fr.size_cap = size_cap;
+    return launch_ccl_frames(s, &fr, 1, FrontShapes{});
 }
 
 }  // namespace atq

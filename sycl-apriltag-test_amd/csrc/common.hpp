@@ -84,7 +84,7 @@ ATQ_DEV uint32_t xcd_tile(uint32_t b, uint32_t n) {
     const uint32_t t = (b % N_XCD) * per + b / N_XCD;
     return t < n ? t : n;
 }
-inline uint32_t xcd_grid(uint32_t n) { return (n + N_XCD - 1) / N_XCD * N_XCD; }
+__host__ __device__ inline uint32_t xcd_grid(uint32_t n) { return (n + N_XCD - 1) / N_XCD * N_XCD; }
 
 // Orderable image of a float (no NaN is ever sorted here): ascending u32
 // order equals ascending float order.

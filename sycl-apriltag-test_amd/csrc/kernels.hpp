@@ -6,16 +6,47 @@ namespace atq {
 
 struct ScanCtx;  // scan.hpp
 
+// One frame of the front-end stages (decimate/threshold, CCL). Several
+// frames go to one launch as a table passed by value; every kernel walks the
+// frames in turn with a persistent grid.
+struct FrontFrame {
+    const uint8_t *in;     // grey input, w_in x h_in
+    uint8_t *gray;         // decimated grey (written when f > 1; may be null)
+    atq_extrema *ext, *fil;  // raw and dilated tile extrema (may be null)
+    uint8_t *thr;          // thresholded, w x h
+    uint32_t *parent, *labels, *sizes;
+    uint32_t w_in, h_in, f, w, h;
+    uint32_t size_cap;     // sizes saturate here (CCL)
+    int32_t min_diff;
+};
+constexpr int FRONT_MAX_FRAMES = 16;
+struct FrontTable {
+    FrontFrame fr[FRONT_MAX_FRAMES];
+    uint32_t n;
+};
+
+// Front-end shapes: threshold regions of rx x ry 4x4-pixel tiles, CCL tiles
+// of 32 x ccl_rows 2x2-pixel blocks. Results do not depend on them.
+struct FrontShapes {
+    int thr_rx = 32, thr_ry = 16, ccl_rows = 16;
+};
+bool front_shapes_supported(const FrontShapes &sh);
+// Every FrontShapes value front_shapes_supported accepts.
+int front_shape_candidates(FrontShapes *out, int max_out);
+
 // threshold.hip
 int launch_decimate(hipStream_t s, const uint8_t *in, size_t w_in, size_t h_in, int f,
                     uint8_t *out);
 int launch_threshold(hipStream_t s, const uint8_t *in, size_t w_in, size_t h_in, int f,
                      uint8_t *gray_out, atq_extrema *ext, atq_extrema *fil, uint8_t *thr,
                      int min_diff);
+// frames[0..n) (n <= FRONT_MAX_FRAMES), all with shape sh.
+int launch_threshold_frames(hipStream_t s, const FrontFrame *frames, int n, const FrontShapes &sh);
 
 // ccl.hip
 int launch_ccl(hipStream_t s, const uint8_t *thr, size_t w, size_t h, uint32_t *parent,
                uint32_t *labels, uint32_t *sizes, uint32_t size_cap = 0xFFFFFFFFu);
+int launch_ccl_frames(hipStream_t s, const FrontFrame *frames, int n, const FrontShapes &sh);
 
 // boundary.hip
 int launch_find_boundaries_sparse(hipStream_t s, const uint32_t *labels, const uint32_t *sizes,

@@ -10,7 +10,10 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
+#include <tuple>
 #include <vector>
 
 #include "pipeline.hpp"
@@ -146,14 +149,48 @@ size_t carve_detector(atq_detector *d, char *base) {
         if (r_ != ATQ_OK) return r_; \
     } while (0)
 
-// threshold -> quads, all on `s`.
-int run_pipeline(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t w_in, size_t h_in) {
+// Validates a frame's dims against the detector and records them.
+int frame_setup(atq_detector *d, size_t w_in, size_t h_in) {
     const int f = d->params.decimate;
     const size_t w = decimated(w_in, f), h = decimated(h_in, f);
     TRY(check_dims(w, h));
     if (w_in > d->max_w_in || h_in > d->max_h_in || w * h > d->caps.npx) return ATQ_ERR_CAPACITY;
     d->w = w;
     d->h = h;
+    return ATQ_OK;
+}
+
+// The front-end description of the detector's frame (after frame_setup).
+FrontFrame front_frame(const atq_detector *d, const uint8_t *gray_in, size_t w_in, size_t h_in) {
+    FrontFrame fr{};
+    fr.in = gray_in;
+    fr.gray = d->gray;
+    fr.ext = d->ext;
+    fr.fil = d->fil;
+    fr.thr = d->thr;
+    fr.parent = d->parent;
+    fr.labels = d->labels;
+    fr.sizes = d->sizes;
+    fr.w_in = (uint32_t)w_in;
+    fr.h_in = (uint32_t)h_in;
+    fr.f = (uint32_t)d->params.decimate;
+    fr.w = (uint32_t)d->w;
+    fr.h = (uint32_t)d->h;
+    // sizes only decide "at least min_component_size" here
+    fr.size_cap = (uint32_t)std::max(d->params.min_component_size, 1);
+    fr.min_diff = d->params.min_white_black_diff;
+    return fr;
+}
+
+int mark(atq_detector *d, hipStream_t s, int i) {
+    if (d->timing && hipEventRecord(d->ev[i], s) != hipSuccess) return ATQ_ERR_HIP;
+    return ATQ_OK;
+}
+
+// boundary -> quads on `s`, after the frame's threshold and CCL.
+int run_back(atq_detector *d, hipStream_t s, const uint8_t *gray_in) {
+    const int f = d->params.decimate;
+    const size_t w = d->w, h = d->h;
     const Caps c = Caps::for_pixels(w * h);  // this frame's own bounds
     Workspace &ws = d->ws;
     atq_counts *cnt = d->counts;
@@ -161,28 +198,16 @@ int run_pipeline(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t 
     sc.flags = d->scan_flags;
     sc.flag_count = scan_flag_count(d->caps);
     sc.error = &cnt->overflow;
-
-    auto mark = [&](int i) -> int {
-        if (d->timing && hipEventRecord(d->ev[i], s) != hipSuccess) return ATQ_ERR_HIP;
-        return ATQ_OK;
-    };
-    // one fill: the scan flags, the pair table and the counts (with the key
-    // width set)
+    // one fill: the scan flags, the pair table, the long-segment counts and
+    // the counts (with the key width set)
     const RadixPlan plan = radix_plan(w * h, 2);
     uint32_t *zero_begin = reinterpret_cast<uint32_t *>(d->scan_flags);
     const uint32_t zero_words = (uint32_t)((reinterpret_cast<char *>(cnt + 1) -
                                             reinterpret_cast<char *>(zero_begin)) / 4);
     const uint32_t lb_word = (uint32_t)(reinterpret_cast<uint32_t *>(&cnt->key_label_bits) - zero_begin);
     TRY(launch_zero_u32(s, zero_begin, nullptr, zero_words, lb_word, (uint32_t)plan.label_bits));
-    TRY(mark(0));
-    TRY(launch_threshold(s, gray_in, w_in, h_in, f, d->gray, d->ext, d->fil, d->thr,
-                         d->params.min_white_black_diff));
-    TRY(mark(1 + ATQ_STAGE_THRESHOLD));
     const uint8_t *grey = f > 1 ? d->gray : gray_in;
-    // sizes only decide "at least min_component_size" here
-    TRY(launch_ccl(s, d->thr, w, h, d->parent, d->labels, d->sizes,
-                   (uint32_t)std::max(d->params.min_component_size, 1)));
-    TRY(mark(1 + ATQ_STAGE_CCL));
+    auto mark = [&](int i) { return ::mark(d, s, i); };
     // Boundary records of the pairs that can pass ValidBlobFilter, in the
     // reference's order, keyed by the packed pair key min<<lb|max with the
     // plane in 2 low bits; the stable sort then yields the reference's
@@ -230,6 +255,18 @@ int run_pipeline(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t 
                      &cnt->n_quads, sc, d->params.min_tag_width));
     TRY(mark(1 + ATQ_STAGE_QUADS));
     return ATQ_OK;
+}
+
+// threshold -> quads, all on `s`.
+int run_pipeline(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t w_in, size_t h_in) {
+    TRY(frame_setup(d, w_in, h_in));
+    TRY(mark(d, s, 0));
+    const FrontFrame fr = front_frame(d, gray_in, w_in, h_in);
+    TRY(launch_threshold_frames(s, &fr, 1, d->shapes));
+    TRY(mark(d, s, 1 + ATQ_STAGE_THRESHOLD));
+    TRY(launch_ccl_frames(s, &fr, 1, d->shapes));
+    TRY(mark(d, s, 1 + ATQ_STAGE_CCL));
+    return run_back(d, s, gray_in);
 }
 
 }  // namespace
@@ -447,6 +484,110 @@ int atq_detect_quads(atq_detector *det, void *stream, const uint8_t *gray_in, si
     return run_pipeline(det, S(stream), gray_in, width_in, height_in);
 }
 
+int atq_detector_set_shapes(atq_detector *det, const atq_shapes *shapes) {
+    if (!det || !shapes) return ATQ_ERR_BAD_ARG;
+    const FrontShapes sh{shapes->threshold_tiles_x, shapes->threshold_tiles_y,
+                         shapes->ccl_block_rows};
+    if (!front_shapes_supported(sh)) return ATQ_ERR_BAD_ARG;
+    det->shapes = sh;
+    return ATQ_OK;
+}
+
+int atq_detector_get_shapes(atq_detector *det, atq_shapes *shapes) {
+    if (!det || !shapes) return ATQ_ERR_BAD_ARG;
+    *shapes = atq_shapes{det->shapes.thr_rx, det->shapes.thr_ry, det->shapes.ccl_rows};
+    return ATQ_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Shapes chosen so far, per (width_in, height_in, decimate).
+std::mutex g_tune_mutex;
+std::map<std::tuple<size_t, size_t, int>, FrontShapes> g_tuned;
+
+// Milliseconds of `reps` back-to-back runs of `run` on s (after one warm-up).
+template <typename Run>
+int time_runs(hipStream_t s, hipEvent_t e0, hipEvent_t e1, int reps, Run run, float *ms) {
+    TRY(run());
+    ATQ_HIP_CHECK(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; r++) TRY(run());
+    ATQ_HIP_CHECK(hipEventRecord(e1, s));
+    ATQ_HIP_CHECK(hipEventSynchronize(e1));
+    ATQ_HIP_CHECK(hipEventElapsedTime(ms, e0, e1));
+    return ATQ_OK;
+}
+
+// Times the threshold shapes (CCL shape fixed), then the CCL shapes on the
+// thresholded frame, and keeps the fastest of each.
+int tune_frame(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t w_in, size_t h_in,
+               FrontShapes *best) {
+    TRY(frame_setup(d, w_in, h_in));
+    const FrontFrame fr = front_frame(d, gray_in, w_in, h_in);
+    FrontShapes cand[16];
+    const int nc = std::min(front_shape_candidates(cand, 16), 16);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess) return ATQ_ERR_HIP;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return ATQ_ERR_HIP;
+    }
+    constexpr int REPS = 5;
+    int rc = ATQ_OK;
+    FrontShapes pick = d->shapes;
+    float best_thr = 1e30f, best_ccl = 1e30f;
+    for (int i = 0; i < nc && rc == ATQ_OK; i++) {
+        if (cand[i].ccl_rows != pick.ccl_rows) continue;  // threshold shapes first
+        float ms = 0;
+        rc = time_runs(s, e0, e1, REPS, [&] { return launch_threshold_frames(s, &fr, 1, cand[i]); },
+                       &ms);
+        if (rc == ATQ_OK && ms < best_thr) {
+            best_thr = ms;
+            pick.thr_rx = cand[i].thr_rx;
+            pick.thr_ry = cand[i].thr_ry;
+        }
+    }
+    if (rc == ATQ_OK) rc = launch_threshold_frames(s, &fr, 1, pick);
+    int rows = pick.ccl_rows;
+    for (int i = 0; i < nc && rc == ATQ_OK; i++) {
+        if (cand[i].thr_rx != pick.thr_rx || cand[i].thr_ry != pick.thr_ry) continue;
+        float ms = 0;
+        rc = time_runs(s, e0, e1, REPS, [&] { return launch_ccl_frames(s, &fr, 1, cand[i]); }, &ms);
+        if (rc == ATQ_OK && ms < best_ccl) {
+            best_ccl = ms;
+            rows = cand[i].ccl_rows;
+        }
+    }
+    pick.ccl_rows = rows;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == ATQ_OK) *best = pick;
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int atq_autotune(atq_detector *det, void *stream, const uint8_t *gray_in, size_t width_in,
+                 size_t height_in, atq_shapes *best) {
+    if (!det || !gray_in) return ATQ_ERR_BAD_ARG;
+    const auto key = std::make_tuple(width_in, height_in, det->params.decimate);
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) {
+        det->shapes = it->second;
+    } else {
+        FrontShapes sh;
+        TRY(tune_frame(det, S(stream), gray_in, width_in, height_in, &sh));
+        g_tuned[key] = sh;
+        det->shapes = sh;
+    }
+    if (best) *best = atq_shapes{det->shapes.thr_rx, det->shapes.thr_ry, det->shapes.ccl_rows};
+    return ATQ_OK;
+}
+
 int atq_detector_buffer(atq_detector *d, int which, void **ptr, size_t *capacity) {
     if (!d || !ptr || !capacity) return ATQ_ERR_BAD_ARG;
     const Caps &c = d->caps;
@@ -527,19 +668,42 @@ struct atq_batch {
     hipEvent_t joins[N_STREAMS] = {};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    bool tuned = false;
 };
 
 namespace {
 
-// Every frame's pipeline on internal streams, forked from and joined to `s`.
+// Frames go to the internal streams round-robin. On each stream, the
+// thresholds and CCLs of its frames run as one launch chain per front-end
+// shape (a table of frames per launch), then each frame's remaining stages;
+// the streams are forked from and joined to `s`.
 int batch_branches(atq_batch *b, hipStream_t s) {
+    const size_t n = b->dets.size();
+    const int ns = (int)std::min<size_t>(atq_batch::N_STREAMS, n);
+    for (size_t i = 0; i < n; i++) TRY(frame_setup(b->dets[i], b->w_in[i], b->h_in[i]));
     ATQ_HIP_CHECK(hipEventRecord(b->fork, s));
-    const int ns = (int)std::min<size_t>(atq_batch::N_STREAMS, b->dets.size());
-    for (int k = 0; k < ns; k++) ATQ_HIP_CHECK(hipStreamWaitEvent(b->streams[k], b->fork, 0));
-    for (size_t i = 0; i < b->dets.size(); i++)
-        TRY(run_pipeline(b->dets[i], b->streams[i % ns], b->inputs[i], b->w_in[i], b->h_in[i]));
     for (int k = 0; k < ns; k++) {
-        ATQ_HIP_CHECK(hipEventRecord(b->joins[k], b->streams[k]));
+        hipStream_t sk = b->streams[k];
+        ATQ_HIP_CHECK(hipStreamWaitEvent(sk, b->fork, 0));
+        std::vector<bool> done(n, false);
+        for (size_t i = (size_t)k; i < n; i += ns) {
+            if (done[i]) continue;
+            const FrontShapes sh = b->dets[i]->shapes;
+            FrontFrame group[FRONT_MAX_FRAMES];
+            int g = 0;
+            for (size_t j = i; j < n && g < FRONT_MAX_FRAMES; j += ns) {
+                const FrontShapes &o = b->dets[j]->shapes;
+                if (done[j] || o.thr_rx != sh.thr_rx || o.thr_ry != sh.thr_ry ||
+                    o.ccl_rows != sh.ccl_rows)
+                    continue;
+                group[g++] = front_frame(b->dets[j], b->inputs[j], b->w_in[j], b->h_in[j]);
+                done[j] = true;
+            }
+            TRY(launch_threshold_frames(sk, group, g, sh));
+            TRY(launch_ccl_frames(sk, group, g, sh));
+        }
+        for (size_t i = (size_t)k; i < n; i += ns) TRY(run_back(b->dets[i], sk, b->inputs[i]));
+        ATQ_HIP_CHECK(hipEventRecord(b->joins[k], sk));
         ATQ_HIP_CHECK(hipStreamWaitEvent(s, b->joins[k], 0));
     }
     return ATQ_OK;
@@ -612,9 +776,18 @@ int atq_batch_frame(atq_batch *b, size_t i, atq_detector **det) {
     return ATQ_OK;
 }
 
+int atq_batch_autotune(atq_batch *b, void *stream) {
+    if (!b) return ATQ_ERR_BAD_ARG;
+    for (size_t i = 0; i < b->dets.size(); i++)
+        TRY(atq_autotune(b->dets[i], stream, b->inputs[i], b->w_in[i], b->h_in[i], nullptr));
+    b->tuned = true;
+    return ATQ_OK;
+}
+
 int atq_batch_detect(atq_batch *b, void *stream, int use_graph) {
     if (!b) return ATQ_ERR_BAD_ARG;
     hipStream_t s = S(stream);
+    if (!b->tuned) TRY(atq_batch_autotune(b, stream));
     if (!use_graph) return batch_branches(b, s);
     if (!s) return ATQ_ERR_BAD_ARG;
     if (!b->exec) {

@@ -83,6 +83,7 @@ struct atq_detector {
     uint32_t *long_n;      // long-segment counts of the two segment sorts, zeroed likewise
     atq_counts *counts;
     atq::Workspace ws;
+    atq::FrontShapes shapes;  // threshold region and CCL tile shapes
     size_t w, h;
     bool sorted_in_alt;
     bool timing;

@@ -2,6 +2,21 @@
 #pragma once
 #include "common.hpp"
 
+// The record types are layout-identical to the reference's
+// (src/main.cpp:435 bit-casts BoundaryPoint to u32, :575 ClusterBounds to
+// vec<int64,4>, :794 LineFitPoint to vec<double,8>; include/fit_quad.h).
+static_assert(sizeof(atq_extrema) == 2, "vec<u8,2>");
+static_assert(sizeof(atq_boundary_point) == 4, "BoundaryPoint");
+static_assert(sizeof(atq_cluster_point) == 8, "ClusterPoint");
+static_assert(sizeof(atq_cluster_extents) == 8, "ClusterExtents");
+static_assert(sizeof(atq_cluster_bounds) == 32, "ClusterBounds");
+static_assert(sizeof(atq_line_fit_point) == 64, "LineFitPoint");
+static_assert(sizeof(atq_peak_extents) == 8, "PeakExtents");
+static_assert(sizeof(atq_corner) == 12, "Corner");
+static_assert(sizeof(atq_fitted_quad) == 272, "FittedQuad");
+static_assert(sizeof(atq_quad_corners) == 40, "QuadCorners");
+static_assert(sizeof(atq_shapes) == 12, "atq_shapes");
+
 namespace atq {
 
 // include/boundary_point.h:53-54, 63-95

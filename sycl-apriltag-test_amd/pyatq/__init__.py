@@ -55,6 +55,10 @@ SIGNATURES = {
     "atq_batch_input": (C.c_int, [_P, _S, C.POINTER(_P)]),
     "atq_batch_detect": (C.c_int, [_P, _P, C.c_int]),
     "atq_batch_frame": (C.c_int, [_P, _S, C.POINTER(_P)]),
+    "atq_batch_autotune": (C.c_int, [_P, _P]),
+    "atq_detector_set_shapes": (C.c_int, [_P, _P]),
+    "atq_detector_get_shapes": (C.c_int, [_P, _P]),
+    "atq_autotune": (C.c_int, [_P, _P, _P, _S, _S, _P]),
     "atq_linefit_corners_forward": (C.c_int, [_P, _P, _S, _S, _P, _S, _P, _P, _S, _P, _P]),
     "atq_linefit_corners_backward": (C.c_int, [_P, _P, _S, _S, _P, _S, _P, _P, _S, _P, _P, _P]),
 }
@@ -75,6 +79,17 @@ class AtqParams(C.Structure):
                 ("min_component_size", C.c_int32), ("min_tag_width", C.c_int32),
                 ("pair_table_log2", C.c_int32)]
 
+
+class AtqShapes(C.Structure):
+    _fields_ = [("threshold_tiles_x", C.c_int32), ("threshold_tiles_y", C.c_int32),
+                ("ccl_block_rows", C.c_int32)]
+
+    def as_tuple(self):
+        return (self.threshold_tiles_x, self.threshold_tiles_y, self.ccl_block_rows)
+
+
+# every (threshold_tiles_x, threshold_tiles_y, ccl_block_rows) the library takes
+SHAPES = [(rx, ry, rows) for rows in (16, 8) for rx, ry in ((32, 16), (64, 8), (32, 8), (16, 16))]
 
 _lib = None
 
@@ -388,6 +403,22 @@ class Detector:
 
     def set_timing(self, on=True):
         check(lib().atq_detector_set_timing(self.handle, 1 if on else 0))
+
+    def set_shapes(self, shapes):
+        sh = AtqShapes(*shapes)
+        check(lib().atq_detector_set_shapes(self.handle, C.byref(sh)))
+
+    def shapes(self):
+        sh = AtqShapes()
+        check(lib().atq_detector_get_shapes(self.handle, C.byref(sh)))
+        return sh.as_tuple()
+
+    def autotune(self, gray_dev, w_in, h_in, stream_handle=None):
+        """Times every shape on this frame and keeps the fastest (atq_autotune)."""
+        s = C.c_void_p(stream_handle) if stream_handle is not None else stream()
+        sh = AtqShapes()
+        check(lib().atq_autotune(self.handle, s, P(gray_dev), w_in, h_in, C.byref(sh)))
+        return sh.as_tuple()
 
     def stage_times(self):
         """Milliseconds per stage of the last frame (needs set_timing(True))."""

@@ -190,3 +190,42 @@ def test_detector_stress_12mp(atq, oracle):
     assert int(o["extents"]["count"].max()) > 16384
     g = run_detector(atq, img, 1)
     check_against_oracle(g, o)
+
+
+def test_every_front_shape_gives_identical_results(atq, oracle):
+    """Threshold regions and CCL tiles only split the work differently: the
+    thresholded image, the labels and the quads are the same for every shape
+    the library takes, and equal to the oracle's."""
+    from pyatq import synth
+    img = synth.plain_1080p(2005).image
+    h_in, w_in = img.shape
+    want, _ = oracle.detect(img, 1)
+    o_labels, _ = oracle.segmentation(oracle.threshold(img)[2])
+    dev = atq.dev_from_np(img)
+    det = atq.Detector(w_in, h_in)
+    with pytest.raises(atq.AtqError):
+        det.set_shapes((48, 8, 16))
+    for sh in atq.SHAPES:
+        det.set_shapes(sh)
+        assert det.shapes() == sh
+        det.detect(dev, w_in, h_in)
+        atq.sync()
+        labels = det.read("LABELS", np.uint32, w_in * h_in)
+        np.testing.assert_array_equal(labels, o_labels.reshape(-1), err_msg=str(sh))
+        np.testing.assert_array_equal(_bytes(det.quads()), _bytes(want), err_msg=str(sh))
+    det.close()
+
+
+def test_autotune_picks_a_supported_shape(atq, oracle):
+    from pyatq import synth
+    img = synth.plain_1080p(2006, decimate=2).image
+    h_in, w_in = img.shape
+    dev = atq.dev_from_np(img)
+    det = atq.Detector(w_in, h_in, decimate=2)
+    sh = det.autotune(dev, w_in, h_in)
+    assert sh in atq.SHAPES and det.shapes() == sh
+    det.detect(dev, w_in, h_in)
+    atq.sync()
+    want, _ = oracle.detect(img, 2)
+    np.testing.assert_array_equal(_bytes(det.quads()), _bytes(want))
+    det.close()
