@@ -28,7 +28,7 @@ struct FrontTable {
 // Front-end shapes: threshold regions of rx x ry 4x4-pixel tiles, CCL tiles
 // of 32 x ccl_rows 2x2-pixel blocks. Results do not depend on them.
 struct FrontShapes {
-    int thr_rx = 32, thr_ry = 16, ccl_rows = 16;
+    int thr_rx = 64, thr_ry = 16, ccl_rows = 16;
 };
 bool front_shapes_supported(const FrontShapes &sh);
 // Every FrontShapes value front_shapes_supported accepts.

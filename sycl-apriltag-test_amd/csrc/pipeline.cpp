@@ -520,7 +520,8 @@ int time_runs(hipStream_t s, hipEvent_t e0, hipEvent_t e1, int reps, Run run, fl
 }
 
 // Times the threshold shapes (CCL shape fixed), then the CCL shapes on the
-// thresholded frame, and keeps the fastest of each.
+// thresholded frame, and keeps the fastest of each. Candidates are timed in
+// interleaved trials of REPS runs; a candidate's time is its best trial.
 int tune_frame(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t w_in, size_t h_in,
                FrontShapes *best) {
     TRY(frame_setup(d, w_in, h_in));
@@ -533,33 +534,41 @@ int tune_frame(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t w_
         (void)hipEventDestroy(e0);
         return ATQ_ERR_HIP;
     }
-    constexpr int REPS = 5;
+    constexpr int REPS = 10, TRIALS = 3;
     int rc = ATQ_OK;
     FrontShapes pick = d->shapes;
-    float best_thr = 1e30f, best_ccl = 1e30f;
-    for (int i = 0; i < nc && rc == ATQ_OK; i++) {
-        if (cand[i].ccl_rows != pick.ccl_rows) continue;  // threshold shapes first
-        float ms = 0;
-        rc = time_runs(s, e0, e1, REPS, [&] { return launch_threshold_frames(s, &fr, 1, cand[i]); },
-                       &ms);
-        if (rc == ATQ_OK && ms < best_thr) {
-            best_thr = ms;
-            pick.thr_rx = cand[i].thr_rx;
-            pick.thr_ry = cand[i].thr_ry;
+    float t_thr[16], t_ccl[16];
+    for (int i = 0; i < nc; i++) t_thr[i] = t_ccl[i] = 1e30f;
+    for (int trial = 0; trial < TRIALS && rc == ATQ_OK; trial++)
+        for (int i = 0; i < nc && rc == ATQ_OK; i++) {
+            if (cand[i].ccl_rows != pick.ccl_rows) continue;  // threshold shapes first
+            float ms = 0;
+            rc = time_runs(s, e0, e1, REPS,
+                           [&] { return launch_threshold_frames(s, &fr, 1, cand[i]); }, &ms);
+            t_thr[i] = std::min(t_thr[i], ms);
         }
+    int bi = -1;
+    for (int i = 0; i < nc; i++)
+        if (cand[i].ccl_rows == pick.ccl_rows && (bi < 0 || t_thr[i] < t_thr[bi])) bi = i;
+    if (bi >= 0) {
+        pick.thr_rx = cand[bi].thr_rx;
+        pick.thr_ry = cand[bi].thr_ry;
     }
     if (rc == ATQ_OK) rc = launch_threshold_frames(s, &fr, 1, pick);
-    int rows = pick.ccl_rows;
-    for (int i = 0; i < nc && rc == ATQ_OK; i++) {
-        if (cand[i].thr_rx != pick.thr_rx || cand[i].thr_ry != pick.thr_ry) continue;
-        float ms = 0;
-        rc = time_runs(s, e0, e1, REPS, [&] { return launch_ccl_frames(s, &fr, 1, cand[i]); }, &ms);
-        if (rc == ATQ_OK && ms < best_ccl) {
-            best_ccl = ms;
-            rows = cand[i].ccl_rows;
+    for (int trial = 0; trial < TRIALS && rc == ATQ_OK; trial++)
+        for (int i = 0; i < nc && rc == ATQ_OK; i++) {
+            if (cand[i].thr_rx != pick.thr_rx || cand[i].thr_ry != pick.thr_ry) continue;
+            float ms = 0;
+            rc = time_runs(s, e0, e1, REPS, [&] { return launch_ccl_frames(s, &fr, 1, cand[i]); },
+                           &ms);
+            t_ccl[i] = std::min(t_ccl[i], ms);
         }
-    }
-    pick.ccl_rows = rows;
+    bi = -1;
+    for (int i = 0; i < nc; i++)
+        if (cand[i].thr_rx == pick.thr_rx && cand[i].thr_ry == pick.thr_ry &&
+            (bi < 0 || t_ccl[i] < t_ccl[bi]))
+            bi = i;
+    if (bi >= 0) pick.ccl_rows = cand[bi].ccl_rows;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc == ATQ_OK) *best = pick;

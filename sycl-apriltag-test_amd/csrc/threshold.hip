@@ -27,128 +27,198 @@ __global__ void k_decimate(const uint8_t *__restrict__ in, uint32_t w_in, uint32
     out[(size_t)y * w + x] = in[(size_t)(y * f) * w_in + (size_t)x * f];
 }
 
+// Decimated pixels x .. x+15 of row y (x a multiple of 16, y < h) as four
+// little-endian words; pixels at or past w read 0.
+ATQ_DEV void load_strip_row(const FrontFrame &F, bool vec_in, int x, int y, uint32_t (&wd)[4]) {
+    const uint32_t w = F.w, w_in = F.w_in, f = F.f;
+    if (f == 1) {
+        const uint8_t *row = F.in + (size_t)y * w_in;
+        if (vec_in && x + 16 <= (int)w) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(row + x);
+            wd[0] = v.x;
+            wd[1] = v.y;
+            wd[2] = v.z;
+            wd[3] = v.w;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                wd[q] = x + 4 * q < (int)w ? *reinterpret_cast<const uint32_t *>(row + x + 4 * q) : 0u;
+        }
+        return;
+    }
+    const uint8_t *row = F.in + (size_t)(y * f) * w_in;
+    if (f == 2 && vec_in && 2 * x + 32 <= (int)w_in) {
+        // even bytes of 32 input bytes
+        const uint4 a = *reinterpret_cast<const uint4 *>(row + 2 * x);
+        const uint4 b = *reinterpret_cast<const uint4 *>(row + 2 * x + 16);
+        auto pack = [](uint32_t u, uint32_t v) {
+            return (u & 0xFFu) | ((u >> 8) & 0xFF00u) | ((v & 0xFFu) << 16) | ((v & 0xFF0000u) << 8);
+        };
+        wd[0] = pack(a.x, a.y);
+        wd[1] = pack(a.z, a.w);
+        wd[2] = pack(b.x, b.y);
+        wd[3] = pack(b.z, b.w);
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        uint32_t word = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int xx = x + 4 * q + j;
+            if (xx < (int)w) word |= (uint32_t)row[(size_t)xx * f] << (8 * j);
+        }
+        wd[q] = word;
+    }
+}
+
+// Min and max of the bytes of four words (one 4x4 tile).
+ATQ_DEV void tile_minmax(uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t &mn,
+                         uint32_t &mx) {
+    mn = 255;
+    mx = 0;
+    const uint32_t v4[4] = {a, b, c, d};
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t v = (v4[k] >> (8 * j)) & 0xFFu;
+            mn = min(mn, v);
+            mx = max(mx, v);
+        }
+}
+
+// Tile extrema of a region and its one-tile halo ring (out-of-image tiles
+// are neutral: min 255, max 0).
 template <int RX, int RY>
 struct ThresholdLds {
-    static constexpr int PW = (RX + 2) * 4;  // region width in pixels incl. halo
-    static constexpr int PH = (RY + 2) * 4;
-    static constexpr int TW = RX + 2, TH = RY + 2;
-    uint32_t pix[PH][PW / 4];
-    uint8_t emin[TH][TW], emax[TH][TW];
-    uint8_t fmin[RY][RX], fmax[RY][RX];
+    uint8_t mn[RY + 2][RX + 2], mx[RY + 2][RX + 2];
 };
 
-// One region of frame F (all 256 threads): stage, extrema, dilation, output.
+// One region of RX x RY tiles of frame F. Each of the 256 threads owns a
+// strip of 4 tiles (4 rows x 16 pixels) and keeps its pixels in registers:
+// it loads them, writes the strip's tile extrema to LDS (threads < the ring
+// size also load one halo tile), and after one barrier dilates its tiles
+// over their 3x3 neighbourhoods and thresholds and stores its 64 pixels.
 template <int RX, int RY>
 ATQ_DEV void threshold_region(ThresholdLds<RX, RY> &L, const FrontFrame &F, uint32_t region,
-                              uint32_t regions_x) {
-    using Lds = ThresholdLds<RX, RY>;
-    constexpr int PW = Lds::PW, PH = Lds::PH, TW = Lds::TW, TH = Lds::TH;
-    const uint8_t *__restrict__ in = F.in;
-    const uint32_t w_in = F.w_in, f = F.f, w = F.w, h = F.h;
-    uint8_t *__restrict__ gray_out = f > 1 ? F.gray : nullptr;
+                              uint32_t regions_x, bool vec_in, bool vec_out) {
+    constexpr int SX = RX / 4;
+    static_assert(RX % 4 == 0 && SX * RY == 256, "one 4-tile strip per thread");
+    constexpr int RING = 2 * (RX + 2) + 2 * RY;
+    static_assert(RING <= 256, "one halo tile per thread");
+    const uint32_t w = F.w, h = F.h;
     const int tw = (int)(w / 4), th = (int)(h / 4);
-    const int tx0 = (int)(region % regions_x) * RX - 1;  // first (halo) tile of the region
-    const int ty0 = (int)(region / regions_x) * RY - 1;
-    const int px0 = tx0 * 4, py0 = ty0 * 4;
     const int tid = threadIdx.x;
+    const int tx0 = (int)(region % regions_x) * RX, ty0 = (int)(region / regions_x) * RY;
+    const int sx = tid % SX, sy = tid / SX;
+    const int ty = ty0 + sy, tx = tx0 + 4 * sx;  // the strip's first tile
+    const bool row_in = ty < th;
 
-    // 1. stage decimated pixels of the region (words of 4 pixels); every
-    // load of a thread is issued before the first LDS store
-    constexpr int WORDS = PH * (PW / 4);
-    constexpr int PER = (WORDS + 255) / 256;
-    uint32_t words[PER];
+    // every load of the thread before any use
+    uint32_t px[4][4] = {};
+    if (row_in) {
 #pragma unroll
-    for (int k = 0; k < PER; k++) {
-        const int i = tid + 256 * k;
-        const int r = i / (PW / 4), c = i % (PW / 4);
-        const int y = py0 + r, x = px0 + c * 4;
-        uint32_t word = 0;
-        if (i < WORDS && y >= 0 && y < (int)h && x >= 0 && x < (int)w) {
+        for (int r = 0; r < 4; r++) load_strip_row(F, vec_in, 4 * tx, 4 * ty + r, px[r]);
+    }
+    int hx = 0, hy = 0;  // halo tile (region-relative, -1 .. RX / RY)
+    if (tid < RING) {
+        if (tid < RX + 2) { hx = tid - 1; hy = -1; }
+        else if (tid < 2 * (RX + 2)) { hx = tid - (RX + 2) - 1; hy = RY; }
+        else if (tid < 2 * (RX + 2) + RY) { hx = -1; hy = tid - 2 * (RX + 2); }
+        else { hx = RX; hy = tid - 2 * (RX + 2) - RY; }
+    }
+    const int htx = tx0 + hx, hty = ty0 + hy;
+    const bool halo_in = tid < RING && htx >= 0 && htx < tw && hty >= 0 && hty < th;
+    uint32_t hp[4] = {0, 0, 0, 0};
+    if (halo_in) {
+        const uint32_t f = F.f;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const uint8_t *row = F.in + (size_t)((4 * hty + r) * f) * F.w_in;
             if (f == 1) {
-                word = *reinterpret_cast<const uint32_t *>(in + (size_t)y * w_in + x);
-            } else if (f == 2 && (w_in & 3) == 0) {
-                const uint32_t *row = reinterpret_cast<const uint32_t *>(in + (size_t)(2 * y) * w_in);
-                uint32_t a = row[(2 * x) / 4], b = row[(2 * x) / 4 + 1];
-                word = (a & 0xFFu) | ((a >> 8) & 0xFF00u) | ((b & 0xFFu) << 16) | ((b & 0xFF0000u) << 8);
+                hp[r] = *reinterpret_cast<const uint32_t *>(row + 4 * htx);
             } else {
-                const uint8_t *row = in + (size_t)(y * f) * w_in;
 #pragma unroll
-                for (int q = 0; q < 4; q++) word |= (uint32_t)row[(size_t)(x + q) * f] << (8 * q);
+                for (int j = 0; j < 4; j++) hp[r] |= (uint32_t)row[(size_t)(4 * htx + j) * f] << (8 * j);
             }
         }
-        words[k] = word;
     }
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
-        const int i = tid + 256 * k;
-        if (i < WORDS) L.pix[i / (PW / 4)][i % (PW / 4)] = words[k];
-    }
-    __syncthreads();
 
-    // 2. raw tile extrema (out-of-image tiles are neutral: min 255, max 0)
-    for (int i = tid; i < TH * TW; i += 256) {
-        int r = i / TW, c = i % TW;
-        int ty = ty0 + r, tx = tx0 + c;
+    // tile extrema of the strip and of the halo tile
+    uint32_t smn[4], smx[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        smn[q] = 255;
+        smx[q] = 0;
+        if (row_in && tx + q < tw) tile_minmax(px[0][q], px[1][q], px[2][q], px[3][q], smn[q], smx[q]);
+        L.mn[sy + 1][4 * sx + q + 1] = (uint8_t)smn[q];
+        L.mx[sy + 1][4 * sx + q + 1] = (uint8_t)smx[q];
+    }
+    if (F.ext && row_in) {
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if (tx + q < tw)
+                F.ext[(size_t)ty * tw + tx + q] = atq_extrema{(uint8_t)smn[q], (uint8_t)smx[q]};
+    }
+    if (tid < RING) {
         uint32_t mn = 255, mx = 0;
-        if (ty >= 0 && ty < th && tx >= 0 && tx < tw) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                uint32_t wv = L.pix[r * 4 + k][c];
-#pragma unroll
-                for (int b = 0; b < 4; b++) {
-                    uint32_t v = (wv >> (8 * b)) & 0xFFu;
-                    mn = min(mn, v);
-                    mx = max(mx, v);
-                }
-            }
-            if (F.ext && r >= 1 && r <= RY && c >= 1 && c <= RX)
-                F.ext[(size_t)ty * tw + tx] = atq_extrema{(uint8_t)mn, (uint8_t)mx};
-        }
-        L.emin[r][c] = (uint8_t)mn;
-        L.emax[r][c] = (uint8_t)mx;
+        if (halo_in) tile_minmax(hp[0], hp[1], hp[2], hp[3], mn, mx);
+        L.mn[hy + 1][hx + 1] = (uint8_t)mn;
+        L.mx[hy + 1][hx + 1] = (uint8_t)mx;
     }
     __syncthreads();
 
-    // 3. 3x3 dilation of the interior tiles
-    for (int i = tid; i < RY * RX; i += 256) {
-        int r = i / RX, c = i % RX;
+    // 3x3 dilation of the strip's tiles, then the threshold of its pixels:
+    // 127 if max-min < min_diff, else 255 if v > min+(max-min)/2, else 0
+    uint32_t out[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int c = 4 * sx + q;  // interior tile column; LDS column c + 1
         uint32_t mn = 255, mx = 0;
 #pragma unroll
         for (int dr = 0; dr < 3; dr++)
 #pragma unroll
             for (int dc = 0; dc < 3; dc++) {
-                mn = min(mn, (uint32_t)L.emin[r + dr][c + dc]);
-                mx = max(mx, (uint32_t)L.emax[r + dr][c + dc]);
+                mn = min(mn, (uint32_t)L.mn[sy + dr][c + dc]);
+                mx = max(mx, (uint32_t)L.mx[sy + dr][c + dc]);
             }
-        L.fmin[r][c] = (uint8_t)mn;
-        L.fmax[r][c] = (uint8_t)mx;
-        int ty = ty0 + 1 + r, tx = tx0 + 1 + c;
-        if (F.fil && ty < th && tx < tw)
-            F.fil[(size_t)ty * tw + tx] = atq_extrema{(uint8_t)mn, (uint8_t)mx};
-    }
-    __syncthreads();
-
-    // 4. threshold the interior: one word (4 pixels of one tile row) per item
-    for (int i = tid; i < RY * 4 * RX; i += 256) {
-        int r = i / RX, c = i % RX;          // pixel row in interior, tile column
-        int y = py0 + 4 + r, x = px0 + 4 + c * 4;
-        if (y >= (int)h || x >= (int)w) continue;
-        uint32_t wv = L.pix[4 + r][1 + c];
-        uint32_t mn = L.fmin[r / 4][c], mx = L.fmax[r / 4][c];
-        uint32_t out;
-        if ((int)(mx - mn) < F.min_diff) {
-            out = 0x7F7F7F7Fu;
-        } else {
-            uint32_t comp = (mn + (mx - mn) / 2) & 0xFFu;
-            out = 0;
+        if (F.fil && row_in && tx + q < tw)
+            F.fil[(size_t)ty * tw + tx + q] = atq_extrema{(uint8_t)mn, (uint8_t)mx};
+        const bool flat = (int)(mx - mn) < F.min_diff;
+        const uint32_t comp = (mn + (mx - mn) / 2) & 0xFFu;
 #pragma unroll
-            for (int b = 0; b < 4; b++) {
-                uint32_t v = (wv >> (8 * b)) & 0xFFu;
-                out |= (v > comp ? 0xFFu : 0u) << (8 * b);
+        for (int r = 0; r < 4; r++) {
+            uint32_t o = 0x7F7F7F7Fu;
+            if (!flat) {
+                o = 0;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const uint32_t v = (px[r][q] >> (8 * j)) & 0xFFu;
+                    o |= (v > comp ? 0xFFu : 0u) << (8 * j);
+                }
+            }
+            out[r][q] = o;
+        }
+    }
+    if (row_in && tx < tw) {
+        uint8_t *gray_out = F.f > 1 ? F.gray : nullptr;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const size_t o = (size_t)(4 * ty + r) * w + 4 * tx;
+            if (vec_out && tx + 4 <= tw) {
+                *reinterpret_cast<uint4 *>(F.thr + o) = make_uint4(out[r][0], out[r][1], out[r][2], out[r][3]);
+                if (gray_out)
+                    *reinterpret_cast<uint4 *>(gray_out + o) = make_uint4(px[r][0], px[r][1], px[r][2], px[r][3]);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    if (tx + q >= tw) break;
+                    *reinterpret_cast<uint32_t *>(F.thr + o + 4 * q) = out[r][q];
+                    if (gray_out) *reinterpret_cast<uint32_t *>(gray_out + o + 4 * q) = px[r][q];
+                }
             }
         }
-        *reinterpret_cast<uint32_t *>(F.thr + (size_t)y * w + x) = out;
-        if (gray_out) *reinterpret_cast<uint32_t *>(gray_out + (size_t)y * w + x) = wv;
     }
     __syncthreads();  // the LDS is reused by the next region
 }
@@ -163,9 +233,13 @@ k_threshold(FrontTable tab) {
         const FrontFrame &F = tab.fr[fi];
         const uint32_t regions_x = (F.w / 4 + RX - 1) / RX;
         const uint32_t n_regions = regions_x * ((F.h / 4 + RY - 1) / RY);
+        const bool vec_in = (F.w_in & 15) == 0 && (reinterpret_cast<uintptr_t>(F.in) & 15) == 0;
+        const bool vec_out = (F.w & 15) == 0 && (reinterpret_cast<uintptr_t>(F.thr) & 15) == 0 &&
+                             (reinterpret_cast<uintptr_t>(F.gray) & 15) == 0;
         for (uint32_t it = blockIdx.x; it < xcd_grid(n_regions); it += gridDim.x) {
             const uint32_t region = xcd_tile(it, n_regions);
-            if (region < n_regions) threshold_region<RX, RY>(L, F, region, regions_x);
+            if (region < n_regions)
+                threshold_region<RX, RY>(L, F, region, regions_x, vec_in, vec_out);
         }
     }
 }
@@ -179,13 +253,13 @@ int launch_decimate(hipStream_t s, const uint8_t *in, size_t w_in, size_t h_in, 
 }
 
 bool front_shapes_supported(const FrontShapes &sh) {
-    const bool thr = (sh.thr_rx == 32 && sh.thr_ry == 16) || (sh.thr_rx == 64 && sh.thr_ry == 8) ||
-                     (sh.thr_rx == 32 && sh.thr_ry == 8) || (sh.thr_rx == 16 && sh.thr_ry == 16);
+    const bool thr = (sh.thr_rx == 64 && sh.thr_ry == 16) || (sh.thr_rx == 32 && sh.thr_ry == 32) ||
+                     (sh.thr_rx == 16 && sh.thr_ry == 64);
     return thr && (sh.ccl_rows == 8 || sh.ccl_rows == 16);
 }
 
 int front_shape_candidates(FrontShapes *out, int max_out) {
-    static const int thr[4][2] = {{32, 16}, {64, 8}, {32, 8}, {16, 16}};
+    static const int thr[3][2] = {{64, 16}, {32, 32}, {16, 64}};
     int n = 0;
     for (int c : {16, 8})
         for (const auto &t : thr) {
@@ -219,10 +293,9 @@ int launch_threshold_frames(hipStream_t s, const FrontFrame *frames, int n, cons
     FrontTable tab{};
     for (int i = 0; i < n; i++) tab.fr[i] = frames[i];
     tab.n = (uint32_t)n;
-    if (sh.thr_rx == 32 && sh.thr_ry == 16) threshold_launch<32, 16>(s, tab);
-    else if (sh.thr_rx == 64) threshold_launch<64, 8>(s, tab);
-    else if (sh.thr_rx == 32) threshold_launch<32, 8>(s, tab);
-    else threshold_launch<16, 16>(s, tab);
+    if (sh.thr_rx == 64) threshold_launch<64, 16>(s, tab);
+    else if (sh.thr_rx == 32) threshold_launch<32, 32>(s, tab);
+    else threshold_launch<16, 64>(s, tab);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
