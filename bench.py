@@ -109,14 +109,19 @@ def stage_profile(atq, det, devs, w, h, n, stream):
     return {k: v / n for k, v in acc.items()}
 
 
-def bench_12mp(atq, synth, stream, steps=10):
+def tuned_shapes(det, dev, w, h, stream, tune):
+    """The autotuner's shapes for this frame's resolution (or the defaults)."""
+    return det.autotune(dev, w, h, stream.handle) if tune else det.shapes()
+
+
+def bench_12mp(atq, synth, stream, steps=10, tune=True):
     """The 4096x3072 stress frame (maze, dense clutter), decimate 1."""
     t = time.time()
     big = synth.stress_12mp().image
     log(f"generated 12MP frame in {time.time() - t:.1f}s")
     dbig = [atq.dev_from_np(big)]
     det = atq.Detector(4096, 3072, decimate=1)
-    shapes = det.autotune(dbig[0], 4096, 3072, stream.handle)
+    shapes = tuned_shapes(det, dbig[0], 4096, 3072, stream, tune)
     wall, ev = time_frames(atq, det, dbig, 4096, 3072, steps, 2, stream)
     stages = stage_profile(atq, det, dbig, 4096, 3072, 5, stream)
     det.detect(dbig[0], 4096, 3072, stream.handle)
@@ -213,6 +218,8 @@ def main():
     ap.add_argument("--only-12mp", action="store_true",
                     help="only the 4096x3072 leg, --steps frames (for profiling)")
     ap.add_argument("--batch-rounds", type=int, default=20)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default front-end shapes (no autotuner runs, e.g. under a profiler)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -235,7 +242,8 @@ def main():
         src.free()
         dst.free()
     if args.only_12mp:
-        print(json.dumps({"stress_12mp": bench_12mp(atq, synth, atq.Stream(), args.steps)}))
+        print(json.dumps({"stress_12mp": bench_12mp(atq, synth, atq.Stream(), args.steps,
+                                                     not args.no_tune)}))
         return
     t = time.time()
     frames = [synth.plain_1080p(2000 + i).image for i in range(args.frames)]
@@ -246,7 +254,7 @@ def main():
     res = {}
     for f in ((1,) if args.skip_f2 else (1, 2)):
         det = atq.Detector(W1080, H1080, decimate=f)
-        shapes = det.autotune(devs[0], W1080, H1080, stream.handle)
+        shapes = tuned_shapes(det, devs[0], W1080, H1080, stream, not args.no_tune)
         wall, ev = time_frames(atq, det, devs, W1080, H1080, args.steps, args.warmup, stream)
         stages = stage_profile(atq, det, devs, W1080, H1080, 20, stream)
         det.detect(devs[0], W1080, H1080, stream.handle)
@@ -303,7 +311,7 @@ def main():
         extra["stage_ms_1080p_f2"] = {k: round(v, 5) for k, v in res[2]["stages"].items()}
 
     if not args.skip_12mp:
-        extra["stress_12mp"] = bench_12mp(atq, synth, stream)
+        extra["stress_12mp"] = bench_12mp(atq, synth, stream, tune=not args.no_tune)
 
     if not args.skip_batch:
         extra["batched_16_mixed"] = bench_batch(atq, synth, stream, args.batch_rounds)
