@@ -339,8 +339,8 @@ enum {
     ATQ_BUF_BOUNDARY_KEYS = 5,   /* u64 packed min<<lb|max, lb = counts.key_label_bits */
     ATQ_BUF_CLUSTER_BOUNDS = 6,  /* filtered atq_cluster_bounds; start
                                     indexes BOUNDARY_POINTS, which holds
-                                    only the records of pairs whose count
-                                    can pass (see pair_filter_off)      */
+                                    only the records of the pairs that pass
+                                    ValidBlobFilter (see pair_filter_off) */
     ATQ_BUF_CLUSTER_POINTS = 7,  /* atq_cluster_point, angle-sorted      */
     ATQ_BUF_CLUSTER_INDEX = 8,   /* u32                                  */
     ATQ_BUF_CLUSTER_EXTENTS = 9, /* atq_cluster_extents                  */

@@ -12,8 +12,9 @@
 //
 // Two forms: the drop-in writes the reference's 4 sparse planes in full;
 // the detector's form writes, densely and in the reference's order, the
-// records of the cluster pairs whose record count can pass ValidBlobFilter.
+// records of the cluster pairs that pass ValidBlobFilter.
 #include "kernels.hpp"
+#include "records.hpp"
 #include "scan.hpp"
 
 namespace atq {
@@ -128,21 +129,25 @@ k_boundaries_sparse(const uint32_t *__restrict__ labels, const uint32_t *__restr
     }
 }
 
-// Filtered compact form (the detector's). A cluster pair whose record count
-// is below 24 or above 2 * (2w + 2h) is rejected by ValidBlobFilter
-// (include/cluster_bounds.h:85-120) whatever else it holds, so its records
-// never reach a cluster; here they are dropped before the key sort:
-//  1. k_pair_count: tiles of 1024 consecutive pixels (4 per thread) merge
-//     their records per pair in an LDS table, then add the per-pair counts
-//     into a global open-addressing table (key min<<32|max). A count already
-//     above the upper limit is
-//     not added to any more (the pairs that span the frame would otherwise
-//     queue on one address), so a count at or below the limit is exact.
-//  2. k_bnd_filter: each tile looks its pairs up, writes its surviving
-//     records densely into its own region (4 per pixel) in raster order and
-//     plane order within a pixel, and its count.
+// Filtered compact form (the detector's). ValidBlobFilter
+// (include/cluster_bounds.h:85-120) looks only at a pair's ClusterBounds
+// reduction (include/cluster_bounds.h:14-83: record count, bounding box,
+// gradient and moment sums), which does not depend on the records' order, so
+// the pairs that fail it are found, and their records dropped, before the
+// key sort:
+//  1. k_pair_count: tiles of 1024 consecutive pixels (4 per thread) reduce
+//     their records per pair in an LDS table, then add each pair's partial
+//     reduction into a global open-addressing table (key min<<32|max). A
+//     pair whose count is already above the upper limit 2 * (2w + 2h) gets
+//     no more adds (the pairs that span the frame would otherwise queue on
+//     one address); such a pair fails the filter, and every pair at or below
+//     the limit is reduced exactly.
+//  2. k_bnd_filter: each tile looks its pairs up, applies ValidBlobFilter to
+//     the reduction, writes the records of the passing pairs densely into its
+//     own region (4 per pixel) in raster order and plane order within a
+//     pixel, and its count.
 //  3. the tile counts are scanned and k_bnd_gather moves the regions
-//     together: the records of the surviving pairs in the reference's order.
+//     together: the records of the valid clusters in the reference's order.
 // If the global table runs out of probes, counts->pair_filter_off is set and
 // every record is kept.
 constexpr int BND_PX_PER_THREAD = 4;
@@ -193,36 +198,69 @@ ATQ_DEV uint32_t global_slot(const PairTable &t, uint64_t key, bool insert) {
     return NO_SLOT;
 }
 
-ATQ_DEV void global_add(const PairTable &t, uint64_t key, uint32_t c, uint32_t max_count,
+// A run's (or a tile's) share of a pair's ClusterBounds reduction, in the
+// table's zero-based form.
+struct PairPart {
+    uint32_t c, nxmin, xmax, nymin, ymax;
+    int32_t gx, gy;
+    int64_t pg;
+    ATQ_DEV void init() {
+        c = nxmin = xmax = nymin = ymax = 0;
+        gx = gy = 0;
+        pg = 0;
+    }
+    ATQ_DEV void add(atq_boundary_point p) {
+        const uint32_t xx = x_with_dx(p.packed_x), yy = y_with_dy(p.packed_x, p.packed_y);
+        const int gxx = grad_x(p.packed_x, p.packed_y), gyy = grad_y(p.packed_x, p.packed_y);
+        c++;
+        nxmin = max(nxmin, 0xFFFFu - xx);
+        xmax = max(xmax, xx);
+        nymin = max(nymin, 0xFFFFu - yy);
+        ymax = max(ymax, yy);
+        gx += gxx;
+        gy += gyy;
+        pg += (int64_t)xx * gxx + (int64_t)yy * gyy;
+    }
+};
+
+ATQ_DEV void global_add(const PairTable &t, uint64_t key, const PairPart &q, uint32_t max_count,
                         atq_counts *counts) {
     const uint32_t h = global_slot(t, key, true);
     if (h == NO_SLOT) {
         counts->pair_filter_off = 1u;
         return;
     }
-    if (__hip_atomic_load(&t.cnt[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= max_count)
-        atomicAdd(&t.cnt[h], c);
+    if (__hip_atomic_load(&t.cnt[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > max_count) return;
+    atomicAdd(&t.cnt[h], q.c);
+    atomicMax(&t.bbox[4 * (size_t)h + 0], q.nxmin);
+    atomicMax(&t.bbox[4 * (size_t)h + 1], q.xmax);
+    atomicMax(&t.bbox[4 * (size_t)h + 2], q.nymin);
+    atomicMax(&t.bbox[4 * (size_t)h + 3], q.ymax);
+    atomicAdd(&t.grad[2 * (size_t)h + 0], q.gx);
+    atomicAdd(&t.grad[2 * (size_t)h + 1], q.gy);
+    atomicAdd(&t.moment[h], (unsigned long long)q.pg);
 }
 
 // Walks a thread's records (4 pixels, planes in order) with a one-entry
-// cache, so runs of one pair reach the LDS table once.
+// cache, so runs of one pair reach the LDS table once with their reduction.
 template <typename F>
 ATQ_DEV void for_each_pair_run(const PixelRecords (&r)[BND_PX_PER_THREAD], F f) {
     uint64_t cur = 0;
-    uint32_t c = 0;
+    PairPart q;
+    q.init();
 #pragma unroll
     for (int k = 0; k < BND_PX_PER_THREAD; k++)
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             if (!((r[k].mask >> p) & 1u)) continue;
             if (r[k].k[p] != cur) {
-                if (c) f(cur, c);
+                if (q.c) f(cur, q);
                 cur = r[k].k[p];
-                c = 0;
+                q.init();
             }
-            c++;
+            q.add(r[k].p[p]);
         }
-    if (c) f(cur, c);
+    if (q.c) f(cur, q);
 }
 
 // Records of the 4 pixels of this thread in tile `tile` (w is a multiple
@@ -249,10 +287,14 @@ k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
              uint32_t h, uint32_t min_size, uint32_t tiles, PairTable table, uint32_t max_count,
              atq_counts *counts) {
     __shared__ uint64_t lk[TILE_SLOTS];
-    __shared__ uint32_t lc[TILE_SLOTS];
+    __shared__ uint32_t lc[TILE_SLOTS], lb[4][TILE_SLOTS];
+    __shared__ int32_t lg[2][TILE_SLOTS];
+    __shared__ unsigned long long lm[TILE_SLOTS];
     for (int i = threadIdx.x; i < TILE_SLOTS; i += 256) {
         lk[i] = 0;
-        lc[i] = 0;
+        lc[i] = lb[0][i] = lb[1][i] = lb[2][i] = lb[3][i] = 0;
+        lg[0][i] = lg[1][i] = 0;
+        lm[i] = 0;
     }
     __syncthreads();
     const uint32_t per = (tiles + gridDim.x - 1) / gridDim.x;
@@ -260,15 +302,36 @@ k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
     for (uint32_t tile = blockIdx.x * per; tile < t_end; tile++) {
         PixelRecords r[BND_PX_PER_THREAD];
         tile_records(labels, sizes, w, h, min_size, tile, r);
-        for_each_pair_run(r, [&](uint64_t key, uint32_t c) {
+        for_each_pair_run(r, [&](uint64_t key, const PairPart &q) {
             const uint32_t sl = tile_slot(lk, key, true);
-            if (sl != NO_SLOT) atomicAdd(&lc[sl], c);
-            else global_add(table, key, c, max_count, counts);
+            if (sl == NO_SLOT) {
+                global_add(table, key, q, max_count, counts);
+                return;
+            }
+            atomicAdd(&lc[sl], q.c);
+            atomicMax(&lb[0][sl], q.nxmin);
+            atomicMax(&lb[1][sl], q.xmax);
+            atomicMax(&lb[2][sl], q.nymin);
+            atomicMax(&lb[3][sl], q.ymax);
+            atomicAdd(&lg[0][sl], q.gx);
+            atomicAdd(&lg[1][sl], q.gy);
+            atomicAdd(&lm[sl], (unsigned long long)q.pg);
         });
     }
     __syncthreads();
     for (int i = threadIdx.x; i < TILE_SLOTS; i += 256)
-        if (lk[i]) global_add(table, ~lk[i], lc[i], max_count, counts);
+        if (lk[i]) {
+            PairPart q;
+            q.c = lc[i];
+            q.nxmin = lb[0][i];
+            q.xmax = lb[1][i];
+            q.nymin = lb[2][i];
+            q.ymax = lb[3][i];
+            q.gx = lg[0][i];
+            q.gy = lg[1][i];
+            q.pg = (int64_t)lm[i];
+            global_add(table, ~lk[i], q, max_count, counts);
+        }
 }
 
 __global__ void __launch_bounds__(256)
@@ -292,11 +355,22 @@ k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
     auto survives = [&](uint64_t key) -> bool {
         const uint32_t g = global_slot(table, key, false);
         if (g == NO_SLOT) return true;
-        const uint32_t c = __hip_atomic_load(&table.cnt[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return c >= 24 && c <= max_count;
+        const uint32_t c = table.cnt[g];
+        if (c < 24 || c > max_count) return false;
+        atq_cluster_bounds b;
+        b.x_min = (uint16_t)(0xFFFFu - table.bbox[4 * (size_t)g + 0]);
+        b.x_max = (uint16_t)table.bbox[4 * (size_t)g + 1];
+        b.y_min = (uint16_t)(0xFFFFu - table.bbox[4 * (size_t)g + 2]);
+        b.y_max = (uint16_t)table.bbox[4 * (size_t)g + 3];
+        b.gx_sum = table.grad[2 * (size_t)g + 0];
+        b.gy_sum = table.grad[2 * (size_t)g + 1];
+        b.pxgx_pygy_sum = (int64_t)table.moment[g];
+        b.start = 0;
+        b.count = c;
+        return valid_blob(b, w, h);
     };
     if (!keep_all) {
-        for_each_pair_run(r, [&](uint64_t key, uint32_t) { tile_slot(lk, key, true); });
+        for_each_pair_run(r, [&](uint64_t key, const PairPart &) { tile_slot(lk, key, true); });
         __syncthreads();
         for (int i = threadIdx.x; i < TILE_SLOTS; i += 256)
             if (lk[i]) lc[i] = survives(~lk[i]) ? 1u : 0u;

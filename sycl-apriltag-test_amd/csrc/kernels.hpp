@@ -52,14 +52,21 @@ int launch_ccl_frames(hipStream_t s, const FrontFrame *frames, int n, const Fron
 int launch_find_boundaries_sparse(hipStream_t s, const uint32_t *labels, const uint32_t *sizes,
                                   size_t w, size_t h, atq_boundary_point *points,
                                   uint64_t *keys, int min_size);
-// The detector's open-addressing table of cluster pairs and record counts
-// (slots = mask + 1, a power of two; zeroed before each frame).
+// The detector's open-addressing table of cluster pairs: per slot the pair
+// key (stored as ~key, 0 = empty), its record count and its ClusterBounds
+// reduction (bbox as 0xFFFF - x_min, x_max, 0xFFFF - y_min, y_max, all
+// max-reduced; the two gradient sums; the moment sum), so that every field
+// starts at zero (slots = mask + 1, a power of two; zeroed before each
+// frame).
 struct PairTable {
     uint64_t *keys;
     uint32_t *cnt;
+    uint32_t *bbox;  // 4 per slot
+    int32_t *grad;   // 2 per slot
+    unsigned long long *moment;
     uint32_t mask;
 };
-// Records of the pairs whose count lies in [24, 2 * (2w + 2h)], packed keys
+// Records of the pairs that pass ValidBlobFilter, packed keys
 // ((min << lb | max) << 2 | plane), in the reference's order; tile_keys /
 // tile_points hold 4 * w * h entries, tile_scratch 2 * tiles + 2.
 int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const uint32_t *sizes,

@@ -136,6 +136,9 @@ size_t carve_detector(atq_detector *d, char *base) {
     d->scan_flags = carve_one<uint64_t>(base, off, 2 * (size_t)scan_flag_count(c));
     d->pair_keys = carve_one<uint64_t>(base, off, c.pair_slots);
     d->pair_cnt = carve_one<uint32_t>(base, off, c.pair_slots);
+    d->pair_bbox = carve_one<uint32_t>(base, off, 4 * (size_t)c.pair_slots);
+    d->pair_grad = carve_one<int32_t>(base, off, 2 * (size_t)c.pair_slots);
+    d->pair_moment = carve_one<unsigned long long>(base, off, c.pair_slots);
     d->long_n = carve_one<uint32_t>(base, off, 2);
     d->counts = carve_one<atq_counts>(base, off, 1);
     off = (off + 255) & ~(size_t)255;
@@ -208,11 +211,12 @@ int run_back(atq_detector *d, hipStream_t s, const uint8_t *gray_in) {
     TRY(launch_zero_u32(s, zero_begin, nullptr, zero_words, lb_word, (uint32_t)plan.label_bits));
     const uint8_t *grey = f > 1 ? d->gray : gray_in;
     auto mark = [&](int i) { return ::mark(d, s, i); };
-    // Boundary records of the pairs that can pass ValidBlobFilter, in the
+    // Boundary records of the pairs that pass ValidBlobFilter, in the
     // reference's order, keyed by the packed pair key min<<lb|max with the
     // plane in 2 low bits; the stable sort then yields the reference's
     // (pair, plane, y, x) order.
-    const PairTable table{d->pair_keys, d->pair_cnt, d->caps.pair_slots - 1};
+    const PairTable table{d->pair_keys, d->pair_cnt,    d->pair_bbox,
+                          d->pair_grad, d->pair_moment, d->caps.pair_slots - 1};
     TRY(launch_find_boundaries_filtered(s, d->labels, d->sizes, w, h, d->params.min_component_size,
                                         (uint32_t)plan.label_bits, table, cnt, ws.keys_alt,
                                         reinterpret_cast<atq_boundary_point *>(ws.vals_alt),
@@ -678,14 +682,42 @@ struct atq_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool tuned = false;
+    std::vector<int> stream_of;  // internal stream of every frame
 };
 
 namespace {
 
-// Frames go to the internal streams round-robin. On each stream, the
-// thresholds and CCLs of its frames run as one launch chain per front-end
-// shape (a table of frames per launch), then each frame's remaining stages;
-// the streams are forked from and joined to `s`.
+// Frames large enough to fill the device by themselves (BATCH_BIG_PX
+// decimated pixels and up) share internal stream 0, so they run one after
+// another instead of competing for the caches; the other frames go
+// round-robin to the remaining streams and fill the gaps of the large ones'
+// latency-bound kernels (all streams round-robin when no frame is large).
+constexpr size_t BATCH_BIG_PX = 8u << 20;
+
+void assign_streams(atq_batch *b, const size_t *w_in, const size_t *h_in,
+                    const int32_t *decimate) {
+    const size_t n = b->dets.size();
+    std::vector<bool> big(n);
+    bool any_big = false;
+    for (size_t i = 0; i < n; i++) {
+        const int f = decimate ? std::max(decimate[i], 1) : 1;
+        big[i] = decimated(w_in[i], f) * decimated(h_in[i], f) >= BATCH_BIG_PX;
+        any_big = any_big || big[i];
+    }
+    const int ns = (int)std::min<size_t>(atq_batch::N_STREAMS, n);
+    const int first = any_big && ns > 1 ? 1 : 0;
+    b->stream_of.assign(n, 0);
+    int next = first;
+    for (size_t i = 0; i < n; i++) {
+        if (big[i] && first) continue;
+        b->stream_of[i] = next;
+        next = next + 1 < ns ? next + 1 : first;
+    }
+}
+
+// On each internal stream, the thresholds and CCLs of its frames run as one
+// launch chain per front-end shape (a table of frames per launch), then each
+// frame's remaining stages; the streams are forked from and joined to `s`.
 int batch_branches(atq_batch *b, hipStream_t s) {
     const size_t n = b->dets.size();
     const int ns = (int)std::min<size_t>(atq_batch::N_STREAMS, n);
@@ -695,15 +727,15 @@ int batch_branches(atq_batch *b, hipStream_t s) {
         hipStream_t sk = b->streams[k];
         ATQ_HIP_CHECK(hipStreamWaitEvent(sk, b->fork, 0));
         std::vector<bool> done(n, false);
-        for (size_t i = (size_t)k; i < n; i += ns) {
-            if (done[i]) continue;
+        for (size_t i = 0; i < n; i++) {
+            if (done[i] || b->stream_of[i] != k) continue;
             const FrontShapes sh = b->dets[i]->shapes;
             FrontFrame group[FRONT_MAX_FRAMES];
             int g = 0;
-            for (size_t j = i; j < n && g < FRONT_MAX_FRAMES; j += ns) {
+            for (size_t j = i; j < n && g < FRONT_MAX_FRAMES; j++) {
                 const FrontShapes &o = b->dets[j]->shapes;
-                if (done[j] || o.thr_rx != sh.thr_rx || o.thr_ry != sh.thr_ry ||
-                    o.ccl_rows != sh.ccl_rows)
+                if (done[j] || b->stream_of[j] != k || o.thr_rx != sh.thr_rx ||
+                    o.thr_ry != sh.thr_ry || o.ccl_rows != sh.ccl_rows)
                     continue;
                 group[g++] = front_frame(b->dets[j], b->inputs[j], b->w_in[j], b->h_in[j]);
                 done[j] = true;
@@ -711,7 +743,8 @@ int batch_branches(atq_batch *b, hipStream_t s) {
             TRY(launch_threshold_frames(sk, group, g, sh));
             TRY(launch_ccl_frames(sk, group, g, sh));
         }
-        for (size_t i = (size_t)k; i < n; i += ns) TRY(run_back(b->dets[i], sk, b->inputs[i]));
+        for (size_t i = 0; i < n; i++)
+            if (b->stream_of[i] == k) TRY(run_back(b->dets[i], sk, b->inputs[i]));
         ATQ_HIP_CHECK(hipEventRecord(b->joins[k], sk));
         ATQ_HIP_CHECK(hipStreamWaitEvent(s, b->joins[k], 0));
     }
@@ -765,6 +798,7 @@ int atq_batch_create(size_t n_frames, const size_t *widths_in, const size_t *hei
     }
     if (rc == ATQ_OK && hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess)
         rc = ATQ_ERR_HIP;
+    if (rc == ATQ_OK) assign_streams(b, widths_in, heights_in, decimate);
     if (rc != ATQ_OK) {
         atq_batch_destroy(b);
         return rc;

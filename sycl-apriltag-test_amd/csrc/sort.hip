@@ -132,6 +132,21 @@ k_rs_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
     uint32_t chunk, G;
     rs_chunks(n, &chunk, &G);
     if (blockIdx.x >= G) return;
+    const uint32_t b = blockIdx.x * chunk, e = min(n, b + chunk);
+    // the first round's records are loaded before the digit bases are
+    // built, so the two rounds of loads overlap
+    uint64_t k[RS_ITEMS];
+    uint32_t v[RS_ITEMS];
+    auto load_round = [&](uint32_t r) {
+        const uint32_t wbase = r + wave * RS_WAVE_SPAN;
+#pragma unroll
+        for (int j = 0; j < RS_ITEMS; j++) {
+            const uint32_t i = wbase + j * 64 + lane;
+            k[j] = i < e ? kin[i] : 0ull;
+            v[j] = i < e ? vin[i] : 0u;
+        }
+    };
+    load_round(b);
     {   // digit bases: exclusive scan of the totals, PER consecutive digits per thread
         uint32_t tv[PER], sum = 0;
 #pragma unroll
@@ -148,9 +163,9 @@ k_rs_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
             run += tv[k];
         }
     }
-    const uint32_t b = blockIdx.x * chunk, e = min(n, b + chunk);
     const uint64_t lt = lanemask_lt();
     for (uint32_t r = b; r < e; r += RS_ROUND) {
+        if (r != b) load_round(r);
         for (int d = t; d < B; d += RS_THREADS) {
             wc[0][d] = 0;
             wc[1][d] = 0;
@@ -158,15 +173,12 @@ k_rs_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
             wc[3][d] = 0;
         }
         __syncthreads();
-        uint64_t k[RS_ITEMS];
-        uint32_t v[RS_ITEMS], dg[RS_ITEMS], rank[RS_ITEMS];
+        uint32_t dg[RS_ITEMS], rank[RS_ITEMS];
         const uint32_t wbase = r + wave * RS_WAVE_SPAN;
 #pragma unroll
         for (int j = 0; j < RS_ITEMS; j++) {
             const uint32_t i = wbase + j * 64 + lane;
             const bool valid = i < e;
-            k[j] = valid ? kin[i] : 0ull;
-            v[j] = valid ? vin[i] : 0u;
             const uint32_t d = (uint32_t)(k[j] >> shift) & (B - 1);
             dg[j] = valid ? d : 0xFFFFFFFFu;
             uint64_t m = __ballot(valid);

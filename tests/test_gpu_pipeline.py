@@ -5,8 +5,8 @@ glue drop-ins chained.
 Bit-exact everywhere, element by element: the GPU writes boundary records
 in the reference's plane-major raster order and every sort is stable, as the
 oracle's are. The detector drops, before its key sort, the records of pairs
-whose record count ValidBlobFilter rejects; its grouped records are compared
-with the oracle's restricted to the other pairs.
+that ValidBlobFilter rejects; its grouped records are compared with the
+oracle's restricted to the valid clusters.
 """
 import numpy as np
 import pytest
@@ -77,17 +77,17 @@ def _bytes(a):
     return np.ascontiguousarray(a).view(np.uint8)
 
 
-def surviving_records(o, w, h):
-    """The oracle's grouped records restricted to the pairs whose record
-    count ValidBlobFilter can accept ([24, 2 * (2w + 2h)]): what the detector
-    keeps when its pair table did not overflow. Also the position of every
-    oracle record in that restricted array."""
+def surviving_records(o):
+    """The oracle's grouped records restricted to the pairs ValidBlobFilter
+    accepts (the oracle's cluster bounds): what the detector keeps when its
+    pair table did not overflow. Also the position of every oracle record in
+    that restricted array."""
     keys, points = o["keys"], o["points"]
-    uniq, inv, cnt = np.unique(keys, return_inverse=True, return_counts=True)
-    ok_pair = (cnt >= 24) & (cnt <= 2 * (2 * w + 2 * h))
-    keep = ok_pair[inv]
+    keep = np.zeros(len(keys), bool)
+    for b in o["bounds"]:
+        keep[int(b["start"]):int(b["start"]) + int(b["count"])] = True
     new_pos = np.cumsum(keep) - keep
-    return keys[keep], points[keep], int(ok_pair.sum()), new_pos
+    return keys[keep], points[keep], len(o["bounds"]), new_pos
 
 
 def check_pair_table(g, o):
@@ -114,7 +114,7 @@ def check_against_oracle(g, o):
     if int(c["pair_filter_off"]):
         keys, points, n_pairs = o["keys"], o["points"], len(o["bounds_all"])
     else:
-        keys, points, n_pairs, new_pos = surviving_records(o, w, h)
+        keys, points, n_pairs, new_pos = surviving_records(o)
         # a cluster's start indexes the detector's own (restricted) records
         o_bounds = o_bounds.copy()
         if len(o_bounds):
