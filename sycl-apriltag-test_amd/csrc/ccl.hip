@@ -119,10 +119,41 @@ ATQ_DEV uint32_t g_find_halving(uint32_t *parent, uint32_t n) {
     }
 }
 
+// The roots of a and b, both paths walked (and halved) in lockstep so that
+// their loads are in flight together.
+ATQ_DEV void g_find_pair(uint32_t *parent, uint32_t &a, uint32_t &b) {
+    bool ra = false, rb = false;  // root reached
+    while (!ra || !rb) {
+        const uint32_t pa = ra ? a : g_load(&parent[a]);
+        const uint32_t pb = rb ? b : g_load(&parent[b]);
+        ra = ra || pa == a;
+        rb = rb || pb == b;
+        const uint32_t ga = ra ? a : g_load(&parent[pa]);
+        const uint32_t gb = rb ? b : g_load(&parent[pb]);
+        if (!ra) {
+            if (ga == pa) {
+                a = pa;
+                ra = true;
+            } else {
+                atomicMin(&parent[a], ga);
+                a = ga;
+            }
+        }
+        if (!rb) {
+            if (gb == pb) {
+                b = pb;
+                rb = true;
+            } else {
+                atomicMin(&parent[b], gb);
+                b = gb;
+            }
+        }
+    }
+}
+
 ATQ_DEV void g_union(uint32_t *parent, uint32_t a, uint32_t b) {
     while (true) {
-        a = g_find_halving(parent, a);
-        b = g_find_halving(parent, b);
+        g_find_pair(parent, a, b);
         if (a < b) {
             uint32_t old = atomicMin(&parent[b], a);
             if (old == b) return;
@@ -333,29 +364,52 @@ ATQ_DEV void ccl_border_tile(uint64_t *hs, const FrontFrame &F, uint32_t tile, u
         y = y0 + 2 * by;
         live = x < (int)w && y < (int)h;
     }
-    uint32_t m = 0, id = 0;
+    // One round of independent loads: the block's three mask pixels, its two
+    // parent words, and for every probe that leaves the tile its pixel and
+    // the two parent words of the block holding it (parent words of local
+    // roots may be lowered by concurrent unions; any value read is an
+    // ancestor in the same tree, which is all a union needs).
+    uint8_t tl = 127, tr = 127, bl = 127;
+    uint32_t own[2] = {0, 0}, nb[6][2];
+    uint8_t pv[6];
+    bool cross[6];
     if (live) {
         const uint8_t *row = thr + (size_t)y * w;
-        m = check_mask(row[x], row[x + 1], row[x + w], x, y, w, h);
-        id = (uint32_t)y * w + (uint32_t)x;
+        tl = row[x];
+        tr = row[x + 1];
+        bl = row[x + w];
+        const uint32_t id = (uint32_t)y * w + (uint32_t)x;
+        own[0] = g_load(&parent[id]);
+        own[1] = g_load(&parent[id + 1]);
     }
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const Probe q = PROBES[i];
+        const int nbx = bx + q.dbx, nby = by + q.dby;
+        const int py = y + q.dy, px = x + q.dx, ny = y + 2 * q.dby, nx = x + 2 * q.dbx;
+        cross[i] = live && !(nbx >= 0 && nbx < TBX && nby >= 0) && py >= 0 && py < (int)h &&
+                   px >= 0 && px < (int)w && ny >= 0 && nx >= 0 && nx < (int)w;
+        pv[i] = 127;
+        nb[i][0] = nb[i][1] = 0;
+        if (cross[i]) {
+            pv[i] = thr[(size_t)py * w + px];
+            const uint32_t nid = (uint32_t)ny * w + (uint32_t)nx;
+            nb[i][0] = g_load(&parent[nid]);
+            nb[i][1] = g_load(&parent[nid + 1]);
+        }
+    }
+    const uint32_t m = live ? check_mask(tl, tr, bl, x, y, w, h) : 0u;
     uint32_t ea[6], eb[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         ea[i] = NONE;
         eb[i] = NONE;
-        const Probe q = PROBES[i];
-        const int nbx = bx + q.dbx, nby = by + q.dby;
-        if (live && (m & (1u << q.bit)) && !(nbx >= 0 && nbx < TBX && nby >= 0)) {
-            const uint8_t v = thr[(size_t)(y + q.dy) * w + (x + q.dx)];
-            if (v == C_WHITE || v == C_BLACK) {
-                const uint32_t c = v == C_WHITE ? 0 : 1;
-                const uint32_t nid = (uint32_t)(y + 2 * q.dby) * w + (uint32_t)(x + 2 * q.dbx);
-                const uint32_t a = g_load(&parent[id + c]) & ~FLAG;
-                const uint32_t b = g_load(&parent[nid + c]) & ~FLAG;
-                ea[i] = min(a, b);
-                eb[i] = max(a, b);
-            }
+        const uint8_t v = pv[i];
+        if (cross[i] && (m & (1u << PROBES[i].bit)) && (v == C_WHITE || v == C_BLACK)) {
+            const uint32_t c = v == C_WHITE ? 0 : 1;
+            const uint32_t a = own[c] & ~FLAG, b = nb[i][c] & ~FLAG;
+            ea[i] = min(a, b);
+            eb[i] = max(a, b);
         }
     }
 #pragma unroll
