@@ -13,6 +13,8 @@
 // Two forms: the drop-in writes the reference's 4 sparse planes in full;
 // the detector's form writes, densely and in the reference's order, the
 // records of the cluster pairs that pass ValidBlobFilter.
+#include <algorithm>
+
 #include "kernels.hpp"
 #include "records.hpp"
 #include "scan.hpp"
@@ -39,9 +41,8 @@ struct QuadLabels {
     uint32_t big_a, big_b;  // bit i: sizes[value] >= min_size
 };
 
-ATQ_DEV void load_quad(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes,
-                       uint32_t w, uint32_t h, uint32_t min_size, uint32_t x0, uint32_t y,
-                       QuadLabels &q) {
+ATQ_DEV void load_labels(const uint32_t *__restrict__ labels, uint32_t w, uint32_t h, uint32_t x0,
+                         uint32_t y, QuadLabels &q) {
     const uint32_t *r0 = labels + (size_t)y * w;
     const uint4 m0 = *reinterpret_cast<const uint4 *>(r0 + x0);
     q.a[0] = x0 > 0 ? r0[x0 - 1] : 0u;
@@ -63,12 +64,33 @@ ATQ_DEV void load_quad(const uint32_t *__restrict__ labels, const uint32_t *__re
 #pragma unroll
         for (int i = 0; i < 6; i++) q.b[i] = 0u;
     }
+}
+
+// The sizes of the 12 labels (one round of loads; `size_bits` consumes them).
+ATQ_DEV void load_sizes(const uint32_t *__restrict__ sizes, const QuadLabels &q, uint32_t (&sz)[12]) {
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        sz[i] = sizes[q.a[i] & LABEL_VALUE_MASK];
+        sz[6 + i] = sizes[q.b[i] & LABEL_VALUE_MASK];
+    }
+}
+
+ATQ_DEV void size_bits(QuadLabels &q, const uint32_t (&sz)[12], uint32_t min_size) {
     q.big_a = q.big_b = 0;
 #pragma unroll
     for (int i = 0; i < 6; i++) {
-        if (sizes[q.a[i] & LABEL_VALUE_MASK] >= min_size) q.big_a |= 1u << i;
-        if (sizes[q.b[i] & LABEL_VALUE_MASK] >= min_size) q.big_b |= 1u << i;
+        if (sz[i] >= min_size) q.big_a |= 1u << i;
+        if (sz[6 + i] >= min_size) q.big_b |= 1u << i;
     }
+}
+
+ATQ_DEV void load_quad(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes,
+                       uint32_t w, uint32_t h, uint32_t min_size, uint32_t x0, uint32_t y,
+                       QuadLabels &q) {
+    load_labels(labels, w, h, x0, y, q);
+    uint32_t sz[12];
+    load_sizes(sizes, q, sz);
+    size_bits(q, sz, min_size);
 }
 
 // The records of pixel x = x0 + k (x >= 1; column 0 is never a local pixel):
@@ -231,7 +253,9 @@ ATQ_DEV void global_add(const PairTable &t, uint64_t key, const PairPart &q, uin
         return;
     }
     if (__hip_atomic_load(&t.cnt[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > max_count) return;
-    atomicAdd(&t.cnt[h], q.c);
+    // a pair whose count passes the limit cannot pass the filter and needs
+    // no bounds; a pair that stays within it sees every add below the limit
+    if (atomicAdd(&t.cnt[h], q.c) + q.c > max_count) return;
     atomicMax(&t.bbox[4 * (size_t)h + 0], q.nxmin);
     atomicMax(&t.bbox[4 * (size_t)h + 1], q.xmax);
     atomicMax(&t.bbox[4 * (size_t)h + 2], q.nymin);
@@ -299,9 +323,35 @@ k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
     __syncthreads();
     const uint32_t per = (tiles + gridDim.x - 1) / gridDim.x;
     const uint32_t t_end = min(tiles, (blockIdx.x + 1) * per);
-    for (uint32_t tile = blockIdx.x * per; tile < t_end; tile++) {
+    // software-pipelined over the tiles: the next tile's labels load while
+    // this tile's sizes are in flight
+    const size_t n = (size_t)w * h;
+    auto pixel0 = [&](uint32_t tile) {
+        return (size_t)tile * BND_TILE + threadIdx.x * BND_PX_PER_THREAD;
+    };
+    QuadLabels cur;
+    uint32_t t0 = blockIdx.x * per;
+    if (t0 < t_end && pixel0(t0) < n)
+        load_labels(labels, w, h, (uint32_t)(pixel0(t0) % w), (uint32_t)(pixel0(t0) / w), cur);
+    for (uint32_t tile = t0; tile < t_end; tile++) {
+        const size_t p0 = pixel0(tile);
+        const bool here = p0 < n;
+        uint32_t sz[12];
+        if (here) load_sizes(sizes, cur, sz);
+        QuadLabels nxt;
+        const size_t p1 = pixel0(tile + 1);
+        if (tile + 1 < t_end && p1 < n)
+            load_labels(labels, w, h, (uint32_t)(p1 % w), (uint32_t)(p1 / w), nxt);
         PixelRecords r[BND_PX_PER_THREAD];
-        tile_records(labels, sizes, w, h, min_size, tile, r);
+#pragma unroll
+        for (int k = 0; k < BND_PX_PER_THREAD; k++) r[k].mask = 0;
+        if (here) {
+            size_bits(cur, sz, min_size);
+            const uint32_t y = (uint32_t)(p0 / w), x0 = (uint32_t)(p0 % w);
+#pragma unroll
+            for (int k = 0; k < BND_PX_PER_THREAD; k++) quad_pixel_records(cur, x0, y, k, 32, r[k]);
+        }
+        cur = nxt;
         for_each_pair_run(r, [&](uint64_t key, const PairPart &q) {
             const uint32_t sl = tile_slot(lk, key, true);
             if (sl == NO_SLOT) {
@@ -465,7 +515,9 @@ int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const
     const unsigned tiles = atq_div_up(w * h, BND_TILE);
     const uint32_t max_count = (uint32_t)(2 * (2 * w + 2 * h));
     uint32_t *tile_counts = tile_scratch, *tile_off = tile_scratch + tiles;
-    k_pair_count<<<min(tiles, 512u), 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h,
+    // a few tiles per work-group (merged in LDS before the global table), at
+    // most ~12 of them at large frames
+    k_pair_count<<<min(tiles, std::max(512u, tiles / 12)), 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h,
                                                    (uint32_t)min_size, tiles, table, max_count,
                                                    counts);
     k_bnd_filter<<<xcd_grid(tiles), 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h,
