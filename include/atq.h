@@ -149,6 +149,16 @@ const char *atq_status_string(int status);
 int atq_decimate(void *stream, const uint8_t *gray_in, size_t width_in,
                  size_t height_in, int factor, uint8_t *gray_out);
 
+/* Colour removal plus decimation (notes.txt:19-22, "1. Remove Colors
+ * 2. Decimate"): out[y][x] = grey of in[f*y][f*x] for interleaved 8-bit
+ * pixels of `channels` bytes (1 grey, 3 RGB, 4 RGBA). The grey of a colour
+ * pixel is the one the reference's loader computes (stbi_load with
+ * STBI_grey, src/main.cpp:265: stbi__compute_y, include/stb_image.h:1834-1836,
+ * (77 r + 150 g + 29 b) >> 8, alpha dropped). */
+int atq_grey_decimate(void *stream, const uint8_t *pixels_in, size_t width_in,
+                      size_t height_in, int channels, int factor,
+                      uint8_t *gray_out);
+
 /* Replaces threshold_image(q, grayscale, extrema, filtered_extrema,
  * thresholded, width, height, deps)  include/threshold.h:6-10.
  * Writes all three outputs like the reference (src/threshold.cpp:8-95). */
@@ -327,9 +337,18 @@ int atq_detector_destroy(atq_detector *det);
  * (atq_detector_buffer ATQ_BUF_QUADS) and counts in ATQ_BUF_COUNTS. */
 int atq_detect_quads(atq_detector *det, void *stream, const uint8_t *gray_in,
                      size_t width_in, size_t height_in);
+/* The same from interleaved RGB (channels 3) or RGBA (4) pixels: the grey
+ * conversion of atq_grey_decimate is fused into the threshold kernel's loads,
+ * and the (decimated) grey image lands in ATQ_BUF_GRAY. Results equal
+ * atq_detect_quads on atq_grey_decimate's output. No reference counterpart
+ * (the reference converts on the host while loading, src/main.cpp:265). */
+int atq_detect_quads_rgb(atq_detector *det, void *stream,
+                         const uint8_t *pixels_in, size_t width_in,
+                         size_t height_in, int channels);
 
 enum {
-    ATQ_BUF_GRAY = 0,        /* u8  decimated grey (== input when f=1)   */
+    ATQ_BUF_GRAY = 0,        /* u8  decimated grey; not written for an
+                                undecimated grey frame (read in place) */
     ATQ_BUF_THRESHOLDED = 1, /* u8  W*H                                  */
     ATQ_BUF_LABELS = 2,      /* u32 W*H                                  */
     ATQ_BUF_SIZES = 3,       /* u32 W*H; at a root its component's pixel

@@ -218,6 +218,39 @@ class Oracle:
         return self.lib.orc_max_threads()
 
 
+def grey_from_pixels(img):
+    """Grey image of interleaved 8-bit pixels (H x W x C, C in {1, 3, 4}) as
+    the reference's loader makes it: stbi_load(..., STBI_grey)
+    (src/main.cpp:265) converts 3- and 4-channel pixels with stbi__compute_y,
+    (77 r + 150 g + 29 b) >> 8, dropping alpha (include/stb_image.h:1834-1836,
+    :1897-1916)."""
+    img = np.asarray(img, np.uint8)
+    if img.ndim == 2 or img.shape[2] == 1:
+        return img.reshape(img.shape[0], img.shape[1]).copy()
+    r, g, b = (img[..., k].astype(np.uint32) for k in range(3))
+    return ((r * 77 + g * 150 + b * 29) >> 8).astype(np.uint8)
+
+
+class RefStb:
+    """The reference loader's colour-to-grey conversion (oracle/_ref build)."""
+
+    def __init__(self, path=None):
+        path = path or os.path.join(_HERE, "_ref", "libref_stb.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.lib = C.CDLL(path)
+        self.lib.ref_stbi_grey.restype = C.c_int
+        self.lib.ref_stbi_grey.argtypes = [_P, C.c_int, C.c_int, _P]
+
+    def grey(self, pixels, channels):
+        pixels = np.ascontiguousarray(pixels, np.uint8).reshape(-1)
+        n = pixels.size // channels
+        out = np.empty(n, np.uint8)
+        if self.lib.ref_stbi_grey(_ptr(pixels), n, channels, _ptr(out)) != 0:
+            raise RuntimeError("ref_stbi_grey failed")
+        return out
+
+
 class RefHeaders:
     """The reference's own SYCL-free header functions (oracle/_ref build)."""
 

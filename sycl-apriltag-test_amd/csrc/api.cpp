@@ -59,6 +59,15 @@ int atq_decimate(void *stream, const uint8_t *gray_in, size_t width_in, size_t h
     return launch_decimate(S(stream), gray_in, width_in, height_in, factor, gray_out);
 }
 
+int atq_grey_decimate(void *stream, const uint8_t *pixels_in, size_t width_in, size_t height_in,
+                      int channels, int factor, uint8_t *gray_out) {
+    if (!pixels_in || !gray_out || factor < 1 || width_in == 0 || height_in == 0 ||
+        (channels != 1 && channels != 3 && channels != 4))
+        return ATQ_ERR_BAD_ARG;
+    if (int r = ensure_init()) return r;
+    return launch_decimate(S(stream), pixels_in, width_in, height_in, factor, gray_out, channels);
+}
+
 int atq_threshold_image(void *stream, const uint8_t *grayscale, atq_extrema *extrema,
                         atq_extrema *filtered_extrema, uint8_t *thresholded, size_t width,
                         size_t height) {

@@ -10,14 +10,15 @@ struct ScanCtx;  // scan.hpp
 // frames go to one launch as a table passed by value; every kernel walks the
 // frames in turn with a persistent grid.
 struct FrontFrame {
-    const uint8_t *in;     // grey input, w_in x h_in
-    uint8_t *gray;         // decimated grey (written when f > 1; may be null)
+    const uint8_t *in;     // input, w_in x h_in pixels of `channels` bytes
+    uint8_t *gray;         // decimated grey (written when f > 1 or channels > 1; may be null)
     atq_extrema *ext, *fil;  // raw and dilated tile extrema (may be null)
     uint8_t *thr;          // thresholded, w x h
     uint32_t *parent, *labels, *sizes;
     uint32_t w_in, h_in, f, w, h;
     uint32_t size_cap;     // sizes saturate here (CCL)
     int32_t min_diff;
+    uint32_t channels;     // 1 grey; 3 RGB or 4 RGBA, converted to grey on load
 };
 constexpr int FRONT_MAX_FRAMES = 16;
 struct FrontTable {
@@ -36,7 +37,7 @@ int front_shape_candidates(FrontShapes *out, int max_out);
 
 // threshold.hip
 int launch_decimate(hipStream_t s, const uint8_t *in, size_t w_in, size_t h_in, int f,
-                    uint8_t *out);
+                    uint8_t *out, int channels = 1);
 int launch_threshold(hipStream_t s, const uint8_t *in, size_t w_in, size_t h_in, int f,
                      uint8_t *gray_out, atq_extrema *ext, atq_extrema *fil, uint8_t *thr,
                      int min_diff);

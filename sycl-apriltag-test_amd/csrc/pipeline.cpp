@@ -153,7 +153,9 @@ size_t carve_detector(atq_detector *d, char *base) {
     } while (0)
 
 // Validates a frame's dims against the detector and records them.
-int frame_setup(atq_detector *d, size_t w_in, size_t h_in) {
+int frame_setup(atq_detector *d, size_t w_in, size_t h_in, int channels = 1) {
+    if (channels != 1 && channels != 3 && channels != 4) return ATQ_ERR_BAD_ARG;
+    d->channels = channels;
     const int f = d->params.decimate;
     const size_t w = decimated(w_in, f), h = decimated(h_in, f);
     TRY(check_dims(w, h));
@@ -182,6 +184,7 @@ FrontFrame front_frame(const atq_detector *d, const uint8_t *gray_in, size_t w_i
     // sizes only decide "at least min_component_size" here
     fr.size_cap = (uint32_t)std::max(d->params.min_component_size, 1);
     fr.min_diff = d->params.min_white_black_diff;
+    fr.channels = (uint32_t)d->channels;
     return fr;
 }
 
@@ -209,7 +212,9 @@ int run_back(atq_detector *d, hipStream_t s, const uint8_t *gray_in) {
                                             reinterpret_cast<char *>(zero_begin)) / 4);
     const uint32_t lb_word = (uint32_t)(reinterpret_cast<uint32_t *>(&cnt->key_label_bits) - zero_begin);
     TRY(launch_zero_u32(s, zero_begin, nullptr, zero_words, lb_word, (uint32_t)plan.label_bits));
-    const uint8_t *grey = f > 1 ? d->gray : gray_in;
+    // the grey image the line fit samples: the input itself only when it is
+    // grey and undecimated
+    const uint8_t *grey = f > 1 || d->channels != 1 ? d->gray : gray_in;
     auto mark = [&](int i) { return ::mark(d, s, i); };
     // Boundary records of the pairs that pass ValidBlobFilter, in the
     // reference's order, keyed by the packed pair key min<<lb|max with the
@@ -262,8 +267,9 @@ int run_back(atq_detector *d, hipStream_t s, const uint8_t *gray_in) {
 }
 
 // threshold -> quads, all on `s`.
-int run_pipeline(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t w_in, size_t h_in) {
-    TRY(frame_setup(d, w_in, h_in));
+int run_pipeline(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t w_in, size_t h_in,
+                 int channels = 1) {
+    TRY(frame_setup(d, w_in, h_in, channels));
     TRY(mark(d, s, 0));
     const FrontFrame fr = front_frame(d, gray_in, w_in, h_in);
     TRY(launch_threshold_frames(s, &fr, 1, d->shapes));
@@ -486,6 +492,12 @@ int atq_detect_quads(atq_detector *det, void *stream, const uint8_t *gray_in, si
                      size_t height_in) {
     if (!det || !gray_in) return ATQ_ERR_BAD_ARG;
     return run_pipeline(det, S(stream), gray_in, width_in, height_in);
+}
+
+int atq_detect_quads_rgb(atq_detector *det, void *stream, const uint8_t *pixels_in,
+                         size_t width_in, size_t height_in, int channels) {
+    if (!det || !pixels_in || (channels != 3 && channels != 4)) return ATQ_ERR_BAD_ARG;
+    return run_pipeline(det, S(stream), pixels_in, width_in, height_in, channels);
 }
 
 int atq_detector_set_shapes(atq_detector *det, const atq_shapes *shapes) {

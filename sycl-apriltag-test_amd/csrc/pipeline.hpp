@@ -87,6 +87,7 @@ struct atq_detector {
     atq::Workspace ws;
     atq::FrontShapes shapes;  // threshold region and CCL tile shapes
     size_t w, h;
+    int channels;  // of the last frame's input: 1 grey, 3 RGB, 4 RGBA
     bool sorted_in_alt;
     bool timing;
     hipEvent_t ev[ATQ_N_STAGES + 1];

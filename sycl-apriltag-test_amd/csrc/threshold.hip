@@ -17,20 +17,50 @@
 
 namespace atq {
 
+// Grey value of an RGB(A) pixel: the reference reads its frames with
+// stbi_load(..., STBI_grey) (src/main.cpp:265), whose conversion of 3- and
+// 4-channel pixels is stbi__compute_y (include/stb_image.h:1834-1836; alpha
+// is dropped).
+ATQ_DEV uint32_t grey_of(const uint8_t *px) {
+    return ((uint32_t)px[0] * 77u + (uint32_t)px[1] * 150u + (uint32_t)px[2] * 29u) >> 8;
+}
+
+// Decimated grey pixel (xx, yy) of frame F: the input pixel (f*xx, f*yy)
+// (point subsample), converted when the input has colour channels.
+ATQ_DEV uint32_t grey_at(const FrontFrame &F, uint32_t xx, uint32_t yy) {
+    const uint8_t *p = F.in + ((size_t)(yy * F.f) * F.w_in + (size_t)xx * F.f) * F.channels;
+    return F.channels == 1 ? (uint32_t)p[0] : grey_of(p);
+}
+
 // Point-subsample decimation (upstream apriltag image_u8_decimate for
-// integer factors; absent from the reference, notes.txt:18-21).
+// integer factors; absent from the reference, notes.txt:18-21), of grey or
+// RGB(A) input.
 __global__ void k_decimate(const uint8_t *__restrict__ in, uint32_t w_in, uint32_t f,
-                           uint32_t w, uint32_t h, uint8_t *__restrict__ out) {
+                           uint32_t w, uint32_t h, uint32_t channels, uint8_t *__restrict__ out) {
     uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t y = blockIdx.y;
     if (x >= w || y >= h) return;
-    out[(size_t)y * w + x] = in[(size_t)(y * f) * w_in + (size_t)x * f];
+    const uint8_t *p = in + ((size_t)(y * f) * w_in + (size_t)x * f) * channels;
+    out[(size_t)y * w + x] = (uint8_t)(channels == 1 ? (uint32_t)p[0] : grey_of(p));
 }
 
 // Decimated pixels x .. x+15 of row y (x a multiple of 16, y < h) as four
 // little-endian words; pixels at or past w read 0.
 ATQ_DEV void load_strip_row(const FrontFrame &F, bool vec_in, int x, int y, uint32_t (&wd)[4]) {
     const uint32_t w = F.w, w_in = F.w_in, f = F.f;
+    if (F.channels != 1) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            uint32_t word = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int xx = x + 4 * q + j;
+                if (xx < (int)w) word |= grey_at(F, (uint32_t)xx, (uint32_t)y) << (8 * j);
+            }
+            wd[q] = word;
+        }
+        return;
+    }
     if (f == 1) {
         const uint8_t *row = F.in + (size_t)y * w_in;
         if (vec_in && x + 16 <= (int)w) {
@@ -136,7 +166,11 @@ ATQ_DEV void threshold_region(ThresholdLds<RX, RY> &L, const FrontFrame &F, uint
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const uint8_t *row = F.in + (size_t)((4 * hty + r) * f) * F.w_in;
-            if (f == 1) {
+            if (F.channels != 1) {
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    hp[r] |= grey_at(F, (uint32_t)(4 * htx + j), (uint32_t)(4 * hty + r)) << (8 * j);
+            } else if (f == 1) {
                 hp[r] = *reinterpret_cast<const uint32_t *>(row + 4 * htx);
             } else {
 #pragma unroll
@@ -202,7 +236,7 @@ ATQ_DEV void threshold_region(ThresholdLds<RX, RY> &L, const FrontFrame &F, uint
         }
     }
     if (row_in && tx < tw) {
-        uint8_t *gray_out = F.f > 1 ? F.gray : nullptr;
+        uint8_t *gray_out = F.f > 1 || F.channels != 1 ? F.gray : nullptr;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const size_t o = (size_t)(4 * ty + r) * w + 4 * tx;
@@ -233,7 +267,8 @@ k_threshold(FrontTable tab) {
         const FrontFrame &F = tab.fr[fi];
         const uint32_t regions_x = (F.w / 4 + RX - 1) / RX;
         const uint32_t n_regions = regions_x * ((F.h / 4 + RY - 1) / RY);
-        const bool vec_in = (F.w_in & 15) == 0 && (reinterpret_cast<uintptr_t>(F.in) & 15) == 0;
+        const bool vec_in = F.channels == 1 && (F.w_in & 15) == 0 &&
+                            (reinterpret_cast<uintptr_t>(F.in) & 15) == 0;
         const bool vec_out = (F.w & 15) == 0 && (reinterpret_cast<uintptr_t>(F.thr) & 15) == 0 &&
                              (reinterpret_cast<uintptr_t>(F.gray) & 15) == 0;
         for (uint32_t it = blockIdx.x; it < xcd_grid(n_regions); it += gridDim.x) {
@@ -245,10 +280,11 @@ k_threshold(FrontTable tab) {
 }
 
 int launch_decimate(hipStream_t s, const uint8_t *in, size_t w_in, size_t h_in, int f,
-                    uint8_t *out) {
+                    uint8_t *out, int channels) {
     size_t w = 1 + (w_in - 1) / f, h = 1 + (h_in - 1) / f;
     dim3 grid(atq_div_up(w, 256), (unsigned)h);
-    k_decimate<<<grid, 256, 0, s>>>(in, (uint32_t)w_in, (uint32_t)f, (uint32_t)w, (uint32_t)h, out);
+    k_decimate<<<grid, 256, 0, s>>>(in, (uint32_t)w_in, (uint32_t)f, (uint32_t)w, (uint32_t)h,
+                                    (uint32_t)channels, out);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
@@ -314,6 +350,7 @@ int launch_threshold(hipStream_t s, const uint8_t *in, size_t w_in, size_t h_in,
     fr.w = (uint32_t)(f > 1 ? 1 + (w_in - 1) / f : w_in);
     fr.h = (uint32_t)(f > 1 ? 1 + (h_in - 1) / f : h_in);
     fr.min_diff = min_diff;
+    fr.channels = 1;
     return launch_threshold_frames(s, &fr, 1, FrontShapes{});
 }
 

@@ -28,6 +28,7 @@ SIGNATURES = {
     "atq_init": (C.c_int, []),
     "atq_status_string": (C.c_char_p, [C.c_int]),
     "atq_decimate": (C.c_int, [_P, _P, _S, _S, C.c_int, _P]),
+    "atq_grey_decimate": (C.c_int, [_P, _P, _S, _S, C.c_int, C.c_int, _P]),
     "atq_threshold_image": (C.c_int, [_P, _P, _P, _P, _P, _S, _S]),
     "atq_decimate_threshold": (C.c_int, [_P, _P, _S, _S, C.c_int, _P, _P, _P, _P]),
     "atq_image_segmentation": (C.c_int, [_P, _P, _P, _P, _P, _S, _S, _S]),
@@ -44,6 +45,7 @@ SIGNATURES = {
     "atq_detector_create": (C.c_int, [_S, _S, _P, C.POINTER(_P)]),
     "atq_detector_destroy": (C.c_int, [_P]),
     "atq_detect_quads": (C.c_int, [_P, _P, _P, _S, _S]),
+    "atq_detect_quads_rgb": (C.c_int, [_P, _P, _P, _S, _S, C.c_int]),
     "atq_detector_buffer": (C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(_S)]),
     "atq_detector_dims": (C.c_int, [_P, C.POINTER(_S), C.POINTER(_S)]),
     "atq_detector_set_timing": (C.c_int, [_P, C.c_int]),
@@ -354,6 +356,14 @@ def decimate(gray_dev, w_in, h_in, f):
     return (w, h), out
 
 
+def grey_decimate(pixels_dev, w_in, h_in, channels, f):
+    """Colour removal + decimation of interleaved pixels (atq_grey_decimate)."""
+    w, h = 1 + (w_in - 1) // f, 1 + (h_in - 1) // f
+    out = dev_empty(w * h)
+    check(lib().atq_grey_decimate(stream(), P(pixels_dev), w_in, h_in, channels, f, P(out)))
+    return (w, h), out
+
+
 def image_segmentation(thr_dev, w, h):
     scratch = dev_empty(4 * w * h)
     labels = dev_empty(4 * w * h)
@@ -394,6 +404,11 @@ class Detector:
     def detect(self, gray_dev, w_in, h_in, stream_handle=None):
         s = C.c_void_p(stream_handle) if stream_handle is not None else stream()
         check(lib().atq_detect_quads(self.handle, s, P(gray_dev), w_in, h_in))
+
+    def detect_rgb(self, pixels_dev, w_in, h_in, channels, stream_handle=None):
+        """Detection from interleaved RGB (3) or RGBA (4) pixels."""
+        s = C.c_void_p(stream_handle) if stream_handle is not None else stream()
+        check(lib().atq_detect_quads_rgb(self.handle, s, P(pixels_dev), w_in, h_in, channels))
 
     def buffer(self, which):
         ptr = C.c_void_p()

@@ -229,3 +229,37 @@ def test_autotune_picks_a_supported_shape(atq, oracle):
     want, _ = oracle.detect(img, 2)
     np.testing.assert_array_equal(_bytes(det.quads()), _bytes(want))
     det.close()
+
+
+def _colour(img, channels, seed):
+    """An RGB(A) frame whose loader grey (stb_image's) is close to img: the
+    grey value spread over the channels with seeded jitter."""
+    rng = np.random.default_rng(seed)
+    h, w = img.shape
+    px = np.repeat(img[:, :, None], channels, axis=2).astype(np.int32)
+    px[:, :, :3] += rng.integers(-20, 21, (h, w, 3))
+    return np.clip(px, 0, 255).astype(np.uint8)
+
+
+@pytest.mark.parametrize("channels,f,w_in", [(3, 1, 800), (4, 2, 800), (3, 3, 790)])
+def test_colour_front_end_matches_oracle(atq, oracle, channels, f, w_in):
+    """RGB(A) input: atq_grey_decimate against the reference loader's
+    conversion (the oracle's, pinned to stb_image), and the detector's fused
+    colour path against the oracle run on that grey image."""
+    import pyoracle
+    from pyatq import synth
+    base = synth.plain_1080p(2007 + f).image[:600, :w_in].copy()
+    px = _colour(base, channels, 11 + f)
+    h_in, w_in = base.shape
+    grey = pyoracle.grey_from_pixels(px)
+    want_grey = oracle.decimate(grey, f) if f > 1 else grey
+    dev = atq.dev_from_np(px)
+    (w, h), g_dev = atq.grey_decimate(dev, w_in, h_in, channels, f)
+    np.testing.assert_array_equal(atq.to_np(g_dev, np.uint8, w * h).reshape(h, w), want_grey)
+    det = atq.Detector(w_in, h_in, decimate=f)
+    det.detect_rgb(dev, w_in, h_in, channels)
+    atq.sync()
+    np.testing.assert_array_equal(det.read("GRAY", np.uint8, w * h), want_grey.reshape(-1))
+    want, _ = oracle.detect(grey, f)
+    np.testing.assert_array_equal(_bytes(det.quads()), _bytes(want))
+    det.close()

@@ -4,8 +4,9 @@ the reference checkout has been compiled into oracle/_ref, the live functions.
 
 Pins calc_theta, the point field accessors (the zero half-pixel mask),
 ClusterBounds inital_from_point / reduce_bounds / cx / cy / dot,
-ValidBlobFilter, compute_initial_linefit (its row index taken from rx), and
-Combinadics<uint8_t, 15, 4, 16> (n_choose_k, decode).
+ValidBlobFilter, compute_initial_linefit (its row index taken from rx),
+Combinadics<uint8_t, 15, 4, 16> (n_choose_k, decode), and the loader's
+colour-to-grey conversion (stb_image's, tests/golden/make_grey_vectors.py).
 """
 import ctypes as C
 import os
@@ -17,6 +18,8 @@ from conftest import GOLDEN
 
 P = C.c_void_p
 V = np.load(os.path.join(GOLDEN, "ref_header_vectors.npz"))
+GV = np.load(os.path.join(GOLDEN, "ref_grey_vectors.npz"))
+REF_STB = os.path.join(os.path.dirname(GOLDEN), "..", "oracle", "_ref", "libref_stb.so")
 
 
 def _bits(a):
@@ -130,3 +133,24 @@ def test_live_reference_headers_agree_on_fresh_inputs(oracle):
         a = np.float32(ref.ref_calc_theta(px, py, cx, cy))
         b = np.float32(oracle.lib.orc_calc_theta(px, py, cx, cy))
         assert a.view(np.uint32) == b.view(np.uint32)
+
+
+def test_grey_conversion_matches_reference_loader():
+    import pyoracle
+    for px, want in ((GV["rgb"], GV["rgb_grey"]), (GV["rgba"], GV["rgba_grey"])):
+        got = pyoracle.grey_from_pixels(px[None, :, :])[0]
+        np.testing.assert_array_equal(got, want)
+    # closed-form corners: black, white, the primaries
+    assert list(pyoracle.grey_from_pixels(np.array([[[0, 0, 0], [255, 255, 255], [255, 0, 0],
+                                                     [0, 255, 0], [0, 0, 255]]], np.uint8))[0]) == \
+        [0, 255, 76, 149, 28]
+
+
+@pytest.mark.skipif(not os.path.exists(REF_STB), reason="reference loader not compiled here (oracle/_ref)")
+def test_live_reference_loader_agrees_on_fresh_pixels():
+    import pyoracle
+    ref = pyoracle.RefStb()
+    rng = np.random.default_rng(7)
+    for c in (3, 4):
+        px = rng.integers(0, 256, (20000, c), dtype=np.uint8)
+        np.testing.assert_array_equal(pyoracle.grey_from_pixels(px[None])[0], ref.grey(px, c))
