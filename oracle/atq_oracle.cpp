@@ -55,18 +55,22 @@ constexpr uint16_t COORDINATE_VALUE_MASK = (1u << 14) - 1;
 constexpr uint32_t LABEL_PIXEL_MASK = 0x80000000u;  // include/segmentation.h:7
 constexpr uint32_t LABEL_VALUE_MASK = 0x7FFFFFFFu;  // include/segmentation.h:9
 
-inline uint16_t extract_half_pixel(uint16_t c) {
-    return (uint16_t)((c & COORDINATE_HALF_PIXEL_MASK) >> 14);
+// Fixed mode (`fx`) reads both bits, as pack_half_pixel (include/boundary_point.h:26-28)
+// stores them: the plane the record came from.
+constexpr uint16_t FIXED_HALF_PIXEL_MASK = (1u << 15) | (1u << 14);
+
+inline uint16_t extract_half_pixel(uint16_t c, bool fx = false) {
+    return (uint16_t)((c & (fx ? FIXED_HALF_PIXEL_MASK : COORDINATE_HALF_PIXEL_MASK)) >> 14);
 }
 // include/boundary_point.h:67-95
-inline int dx_of(uint16_t px) {
-    switch (extract_half_pixel(px)) {
+inline int dx_of(uint16_t px, bool fx = false) {
+    switch (extract_half_pixel(px, fx)) {
     case RIGHT: return 1; case BOTTOM_LEFT: return -1;
     case BOTTOM: return 0; case BOTTOM_RIGHT: return 1; default: return 0;
     }
 }
-inline int dy_of(uint16_t px) {
-    switch (extract_half_pixel(px)) {
+inline int dy_of(uint16_t px, bool fx = false) {
+    switch (extract_half_pixel(px, fx)) {
     case RIGHT: return 0; case BOTTOM_LEFT: return 1;
     case BOTTOM: return 1; case BOTTOM_RIGHT: return 1; default: return 0;
     }
@@ -77,16 +81,16 @@ struct BoundaryPoint {
     bool b2w() const { return packed_y & COORDINATE_COLOR_DIRECTION_MASK; }
     uint16_t xv() const { return packed_x & COORDINATE_VALUE_MASK; }
     uint16_t yv() const { return packed_y & COORDINATE_VALUE_MASK; }
-    uint16_t x_with_dx() const { return (uint16_t)(2 * xv() + dx_of(packed_x)); }
-    uint16_t y_with_dy() const { return (uint16_t)(2 * yv() + dy_of(packed_x)); }
-    int gx() const { return b2w() ? dx_of(packed_x) : -dx_of(packed_x); }
-    int gy() const { return b2w() ? dy_of(packed_x) : -dy_of(packed_x); }
+    uint16_t x_with_dx(bool fx = false) const { return (uint16_t)(2 * xv() + dx_of(packed_x, fx)); }
+    uint16_t y_with_dy(bool fx = false) const { return (uint16_t)(2 * yv() + dy_of(packed_x, fx)); }
+    int gx(bool fx = false) const { return b2w() ? dx_of(packed_x, fx) : -dx_of(packed_x, fx); }
+    int gy(bool fx = false) const { return b2w() ? dy_of(packed_x, fx) : -dy_of(packed_x, fx); }
     // include/boundary_point.h:48-61
-    float calc_theta(float cx, float cy) const {
+    float calc_theta(float cx, float cy, bool fx = false) const {
         const float quadrants[2][2] = {{-1.0f * (1 << 16), (float)(1 << 17)},
                                        {0.0f, (float)(1 << 16)}};
-        float dx = (float)x_with_dx() - cx;
-        float dy = (float)y_with_dy() - cy;
+        float dx = (float)x_with_dx(fx) - cx;
+        float dy = (float)y_with_dy(fx) - cy;
         float q = quadrants[dx > 0][dy > 0];
         if ((dx > 0) == (dy > 0)) return q + dy / dx;
         return q - dx / dy;
@@ -106,13 +110,14 @@ struct ClusterBounds {
     int64_t pxgx_pygy_sum;
     uint32_t start, count;
 
-    static ClusterBounds from_point(BoundaryPoint p, uint32_t start) {
+    static ClusterBounds from_point(BoundaryPoint p, uint32_t start, bool fx = false) {
         ClusterBounds b;
-        b.x_min = b.x_max = p.x_with_dx();
-        b.y_min = b.y_max = p.y_with_dy();
-        b.gx_sum = p.gx();
-        b.gy_sum = p.gy();
-        b.pxgx_pygy_sum = (int64_t)p.x_with_dx() * p.gx() + (int64_t)p.y_with_dy() * p.gy();
+        b.x_min = b.x_max = p.x_with_dx(fx);
+        b.y_min = b.y_max = p.y_with_dy(fx);
+        b.gx_sum = p.gx(fx);
+        b.gy_sum = p.gy(fx);
+        b.pxgx_pygy_sum =
+            (int64_t)p.x_with_dx(fx) * p.gx(fx) + (int64_t)p.y_with_dy(fx) * p.gy(fx);
         b.start = start;
         b.count = 1;
         return b;
@@ -190,14 +195,14 @@ inline float hypot_f(float a, float b) {
     return (float)std::sqrt(da * da + db * db);
 }
 
-// include/line_fit_point.h:50-86 (with its `y = rx / 2`)
+// include/line_fit_point.h:50-86 (with its `y = rx / 2`; fixed mode: `ry / 2`)
 inline LineFitPoint compute_initial_linefit(ClusterPoint p, uint32_t width, uint32_t height,
-                                            const uint8_t *grey) {
+                                            const uint8_t *grey, bool fx = false) {
     BoundaryPoint bp{p.packed_x, p.packed_y};
-    int64_t rx = (int64_t)bp.x_with_dx() + 1;
-    int64_t ry = (int64_t)bp.y_with_dy() + 1;
+    int64_t rx = (int64_t)bp.x_with_dx(fx) + 1;
+    int64_t ry = (int64_t)bp.y_with_dy(fx) + 1;
     uint32_t x = (uint32_t)(rx / 2);
-    uint32_t y = (uint32_t)(rx / 2);
+    uint32_t y = (uint32_t)((fx ? ry : rx) / 2);
     int64_t W = 1;
     if (x > 0 && x + 1 < width && y > 0 && y + 1 < height) {
         int gx = (int)grey[(size_t)y * width + x + 1] - (int)grey[(size_t)y * width + x - 1];
