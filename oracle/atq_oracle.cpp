@@ -647,17 +647,21 @@ int64_t orc_compact_sort(const uint32_t *points_sparse, const uint64_t *keys_spa
 }
 
 // Cluster bounds: src/main.cpp:571-585 (reduce_by_segment of inital_from_point(p, i)).
-int64_t orc_cluster_bounds(const uint32_t *pts_u32, const uint64_t *keys, int64_t n,
-                           void *out_v) {
+static int64_t cluster_bounds_m(const uint32_t *pts_u32, const uint64_t *keys, int64_t n,
+                                void *out_v, bool fx) {
     const BoundaryPoint *pts = reinterpret_cast<const BoundaryPoint *>(pts_u32);
     ClusterBounds *out = reinterpret_cast<ClusterBounds *>(out_v);
     int64_t m = 0;
     for (int64_t i = 0; i < n; i++) {
-        ClusterBounds b = ClusterBounds::from_point(pts[i], (uint32_t)i);
+        ClusterBounds b = ClusterBounds::from_point(pts[i], (uint32_t)i, fx);
         if (i == 0 || keys[i] != keys[i - 1]) out[m++] = b;
         else out[m - 1] = reduce_bounds(out[m - 1], b);
     }
     return m;
+}
+int64_t orc_cluster_bounds(const uint32_t *pts_u32, const uint64_t *keys, int64_t n,
+                           void *out_v) {
+    return cluster_bounds_m(pts_u32, keys, n, out_v, false);
 }
 
 // Filter: src/main.cpp:604-609 (copy_if with ValidBlobFilter(width, height)).
@@ -673,8 +677,8 @@ int64_t orc_filter_bounds(const void *in_v, int64_t n, int64_t w, int64_t h, voi
 
 // Cluster assignment: src/main.cpp:621-675 (lower_bound over the filtered clusters, the
 // theta of each point, copy_if on the index sentinel). Returns N_fp.
-int64_t orc_cluster_points(const uint32_t *pts_u32, int64_t n, const void *filt_v, int64_t ncl,
-                           void *out_pts_v, uint32_t *out_idx) {
+static int64_t cluster_points_m(const uint32_t *pts_u32, int64_t n, const void *filt_v,
+                                int64_t ncl, void *out_pts_v, uint32_t *out_idx, bool fx) {
     const BoundaryPoint *pts = reinterpret_cast<const BoundaryPoint *>(pts_u32);
     const ClusterBounds *filt = reinterpret_cast<const ClusterBounds *>(filt_v);
     ClusterPoint *out = reinterpret_cast<ClusterPoint *>(out_pts_v);
@@ -685,11 +689,16 @@ int64_t orc_cluster_points(const uint32_t *pts_u32, int64_t n, const void *filt_
             [](const ClusterBounds &a, uint32_t b) { return (a.start + a.count) <= b; });
         if (it == filt + ncl) continue;
         if ((uint32_t)i < it->start || (uint32_t)i >= it->start + it->count) continue;
-        out[m] = ClusterPoint{pts[i].packed_x, pts[i].packed_y, pts[i].calc_theta(it->cx(), it->cy())};
+        out[m] = ClusterPoint{pts[i].packed_x, pts[i].packed_y,
+                              pts[i].calc_theta(it->cx(), it->cy(), fx)};
         out_idx[m] = (uint32_t)(it - filt);
         m++;
     }
     return m;
+}
+int64_t orc_cluster_points(const uint32_t *pts_u32, int64_t n, const void *filt_v, int64_t ncl,
+                           void *out_pts_v, uint32_t *out_idx) {
+    return cluster_points_m(pts_u32, n, filt_v, ncl, out_pts_v, out_idx, false);
 }
 
 // Extents: src/main.cpp:700-717
@@ -720,13 +729,17 @@ void orc_angle_sort(void *pts_v, uint32_t *idx, int64_t n) {
 }
 
 // Line-fit points: src/main.cpp:768-775
-void orc_linefit(const void *pts_v, int64_t n, const uint8_t *grey, int64_t w, int64_t h,
-                 void *out_v) {
+static void linefit_m(const void *pts_v, int64_t n, const uint8_t *grey, int64_t w, int64_t h,
+                      void *out_v, bool fx) {
     const ClusterPoint *pts = reinterpret_cast<const ClusterPoint *>(pts_v);
     LineFitPoint *out = reinterpret_cast<LineFitPoint *>(out_v);
 #pragma omp parallel for
     for (int64_t i = 0; i < n; i++)
-        out[i] = compute_initial_linefit(pts[i], (uint32_t)w, (uint32_t)h, grey);
+        out[i] = compute_initial_linefit(pts[i], (uint32_t)w, (uint32_t)h, grey, fx);
+}
+void orc_linefit(const void *pts_v, int64_t n, const uint8_t *grey, int64_t w, int64_t h,
+                 void *out_v) {
+    linefit_m(pts_v, n, grey, w, h, out_v, false);
 }
 
 // Moment prefix: src/main.cpp:793-801 (inclusive_scan_by_segment keyed by
@@ -863,8 +876,9 @@ static void sort_4(size_t a[4]) {
 // combination numbers per peak segment; segment e is paired with
 // cluster_extents[e], as the reference does. `fitted` (E entries) is zeroed here as
 // src/main.cpp:318-319 does. n_points bounds reads of `points`.
-int64_t orc_do_indexing(const void *ext_v, int64_t E, const void *corners_v, const void *pts_v,
-                        int64_t n_points, const void *cext_v, void *fitted_v) {
+static int64_t do_indexing_m(const void *ext_v, int64_t E, const void *corners_v,
+                             const void *pts_v, int64_t n_points, const void *cext_v,
+                             void *fitted_v, bool fx) {
     const PeakExtents *ext = reinterpret_cast<const PeakExtents *>(ext_v);
     const Corner *corners = reinterpret_cast<const Corner *>(corners_v);
     const LineFitPoint *pts = reinterpret_cast<const LineFitPoint *>(pts_v);
@@ -878,7 +892,8 @@ int64_t orc_do_indexing(const void *ext_v, int64_t E, const void *corners_v, con
 #pragma omp parallel for reduction(+ : oob_total)
     for (int64_t e = 0; e < E; e++) {
         PeakExtents pk = ext[e];
-        ClusterExtents ce = cext[e];
+        // fixed mode: the segment's own cluster
+        ClusterExtents ce = fx ? cext[corners[pk.start].cluster_index] : cext[e];
         float cur_err[256];
         FittedQuad cand[256];
         uint64_t oob = 0;
@@ -928,6 +943,10 @@ int64_t orc_do_indexing(const void *ext_v, int64_t E, const void *corners_v, con
         }
     }
     return (int64_t)oob_total;
+}
+int64_t orc_do_indexing(const void *ext_v, int64_t E, const void *corners_v, const void *pts_v,
+                        int64_t n_points, const void *cext_v, void *fitted_v) {
+    return do_indexing_m(ext_v, E, corners_v, pts_v, n_points, cext_v, fitted_v, false);
 }
 
 // process_fitted_quads: src/main.cpp:1035-1162, minus decode_quads.
@@ -1020,11 +1039,201 @@ int64_t orc_process_quads(const void *fq_v, int64_t E, void *out_v) {
     return m;
 }
 
+// ======================================================================
+// Fixed mode. The functions above restate the reference; these correct
+// seven of its behaviours and have no reference counterpart, so they are
+// pinned against textbook definitions instead (tests/test_oracle_fixed.py):
+//  * CCL: pixel-level, white 8-connected, black 4-connected, 127 pixels
+//    unlabelled. The reference's block graph (src/segmentation.cpp:139-255)
+//    links colour sets through a colour-agnostic neighbour mask.
+//  * labels: white 0x80000000 | (root + 1), black root + 1, 127 -> 0, with
+//    root the component's minimum pixel index; `sizes` has w*h + 1 entries
+//    and sizes[0] = 0, so neither 127 pixels nor out-of-image neighbours
+//    (both read as label 0) are ever a large component. The reference
+//    counts both into sizes[0] (src/find_boundaries.cpp:49-55, 82).
+//  * boundaries: column 0 is a local pixel like every other (the reference's
+//    tiling skips it, src/find_boundaries.cpp:66-69); records are kept by
+//    key, so the record of pixel (0, 0) RIGHT white-to-black, whose 32 bits
+//    are zero, is kept.
+//  * records decode both half-pixel bits: x2 = 2x + dx, y2 = 2y + dy,
+//    (gx, gy) = +-(dx, dy) (include/boundary_point.h:16 masks with &).
+//  * the line-fit weight samples pixel (rx/2, ry/2) (include/line_fit_point.h:63
+//    takes the row from rx).
+//  * fit_lines: global index 0 is a corner candidate like every other point
+//    (src/fit_quad.cpp:184-187 never evaluates it).
+//  * do_indexing: a peak segment is fitted against its own cluster's
+//    extents (src/fit_quad.cpp:405-408 takes cluster_extents[e]).
+
+// Root of n (the minimum of its class: parents are always smaller), halving
+// the path; the result does not depend on the tree's shape.
+static uint32_t fx_find(uint32_t *par, uint32_t n) {
+    while (par[n] != n) {
+        par[n] = par[par[n]];
+        n = par[n];
+    }
+    return n;
+}
+static void fx_union(uint32_t *par, uint32_t a, uint32_t b) {
+    a = fx_find(par, a);
+    b = fx_find(par, b);
+    if (a < b) par[b] = a;
+    else if (b < a) par[a] = b;
+}
+
+// thr (w*h, values 0/127/255) -> labels (w*h), sizes (w*h + 1).
+void orc_fx_segmentation(const uint8_t *thr, int64_t w, int64_t h, uint32_t *labels,
+                         uint32_t *sizes) {
+    const int64_t n = w * h;
+    std::vector<uint32_t> par((size_t)n);
+    std::iota(par.begin(), par.end(), 0u);
+    for (int64_t y = 0; y < h; y++)
+        for (int64_t x = 0; x < w; x++) {
+            const int64_t p = y * w + x;
+            const uint8_t v = thr[p];
+            if (v == 127) continue;
+            auto link = [&](int64_t q) {
+                if (thr[q] == v) fx_union(par.data(), (uint32_t)p, (uint32_t)q);
+            };
+            if (x > 0) link(p - 1);
+            if (y > 0) {
+                link(p - w);
+                if (v == 255) {
+                    if (x > 0) link(p - w - 1);
+                    if (x + 1 < w) link(p - w + 1);
+                }
+            }
+        }
+    std::memset(sizes, 0, sizeof(uint32_t) * (size_t)(n + 1));
+    for (int64_t p = 0; p < n; p++) {
+        const uint8_t v = thr[p];
+        if (v == 127) {
+            labels[p] = 0;
+            continue;
+        }
+        const uint32_t r = fx_find(par.data(), (uint32_t)p);
+        labels[p] = (v == 255 ? LABEL_PIXEL_MASK : 0u) | (r + 1);
+        sizes[r + 1]++;
+    }
+}
+
+// The reference's four record planes (RIGHT, BOTTOM_LEFT with its duplicate
+// rule, BOTTOM, BOTTOM_RIGHT; src/find_boundaries.cpp:94-170) for every
+// pixel, column 0 included. points/keys: 4*w*h, pre-filled like the
+// reference's (0 / all ones).
+void orc_fx_find_boundaries(const uint32_t *labels, const uint32_t *sizes, int64_t w, int64_t h,
+                            uint32_t *points_out, uint64_t *keys) {
+    BoundaryPoint *points = reinterpret_cast<BoundaryPoint *>(points_out);
+    const int64_t n = w * h;
+    std::memset(points, 0, sizeof(BoundaryPoint) * 4 * n);
+    std::memset(keys, 0xFF, sizeof(uint64_t) * 4 * n);
+    auto at = [&](int64_t x, int64_t y) -> uint32_t {
+        return (x < 0 || x >= w || y < 0 || y >= h) ? 0u : labels[y * w + x];
+    };
+    auto big = [&](uint32_t l) { return sizes[l & LABEL_VALUE_MASK] >= 25; };
+#pragma omp parallel for
+    for (int64_t y = 0; y < h; y++)
+        for (int64_t x = 0; x < w; x++) {
+            const uint32_t L = at(x, y);
+            const uint32_t lv = L & LABEL_VALUE_MASK, wl = L & LABEL_PIXEL_MASK;
+            auto handle = [&](int dy, int dx, int64_t plane, uint16_t hp) {
+                const uint32_t T = at(x + dx, y + dy);
+                const uint32_t wt = T & LABEL_PIXEL_MASK, tv = T & LABEL_VALUE_MASK;
+                if (wt == wl || !big(L) || !big(T)) return;
+                const int64_t oi = n * plane + y * w + x;
+                points[oi] = BoundaryPoint{(uint16_t)((uint16_t)x | (uint16_t)(hp << 14)),
+                                           wt > wl ? (uint16_t)(COORDINATE_COLOR_DIRECTION_MASK | y)
+                                                   : (uint16_t)y};
+                keys[oi] = ((uint64_t)std::min(lv, tv) << 32) | (uint64_t)std::max(lv, tv);
+            };
+            handle(0, 1, 0, RIGHT);
+            const uint32_t left = at(x - 1, y), down = at(x, y + 1);
+            if (!((left & LABEL_PIXEL_MASK) != (down & LABEL_PIXEL_MASK) && big(left) && big(down)))
+                handle(1, -1, 1, BOTTOM_LEFT);
+            handle(1, 0, 2, BOTTOM);
+            handle(1, 1, 3, BOTTOM_RIGHT);
+        }
+}
+
+// Compaction on the key (every emitted record has a key below all ones)
+// and the stable key sort. Returns N_bp.
+int64_t orc_fx_compact_sort(const uint32_t *points_sparse, const uint64_t *keys_sparse,
+                            int64_t n_sparse, uint32_t *out_points, uint64_t *out_keys) {
+    std::vector<int64_t> idx;
+    for (int64_t i = 0; i < n_sparse; i++)
+        if (keys_sparse[i] != ~0ull) idx.push_back(i);
+    __gnu_parallel::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) {
+        return keys_sparse[a] < keys_sparse[b];
+    });
+    for (size_t i = 0; i < idx.size(); i++) {
+        out_points[i] = points_sparse[idx[i]];
+        out_keys[i] = keys_sparse[idx[i]];
+    }
+    return (int64_t)idx.size();
+}
+
+int64_t orc_fx_cluster_bounds(const uint32_t *pts_u32, const uint64_t *keys, int64_t n,
+                              void *out_v) {
+    return cluster_bounds_m(pts_u32, keys, n, out_v, true);
+}
+int64_t orc_fx_cluster_points(const uint32_t *pts_u32, int64_t n, const void *filt_v,
+                              int64_t ncl, void *out_pts_v, uint32_t *out_idx) {
+    return cluster_points_m(pts_u32, n, filt_v, ncl, out_pts_v, out_idx, true);
+}
+void orc_fx_linefit(const void *pts_v, int64_t n, const uint8_t *grey, int64_t w, int64_t h,
+                    void *out_v) {
+    linefit_m(pts_v, n, grey, w, h, out_v, true);
+}
+
+// fit_lines as the per-cluster cyclic computation, for every point: window
+// errors e(p) of the 2k+1 points centred on p (k = min(20, n/12), none when
+// k < 2), f(p) = sum_t F[t] e(p+t-3) accumulated in order t = 0..6, and a
+// corner where f(p) > f(p-1) and f(p) > f(p+1), indices modulo the cluster
+// size n. `out` (n entries) is zeroed first.
+void orc_fx_fit_lines(const void *pts_v, const uint32_t *cidx, const void *ext_v, int64_t n,
+                      void *out_v) {
+    const LineFitPoint *pts = reinterpret_cast<const LineFitPoint *>(pts_v);
+    const ClusterExtents *ext = reinterpret_cast<const ClusterExtents *>(ext_v);
+    Corner *out = reinterpret_cast<Corner *>(out_v);
+    std::memset(out, 0, sizeof(Corner) * n);
+#pragma omp parallel for
+    for (int64_t x = 0; x < n; x++) {
+        const ClusterExtents e = ext[cidx[x]];
+        const size_t cnt = e.count, k = std::min((size_t)20, cnt / 12);
+        if (k < 2) continue;
+        const size_t j = (size_t)x - e.start;
+        auto err_at = [&](size_t pos) {
+            size_t total;
+            LineFitPoint m = get_moment(pts + e.start, cnt, (pos + cnt - k) % cnt, (pos + k) % cnt,
+                                        &total, (int64_t)cnt, nullptr);
+            float err;
+            fit_line(m, total, nullptr, &err, nullptr);
+            return err;
+        };
+        float er[9];
+        for (int d = 0; d < 9; d++) er[d] = err_at((j + 4 * cnt + d - 4) % cnt);
+        float f[3];
+        for (int q = 0; q < 3; q++) {
+            float acc = 0.0f;
+            for (int t = 0; t < 7; t++) {
+                float t1 = er[q + t] * FILTER_DATA[t];
+                acc = acc + t1;
+            }
+            f[q] = acc;
+        }
+        if (f[1] > f[0] && f[1] > f[2]) out[x] = Corner{(uint32_t)j, cidx[x], f[1]};
+    }
+}
+
+int64_t orc_fx_do_indexing(const void *ext_v, int64_t E, const void *corners_v, const void *pts_v,
+                           int64_t n_points, const void *cext_v, void *fitted_v) {
+    return do_indexing_m(ext_v, E, corners_v, pts_v, n_points, cext_v, fitted_v, true);
+}
+
 // Whole front end (src/main.cpp:322-1023 minus debug dumps and decoding) on
 // one grey frame, decimated by f first. `out` receives up to max_out quads;
 // the return value is the total number of quads.
-int64_t orc_detect(const uint8_t *gray_in, int64_t w_in, int64_t h_in, int f, int nthreads,
-                   void *out, int64_t max_out, orc_stats *st) {
+static int64_t detect_m(const uint8_t *gray_in, int64_t w_in, int64_t h_in, int f, int nthreads,
+                        void *out, int64_t max_out, orc_stats *st, bool fx) {
 #ifdef _OPENMP
     int saved = omp_get_max_threads();
     omp_set_num_threads(nthreads_or_default(nthreads));
@@ -1044,34 +1253,38 @@ int64_t orc_detect(const uint8_t *gray_in, int64_t w_in, int64_t h_in, int f, in
     const int64_t n = w * h;
     std::vector<uint8_t> ext((size_t)(2 * (w / 4) * (h / 4))), fil(ext.size()), thr((size_t)n);
     orc_threshold(g, w, h, ext.data(), fil.data(), thr.data());
-    std::vector<uint32_t> scratch((size_t)n), labels((size_t)n), sizes((size_t)n);
-    orc_segmentation(thr.data(), w, h, scratch.data(), labels.data(), sizes.data());
+    std::vector<uint32_t> scratch((size_t)n), labels((size_t)n), sizes((size_t)n + 1);
+    if (fx) orc_fx_segmentation(thr.data(), w, h, labels.data(), sizes.data());
+    else orc_segmentation(thr.data(), w, h, scratch.data(), labels.data(), sizes.data());
     std::vector<uint32_t> sp((size_t)(4 * n));
     std::vector<uint64_t> sk((size_t)(4 * n));
-    orc_find_boundaries(labels.data(), sizes.data(), w, h, sp.data(), sk.data());
+    if (fx) orc_fx_find_boundaries(labels.data(), sizes.data(), w, h, sp.data(), sk.data());
+    else orc_find_boundaries(labels.data(), sizes.data(), w, h, sp.data(), sk.data());
     std::vector<uint32_t> cp((size_t)(4 * n));
     std::vector<uint64_t> ck((size_t)(4 * n));
-    int64_t nbp = orc_compact_sort(sp.data(), sk.data(), 4 * n, cp.data(), ck.data());
+    int64_t nbp = fx ? orc_fx_compact_sort(sp.data(), sk.data(), 4 * n, cp.data(), ck.data())
+                     : orc_compact_sort(sp.data(), sk.data(), 4 * n, cp.data(), ck.data());
     sp.clear(); sp.shrink_to_fit(); sk.clear(); sk.shrink_to_fit();
     std::vector<ClusterBounds> bounds((size_t)std::max<int64_t>(nbp, 1)), filt(bounds.size());
-    int64_t npairs = orc_cluster_bounds(cp.data(), ck.data(), nbp, bounds.data());
+    int64_t npairs = cluster_bounds_m(cp.data(), ck.data(), nbp, bounds.data(), fx);
     int64_t ncl = orc_filter_bounds(bounds.data(), npairs, w, h, filt.data());
     std::vector<ClusterPoint> fpts((size_t)std::max<int64_t>(nbp, 1));
     std::vector<uint32_t> fidx(fpts.size());
-    int64_t nfp = orc_cluster_points(cp.data(), nbp, filt.data(), ncl, fpts.data(), fidx.data());
+    int64_t nfp = cluster_points_m(cp.data(), nbp, filt.data(), ncl, fpts.data(), fidx.data(), fx);
     std::vector<ClusterExtents> cext((size_t)std::max<int64_t>(ncl, 1));
     orc_cluster_extents(filt.data(), ncl, cext.data());
     orc_angle_sort(fpts.data(), fidx.data(), nfp);
     std::vector<LineFitPoint> lf((size_t)std::max<int64_t>(nfp, 1)), lfp(lf.size());
-    orc_linefit(fpts.data(), nfp, g, w, h, lf.data());
+    linefit_m(fpts.data(), nfp, g, w, h, lf.data(), fx);
     orc_linefit_prefix(fidx.data(), lf.data(), nfp, lfp.data());
     std::vector<Corner> corners((size_t)std::max<int64_t>(nfp, 1)), cc(corners.size());
-    orc_fit_lines(lfp.data(), fidx.data(), cext.data(), nfp, corners.data());
+    if (fx) orc_fx_fit_lines(lfp.data(), fidx.data(), cext.data(), nfp, corners.data());
+    else orc_fit_lines(lfp.data(), fidx.data(), cext.data(), nfp, corners.data());
     std::vector<PeakExtents> seg(corners.size());
     int64_t ncorner = 0;
     int64_t E = orc_corner_segments(corners.data(), nfp, cc.data(), seg.data(), &ncorner);
     std::vector<FittedQuad> fq((size_t)std::max<int64_t>(E, 1));
-    int64_t oob = orc_do_indexing(seg.data(), E, cc.data(), lfp.data(), nfp, cext.data(), fq.data());
+    int64_t oob = do_indexing_m(seg.data(), E, cc.data(), lfp.data(), nfp, cext.data(), fq.data(), fx);
     std::vector<QuadCorners> quads((size_t)std::max<int64_t>(E, 1));
     int64_t nq = orc_process_quads(fq.data(), E, quads.data());
     if (out && max_out > 0)
@@ -1091,6 +1304,15 @@ int64_t orc_detect(const uint8_t *gray_in, int64_t w_in, int64_t h_in, int f, in
 #endif
     return nq;
 }
+int64_t orc_detect(const uint8_t *gray_in, int64_t w_in, int64_t h_in, int f, int nthreads,
+                   void *out, int64_t max_out, orc_stats *st) {
+    return detect_m(gray_in, w_in, h_in, f, nthreads, out, max_out, st, false);
+}
+// The same in fixed mode.
+int64_t orc_fx_detect(const uint8_t *gray_in, int64_t w_in, int64_t h_in, int f, int nthreads,
+                      void *out, int64_t max_out, orc_stats *st) {
+    return detect_m(gray_in, w_in, h_in, f, nthreads, out, max_out, st, true);
+}
 
 // Single-call helpers used by tests.
 void orc_fit_line(const void *m, int64_t num, float *line, float *err, float *mse) {
@@ -1105,6 +1327,17 @@ void orc_get_moment(const void *pts, int64_t n, int64_t i0, int64_t i1, void *ou
 }
 float orc_calc_theta(uint16_t px, uint16_t py, float cx, float cy) {
     return BoundaryPoint{px, py}.calc_theta(cx, cy);
+}
+float orc_fx_calc_theta(uint16_t px, uint16_t py, float cx, float cy) {
+    return BoundaryPoint{px, py}.calc_theta(cx, cy, true);
+}
+void orc_fx_bounds_from_point(uint16_t px, uint16_t py, uint32_t start, void *out) {
+    *reinterpret_cast<ClusterBounds *>(out) = ClusterBounds::from_point(BoundaryPoint{px, py}, start, true);
+}
+void orc_fx_initial_linefit(uint16_t px, uint16_t py, int64_t w, int64_t h, const uint8_t *grey,
+                            void *out) {
+    *reinterpret_cast<LineFitPoint *>(out) =
+        compute_initial_linefit(ClusterPoint{px, py, 0.0f}, (uint32_t)w, (uint32_t)h, grey, true);
 }
 void orc_bounds_cx_cy_dot(const void *b, float *out3) {
     const ClusterBounds *c = reinterpret_cast<const ClusterBounds *>(b);

@@ -34,8 +34,18 @@ def _ptr(a):
     return a.ctypes.data_as(_P)
 
 
+# Stages with a fixed-mode variant (orc_fx_*): see the fixed-mode section of
+# atq_oracle.cpp for what each corrects.
+_FIXED_STAGES = ("find_boundaries", "compact_sort", "cluster_bounds", "cluster_points",
+                 "linefit", "fit_lines", "do_indexing", "detect")
+
+
 class Oracle:
-    def __init__(self, path=None):
+    """The oracle's stages; `fixed=True` selects fixed mode (the reference's
+    behaviours corrected), the default compat mode restates the reference."""
+
+    def __init__(self, path=None, fixed=False):
+        self.fixed = bool(fixed)
         path = path or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             raise FileNotFoundError(f"{path} missing: run `make -C oracle`")
@@ -70,11 +80,21 @@ class Oracle:
             "orc_combinadics_decode": (None, [C.c_int, _P]),
             "orc_hypotf": (C.c_float, [C.c_float, C.c_float]),
             "orc_max_threads": (C.c_int, []),
+            "orc_fx_segmentation": (None, [_P, _I, _I, _P, _P]),
+            "orc_fx_calc_theta": (C.c_float, [C.c_uint16, C.c_uint16, C.c_float, C.c_float]),
+            "orc_fx_bounds_from_point": (None, [C.c_uint16, C.c_uint16, C.c_uint32, _P]),
+            "orc_fx_initial_linefit": (None, [C.c_uint16, C.c_uint16, _I, _I, _P, _P]),
         }
+        for stage in _FIXED_STAGES:
+            sig["orc_fx_" + stage] = sig["orc_" + stage]
         for name, (res, args) in sig.items():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
+
+    def _f(self, stage):
+        prefix = "orc_fx_" if self.fixed and stage in _FIXED_STAGES else "orc_"
+        return getattr(self.lib, prefix + stage)
 
     # ---- stages ------------------------------------------------------------
     def decimate(self, img, f):
@@ -97,8 +117,12 @@ class Oracle:
     def segmentation(self, thr):
         h, w = thr.shape
         thr = np.ascontiguousarray(thr)
-        scratch = np.empty((h, w), np.uint32)
         labels = np.empty((h, w), np.uint32)
+        if self.fixed:  # sizes[0] = 0; component sizes at root + 1
+            sizes = np.empty(h * w + 1, np.uint32)
+            self.lib.orc_fx_segmentation(_ptr(thr), w, h, _ptr(labels), _ptr(sizes))
+            return labels, sizes
+        scratch = np.empty((h, w), np.uint32)
         sizes = np.empty(h * w, np.uint32)
         self.lib.orc_segmentation(_ptr(thr), w, h, _ptr(scratch), _ptr(labels), _ptr(sizes))
         return labels, sizes
@@ -107,7 +131,7 @@ class Oracle:
         h, w = labels.shape
         pts = np.empty(4 * w * h, np.uint32)
         keys = np.empty(4 * w * h, np.uint64)
-        self.lib.orc_find_boundaries(_ptr(np.ascontiguousarray(labels)), _ptr(sizes), w, h,
+        self._f("find_boundaries")(_ptr(np.ascontiguousarray(labels)), _ptr(sizes), w, h,
                                      _ptr(pts), _ptr(keys))
         return pts, keys
 
@@ -115,12 +139,12 @@ class Oracle:
         n = len(pts)
         op = np.empty(n, np.uint32)
         ok = np.empty(n, np.uint64)
-        m = self.lib.orc_compact_sort(_ptr(pts), _ptr(keys), n, _ptr(op), _ptr(ok))
+        m = self._f("compact_sort")(_ptr(pts), _ptr(keys), n, _ptr(op), _ptr(ok))
         return op[:m].copy(), ok[:m].copy()
 
     def cluster_bounds(self, pts, keys):
         out = np.empty(max(len(pts), 1), R.CLUSTER_BOUNDS)
-        m = self.lib.orc_cluster_bounds(_ptr(pts), _ptr(keys), len(pts), _ptr(out))
+        m = self._f("cluster_bounds")(_ptr(pts), _ptr(keys), len(pts), _ptr(out))
         return out[:m].copy()
 
     def filter_bounds(self, bounds, w, h):
@@ -132,7 +156,7 @@ class Oracle:
         n = len(pts)
         op = np.empty(max(n, 1), R.CLUSTER_POINT)
         oi = np.empty(max(n, 1), np.uint32)
-        m = self.lib.orc_cluster_points(_ptr(pts), n, _ptr(filt), len(filt), _ptr(op), _ptr(oi))
+        m = self._f("cluster_points")(_ptr(pts), n, _ptr(filt), len(filt), _ptr(op), _ptr(oi))
         return op[:m].copy(), oi[:m].copy()
 
     def cluster_extents(self, filt):
@@ -149,7 +173,7 @@ class Oracle:
     def linefit(self, cpts, gray):
         h, w = gray.shape
         out = np.empty(max(len(cpts), 1), R.LINE_FIT_POINT)
-        self.lib.orc_linefit(_ptr(cpts), len(cpts), _ptr(np.ascontiguousarray(gray)), w, h, _ptr(out))
+        self._f("linefit")(_ptr(cpts), len(cpts), _ptr(np.ascontiguousarray(gray)), w, h, _ptr(out))
         return out[:len(cpts)].copy()
 
     def linefit_prefix(self, idx, lf):
@@ -159,7 +183,7 @@ class Oracle:
 
     def fit_lines(self, lfp, idx, ext):
         out = np.empty(max(len(lfp), 1), R.CORNER)
-        self.lib.orc_fit_lines(_ptr(lfp), _ptr(idx), _ptr(ext), len(lfp), _ptr(out))
+        self._f("fit_lines")(_ptr(lfp), _ptr(idx), _ptr(ext), len(lfp), _ptr(out))
         return out[:len(lfp)].copy()
 
     def corner_segments(self, corners):
@@ -172,7 +196,7 @@ class Oracle:
 
     def do_indexing(self, seg, cc, lfp, cext):
         out = np.empty(max(len(seg), 1), R.FITTED_QUAD)
-        oob = self.lib.orc_do_indexing(_ptr(seg), len(seg), _ptr(cc), _ptr(lfp), len(lfp),
+        oob = self._f("do_indexing")(_ptr(seg), len(seg), _ptr(cc), _ptr(lfp), len(lfp),
                                        _ptr(cext), _ptr(out))
         return out[:len(seg)].copy(), oob
 
@@ -211,7 +235,7 @@ class Oracle:
         h, w = img.shape
         out = np.empty(max_out, R.QUAD_CORNERS)
         st = Stats()
-        n = self.lib.orc_detect(_ptr(img), w, h, f, nthreads, _ptr(out), max_out, C.byref(st))
+        n = self._f("detect")(_ptr(img), w, h, f, nthreads, _ptr(out), max_out, C.byref(st))
         return out[:min(n, max_out)].copy(), st.as_dict()
 
     def max_threads(self):

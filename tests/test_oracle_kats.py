@@ -269,8 +269,9 @@ def test_filter_taps_are_float_gaussian():
     np.testing.assert_array_equal(FILTER, np.exp(-(d * d) / 2.0).astype(np.float32))
 
 
-def _cyclic_corners(oracle, lfp, idx, ext):
-    """The per-cluster cyclic window / filter / strict maximum, in float32."""
+def _cyclic_corners(oracle, lfp, idx, ext, first=1):
+    """The per-cluster cyclic window / filter / strict maximum, in float32,
+    for the points at global index >= first."""
     n_all = len(lfp)
     out = np.zeros(n_all, R.CORNER)
     for c, (s, n) in enumerate(zip(ext["start"], ext["count"])):
@@ -294,7 +295,7 @@ def _cyclic_corners(oracle, lfp, idx, ext):
             f[p] = acc
         for p in range(n):
             x = s + p
-            if x >= 1 and f[p] > f[(p - 1) % n] and f[p] > f[(p + 1) % n]:
+            if x >= first and f[p] > f[(p - 1) % n] and f[p] > f[(p + 1) % n]:
                 out[x] = (p, c, f[p])
     return out
 
