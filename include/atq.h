@@ -111,7 +111,27 @@ typedef struct atq_params {
                                     2^n slots, n in [8, 24]; 0 = automatic
                                     (about one slot per 64 pixels, 2^14 to
                                     2^20). No counterpart in the reference. */
+    int32_t mode;                /* detector only: ATQ_MODE_COMPAT (0, the
+                                    reference's results) or ATQ_MODE_FIXED */
 } atq_params;
+
+/* Detector modes. COMPAT reproduces the reference bit for bit. FIXED corrects
+ * seven of its behaviours (no reference counterpart; never used for parity):
+ *  - CCL is pixel-level, white 8-connected, black 4-connected, with no links
+ *    through the reference's colour-agnostic neighbour mask
+ *    (src/segmentation.cpp:139-255). Labels are root + 1 (white with the
+ *    0x80000000 bit), 0 for 127 pixels; ATQ_BUF_SIZES holds the count of the
+ *    component with root r at index r + 1 and 0 at index 0, so neither 127
+ *    pixels nor out-of-image neighbours form boundary records
+ *    (src/main.cpp:310-311, src/find_boundaries.cpp:49-55);
+ *  - column 0 is a local pixel of the boundary pass (src/find_boundaries.cpp:66-69);
+ *  - records decode both half-pixel bits: doubled coordinates 2x + dx,
+ *    2y + dy and gradient +-(dx, dy) of the plane's neighbour offset
+ *    (include/boundary_point.h:16);
+ *  - the line-fit weight samples pixel (rx/2, ry/2) (include/line_fit_point.h:63);
+ *  - fit_lines considers global point 0 like every other (src/fit_quad.cpp:184-187);
+ *  - do_indexing fits each peak segment to its own cluster (src/fit_quad.cpp:405-408). */
+enum { ATQ_MODE_COMPAT = 0, ATQ_MODE_FIXED = 1 };
 
 /* Per-frame counts written by the pipeline (device memory, 16 x u32). */
 typedef struct atq_counts {

@@ -93,15 +93,17 @@ ATQ_DEV void load_quad(const uint32_t *__restrict__ labels, const uint32_t *__re
     size_bits(q, sz, min_size);
 }
 
-// The records of pixel x = x0 + k (x >= 1; column 0 is never a local pixel):
-// RIGHT, BOTTOM_LEFT (unless left and down differ in white bit and are both
-// large), BOTTOM, BOTTOM_RIGHT, each when the white bits differ and both
-// components are large (src/find_boundaries.cpp:94-170).
+// The records of pixel x = x0 + k (x >= 1; column 0 is never a local pixel,
+// except in fixed mode): RIGHT, BOTTOM_LEFT (unless left and down differ in
+// white bit and are both large), BOTTOM, BOTTOM_RIGHT, each when the white
+// bits differ and both components are large (src/find_boundaries.cpp:94-170).
+// In fixed mode label 0 (127 pixels, out of image) has size 0, so it never
+// counts as large.
 ATQ_DEV void quad_pixel_records(const QuadLabels &q, uint32_t x0, uint32_t y, int k,
-                                uint32_t key_shift, PixelRecords &out) {
+                                uint32_t key_shift, PixelRecords &out, bool fx = false) {
     out.mask = 0;
     const uint32_t x = x0 + k;
-    if (x == 0) return;
+    if (x == 0 && !fx) return;
     const uint32_t L = q.a[k + 1];
     const uint32_t lv = L & LABEL_VALUE_MASK, wl = L & LABEL_PIXEL_MASK;
     const bool local_big = (q.big_a >> (k + 1)) & 1u;
@@ -231,9 +233,9 @@ struct PairPart {
         gx = gy = 0;
         pg = 0;
     }
-    ATQ_DEV void add(atq_boundary_point p) {
-        const uint32_t xx = x_with_dx(p.packed_x), yy = y_with_dy(p.packed_x, p.packed_y);
-        const int gxx = grad_x(p.packed_x, p.packed_y), gyy = grad_y(p.packed_x, p.packed_y);
+    ATQ_DEV void add(atq_boundary_point p, bool fx) {
+        const uint32_t xx = x_with_dx(p.packed_x, fx), yy = y_with_dy(p.packed_x, p.packed_y, fx);
+        const int gxx = grad_x(p.packed_x, p.packed_y, fx), gyy = grad_y(p.packed_x, p.packed_y, fx);
         c++;
         nxmin = max(nxmin, 0xFFFFu - xx);
         xmax = max(xmax, xx);
@@ -268,7 +270,7 @@ ATQ_DEV void global_add(const PairTable &t, uint64_t key, const PairPart &q, uin
 // Walks a thread's records (4 pixels, planes in order) with a one-entry
 // cache, so runs of one pair reach the LDS table once with their reduction.
 template <typename F>
-ATQ_DEV void for_each_pair_run(const PixelRecords (&r)[BND_PX_PER_THREAD], F f) {
+ATQ_DEV void for_each_pair_run(const PixelRecords (&r)[BND_PX_PER_THREAD], F f, bool fx) {
     uint64_t cur = 0;
     PairPart q;
     q.init();
@@ -282,7 +284,7 @@ ATQ_DEV void for_each_pair_run(const PixelRecords (&r)[BND_PX_PER_THREAD], F f) 
                 cur = r[k].k[p];
                 q.init();
             }
-            q.add(r[k].p[p]);
+            q.add(r[k].p[p], fx);
         }
     if (q.c) f(cur, q);
 }
@@ -290,7 +292,8 @@ ATQ_DEV void for_each_pair_run(const PixelRecords (&r)[BND_PX_PER_THREAD], F f) 
 // Records of the 4 pixels of this thread in tile `tile` (w is a multiple
 // of 4, so the 4 pixels share a row).
 ATQ_DEV void tile_records(const uint32_t *labels, const uint32_t *sizes, uint32_t w, uint32_t h,
-                          uint32_t min_size, uint32_t tile, PixelRecords (&r)[BND_PX_PER_THREAD]) {
+                          uint32_t min_size, uint32_t tile, PixelRecords (&r)[BND_PX_PER_THREAD],
+                          bool fx) {
     const size_t n = (size_t)w * h;
     const size_t p0 = (size_t)tile * BND_TILE + threadIdx.x * BND_PX_PER_THREAD;
 #pragma unroll
@@ -300,12 +303,14 @@ ATQ_DEV void tile_records(const uint32_t *labels, const uint32_t *sizes, uint32_
     QuadLabels q;
     load_quad(labels, sizes, w, h, min_size, x0, y, q);
 #pragma unroll
-    for (int k = 0; k < BND_PX_PER_THREAD; k++) quad_pixel_records(q, x0, y, k, 32, r[k]);
+    for (int k = 0; k < BND_PX_PER_THREAD; k++) quad_pixel_records(q, x0, y, k, 32, r[k], fx);
 }
 
 // Each work-group takes a contiguous run of tiles and merges all their
 // records in its LDS table before touching the global one, so a pair that
 // spans the frame costs one global add per work-group, not per tile.
+// FX: fixed mode (record decode and column 0).
+template <bool FX>
 __global__ void __launch_bounds__(256)
 k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes, uint32_t w,
              uint32_t h, uint32_t min_size, uint32_t tiles, PairTable table, uint32_t max_count,
@@ -349,7 +354,8 @@ k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
             size_bits(cur, sz, min_size);
             const uint32_t y = (uint32_t)(p0 / w), x0 = (uint32_t)(p0 % w);
 #pragma unroll
-            for (int k = 0; k < BND_PX_PER_THREAD; k++) quad_pixel_records(cur, x0, y, k, 32, r[k]);
+            for (int k = 0; k < BND_PX_PER_THREAD; k++)
+                quad_pixel_records(cur, x0, y, k, 32, r[k], FX);
         }
         cur = nxt;
         for_each_pair_run(r, [&](uint64_t key, const PairPart &q) {
@@ -366,7 +372,7 @@ k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
             atomicAdd(&lg[0][sl], q.gx);
             atomicAdd(&lg[1][sl], q.gy);
             atomicAdd(&lm[sl], (unsigned long long)q.pg);
-        });
+        }, FX);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < TILE_SLOTS; i += 256)
@@ -384,6 +390,7 @@ k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
         }
 }
 
+template <bool FX>
 __global__ void __launch_bounds__(256)
 k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes, uint32_t w,
              uint32_t h, uint32_t min_size, uint32_t lb, PairTable table, uint32_t max_count,
@@ -401,7 +408,7 @@ k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
     const bool keep_all = counts->pair_filter_off != 0;
     __syncthreads();
     PixelRecords r[BND_PX_PER_THREAD];
-    tile_records(labels, sizes, w, h, min_size, tile, r);
+    tile_records(labels, sizes, w, h, min_size, tile, r, FX);
     auto survives = [&](uint64_t key) -> bool {
         const uint32_t g = global_slot(table, key, false);
         if (g == NO_SLOT) return true;
@@ -420,7 +427,7 @@ k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
         return valid_blob(b, w, h);
     };
     if (!keep_all) {
-        for_each_pair_run(r, [&](uint64_t key, const PairPart &) { tile_slot(lk, key, true); });
+        for_each_pair_run(r, [&](uint64_t key, const PairPart &) { tile_slot(lk, key, true); }, FX);
         __syncthreads();
         for (int i = threadIdx.x; i < TILE_SLOTS; i += 256)
             if (lk[i]) lc[i] = survives(~lk[i]) ? 1u : 0u;
@@ -511,18 +518,26 @@ int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const
                                     const PairTable &table, atq_counts *counts,
                                     uint64_t *tile_keys, atq_boundary_point *tile_points,
                                     uint32_t *tile_scratch, uint64_t *keys,
-                                    atq_boundary_point *points, ScanCtx &sc) {
+                                    atq_boundary_point *points, ScanCtx &sc, bool fx) {
     const unsigned tiles = atq_div_up(w * h, BND_TILE);
     const uint32_t max_count = (uint32_t)(2 * (2 * w + 2 * h));
     uint32_t *tile_counts = tile_scratch, *tile_off = tile_scratch + tiles;
     // a few tiles per work-group (merged in LDS before the global table), at
     // most ~12 of them at large frames
-    k_pair_count<<<min(tiles, std::max(512u, tiles / 12)), 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h,
-                                                   (uint32_t)min_size, tiles, table, max_count,
-                                                   counts);
-    k_bnd_filter<<<xcd_grid(tiles), 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h,
-                                                 (uint32_t)min_size, label_bits, table, max_count,
-                                                 counts, tiles, tile_keys, tile_points, tile_counts);
+    const unsigned pg = min(tiles, std::max(512u, tiles / 12));
+    if (fx) {
+        k_pair_count<true><<<pg, 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h,
+                                              (uint32_t)min_size, tiles, table, max_count, counts);
+        k_bnd_filter<true><<<xcd_grid(tiles), 256, 0, s>>>(
+            labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size, label_bits, table,
+            max_count, counts, tiles, tile_keys, tile_points, tile_counts);
+    } else {
+        k_pair_count<false><<<pg, 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h,
+                                               (uint32_t)min_size, tiles, table, max_count, counts);
+        k_bnd_filter<false><<<xcd_grid(tiles), 256, 0, s>>>(
+            labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size, label_bits, table,
+            max_count, counts, tiles, tile_keys, tile_points, tile_counts);
+    }
     int r = device_scan(s, nullptr, tiles, TileCount{tile_counts}, TileOffset{tile_off},
                         sc, &counts->n_boundary);
     if (r) return r;

@@ -560,4 +560,284 @@ int launch_ccl(hipStream_t s, const uint8_t *thr, size_t w, size_t h, uint32_t *
     return launch_ccl_frames(s, &fr, 1, FrontShapes{});
 }
 
+
+// ---------------------------------------------------------------------------
+// Fixed mode: pixel-level labelling, white 8-connected, black 4-connected,
+// 127 pixels unlabelled (the reference's block graph instead links colour
+// sets through a colour-agnostic neighbour mask, src/segmentation.cpp:139-255).
+// Labels: white 0x80000000 | (root + 1), black root + 1, 127 -> 0, root the
+// component's minimum pixel index; sizes[root + 1] = its pixel count
+// (saturating near size_cap like the compat form), sizes[0] = 0, so label 0
+// (127 pixels, out-of-image neighbours) never reads as a large component.
+//
+// The same four steps as the compat form:
+//  1. k_fx_local: a work-group owns a 64 x 16 pixel tile, each wave four of
+//     its rows, one pixel per lane. One ballot per row gives every pixel the
+//     first pixel of its same-colour run (the horizontal links); the links to
+//     the row above (up; up-left and up-right for white) go through an LDS
+//     union-find, a union that repeats the previous lane's pair skipped.
+//     Writes parent words (FLAG | local root for non-roots) and each local
+//     root's count at sizes[root + 1].
+//  2. k_fx_border: the edges that leave a tile - every 8-neighbour edge is
+//     owned by its later pixel in raster order (left, up-left, up, up-right),
+//     so a tile owns those of its top row (up, up-left, up-right), its left
+//     column (left, up-left) and its right column (up-right) - as distinct
+//     pairs of local roots, merged on the global parent array.
+//  3. k_fx_flatten, 4. k_fx_final: as k_ccl_flatten / k_ccl_final per pixel.
+namespace {
+constexpr int FX_TX = 64, FX_TY = 16;  // pixel tile: one row per wave lane set
+constexpr int FX_PX = FX_TX * FX_TY;
+constexpr int FX_BORDER_ITEMS = FX_TX + FX_TY + (FX_TY - 1);  // top, left, right (rows >= 1)
+
+struct FxLds {
+    uint32_t px[FX_TY][FX_TX / 4];  // 4 pixels per word
+    uint32_t lp[FX_PX];
+    uint32_t cnt[FX_PX];
+};
+
+ATQ_DEV uint8_t fx_px(const FxLds &L, int r, int c) {
+    return (uint8_t)(L.px[r][c >> 2] >> (8 * (c & 3)));
+}
+
+ATQ_DEV uint32_t fx_tiles_x(uint32_t w) { return (w + FX_TX - 1) / FX_TX; }
+ATQ_DEV uint32_t fx_tiles(uint32_t w, uint32_t h) { return fx_tiles_x(w) * ((h + FX_TY - 1) / FX_TY); }
+
+ATQ_DEV void fx_local_tile(FxLds &L, const FrontFrame &F, uint32_t tile) {
+    const uint32_t w = F.w, h = F.h, tx = fx_tiles_x(w);
+    const int x0 = (int)(tile % tx) * FX_TX, y0 = (int)(tile / tx) * FX_TY;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    {
+        // 4 pixels per thread; w is a multiple of 4, so a word never
+        // straddles the image's right edge
+        const int r = tid >> 4, c4 = (tid & 15) * 4;
+        const int y = y0 + r, x = x0 + c4;
+        L.px[r][c4 >> 2] = (y < (int)h && x < (int)w)
+                               ? *reinterpret_cast<const uint32_t *>(F.thr + (size_t)y * w + x)
+                               : 0x7F7F7F7Fu;
+    }
+    for (int i = tid; i < FX_PX; i += 256) L.cnt[i] = 0;
+    __syncthreads();
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+    // runs: every pixel points at the first pixel of its same-colour run
+#pragma unroll
+    for (int k = 0; k < FX_TY / 4; k++) {
+        const int r = 4 * wave + k;
+        const uint8_t v = fx_px(L, r, lane);
+        const bool cont = lane > 0 && v != 127 && fx_px(L, r, lane - 1) == v;
+        const uint64_t starts = __ballot(!cont) & upto;
+        L.lp[r * FX_TX + lane] = r * FX_TX + (63 - __clzll((long long)starts));
+    }
+    __syncthreads();
+    // links to the row above, within the tile
+#pragma unroll
+    for (int k = 0; k < FX_TY / 4; k++) {
+        const int r = 4 * wave + k;
+        const uint8_t v = fx_px(L, r, lane);
+        const uint32_t me = L.lp[r * FX_TX + lane];
+#pragma unroll
+        for (int d = -1; d <= 1; d++) {
+            const int c = lane + d;
+            const bool link = r > 0 && v != 127 && c >= 0 && c < FX_TX && (d == 0 || v == C_WHITE) &&
+                              fx_px(L, r - 1, c) == v;
+            const uint32_t ra = link ? me : 0xFFFFFFFFu;
+            const uint32_t rb = link ? L.lp[(r - 1) * FX_TX + c] : 0xFFFFFFFFu;
+            const uint32_t pa = __shfl_up(ra, 1, 64), pb = __shfl_up(rb, 1, 64);
+            const bool dup = lane > 0 && pa == ra && pb == rb;
+            if (link && !dup) lds_union(L.lp, ra, rb);
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < FX_PX; i += 256) L.lp[i] = lds_find(L.lp, i);
+    __syncthreads();
+    for (int i = tid; i < FX_PX; i += 256)
+        if (fx_px(L, i / FX_TX, i % FX_TX) != 127) atomicAdd(&L.cnt[L.lp[i]], 1u);
+    __syncthreads();
+    for (int i = tid; i < FX_PX; i += 256) {
+        const int y = y0 + i / FX_TX, x = x0 + i % FX_TX;
+        if (y >= (int)h || x >= (int)w) continue;
+        const uint32_t r = L.lp[i];
+        const uint32_t gr = (uint32_t)(y0 + (int)(r / FX_TX)) * w + (uint32_t)(x0 + (int)(r % FX_TX));
+        const uint32_t gi = (uint32_t)y * w + (uint32_t)x;
+        F.parent[gi] = r == (uint32_t)i ? gr : (gr | FLAG);
+        if (r == (uint32_t)i) F.sizes[gi + 1] = L.cnt[i];
+    }
+    __syncthreads();  // the LDS is reused by the next tile
+}
+}  // namespace
+
+__global__ void __launch_bounds__(256)
+k_fx_local(FrontTable tab) {
+    __shared__ FxLds L;
+    for (uint32_t fi = 0; fi < tab.n; fi++) {
+        const FrontFrame &F = tab.fr[fi];
+        if (blockIdx.x == 0 && threadIdx.x == 0) F.sizes[0] = 0;
+        const uint32_t n_tiles = fx_tiles(F.w, F.h);
+        for (uint32_t it = blockIdx.x; it < xcd_grid(n_tiles); it += gridDim.x) {
+            const uint32_t tile = xcd_tile(it, n_tiles);
+            if (tile < n_tiles) fx_local_tile(L, F, tile);
+        }
+    }
+}
+
+// The border edges of one tile (two waves, item k < FX_BORDER_ITEMS: the
+// top row, the left column, then the right column below its first row).
+// One round of independent loads: the item's pixel and parent word, each
+// probed neighbour's pixel and parent word.
+ATQ_DEV void fx_border_tile(uint64_t *hs, const FrontFrame &F, uint32_t tile, uint32_t k) {
+    constexpr uint32_t NONE = 0xFFFFFFFFu;
+    constexpr uint64_t EMPTY = ~0ull;
+    const uint32_t w = F.w, h = F.h, tx = fx_tiles_x(w);
+    const int x0 = (int)(tile % tx) * FX_TX, y0 = (int)(tile / tx) * FX_TY;
+    int x = 0, y = 0, dx[3] = {0, 0, 0}, dy[3] = {0, 0, 0}, np = 0;
+    if (k < (uint32_t)FX_TX) {  // top row: up-left, up, up-right
+        x = x0 + (int)k;
+        y = y0;
+        np = 3;
+        dx[0] = -1; dx[1] = 0; dx[2] = 1;
+        dy[0] = dy[1] = dy[2] = -1;
+    } else if (k < (uint32_t)(FX_TX + FX_TY)) {  // left column: left, up-left
+        x = x0;
+        y = y0 + (int)(k - FX_TX);
+        np = y > y0 ? 2 : 1;
+        dx[0] = -1; dy[0] = 0;
+        dx[1] = -1; dy[1] = -1;
+    } else if (k < (uint32_t)FX_BORDER_ITEMS) {  // right column below row 0: up-right
+        x = x0 + FX_TX - 1;
+        y = y0 + 1 + (int)(k - FX_TX - FX_TY);
+        np = 1;
+        dx[0] = 1; dy[0] = -1;
+    }
+    const bool live = np > 0 && x < (int)w && y < (int)h;
+    uint8_t v = 127, pv[3] = {127, 127, 127};
+    uint32_t own = 0, nb[3] = {0, 0, 0};
+    if (live) {
+        const uint32_t gi = (uint32_t)y * w + (uint32_t)x;
+        v = F.thr[gi];
+        own = g_load(&F.parent[gi]);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const int qx = x + dx[i], qy = y + dy[i];
+        if (live && i < np && qx >= 0 && qx < (int)w && qy >= 0) {
+            const uint32_t gq = (uint32_t)qy * w + (uint32_t)qx;
+            pv[i] = F.thr[gq];
+            nb[i] = g_load(&F.parent[gq]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const bool diag = dx[i] != 0 && dy[i] != 0;
+        if (v == 127 || pv[i] != v || (diag && v != C_WHITE)) continue;
+        const uint32_t a = own & ~FLAG, b = nb[i] & ~FLAG;
+        if (a == b) continue;
+        const uint32_t ea = min(a, b), eb = max(a, b);
+        const uint64_t key = ((uint64_t)ea << 32) | eb;
+        uint32_t hsl = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 32) & (BORDER_SLOTS - 1);
+        bool mine = true;  // the set is full: merge it here
+        for (int p = 0; p < BORDER_SLOTS; p++, hsl = (hsl + 1) & (BORDER_SLOTS - 1)) {
+            const uint64_t old = atomicCAS((unsigned long long *)&hs[hsl], (unsigned long long)EMPTY,
+                                           (unsigned long long)key);
+            if (old == EMPTY) break;
+            if (old == key) {
+                mine = false;
+                break;
+            }
+        }
+        if (mine) g_union(F.parent, ea, eb);
+    }
+    (void)NONE;
+}
+
+// Two tiles per work-group round, 128 threads each.
+__global__ void __launch_bounds__(256)
+k_fx_border(FrontTable tab) {
+    __shared__ uint64_t set[2][BORDER_SLOTS];
+    const uint32_t half = threadIdx.x >> 7, k = threadIdx.x & 127;
+    uint64_t *hs = set[half];
+    for (uint32_t fi = 0; fi < tab.n; fi++) {
+        const FrontFrame &F = tab.fr[fi];
+        const uint32_t n_tiles = fx_tiles(F.w, F.h);
+        for (uint32_t g = blockIdx.x; g < (n_tiles + 1) / 2; g += gridDim.x) {
+            for (int i = k; i < BORDER_SLOTS; i += 128) hs[i] = ~0ull;
+            __syncthreads();
+            const uint32_t tile = 2 * g + half;
+            if (tile < n_tiles) fx_border_tile(hs, F, tile, k);
+            __syncthreads();
+        }
+    }
+}
+
+// Every local root (a parent word without FLAG that is not its own index
+// is a local root merged by the border pass) gets its final root and adds
+// its count into sizes[root + 1]; 4 pixels per thread.
+__global__ void __launch_bounds__(256)
+k_fx_flatten(FrontTable tab) {
+    for (uint32_t fi = 0; fi < tab.n; fi++) {
+        const FrontFrame &F = tab.fr[fi];
+        uint32_t *__restrict__ parent = F.parent;
+        uint32_t *__restrict__ sizes = F.sizes;
+        const uint32_t quads = F.w * F.h / 4, size_cap = F.size_cap;
+        for (uint32_t q = blockIdx.x * 256 + threadIdx.x; q < quads; q += gridDim.x * 256) {
+            const uint4 v4 = *reinterpret_cast<const uint4 *>(parent + 4 * (size_t)q);
+            const uint32_t v[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+            for (uint32_t j = 0; j < 4; j++) {
+                const uint32_t n = 4 * q + j;
+                if ((v[j] & FLAG) || v[j] == n) continue;
+                const uint32_t r = g_find_halving(parent, v[j]);
+                atomicMin(&parent[n], r);
+                const uint32_t sz = sizes[n + 1];
+                if (sz && g_load(&sizes[r + 1]) < size_cap) atomicAdd(&sizes[r + 1], sz);
+            }
+        }
+    }
+}
+
+// Labels, 4 pixels per thread.
+__global__ void __launch_bounds__(256)
+k_fx_final(FrontTable tab) {
+    for (uint32_t fi = 0; fi < tab.n; fi++) {
+        const FrontFrame &F = tab.fr[fi];
+        const uint32_t *__restrict__ parent = F.parent;
+        const uint32_t quads = F.w * F.h / 4;
+        for (uint32_t q = blockIdx.x * 256 + threadIdx.x; q < quads; q += gridDim.x * 256) {
+            const uint4 v4 = *reinterpret_cast<const uint4 *>(parent + 4 * (size_t)q);
+            const uint32_t t4 = *reinterpret_cast<const uint32_t *>(F.thr + 4 * (size_t)q);
+            const uint32_t v[4] = {v4.x, v4.y, v4.z, v4.w};
+            uint32_t r[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) r[j] = (v[j] & FLAG) ? parent[v[j] & ~FLAG] : v[j];
+            uint32_t lab[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t p = (t4 >> (8 * j)) & 0xFFu;
+                lab[j] = p == C_WHITE ? (LABEL_PIXEL_MASK | (r[j] + 1))
+                                      : (p == C_BLACK ? r[j] + 1 : 0u);
+            }
+            *reinterpret_cast<uint4 *>(F.labels + 4 * (size_t)q) = make_uint4(lab[0], lab[1], lab[2], lab[3]);
+        }
+    }
+}
+
+int launch_ccl_fixed_frames(hipStream_t s, const FrontFrame *frames, int n) {
+    if (n < 1 || n > FRONT_MAX_FRAMES) return ATQ_ERR_BAD_ARG;
+    FrontTable tab{};
+    uint32_t tiles = 0, quads = 0;
+    for (int i = 0; i < n; i++) {
+        tab.fr[i] = frames[i];
+        const uint32_t w = frames[i].w, h = frames[i].h;
+        tiles = std::max(tiles, (uint32_t)(atq_div_up(w, FX_TX) * atq_div_up(h, FX_TY)));
+        quads = std::max(quads, w * h / 4);
+    }
+    tab.n = (uint32_t)n;
+    const uint32_t lg = std::min<uint32_t>(xcd_grid(tiles), 8192u);
+    const uint32_t bg = std::min<uint32_t>(atq_div_up(tiles, 2), 4096u);
+    const uint32_t fg = std::min<uint32_t>(atq_div_up(quads, 256), 16384u);
+    k_fx_local<<<lg ? lg : 8, 256, 0, s>>>(tab);
+    k_fx_border<<<bg ? bg : 1, 256, 0, s>>>(tab);
+    k_fx_flatten<<<fg ? fg : 1, 256, 0, s>>>(tab);
+    k_fx_final<<<fg ? fg : 1, 256, 0, s>>>(tab);
+    return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
+}
+
 }  // namespace atq

@@ -83,9 +83,9 @@ struct BoundsAcc {
         gxs = gys = 0;
         pg = 0;
     }
-    ATQ_DEV void add(atq_boundary_point p) {
-        const uint32_t xx = x_with_dx(p.packed_x), yy = y_with_dy(p.packed_x, p.packed_y);
-        const int gx = grad_x(p.packed_x, p.packed_y), gy = grad_y(p.packed_x, p.packed_y);
+    ATQ_DEV void add(atq_boundary_point p, bool fx) {
+        const uint32_t xx = x_with_dx(p.packed_x, fx), yy = y_with_dy(p.packed_x, p.packed_y, fx);
+        const int gx = grad_x(p.packed_x, p.packed_y, fx), gy = grad_y(p.packed_x, p.packed_y, fx);
         xmin = min(xmin, xx);
         xmax = max(xmax, xx);
         ymin = min(ymin, yy);
@@ -121,6 +121,8 @@ struct BoundsAcc {
 
 constexpr int SB_NT = 1024, SB_IPT = 8;  // k_seg_bounds: a segment's records per round
 
+// FX: fixed mode (both half-pixel bits decoded).
+template <bool FX>
 __global__ void __launch_bounds__(SB_NT)
 k_seg_bounds(const atq_boundary_point *__restrict__ pts, const uint32_t *__restrict__ seg_start,
              const atq_counts *counts, uint32_t w, uint32_t h, uint32_t *__restrict__ valid,
@@ -148,7 +150,7 @@ k_seg_bounds(const atq_boundary_point *__restrict__ pts, const uint32_t *__restr
             }
 #pragma unroll
             for (int k = 0; k < SB_IPT; k++)
-                if (i0 + SB_NT * k < e) acc.add(q[k]);
+                if (i0 + SB_NT * k < e) acc.add(q[k], FX);
         }
         acc.wave_reduce();
         if (lane == 0) part[wave] = acc;
@@ -175,6 +177,7 @@ k_seg_bounds(const atq_boundary_point *__restrict__ pts, const uint32_t *__restr
 }
 
 // Each record of a valid segment goes to its final slot with its slope.
+template <bool FX>
 __global__ void __launch_bounds__(256)
 k_assign(const atq_boundary_point *__restrict__ pts, const uint32_t *__restrict__ seg_id,
          const uint32_t *__restrict__ seg_start, const uint32_t *__restrict__ seg_cluster,
@@ -189,7 +192,7 @@ k_assign(const atq_boundary_point *__restrict__ pts, const uint32_t *__restrict_
         uint32_t pos = ext[c].start + (i - seg_start[s]);
         atq_boundary_point p = pts[i];
         const atq_cluster_bounds &b = bounds[c];
-        float th = calc_theta(p.packed_x, p.packed_y, bounds_cx(b), bounds_cy(b));
+        float th = calc_theta(p.packed_x, p.packed_y, bounds_cx(b), bounds_cy(b), FX);
         cpts[pos] = atq_cluster_point{p.packed_x, p.packed_y, th};
         cidx[pos] = c;
     }
@@ -266,19 +269,21 @@ constexpr uint32_t FIT_STAGE_WORDS = 6 * FIT_STAGE;
 ATQ_DEV void fit_terms(const atq_cluster_point *in, uint32_t start, uint32_t e0, uint32_t m,
                        const uint32_t (&src)[SEG_IPT], const uint8_t *grey, uint32_t w,
                        uint32_t h, atq_cluster_point (&p)[SEG_IPT], uint32_t (&W)[SEG_IPT],
-                       uint32_t (&RX)[SEG_IPT], uint32_t (&RY)[SEG_IPT], int64_t (&sum)[6]) {
+                       uint32_t (&RX)[SEG_IPT], uint32_t (&RY)[SEG_IPT], int64_t (&sum)[6],
+                       bool fx) {
 #pragma unroll
     for (int i = 0; i < SEG_IPT; i++) p[i] = in[start + (e0 + i < m ? src[i] : 0u)];
     uint32_t g4[SEG_IPT][4];
 #pragma unroll
-    for (int i = 0; i < SEG_IPT; i++) linefit_gather(p[i].packed_x, p[i].packed_y, w, h, grey, g4[i]);
+    for (int i = 0; i < SEG_IPT; i++)
+        linefit_gather(p[i].packed_x, p[i].packed_y, w, h, grey, g4[i], fx);
 #pragma unroll
     for (int q = 0; q < 6; q++) sum[q] = 0;
 #pragma unroll
     for (int i = 0; i < SEG_IPT; i++) {
         W[i] = RX[i] = RY[i] = 0;
         if (e0 + i >= m) continue;
-        linefit_terms_from(p[i].packed_x, p[i].packed_y, w, h, g4[i], &W[i], &RX[i], &RY[i]);
+        linefit_terms_from(p[i].packed_x, p[i].packed_y, w, h, g4[i], &W[i], &RX[i], &RY[i], fx);
         const atq_line_fit_point f = linefit_from_terms(W[i], RX[i], RY[i]);
         sum[0] += f.Mx;
         sum[1] += f.My;
@@ -302,7 +307,7 @@ ATQ_DEV void fit_chunk(const atq_cluster_point *in, atq_cluster_point *out, uint
                        uint32_t base, uint32_t m, const uint32_t (&src)[SEG_IPT],
                        const uint8_t *grey, uint32_t w, uint32_t h, atq_line_fit_point *lf,
                        const int64_t (&carry)[6], int64_t (&tot)[6], int64_t *lds,
-                       int64_t *stage) {
+                       int64_t *stage, bool fx) {
     const uint32_t t = threadIdx.x, e0 = t * SEG_IPT;
     atq_cluster_point p[SEG_IPT];
     uint32_t W[SEG_IPT], RX[SEG_IPT], RY[SEG_IPT];
@@ -314,7 +319,7 @@ ATQ_DEV void fit_chunk(const atq_cluster_point *in, atq_cluster_point *out, uint
                 if (e0 + i < m) out[start + base + e0 + i] = in[start + src[i]];
         return;
     }
-    fit_terms(in, start, e0, m, src, grey, w, h, p, W, RX, RY, sum);
+    fit_terms(in, start, e0, m, src, grey, w, h, p, W, RX, RY, sum, fx);
     if (out) {
 #pragma unroll
         for (int i = 0; i < SEG_IPT; i++)
@@ -358,6 +363,7 @@ ATQ_DEV void fit_chunk(const atq_cluster_point *in, atq_cluster_point *out, uint
 
 // Clusters of at most SEG_CHUNK points are finished inside the sort, from
 // the sorted keys in registers.
+template <bool FX>
 struct AngleFinish {
     const atq_cluster_point *in;
     atq_cluster_point *out;
@@ -375,7 +381,7 @@ struct AngleFinish {
         // the scan's scratch sits at the far end of the other half
         int64_t *l = reinterpret_cast<int64_t *>(lds);
         fit_chunk(in, out, start, 0, n, src, grey, w, h, lf, zero, tot,
-                  l + 2 * SEG_CHUNK - (6 * SEG_NT / 64 + 6), l);
+                  l + 2 * SEG_CHUNK - (6 * SEG_NT / 64 + 6), l, FX);
     }
 };
 
@@ -390,6 +396,7 @@ ATQ_DEV uint32_t fit_slot(uint32_t c, uint32_t start, uint32_t k) {
 // Sums of the line-fit moments of every chunk but the last of the listed
 // long clusters, into the slot table (the carries the chunk-parallel fit
 // adds).
+template <bool FX>
 __global__ void __launch_bounds__(SEG_NT)
 k_fit_reduce(const atq_cluster_point *__restrict__ in, const uint64_t *__restrict__ keys0,
              const uint64_t *__restrict__ keys1, const atq_cluster_extents *__restrict__ ext,
@@ -411,7 +418,7 @@ k_fit_reduce(const atq_cluster_point *__restrict__ in, const uint64_t *__restric
         atq_cluster_point p[SEG_IPT];
         uint32_t W[SEG_IPT], RX[SEG_IPT], RY[SEG_IPT];
         int64_t sum[6];
-        fit_terms(in, start, e0, SEG_CHUNK, src, grey, w, h, p, W, RX, RY, sum);
+        fit_terms(in, start, e0, SEG_CHUNK, src, grey, w, h, p, W, RX, RY, sum, FX);
 #pragma unroll
         for (int q = 0; q < 6; q++) {
             int64_t x = sum[q];
@@ -442,7 +449,7 @@ ATQ_DEV int64_t uniform64(int64_t v) {
 // from the sums of the cluster's earlier chunks (slot table, k_fit_reduce).
 // SEQ: every cluster (n_clusters_dev) in input order, one work-group per
 // cluster walking its chunks and carrying the sums itself.
-template <bool SEQ>
+template <bool SEQ, bool FX>
 __global__ void __launch_bounds__(SEG_NT)
 k_fit_chunks(const atq_cluster_point *__restrict__ in, atq_cluster_point *__restrict__ out,
              const uint64_t *__restrict__ keys0, const uint64_t *__restrict__ keys1,
@@ -474,7 +481,7 @@ k_fit_chunks(const atq_cluster_point *__restrict__ in, atq_cluster_point *__rest
                 src[i] = j < n ? (keys ? (uint32_t)keys[j] : j) : 0u;
             }
             fit_chunk(in, out, start, bb, min(SEG_CHUNK, n - bb), src, grey, w, h, lf, carry, tot,
-                      lds, stage);
+                      lds, stage, FX);
             if (SEQ)
 #pragma unroll
                 for (int q = 0; q < 6; q++) carry[q] += uniform64(tot[q]);
@@ -505,8 +512,9 @@ int launch_segments(hipStream_t s, const uint64_t *keys, uint32_t key_shift, con
 int launch_bounds_filter(hipStream_t s, const atq_boundary_point *pts, const uint32_t *seg_start,
                          atq_counts *counts, uint32_t cap, uint32_t w, uint32_t h, uint32_t *valid,
                          atq_cluster_bounds *seg_bounds, uint32_t *seg_cluster,
-                         atq_cluster_bounds *bounds, ScanCtx &sc) {
-    k_seg_bounds<<<1024, SB_NT, 0, s>>>(pts, seg_start, counts, w, h, valid, seg_bounds);
+                         atq_cluster_bounds *bounds, ScanCtx &sc, bool fx) {
+    if (fx) k_seg_bounds<true><<<1024, SB_NT, 0, s>>>(pts, seg_start, counts, w, h, valid, seg_bounds);
+    else k_seg_bounds<false><<<1024, SB_NT, 0, s>>>(pts, seg_start, counts, w, h, valid, seg_bounds);
     return device_scan(s, &counts->n_pairs, cap, ValidFlag{valid},
                        ValidOut{seg_cluster, seg_bounds, bounds}, sc, &counts->n_clusters);
 }
@@ -520,34 +528,55 @@ int launch_extents(hipStream_t s, const atq_cluster_bounds *bounds, atq_counts *
 int launch_assign(hipStream_t s, const atq_boundary_point *pts, const uint32_t *seg_id,
                   const uint32_t *seg_start, const uint32_t *seg_cluster,
                   const atq_cluster_bounds *bounds, const atq_cluster_extents *ext,
-                  const uint32_t *n_dev, uint32_t cap, atq_cluster_point *cpts, uint32_t *cidx) {
-    k_assign<<<2048, 256, 0, s>>>(pts, seg_id, seg_start, seg_cluster, bounds, ext, n_dev, cap,
-                                  cpts, cidx);
+                  const uint32_t *n_dev, uint32_t cap, atq_cluster_point *cpts, uint32_t *cidx,
+                  bool fx) {
+    if (fx)
+        k_assign<true><<<2048, 256, 0, s>>>(pts, seg_id, seg_start, seg_cluster, bounds, ext,
+                                            n_dev, cap, cpts, cidx);
+    else
+        k_assign<false><<<2048, 256, 0, s>>>(pts, seg_id, seg_start, seg_cluster, bounds, ext,
+                                             n_dev, cap, cpts, cidx);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
+
+namespace {
+template <bool FX>
+int angle_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out,
+              const atq_cluster_extents *ext, const atq_counts *counts, uint32_t cap_cl,
+              uint32_t max_cluster, const uint8_t *grey, uint32_t w, uint32_t h,
+              atq_line_fit_point *lf, uint64_t *keys0, uint64_t *keys1, int64_t *slot_sums,
+              SegLong lng) {
+    int r = seg_sort(s, AngleKey{in}, AngleFinish<FX>{in, out, grey, w, h, lf}, ext,
+                     &counts->n_clusters, cap_cl, max_cluster, keys0, keys1, lng);
+    if (r || max_cluster <= SEG_CHUNK) return r;
+    const uint32_t max_chunks = seg_max_chunks(max_cluster);
+    if (lf)
+        k_fit_reduce<FX><<<512, SEG_NT, 0, s>>>(in, keys0, keys1, ext, lng, max_chunks, grey, w,
+                                                h, slot_sums);
+    k_fit_chunks<false, FX><<<512, SEG_NT, 0, s>>>(in, out, keys0, keys1, ext,
+                                                   &counts->n_clusters, cap_cl, lng, max_chunks,
+                                                   grey, w, h, lf, slot_sums);
+    return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
+}
+}  // namespace
 
 int launch_angle_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out,
                      const atq_cluster_extents *ext, const atq_counts *counts, uint32_t cap_cl,
                      uint32_t max_cluster, const uint8_t *grey, uint32_t w, uint32_t h,
                      atq_line_fit_point *lf, uint64_t *keys0, uint64_t *keys1, int64_t *slot_sums,
-                     SegLong lng) {
-    int r = seg_sort(s, AngleKey{in}, AngleFinish{in, out, grey, w, h, lf}, ext,
-                     &counts->n_clusters, cap_cl, max_cluster, keys0, keys1, lng);
-    if (r || max_cluster <= SEG_CHUNK) return r;
-    const uint32_t max_chunks = seg_max_chunks(max_cluster);
-    if (lf)
-        k_fit_reduce<<<512, SEG_NT, 0, s>>>(in, keys0, keys1, ext, lng, max_chunks, grey, w, h,
-                                            slot_sums);
-    k_fit_chunks<false><<<512, SEG_NT, 0, s>>>(in, out, keys0, keys1, ext, &counts->n_clusters,
-                                               cap_cl, lng, max_chunks, grey, w, h, lf, slot_sums);
-    return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
+                     SegLong lng, bool fx) {
+    return fx ? angle_fit<true>(s, in, out, ext, counts, cap_cl, max_cluster, grey, w, h, lf,
+                                keys0, keys1, slot_sums, lng)
+              : angle_fit<false>(s, in, out, ext, counts, cap_cl, max_cluster, grey, w, h, lf,
+                                 keys0, keys1, slot_sums, lng);
 }
 
 int launch_cluster_fit(hipStream_t s, const atq_cluster_point *in, const atq_cluster_extents *ext,
                        const uint32_t *n_clusters_dev, uint32_t cap_cl, const uint8_t *grey,
                        uint32_t w, uint32_t h, atq_line_fit_point *lf) {
-    k_fit_chunks<true><<<1024, SEG_NT, 0, s>>>(in, nullptr, nullptr, nullptr, ext, n_clusters_dev,
-                                               cap_cl, SegLong{}, 1, grey, w, h, lf, nullptr);
+    k_fit_chunks<true, false><<<1024, SEG_NT, 0, s>>>(in, nullptr, nullptr, nullptr, ext,
+                                                      n_clusters_dev, cap_cl, SegLong{}, 1, grey,
+                                                      w, h, lf, nullptr);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 

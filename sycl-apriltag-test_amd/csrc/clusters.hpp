@@ -15,24 +15,25 @@ int launch_segments(hipStream_t s, const uint64_t *keys, uint32_t key_shift, con
 int launch_bounds_filter(hipStream_t s, const atq_boundary_point *pts, const uint32_t *seg_start,
                          atq_counts *counts, uint32_t cap, uint32_t w, uint32_t h, uint32_t *valid,
                          atq_cluster_bounds *seg_bounds, uint32_t *seg_cluster,
-                         atq_cluster_bounds *bounds, ScanCtx &sc);
+                         atq_cluster_bounds *bounds, ScanCtx &sc, bool fx = false);
 int launch_extents(hipStream_t s, const atq_cluster_bounds *bounds, atq_counts *counts, uint32_t cap,
                    atq_cluster_extents *ext, ScanCtx &sc);
 int launch_assign(hipStream_t s, const atq_boundary_point *pts, const uint32_t *seg_id,
                   const uint32_t *seg_start, const uint32_t *seg_cluster,
                   const atq_cluster_bounds *bounds, const atq_cluster_extents *ext,
-                  const uint32_t *n_dev, uint32_t cap, atq_cluster_point *cpts, uint32_t *cidx);
+                  const uint32_t *n_dev, uint32_t cap, atq_cluster_point *cpts, uint32_t *cidx,
+                  bool fx = false);
 // Angle sort of every cluster's points (src/main.cpp:737-750), the points
 // in that order to `out` and their line-fit prefix sums to `lf` (may be
 // null; src/main.cpp:768-801). max_cluster bounds the cluster length.
 // keys0/keys1: two key buffers of cap entries; slot_sums: 6 * (cap_cl + cap
 // / SEG_CHUNK + 2) int64; lng: a long-segment list of seg_long_cap(cap)
-// entries whose count is zero.
+// entries whose count is zero. fx: fixed mode (record decode, weight row).
 int launch_angle_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out,
                      const atq_cluster_extents *ext, const atq_counts *counts, uint32_t cap_cl,
                      uint32_t max_cluster, const uint8_t *grey, uint32_t w, uint32_t h,
                      atq_line_fit_point *lf, uint64_t *keys0, uint64_t *keys1, int64_t *slot_sums,
-                     SegLong lng);
+                     SegLong lng, bool fx = false);
 // Line-fit prefix sums of points already in angle order (one work-group per
 // cluster).
 int launch_cluster_fit(hipStream_t s, const atq_cluster_point *in, const atq_cluster_extents *ext,
@@ -46,10 +47,12 @@ int launch_run_extents(hipStream_t s, const uint32_t *cidx, const uint32_t *n_de
 
 // fitquad.hip
 // fit_lines (src/fit_quad.cpp:153-375, per point). errs (cap
-// floats, may be null) holds the window errors between two kernels.
+// floats, may be null) holds the window errors between two kernels. fx:
+// fixed mode (global index 0 is a candidate too).
 int launch_fit_lines(hipStream_t s, const atq_line_fit_point *lf, const uint32_t *cidx,
                      const atq_cluster_extents *ext, const uint32_t *n_points_dev, uint32_t cap,
-                     atq_corner *sparse, uint32_t *ccount, float *errs = nullptr);
+                     atq_corner *sparse, uint32_t *ccount, float *errs = nullptr,
+                     bool fx = false);
 int launch_set_u32(hipStream_t s, uint32_t *p, uint32_t v);
 // Zeroes p[0 .. min(*n_dev, cap)) (n_dev null: cap words), except that
 // p[set_idx] gets set_val.
@@ -66,7 +69,7 @@ int launch_do_indexing(hipStream_t s, const atq_peak_extents *peaks, const uint3
                        uint32_t cap, const atq_corner *corners, const atq_line_fit_point *lf,
                        const uint32_t *n_points_dev, uint32_t n_points_host,
                        const atq_cluster_extents *ext, atq_fitted_quad *fitted,
-                       uint32_t *oob_counter);
+                       uint32_t *oob_counter, bool fx = false);
 int launch_quads(hipStream_t s, const atq_fitted_quad *fitted, const uint32_t *n_fitted_dev,
                  uint32_t cap, atq_quad_corners *tmp, uint32_t *flags, atq_quad_corners *out,
                  uint32_t *n_out_dev, ScanCtx &sc, int min_tag_width);

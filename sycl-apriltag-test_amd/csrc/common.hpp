@@ -30,16 +30,22 @@ constexpr uint32_t OVF_BOUNDARY = 1u << 0;
 constexpr uint32_t OVF_CORNERS = 1u << 1;
 constexpr uint32_t OVF_SCAN = 1u << 2;  // a look-back wait gave up: results void
 
-ATQ_DEV int half_pixel_dx(uint16_t packed_x) {
-    switch ((packed_x & HALF_PIXEL_MASK) >> 14) {
+// The record's half-pixel: always RIGHT (0) as the reference decodes it; in
+// fixed mode (fx) both bits, the plane the record came from (0 RIGHT,
+// 1 BOTTOM_LEFT, 2 BOTTOM, 3 BOTTOM_RIGHT; include/boundary_point.h:8-28).
+ATQ_DEV uint32_t half_pixel(uint16_t packed_x, bool fx) {
+    return fx ? ((uint32_t)packed_x >> 14) & 3u : (uint32_t)(packed_x & HALF_PIXEL_MASK) >> 14;
+}
+ATQ_DEV int half_pixel_dx(uint16_t packed_x, bool fx = false) {
+    switch (half_pixel(packed_x, fx)) {
     case 0: return 1;
     case 1: return -1;
     case 2: return 0;
     default: return 1;
     }
 }
-ATQ_DEV int half_pixel_dy(uint16_t packed_x) {
-    return ((packed_x & HALF_PIXEL_MASK) >> 14) == 0 ? 0 : 1;
+ATQ_DEV int half_pixel_dy(uint16_t packed_x, bool fx = false) {
+    return half_pixel(packed_x, fx) == 0 ? 0 : 1;
 }
 
 // glibc's hypotf: the binary64 root of the exactly-rounded binary64 sum.

@@ -8,13 +8,14 @@
 // filter f(p) = sum_t F[t] e(p+t-3) accumulated in order, and a strict local
 // maximum test f(p) > f(p-1), f(p) > f(p+1), all indices modulo the cluster
 // size n; global index 0 is never a candidate (its work-group slot is a
-// halo). Here one thread per point evaluates that definition directly from
-// the prefix-summed moments.
+// halo; in fixed mode it is). Here one thread per point evaluates that
+// definition directly from the prefix-summed moments.
 //
 // Corners (error != 0) are compacted, grouped per cluster and sorted by
 // (error desc, index asc) - src/main.cpp:832-907 with a stable sort. do_indexing runs one 256-item work-group per peak
 // segment like the reference (src/fit_quad.cpp:392-535), pairing segment e
-// with cluster extents[e] as the reference does. The quad checks of
+// with cluster extents[e] as the reference does (fixed mode: with its own
+// cluster's extents). The quad checks of
 // process_fitted_quads run one thread per fitted quad.
 #include "clusters.hpp"
 #include "segsort.hpp"
@@ -169,12 +170,13 @@ k_window_err(const atq_line_fit_point *__restrict__ lf, const uint32_t *__restri
 // Per point: the filtered errors at positions j-1, j, j+1 (cyclic in the
 // cluster, 7-tap filter over 9 window errors) and a corner where the middle
 // one is a strict maximum. Window errors come from `errs` (k_window_err) or,
-// when it is null, are computed here.
+// when it is null, are computed here. Points below global index `first`
+// (1 as in the reference, 0 in fixed mode) are never corners.
 __global__ void __launch_bounds__(256)
 k_fit_lines(const atq_line_fit_point *__restrict__ lf, const uint32_t *__restrict__ cidx,
             const atq_cluster_extents *__restrict__ ext, const uint32_t *n_dev, uint32_t cap,
             const float *__restrict__ errs, atq_corner *__restrict__ sparse,
-            uint32_t *__restrict__ ccount) {
+            uint32_t *__restrict__ ccount, uint32_t first) {
     const uint32_t N = n_dev ? min(*n_dev, cap) : cap;
     for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N; x += gridDim.x * blockDim.x) {
         const uint32_t c = cidx[x];
@@ -182,7 +184,7 @@ k_fit_lines(const atq_line_fit_point *__restrict__ lf, const uint32_t *__restric
         const uint32_t s = e.start, n = e.count, j = x - s;
         const uint32_t k = min(20u, n / 12u);
         atq_corner out{0, 0, 0.0f};
-        if (x >= 1 && k >= 2) {
+        if (x >= first && k >= 2) {
             const atq_line_fit_point *p = lf + s;
             float er[9];
 #pragma unroll
@@ -310,7 +312,7 @@ k_do_indexing(const atq_peak_extents *__restrict__ peaks, const uint32_t *n_seg_
               const atq_corner *__restrict__ corners, const atq_line_fit_point *__restrict__ lf,
               const uint32_t *n_points_dev, uint32_t n_points_host,
               const atq_cluster_extents *__restrict__ ext, atq_fitted_quad *__restrict__ fitted,
-              uint32_t *oob) {
+              uint32_t *oob, uint32_t fx) {
     __shared__ float red_f[4];
     __shared__ uint32_t red_u[4];
     const uint32_t E = n_seg_dev ? min(*n_seg_dev, cap) : cap;
@@ -318,7 +320,8 @@ k_do_indexing(const atq_peak_extents *__restrict__ peaks, const uint32_t *n_seg_
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
     for (uint32_t e = blockIdx.x; e < E; e += gridDim.x) {
         const atq_peak_extents pk = peaks[e];
-        const atq_cluster_extents ce = ext[e];  // paired by position, as the reference does
+        // paired by position as the reference does, or (fixed mode) by cluster
+        const atq_cluster_extents ce = ext[fx ? corners[pk.start].cluster_index : e];
         const int64_t bound = (int64_t)NP - (int64_t)ce.start;
         const atq_line_fit_point *p = lf + ce.start;
         const uint32_t ncomb = binom(min(pk.count, 10u), 4);
@@ -467,10 +470,11 @@ int launch_zero_u32(hipStream_t s, uint32_t *p, const uint32_t *n_dev, uint32_t 
 
 int launch_fit_lines(hipStream_t s, const atq_line_fit_point *lf, const uint32_t *cidx,
                      const atq_cluster_extents *ext, const uint32_t *n_points_dev, uint32_t cap,
-                     atq_corner *sparse, uint32_t *ccount, float *errs) {
+                     atq_corner *sparse, uint32_t *ccount, float *errs, bool fx) {
     unsigned grid = min(4096u, atq_div_up(cap, 256) + 1);
     if (errs) k_window_err<<<grid, 256, 0, s>>>(lf, cidx, ext, n_points_dev, cap, errs);
-    k_fit_lines<<<grid, 256, 0, s>>>(lf, cidx, ext, n_points_dev, cap, errs, sparse, ccount);
+    k_fit_lines<<<grid, 256, 0, s>>>(lf, cidx, ext, n_points_dev, cap, errs, sparse, ccount,
+                                     fx ? 0u : 1u);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
@@ -497,9 +501,9 @@ int launch_do_indexing(hipStream_t s, const atq_peak_extents *peaks, const uint3
                        uint32_t cap, const atq_corner *corners, const atq_line_fit_point *lf,
                        const uint32_t *n_points_dev, uint32_t n_points_host,
                        const atq_cluster_extents *ext, atq_fitted_quad *fitted,
-                       uint32_t *oob_counter) {
+                       uint32_t *oob_counter, bool fx) {
     k_do_indexing<<<1024, 256, 0, s>>>(peaks, n_seg_dev, cap, corners, lf, n_points_dev,
-                                       n_points_host, ext, fitted, oob_counter);
+                                       n_points_host, ext, fitted, oob_counter, fx ? 1u : 0u);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 

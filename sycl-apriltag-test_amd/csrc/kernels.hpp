@@ -48,6 +48,9 @@ int launch_threshold_frames(hipStream_t s, const FrontFrame *frames, int n, cons
 int launch_ccl(hipStream_t s, const uint8_t *thr, size_t w, size_t h, uint32_t *parent,
                uint32_t *labels, uint32_t *sizes, uint32_t size_cap = 0xFFFFFFFFu);
 int launch_ccl_frames(hipStream_t s, const FrontFrame *frames, int n, const FrontShapes &sh);
+// Fixed-mode labelling (white 8-connected, black 4-connected, labels root + 1,
+// sizes of w * h + 1 entries with sizes[0] = 0).
+int launch_ccl_fixed_frames(hipStream_t s, const FrontFrame *frames, int n);
 
 // boundary.hip
 int launch_find_boundaries_sparse(hipStream_t s, const uint32_t *labels, const uint32_t *sizes,
@@ -69,13 +72,15 @@ struct PairTable {
 };
 // Records of the pairs that pass ValidBlobFilter, packed keys
 // ((min << lb | max) << 2 | plane), in the reference's order; tile_keys /
-// tile_points hold 4 * w * h entries, tile_scratch 2 * tiles + 2.
+// tile_points hold 4 * w * h entries, tile_scratch 2 * tiles + 2. fx: fixed
+// mode (column 0 local, both half-pixel bits decoded; the labels and sizes
+// must come from the fixed-mode CCL).
 int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const uint32_t *sizes,
                                     size_t w, size_t h, int min_size, uint32_t label_bits,
                                     const PairTable &table, atq_counts *counts,
                                     uint64_t *tile_keys, atq_boundary_point *tile_points,
                                     uint32_t *tile_scratch, uint64_t *keys,
-                                    atq_boundary_point *points, ScanCtx &sc);
+                                    atq_boundary_point *points, ScanCtx &sc, bool fx = false);
 constexpr unsigned BOUNDARY_TILE_PX = 1024;  // pixels per compaction tile
 
 // diff.hip

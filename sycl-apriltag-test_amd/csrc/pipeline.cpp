@@ -120,7 +120,7 @@ size_t carve_detector(atq_detector *d, char *base) {
     d->fil = carve_one<atq_extrema>(base, off, c.npx / 16 + 1);
     d->parent = carve_one<uint32_t>(base, off, c.npx);
     d->labels = carve_one<uint32_t>(base, off, c.npx);
-    d->sizes = carve_one<uint32_t>(base, off, c.npx);
+    d->sizes = carve_one<uint32_t>(base, off, c.npx + 1);  // fixed mode: root + 1
     d->keys = carve_one<uint64_t>(base, off, c.cap);
     d->vals = carve_one<uint32_t>(base, off, c.cap);
     d->bounds = carve_one<atq_cluster_bounds>(base, off, c.cap_cl);
@@ -222,11 +222,12 @@ int run_back(atq_detector *d, hipStream_t s, const uint8_t *gray_in) {
     // (pair, plane, y, x) order.
     const PairTable table{d->pair_keys, d->pair_cnt,    d->pair_bbox,
                           d->pair_grad, d->pair_moment, d->caps.pair_slots - 1};
+    const bool fx = d->params.mode == ATQ_MODE_FIXED;
     TRY(launch_find_boundaries_filtered(s, d->labels, d->sizes, w, h, d->params.min_component_size,
                                         (uint32_t)plan.label_bits, table, cnt, ws.keys_alt,
                                         reinterpret_cast<atq_boundary_point *>(ws.vals_alt),
                                         ws.tiles, d->keys,
-                                        reinterpret_cast<atq_boundary_point *>(d->vals), sc));
+                                        reinterpret_cast<atq_boundary_point *>(d->vals), sc, fx));
     TRY(mark(1 + ATQ_STAGE_BOUNDARY));
     bool alt = false;
     TRY(radix_sort_packed(s, plan, d->keys, d->vals, ws.keys_alt, ws.vals_alt, &cnt->n_boundary,
@@ -238,27 +239,27 @@ int run_back(atq_detector *d, hipStream_t s, const uint8_t *gray_in) {
         reinterpret_cast<const atq_boundary_point *>(alt ? ws.vals_alt : d->vals);
     TRY(launch_segments(s, K, 2, &cnt->n_boundary, cnt, c.cap, ws.seg_id, ws.seg_start, sc));
     TRY(launch_bounds_filter(s, V, ws.seg_start, cnt, c.cap, (uint32_t)w, (uint32_t)h, ws.valid,
-                             ws.seg_bounds, ws.seg_cluster, d->bounds, sc));
+                             ws.seg_bounds, ws.seg_cluster, d->bounds, sc, fx));
     TRY(launch_extents(s, d->bounds, cnt, c.cap_cl, d->cext, sc));
     TRY(launch_assign(s, V, ws.seg_id, ws.seg_start, ws.seg_cluster, d->bounds, d->cext,
-                      &cnt->n_boundary, c.cap, ws.cpts_tmp, d->cidx));
+                      &cnt->n_boundary, c.cap, ws.cpts_tmp, d->cidx, fx));
     TRY(mark(1 + ATQ_STAGE_CLUSTERS));
     // a valid cluster holds at most 2 * (2w + 2h) points (valid_blob)
     const uint32_t max_cluster = (uint32_t)std::min<size_t>(c.cap, 4 * (w + h));
     uint64_t *ak = ws.angle_keys();
     TRY(launch_angle_fit(s, ws.cpts_tmp, d->cpts, d->cext, cnt, c.cap_cl, max_cluster, grey,
                          (uint32_t)w, (uint32_t)h, d->lf, ak, ak + c.cap, ws.fit_sums,
-                         ws.long_segs(0, &d->long_n[0], c)));
+                         ws.long_segs(0, &d->long_n[0], c), fx));
     TRY(mark(1 + ATQ_STAGE_ANGLE_FIT));
     TRY(launch_fit_lines(s, d->lf, d->cidx, d->cext, &cnt->n_cluster_points, c.cap, d->sparse,
-                         nullptr, reinterpret_cast<float *>(ws.valid)));
+                         nullptr, reinterpret_cast<float *>(ws.valid), fx));
     TRY(mark(1 + ATQ_STAGE_FIT_LINES));
     TRY(launch_corner_segments(s, d->sparse, &cnt->n_cluster_points, c.cap, cnt, c.cap_cl,
                                max_cluster, d->peaks, d->corners, ws.angle_keys(),
                                ws.long_segs(1, &d->long_n[1], c), sc));
     TRY(mark(1 + ATQ_STAGE_CORNERS));
     TRY(launch_do_indexing(s, d->peaks, &cnt->n_segments, c.cap_cl, d->corners, d->lf,
-                           &cnt->n_cluster_points, 0, d->cext, d->fitted, &cnt->oob_reads));
+                           &cnt->n_cluster_points, 0, d->cext, d->fitted, &cnt->oob_reads, fx));
     TRY(mark(1 + ATQ_STAGE_INDEXING));
     TRY(launch_quads(s, d->fitted, &cnt->n_segments, c.cap_cl, ws.qtmp, ws.qflags, d->quads,
                      &cnt->n_quads, sc, d->params.min_tag_width));
@@ -274,7 +275,8 @@ int run_pipeline(atq_detector *d, hipStream_t s, const uint8_t *gray_in, size_t 
     const FrontFrame fr = front_frame(d, gray_in, w_in, h_in);
     TRY(launch_threshold_frames(s, &fr, 1, d->shapes));
     TRY(mark(d, s, 1 + ATQ_STAGE_THRESHOLD));
-    TRY(launch_ccl_frames(s, &fr, 1, d->shapes));
+    if (d->params.mode == ATQ_MODE_FIXED) TRY(launch_ccl_fixed_frames(s, &fr, 1));
+    else TRY(launch_ccl_frames(s, &fr, 1, d->shapes));
     TRY(mark(d, s, 1 + ATQ_STAGE_CCL));
     return run_back(d, s, gray_in);
 }
@@ -450,9 +452,10 @@ int atq_detector_create(size_t max_width_in, size_t max_height_in, const atq_par
                         atq_detector **out) {
     if (!out) return ATQ_ERR_BAD_ARG;
     *out = nullptr;
-    atq_params p{1, 5, 25, 8, 0};
+    atq_params p{1, 5, 25, 8, 0, ATQ_MODE_COMPAT};
     if (params) p = *params;
     if (p.decimate < 1) return ATQ_ERR_BAD_ARG;
+    if (p.mode != ATQ_MODE_COMPAT && p.mode != ATQ_MODE_FIXED) return ATQ_ERR_BAD_ARG;
     if (p.pair_table_log2 != 0 && (p.pair_table_log2 < 8 || p.pair_table_log2 > 24))
         return ATQ_ERR_BAD_ARG;
     const size_t w = decimated(max_width_in, p.decimate), h = decimated(max_height_in, p.decimate);
@@ -621,7 +624,7 @@ int atq_detector_buffer(atq_detector *d, int which, void **ptr, size_t *capacity
     case ATQ_BUF_GRAY: *ptr = d->gray; *capacity = c.npx; break;
     case ATQ_BUF_THRESHOLDED: *ptr = d->thr; *capacity = c.npx; break;
     case ATQ_BUF_LABELS: *ptr = d->labels; *capacity = c.npx; break;
-    case ATQ_BUF_SIZES: *ptr = d->sizes; *capacity = c.npx; break;
+    case ATQ_BUF_SIZES: *ptr = d->sizes; *capacity = c.npx + 1; break;
     case ATQ_BUF_BOUNDARY_POINTS: *ptr = alt ? (void *)d->ws.vals_alt : (void *)d->vals; *capacity = c.cap; break;
     case ATQ_BUF_BOUNDARY_KEYS: *ptr = alt ? (void *)d->ws.keys_alt : (void *)d->keys; *capacity = c.cap; break;
     case ATQ_BUF_CLUSTER_BOUNDS: *ptr = d->bounds; *capacity = c.cap_cl; break;
@@ -742,18 +745,21 @@ int batch_branches(atq_batch *b, hipStream_t s) {
         for (size_t i = 0; i < n; i++) {
             if (done[i] || b->stream_of[i] != k) continue;
             const FrontShapes sh = b->dets[i]->shapes;
+            const int mode = b->dets[i]->params.mode;
             FrontFrame group[FRONT_MAX_FRAMES];
             int g = 0;
             for (size_t j = i; j < n && g < FRONT_MAX_FRAMES; j++) {
                 const FrontShapes &o = b->dets[j]->shapes;
                 if (done[j] || b->stream_of[j] != k || o.thr_rx != sh.thr_rx ||
-                    o.thr_ry != sh.thr_ry || o.ccl_rows != sh.ccl_rows)
+                    o.thr_ry != sh.thr_ry || o.ccl_rows != sh.ccl_rows ||
+                    b->dets[j]->params.mode != mode)
                     continue;
                 group[g++] = front_frame(b->dets[j], b->inputs[j], b->w_in[j], b->h_in[j]);
                 done[j] = true;
             }
             TRY(launch_threshold_frames(sk, group, g, sh));
-            TRY(launch_ccl_frames(sk, group, g, sh));
+            if (mode == ATQ_MODE_FIXED) TRY(launch_ccl_fixed_frames(sk, group, g));
+            else TRY(launch_ccl_frames(sk, group, g, sh));
         }
         for (size_t i = 0; i < n; i++)
             if (b->stream_of[i] == k) TRY(run_back(b->dets[i], sk, b->inputs[i]));

@@ -20,18 +20,19 @@ static_assert(sizeof(atq_shapes) == 12, "atq_shapes");
 namespace atq {
 
 // include/boundary_point.h:53-54, 63-95
-ATQ_DEV uint32_t x_with_dx(uint16_t px) {
-    return (uint16_t)(2 * (px & COORD_VALUE_MASK) + half_pixel_dx(px));
+// (fx: fixed mode, both half-pixel bits decoded)
+ATQ_DEV uint32_t x_with_dx(uint16_t px, bool fx = false) {
+    return (uint16_t)(2 * (px & COORD_VALUE_MASK) + half_pixel_dx(px, fx));
 }
-ATQ_DEV uint32_t y_with_dy(uint16_t px, uint16_t py) {
-    return (uint16_t)(2 * (py & COORD_VALUE_MASK) + half_pixel_dy(px));
+ATQ_DEV uint32_t y_with_dy(uint16_t px, uint16_t py, bool fx = false) {
+    return (uint16_t)(2 * (py & COORD_VALUE_MASK) + half_pixel_dy(px, fx));
 }
 ATQ_DEV bool black_to_white(uint16_t py) { return (py & COLOR_DIRECTION_MASK) != 0; }
-ATQ_DEV int grad_x(uint16_t px, uint16_t py) {
-    return black_to_white(py) ? half_pixel_dx(px) : -half_pixel_dx(px);
+ATQ_DEV int grad_x(uint16_t px, uint16_t py, bool fx = false) {
+    return black_to_white(py) ? half_pixel_dx(px, fx) : -half_pixel_dx(px, fx);
 }
-ATQ_DEV int grad_y(uint16_t px, uint16_t py) {
-    return black_to_white(py) ? half_pixel_dy(px) : -half_pixel_dy(px);
+ATQ_DEV int grad_y(uint16_t px, uint16_t py, bool fx = false) {
+    return black_to_white(py) ? half_pixel_dy(px, fx) : -half_pixel_dy(px, fx);
 }
 
 // include/cluster_bounds.h:47-49 (float*0.5f, double add, back to float)
@@ -64,9 +65,9 @@ ATQ_DEV bool valid_blob(const atq_cluster_bounds &b, uint32_t w, uint32_t h) {
 }
 
 // include/boundary_point.h:48-61
-ATQ_DEV float calc_theta(uint16_t px, uint16_t py, float cx, float cy) {
-    float dx = (float)x_with_dx(px) - cx;
-    float dy = (float)y_with_dy(px, py) - cy;
+ATQ_DEV float calc_theta(uint16_t px, uint16_t py, float cx, float cy, bool fx = false) {
+    float dx = (float)x_with_dx(px, fx) - cx;
+    float dy = (float)y_with_dy(px, py, fx) - cy;
     bool a = dx > 0, b = dy > 0;
     float q = a ? (b ? 65536.0f : 0.0f) : (b ? 131072.0f : -65536.0f);
     if (a == b) return q + dy / dx;
@@ -74,15 +75,15 @@ ATQ_DEV float calc_theta(uint16_t px, uint16_t py, float cx, float cy) {
 }
 
 // include/line_fit_point.h:50-86, including the row index taken from rx
-// (`uint32_t y = rx / 2`): the weight W and the doubled coordinates + 1 that
-// every moment of the point is built from.
+// (`uint32_t y = rx / 2`; fixed mode: ry / 2): the weight W and the doubled
+// coordinates + 1 that every moment of the point is built from.
 ATQ_DEV void linefit_terms(uint16_t px, uint16_t py, uint32_t w, uint32_t h,
                            const uint8_t *__restrict__ grey, uint32_t *W_out, uint32_t *rx_out,
-                           uint32_t *ry_out) {
-    const uint32_t rx = x_with_dx(px) + 1;
-    const uint32_t ry = y_with_dy(px, py) + 1;
+                           uint32_t *ry_out, bool fx = false) {
+    const uint32_t rx = x_with_dx(px, fx) + 1;
+    const uint32_t ry = y_with_dy(px, py, fx) + 1;
     const uint32_t x = rx / 2;
-    const uint32_t y = rx / 2;
+    const uint32_t y = (fx ? ry : rx) / 2;
     uint32_t W = 1;
     if (x > 0 && x + 1 < w && y > 0 && y + 1 < h) {
         int gx = (int)grey[(size_t)y * w + x + 1] - (int)grey[(size_t)y * w + x - 1];
@@ -97,24 +98,23 @@ ATQ_DEV void linefit_terms(uint16_t px, uint16_t py, uint32_t w, uint32_t h,
 // linefit_terms in two steps: the four grey pixels it may read (0 where it
 // would not read), then the terms from them.
 ATQ_DEV void linefit_gather(uint16_t px, uint16_t py, uint32_t w, uint32_t h,
-                            const uint8_t *__restrict__ grey, uint32_t (&g)[4]) {
-    const uint32_t rx = x_with_dx(px) + 1;
-    const uint32_t x = rx / 2, y = rx / 2;
+                            const uint8_t *__restrict__ grey, uint32_t (&g)[4], bool fx = false) {
+    const uint32_t rx = x_with_dx(px, fx) + 1;
+    const uint32_t x = rx / 2, y = (fx ? y_with_dy(px, py, fx) + 1 : rx) / 2;
     const bool in = x > 0 && x + 1 < w && y > 0 && y + 1 < h;
     const size_t c = in ? (size_t)y * w + x : (size_t)w + 1;  // any in-image pixel when unused
     g[0] = in ? grey[c + 1] : 0u;
     g[1] = in ? grey[c - 1] : 0u;
     g[2] = in ? grey[c + w] : 0u;
     g[3] = in ? grey[c - w] : 0u;
-    (void)py;
 }
 ATQ_DEV void linefit_terms_from(uint16_t px, uint16_t py, uint32_t w, uint32_t h,
                                 const uint32_t (&g)[4], uint32_t *W_out, uint32_t *rx_out,
-                                uint32_t *ry_out) {
-    const uint32_t rx = x_with_dx(px) + 1;
-    const uint32_t ry = y_with_dy(px, py) + 1;
+                                uint32_t *ry_out, bool fx = false) {
+    const uint32_t rx = x_with_dx(px, fx) + 1;
+    const uint32_t ry = y_with_dy(px, py, fx) + 1;
     const uint32_t x = rx / 2;
-    const uint32_t y = rx / 2;
+    const uint32_t y = (fx ? ry : rx) / 2;
     uint32_t W = 1;
     if (x > 0 && x + 1 < w && y > 0 && y + 1 < h) {
         const int gx = (int)g[0] - (int)g[1];

@@ -79,7 +79,7 @@ class AtqError(RuntimeError):
 class AtqParams(C.Structure):
     _fields_ = [("decimate", C.c_int32), ("min_white_black_diff", C.c_int32),
                 ("min_component_size", C.c_int32), ("min_tag_width", C.c_int32),
-                ("pair_table_log2", C.c_int32)]
+                ("pair_table_log2", C.c_int32), ("mode", C.c_int32)]
 
 
 class AtqShapes(C.Structure):
@@ -383,8 +383,9 @@ def find_boundaries(labels_dev, sizes_dev, w, h):
 class Detector:
     """Owns the per-frame buffers (allocated once) of the whole pipeline."""
 
-    def __init__(self, max_w_in, max_h_in, decimate=1, pair_table_log2=0):
-        self.params = AtqParams(decimate, 5, 25, 8, pair_table_log2)
+    def __init__(self, max_w_in, max_h_in, decimate=1, pair_table_log2=0, fixed=False):
+        """fixed=True: the detector's fixed mode (ATQ_MODE_FIXED)."""
+        self.params = AtqParams(decimate, 5, 25, 8, pair_table_log2, 1 if fixed else 0)
         h = C.c_void_p()
         init()
         check(lib().atq_detector_create(max_w_in, max_h_in, C.byref(self.params), C.byref(h)))
