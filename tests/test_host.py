@@ -85,3 +85,27 @@ def test_product_path_does_not_touch_oracle():
             if f.endswith((".py", ".hip", ".cpp", ".hpp")):
                 txt = open(os.path.join(dp, f), errors="ignore").read()
                 assert "pyoracle" not in txt and "liboracle" not in txt, f
+
+
+def test_params_layout_has_mode():
+    import ctypes as C
+    import pyatq
+    assert C.sizeof(pyatq.AtqParams) == 6 * 4
+    assert [n for n, _ in pyatq.AtqParams._fields_][-1] == "mode"
+
+
+def test_mode_report_corner_matching():
+    """The report's matching: a quad equal to a tag up to corner order and
+    direction scores 0; decimated corners map back by (c - 0.5) * f + 0.5."""
+    from pyatq import records as R
+    from pyatq.modes import corner_errors
+    truth = [np.array([[10.0, 10.0], [50.0, 12.0], [48.0, 52.0], [9.0, 49.0]]),
+             np.array([[200.0, 200.0], [240.0, 200.0], [240.0, 240.0], [200.0, 240.0]])]
+    q = np.zeros(2, R.QUAD_CORNERS)
+    q["corners"][0] = truth[0][[2, 1, 0, 3]]          # reversed, rotated
+    q["corners"][1] = (truth[1] - 0.5) / 2 + 0.5 + 0.25  # decimated by 2, 0.5 px off in x and y
+    e = corner_errors(q, truth, f=1)
+    assert e[0] == 0.0
+    e2 = corner_errors(q[1:], truth, f=2)
+    np.testing.assert_allclose(e2[1], np.sqrt(0.5))
+    assert np.isinf(corner_errors(q[:0], truth)).all()
