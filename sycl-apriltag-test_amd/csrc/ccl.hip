@@ -9,7 +9,7 @@
 // links the block's colour-c node to the colour-c node of the block holding
 // it. Components are labelled by their minimum node id (min-linking).
 //
-// gfx950 organisation (four kernels, the reference has five; each takes a
+// gfx950 organisation (three kernels, the reference has five; each takes a
 // table of frames and walks them with a persistent grid):
 //  1. k_ccl_local: a work-group owns a TBX x TBY tile of blocks (TBY = 8 or
 //     16, chosen per resolution by the autotuner), stages its
@@ -21,11 +21,10 @@
 //     roots its border edges join and merges each once on the global parent
 //     array with the reference's lock-free atomicMin union
 //     (include/union_find.h:50-77), halving paths as it walks.
-//  3. k_ccl_flatten: every local root is pointed straight at its final
-//     root and adds its count into the root's size.
-//  4. k_ccl_final: each block reads its two roots (at most two loads) and
-//     writes the four label words (white = 0x80000000|root, black = root,
-//     127 -> 0).
+//  3. k_ccl_resolve: each block finds the final roots of its two nodes
+//     (path halving), a merged local root adds its pixel count into its
+//     root's size, and the block writes its four label words (white =
+//     0x80000000|root, black = root, 127 -> 0).
 #include <algorithm>
 
 #include "kernels.hpp"
@@ -451,62 +450,52 @@ k_ccl_border(FrontTable tab) {
     }
 }
 
-// After the border merge: every local root gets its final root (path
-// halving, then the root itself), and adds its pixel count into the root's
-// size. Once the root's size reaches size_cap further adds are skipped (the
-// detector only asks whether a size reaches min_component_size; the drop-in
-// passes UINT32_MAX for exact sizes).
-// Work items of flatten / final: one 256-block run of a block row.
+// After the border merge, one thread per block: the final root of each of
+// its two nodes (a non-root node's word names its local root; a local root's
+// word is itself or, if the border pass merged it, a node above it) and the
+// four label words (white = 0x80000000 | root, black = root, 127 -> 0). A
+// merged local root halves its path on the way and adds its pixel count into
+// its final root's size; non-root nodes only read, since every block of a
+// component walks the same path. Once a size reaches size_cap further adds
+// are skipped (the detector only asks whether a size reaches
+// min_component_size; the drop-in passes UINT32_MAX for exact sizes).
+// Work items: one 256-block run of a block row.
 ATQ_DEV uint32_t block_runs_x(uint32_t w) { return ((w / 2) + 255) / 256; }
 
 __global__ void __launch_bounds__(256)
-k_ccl_flatten(FrontTable tab) {
+k_ccl_resolve(FrontTable tab) {
     for (uint32_t fi = 0; fi < tab.n; fi++) {
         const FrontFrame &F = tab.fr[fi];
         uint32_t *__restrict__ parent = F.parent;
         uint32_t *__restrict__ sizes = F.sizes;
+        const uint8_t *__restrict__ thr = F.thr;
+        uint32_t *__restrict__ labels = F.labels;
         const uint32_t w = F.w, h = F.h, runs_x = block_runs_x(w), size_cap = F.size_cap;
         for (uint32_t it = blockIdx.x; it < runs_x * (h / 2); it += gridDim.x) {
             const uint32_t bx = (it % runs_x) * 256 + threadIdx.x, by = it / runs_x;
             if (2 * bx >= w) continue;
             const uint32_t id = 2 * by * w + 2 * bx;
-#pragma unroll
-            for (uint32_t c = 0; c < 2; c++) {
-                const uint32_t n = id + c;
-                const uint32_t v = parent[n];
-                if ((v & FLAG) || v == n) continue;  // not a local root, or a final root
-                const uint32_t r = g_find_halving(parent, v);
-                atomicMin(&parent[n], r);
-                const uint32_t sz = sizes[n];
-                if (sz && g_load(&sizes[r]) < size_cap) atomicAdd(&sizes[r], sz);
-            }
-        }
-    }
-}
-
-// Labels: white = 0x80000000 | root, black = root, 127 -> 0. A node's word
-// is its final root (a local root, flattened) or FLAG | its local root.
-__global__ void __launch_bounds__(256)
-k_ccl_final(FrontTable tab) {
-    for (uint32_t fi = 0; fi < tab.n; fi++) {
-        const FrontFrame &F = tab.fr[fi];
-        const uint32_t *__restrict__ parent = F.parent;
-        const uint8_t *__restrict__ thr = F.thr;
-        uint32_t *__restrict__ labels = F.labels;
-        const uint32_t w = F.w, h = F.h, runs_x = block_runs_x(w);
-        for (uint32_t it = blockIdx.x; it < runs_x * (h / 2); it += gridDim.x) {
-            const uint32_t bx = (it % runs_x) * 256 + threadIdx.x, by = it / runs_x;
-            if (2 * bx >= w) continue;
-            const uint32_t id = 2 * by * w + 2 * bx;
             const uint2 v = *reinterpret_cast<const uint2 *>(parent + id);
-            const uint32_t r0 = (v.x & FLAG) ? parent[v.x & ~FLAG] : v.x;
-            const uint32_t r1 = (v.y & FLAG) ? parent[v.y & ~FLAG] : v.y;
             const uint16_t top = *reinterpret_cast<const uint16_t *>(thr + id);
             const uint16_t bot = *reinterpret_cast<const uint16_t *>(thr + id + w);
+            uint32_t r[2];
+#pragma unroll
+            for (uint32_t c = 0; c < 2; c++) {
+                const uint32_t n = id + c, word = c ? v.y : v.x;
+                if (word & FLAG) {
+                    r[c] = g_find(parent, word & ~FLAG);  // read-only: many blocks share the path
+                } else if (word == n) {
+                    r[c] = n;
+                } else {
+                    r[c] = g_find_halving(parent, word);
+                    const uint32_t sz = sizes[n];
+                    if (sz && g_load(&sizes[r[c]]) < size_cap) atomicAdd(&sizes[r[c]], sz);
+                }
+            }
             auto lab = [&](uint32_t p) -> uint32_t {
                 p &= 0xFFu;
-                if (p == C_WHITE) return LABEL_PIXEL_MASK | r0;
-                if (p == C_BLACK) return r1;
+                if (p == C_WHITE) return LABEL_PIXEL_MASK | r[0];
+                if (p == C_BLACK) return r[1];
                 return 0u;
             };
             *reinterpret_cast<uint2 *>(labels + id) = make_uint2(lab(top), lab(top >> 8));
@@ -532,8 +521,7 @@ void ccl_launch(hipStream_t s, const FrontTable &tab) {
     const uint32_t fg = std::min<uint32_t>(runs, 16384u);
     k_ccl_local<TBY><<<lg ? lg : 8, 256, 0, s>>>(tab);
     k_ccl_border<TBY><<<bg ? bg : 1, 256, 0, s>>>(tab);
-    k_ccl_flatten<<<fg ? fg : 1, 256, 0, s>>>(tab);
-    k_ccl_final<<<fg ? fg : 1, 256, 0, s>>>(tab);
+    k_ccl_resolve<<<fg ? fg : 1, 256, 0, s>>>(tab);
 }
 }  // namespace
 
@@ -570,7 +558,7 @@ int launch_ccl(hipStream_t s, const uint8_t *thr, size_t w, size_t h, uint32_t *
 // (saturating near size_cap like the compat form), sizes[0] = 0, so label 0
 // (127 pixels, out-of-image neighbours) never reads as a large component.
 //
-// The same four steps as the compat form:
+// Four steps (the compat form's first two; roots and labels apart):
 //  1. k_fx_local: a work-group owns a 64 x 16 pixel tile, each wave four of
 //     its rows, one pixel per lane. One ballot per row gives every pixel the
 //     first pixel of its same-colour run (the horizontal links); the links to
@@ -583,7 +571,9 @@ int launch_ccl(hipStream_t s, const uint8_t *thr, size_t w, size_t h, uint32_t *
 //     so a tile owns those of its top row (up, up-left, up-right), its left
 //     column (left, up-left) and its right column (up-right) - as distinct
 //     pairs of local roots, merged on the global parent array.
-//  3. k_fx_flatten, 4. k_fx_final: as k_ccl_flatten / k_ccl_final per pixel.
+//  3. k_fx_flatten: each merged local root finds its final root (path
+//     halving), points at it and adds its count into sizes[root + 1].
+//  4. k_fx_final: the label words, one load per non-root pixel.
 namespace {
 constexpr int FX_TX = 64, FX_TY = 16;  // pixel tile: one row per wave lane set
 constexpr int FX_PX = FX_TX * FX_TY;
