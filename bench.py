@@ -212,6 +212,7 @@ def main():
     ap.add_argument("--skip-12mp", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-f2", action="store_true", help="no decimate-2 leg")
+    ap.add_argument("--skip-fixed", action="store_true", help="no fixed-mode leg")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="first copy this many MiB device to device (a counter reference "
                          "for profiler runs)")
@@ -309,6 +310,24 @@ def main():
         extra["mpix_per_s_decimate2"] = round(steps * W1080 * H1080 / res[2]["wall"] / 1e6, 2)
         extra["frames_per_s_decimate2"] = round(steps / res[2]["wall"], 2)
         extra["stage_ms_1080p_f2"] = {k: round(v, 5) for k, v in res[2]["stages"].items()}
+
+    if not args.skip_fixed:
+        # the detector's fixed mode (reference behaviours corrected) on the
+        # same frames, for comparison; not the headline
+        det = atq.Detector(W1080, H1080, decimate=1, fixed=True)
+        tuned_shapes(det, devs[0], W1080, H1080, stream, not args.no_tune)
+        wall, _ = time_frames(atq, det, devs, W1080, H1080, args.steps, args.warmup, stream)
+        stages = stage_profile(atq, det, devs, W1080, H1080, 20, stream)
+        det.detect(devs[0], W1080, H1080, stream.handle)
+        stream.sync()
+        c = det.counts()
+        det.close()
+        extra["fixed_mode_1080p_f1"] = {
+            "mpix_per_s": round(steps * W1080 * H1080 / wall / 1e6, 2),
+            "ms_per_frame": round(wall / steps * 1e3, 4),
+            "stage_ms": {k: round(v, 5) for k, v in stages.items()},
+            "n_clusters": int(c["n_clusters"]), "n_quads": int(c["n_quads"])}
+        log(f"fixed mode: {extra['fixed_mode_1080p_f1']['ms_per_frame']} ms/frame")
 
     if not args.skip_12mp:
         extra["stress_12mp"] = bench_12mp(atq, synth, stream, tune=not args.no_tune)
