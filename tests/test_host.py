@@ -109,3 +109,14 @@ def test_mode_report_corner_matching():
     e2 = corner_errors(q[1:], truth, f=2)
     np.testing.assert_allclose(e2[1], np.sqrt(0.5))
     assert np.isinf(corner_errors(q[:0], truth)).all()
+
+
+def test_detector_rejects_unknown_mode():
+    """atq_params.mode is checked before any device work."""
+    import ctypes as C
+    import pyatq
+    for mode in (2, -1):
+        p = pyatq.AtqParams(1, 5, 25, 8, 0, mode)
+        h = C.c_void_p()
+        rc = pyatq.lib().atq_detector_create(640, 480, C.byref(p), C.byref(h))
+        assert rc == 4 and not h.value, (mode, rc)
