@@ -371,9 +371,10 @@ enum {
                                 undecimated grey frame (read in place) */
     ATQ_BUF_THRESHOLDED = 1, /* u8  W*H                                  */
     ATQ_BUF_LABELS = 2,      /* u32 W*H                                  */
-    ATQ_BUF_SIZES = 3,       /* u32 W*H; at a root its component's pixel
-                                count, exact below min_component_size and
-                                only ">= min_component_size" above       */
+    ATQ_BUF_SIZES = 3,       /* u32 W*H + 1; at a root its component's
+                                pixel count (fixed mode: at root + 1), exact
+                                below min_component_size and only
+                                ">= min_component_size" above            */
     ATQ_BUF_BOUNDARY_POINTS = 4, /* atq_boundary_point, grouped by key   */
     ATQ_BUF_BOUNDARY_KEYS = 5,   /* u64 packed min<<lb|max, lb = counts.key_label_bits */
     ATQ_BUF_CLUSTER_BOUNDS = 6,  /* filtered atq_cluster_bounds; start

@@ -247,6 +247,27 @@ struct PairPart {
     }
 };
 
+// The gradient sums of a pair as one 64-bit word: gy * 2^32 + gx. Both sums
+// stay far inside int32 (at most 2 * (2w + 2h) records of +-1 each), so the
+// 64-bit sum of such words is their exact combination and splits back.
+ATQ_DEV unsigned long long grad_pack(int32_t gx, int32_t gy) {
+    return (unsigned long long)((int64_t)gy * 4294967296ll + (int64_t)gx);
+}
+ATQ_DEV void grad_unpack(unsigned long long v, int32_t *gx, int32_t *gy) {
+    *gx = (int32_t)(uint32_t)v;
+    *gy = (int32_t)(((int64_t)v - (int64_t)*gx) >> 32);
+}
+
+// The bounds part of an add (the pair's count already added).
+ATQ_DEV void global_bounds(const PairTable &t, uint32_t h, const PairPart &q) {
+    atomicMax(&t.bbox[4 * (size_t)h + 0], q.nxmin);
+    atomicMax(&t.bbox[4 * (size_t)h + 1], q.xmax);
+    atomicMax(&t.bbox[4 * (size_t)h + 2], q.nymin);
+    atomicMax(&t.bbox[4 * (size_t)h + 3], q.ymax);
+    atomicAdd(&t.grad[h], grad_pack(q.gx, q.gy));
+    atomicAdd(&t.moment[h], (unsigned long long)q.pg);
+}
+
 ATQ_DEV void global_add(const PairTable &t, uint64_t key, const PairPart &q, uint32_t max_count,
                         atq_counts *counts) {
     const uint32_t h = global_slot(t, key, true);
@@ -258,13 +279,59 @@ ATQ_DEV void global_add(const PairTable &t, uint64_t key, const PairPart &q, uin
     // a pair whose count passes the limit cannot pass the filter and needs
     // no bounds; a pair that stays within it sees every add below the limit
     if (atomicAdd(&t.cnt[h], q.c) + q.c > max_count) return;
-    atomicMax(&t.bbox[4 * (size_t)h + 0], q.nxmin);
-    atomicMax(&t.bbox[4 * (size_t)h + 1], q.xmax);
-    atomicMax(&t.bbox[4 * (size_t)h + 2], q.nymin);
-    atomicMax(&t.bbox[4 * (size_t)h + 3], q.ymax);
-    atomicAdd(&t.grad[2 * (size_t)h + 0], q.gx);
-    atomicAdd(&t.grad[2 * (size_t)h + 1], q.gy);
-    atomicAdd(&t.moment[h], (unsigned long long)q.pg);
+    global_bounds(t, h, q);
+}
+
+// global_add for the two pairs of the LDS table slots threadIdx.x and
+// threadIdx.x + 256, their probes, counts and bounds in lockstep so that the
+// two chains of dependent atomics overlap.
+ATQ_DEV void global_add2(const PairTable &t, const uint64_t (&key)[2], const PairPart (&q)[2],
+                         const bool (&live)[2], uint32_t max_count, atq_counts *counts) {
+    uint32_t h[2], slot[2] = {NO_SLOT, NO_SLOT};
+    bool done[2];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        h[j] = pair_hash(key[j]) & t.mask;
+        done[j] = !live[j];
+    }
+    for (int p = 0; p < PAIR_PROBES && !(done[0] && done[1]); p++) {
+        uint64_t k[2];
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+            k[j] = done[j] ? 0ull
+                           : __hip_atomic_load(&t.keys[h[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            if (done[j]) continue;
+            const uint64_t nk = ~key[j];
+            bool found = k[j] == nk;
+            if (!found && k[j] == 0) {
+                const uint64_t old = atomicCAS((unsigned long long *)&t.keys[h[j]], 0ull,
+                                               (unsigned long long)nk);
+                found = old == 0 || old == nk;
+            }
+            if (found) {
+                slot[j] = h[j];
+                done[j] = true;
+            } else {
+                h[j] = (h[j] + 1) & t.mask;
+            }
+        }
+    }
+    bool add[2];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        if (live[j] && slot[j] == NO_SLOT) counts->pair_filter_off = 1u;
+        add[j] = slot[j] != NO_SLOT &&
+                 __hip_atomic_load(&t.cnt[slot[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= max_count;
+    }
+    uint32_t old[2] = {0, 0};
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+        if (add[j]) old[j] = atomicAdd(&t.cnt[slot[j]], q[j].c);
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+        if (add[j] && old[j] + q[j].c <= max_count) global_bounds(t, slot[j], q[j]);
 }
 
 // Walks a thread's records (4 pixels, planes in order) with a one-entry
@@ -317,12 +384,11 @@ k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
              atq_counts *counts) {
     __shared__ uint64_t lk[TILE_SLOTS];
     __shared__ uint32_t lc[TILE_SLOTS], lb[4][TILE_SLOTS];
-    __shared__ int32_t lg[2][TILE_SLOTS];
-    __shared__ unsigned long long lm[TILE_SLOTS];
+    __shared__ unsigned long long lg[TILE_SLOTS], lm[TILE_SLOTS];  // packed gradient, moment
     for (int i = threadIdx.x; i < TILE_SLOTS; i += 256) {
         lk[i] = 0;
         lc[i] = lb[0][i] = lb[1][i] = lb[2][i] = lb[3][i] = 0;
-        lg[0][i] = lg[1][i] = 0;
+        lg[i] = 0;
         lm[i] = 0;
     }
     __syncthreads();
@@ -369,25 +435,29 @@ k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
             atomicMax(&lb[1][sl], q.xmax);
             atomicMax(&lb[2][sl], q.nymin);
             atomicMax(&lb[3][sl], q.ymax);
-            atomicAdd(&lg[0][sl], q.gx);
-            atomicAdd(&lg[1][sl], q.gy);
+            atomicAdd(&lg[sl], grad_pack(q.gx, q.gy));
             atomicAdd(&lm[sl], (unsigned long long)q.pg);
         }, FX);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < TILE_SLOTS; i += 256)
-        if (lk[i]) {
-            PairPart q;
-            q.c = lc[i];
-            q.nxmin = lb[0][i];
-            q.xmax = lb[1][i];
-            q.nymin = lb[2][i];
-            q.ymax = lb[3][i];
-            q.gx = lg[0][i];
-            q.gy = lg[1][i];
-            q.pg = (int64_t)lm[i];
-            global_add(table, ~lk[i], q, max_count, counts);
-        }
+    static_assert(TILE_SLOTS == 2 * 256, "two table slots per thread");
+    uint64_t key[2];
+    PairPart q[2];
+    bool live[2];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int i = threadIdx.x + 256 * j;
+        live[j] = lk[i] != 0;
+        key[j] = ~lk[i];
+        q[j].c = lc[i];
+        q[j].nxmin = lb[0][i];
+        q[j].xmax = lb[1][i];
+        q[j].nymin = lb[2][i];
+        q[j].ymax = lb[3][i];
+        grad_unpack(lg[i], &q[j].gx, &q[j].gy);
+        q[j].pg = (int64_t)lm[i];
+    }
+    global_add2(table, key, q, live, max_count, counts);
 }
 
 template <bool FX>
@@ -419,8 +489,7 @@ k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
         b.x_max = (uint16_t)table.bbox[4 * (size_t)g + 1];
         b.y_min = (uint16_t)(0xFFFFu - table.bbox[4 * (size_t)g + 2]);
         b.y_max = (uint16_t)table.bbox[4 * (size_t)g + 3];
-        b.gx_sum = table.grad[2 * (size_t)g + 0];
-        b.gy_sum = table.grad[2 * (size_t)g + 1];
+        grad_unpack(table.grad[g], &b.gx_sum, &b.gy_sum);
         b.pxgx_pygy_sum = (int64_t)table.moment[g];
         b.start = 0;
         b.count = c;

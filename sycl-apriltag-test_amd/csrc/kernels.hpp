@@ -66,7 +66,7 @@ struct PairTable {
     uint64_t *keys;
     uint32_t *cnt;
     uint32_t *bbox;  // 4 per slot
-    int32_t *grad;   // 2 per slot
+    unsigned long long *grad;  // gy * 2^32 + gx per slot
     unsigned long long *moment;
     uint32_t mask;
 };

@@ -137,7 +137,7 @@ size_t carve_detector(atq_detector *d, char *base) {
     d->pair_keys = carve_one<uint64_t>(base, off, c.pair_slots);
     d->pair_cnt = carve_one<uint32_t>(base, off, c.pair_slots);
     d->pair_bbox = carve_one<uint32_t>(base, off, 4 * (size_t)c.pair_slots);
-    d->pair_grad = carve_one<int32_t>(base, off, 2 * (size_t)c.pair_slots);
+    d->pair_grad = carve_one<unsigned long long>(base, off, (size_t)c.pair_slots);
     d->pair_moment = carve_one<unsigned long long>(base, off, c.pair_slots);
     d->long_n = carve_one<uint32_t>(base, off, 2);
     d->counts = carve_one<atq_counts>(base, off, 1);

@@ -80,7 +80,7 @@ struct atq_detector {
     uint64_t *scan_flags;  // two look-back flag arrays, zeroed before each frame
     uint64_t *pair_keys;   // pair table, zeroed with counts before each frame
     uint32_t *pair_cnt, *pair_bbox;
-    int32_t *pair_grad;
+    unsigned long long *pair_grad;
     unsigned long long *pair_moment;
     uint32_t *long_n;      // long-segment counts of the two segment sorts, zeroed likewise
     atq_counts *counts;
