@@ -159,8 +159,8 @@ k_boundaries_sparse(const uint32_t *__restrict__ labels, const uint32_t *__restr
 // gradient and moment sums), which does not depend on the records' order, so
 // the pairs that fail it are found, and their records dropped, before the
 // key sort:
-//  1. k_pair_count: tiles of 1024 consecutive pixels (4 per thread) reduce
-//     their records per pair in an LDS table, then add each pair's partial
+//  1. k_pair_count: regions of 64 x 16 pixels (4 per thread) reduce their
+//     records per pair in an LDS table, then add each pair's partial
 //     reduction into a global open-addressing table (key min<<32|max). A
 //     pair whose count is already above the upper limit 2 * (2w + 2h) gets
 //     no more adds (the pairs that span the frame would otherwise queue on
@@ -176,6 +176,8 @@ k_boundaries_sparse(const uint32_t *__restrict__ labels, const uint32_t *__restr
 // every record is kept.
 constexpr int BND_PX_PER_THREAD = 4;
 constexpr int BND_TILE = 256 * BND_PX_PER_THREAD;
+constexpr uint32_t PC_RX = 64, PC_RY = 16;  // k_pair_count's regions (256 x 4 pixels)
+static_assert(PC_RX * PC_RY == BND_TILE, "one pixel quad per thread");
 constexpr int TILE_SLOTS = 512;   // LDS pair table of a tile
 constexpr int PAIR_PROBES = 64;   // global table probe limit
 constexpr uint32_t NO_SLOT = 0xFFFFFFFFu;
@@ -373,14 +375,14 @@ ATQ_DEV void tile_records(const uint32_t *labels, const uint32_t *sizes, uint32_
     for (int k = 0; k < BND_PX_PER_THREAD; k++) quad_pixel_records(q, x0, y, k, 32, r[k], fx);
 }
 
-// Each work-group takes a contiguous run of tiles and merges all their
-// records in its LDS table before touching the global one, so a pair that
-// spans the frame costs one global add per work-group, not per tile.
+// Each work-group takes a contiguous run of regions (a vertical strip) and
+// merges all their records in its LDS table before touching the global one,
+// so a pair costs one global add per work-group it meets, not per region.
 // FX: fixed mode (record decode and column 0).
 template <bool FX>
 __global__ void __launch_bounds__(256)
 k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes, uint32_t w,
-             uint32_t h, uint32_t min_size, uint32_t tiles, PairTable table, uint32_t max_count,
+             uint32_t h, uint32_t min_size, PairTable table, uint32_t max_count,
              atq_counts *counts) {
     __shared__ uint64_t lk[TILE_SLOTS];
     __shared__ uint32_t lc[TILE_SLOTS], lb[4][TILE_SLOTS];
@@ -392,38 +394,46 @@ k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
         lm[i] = 0;
     }
     __syncthreads();
-    const uint32_t per = (tiles + gridDim.x - 1) / gridDim.x;
-    const uint32_t t_end = min(tiles, (blockIdx.x + 1) * per);
-    // software-pipelined over the tiles: the next tile's labels load while
-    // this tile's sizes are in flight
-    const size_t n = (size_t)w * h;
-    auto pixel0 = [&](uint32_t tile) {
-        return (size_t)tile * BND_TILE + threadIdx.x * BND_PX_PER_THREAD;
+    // regions of PC_RX x PC_RY pixels (16 threads per row, 4 pixels each),
+    // numbered down the columns of regions, so that a work-group's run of
+    // regions is a compact vertical strip: a pair then spans few work-groups
+    // and reaches the global table from few of them
+    const uint32_t ry_n = (h + PC_RY - 1) / PC_RY;
+    const uint32_t regions = (w + PC_RX - 1) / PC_RX * ry_n;
+    const uint32_t per = (regions + gridDim.x - 1) / gridDim.x;
+    const uint32_t t_end = min(regions, (blockIdx.x + 1) * per);
+    auto origin = [&](uint32_t reg, uint32_t &x0, uint32_t &y) {
+        x0 = reg / ry_n * PC_RX + (threadIdx.x % (PC_RX / 4)) * 4;
+        y = reg % ry_n * PC_RY + threadIdx.x / (PC_RX / 4);
+        return x0 < w && y < h;
     };
+    // software-pipelined over the regions: the next region's labels load
+    // while this region's sizes are in flight
     QuadLabels cur;
-    uint32_t t0 = blockIdx.x * per;
-    if (t0 < t_end && pixel0(t0) < n)
-        load_labels(labels, w, h, (uint32_t)(pixel0(t0) % w), (uint32_t)(pixel0(t0) / w), cur);
-    for (uint32_t tile = t0; tile < t_end; tile++) {
-        const size_t p0 = pixel0(tile);
-        const bool here = p0 < n;
+    const uint32_t t0 = blockIdx.x * per;
+    uint32_t cx0, cy;
+    bool here = t0 < t_end && origin(t0, cx0, cy);
+    if (here) load_labels(labels, w, h, cx0, cy, cur);
+    for (uint32_t reg = t0; reg < t_end; reg++) {
         uint32_t sz[12];
         if (here) load_sizes(sizes, cur, sz);
         QuadLabels nxt;
-        const size_t p1 = pixel0(tile + 1);
-        if (tile + 1 < t_end && p1 < n)
-            load_labels(labels, w, h, (uint32_t)(p1 % w), (uint32_t)(p1 / w), nxt);
+        uint32_t nx0 = 0, ny = 0;
+        const bool next_here = reg + 1 < t_end && origin(reg + 1, nx0, ny);
+        if (next_here) load_labels(labels, w, h, nx0, ny, nxt);
         PixelRecords r[BND_PX_PER_THREAD];
 #pragma unroll
         for (int k = 0; k < BND_PX_PER_THREAD; k++) r[k].mask = 0;
         if (here) {
             size_bits(cur, sz, min_size);
-            const uint32_t y = (uint32_t)(p0 / w), x0 = (uint32_t)(p0 % w);
 #pragma unroll
             for (int k = 0; k < BND_PX_PER_THREAD; k++)
-                quad_pixel_records(cur, x0, y, k, 32, r[k], FX);
+                quad_pixel_records(cur, cx0, cy, k, 32, r[k], FX);
         }
         cur = nxt;
+        here = next_here;
+        cx0 = nx0;
+        cy = ny;
         for_each_pair_run(r, [&](uint64_t key, const PairPart &q) {
             const uint32_t sl = tile_slot(lk, key, true);
             if (sl == NO_SLOT) {
@@ -593,16 +603,17 @@ int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const
     uint32_t *tile_counts = tile_scratch, *tile_off = tile_scratch + tiles;
     // a few tiles per work-group (merged in LDS before the global table), at
     // most ~12 of them at large frames
-    const unsigned pg = min(tiles, std::max(512u, tiles / 12));
+    const unsigned regions = atq_div_up(w, PC_RX) * atq_div_up(h, PC_RY);
+    const unsigned pg = min(regions, std::max(512u, regions / 12));
     if (fx) {
         k_pair_count<true><<<pg, 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h,
-                                              (uint32_t)min_size, tiles, table, max_count, counts);
+                                              (uint32_t)min_size, table, max_count, counts);
         k_bnd_filter<true><<<xcd_grid(tiles), 256, 0, s>>>(
             labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size, label_bits, table,
             max_count, counts, tiles, tile_keys, tile_points, tile_counts);
     } else {
         k_pair_count<false><<<pg, 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h,
-                                               (uint32_t)min_size, tiles, table, max_count, counts);
+                                               (uint32_t)min_size, table, max_count, counts);
         k_bnd_filter<false><<<xcd_grid(tiles), 256, 0, s>>>(
             labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size, label_bits, table,
             max_count, counts, tiles, tile_keys, tile_points, tile_counts);
