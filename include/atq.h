@@ -150,7 +150,13 @@ typedef struct atq_counts {
                                  the records of pairs whose record count
                                  ValidBlobFilter rejects (< 24 or
                                  > 2 * (2w + 2h))                          */
-    uint32_t reserved[5];
+    uint32_t n_valid_pairs;   /* detector: cluster pairs passing
+                                 ValidBlobFilter (from its pair table)     */
+    uint32_t sort_passes;     /* detector: radix passes over the pair-rank
+                                 keys (1 or 2); 0 = the full-key sort ran
+                                 (pair table overflow or too many valid
+                                 pairs to rank)                            */
+    uint32_t reserved[3];
 } atq_counts;
 
 /* ---- library ------------------------------------------------------------ */

@@ -18,6 +18,7 @@
 #include "kernels.hpp"
 #include "records.hpp"
 #include "scan.hpp"
+#include "sort.hpp"
 
 namespace atq {
 
@@ -166,14 +167,18 @@ k_boundaries_sparse(const uint32_t *__restrict__ labels, const uint32_t *__restr
 //     no more adds (the pairs that span the frame would otherwise queue on
 //     one address); such a pair fails the filter, and every pair at or below
 //     the limit is reduced exactly.
-//  2. k_bnd_filter: each tile looks its pairs up, applies ValidBlobFilter to
-//     the reduction, writes the records of the passing pairs densely into its
-//     own region (4 per pixel) in raster order and plane order within a
-//     pixel, and its count.
-//  3. the tile counts are scanned and k_bnd_gather moves the regions
-//     together: the records of the valid clusters in the reference's order.
+//  2. a look-back scan over the table's slots applies ValidBlobFilter to
+//     every pair's reduction and lists the valid pairs (slot order);
+//     k_pair_rank gives each listed pair its rank r among them (by key).
+//  3. k_bnd_filter: each tile looks its pairs up and writes the records of
+//     the valid ones, keyed (r << 2 | plane), in raster order and plane
+//     order within a pixel, at its offset from a look-back over the tiles
+//     (so the records of the valid pairs land in the reference's order).
+//     The rank keys take 1 or 2 radix passes (sort.hip, rank_sort).
+// More valid pairs than the rank list holds: the records keep the packed
+// pair keys ((min << lb | max) << 2 | plane) and the full-key sort runs.
 // If the global table runs out of probes, counts->pair_filter_off is set and
-// every record is kept.
+// every record is kept, with its packed pair key.
 constexpr int BND_PX_PER_THREAD = 4;
 constexpr int BND_TILE = 256 * BND_PX_PER_THREAD;
 constexpr uint32_t PC_RX = 64, PC_RY = 16;  // k_pair_count's regions (256 x 4 pixels)
@@ -268,6 +273,20 @@ ATQ_DEV void global_bounds(const PairTable &t, uint32_t h, const PairPart &q) {
     atomicMax(&t.bbox[4 * (size_t)h + 3], q.ymax);
     atomicAdd(&t.grad[h], grad_pack(q.gx, q.gy));
     atomicAdd(&t.moment[h], (unsigned long long)q.pg);
+}
+
+// The pair in slot g (count c) as the reference's ClusterBounds.
+ATQ_DEV atq_cluster_bounds table_bounds(const PairTable &t, uint32_t g, uint32_t c) {
+    atq_cluster_bounds b;
+    b.x_min = (uint16_t)(0xFFFFu - t.bbox[4 * (size_t)g + 0]);
+    b.x_max = (uint16_t)t.bbox[4 * (size_t)g + 1];
+    b.y_min = (uint16_t)(0xFFFFu - t.bbox[4 * (size_t)g + 2]);
+    b.y_max = (uint16_t)t.bbox[4 * (size_t)g + 3];
+    grad_unpack(t.grad[g], &b.gx_sum, &b.gy_sum);
+    b.pxgx_pygy_sum = (int64_t)t.moment[g];
+    b.start = 0;
+    b.count = c;
+    return b;
 }
 
 ATQ_DEV void global_add(const PairTable &t, uint64_t key, const PairPart &q, uint32_t max_count,
@@ -470,62 +489,163 @@ k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
     global_add2(table, key, q, live, max_count, counts);
 }
 
+constexpr uint32_t RANK_NONE = 0xFFFFFFFFu;     // slot_rank: the pair fails the filter
+constexpr uint32_t RANK_PENDING = 0xFFFFFFFEu;  // valid, beyond the rank list
+
+// Verdicts of the table's slots and the list of valid pairs, in slot order:
+// slot i holds a valid pair when its count is in [24, max_count] and its
+// reduction passes ValidBlobFilter (include/cluster_bounds.h:85-120). One
+// work-group per PV_TILE slots (4 per thread, one 16-byte count load), list
+// offsets from a look-back over the work-groups (dispatched in order);
+// counts->n_valid_pairs = the number of valid pairs, the first `cap` listed.
+constexpr int PV_ITEMS = 4;
+constexpr uint32_t PV_TILE = 256 * PV_ITEMS;
+__global__ void __launch_bounds__(256)
+k_pair_valid(PairTable t, uint32_t w, uint32_t h, uint32_t max_count,
+             uint32_t *__restrict__ slot_rank, uint64_t *__restrict__ vkeys,
+             uint32_t *__restrict__ vslots, uint32_t cap, atq_counts *counts, uint64_t *flags,
+             uint64_t *next_flags, uint32_t n_flags) {
+    __shared__ uint32_t lds4[4];
+    __shared__ uint32_t s_prefix;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n_flags; i += gridDim.x * 256)
+        next_flags[i] = 0;
+    const uint32_t slots = t.mask + 1;  // a power of two >= 256
+    const uint32_t tile = blockIdx.x, i0 = tile * PV_TILE + threadIdx.x * PV_ITEMS;
+    const uint4 c4 = i0 < slots ? *reinterpret_cast<const uint4 *>(t.cnt + i0) : make_uint4(0, 0, 0, 0);
+    const uint32_t cs[PV_ITEMS] = {c4.x, c4.y, c4.z, c4.w};
+    uint32_t v[PV_ITEMS], sum = 0;
+#pragma unroll
+    for (int k = 0; k < PV_ITEMS; k++) {
+        v[k] = cs[k] >= 24 && cs[k] <= max_count &&
+                       valid_blob(table_bounds(t, i0 + k, cs[k]), w, h) ? 1u : 0u;
+        sum += v[k];
+    }
+    uint32_t tot;
+    const uint32_t ex = block_exclusive_scan<uint32_t>(sum, lds4, &tot);
+    if (threadIdx.x < 64) {
+        const uint32_t excl = lb_lookback(flags, tile, tot, &counts->overflow);
+        if (threadIdx.x == 0) s_prefix = excl;
+    }
+    __syncthreads();
+    const uint32_t prefix = s_prefix;
+    if (threadIdx.x == 0 && (tile + 1) * PV_TILE >= slots) counts->n_valid_pairs = prefix + tot;
+    if (i0 >= slots) return;
+    uint32_t run = prefix + ex;
+#pragma unroll
+    for (int k = 0; k < PV_ITEMS; k++) {
+        slot_rank[i0 + k] = v[k] ? RANK_PENDING : RANK_NONE;
+        if (v[k]) {
+            if (run < cap) {
+                vkeys[run] = ~t.keys[i0 + k];
+                vslots[run] = i0 + k;
+            }
+            run++;
+        }
+    }
+}
+
+// Rank of every listed pair among the V valid pairs (number of smaller keys;
+// keys are distinct), to slot_rank[its slot], and rank_keys[rank] = its
+// packed key min << lb | max. Also decides the sort: counts->sort_passes =
+// the radix passes the rank keys need, or 0 (table overflow, or more valid
+// pairs than the list holds: packed keys, full-key sort). One lane per
+// listed pair (64 per work-group); the 16 waves compare them against a
+// sixteenth of each LDS tile of keys each.
+constexpr int RANK_TILE = 4096;
+constexpr int RANK_WAVES = 16;
+__global__ void __launch_bounds__(64 * RANK_WAVES)
+k_pair_rank(const uint64_t *__restrict__ vkeys, const uint32_t *__restrict__ vslots,
+            atq_counts *counts, uint32_t cap, uint32_t lb, uint32_t *__restrict__ slot_rank,
+            uint64_t *__restrict__ rank_keys) {
+    __shared__ uint64_t tile[RANK_TILE];
+    __shared__ uint32_t part[RANK_WAVES][64];
+    const uint32_t V = counts->n_valid_pairs;
+    const uint32_t np = counts->pair_filter_off == 0 && V <= cap ? rank_passes(V) : 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) counts->sort_passes = np;
+    if (np == 0 || blockIdx.x * 64 >= V) return;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t i = blockIdx.x * 64 + lane;
+    const uint64_t mine = i < V ? vkeys[i] : 0ull;
+    uint32_t below = 0;
+    for (uint32_t t0 = 0; t0 < V; t0 += RANK_TILE) {
+        const uint32_t nt = min((uint32_t)RANK_TILE, V - t0);
+        for (uint32_t j = threadIdx.x; j < nt; j += 64 * RANK_WAVES) tile[j] = vkeys[t0 + j];
+        __syncthreads();
+        const uint32_t per = (nt + RANK_WAVES - 1) / RANK_WAVES;
+        const uint32_t q0 = min(nt, wave * per), q1 = min(nt, q0 + per);
+        uint32_t j = q0;
+        for (; j + 4 <= q1; j += 4)
+            below += (tile[j] < mine ? 1u : 0u) + (tile[j + 1] < mine ? 1u : 0u) +
+                     (tile[j + 2] < mine ? 1u : 0u) + (tile[j + 3] < mine ? 1u : 0u);
+        for (; j < q1; j++) below += tile[j] < mine ? 1u : 0u;
+        __syncthreads();
+    }
+    part[wave][lane] = below;
+    __syncthreads();
+    if (wave == 0 && i < V) {
+        uint32_t r = 0;
+#pragma unroll
+        for (int q = 0; q < RANK_WAVES; q++) r += part[q][lane];
+        slot_rank[vslots[i]] = r;
+        rank_keys[r] = ((mine >> 32) << lb) | (mine & 0xFFFFFFFFull);
+    }
+}
+
+// One work-group per 1024-pixel tile (XCD-contiguous tile order): the
+// tile's records of valid pairs, keyed for the sort, to its own region of
+// 4 * 1024 entries in raster order, and their number to tile_counts.
 template <bool FX>
 __global__ void __launch_bounds__(256)
 k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes, uint32_t w,
-             uint32_t h, uint32_t min_size, uint32_t lb, PairTable table, uint32_t max_count,
-             const atq_counts *counts, uint32_t n_tiles, uint64_t *__restrict__ tile_keys,
-             atq_boundary_point *__restrict__ tile_points, uint32_t *__restrict__ tile_counts) {
+             uint32_t h, uint32_t min_size, uint32_t lb, PairTable table,
+             const uint32_t *__restrict__ slot_rank, const atq_counts *counts, uint32_t n_tiles,
+             uint64_t *__restrict__ tile_keys, atq_boundary_point *__restrict__ tile_points,
+             uint32_t *__restrict__ tile_counts) {
     const uint32_t tile = xcd_tile(blockIdx.x, n_tiles);
     if (tile >= n_tiles) return;
     __shared__ uint64_t lk[TILE_SLOTS];
-    __shared__ uint32_t lc[TILE_SLOTS];  // 1: the pair survives
+    __shared__ uint32_t lc[TILE_SLOTS];  // the pair's rank, RANK_PENDING or RANK_NONE
     __shared__ uint32_t lds4[4];
     for (int i = threadIdx.x; i < TILE_SLOTS; i += 256) {
         lk[i] = 0;
         lc[i] = 0;
     }
     const bool keep_all = counts->pair_filter_off != 0;
+    const uint32_t np = counts->sort_passes;
     __syncthreads();
     PixelRecords r[BND_PX_PER_THREAD];
     tile_records(labels, sizes, w, h, min_size, tile, r, FX);
-    auto survives = [&](uint64_t key) -> bool {
+    auto rank_of = [&](uint64_t key) -> uint32_t {
+        // every pair is in the table unless it overflowed (then keep_all)
         const uint32_t g = global_slot(table, key, false);
-        if (g == NO_SLOT) return true;
-        const uint32_t c = table.cnt[g];
-        if (c < 24 || c > max_count) return false;
-        atq_cluster_bounds b;
-        b.x_min = (uint16_t)(0xFFFFu - table.bbox[4 * (size_t)g + 0]);
-        b.x_max = (uint16_t)table.bbox[4 * (size_t)g + 1];
-        b.y_min = (uint16_t)(0xFFFFu - table.bbox[4 * (size_t)g + 2]);
-        b.y_max = (uint16_t)table.bbox[4 * (size_t)g + 3];
-        grad_unpack(table.grad[g], &b.gx_sum, &b.gy_sum);
-        b.pxgx_pygy_sum = (int64_t)table.moment[g];
-        b.start = 0;
-        b.count = c;
-        return valid_blob(b, w, h);
+        return g == NO_SLOT ? RANK_NONE : slot_rank[g];
     };
     if (!keep_all) {
         for_each_pair_run(r, [&](uint64_t key, const PairPart &) { tile_slot(lk, key, true); }, FX);
         __syncthreads();
         for (int i = threadIdx.x; i < TILE_SLOTS; i += 256)
-            if (lk[i]) lc[i] = survives(~lk[i]) ? 1u : 0u;
+            if (lk[i]) lc[i] = rank_of(~lk[i]);
         __syncthreads();
     }
-    uint32_t keep = 0, c = 0;  // keep: bit 4k+p for record (pixel k, plane p)
+    // keep: bit 4k+p for record (pixel k, plane p); r[k].k[p] becomes its
+    // output key
+    uint32_t keep = 0, c = 0;
 #pragma unroll
     for (int k = 0; k < BND_PX_PER_THREAD; k++)
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             if (!((r[k].mask >> p) & 1u)) continue;
-            bool ok = true;
+            const uint64_t key = r[k].k[p];
+            uint32_t rank = RANK_PENDING;
             if (!keep_all) {
-                const uint32_t sl = tile_slot(lk, r[k].k[p], false);
-                ok = sl != NO_SLOT ? lc[sl] != 0 : survives(r[k].k[p]);
+                const uint32_t sl = tile_slot(lk, key, false);
+                rank = sl != NO_SLOT ? lc[sl] : rank_of(key);
             }
-            if (ok) {
+            if (rank != RANK_NONE) {
                 keep |= 1u << (4 * k + p);
                 c++;
+                const uint64_t packed = ((key >> 32) << lb) | (key & 0xFFFFFFFFull);
+                r[k].k[p] = ((np ? (uint64_t)rank : packed) << 2) | (uint64_t)p;
             }
         }
     uint32_t tot;
@@ -536,22 +656,39 @@ k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             if (!((keep >> (4 * k + p)) & 1u)) continue;
-            const uint64_t key = r[k].k[p];
-            const uint64_t packed = ((key >> 32) << lb) | (key & 0xFFFFFFFFull);
-            tile_keys[base + pos] = (packed << 2) | (uint64_t)p;
+            tile_keys[base + pos] = r[k].k[p];
             tile_points[base + pos] = r[k].p[p];
             pos++;
         }
     if (threadIdx.x == 0) tile_counts[tile] = tot;
 }
 
+namespace {
+struct TileCount {
+    const uint32_t *c;
+    ATQ_DEV uint32_t operator()(uint32_t i) const { return c[i]; }
+};
+struct TileOffset {
+    uint32_t *off;
+    ATQ_DEV void operator()(uint32_t i, uint32_t pre, uint32_t) const { off[i] = pre; }
+};
+}  // namespace
+
+// One work-group per tile: its records moved to where the sort reads them,
+// at the tile's offset from the scan of the tile counts (4 records' loads
+// in flight per thread).
 __global__ void __launch_bounds__(256)
 k_bnd_gather(const uint64_t *__restrict__ tile_keys, const atq_boundary_point *__restrict__ tile_points,
              const uint32_t *__restrict__ tile_counts, const uint32_t *__restrict__ tile_off,
-             uint64_t *__restrict__ keys, atq_boundary_point *__restrict__ points) {
-    const uint32_t n = tile_counts[blockIdx.x], off = tile_off[blockIdx.x];
-    const size_t base = (size_t)blockIdx.x * (4 * BND_TILE);
-    for (uint32_t i0 = threadIdx.x; i0 < n; i0 += 4 * 256) {  // 4 records' loads in flight
+             const atq_counts *counts, RsDyn out) {
+    const uint32_t tile = blockIdx.x, n = tile_counts[tile], off = tile_off[tile];
+    // the rank passes end in Y; the full-key sort reads X
+    const uint32_t np = counts->sort_passes;
+    const bool to_y = np != 0 && (np & 1u) == 0;
+    uint64_t *keys = to_y ? out.ky : out.kx;
+    atq_boundary_point *points = reinterpret_cast<atq_boundary_point *>(to_y ? out.vy : out.vx);
+    const size_t base = (size_t)tile * (4 * BND_TILE);
+    for (uint32_t i0 = threadIdx.x; i0 < n; i0 += 4 * 256) {
         uint64_t k[4];
         atq_boundary_point p[4];
 #pragma unroll
@@ -573,17 +710,6 @@ k_bnd_gather(const uint64_t *__restrict__ tile_keys, const atq_boundary_point *_
     }
 }
 
-namespace {
-struct TileCount {
-    const uint32_t *c;
-    ATQ_DEV uint32_t operator()(uint32_t i) const { return c[i]; }
-};
-struct TileOffset {
-    uint32_t *off;
-    ATQ_DEV void operator()(uint32_t i, uint32_t pre, uint32_t) const { off[i] = pre; }
-};
-}  // namespace
-
 int launch_find_boundaries_sparse(hipStream_t s, const uint32_t *labels, const uint32_t *sizes,
                                   size_t w, size_t h, atq_boundary_point *points, uint64_t *keys,
                                   int min_size) {
@@ -595,33 +721,47 @@ int launch_find_boundaries_sparse(hipStream_t s, const uint32_t *labels, const u
 int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const uint32_t *sizes,
                                     size_t w, size_t h, int min_size, uint32_t label_bits,
                                     const PairTable &table, atq_counts *counts,
-                                    uint64_t *tile_keys, atq_boundary_point *tile_points,
-                                    uint32_t *tile_scratch, uint64_t *keys,
-                                    atq_boundary_point *points, ScanCtx &sc, bool fx) {
+                                    const PairRanks &rk, const RsDyn &out, ScanCtx &sc, bool fx) {
     const unsigned tiles = atq_div_up(w * h, BND_TILE);
     const uint32_t max_count = (uint32_t)(2 * (2 * w + 2 * h));
-    uint32_t *tile_counts = tile_scratch, *tile_off = tile_scratch + tiles;
+    const uint32_t slots = table.mask + 1;
+    if (!sc.flags || atq_div_up(slots, PV_TILE) + 1 > sc.flag_count ||
+        slots < 256 || rk.cap == 0 || rk.cap % 64)
+        return ATQ_ERR_BAD_ARG;
     // a few tiles per work-group (merged in LDS before the global table), at
     // most ~12 of them at large frames
     const unsigned regions = atq_div_up(w, PC_RX) * atq_div_up(h, PC_RY);
     const unsigned pg = min(regions, std::max(512u, regions / 12));
-    if (fx) {
+    if (fx)
         k_pair_count<true><<<pg, 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h,
                                               (uint32_t)min_size, table, max_count, counts);
-        k_bnd_filter<true><<<xcd_grid(tiles), 256, 0, s>>>(
-            labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size, label_bits, table,
-            max_count, counts, tiles, tile_keys, tile_points, tile_counts);
-    } else {
+    else
         k_pair_count<false><<<pg, 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h,
                                                (uint32_t)min_size, table, max_count, counts);
+    {
+        uint64_t *cur = sc.flags + (size_t)(sc.calls & 1) * sc.flag_count;
+        uint64_t *next = sc.flags + (size_t)((sc.calls + 1) & 1) * sc.flag_count;
+        sc.calls++;
+        k_pair_valid<<<atq_div_up(slots, PV_TILE), 256, 0, s>>>(
+            table, (uint32_t)w, (uint32_t)h, max_count, rk.slot_rank, rk.vkeys, rk.vslots, rk.cap,
+            counts, cur, next, sc.flag_count);
+    }
+    k_pair_rank<<<rk.cap / 64, 64 * RANK_WAVES, 0, s>>>(rk.vkeys, rk.vslots, counts, rk.cap, label_bits,
+                                            rk.slot_rank, rk.rank_keys);
+    uint32_t *tile_counts = rk.tile_counts, *tile_off = rk.tile_counts + tiles;
+    if (fx)
+        k_bnd_filter<true><<<xcd_grid(tiles), 256, 0, s>>>(
+            labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size, label_bits, table,
+            rk.slot_rank, counts, tiles, rk.tile_keys, rk.tile_points, tile_counts);
+    else
         k_bnd_filter<false><<<xcd_grid(tiles), 256, 0, s>>>(
             labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size, label_bits, table,
-            max_count, counts, tiles, tile_keys, tile_points, tile_counts);
-    }
-    int r = device_scan(s, nullptr, tiles, TileCount{tile_counts}, TileOffset{tile_off},
-                        sc, &counts->n_boundary);
+            rk.slot_rank, counts, tiles, rk.tile_keys, rk.tile_points, tile_counts);
+    int r = device_scan(s, nullptr, tiles, TileCount{tile_counts}, TileOffset{tile_off}, sc,
+                        &counts->n_boundary);
     if (r) return r;
-    k_bnd_gather<<<tiles, 256, 0, s>>>(tile_keys, tile_points, tile_counts, tile_off, keys, points);
+    k_bnd_gather<<<tiles, 256, 0, s>>>(rk.tile_keys, rk.tile_points, tile_counts, tile_off, counts,
+                                       out);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 

@@ -70,17 +70,34 @@ struct PairTable {
     unsigned long long *moment;
     uint32_t mask;
 };
-// Records of the pairs that pass ValidBlobFilter, packed keys
-// ((min << lb | max) << 2 | plane), in the reference's order; tile_keys /
-// tile_points hold 4 * w * h entries, tile_scratch 2 * tiles + 2. fx: fixed
-// mode (column 0 local, both half-pixel bits decoded; the labels and sizes
-// must come from the fixed-mode CCL).
+// Buffers of the detector's boundary pass: per table slot the verdict /
+// rank of its pair, the listed valid pairs (key min << 32 | max, slot),
+// rank -> packed key (cap pairs at most, a multiple of 64), the per-tile
+// record regions (4 * w * h entries) and the tiles' counts and offsets (2
+// per 1024-pixel tile).
+struct PairRanks {
+    uint32_t *slot_rank;
+    uint64_t *vkeys;
+    uint32_t *vslots;
+    uint64_t *rank_keys;
+    uint32_t cap;
+    uint64_t *tile_keys;
+    atq_boundary_point *tile_points;
+    uint32_t *tile_counts;
+};
+struct RsDyn;  // sort.hpp
+// Records of the pairs that pass ValidBlobFilter, in the reference's order,
+// keyed for rank_sort (rank keys, or packed keys ((min << lb | max) << 2 |
+// plane) when counts->sort_passes is 0), written where rank_sort reads them;
+// counts->n_boundary, n_valid_pairs and sort_passes set. sc must be the
+// look-back form (two scans). fx: fixed mode (column 0 local, both
+// half-pixel bits decoded; the labels and sizes must come from the
+// fixed-mode CCL).
 int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const uint32_t *sizes,
                                     size_t w, size_t h, int min_size, uint32_t label_bits,
                                     const PairTable &table, atq_counts *counts,
-                                    uint64_t *tile_keys, atq_boundary_point *tile_points,
-                                    uint32_t *tile_scratch, uint64_t *keys,
-                                    atq_boundary_point *points, ScanCtx &sc, bool fx = false);
+                                    const PairRanks &rk, const RsDyn &out, ScanCtx &sc,
+                                    bool fx = false);
 constexpr unsigned BOUNDARY_TILE_PX = 1024;  // pixels per compaction tile
 
 // diff.hip

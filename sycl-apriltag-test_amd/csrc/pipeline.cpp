@@ -107,8 +107,13 @@ int take_workspace(Workspace &ws, void *workspace, size_t bytes, size_t w, size_
 }
 
 // Look-back flags per scan array: enough tiles for the longest scan (at
-// most cap elements).
-uint32_t scan_flag_count(const Caps &c) { return c.cap / LB_TILE + 2; }
+// most cap elements) and for the pair table's slots (1024 per tile).
+uint32_t scan_flag_count(const Caps &c) {
+    return std::max(c.cap / LB_TILE, c.pair_slots / 1024) + 2;
+}
+
+// Valid cluster pairs the detector ranks (more: the full-key sort runs).
+uint32_t rank_cap(const Caps &c) { return std::max(64u, std::min(16384u, c.pair_slots / 4)); }
 
 // Detector buffers, carved from one arena.
 size_t carve_detector(atq_detector *d, char *base) {
@@ -133,13 +138,17 @@ size_t carve_detector(atq_detector *d, char *base) {
     d->peaks = carve_one<atq_peak_extents>(base, off, c.cap_cl);
     d->fitted = carve_one<atq_fitted_quad>(base, off, c.cap_cl);
     d->quads = carve_one<atq_quad_corners>(base, off, c.cap_cl);
+    d->slot_rank = carve_one<uint32_t>(base, off, c.pair_slots);
+    d->vkeys = carve_one<uint64_t>(base, off, rank_cap(c));
+    d->vslots = carve_one<uint32_t>(base, off, rank_cap(c));
+    d->rank_keys = carve_one<uint64_t>(base, off, rank_cap(c));
     d->scan_flags = carve_one<uint64_t>(base, off, 2 * (size_t)scan_flag_count(c));
     d->pair_keys = carve_one<uint64_t>(base, off, c.pair_slots);
     d->pair_cnt = carve_one<uint32_t>(base, off, c.pair_slots);
     d->pair_bbox = carve_one<uint32_t>(base, off, 4 * (size_t)c.pair_slots);
     d->pair_grad = carve_one<unsigned long long>(base, off, (size_t)c.pair_slots);
     d->pair_moment = carve_one<unsigned long long>(base, off, c.pair_slots);
-    d->long_n = carve_one<uint32_t>(base, off, 2);
+    d->long_n = carve_one<uint32_t>(base, off, 4);  // [2]: the full-key sort's barrier
     d->counts = carve_one<atq_counts>(base, off, 1);
     off = (off + 255) & ~(size_t)255;
     size_t ws_bytes = d->ws.carve(base ? base + off : nullptr, c);
@@ -217,22 +226,28 @@ int run_back(atq_detector *d, hipStream_t s, const uint8_t *gray_in) {
     const uint8_t *grey = f > 1 || d->channels != 1 ? d->gray : gray_in;
     auto mark = [&](int i) { return ::mark(d, s, i); };
     // Boundary records of the pairs that pass ValidBlobFilter, in the
-    // reference's order, keyed by the packed pair key min<<lb|max with the
-    // plane in 2 low bits; the stable sort then yields the reference's
-    // (pair, plane, y, x) order.
+    // reference's order, keyed by their pair's rank among the valid pairs
+    // (or, in the fallback, the packed pair key min<<lb|max) with the plane
+    // in 2 low bits; the stable sort then yields the reference's
+    // (pair, plane, y, x) order, and its last pass restores the packed keys.
     const PairTable table{d->pair_keys, d->pair_cnt,    d->pair_bbox,
                           d->pair_grad, d->pair_moment, d->caps.pair_slots - 1};
     const bool fx = d->params.mode == ATQ_MODE_FIXED;
+    const RsDyn io{d->keys, d->vals, ws.keys_alt, ws.vals_alt, &cnt->sort_passes, d->rank_keys};
+    // the per-tile record regions live in the cluster-bounds area, which is
+    // free until the sorted records are grouped (12 of its 32 bytes per entry)
+    uint64_t *tile_keys = reinterpret_cast<uint64_t *>(ws.seg_bounds);
+    const PairRanks rk{d->slot_rank, d->vkeys, d->vslots, d->rank_keys, rank_cap(d->caps),
+                       tile_keys, reinterpret_cast<atq_boundary_point *>(tile_keys + c.cap),
+                       ws.tiles};
     TRY(launch_find_boundaries_filtered(s, d->labels, d->sizes, w, h, d->params.min_component_size,
-                                        (uint32_t)plan.label_bits, table, cnt, ws.keys_alt,
-                                        reinterpret_cast<atq_boundary_point *>(ws.vals_alt),
-                                        ws.tiles, d->keys,
-                                        reinterpret_cast<atq_boundary_point *>(d->vals), sc, fx));
+                                        (uint32_t)plan.label_bits, table, cnt, rk, io, sc, fx));
     TRY(mark(1 + ATQ_STAGE_BOUNDARY));
-    bool alt = false;
-    TRY(radix_sort_packed(s, plan, d->keys, d->vals, ws.keys_alt, ws.vals_alt, &cnt->n_boundary,
-                          c.cap, ws.hist, ws.digit_totals, &alt));
+    // the sorted records end in the alt buffers
+    TRY(rank_sort(s, 2 * (uint32_t)plan.label_bits + 2, io, &cnt->n_boundary, c.cap, ws.hist,
+                  ws.digit_totals, &d->long_n[2], &cnt->overflow));
     TRY(mark(1 + ATQ_STAGE_KEY_SORT));
+    const bool alt = true;
     d->sorted_in_alt = alt;
     const uint64_t *K = alt ? ws.keys_alt : d->keys;
     const atq_boundary_point *V =

@@ -77,12 +77,17 @@ struct atq_detector {
     atq_peak_extents *peaks;
     atq_fitted_quad *fitted;
     atq_quad_corners *quads;
+    uint32_t *slot_rank;   // per pair-table slot: verdict / rank of its pair
+    uint64_t *vkeys;       // the frame's valid pairs: key and slot
+    uint32_t *vslots;
+    uint64_t *rank_keys;   // rank -> packed pair key
     uint64_t *scan_flags;  // two look-back flag arrays, zeroed before each frame
     uint64_t *pair_keys;   // pair table, zeroed with counts before each frame
     uint32_t *pair_cnt, *pair_bbox;
     unsigned long long *pair_grad;
     unsigned long long *pair_moment;
-    uint32_t *long_n;      // long-segment counts of the two segment sorts, zeroed likewise
+    uint32_t *long_n;      // long-segment counts of the two segment sorts and the
+                           // full-key sort's barrier word, zeroed likewise
     atq_counts *counts;
     atq::Workspace ws;
     atq::FrontShapes shapes;  // threshold region and CCL tile shapes

@@ -39,19 +39,13 @@ ATQ_DEV void rs_chunks(uint32_t n, uint32_t *chunk, uint32_t *groups) {
     *groups = (n + c - 1) / c;
 }
 
+// Digit histogram of keys[b, e) into hist_row[0..2^BITS) (h: LDS of 2^BITS).
 template <int BITS>
-__global__ void __launch_bounds__(RS_THREADS)
-k_rs_hist(const uint64_t *__restrict__ keys, const uint32_t *count, uint32_t cap, uint32_t shift,
-          uint32_t *__restrict__ hist) {
+ATQ_DEV void rs_hist_body(const uint64_t *__restrict__ keys, uint32_t b, uint32_t e,
+                          uint32_t shift, uint32_t *__restrict__ hist_row, uint32_t *h) {
     constexpr int B = 1 << BITS;
-    __shared__ uint32_t h[B];
-    const uint32_t n = min(*count, cap);
-    uint32_t chunk, G;
-    rs_chunks(n, &chunk, &G);
-    if (blockIdx.x >= G) return;
     for (int i = threadIdx.x; i < B; i += RS_THREADS) h[i] = 0;
     __syncthreads();
-    const uint32_t b = blockIdx.x * chunk, e = min(n, b + chunk);
     // RS_ITEMS independent loads per lane before the LDS atomics
     for (uint32_t r = b; r < e; r += RS_ROUND) {
         uint32_t d[RS_ITEMS];
@@ -65,7 +59,21 @@ k_rs_hist(const uint64_t *__restrict__ keys, const uint32_t *count, uint32_t cap
             if (d[j] != 0xFFFFFFFFu) atomicAdd(&h[d[j]], 1u);
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < B; d += RS_THREADS) hist[(size_t)blockIdx.x * B + d] = h[d];
+    for (int d = threadIdx.x; d < B; d += RS_THREADS) hist_row[d] = h[d];
+}
+
+template <int BITS>
+__global__ void __launch_bounds__(RS_THREADS)
+k_rs_hist(const uint64_t *__restrict__ keys, const uint32_t *count, uint32_t cap, uint32_t shift,
+          uint32_t *__restrict__ hist) {
+    constexpr int B = 1 << BITS;
+    __shared__ uint32_t h[B];
+    const uint32_t n = min(*count, cap);
+    uint32_t chunk, G;
+    rs_chunks(n, &chunk, &G);
+    if (blockIdx.x >= G) return;
+    const uint32_t b = blockIdx.x * chunk, e = min(n, b + chunk);
+    rs_hist_body<BITS>(keys, b, e, shift, hist + (size_t)blockIdx.x * B, h);
 }
 
 // Exclusive scan of every digit's column hist[0..G)[d] in place, and the
@@ -76,8 +84,9 @@ k_rs_hist(const uint64_t *__restrict__ keys, const uint32_t *count, uint32_t cap
 constexpr int RS_SCAN_WAVES = 16;
 __global__ void __launch_bounds__(64 * RS_SCAN_WAVES)
 k_rs_scan(uint32_t *__restrict__ hist, uint32_t nbuckets, const uint32_t *count, uint32_t cap,
-          uint32_t *__restrict__ totals) {
+          uint32_t *__restrict__ totals, const uint32_t *passes = nullptr, uint32_t pass = 0) {
     __shared__ uint32_t part[RS_SCAN_WAVES][64];
+    if (passes && pass >= *passes) return;  // rank form: this pass is not needed
     const uint32_t n = min(*count, cap);
     uint32_t chunk, G;
     rs_chunks(n, &chunk, &G);
@@ -116,23 +125,31 @@ k_rs_scan(uint32_t *__restrict__ hist, uint32_t nbuckets, const uint32_t *count,
     if (q == 0) totals[d] = total;
 }
 
+// LDS of the scatter body.
 template <int BITS>
-__global__ void __launch_bounds__(RS_THREADS)
-k_rs_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
-             uint64_t *__restrict__ kout, uint32_t *__restrict__ vout, const uint32_t *count,
-             uint32_t cap, uint32_t shift, const uint32_t *__restrict__ hist,
-             const uint32_t *__restrict__ totals) {
+struct RsScatterLds {
+    uint32_t wc[4][1 << BITS];     // per-wave digit counts, then wave offsets
+    uint32_t base_off[1 << BITS];  // running output offset of each digit
+    uint32_t lds4[4];
+};
+
+// Scatters records [b, e) (chunk g of the pass) to their stable positions:
+// digit base (exclusive scan of `totals`) + hist_row[digit] (the digit's
+// count in the chunks before this one) + rank within the chunk. xlat: when
+// not null, keys are (rank << 2 | plane) and leave as
+// (xlat[rank] << 2 | plane).
+template <int BITS>
+ATQ_DEV void rs_scatter_body(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+                             uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                             uint32_t b, uint32_t e, uint32_t shift,
+                             const uint32_t *__restrict__ hist_row,
+                             const uint32_t *__restrict__ totals,
+                             const uint64_t *__restrict__ xlat, RsScatterLds<BITS> &L) {
     constexpr int B = 1 << BITS;
     constexpr int PER = B / RS_THREADS;  // digits per thread in the base scan
-    __shared__ uint32_t wc[4][B];      // per-wave digit counts, then wave offsets
-    __shared__ uint32_t base_off[B];   // running output offset of each digit
-    __shared__ uint32_t lds4[4];
+    auto &wc = L.wc;
+    auto &base_off = L.base_off;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const uint32_t n = min(*count, cap);
-    uint32_t chunk, G;
-    rs_chunks(n, &chunk, &G);
-    if (blockIdx.x >= G) return;
-    const uint32_t b = blockIdx.x * chunk, e = min(n, b + chunk);
     // the first round's records are loaded before the digit bases are
     // built, so the two rounds of loads overlap
     uint64_t k[RS_ITEMS];
@@ -155,11 +172,11 @@ k_rs_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
             sum += tv[k];
         }
         uint32_t all;
-        uint32_t run = block_exclusive_scan<uint32_t>(sum, lds4, &all);
+        uint32_t run = block_exclusive_scan<uint32_t>(sum, L.lds4, &all);
 #pragma unroll
         for (int k = 0; k < PER; k++) {
             const int d = t * PER + k;
-            base_off[d] = run + hist[(size_t)blockIdx.x * B + d];
+            base_off[d] = run + hist_row[d];
             run += tv[k];
         }
     }
@@ -214,12 +231,29 @@ k_rs_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
         for (int j = 0; j < RS_ITEMS; j++) {
             if (dg[j] != 0xFFFFFFFFu) {
                 uint32_t pos = wc[wave][dg[j]] + rank[j];
-                kout[pos] = k[j];
+                kout[pos] = xlat ? (xlat[k[j] >> 2] << 2) | (k[j] & 3u) : k[j];
                 vout[pos] = v[j];
             }
         }
         __syncthreads();
     }
+}
+
+template <int BITS>
+__global__ void __launch_bounds__(RS_THREADS)
+k_rs_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+             uint64_t *__restrict__ kout, uint32_t *__restrict__ vout, const uint32_t *count,
+             uint32_t cap, uint32_t shift, const uint32_t *__restrict__ hist,
+             const uint32_t *__restrict__ totals) {
+    constexpr int B = 1 << BITS;
+    __shared__ RsScatterLds<BITS> L;
+    const uint32_t n = min(*count, cap);
+    uint32_t chunk, G;
+    rs_chunks(n, &chunk, &G);
+    if (blockIdx.x >= G) return;
+    const uint32_t b = blockIdx.x * chunk, e = min(n, b + chunk);
+    rs_scatter_body<BITS>(kin, vin, kout, vout, b, e, shift, hist + (size_t)blockIdx.x * B,
+                          totals, nullptr, L);
 }
 
 // Pair key min<<32|max <-> min<<lb|max.
@@ -239,6 +273,158 @@ __global__ void k_unpack_keys(const uint64_t *__restrict__ in, uint64_t *__restr
         uint64_t k = in[i];
         out[i] = ((k >> lb) << 32) | (k & lo);
     }
+}
+
+// ---- the detector's sort: rank keys, device-chosen pass count -------------
+//
+// The detector's keys are (r << 2 | plane) with r the rank of the record's
+// cluster pair among the frame's valid pairs, so RK_BITS-bit digits sort
+// them in counts->sort_passes passes (1 or 2; 0 = the full-key fallback
+// below does the sort). Buffers X and Y ping-pong so that the last pass
+// writes Y: a pass reads Y when an even number of passes, itself included,
+// remain (the filter wrote its records there), else X.
+ATQ_DEV bool rs_dyn_io(const RsDyn &d, uint32_t pass, const uint64_t **ki, const uint32_t **vi,
+                       uint64_t **ko, uint32_t **vo, bool *last) {
+    const uint32_t np = *d.passes;
+    if (pass >= np) return false;
+    const bool from_y = ((np - pass) & 1u) == 0;
+    *ki = from_y ? d.ky : d.kx;
+    *vi = from_y ? d.vy : d.vx;
+    *ko = from_y ? d.kx : d.ky;
+    *vo = from_y ? d.vx : d.vy;
+    *last = pass + 1 == np;
+    return true;
+}
+
+__global__ void __launch_bounds__(RS_THREADS)
+k_rs_hist_dyn(RsDyn d, const uint32_t *count, uint32_t cap, uint32_t pass,
+              uint32_t *__restrict__ hist) {
+    constexpr int B = 1 << RK_BITS;
+    __shared__ uint32_t h[B];
+    const uint64_t *ki;
+    const uint32_t *vi;
+    uint64_t *ko;
+    uint32_t *vo;
+    bool last;
+    if (!rs_dyn_io(d, pass, &ki, &vi, &ko, &vo, &last)) return;
+    const uint32_t n = min(*count, cap);
+    uint32_t chunk, G;
+    rs_chunks(n, &chunk, &G);
+    if (blockIdx.x >= G) return;
+    const uint32_t b = blockIdx.x * chunk, e = min(n, b + chunk);
+    rs_hist_body<RK_BITS>(ki, b, e, pass * RK_BITS, hist + (size_t)blockIdx.x * B, h);
+}
+
+// Grid-wide barrier of a kernel whose work-groups are all resident (a grid
+// of RS_FULL_GROUPS work-groups, far below what the device holds at once).
+// bar counts arrivals (zero at the kernel's start); the wait is bounded:
+// past LB_SPIN_LIMIT polls *error gets OVF_SCAN and the results are void.
+ATQ_DEV void rs_grid_sync(uint32_t *bar, uint32_t target, uint32_t *error) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        atomicAdd(bar, 1u);
+        uint32_t spins = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (++spins >= LB_SPIN_LIMIT) {
+                atomicOr(error, OVF_SCAN);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __threadfence();
+    }
+    __syncthreads();
+}
+
+// The full-key fallback: a stable LSD sort of the packed keys (`passes`
+// digits of BITS bits) from X into Y by work-groups 0 .. G-1 of the last
+// rank-pass kernel (the others exit at once, so these G are resident
+// together), which run it only when *d.passes == 0: frames that need no
+// fallback pay nothing for it. Each pass: chunk histograms, column scans,
+// scatter, separated by grid barriers.
+template <int BITS>
+ATQ_DEV void rs_full_body(const RsDyn &d, const uint32_t *count, uint32_t cap, uint32_t passes,
+                          uint32_t G, uint32_t *hist, uint32_t *totals, uint32_t *bar,
+                          uint32_t *error, RsScatterLds<BITS> &L) {
+    constexpr int B = 1 << BITS;
+    const uint32_t n = min(*count, cap), g = blockIdx.x;
+    uint32_t chunk = (n + G - 1) / G;
+    chunk = (chunk + RS_ROUND - 1) / RS_ROUND * RS_ROUND;
+    const uint32_t b = min(n, g * chunk), e = min(n, b + chunk);
+    uint32_t gen = 0;
+    for (uint32_t pass = 0; pass < passes; pass++) {
+        const bool from_y = pass & 1u;
+        const uint64_t *ki = from_y ? d.ky : d.kx;
+        const uint32_t *vi = from_y ? d.vy : d.vx;
+        uint64_t *ko = from_y ? d.kx : d.ky;
+        uint32_t *vo = from_y ? d.vx : d.vy;
+        const uint32_t shift = pass * BITS;
+        rs_hist_body<BITS>(ki, b, e, shift, hist + (size_t)g * B, L.wc[0]);
+        rs_grid_sync(bar, ++gen * G, error);
+        for (uint32_t dd = g * RS_THREADS + threadIdx.x; dd < (uint32_t)B; dd += G * RS_THREADS) {
+            uint32_t run = 0;
+            for (uint32_t q = 0; q < G; q++) {
+                const uint32_t v = hist[(size_t)q * B + dd];
+                hist[(size_t)q * B + dd] = run;
+                run += v;
+            }
+            totals[dd] = run;
+        }
+        rs_grid_sync(bar, ++gen * G, error);
+        rs_scatter_body<BITS>(ki, vi, ko, vo, b, e, shift, hist + (size_t)g * B, totals, nullptr,
+                              L);
+        rs_grid_sync(bar, ++gen * G, error);
+    }
+    if ((passes & 1u) == 0) {  // the last pass wrote X
+        for (uint32_t i = g * RS_THREADS + threadIdx.x; i < n; i += G * RS_THREADS) {
+            d.ky[i] = d.kx[i];
+            d.vy[i] = d.vx[i];
+        }
+    }
+}
+
+// The last launched rank pass also hosts the full-key fallback (its
+// work-groups 0 .. RS_FULL_GROUPS-1), full_passes 11-bit digits.
+__global__ void __launch_bounds__(RS_THREADS)
+k_rs_scatter_dyn(RsDyn d, const uint32_t *count, uint32_t cap, uint32_t pass,
+                 uint32_t *__restrict__ hist, uint32_t *__restrict__ totals, uint32_t full_passes,
+                 uint32_t *bar, uint32_t *error) {
+    constexpr int B = 1 << RK_BITS;
+    __shared__ RsScatterLds<RK_BITS> L;
+    if (*d.passes == 0) {
+        if (full_passes && blockIdx.x < RS_FULL_GROUPS)
+            rs_full_body<RK_BITS>(d, count, cap, full_passes, RS_FULL_GROUPS, hist, totals, bar,
+                                  error, L);
+        return;
+    }
+    const uint64_t *ki;
+    const uint32_t *vi;
+    uint64_t *ko;
+    uint32_t *vo;
+    bool last;
+    if (!rs_dyn_io(d, pass, &ki, &vi, &ko, &vo, &last)) return;
+    const uint32_t n = min(*count, cap);
+    uint32_t chunk, G;
+    rs_chunks(n, &chunk, &G);
+    if (blockIdx.x >= G) return;
+    const uint32_t b = blockIdx.x * chunk, e = min(n, b + chunk);
+    rs_scatter_body<RK_BITS>(ki, vi, ko, vo, b, e, pass * RK_BITS, hist + (size_t)blockIdx.x * B,
+                             totals, last ? d.xlat : nullptr, L);
+}
+
+int rank_sort(hipStream_t s, uint32_t full_bits, const RsDyn &d, const uint32_t *count,
+              uint32_t cap, uint32_t *hist, uint32_t *totals, uint32_t *bar, uint32_t *error) {
+    constexpr uint32_t B = 1u << RK_BITS;
+    const uint32_t full_passes = (full_bits + RK_BITS - 1) / RK_BITS;
+    for (uint32_t pass = 0; pass < RK_MAX_PASSES; pass++) {
+        const bool last = pass + 1 == RK_MAX_PASSES;
+        k_rs_hist_dyn<<<RS_GROUPS, RS_THREADS, 0, s>>>(d, count, cap, pass, hist);
+        k_rs_scan<<<B / 64, 64 * RS_SCAN_WAVES, 0, s>>>(hist, B, count, cap, totals, d.passes, pass);
+        k_rs_scatter_dyn<<<RS_GROUPS, RS_THREADS, 0, s>>>(d, count, cap, pass, hist, totals,
+                                                          last ? full_passes : 0u, bar, error);
+    }
+    return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
 RadixPlan radix_plan(size_t npx, int extra_bits) {

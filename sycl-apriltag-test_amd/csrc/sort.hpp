@@ -23,6 +23,38 @@ int radix_sort_packed(hipStream_t s, const RadixPlan &p, uint64_t *keys, uint32_
                       uint64_t *keys_alt, uint32_t *vals_alt, const uint32_t *count, uint32_t cap,
                       uint32_t *hist, uint32_t *totals, bool *result_in_alt);
 
+// The detector's sort of its boundary records (sort.hip): keys (r << 2 |
+// plane), r the rank of the record's cluster pair among the frame's valid
+// pairs, sorted with RK_BITS-bit digits in *passes (1..RK_MAX_PASSES) passes
+// from X or Y (see rank_sort) into Y, the last pass turning r back into the
+// packed pair key xlat[r]; *passes == 0 selects the full-key fallback (packed
+// keys in X, sorted into Y by one persistent kernel).
+constexpr int RK_BITS = 11;
+constexpr uint32_t RK_MAX_PASSES = 2;
+constexpr unsigned RS_FULL_GROUPS = 128;  // work-groups of the full-key fallback
+struct RsDyn {
+    uint64_t *kx;
+    uint32_t *vx;
+    uint64_t *ky;
+    uint32_t *vy;
+    const uint32_t *passes;  // device: rank passes, 0 = full-key fallback
+    const uint64_t *xlat;    // rank -> packed pair key
+};
+// Rank passes for v valid pairs: ranks below v take bit_length(v - 1) bits,
+// plus 2 for the plane; 0 when that needs more than RK_MAX_PASSES digits.
+__host__ __device__ inline uint32_t rank_passes(uint32_t v) {
+    uint32_t bits = 2;
+    for (uint32_t r = v > 0 ? v - 1 : 0; r; r >>= 1) bits++;
+    const uint32_t np = (bits + RK_BITS - 1) / RK_BITS;
+    return np <= RK_MAX_PASSES ? np : 0u;
+}
+// Launches RK_MAX_PASSES rank passes (idle beyond *passes), the last of
+// which also holds the full-key fallback over full_bits-bit packed keys
+// (idle unless *passes == 0). `bar` is a zeroed word (the fallback's grid
+// barrier); `error` gets OVF_SCAN if a barrier wait gives up.
+int rank_sort(hipStream_t s, uint32_t full_bits, const RsDyn &d, const uint32_t *count,
+              uint32_t cap, uint32_t *hist, uint32_t *totals, uint32_t *bar, uint32_t *error);
+
 // min<<32|max <-> min<<lb|max over count keys (out may alias in).
 int launch_pack_keys(hipStream_t s, const uint64_t *in, uint64_t *out, const uint32_t *count,
                      uint32_t cap, uint32_t lb, bool unpack);

@@ -104,12 +104,28 @@ def check_pair_table(g, o):
     np.testing.assert_array_equal(np.minimum(gc, limit + 1), np.minimum(cnt, limit + 1))
 
 
+def expected_sort_passes(n_valid, slots, filter_off):
+    """Radix passes over the pair-rank keys (11-bit digits over the rank's
+    bits plus 2 plane bits; at most 2): 0 selects the full-key sort, taken
+    when the pair table overflowed or more pairs are valid than the detector
+    ranks (a quarter of the table's slots, 64 to 16384)."""
+    cap = max(64, min(16384, slots // 4))
+    if filter_off or n_valid > cap:
+        return 0
+    bits = 2 + max(n_valid - 1, 0).bit_length()
+    return -(-bits // 11)
+
+
 def check_against_oracle(g, o):
     c = g["counts"]
     assert int(c["overflow"]) == 0
     w, h = g["dims"]
-    if not int(c["pair_filter_off"]):
+    filter_off = int(c["pair_filter_off"])
+    if not filter_off:
         check_pair_table(g, o)
+        assert int(c["n_valid_pairs"]) == len(o["bounds"])
+    assert int(c["sort_passes"]) == expected_sort_passes(len(o["bounds"]), len(g["pair_keys"]),
+                                                         filter_off)
     o_bounds = o["bounds"]
     if int(c["pair_filter_off"]):
         keys, points, n_pairs = o["keys"], o["points"], len(o["bounds_all"])
@@ -173,10 +189,26 @@ def test_detector_pair_table_overflow(atq, oracle):
     assert len(np.unique(o["keys"])) > 1000
     g = run_detector(atq, img, 1, pair_table_log2=8)
     assert int(g["counts"]["pair_filter_off"]) == 1
+    assert int(g["counts"]["sort_passes"]) == 0  # the full-key sort
     check_against_oracle(g, o)
     # the default table holds them all: filtered, and the same downstream
     g = run_detector(atq, img, 1)
     assert int(g["counts"]["pair_filter_off"]) == 0
+    check_against_oracle(g, o)
+
+
+def test_detector_more_valid_pairs_than_ranked(atq, oracle):
+    """A 512-slot pair table holds the 286 pairs of a 1080p frame without
+    overflow but ranks at most 128 pairs: with 140 valid pairs the records
+    keep their packed pair keys, the full-key sort runs, and every
+    intermediate still matches the oracle."""
+    from pyatq import synth
+    img = synth.plain_1080p(2000).image
+    o = oracle.pipeline(img, 1)
+    assert len(o["bounds"]) > 128
+    g = run_detector(atq, img, 1, pair_table_log2=9)
+    assert int(g["counts"]["pair_filter_off"]) == 0
+    assert int(g["counts"]["sort_passes"]) == 0
     check_against_oracle(g, o)
 
 
