@@ -497,14 +497,15 @@ constexpr uint32_t RANK_PENDING = 0xFFFFFFFEu;  // valid, beyond the rank list
 // reduction passes ValidBlobFilter (include/cluster_bounds.h:85-120). One
 // work-group per PV_TILE slots (4 per thread, one 16-byte count load), list
 // offsets from a look-back over the work-groups (dispatched in order);
-// counts->n_valid_pairs = the number of valid pairs, the first `cap` listed.
+// counts->n_valid_pairs = the number of valid pairs, the first `cap` listed
+// with their key, slot and record count.
 constexpr int PV_ITEMS = 4;
 constexpr uint32_t PV_TILE = 256 * PV_ITEMS;
 __global__ void __launch_bounds__(256)
 k_pair_valid(PairTable t, uint32_t w, uint32_t h, uint32_t max_count,
              uint32_t *__restrict__ slot_rank, uint64_t *__restrict__ vkeys,
-             uint32_t *__restrict__ vslots, uint32_t cap, atq_counts *counts, uint64_t *flags,
-             uint64_t *next_flags, uint32_t n_flags) {
+             uint32_t *__restrict__ vslots, uint32_t *__restrict__ vcnt, uint32_t cap,
+             atq_counts *counts, uint64_t *flags, uint64_t *next_flags, uint32_t n_flags) {
     __shared__ uint32_t lds4[4];
     __shared__ uint32_t s_prefix;
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n_flags; i += gridDim.x * 256)
@@ -538,27 +539,37 @@ k_pair_valid(PairTable t, uint32_t w, uint32_t h, uint32_t max_count,
             if (run < cap) {
                 vkeys[run] = ~t.keys[i0 + k];
                 vslots[run] = i0 + k;
+                vcnt[run] = cs[k];
             }
             run++;
         }
     }
 }
 
-// Rank of every listed pair among the V valid pairs (number of smaller keys;
-// keys are distinct), to slot_rank[its slot], and rank_keys[rank] = its
-// packed key min << lb | max. Also decides the sort: counts->sort_passes =
-// the radix passes the rank keys need, or 0 (table overflow, or more valid
-// pairs than the list holds: packed keys, full-key sort). One lane per
-// listed pair (64 per work-group); the 16 waves compare them against a
-// sixteenth of each LDS tile of keys each.
+// Rank r of every listed pair among the V valid pairs (number of smaller
+// keys; keys are distinct) and its start (the records of the pairs with
+// smaller keys): slot_rank[its slot] = r, rank_keys[r] = its packed key
+// min << lb | max, and, since in rank mode the sorted records are exactly
+// the valid pairs' records grouped in key order, cluster r's bounds (the
+// table's exact reduction, include/cluster_bounds.h:14-83, starting at
+// start), its centre (cx, cy) (include/cluster_bounds.h:47-49) and
+// extents {start, count} (src/main.cpp:700-717) and the counts of pairs,
+// clusters and their points. Also decides the sort:
+// counts->sort_passes = the radix passes the rank keys need, or 0 (table
+// overflow, or more valid pairs than the list holds: packed keys, fallback
+// sort and cluster stage). One lane per listed pair (64 per work-group);
+// the 16 waves compare them against a sixteenth of each LDS tile each.
 constexpr int RANK_TILE = 4096;
 constexpr int RANK_WAVES = 16;
 __global__ void __launch_bounds__(64 * RANK_WAVES)
-k_pair_rank(const uint64_t *__restrict__ vkeys, const uint32_t *__restrict__ vslots,
+k_pair_rank(PairTable table, const uint64_t *__restrict__ vkeys,
+            const uint32_t *__restrict__ vslots, const uint32_t *__restrict__ vcnt,
             atq_counts *counts, uint32_t cap, uint32_t lb, uint32_t *__restrict__ slot_rank,
-            uint64_t *__restrict__ rank_keys) {
+            uint64_t *__restrict__ rank_keys, atq_cluster_bounds *__restrict__ bounds,
+            atq_cluster_extents *__restrict__ ext, float2 *__restrict__ centre) {
     __shared__ uint64_t tile[RANK_TILE];
-    __shared__ uint32_t part[RANK_WAVES][64];
+    __shared__ uint32_t tcnt[RANK_TILE];
+    __shared__ uint32_t part[RANK_WAVES][64], spart[RANK_WAVES][64];
     const uint32_t V = counts->n_valid_pairs;
     const uint32_t np = counts->pair_filter_off == 0 && V <= cap ? rank_passes(V) : 0u;
     if (blockIdx.x == 0 && threadIdx.x == 0) counts->sort_passes = np;
@@ -566,28 +577,46 @@ k_pair_rank(const uint64_t *__restrict__ vkeys, const uint32_t *__restrict__ vsl
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t i = blockIdx.x * 64 + lane;
     const uint64_t mine = i < V ? vkeys[i] : 0ull;
-    uint32_t below = 0;
+    uint32_t below = 0, start = 0;
     for (uint32_t t0 = 0; t0 < V; t0 += RANK_TILE) {
         const uint32_t nt = min((uint32_t)RANK_TILE, V - t0);
-        for (uint32_t j = threadIdx.x; j < nt; j += 64 * RANK_WAVES) tile[j] = vkeys[t0 + j];
+        for (uint32_t j = threadIdx.x; j < nt; j += 64 * RANK_WAVES) {
+            tile[j] = vkeys[t0 + j];
+            tcnt[j] = vcnt[t0 + j];
+        }
         __syncthreads();
         const uint32_t per = (nt + RANK_WAVES - 1) / RANK_WAVES;
         const uint32_t q0 = min(nt, wave * per), q1 = min(nt, q0 + per);
-        uint32_t j = q0;
-        for (; j + 4 <= q1; j += 4)
-            below += (tile[j] < mine ? 1u : 0u) + (tile[j + 1] < mine ? 1u : 0u) +
-                     (tile[j + 2] < mine ? 1u : 0u) + (tile[j + 3] < mine ? 1u : 0u);
-        for (; j < q1; j++) below += tile[j] < mine ? 1u : 0u;
+        for (uint32_t j = q0; j < q1; j++) {
+            const bool lt = tile[j] < mine;
+            below += lt ? 1u : 0u;
+            start += lt ? tcnt[j] : 0u;
+        }
         __syncthreads();
     }
     part[wave][lane] = below;
+    spart[wave][lane] = start;
     __syncthreads();
     if (wave == 0 && i < V) {
-        uint32_t r = 0;
+        uint32_t r = 0, st = 0;
 #pragma unroll
-        for (int q = 0; q < RANK_WAVES; q++) r += part[q][lane];
-        slot_rank[vslots[i]] = r;
+        for (int q = 0; q < RANK_WAVES; q++) {
+            r += part[q][lane];
+            st += spart[q][lane];
+        }
+        const uint32_t slot = vslots[i], c = vcnt[i];
+        slot_rank[slot] = r;
         rank_keys[r] = ((mine >> 32) << lb) | (mine & 0xFFFFFFFFull);
+        atq_cluster_bounds b = table_bounds(table, slot, c);
+        b.start = st;
+        bounds[r] = b;
+        ext[r] = atq_cluster_extents{st, c};
+        centre[r] = make_float2(bounds_cx(b), bounds_cy(b));
+        if (r + 1 == V) {
+            counts->n_pairs = V;
+            counts->n_clusters = V;
+            counts->n_cluster_points = st + c;
+        }
     }
 }
 
@@ -663,46 +692,58 @@ k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
     if (threadIdx.x == 0) tile_counts[tile] = tot;
 }
 
-namespace {
-struct TileCount {
-    const uint32_t *c;
-    ATQ_DEV uint32_t operator()(uint32_t i) const { return c[i]; }
-};
-struct TileOffset {
-    uint32_t *off;
-    ATQ_DEV void operator()(uint32_t i, uint32_t pre, uint32_t) const { off[i] = pre; }
-};
-}  // namespace
-
-// One work-group per tile: its records moved to where the sort reads them,
-// at the tile's offset from the scan of the tile counts (4 records' loads
-// in flight per thread).
+// One work-group per GATHER_TILES consecutive tiles: the records before them
+// (the sum of the earlier tiles' counts, read by all 256 threads), then the
+// tiles' records, in tile order, moved as one run to where the sort reads
+// them (4 records' loads in flight per thread); the last work-group sets
+// n_boundary.
+constexpr uint32_t GATHER_TILES = 8;
 __global__ void __launch_bounds__(256)
 k_bnd_gather(const uint64_t *__restrict__ tile_keys, const atq_boundary_point *__restrict__ tile_points,
-             const uint32_t *__restrict__ tile_counts, const uint32_t *__restrict__ tile_off,
-             const atq_counts *counts, RsDyn out) {
-    const uint32_t tile = blockIdx.x, n = tile_counts[tile], off = tile_off[tile];
+             const uint32_t *__restrict__ tile_counts, uint32_t n_tiles, atq_counts *counts,
+             RsDyn out) {
+    __shared__ uint32_t lds4[4];
+    __shared__ uint32_t cnt[GATHER_TILES], pre[GATHER_TILES + 1];
+    const uint32_t t0 = blockIdx.x * GATHER_TILES, nt = min(n_tiles - t0, GATHER_TILES);
+    if (threadIdx.x < GATHER_TILES) cnt[threadIdx.x] = threadIdx.x < nt ? tile_counts[t0 + threadIdx.x] : 0u;
+    uint32_t before = 0;
+    for (uint32_t i = threadIdx.x; i < t0; i += 256) before += tile_counts[i];
+    const uint32_t off = block_reduce_sum<uint32_t>(before, lds4);  // (its barriers publish cnt)
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t q = 0; q < GATHER_TILES; q++) {
+            pre[q] = run;
+            run += cnt[q];
+        }
+        pre[GATHER_TILES] = run;
+        if (t0 + nt == n_tiles) counts->n_boundary = off + run;
+    }
+    __syncthreads();
+    const uint32_t total = pre[GATHER_TILES];
     // the rank passes end in Y; the full-key sort reads X
     const uint32_t np = counts->sort_passes;
     const bool to_y = np != 0 && (np & 1u) == 0;
     uint64_t *keys = to_y ? out.ky : out.kx;
     atq_boundary_point *points = reinterpret_cast<atq_boundary_point *>(to_y ? out.vy : out.vx);
-    const size_t base = (size_t)tile * (4 * BND_TILE);
-    for (uint32_t i0 = threadIdx.x; i0 < n; i0 += 4 * 256) {
+    for (uint32_t i0 = threadIdx.x; i0 < total; i0 += 4 * 256) {
         uint64_t k[4];
         atq_boundary_point p[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const uint32_t i = i0 + 256 * q;
-            if (i < n) {
-                k[q] = tile_keys[base + i];
-                p[q] = tile_points[base + i];
+            if (i < total) {
+                uint32_t tq = 0;
+#pragma unroll
+                for (uint32_t j = 1; j < GATHER_TILES; j++) tq += i >= pre[j] ? 1u : 0u;
+                const size_t src = (size_t)(t0 + tq) * (4 * BND_TILE) + (i - pre[tq]);
+                k[q] = tile_keys[src];
+                p[q] = tile_points[src];
             }
         }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const uint32_t i = i0 + 256 * q;
-            if (i < n) {
+            if (i < total) {
                 keys[off + i] = k[q];
                 points[off + i] = p[q];
             }
@@ -743,12 +784,14 @@ int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const
         uint64_t *next = sc.flags + (size_t)((sc.calls + 1) & 1) * sc.flag_count;
         sc.calls++;
         k_pair_valid<<<atq_div_up(slots, PV_TILE), 256, 0, s>>>(
-            table, (uint32_t)w, (uint32_t)h, max_count, rk.slot_rank, rk.vkeys, rk.vslots, rk.cap,
-            counts, cur, next, sc.flag_count);
+            table, (uint32_t)w, (uint32_t)h, max_count, rk.slot_rank, rk.vkeys, rk.vslots, rk.vcnt,
+            rk.cap, counts, cur, next, sc.flag_count);
     }
-    k_pair_rank<<<rk.cap / 64, 64 * RANK_WAVES, 0, s>>>(rk.vkeys, rk.vslots, counts, rk.cap, label_bits,
-                                            rk.slot_rank, rk.rank_keys);
-    uint32_t *tile_counts = rk.tile_counts, *tile_off = rk.tile_counts + tiles;
+    k_pair_rank<<<rk.cap / 64, 64 * RANK_WAVES, 0, s>>>(table, rk.vkeys, rk.vslots, rk.vcnt, counts,
+                                                         rk.cap, label_bits, rk.slot_rank,
+                                                         rk.rank_keys, out.cl.bounds, out.cl.ext,
+                                                         out.cl.centre);
+    uint32_t *tile_counts = rk.tile_counts;
     if (fx)
         k_bnd_filter<true><<<xcd_grid(tiles), 256, 0, s>>>(
             labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size, label_bits, table,
@@ -757,11 +800,8 @@ int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const
         k_bnd_filter<false><<<xcd_grid(tiles), 256, 0, s>>>(
             labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size, label_bits, table,
             rk.slot_rank, counts, tiles, rk.tile_keys, rk.tile_points, tile_counts);
-    int r = device_scan(s, nullptr, tiles, TileCount{tile_counts}, TileOffset{tile_off}, sc,
-                        &counts->n_boundary);
-    if (r) return r;
-    k_bnd_gather<<<tiles, 256, 0, s>>>(rk.tile_keys, rk.tile_points, tile_counts, tile_off, counts,
-                                       out);
+    k_bnd_gather<<<atq_div_up(tiles, GATHER_TILES), 256, 0, s>>>(rk.tile_keys, rk.tile_points,
+                                                                 tile_counts, tiles, counts, out);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 

@@ -71,54 +71,8 @@ struct ExtentOut {
 
 }  // namespace
 
-// One work-group per segment: reduce ClusterBounds and apply
-// ValidBlobFilter. Each thread loads 4 records per round before using them.
-struct BoundsAcc {
-    uint32_t xmin, xmax, ymin, ymax;
-    int32_t gxs, gys;
-    int64_t pg;
-    ATQ_DEV void init() {
-        xmin = ymin = 0xFFFF;
-        xmax = ymax = 0;
-        gxs = gys = 0;
-        pg = 0;
-    }
-    ATQ_DEV void add(atq_boundary_point p, bool fx) {
-        const uint32_t xx = x_with_dx(p.packed_x, fx), yy = y_with_dy(p.packed_x, p.packed_y, fx);
-        const int gx = grad_x(p.packed_x, p.packed_y, fx), gy = grad_y(p.packed_x, p.packed_y, fx);
-        xmin = min(xmin, xx);
-        xmax = max(xmax, xx);
-        ymin = min(ymin, yy);
-        ymax = max(ymax, yy);
-        gxs += gx;
-        gys += gy;
-        pg += (int64_t)xx * gx + (int64_t)yy * gy;
-    }
-    ATQ_DEV void merge(const BoundsAcc &o) {
-        xmin = min(xmin, o.xmin);
-        xmax = max(xmax, o.xmax);
-        ymin = min(ymin, o.ymin);
-        ymax = max(ymax, o.ymax);
-        gxs += o.gxs;
-        gys += o.gys;
-        pg += o.pg;
-    }
-    ATQ_DEV void wave_reduce() {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            BoundsAcc t;
-            t.xmin = __shfl_xor(xmin, o, 64);
-            t.xmax = __shfl_xor(xmax, o, 64);
-            t.ymin = __shfl_xor(ymin, o, 64);
-            t.ymax = __shfl_xor(ymax, o, 64);
-            t.gxs = __shfl_xor(gxs, o, 64);
-            t.gys = __shfl_xor(gys, o, 64);
-            t.pg = __shfl_xor(pg, o, 64);
-            merge(t);
-        }
-    }
-};
-
+// One work-group per segment: reduce ClusterBounds (BoundsAcc) and apply
+// ValidBlobFilter. Each thread loads 8 records per round before using them.
 constexpr int SB_NT = 1024, SB_IPT = 8;  // k_seg_bounds: a segment's records per round
 
 // FX: fixed mode (both half-pixel bits decoded).
@@ -247,9 +201,15 @@ struct RunOut {
 
 // Angle sort keys: slope << 32 | position in the cluster, so equal slopes
 // keep the input order (the reference's sort, made stable).
+// The angle sort's key of point j of the cluster at `start`: its slope's
+// order-preserving image over its position. When cidx is set it also records
+// the point's cluster index (every point's key is generated exactly once,
+// by the chunk sort, and a cluster's points keep their positions' range).
 struct AngleKey {
     const atq_cluster_point *in;
-    ATQ_DEV uint64_t operator()(uint32_t, uint32_t start, uint32_t j) const {
+    uint32_t *cidx;
+    ATQ_DEV uint64_t operator()(uint32_t c, uint32_t start, uint32_t j) const {
+        if (cidx) cidx[start + j] = c;
         return ((uint64_t)float_key(in[start + j].slope) << 32) | j;
     }
 };
@@ -545,8 +505,8 @@ int angle_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out
               const atq_cluster_extents *ext, const atq_counts *counts, uint32_t cap_cl,
               uint32_t max_cluster, const uint8_t *grey, uint32_t w, uint32_t h,
               atq_line_fit_point *lf, uint64_t *keys0, uint64_t *keys1, int64_t *slot_sums,
-              SegLong lng) {
-    int r = seg_sort(s, AngleKey{in}, AngleFinish<FX>{in, out, grey, w, h, lf}, ext,
+              SegLong lng, uint32_t *cidx) {
+    int r = seg_sort(s, AngleKey{in, cidx}, AngleFinish<FX>{in, out, grey, w, h, lf}, ext,
                      &counts->n_clusters, cap_cl, max_cluster, keys0, keys1, lng);
     if (r || max_cluster <= SEG_CHUNK) return r;
     const uint32_t max_chunks = seg_max_chunks(max_cluster);
@@ -564,11 +524,11 @@ int launch_angle_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_poi
                      const atq_cluster_extents *ext, const atq_counts *counts, uint32_t cap_cl,
                      uint32_t max_cluster, const uint8_t *grey, uint32_t w, uint32_t h,
                      atq_line_fit_point *lf, uint64_t *keys0, uint64_t *keys1, int64_t *slot_sums,
-                     SegLong lng, bool fx) {
+                     SegLong lng, bool fx, uint32_t *cidx) {
     return fx ? angle_fit<true>(s, in, out, ext, counts, cap_cl, max_cluster, grey, w, h, lf,
-                                keys0, keys1, slot_sums, lng)
+                                keys0, keys1, slot_sums, lng, cidx)
               : angle_fit<false>(s, in, out, ext, counts, cap_cl, max_cluster, grey, w, h, lf,
-                                 keys0, keys1, slot_sums, lng);
+                                 keys0, keys1, slot_sums, lng, cidx);
 }
 
 int launch_cluster_fit(hipStream_t s, const atq_cluster_point *in, const atq_cluster_extents *ext,

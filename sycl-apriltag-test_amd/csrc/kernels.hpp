@@ -19,6 +19,10 @@ struct FrontFrame {
     uint32_t size_cap;     // sizes saturate here (CCL)
     int32_t min_diff;
     uint32_t channels;     // 1 grey; 3 RGB or 4 RGBA, converted to grey on load
+    // words the threshold kernel zeroes for the later stages (16-byte
+    // aligned; may be null), except clear[clear_idx] = clear_val
+    uint32_t *clear;
+    uint32_t clear_words, clear_idx, clear_val;
 };
 constexpr int FRONT_MAX_FRAMES = 16;
 struct FrontTable {
@@ -71,14 +75,15 @@ struct PairTable {
     uint32_t mask;
 };
 // Buffers of the detector's boundary pass: per table slot the verdict /
-// rank of its pair, the listed valid pairs (key min << 32 | max, slot),
-// rank -> packed key (cap pairs at most, a multiple of 64), the per-tile
+// rank of its pair, the listed valid pairs (key min << 32 | max, slot,
+// record count), rank -> packed key (cap pairs at most, a multiple of 64), the per-tile
 // record regions (4 * w * h entries) and the tiles' counts and offsets (2
 // per 1024-pixel tile).
 struct PairRanks {
     uint32_t *slot_rank;
     uint64_t *vkeys;
     uint32_t *vslots;
+    uint32_t *vcnt;
     uint64_t *rank_keys;
     uint32_t cap;
     uint64_t *tile_keys;
@@ -89,7 +94,9 @@ struct RsDyn;  // sort.hpp
 // Records of the pairs that pass ValidBlobFilter, in the reference's order,
 // keyed for rank_sort (rank keys, or packed keys ((min << lb | max) << 2 |
 // plane) when counts->sort_passes is 0), written where rank_sort reads them;
-// counts->n_boundary, n_valid_pairs and sort_passes set. sc must be the
+// counts->n_boundary, n_valid_pairs and sort_passes set; in rank mode also
+// the clusters' bounds and extents (out.cl) and counts n_pairs, n_clusters
+// and n_cluster_points. sc must be the
 // look-back form (two scans). fx: fixed mode (column 0 local, both
 // half-pixel bits decoded; the labels and sizes must come from the
 // fixed-mode CCL).

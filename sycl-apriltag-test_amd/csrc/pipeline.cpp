@@ -141,6 +141,8 @@ size_t carve_detector(atq_detector *d, char *base) {
     d->slot_rank = carve_one<uint32_t>(base, off, c.pair_slots);
     d->vkeys = carve_one<uint64_t>(base, off, rank_cap(c));
     d->vslots = carve_one<uint32_t>(base, off, rank_cap(c));
+    d->vcnt = carve_one<uint32_t>(base, off, rank_cap(c));
+    d->centres = carve_one<float2>(base, off, rank_cap(c));
     d->rank_keys = carve_one<uint64_t>(base, off, rank_cap(c));
     d->scan_flags = carve_one<uint64_t>(base, off, 2 * (size_t)scan_flag_count(c));
     d->pair_keys = carve_one<uint64_t>(base, off, c.pair_slots);
@@ -194,6 +196,15 @@ FrontFrame front_frame(const atq_detector *d, const uint8_t *gray_in, size_t w_i
     fr.size_cap = (uint32_t)std::max(d->params.min_component_size, 1);
     fr.min_diff = d->params.min_white_black_diff;
     fr.channels = (uint32_t)d->channels;
+    // one fill for the later stages: the scan flags, the pair table, the
+    // long-segment counts and the counts (carved contiguously), with the
+    // counts' key width set
+    uint32_t *zero_begin = reinterpret_cast<uint32_t *>(d->scan_flags);
+    fr.clear = zero_begin;
+    fr.clear_words = (uint32_t)((reinterpret_cast<char *>(d->counts + 1) -
+                                 reinterpret_cast<char *>(zero_begin)) / 4);
+    fr.clear_idx = (uint32_t)(reinterpret_cast<uint32_t *>(&d->counts->key_label_bits) - zero_begin);
+    fr.clear_val = (uint32_t)radix_plan(d->w * d->h, 2).label_bits;
     return fr;
 }
 
@@ -213,14 +224,9 @@ int run_back(atq_detector *d, hipStream_t s, const uint8_t *gray_in) {
     sc.flags = d->scan_flags;
     sc.flag_count = scan_flag_count(d->caps);
     sc.error = &cnt->overflow;
-    // one fill: the scan flags, the pair table, the long-segment counts and
-    // the counts (with the key width set)
+    // the scan flags, the pair table, the long-segment counts and the counts
+    // (with the key width set) were zeroed by this frame's threshold kernel
     const RadixPlan plan = radix_plan(w * h, 2);
-    uint32_t *zero_begin = reinterpret_cast<uint32_t *>(d->scan_flags);
-    const uint32_t zero_words = (uint32_t)((reinterpret_cast<char *>(cnt + 1) -
-                                            reinterpret_cast<char *>(zero_begin)) / 4);
-    const uint32_t lb_word = (uint32_t)(reinterpret_cast<uint32_t *>(&cnt->key_label_bits) - zero_begin);
-    TRY(launch_zero_u32(s, zero_begin, nullptr, zero_words, lb_word, (uint32_t)plan.label_bits));
     // the grey image the line fit samples: the input itself only when it is
     // grey and undecimated
     const uint8_t *grey = f > 1 || d->channels != 1 ? d->gray : gray_in;
@@ -233,38 +239,46 @@ int run_back(atq_detector *d, hipStream_t s, const uint8_t *gray_in) {
     const PairTable table{d->pair_keys, d->pair_cnt,    d->pair_bbox,
                           d->pair_grad, d->pair_moment, d->caps.pair_slots - 1};
     const bool fx = d->params.mode == ATQ_MODE_FIXED;
-    const RsDyn io{d->keys, d->vals, ws.keys_alt, ws.vals_alt, &cnt->sort_passes, d->rank_keys};
+    RsClusters cl{};
+    cl.bounds = d->bounds;
+    cl.ext = d->cext;
+    cl.centre = d->centres;
+    cl.cpts = ws.cpts_tmp;
+    cl.cidx = d->cidx;
+    cl.seg_id = ws.seg_id;
+    cl.seg_start = ws.seg_start;
+    cl.valid = ws.valid;
+    cl.seg_cluster = ws.seg_cluster;
+    cl.seg_bounds = ws.seg_bounds;
+    cl.part = ws.scan;
+    cl.counts = cnt;
+    cl.w = (uint32_t)w;
+    cl.h = (uint32_t)h;
+    cl.fx = fx ? 1u : 0u;
+    const RsDyn io{d->keys, d->vals, ws.keys_alt, ws.vals_alt, &cnt->sort_passes, d->rank_keys, cl};
     // the per-tile record regions live in the cluster-bounds area, which is
-    // free until the sorted records are grouped (12 of its 32 bytes per entry)
+    // free until the fallback's cluster stage (12 of its 32 bytes per entry)
     uint64_t *tile_keys = reinterpret_cast<uint64_t *>(ws.seg_bounds);
-    const PairRanks rk{d->slot_rank, d->vkeys, d->vslots, d->rank_keys, rank_cap(d->caps),
+    const PairRanks rk{d->slot_rank, d->vkeys, d->vslots, d->vcnt, d->rank_keys, rank_cap(d->caps),
                        tile_keys, reinterpret_cast<atq_boundary_point *>(tile_keys + c.cap),
                        ws.tiles};
     TRY(launch_find_boundaries_filtered(s, d->labels, d->sizes, w, h, d->params.min_component_size,
                                         (uint32_t)plan.label_bits, table, cnt, rk, io, sc, fx));
     TRY(mark(1 + ATQ_STAGE_BOUNDARY));
-    // the sorted records end in the alt buffers
+    // the sorted records end in the alt buffers, the clusters' points in
+    // cpts_tmp with their bounds and extents (the angle sort records each
+    // point's cluster index)
     TRY(rank_sort(s, 2 * (uint32_t)plan.label_bits + 2, io, &cnt->n_boundary, c.cap, ws.hist,
                   ws.digit_totals, &d->long_n[2], &cnt->overflow));
     TRY(mark(1 + ATQ_STAGE_KEY_SORT));
-    const bool alt = true;
-    d->sorted_in_alt = alt;
-    const uint64_t *K = alt ? ws.keys_alt : d->keys;
-    const atq_boundary_point *V =
-        reinterpret_cast<const atq_boundary_point *>(alt ? ws.vals_alt : d->vals);
-    TRY(launch_segments(s, K, 2, &cnt->n_boundary, cnt, c.cap, ws.seg_id, ws.seg_start, sc));
-    TRY(launch_bounds_filter(s, V, ws.seg_start, cnt, c.cap, (uint32_t)w, (uint32_t)h, ws.valid,
-                             ws.seg_bounds, ws.seg_cluster, d->bounds, sc, fx));
-    TRY(launch_extents(s, d->bounds, cnt, c.cap_cl, d->cext, sc));
-    TRY(launch_assign(s, V, ws.seg_id, ws.seg_start, ws.seg_cluster, d->bounds, d->cext,
-                      &cnt->n_boundary, c.cap, ws.cpts_tmp, d->cidx, fx));
+    d->sorted_in_alt = true;
     TRY(mark(1 + ATQ_STAGE_CLUSTERS));
     // a valid cluster holds at most 2 * (2w + 2h) points (valid_blob)
     const uint32_t max_cluster = (uint32_t)std::min<size_t>(c.cap, 4 * (w + h));
     uint64_t *ak = ws.angle_keys();
     TRY(launch_angle_fit(s, ws.cpts_tmp, d->cpts, d->cext, cnt, c.cap_cl, max_cluster, grey,
                          (uint32_t)w, (uint32_t)h, d->lf, ak, ak + c.cap, ws.fit_sums,
-                         ws.long_segs(0, &d->long_n[0], c), fx));
+                         ws.long_segs(0, &d->long_n[0], c), fx, d->cidx));
     TRY(mark(1 + ATQ_STAGE_ANGLE_FIT));
     TRY(launch_fit_lines(s, d->lf, d->cidx, d->cext, &cnt->n_cluster_points, c.cap, d->sparse,
                          nullptr, reinterpret_cast<float *>(ws.valid), fx));

@@ -79,8 +79,9 @@ struct atq_detector {
     atq_quad_corners *quads;
     uint32_t *slot_rank;   // per pair-table slot: verdict / rank of its pair
     uint64_t *vkeys;       // the frame's valid pairs: key and slot
-    uint32_t *vslots;
+    uint32_t *vslots, *vcnt;
     uint64_t *rank_keys;   // rank -> packed pair key
+    float2 *centres;       // rank -> the cluster's (cx, cy)
     uint64_t *scan_flags;  // two look-back flag arrays, zeroed before each frame
     uint64_t *pair_keys;   // pair table, zeroed with counts before each frame
     uint32_t *pair_cnt, *pair_bbox;

@@ -64,6 +64,54 @@ ATQ_DEV bool valid_blob(const atq_cluster_bounds &b, uint32_t w, uint32_t h) {
     return !(bounds_dot(b) < 0.0f);
 }
 
+// A ClusterBounds reduction in progress (include/cluster_bounds.h:14-83):
+// doubled-coordinate bounding box, gradient sums and the moment sum.
+struct BoundsAcc {
+    uint32_t xmin, xmax, ymin, ymax;
+    int32_t gxs, gys;
+    int64_t pg;
+    ATQ_DEV void init() {
+        xmin = ymin = 0xFFFF;
+        xmax = ymax = 0;
+        gxs = gys = 0;
+        pg = 0;
+    }
+    ATQ_DEV void add(atq_boundary_point p, bool fx) {
+        const uint32_t xx = x_with_dx(p.packed_x, fx), yy = y_with_dy(p.packed_x, p.packed_y, fx);
+        const int gx = grad_x(p.packed_x, p.packed_y, fx), gy = grad_y(p.packed_x, p.packed_y, fx);
+        xmin = min(xmin, xx);
+        xmax = max(xmax, xx);
+        ymin = min(ymin, yy);
+        ymax = max(ymax, yy);
+        gxs += gx;
+        gys += gy;
+        pg += (int64_t)xx * gx + (int64_t)yy * gy;
+    }
+    ATQ_DEV void merge(const BoundsAcc &o) {
+        xmin = min(xmin, o.xmin);
+        xmax = max(xmax, o.xmax);
+        ymin = min(ymin, o.ymin);
+        ymax = max(ymax, o.ymax);
+        gxs += o.gxs;
+        gys += o.gys;
+        pg += o.pg;
+    }
+    ATQ_DEV void wave_reduce() {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            BoundsAcc t;
+            t.xmin = __shfl_xor(xmin, o, 64);
+            t.xmax = __shfl_xor(xmax, o, 64);
+            t.ymin = __shfl_xor(ymin, o, 64);
+            t.ymax = __shfl_xor(ymax, o, 64);
+            t.gxs = __shfl_xor(gxs, o, 64);
+            t.gys = __shfl_xor(gys, o, 64);
+            t.pg = __shfl_xor(pg, o, 64);
+            merge(t);
+        }
+    }
+};
+
 // include/boundary_point.h:48-61
 ATQ_DEV float calc_theta(uint16_t px, uint16_t py, float cx, float cy, bool fx = false) {
     float dx = (float)x_with_dx(px, fx) - cx;

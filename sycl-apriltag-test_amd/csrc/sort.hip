@@ -20,6 +20,7 @@
 //                 waves' counts are scanned in LDS, and every record is
 //                 written at digit base + chunk offset + wave offset + rank.
 // Records keep their input order within a digit, so the sort is stable.
+#include "records.hpp"
 #include "sort.hpp"
 
 namespace atq {
@@ -135,16 +136,18 @@ struct RsScatterLds {
 
 // Scatters records [b, e) (chunk g of the pass) to their stable positions:
 // digit base (exclusive scan of `totals`) + hist_row[digit] (the digit's
-// count in the chunks before this one) + rank within the chunk. xlat: when
-// not null, keys are (rank << 2 | plane) and leave as
-// (xlat[rank] << 2 | plane).
+// count in the chunks before this one) + rank within the chunk. fin: when
+// not null (the last rank pass), keys are (r << 2 | plane) and leave as
+// (xlat[r] << 2 | plane), and each record also becomes the cluster point of
+// cluster r at its position (src/main.cpp:621-675; its cluster index is
+// recorded by the angle sort).
 template <int BITS>
 ATQ_DEV void rs_scatter_body(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                              uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
                              uint32_t b, uint32_t e, uint32_t shift,
                              const uint32_t *__restrict__ hist_row,
-                             const uint32_t *__restrict__ totals,
-                             const uint64_t *__restrict__ xlat, RsScatterLds<BITS> &L) {
+                             const uint32_t *__restrict__ totals, const RsDyn *fin,
+                             RsScatterLds<BITS> &L) {
     constexpr int B = 1 << BITS;
     constexpr int PER = B / RS_THREADS;  // digits per thread in the base scan
     auto &wc = L.wc;
@@ -215,6 +218,17 @@ ATQ_DEV void rs_scatter_body(const uint64_t *__restrict__ kin, const uint32_t *_
             before = __shfl(before, leader, 64);
             rank[j] = before + (uint32_t)__popcll(m & lt);
         }
+        // the last pass's per-cluster loads, in flight during the offsets below
+        uint64_t xk[RS_ITEMS];
+        float2 cc[RS_ITEMS];
+        if (fin) {
+#pragma unroll
+            for (int j = 0; j < RS_ITEMS; j++) {
+                const uint32_t r = dg[j] != 0xFFFFFFFFu ? (uint32_t)(k[j] >> 2) : 0u;
+                xk[j] = fin->xlat[r];
+                cc[j] = fin->cl.centre[r];
+            }
+        }
         __syncthreads();
         for (int d = t; d < B; d += RS_THREADS) {
             uint32_t run = base_off[d];
@@ -231,7 +245,14 @@ ATQ_DEV void rs_scatter_body(const uint64_t *__restrict__ kin, const uint32_t *_
         for (int j = 0; j < RS_ITEMS; j++) {
             if (dg[j] != 0xFFFFFFFFu) {
                 uint32_t pos = wc[wave][dg[j]] + rank[j];
-                kout[pos] = xlat ? (xlat[k[j] >> 2] << 2) | (k[j] & 3u) : k[j];
+                if (fin) {
+                    kout[pos] = (xk[j] << 2) | (k[j] & 3u);
+                    const uint16_t px = (uint16_t)v[j], py = (uint16_t)(v[j] >> 16);
+                    fin->cl.cpts[pos] = atq_cluster_point{
+                        px, py, calc_theta(px, py, cc[j].x, cc[j].y, fin->cl.fx != 0)};
+                } else {
+                    kout[pos] = k[j];
+                }
                 vout[pos] = v[j];
             }
         }
@@ -337,16 +358,144 @@ ATQ_DEV void rs_grid_sync(uint32_t *bar, uint32_t target, uint32_t *error) {
     __syncthreads();
 }
 
-// The full-key fallback: a stable LSD sort of the packed keys (`passes`
-// digits of BITS bits) from X into Y by work-groups 0 .. G-1 of the last
-// rank-pass kernel (the others exit at once, so these G are resident
-// together), which run it only when *d.passes == 0: frames that need no
-// fallback pay nothing for it. Each pass: chunk histograms, column scans,
-// scatter, separated by grid barriers.
+// Work-group g's share [*b, *e) of n items split over G work-groups.
+ATQ_DEV void rs_share(uint32_t n, uint32_t G, uint32_t g, uint32_t *b, uint32_t *e) {
+    const uint32_t per = (n + G - 1) / G;
+    *b = min(n, g * per);
+    *e = min(n, *b + per);
+}
+
+// Exclusive scan of v over the items [b, e) of work-group g of G (v(i) in
+// {0, 1, ...}); o(i, prefix, value) for each; part: G words; returns the
+// total over all work-groups. Contains grid barriers: every work-group calls
+// it.
+template <typename V, typename O>
+ATQ_DEV uint32_t rs_grid_scan(uint32_t b, uint32_t e, uint32_t G, V v, O o, uint32_t *part,
+                              uint32_t *bar, uint32_t &gen, uint32_t *error, uint32_t *lds4) {
+    const uint32_t g = blockIdx.x, t = threadIdx.x;
+    uint32_t own = 0;
+    for (uint32_t i = b + t; i < e; i += RS_THREADS) own += v(i);
+    own = block_reduce_sum<uint32_t>(own, lds4);
+    if (t == 0) part[g] = own;
+    rs_grid_sync(bar, ++gen * G, error);
+    uint32_t pre = 0, all = 0;
+    for (uint32_t q = t; q < G; q += RS_THREADS) {
+        const uint32_t x = part[q];
+        if (q < g) pre += x;
+        all += x;
+    }
+    pre = block_reduce_sum<uint32_t>(pre, lds4);
+    all = block_reduce_sum<uint32_t>(all, lds4);
+    for (uint32_t i0 = b; i0 < e; i0 += RS_THREADS) {
+        const uint32_t i = i0 + t;
+        const uint32_t x = i < e ? v(i) : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan<uint32_t>(x, lds4, &tot);
+        if (i < e) o(i, pre + ex, x);
+        pre += tot;
+    }
+    return all;
+}
+
+// The fallback's cluster stage over the n sorted records in Y (packed keys):
+// segments of equal pair (src/main.cpp:571-585), each segment's
+// ClusterBounds and ValidBlobFilter (src/main.cpp:604-609), the valid ones
+// numbered in key order with their extents (src/main.cpp:700-717), and each
+// of their records placed as a cluster point with its slope
+// (src/main.cpp:621-675).
+ATQ_DEV void rs_fallback_clusters(const RsDyn &d, uint32_t n, uint32_t G, uint32_t *bar,
+                                  uint32_t &gen, uint32_t *error, uint32_t *lds4,
+                                  BoundsAcc *wpart) {
+    const RsClusters &c = d.cl;
+    const uint32_t g = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const bool fx = c.fx != 0;
+    const uint64_t *K = d.ky;
+    const atq_boundary_point *P = reinterpret_cast<const atq_boundary_point *>(d.vy);
+    uint32_t b, e;
+    // segments
+    rs_share(n, G, g, &b, &e);
+    auto head = [&](uint32_t i) -> uint32_t { return i == 0 || (K[i] >> 2) != (K[i - 1] >> 2); };
+    const uint32_t S = rs_grid_scan(b, e, G, head, [&](uint32_t i, uint32_t pre, uint32_t v) {
+        c.seg_id[i] = pre + v - 1;
+        if (v) c.seg_start[pre] = i;
+        if (i + 1 == n) c.seg_start[pre + v] = n;
+    }, c.part, bar, gen, error, lds4);
+    if (g == 0 && t == 0) c.counts->n_pairs = S;
+    rs_grid_sync(bar, ++gen * G, error);
+    // bounds and filter, one work-group per segment
+    const uint64_t max_px = 2ull * (2ull * c.w + 2ull * c.h);
+    for (uint32_t s = g; s < S; s += G) {
+        const uint32_t sb = c.seg_start[s], se = c.seg_start[s + 1], cnt = se - sb;
+        if (cnt < 24 || cnt > max_px) {
+            if (t == 0) c.valid[s] = 0;
+            continue;
+        }
+        BoundsAcc acc;
+        acc.init();
+        for (uint32_t i = sb + t; i < se; i += RS_THREADS) acc.add(P[i], fx);
+        acc.wave_reduce();
+        if (lane == 0) wpart[wave] = acc;
+        __syncthreads();
+        if (t == 0) {
+            for (int q = 1; q < RS_THREADS / 64; q++) acc.merge(wpart[q]);
+            atq_cluster_bounds cb;
+            cb.x_min = (uint16_t)acc.xmin;
+            cb.x_max = (uint16_t)acc.xmax;
+            cb.y_min = (uint16_t)acc.ymin;
+            cb.y_max = (uint16_t)acc.ymax;
+            cb.gx_sum = acc.gxs;
+            cb.gy_sum = acc.gys;
+            cb.pxgx_pygy_sum = acc.pg;
+            cb.start = sb;
+            cb.count = cnt;
+            const bool ok = valid_blob(cb, c.w, c.h);
+            c.valid[s] = ok;
+            if (ok) c.seg_bounds[s] = cb;
+        }
+        __syncthreads();
+    }
+    rs_grid_sync(bar, ++gen * G, error);
+    // valid segments numbered in key order
+    rs_share(S, G, g, &b, &e);
+    const uint32_t C = rs_grid_scan(b, e, G, [&](uint32_t s) { return c.valid[s]; },
+        [&](uint32_t s, uint32_t pre, uint32_t v) {
+            c.seg_cluster[s] = v ? pre : 0xFFFFFFFFu;
+            if (v) c.bounds[pre] = c.seg_bounds[s];
+        }, c.part + G, bar, gen, error, lds4);
+    if (g == 0 && t == 0) c.counts->n_clusters = C;
+    rs_grid_sync(bar, ++gen * G, error);
+    // extents
+    rs_share(C, G, g, &b, &e);
+    const uint32_t NP = rs_grid_scan(b, e, G, [&](uint32_t q) { return c.bounds[q].count; },
+        [&](uint32_t q, uint32_t pre, uint32_t v) { c.ext[q] = atq_cluster_extents{pre, v}; },
+        c.part + 2 * G, bar, gen, error, lds4);
+    if (g == 0 && t == 0) c.counts->n_cluster_points = NP;
+    rs_grid_sync(bar, ++gen * G, error);
+    // cluster points
+    for (uint32_t i = g * RS_THREADS + t; i < n; i += G * RS_THREADS) {
+        const uint32_t s = c.seg_id[i], cl = c.seg_cluster[s];
+        if (cl == 0xFFFFFFFFu) continue;
+        const uint32_t pos = c.ext[cl].start + (i - c.seg_start[s]);
+        const atq_boundary_point p = P[i];
+        const atq_cluster_bounds cb = c.bounds[cl];
+        c.cpts[pos] = atq_cluster_point{p.packed_x, p.packed_y,
+                                        calc_theta(p.packed_x, p.packed_y, bounds_cx(cb),
+                                                   bounds_cy(cb), fx)};
+        c.cidx[pos] = cl;
+    }
+}
+
+// The fallback: a stable LSD sort of the packed keys (`passes` digits of
+// BITS bits) from X into Y, then the cluster stage (rs_fallback_clusters),
+// by work-groups 0 .. G-1 of the last rank-pass kernel (the others exit at
+// once, so these G are resident together), which run it only when
+// *d.passes == 0: frames that need no fallback pay nothing for it. Each
+// pass: chunk histograms, column scans, scatter, separated by grid
+// barriers.
 template <int BITS>
 ATQ_DEV void rs_full_body(const RsDyn &d, const uint32_t *count, uint32_t cap, uint32_t passes,
                           uint32_t G, uint32_t *hist, uint32_t *totals, uint32_t *bar,
-                          uint32_t *error, RsScatterLds<BITS> &L) {
+                          uint32_t *error, RsScatterLds<BITS> &L, BoundsAcc *wpart) {
     constexpr int B = 1 << BITS;
     const uint32_t n = min(*count, cap), g = blockIdx.x;
     uint32_t chunk = (n + G - 1) / G;
@@ -381,7 +530,9 @@ ATQ_DEV void rs_full_body(const RsDyn &d, const uint32_t *count, uint32_t cap, u
             d.ky[i] = d.kx[i];
             d.vy[i] = d.vx[i];
         }
+        rs_grid_sync(bar, ++gen * G, error);
     }
+    rs_fallback_clusters(d, n, G, bar, gen, error, L.lds4, wpart);
 }
 
 // The last launched rank pass also hosts the full-key fallback (its
@@ -392,10 +543,11 @@ k_rs_scatter_dyn(RsDyn d, const uint32_t *count, uint32_t cap, uint32_t pass,
                  uint32_t *bar, uint32_t *error) {
     constexpr int B = 1 << RK_BITS;
     __shared__ RsScatterLds<RK_BITS> L;
+    __shared__ BoundsAcc wpart[RS_THREADS / 64];
     if (*d.passes == 0) {
         if (full_passes && blockIdx.x < RS_FULL_GROUPS)
             rs_full_body<RK_BITS>(d, count, cap, full_passes, RS_FULL_GROUPS, hist, totals, bar,
-                                  error, L);
+                                  error, L, wpart);
         return;
     }
     const uint64_t *ki;
@@ -410,7 +562,7 @@ k_rs_scatter_dyn(RsDyn d, const uint32_t *count, uint32_t cap, uint32_t pass,
     if (blockIdx.x >= G) return;
     const uint32_t b = blockIdx.x * chunk, e = min(n, b + chunk);
     rs_scatter_body<RK_BITS>(ki, vi, ko, vo, b, e, pass * RK_BITS, hist + (size_t)blockIdx.x * B,
-                             totals, last ? d.xlat : nullptr, L);
+                             totals, last ? &d : nullptr, L);
 }
 
 int rank_sort(hipStream_t s, uint32_t full_bits, const RsDyn &d, const uint32_t *count,

@@ -26,12 +26,30 @@ int radix_sort_packed(hipStream_t s, const RadixPlan &p, uint64_t *keys, uint32_
 // The detector's sort of its boundary records (sort.hip): keys (r << 2 |
 // plane), r the rank of the record's cluster pair among the frame's valid
 // pairs, sorted with RK_BITS-bit digits in *passes (1..RK_MAX_PASSES) passes
-// from X or Y (see rank_sort) into Y, the last pass turning r back into the
-// packed pair key xlat[r]; *passes == 0 selects the full-key fallback (packed
-// keys in X, sorted into Y by one persistent kernel).
+// from X or Y (see rank_sort) into Y. Every kept record belongs to a valid
+// pair, so cluster r is rank r: the last pass turns r back into the packed
+// pair key xlat[r] and also writes each record as the cluster point of
+// cluster r (slope about centre[r], which the ranking wrote with the
+// bounds and extents). *passes == 0 selects the fallback: packed keys in X, sorted
+// into Y by one persistent kernel that then groups the records into
+// segments, reduces and filters them and assigns the cluster points itself.
 constexpr int RK_BITS = 11;
 constexpr uint32_t RK_MAX_PASSES = 2;
-constexpr unsigned RS_FULL_GROUPS = 128;  // work-groups of the full-key fallback
+constexpr unsigned RS_FULL_GROUPS = 128;  // work-groups of the fallback
+// Where the sort leaves the clusters (src/main.cpp:571-675), and the
+// fallback's segment tables (cap entries each).
+struct RsClusters {
+    atq_cluster_bounds *bounds;
+    atq_cluster_extents *ext;
+    float2 *centre;           // rank mode: each cluster's (cx, cy) from its bounds
+    atq_cluster_point *cpts;  // records in cluster order, with their slopes
+    uint32_t *cidx;           // fallback: cluster of each
+    uint32_t *seg_id, *seg_start, *valid, *seg_cluster;
+    atq_cluster_bounds *seg_bounds;
+    uint32_t *part;  // 3 * RS_FULL_GROUPS
+    atq_counts *counts;
+    uint32_t w, h, fx;
+};
 struct RsDyn {
     uint64_t *kx;
     uint32_t *vx;
@@ -39,6 +57,7 @@ struct RsDyn {
     uint32_t *vy;
     const uint32_t *passes;  // device: rank passes, 0 = full-key fallback
     const uint64_t *xlat;    // rank -> packed pair key
+    RsClusters cl;
 };
 // Rank passes for v valid pairs: ranks below v take bit_length(v - 1) bits,
 // plus 2 for the plane; 0 when that needs more than RK_MAX_PASSES digits.
@@ -49,9 +68,9 @@ __host__ __device__ inline uint32_t rank_passes(uint32_t v) {
     return np <= RK_MAX_PASSES ? np : 0u;
 }
 // Launches RK_MAX_PASSES rank passes (idle beyond *passes), the last of
-// which also holds the full-key fallback over full_bits-bit packed keys
-// (idle unless *passes == 0). `bar` is a zeroed word (the fallback's grid
-// barrier); `error` gets OVF_SCAN if a barrier wait gives up.
+// which also holds the fallback over full_bits-bit packed keys (idle unless
+// *passes == 0). `bar` is a zeroed word (the fallback's grid barrier);
+// `error` gets OVF_SCAN if a barrier wait gives up.
 int rank_sort(hipStream_t s, uint32_t full_bits, const RsDyn &d, const uint32_t *count,
               uint32_t cap, uint32_t *hist, uint32_t *totals, uint32_t *bar, uint32_t *error);
 
