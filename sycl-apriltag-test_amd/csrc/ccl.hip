@@ -466,6 +466,7 @@ __global__ void __launch_bounds__(256)
 k_ccl_resolve(FrontTable tab) {
     for (uint32_t fi = 0; fi < tab.n; fi++) {
         const FrontFrame &F = tab.fr[fi];
+        front_clear(F);  // the later stages' tables and counts
         uint32_t *__restrict__ parent = F.parent;
         uint32_t *__restrict__ sizes = F.sizes;
         const uint8_t *__restrict__ thr = F.thr;
@@ -788,6 +789,7 @@ __global__ void __launch_bounds__(256)
 k_fx_final(FrontTable tab) {
     for (uint32_t fi = 0; fi < tab.n; fi++) {
         const FrontFrame &F = tab.fr[fi];
+        front_clear(F);  // the later stages' tables and counts
         const uint32_t *__restrict__ parent = F.parent;
         const uint32_t quads = F.w * F.h / 4;
         for (uint32_t q = blockIdx.x * 256 + threadIdx.x; q < quads; q += gridDim.x * 256) {

@@ -19,11 +19,27 @@ struct FrontFrame {
     uint32_t size_cap;     // sizes saturate here (CCL)
     int32_t min_diff;
     uint32_t channels;     // 1 grey; 3 RGB or 4 RGBA, converted to grey on load
-    // words the threshold kernel zeroes for the later stages (16-byte
+    // words the last CCL kernel zeroes for the later stages (16-byte
     // aligned; may be null), except clear[clear_idx] = clear_val
     uint32_t *clear;
     uint32_t clear_words, clear_idx, clear_val;
 };
+
+// The frame's clear (FrontFrame::clear), grid-stride, 16 bytes per store.
+ATQ_DEV void front_clear(const FrontFrame &F) {
+    if (!F.clear) return;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; 4 * i < F.clear_words;
+         i += gridDim.x * blockDim.x) {
+        uint32_t v[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) v[j] = 4 * i + j == F.clear_idx ? F.clear_val : 0u;
+        if (4 * i + 4 <= F.clear_words) {
+            *reinterpret_cast<uint4 *>(F.clear + 4 * i) = make_uint4(v[0], v[1], v[2], v[3]);
+        } else {
+            for (uint32_t j = 0; 4 * i + j < F.clear_words; j++) F.clear[4 * i + j] = v[j];
+        }
+    }
+}
 constexpr int FRONT_MAX_FRAMES = 16;
 struct FrontTable {
     FrontFrame fr[FRONT_MAX_FRAMES];

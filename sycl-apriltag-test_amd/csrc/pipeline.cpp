@@ -225,7 +225,7 @@ int run_back(atq_detector *d, hipStream_t s, const uint8_t *gray_in) {
     sc.flag_count = scan_flag_count(d->caps);
     sc.error = &cnt->overflow;
     // the scan flags, the pair table, the long-segment counts and the counts
-    // (with the key width set) were zeroed by this frame's threshold kernel
+    // (with the key width set) were zeroed by this frame's last CCL kernel
     const RadixPlan plan = radix_plan(w * h, 2);
     // the grey image the line fit samples: the input itself only when it is
     // grey and undecimated
@@ -731,31 +731,32 @@ struct atq_batch {
 
 namespace {
 
-// Frames large enough to fill the device by themselves (BATCH_BIG_PX
-// decimated pixels and up) share internal stream 0, so they run one after
-// another instead of competing for the caches; the other frames go
-// round-robin to the remaining streams and fill the gaps of the large ones'
-// latency-bound kernels (all streams round-robin when no frame is large).
-constexpr size_t BATCH_BIG_PX = 8u << 20;
-
+// Frames go to the internal streams longest first, each to the stream with
+// the least work so far (work of a frame: its decimated pixels plus 0.5 M
+// for the per-frame launches), which spreads the large frames over the
+// streams and fills the gaps of their latency-bound kernels with the small
+// ones (measured on the mixed 16-frame batch: slightly ahead of keeping the
+// frames above 8 Mpx on one stream; 4 streams ahead of 2, 6 or 8).
 void assign_streams(atq_batch *b, const size_t *w_in, const size_t *h_in,
                     const int32_t *decimate) {
     const size_t n = b->dets.size();
-    std::vector<bool> big(n);
-    bool any_big = false;
+    const int ns = (int)std::min<size_t>(atq_batch::N_STREAMS, n);
+    std::vector<double> cost(n), load(ns, 0.0);
+    std::vector<size_t> order(n);
     for (size_t i = 0; i < n; i++) {
         const int f = decimate ? std::max(decimate[i], 1) : 1;
-        big[i] = decimated(w_in[i], f) * decimated(h_in[i], f) >= BATCH_BIG_PX;
-        any_big = any_big || big[i];
+        cost[i] = (double)(decimated(w_in[i], f) * decimated(h_in[i], f)) + 0.5e6;
+        order[i] = i;
     }
-    const int ns = (int)std::min<size_t>(atq_batch::N_STREAMS, n);
-    const int first = any_big && ns > 1 ? 1 : 0;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t c) { return cost[a] > cost[c]; });
     b->stream_of.assign(n, 0);
-    int next = first;
-    for (size_t i = 0; i < n; i++) {
-        if (big[i] && first) continue;
-        b->stream_of[i] = next;
-        next = next + 1 < ns ? next + 1 : first;
+    for (size_t i : order) {
+        int best = 0;
+        for (int k = 1; k < ns; k++)
+            if (load[k] < load[best]) best = k;
+        b->stream_of[i] = best;
+        load[best] += cost[i];
     }
 }
 

@@ -271,19 +271,6 @@ k_threshold(FrontTable tab) {
                             (reinterpret_cast<uintptr_t>(F.in) & 15) == 0;
         const bool vec_out = (F.w & 15) == 0 && (reinterpret_cast<uintptr_t>(F.thr) & 15) == 0 &&
                              (reinterpret_cast<uintptr_t>(F.gray) & 15) == 0;
-        if (F.clear) {  // the later stages' tables and counts, 16 bytes per store
-            for (uint32_t i = blockIdx.x * 256 + threadIdx.x; 4 * i < F.clear_words;
-                 i += gridDim.x * 256) {
-                uint32_t v[4];
-#pragma unroll
-                for (int j = 0; j < 4; j++) v[j] = 4 * i + j == F.clear_idx ? F.clear_val : 0u;
-                if (4 * i + 4 <= F.clear_words) {
-                    *reinterpret_cast<uint4 *>(F.clear + 4 * i) = make_uint4(v[0], v[1], v[2], v[3]);
-                } else {
-                    for (uint32_t j = 0; 4 * i + j < F.clear_words; j++) F.clear[4 * i + j] = v[j];
-                }
-            }
-        }
         for (uint32_t it = blockIdx.x; it < xcd_grid(n_regions); it += gridDim.x) {
             const uint32_t region = xcd_tile(it, n_regions);
             if (region < n_regions)
