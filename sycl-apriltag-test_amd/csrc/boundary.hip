@@ -620,21 +620,31 @@ k_pair_rank(PairTable table, const uint64_t *__restrict__ vkeys,
     }
 }
 
+// Tiles per group of the gather's running sums.
+constexpr uint32_t GATHER_GROUP = 64;
+__host__ __device__ inline uint32_t tile_groups(uint32_t n_tiles) {
+    return (n_tiles + GATHER_GROUP - 1) / GATHER_GROUP;
+}
+
 // One work-group per 1024-pixel tile (XCD-contiguous tile order): the
-// tile's records of valid pairs, keyed for the sort, to its own region of
-// 4 * 1024 entries in raster order, and their number to tile_counts.
+// tile's records of valid pairs, keyed for the sort (the rank r alone: the
+// records reach the sort plane by plane, so a stable sort by r gives the
+// reference's (pair, plane, y, x) order; fallback: the packed pair key with
+// the plane), densely to the tile's region, plane by plane, each plane in
+// raster order; per plane, the counts to tile_counts[plane][tile] and added
+// to group_sums[plane][tile / GATHER_GROUP] (zero at launch).
 template <bool FX>
 __global__ void __launch_bounds__(256)
 k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes, uint32_t w,
              uint32_t h, uint32_t min_size, uint32_t lb, PairTable table,
              const uint32_t *__restrict__ slot_rank, const atq_counts *counts, uint32_t n_tiles,
              uint64_t *__restrict__ tile_keys, atq_boundary_point *__restrict__ tile_points,
-             uint32_t *__restrict__ tile_counts) {
+             uint32_t *__restrict__ tile_counts, uint32_t *__restrict__ group_sums) {
     const uint32_t tile = xcd_tile(blockIdx.x, n_tiles);
     if (tile >= n_tiles) return;
     __shared__ uint64_t lk[TILE_SLOTS];
     __shared__ uint32_t lc[TILE_SLOTS];  // the pair's rank, RANK_PENDING or RANK_NONE
-    __shared__ uint32_t lds4[4];
+    __shared__ uint64_t lds8[4];
     for (int i = threadIdx.x; i < TILE_SLOTS; i += 256) {
         lk[i] = 0;
         lc[i] = 0;
@@ -657,8 +667,9 @@ k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
         __syncthreads();
     }
     // keep: bit 4k+p for record (pixel k, plane p); r[k].k[p] becomes its
-    // output key
-    uint32_t keep = 0, c = 0;
+    // output key; c: the thread's kept records per plane, 16 bits each
+    uint32_t keep = 0;
+    uint64_t c = 0;
 #pragma unroll
     for (int k = 0; k < BND_PX_PER_THREAD; k++)
 #pragma unroll
@@ -672,54 +683,112 @@ k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
             }
             if (rank != RANK_NONE) {
                 keep |= 1u << (4 * k + p);
-                c++;
+                c += 1ull << (16 * p);
                 const uint64_t packed = ((key >> 32) << lb) | (key & 0xFFFFFFFFull);
-                r[k].k[p] = ((np ? (uint64_t)rank : packed) << 2) | (uint64_t)p;
+                r[k].k[p] = np ? (uint64_t)rank : (packed << 2) | (uint64_t)p;
             }
         }
-    uint32_t tot;
-    uint32_t pos = block_exclusive_scan<uint32_t>(c, lds4, &tot);
+    // the tile's records densely to its region, plane by plane, each plane
+    // in raster order
+    uint64_t tot;
+    const uint64_t pre = block_exclusive_scan<uint64_t>(c, lds8, &tot);
     const size_t base = (size_t)tile * (4 * BND_TILE);
+    uint32_t pos[4], plane_at = 0;
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        pos[p] = plane_at + ((uint32_t)(pre >> (16 * p)) & 0xFFFFu);
+        plane_at += (uint32_t)(tot >> (16 * p)) & 0xFFFFu;
+    }
 #pragma unroll
     for (int k = 0; k < BND_PX_PER_THREAD; k++)
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             if (!((keep >> (4 * k + p)) & 1u)) continue;
-            tile_keys[base + pos] = r[k].k[p];
-            tile_points[base + pos] = r[k].p[p];
-            pos++;
+            const size_t at = base + pos[p]++;
+            tile_keys[at] = r[k].k[p];
+            tile_points[at] = r[k].p[p];
         }
-    if (threadIdx.x == 0) tile_counts[tile] = tot;
+    if (threadIdx.x < 4) {
+        const uint32_t n = (uint32_t)(tot >> (16 * threadIdx.x)) & 0xFFFFu;
+        tile_counts[threadIdx.x * n_tiles + tile] = n;
+        if (n) atomicAdd(&group_sums[threadIdx.x * tile_groups(n_tiles) + tile / GATHER_GROUP], n);
+    }
 }
 
-// One work-group per GATHER_TILES consecutive tiles: the records before them
-// (the sum of the earlier tiles' counts, read by all 256 threads), then the
-// tiles' records, in tile order, moved as one run to where the sort reads
-// them (4 records' loads in flight per thread); the last work-group sets
-// n_boundary.
+// One work-group per GATHER_TILES consecutive tiles: the sort's input is
+// every tile's plane-0 records in tile order, then the plane-1 records, and
+// so on. The work-group finds where each plane's records of its tiles start
+// (the plane totals and the records of the earlier tiles, from the group
+// sums and its own group's tile counts), then moves its tiles' records, the
+// four planes' runs as one loop (4 records' loads in flight per thread).
 constexpr uint32_t GATHER_TILES = 8;
+static_assert(GATHER_GROUP % GATHER_TILES == 0, "a work-group's tiles share a group");
+ATQ_DEV uint32_t wave_sum(uint32_t x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
 __global__ void __launch_bounds__(256)
 k_bnd_gather(const uint64_t *__restrict__ tile_keys, const atq_boundary_point *__restrict__ tile_points,
-             const uint32_t *__restrict__ tile_counts, uint32_t n_tiles, atq_counts *counts,
-             RsDyn out) {
-    __shared__ uint32_t lds4[4];
-    __shared__ uint32_t cnt[GATHER_TILES], pre[GATHER_TILES + 1];
+             const uint32_t *__restrict__ tile_counts, const uint32_t *__restrict__ group_sums,
+             uint32_t n_tiles, atq_counts *counts, RsDyn out) {
+    __shared__ uint32_t acc[8];                         // plane totals, records before
+    __shared__ uint32_t cnt[4][GATHER_TILES], pre[4][GATHER_TILES + 1], run_start[5], dst[4];
+    __shared__ uint32_t plane_off[4][GATHER_TILES];  // a plane's first record in its tile's region
     const uint32_t t0 = blockIdx.x * GATHER_TILES, nt = min(n_tiles - t0, GATHER_TILES);
-    if (threadIdx.x < GATHER_TILES) cnt[threadIdx.x] = threadIdx.x < nt ? tile_counts[t0 + threadIdx.x] : 0u;
-    uint32_t before = 0;
-    for (uint32_t i = threadIdx.x; i < t0; i += 256) before += tile_counts[i];
-    const uint32_t off = block_reduce_sum<uint32_t>(before, lds4);  // (its barriers publish cnt)
-    if (threadIdx.x == 0) {
-        uint32_t run = 0;
-        for (uint32_t q = 0; q < GATHER_TILES; q++) {
-            pre[q] = run;
-            run += cnt[q];
-        }
-        pre[GATHER_TILES] = run;
-        if (t0 + nt == n_tiles) counts->n_boundary = off + run;
+    const uint32_t ng = tile_groups(n_tiles), g0 = t0 / GATHER_GROUP;
+    if (threadIdx.x < 8) acc[threadIdx.x] = 0;
+    if (threadIdx.x < 4 * GATHER_TILES) {
+        const uint32_t p = threadIdx.x / GATHER_TILES, q = threadIdx.x % GATHER_TILES;
+        cnt[p][q] = q < nt ? tile_counts[p * n_tiles + t0 + q] : 0u;
     }
     __syncthreads();
-    const uint32_t total = pre[GATHER_TILES];
+    uint32_t tot[4] = {0, 0, 0, 0}, bef[4] = {0, 0, 0, 0};
+    for (uint32_t q = threadIdx.x; q < ng; q += 256)
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const uint32_t v = group_sums[p * ng + q];
+            tot[p] += v;
+            if (q < g0) bef[p] += v;
+        }
+    for (uint32_t t = g0 * GATHER_GROUP + threadIdx.x; t < t0; t += 256)
+#pragma unroll
+        for (int p = 0; p < 4; p++) bef[p] += tile_counts[p * n_tiles + t];
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const uint32_t a = wave_sum(tot[p]), b = wave_sum(bef[p]);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&acc[p], a);
+            atomicAdd(&acc[4 + p], b);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t plane_base = 0, r = 0;
+        for (int p = 0; p < 4; p++) {
+            dst[p] = plane_base + acc[4 + p];
+            plane_base += acc[p];
+            run_start[p] = r;
+            uint32_t x = 0;
+            for (uint32_t q = 0; q < GATHER_TILES; q++) {
+                pre[p][q] = x;
+                x += cnt[p][q];
+            }
+            pre[p][GATHER_TILES] = x;
+            r += x;
+        }
+        run_start[4] = r;
+        for (uint32_t q = 0; q < GATHER_TILES; q++) {
+            uint32_t x = 0;
+            for (int p = 0; p < 4; p++) {
+                plane_off[p][q] = x;
+                x += cnt[p][q];
+            }
+        }
+        if (blockIdx.x == 0) counts->n_boundary = plane_base;
+    }
+    __syncthreads();
+    const uint32_t total = run_start[4];
     // the rank passes end in Y; the full-key sort reads X
     const uint32_t np = counts->sort_passes;
     const bool to_y = np != 0 && (np & 1u) == 0;
@@ -727,25 +796,32 @@ k_bnd_gather(const uint64_t *__restrict__ tile_keys, const atq_boundary_point *_
     atq_boundary_point *points = reinterpret_cast<atq_boundary_point *>(to_y ? out.vy : out.vx);
     for (uint32_t i0 = threadIdx.x; i0 < total; i0 += 4 * 256) {
         uint64_t k[4];
-        atq_boundary_point p[4];
+        atq_boundary_point pt[4];
+        uint32_t to[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const uint32_t i = i0 + 256 * q;
+        for (int u = 0; u < 4; u++) {
+            const uint32_t i = i0 + 256 * u;
             if (i < total) {
+                uint32_t p = 0;
+#pragma unroll
+                for (int j = 1; j < 4; j++) p += i >= run_start[j] ? 1u : 0u;
+                const uint32_t ip = i - run_start[p];
                 uint32_t tq = 0;
 #pragma unroll
-                for (uint32_t j = 1; j < GATHER_TILES; j++) tq += i >= pre[j] ? 1u : 0u;
-                const size_t src = (size_t)(t0 + tq) * (4 * BND_TILE) + (i - pre[tq]);
-                k[q] = tile_keys[src];
-                p[q] = tile_points[src];
+                for (uint32_t j = 1; j < GATHER_TILES; j++) tq += ip >= pre[p][j] ? 1u : 0u;
+                const size_t src = (size_t)(t0 + tq) * (4 * BND_TILE) + plane_off[p][tq] +
+                                   (ip - pre[p][tq]);
+                k[u] = tile_keys[src];
+                pt[u] = tile_points[src];
+                to[u] = dst[p] + ip;
             }
         }
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const uint32_t i = i0 + 256 * q;
+        for (int u = 0; u < 4; u++) {
+            const uint32_t i = i0 + 256 * u;
             if (i < total) {
-                keys[off + i] = k[q];
-                points[off + i] = p[q];
+                keys[to[u]] = k[u];
+                points[to[u]] = pt[u];
             }
         }
     }
@@ -795,13 +871,13 @@ int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const
     if (fx)
         k_bnd_filter<true><<<xcd_grid(tiles), 256, 0, s>>>(
             labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size, label_bits, table,
-            rk.slot_rank, counts, tiles, rk.tile_keys, rk.tile_points, tile_counts);
+            rk.slot_rank, counts, tiles, rk.tile_keys, rk.tile_points, tile_counts, rk.group_sums);
     else
         k_bnd_filter<false><<<xcd_grid(tiles), 256, 0, s>>>(
             labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size, label_bits, table,
-            rk.slot_rank, counts, tiles, rk.tile_keys, rk.tile_points, tile_counts);
-    k_bnd_gather<<<atq_div_up(tiles, GATHER_TILES), 256, 0, s>>>(rk.tile_keys, rk.tile_points,
-                                                                 tile_counts, tiles, counts, out);
+            rk.slot_rank, counts, tiles, rk.tile_keys, rk.tile_points, tile_counts, rk.group_sums);
+    k_bnd_gather<<<atq_div_up(tiles, GATHER_TILES), 256, 0, s>>>(
+        rk.tile_keys, rk.tile_points, tile_counts, rk.group_sums, tiles, counts, out);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 

@@ -105,11 +105,13 @@ struct PairRanks {
     uint64_t *tile_keys;
     atq_boundary_point *tile_points;
     uint32_t *tile_counts;
+    uint32_t *group_sums;
 };
 struct RsDyn;  // sort.hpp
-// Records of the pairs that pass ValidBlobFilter, in the reference's order,
-// keyed for rank_sort (rank keys, or packed keys ((min << lb | max) << 2 |
-// plane) when counts->sort_passes is 0), written where rank_sort reads them;
+// Records of the pairs that pass ValidBlobFilter, plane by plane, each plane
+// in raster order, keyed for rank_sort (rank keys, or packed keys
+// ((min << lb | max) << 2 | plane) when counts->sort_passes is 0), written
+// where rank_sort reads them;
 // counts->n_boundary, n_valid_pairs and sort_passes set; in rank mode also
 // the clusters' bounds and extents (out.cl) and counts n_pairs, n_clusters
 // and n_cluster_points. sc must be the

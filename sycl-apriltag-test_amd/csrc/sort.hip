@@ -137,7 +137,7 @@ struct RsScatterLds {
 // Scatters records [b, e) (chunk g of the pass) to their stable positions:
 // digit base (exclusive scan of `totals`) + hist_row[digit] (the digit's
 // count in the chunks before this one) + rank within the chunk. fin: when
-// not null (the last rank pass), keys are (r << 2 | plane) and leave as
+// not null (the last rank pass), keys are ranks r and leave as
 // (xlat[r] << 2 | plane), and each record also becomes the cluster point of
 // cluster r at its position (src/main.cpp:621-675; its cluster index is
 // recorded by the angle sort).
@@ -224,7 +224,7 @@ ATQ_DEV void rs_scatter_body(const uint64_t *__restrict__ kin, const uint32_t *_
         if (fin) {
 #pragma unroll
             for (int j = 0; j < RS_ITEMS; j++) {
-                const uint32_t r = dg[j] != 0xFFFFFFFFu ? (uint32_t)(k[j] >> 2) : 0u;
+                const uint32_t r = dg[j] != 0xFFFFFFFFu ? (uint32_t)k[j] : 0u;
                 xk[j] = fin->xlat[r];
                 cc[j] = fin->cl.centre[r];
             }
@@ -246,7 +246,7 @@ ATQ_DEV void rs_scatter_body(const uint64_t *__restrict__ kin, const uint32_t *_
             if (dg[j] != 0xFFFFFFFFu) {
                 uint32_t pos = wc[wave][dg[j]] + rank[j];
                 if (fin) {
-                    kout[pos] = (xk[j] << 2) | (k[j] & 3u);
+                    kout[pos] = (xk[j] << 2) | ((v[j] >> 14) & 3u);  // the record's plane
                     const uint16_t px = (uint16_t)v[j], py = (uint16_t)(v[j] >> 16);
                     fin->cl.cpts[pos] = atq_cluster_point{
                         px, py, calc_theta(px, py, cc[j].x, cc[j].y, fin->cl.fx != 0)};

@@ -106,14 +106,15 @@ def check_pair_table(g, o):
 
 def expected_sort_passes(n_valid, slots, filter_off):
     """Radix passes over the pair-rank keys (11-bit digits over the rank's
-    bits plus 2 plane bits; at most 2): 0 selects the full-key sort, taken
-    when the pair table overflowed or more pairs are valid than the detector
-    ranks (a quarter of the table's slots, 64 to 16384)."""
+    bits; the records reach the sort plane by plane, so the plane is not
+    part of the key; at most 2): 0 selects the full-key sort, taken when the
+    pair table overflowed or more pairs are valid than the detector ranks (a
+    quarter of the table's slots, 64 to 16384)."""
     cap = max(64, min(16384, slots // 4))
     if filter_off or n_valid > cap:
         return 0
-    bits = 2 + max(n_valid - 1, 0).bit_length()
-    return -(-bits // 11)
+    bits = max(n_valid - 1, 0).bit_length()
+    return max(1, -(-bits // 11))
 
 
 def check_against_oracle(g, o):
