@@ -156,7 +156,10 @@ typedef struct atq_counts {
                                  keys (1 or 2); 0 = the full-key sort ran
                                  (pair table overflow or too many valid
                                  pairs to rank)                            */
-    uint32_t reserved[3];
+    uint32_t rank_shift;      /* detector: 2 = rank keys carry the plane in
+                                 2 low bits (records in raster order); 0 =
+                                 records reach the sort plane by plane     */
+    uint32_t reserved[2];
 } atq_counts;
 
 /* ---- library ------------------------------------------------------------ */

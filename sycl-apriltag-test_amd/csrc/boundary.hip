@@ -217,7 +217,12 @@ ATQ_DEV uint32_t global_slot(const PairTable &t, uint64_t key, bool insert) {
     const uint64_t nk = ~key;
     uint32_t h = pair_hash(key) & t.mask;
     for (int p = 0; p < PAIR_PROBES; p++, h = (h + 1) & t.mask) {
-        const uint64_t k = __hip_atomic_load(&t.keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // inserting: other work-groups insert concurrently, so the load is
+        // coherent; looking up (a later kernel): the table is settled, and a
+        // plain load may be served by the cache
+        const uint64_t k = insert ? __hip_atomic_load(&t.keys[h], __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT)
+                                  : t.keys[h];
         if (k == nk) return h;
         if (k == 0) {
             if (!insert) return NO_SLOT;
@@ -554,8 +559,9 @@ k_pair_valid(PairTable t, uint32_t w, uint32_t h, uint32_t max_count,
 // table's exact reduction, include/cluster_bounds.h:14-83, starting at
 // start), its centre (cx, cy) (include/cluster_bounds.h:47-49) and
 // extents {start, count} (src/main.cpp:700-717) and the counts of pairs,
-// clusters and their points. Also decides the sort:
-// counts->sort_passes = the radix passes the rank keys need, or 0 (table
+// clusters and their points. Also decides the sort: counts->rank_shift
+// (rank_shift_for) and counts->sort_passes = the radix passes the rank keys
+// need, or 0 (table
 // overflow, or more valid pairs than the list holds: packed keys, fallback
 // sort and cluster stage). One lane per listed pair (64 per work-group);
 // the 16 waves compare them against a sixteenth of each LDS tile each.
@@ -571,8 +577,12 @@ k_pair_rank(PairTable table, const uint64_t *__restrict__ vkeys,
     __shared__ uint32_t tcnt[RANK_TILE];
     __shared__ uint32_t part[RANK_WAVES][64], spart[RANK_WAVES][64];
     const uint32_t V = counts->n_valid_pairs;
-    const uint32_t np = counts->pair_filter_off == 0 && V <= cap ? rank_passes(V) : 0u;
-    if (blockIdx.x == 0 && threadIdx.x == 0) counts->sort_passes = np;
+    const uint32_t shift = rank_shift_for(V);
+    const uint32_t np = counts->pair_filter_off == 0 && V <= cap ? rank_passes(V, shift) : 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        counts->sort_passes = np;
+        counts->rank_shift = shift;
+    }
     if (np == 0 || blockIdx.x * 64 >= V) return;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t i = blockIdx.x * 64 + lane;
@@ -620,26 +630,22 @@ k_pair_rank(PairTable table, const uint64_t *__restrict__ vkeys,
     }
 }
 
-// Tiles per group of the gather's running sums.
-constexpr uint32_t GATHER_GROUP = 64;
-__host__ __device__ inline uint32_t tile_groups(uint32_t n_tiles) {
-    return (n_tiles + GATHER_GROUP - 1) / GATHER_GROUP;
-}
-
 // One work-group per 1024-pixel tile (XCD-contiguous tile order): the
-// tile's records of valid pairs, keyed for the sort (the rank r alone: the
-// records reach the sort plane by plane, so a stable sort by r gives the
-// reference's (pair, plane, y, x) order; fallback: the packed pair key with
-// the plane), densely to the tile's region, plane by plane, each plane in
-// raster order; per plane, the counts to tile_counts[plane][tile] and added
-// to group_sums[plane][tile / GATHER_GROUP] (zero at launch).
+// tile's records of valid pairs, keyed for the sort, densely to the tile's
+// region. Raster mode (counts->rank_shift 2, and the fallback): keys
+// r << 2 | plane (fallback: the packed pair key with the plane), records in
+// raster order, their number to tile_counts[0][tile] (planes 1-3: 0).
+// Otherwise keys r, records plane by plane, each plane in raster order, the
+// counts per plane to tile_counts[plane][tile]; the gather then lays out all
+// plane-0 records first, so a stable sort by r gives the reference's
+// (pair, plane, y, x) order.
 template <bool FX>
 __global__ void __launch_bounds__(256)
 k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes, uint32_t w,
              uint32_t h, uint32_t min_size, uint32_t lb, PairTable table,
              const uint32_t *__restrict__ slot_rank, const atq_counts *counts, uint32_t n_tiles,
              uint64_t *__restrict__ tile_keys, atq_boundary_point *__restrict__ tile_points,
-             uint32_t *__restrict__ tile_counts, uint32_t *__restrict__ group_sums) {
+             uint32_t *__restrict__ tile_counts) {
     const uint32_t tile = xcd_tile(blockIdx.x, n_tiles);
     if (tile >= n_tiles) return;
     __shared__ uint64_t lk[TILE_SLOTS];
@@ -651,6 +657,7 @@ k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
     }
     const bool keep_all = counts->pair_filter_off != 0;
     const uint32_t np = counts->sort_passes;
+    const bool raster = np == 0 || counts->rank_shift != 0;  // else plane by plane
     __syncthreads();
     PixelRecords r[BND_PX_PER_THREAD];
     tile_records(labels, sizes, w, h, min_size, tile, r, FX);
@@ -685,14 +692,17 @@ k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
                 keep |= 1u << (4 * k + p);
                 c += 1ull << (16 * p);
                 const uint64_t packed = ((key >> 32) << lb) | (key & 0xFFFFFFFFull);
-                r[k].k[p] = np ? (uint64_t)rank : (packed << 2) | (uint64_t)p;
+                r[k].k[p] = np ? (raster ? ((uint64_t)rank << 2) | (uint64_t)p : (uint64_t)rank)
+                               : (packed << 2) | (uint64_t)p;
             }
         }
+    if (raster) c = (c & 0xFFFFu) + ((c >> 16) & 0xFFFFu) + ((c >> 32) & 0xFFFFu) + (c >> 48);
     // the tile's records densely to its region, plane by plane, each plane
     // in raster order
     uint64_t tot;
     const uint64_t pre = block_exclusive_scan<uint64_t>(c, lds8, &tot);
     const size_t base = (size_t)tile * (4 * BND_TILE);
+    // raster mode: one count (in plane 0's field), every record in raster order
     uint32_t pos[4], plane_at = 0;
 #pragma unroll
     for (int p = 0; p < 4; p++) {
@@ -704,96 +714,133 @@ k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             if (!((keep >> (4 * k + p)) & 1u)) continue;
-            const size_t at = base + pos[p]++;
+            const size_t at = base + (raster ? pos[0]++ : pos[p]++);
             tile_keys[at] = r[k].k[p];
             tile_points[at] = r[k].p[p];
         }
-    if (threadIdx.x < 4) {
-        const uint32_t n = (uint32_t)(tot >> (16 * threadIdx.x)) & 0xFFFFu;
-        tile_counts[threadIdx.x * n_tiles + tile] = n;
-        if (n) atomicAdd(&group_sums[threadIdx.x * tile_groups(n_tiles) + tile / GATHER_GROUP], n);
-    }
+    if (threadIdx.x < 4) tile_counts[threadIdx.x * n_tiles + tile] = (uint32_t)(tot >> (16 * threadIdx.x)) & 0xFFFFu;
 }
+
+namespace {
+// Plane-major tile counts (tile_counts[plane][tile]) and their offsets.
+struct TileCount {
+    const uint32_t *c;
+    ATQ_DEV uint32_t operator()(uint32_t i) const { return c[i]; }
+};
+struct TileOffset {
+    uint32_t *off;
+    ATQ_DEV void operator()(uint32_t i, uint32_t pre, uint32_t) const { off[i] = pre; }
+};
+}  // namespace
 
 // One work-group per GATHER_TILES consecutive tiles: the sort's input is
 // every tile's plane-0 records in tile order, then the plane-1 records, and
-// so on. The work-group finds where each plane's records of its tiles start
-// (the plane totals and the records of the earlier tiles, from the group
-// sums and its own group's tile counts), then moves its tiles' records, the
-// four planes' runs as one loop (4 records' loads in flight per thread).
+// so on (tile_off: the exclusive scan of the plane-major tile counts; null:
+// each work-group sums the counts itself and work-group 0 sets n_boundary,
+// cheaper than a scan launch for frames of up to GATHER_SUM_TILES tiles).
+// The work-group moves its tiles' records, the four planes' runs as one
+// loop (4 records' loads in flight per thread).
 constexpr uint32_t GATHER_TILES = 8;
-static_assert(GATHER_GROUP % GATHER_TILES == 0, "a work-group's tiles share a group");
+constexpr uint32_t GATHER_SUM_TILES = 2048;
 ATQ_DEV uint32_t wave_sum(uint32_t x) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
     return x;
 }
-__global__ void __launch_bounds__(256)
-k_bnd_gather(const uint64_t *__restrict__ tile_keys, const atq_boundary_point *__restrict__ tile_points,
-             const uint32_t *__restrict__ tile_counts, const uint32_t *__restrict__ group_sums,
-             uint32_t n_tiles, atq_counts *counts, RsDyn out) {
-    __shared__ uint32_t acc[8];                         // plane totals, records before
-    __shared__ uint32_t cnt[4][GATHER_TILES], pre[4][GATHER_TILES + 1], run_start[5], dst[4];
-    __shared__ uint32_t plane_off[4][GATHER_TILES];  // a plane's first record in its tile's region
+struct GatherLds {
+    uint32_t cnt[4][GATHER_TILES], pre[4][GATHER_TILES + 1], run_start[5];
+    uint32_t dst[4][GATHER_TILES];        // where a tile's plane run goes
+    uint32_t plane_off[4][GATHER_TILES];  // a plane's first record in its tile's region
+    uint32_t acc[8];  // without tile_off: plane totals, records before the tiles
+};
+
+// The gather of NP planes (1: raster mode, every count in plane 0's row).
+template <int NP>
+ATQ_DEV void gather_tiles(const uint64_t *__restrict__ tile_keys,
+                          const atq_boundary_point *__restrict__ tile_points,
+                          const uint32_t *__restrict__ tile_counts,
+                          const uint32_t *__restrict__ tile_off, uint32_t n_tiles,
+                          atq_counts *counts, uint64_t *keys, atq_boundary_point *points,
+                          GatherLds &L) {
+    auto &cnt = L.cnt;
+    auto &pre = L.pre;
+    auto &run_start = L.run_start;
+    auto &dst = L.dst;
+    auto &plane_off = L.plane_off;
+    auto &acc = L.acc;
     const uint32_t t0 = blockIdx.x * GATHER_TILES, nt = min(n_tiles - t0, GATHER_TILES);
-    const uint32_t ng = tile_groups(n_tiles), g0 = t0 / GATHER_GROUP;
     if (threadIdx.x < 8) acc[threadIdx.x] = 0;
     if (threadIdx.x < 4 * GATHER_TILES) {
         const uint32_t p = threadIdx.x / GATHER_TILES, q = threadIdx.x % GATHER_TILES;
-        cnt[p][q] = q < nt ? tile_counts[p * n_tiles + t0 + q] : 0u;
+        cnt[p][q] = q < nt && p < (uint32_t)NP ? tile_counts[p * n_tiles + t0 + q] : 0u;
+        dst[p][q] = q < nt && p < (uint32_t)NP && tile_off ? tile_off[p * n_tiles + t0 + q] : 0u;
     }
     __syncthreads();
-    uint32_t tot[4] = {0, 0, 0, 0}, bef[4] = {0, 0, 0, 0};
-    for (uint32_t q = threadIdx.x; q < ng; q += 256)
+    if (!tile_off) {  // every tile's counts, summed here
+        // every load in flight before the first is used (n_tiles <= 256 * 8)
+        constexpr int PER = GATHER_SUM_TILES / 256;
+        uint32_t v[PER][NP];
 #pragma unroll
-        for (int p = 0; p < 4; p++) {
-            const uint32_t v = group_sums[p * ng + q];
-            tot[p] += v;
-            if (q < g0) bef[p] += v;
+        for (int j = 0; j < PER; j++) {
+            const uint32_t t = threadIdx.x + 256 * j;
+#pragma unroll
+            for (int p = 0; p < NP; p++) v[j][p] = t < n_tiles ? tile_counts[p * n_tiles + t] : 0u;
         }
-    for (uint32_t t = g0 * GATHER_GROUP + threadIdx.x; t < t0; t += 256)
+        uint32_t tot[NP], bef[NP];
 #pragma unroll
-        for (int p = 0; p < 4; p++) bef[p] += tile_counts[p * n_tiles + t];
+        for (int p = 0; p < NP; p++) tot[p] = bef[p] = 0;
 #pragma unroll
-    for (int p = 0; p < 4; p++) {
-        const uint32_t a = wave_sum(tot[p]), b = wave_sum(bef[p]);
-        if ((threadIdx.x & 63) == 0) {
-            atomicAdd(&acc[p], a);
-            atomicAdd(&acc[4 + p], b);
+        for (int j = 0; j < PER; j++) {
+            const bool before = threadIdx.x + 256 * j < t0;
+#pragma unroll
+            for (int p = 0; p < NP; p++) {
+                tot[p] += v[j][p];
+                bef[p] += before ? v[j][p] : 0u;
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            const uint32_t a = wave_sum(tot[p]), b = wave_sum(bef[p]);
+            if ((threadIdx.x & 63) == 0) {
+                atomicAdd(&acc[p], a);
+                atomicAdd(&acc[4 + p], b);
+            }
+        }
+        __syncthreads();
+    }
+    // per (plane p, tile q), by 32 threads: the run's start among the
+    // work-group's records of plane p, the plane's first record in the
+    // tile's region and, without tile_off, the run's destination
+    if (threadIdx.x < 4 * GATHER_TILES) {
+        const uint32_t p = threadIdx.x / GATHER_TILES, q = threadIdx.x % GATHER_TILES;
+        uint32_t before_q = 0, before_p = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < GATHER_TILES; j++) before_q += j < q ? cnt[p][j] : 0u;
+#pragma unroll
+        for (uint32_t j = 0; j < 4; j++) before_p += j < p ? cnt[j][q] : 0u;
+        pre[p][q] = before_q;
+        plane_off[p][q] = before_p;
+        if (q == GATHER_TILES - 1) pre[p][GATHER_TILES] = before_q + cnt[p][q];
+        if (!tile_off) {
+            uint32_t base = acc[4 + p];
+#pragma unroll
+            for (uint32_t j = 0; j < 4; j++) base += j < p ? acc[j] : 0u;
+            dst[p][q] = base + before_q;
+            if (blockIdx.x == 0 && threadIdx.x == 0)
+                counts->n_boundary = acc[0] + acc[1] + acc[2] + acc[3];
         }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t plane_base = 0, r = 0;
+        uint32_t r = 0;
         for (int p = 0; p < 4; p++) {
-            dst[p] = plane_base + acc[4 + p];
-            plane_base += acc[p];
             run_start[p] = r;
-            uint32_t x = 0;
-            for (uint32_t q = 0; q < GATHER_TILES; q++) {
-                pre[p][q] = x;
-                x += cnt[p][q];
-            }
-            pre[p][GATHER_TILES] = x;
-            r += x;
+            r += pre[p][GATHER_TILES];
         }
         run_start[4] = r;
-        for (uint32_t q = 0; q < GATHER_TILES; q++) {
-            uint32_t x = 0;
-            for (int p = 0; p < 4; p++) {
-                plane_off[p][q] = x;
-                x += cnt[p][q];
-            }
-        }
-        if (blockIdx.x == 0) counts->n_boundary = plane_base;
     }
     __syncthreads();
     const uint32_t total = run_start[4];
-    // the rank passes end in Y; the full-key sort reads X
-    const uint32_t np = counts->sort_passes;
-    const bool to_y = np != 0 && (np & 1u) == 0;
-    uint64_t *keys = to_y ? out.ky : out.kx;
-    atq_boundary_point *points = reinterpret_cast<atq_boundary_point *>(to_y ? out.vy : out.vx);
     for (uint32_t i0 = threadIdx.x; i0 < total; i0 += 4 * 256) {
         uint64_t k[4];
         atq_boundary_point pt[4];
@@ -804,16 +851,16 @@ k_bnd_gather(const uint64_t *__restrict__ tile_keys, const atq_boundary_point *_
             if (i < total) {
                 uint32_t p = 0;
 #pragma unroll
-                for (int j = 1; j < 4; j++) p += i >= run_start[j] ? 1u : 0u;
+                for (int j = 1; j < NP; j++) p += i >= run_start[j] ? 1u : 0u;
                 const uint32_t ip = i - run_start[p];
                 uint32_t tq = 0;
 #pragma unroll
                 for (uint32_t j = 1; j < GATHER_TILES; j++) tq += ip >= pre[p][j] ? 1u : 0u;
-                const size_t src = (size_t)(t0 + tq) * (4 * BND_TILE) + plane_off[p][tq] +
-                                   (ip - pre[p][tq]);
+                const uint32_t within = ip - pre[p][tq];
+                const size_t src = (size_t)(t0 + tq) * (4 * BND_TILE) + plane_off[p][tq] + within;
                 k[u] = tile_keys[src];
                 pt[u] = tile_points[src];
-                to[u] = dst[p] + ip;
+                to[u] = dst[p][tq] + within;
             }
         }
 #pragma unroll
@@ -825,6 +872,22 @@ k_bnd_gather(const uint64_t *__restrict__ tile_keys, const atq_boundary_point *_
             }
         }
     }
+}
+
+__global__ void __launch_bounds__(256)
+k_bnd_gather(const uint64_t *__restrict__ tile_keys, const atq_boundary_point *__restrict__ tile_points,
+             const uint32_t *__restrict__ tile_counts, const uint32_t *__restrict__ tile_off,
+             uint32_t n_tiles, atq_counts *counts, RsDyn out) {
+    __shared__ GatherLds L;
+    // the rank passes end in Y; the full-key sort reads X
+    const uint32_t np = counts->sort_passes;
+    const bool to_y = np != 0 && (np & 1u) == 0;
+    uint64_t *keys = to_y ? out.ky : out.kx;
+    atq_boundary_point *points = reinterpret_cast<atq_boundary_point *>(to_y ? out.vy : out.vx);
+    if (np == 0 || counts->rank_shift != 0)
+        gather_tiles<1>(tile_keys, tile_points, tile_counts, tile_off, n_tiles, counts, keys, points, L);
+    else
+        gather_tiles<4>(tile_keys, tile_points, tile_counts, tile_off, n_tiles, counts, keys, points, L);
 }
 
 int launch_find_boundaries_sparse(hipStream_t s, const uint32_t *labels, const uint32_t *sizes,
@@ -843,6 +906,7 @@ int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const
     const uint32_t max_count = (uint32_t)(2 * (2 * w + 2 * h));
     const uint32_t slots = table.mask + 1;
     if (!sc.flags || atq_div_up(slots, PV_TILE) + 1 > sc.flag_count ||
+        4 * tiles / LB_TILE + 1 > sc.flag_count ||
         slots < 256 || rk.cap == 0 || rk.cap % 64)
         return ATQ_ERR_BAD_ARG;
     // a few tiles per work-group (merged in LDS before the global table), at
@@ -871,13 +935,20 @@ int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const
     if (fx)
         k_bnd_filter<true><<<xcd_grid(tiles), 256, 0, s>>>(
             labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size, label_bits, table,
-            rk.slot_rank, counts, tiles, rk.tile_keys, rk.tile_points, tile_counts, rk.group_sums);
+            rk.slot_rank, counts, tiles, rk.tile_keys, rk.tile_points, tile_counts);
     else
         k_bnd_filter<false><<<xcd_grid(tiles), 256, 0, s>>>(
             labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size, label_bits, table,
-            rk.slot_rank, counts, tiles, rk.tile_keys, rk.tile_points, tile_counts, rk.group_sums);
+            rk.slot_rank, counts, tiles, rk.tile_keys, rk.tile_points, tile_counts);
+    uint32_t *tile_off = nullptr;
+    if (tiles > GATHER_SUM_TILES) {
+        tile_off = tile_counts + 4 * tiles;
+        int r = device_scan(s, nullptr, 4 * tiles, TileCount{tile_counts}, TileOffset{tile_off}, sc,
+                            &counts->n_boundary);
+        if (r) return r;
+    }
     k_bnd_gather<<<atq_div_up(tiles, GATHER_TILES), 256, 0, s>>>(
-        rk.tile_keys, rk.tile_points, tile_counts, rk.group_sums, tiles, counts, out);
+        rk.tile_keys, rk.tile_points, tile_counts, tile_off, tiles, counts, out);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 

@@ -95,9 +95,13 @@ ATQ_DEV uint32_t g_load(const uint32_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-ATQ_DEV uint32_t g_find(const uint32_t *parent, uint32_t n) {
+// Root of n by plain (cacheable) loads, once every union is done: words
+// change afterwards only by path halving, which lowers a word to another
+// ancestor of the same root, so a stale value still leads there (indices
+// decrease along the path, so the walk ends).
+ATQ_DEV uint32_t g_find_settled(const uint32_t *parent, uint32_t n) {
     while (true) {
-        uint32_t p = g_load(&parent[n]) & ~FLAG;
+        uint32_t p = parent[n] & ~FLAG;
         if (p == n) return n;
         n = p;
     }
@@ -366,8 +370,8 @@ ATQ_DEV void ccl_border_tile(uint64_t *hs, const FrontFrame &F, uint32_t tile, u
     // One round of independent loads: the block's three mask pixels, its two
     // parent words, and for every probe that leaves the tile its pixel and
     // the two parent words of the block holding it (parent words of local
-    // roots may be lowered by concurrent unions; any value read is an
-    // ancestor in the same tree, which is all a union needs).
+    // roots may be lowered by concurrent unions; any value read, a cached
+    // one too, is an ancestor in the same tree, which is all a union needs).
     uint8_t tl = 127, tr = 127, bl = 127;
     uint32_t own[2] = {0, 0}, nb[6][2];
     uint8_t pv[6];
@@ -378,8 +382,8 @@ ATQ_DEV void ccl_border_tile(uint64_t *hs, const FrontFrame &F, uint32_t tile, u
         tr = row[x + 1];
         bl = row[x + w];
         const uint32_t id = (uint32_t)y * w + (uint32_t)x;
-        own[0] = g_load(&parent[id]);
-        own[1] = g_load(&parent[id + 1]);
+        own[0] = parent[id];
+        own[1] = parent[id + 1];
     }
 #pragma unroll
     for (int i = 0; i < 6; i++) {
@@ -393,8 +397,8 @@ ATQ_DEV void ccl_border_tile(uint64_t *hs, const FrontFrame &F, uint32_t tile, u
         if (cross[i]) {
             pv[i] = thr[(size_t)py * w + px];
             const uint32_t nid = (uint32_t)ny * w + (uint32_t)nx;
-            nb[i][0] = g_load(&parent[nid]);
-            nb[i][1] = g_load(&parent[nid + 1]);
+            nb[i][0] = parent[nid];
+            nb[i][1] = parent[nid + 1];
         }
     }
     const uint32_t m = live ? check_mask(tl, tr, bl, x, y, w, h) : 0u;
@@ -484,7 +488,7 @@ k_ccl_resolve(FrontTable tab) {
             for (uint32_t c = 0; c < 2; c++) {
                 const uint32_t n = id + c, word = c ? v.y : v.x;
                 if (word & FLAG) {
-                    r[c] = g_find(parent, word & ~FLAG);  // read-only: many blocks share the path
+                    r[c] = g_find_settled(parent, word & ~FLAG);  // read-only: many blocks share the path
                 } else if (word == n) {
                     r[c] = n;
                 } else {

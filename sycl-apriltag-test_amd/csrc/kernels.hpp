@@ -105,7 +105,6 @@ struct PairRanks {
     uint64_t *tile_keys;
     atq_boundary_point *tile_points;
     uint32_t *tile_counts;
-    uint32_t *group_sums;
 };
 struct RsDyn;  // sort.hpp
 // Records of the pairs that pass ValidBlobFilter, plane by plane, each plane

@@ -63,7 +63,7 @@ size_t Workspace::carve(char *base, const Caps &c) {
     cpts_tmp = carve_one<atq_cluster_point>(base, off, c.cap);
     fit_sums = carve_one<int64_t>(base, off, 6 * ((size_t)c.cap_cl + c.cap / SEG_CHUNK + 2));
     long_list = carve_one<uint32_t>(base, off, 2 * (size_t)seg_long_cap(c.cap));
-    tiles = carve_one<uint32_t>(base, off, 4 * ((size_t)c.npx / BOUNDARY_TILE_PX + 1));
+    tiles = carve_one<uint32_t>(base, off, 8 * ((size_t)c.npx / BOUNDARY_TILE_PX + 1));
     qtmp = carve_one<atq_quad_corners>(base, off, c.cap_cl);
     qflags = carve_one<uint32_t>(base, off, c.cap_cl);
     scan = carve_one<uint32_t>(base, off, SCAN_GROUPS + 1);
@@ -151,8 +151,6 @@ size_t carve_detector(atq_detector *d, char *base) {
     d->pair_grad = carve_one<unsigned long long>(base, off, (size_t)c.pair_slots);
     d->pair_moment = carve_one<unsigned long long>(base, off, c.pair_slots);
     d->long_n = carve_one<uint32_t>(base, off, 4);  // [2]: the full-key sort's barrier
-    // the boundary pass's per-plane sums of 64-tile groups
-    d->group_sums = carve_one<uint32_t>(base, off, 4 * ((size_t)c.npx / (64 * BOUNDARY_TILE_PX) + 2));
     d->counts = carve_one<atq_counts>(base, off, 1);
     off = (off + 255) & ~(size_t)255;
     size_t ws_bytes = d->ws.carve(base ? base + off : nullptr, c);
@@ -199,8 +197,8 @@ FrontFrame front_frame(const atq_detector *d, const uint8_t *gray_in, size_t w_i
     fr.min_diff = d->params.min_white_black_diff;
     fr.channels = (uint32_t)d->channels;
     // one fill for the later stages: the scan flags, the pair table, the
-    // long-segment counts, the boundary pass's group sums and the counts
-    // (carved contiguously), with the counts' key width set
+    // long-segment counts and the counts (carved contiguously), with the
+    // counts' key width set
     uint32_t *zero_begin = reinterpret_cast<uint32_t *>(d->scan_flags);
     fr.clear = zero_begin;
     fr.clear_words = (uint32_t)((reinterpret_cast<char *>(d->counts + 1) -
@@ -257,13 +255,14 @@ int run_back(atq_detector *d, hipStream_t s, const uint8_t *gray_in) {
     cl.w = (uint32_t)w;
     cl.h = (uint32_t)h;
     cl.fx = fx ? 1u : 0u;
-    const RsDyn io{d->keys, d->vals, ws.keys_alt, ws.vals_alt, &cnt->sort_passes, d->rank_keys, cl};
+    const RsDyn io{d->keys,        d->vals,          ws.keys_alt, ws.vals_alt,
+                   &cnt->sort_passes, &cnt->rank_shift, d->rank_keys, cl};
     // the per-tile record regions live in the cluster-bounds area, which is
     // free until the fallback's cluster stage (12 of its 32 bytes per entry)
     uint64_t *tile_keys = reinterpret_cast<uint64_t *>(ws.seg_bounds);
     const PairRanks rk{d->slot_rank, d->vkeys, d->vslots, d->vcnt, d->rank_keys, rank_cap(d->caps),
                        tile_keys, reinterpret_cast<atq_boundary_point *>(tile_keys + c.cap),
-                       ws.tiles, d->group_sums};
+                       ws.tiles};
     TRY(launch_find_boundaries_filtered(s, d->labels, d->sizes, w, h, d->params.min_component_size,
                                         (uint32_t)plan.label_bits, table, cnt, rk, io, sc, fx));
     TRY(mark(1 + ATQ_STAGE_BOUNDARY));

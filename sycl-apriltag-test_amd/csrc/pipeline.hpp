@@ -89,7 +89,6 @@ struct atq_detector {
     unsigned long long *pair_moment;
     uint32_t *long_n;      // long-segment counts of the two segment sorts and the
                            // full-key sort's barrier word, zeroed likewise
-    uint32_t *group_sums;  // boundary records per plane and 64-tile group, zeroed likewise
     atq_counts *counts;
     atq::Workspace ws;
     atq::FrontShapes shapes;  // threshold region and CCL tile shapes

@@ -137,8 +137,8 @@ struct RsScatterLds {
 // Scatters records [b, e) (chunk g of the pass) to their stable positions:
 // digit base (exclusive scan of `totals`) + hist_row[digit] (the digit's
 // count in the chunks before this one) + rank within the chunk. fin: when
-// not null (the last rank pass), keys are ranks r and leave as
-// (xlat[r] << 2 | plane), and each record also becomes the cluster point of
+// not null (the last rank pass), keys hold ranks r (key >> rank_shift) and
+// leave as (xlat[r] << 2 | plane), and each record also becomes the cluster point of
 // cluster r at its position (src/main.cpp:621-675; its cluster index is
 // recorded by the angle sort).
 template <int BITS>
@@ -222,9 +222,10 @@ ATQ_DEV void rs_scatter_body(const uint64_t *__restrict__ kin, const uint32_t *_
         uint64_t xk[RS_ITEMS];
         float2 cc[RS_ITEMS];
         if (fin) {
+            const uint32_t rsh = *fin->shift;
 #pragma unroll
             for (int j = 0; j < RS_ITEMS; j++) {
-                const uint32_t r = dg[j] != 0xFFFFFFFFu ? (uint32_t)k[j] : 0u;
+                const uint32_t r = dg[j] != 0xFFFFFFFFu ? (uint32_t)(k[j] >> rsh) : 0u;
                 xk[j] = fin->xlat[r];
                 cc[j] = fin->cl.centre[r];
             }

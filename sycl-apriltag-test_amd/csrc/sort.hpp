@@ -23,18 +23,21 @@ int radix_sort_packed(hipStream_t s, const RadixPlan &p, uint64_t *keys, uint32_
                       uint64_t *keys_alt, uint32_t *vals_alt, const uint32_t *count, uint32_t cap,
                       uint32_t *hist, uint32_t *totals, bool *result_in_alt);
 
-// The detector's sort of its boundary records (sort.hip): keys r, the rank
-// of the record's cluster pair among the frame's valid pairs (the records
-// arrive plane by plane, each plane in raster order, so a stable sort by r
-// gives the reference's (pair, plane, y, x) order), sorted with
-// RK_BITS-bit digits in *passes (1..RK_MAX_PASSES) passes from X or Y (see
-// rank_sort) into Y. Every kept record belongs to a valid pair, so cluster r
-// is rank r: the last pass writes the packed key (xlat[r] << 2 | plane) and
-// also each record as the cluster point of cluster r (slope about
-// centre[r], which the ranking wrote with the bounds and extents).
-// *passes == 0 selects the fallback: packed keys (with the plane) in X,
-// sorted into Y by one persistent kernel that then groups the records into
-// segments, reduces and filters them and assigns the cluster points itself.
+// The detector's sort of its boundary records (sort.hip): keys built from
+// r, the rank of the record's cluster pair among the frame's valid pairs,
+// either r << 2 | plane with the records in raster order (rank_shift 2), or
+// r alone with the records arriving plane by plane, each plane in raster
+// order (rank_shift 0; one bit-pair shorter, used when that saves a pass);
+// either way a stable sort gives the reference's (pair, plane, y, x) order.
+// Sorted with RK_BITS-bit digits in *passes (1..RK_MAX_PASSES) passes from
+// X or Y (see rank_sort) into Y. Every kept record belongs to a valid pair,
+// so cluster r is rank r: the last pass writes the packed key
+// (xlat[r] << 2 | plane) and also each record as the cluster point of
+// cluster r (slope about centre[r], which the ranking wrote with the bounds
+// and extents). *passes == 0 selects the fallback: packed keys (with the
+// plane) in X, sorted into Y by one persistent kernel that then groups the
+// records into segments, reduces and filters them and assigns the cluster
+// points itself.
 constexpr int RK_BITS = 11;
 constexpr uint32_t RK_MAX_PASSES = 2;
 constexpr unsigned RS_FULL_GROUPS = 128;  // work-groups of the fallback
@@ -58,16 +61,24 @@ struct RsDyn {
     uint64_t *ky;
     uint32_t *vy;
     const uint32_t *passes;  // device: rank passes, 0 = full-key fallback
+    const uint32_t *shift;   // device: rank_shift (rank = key >> shift)
     const uint64_t *xlat;    // rank -> packed pair key
     RsClusters cl;
 };
-// Rank passes for v valid pairs: ranks below v take bit_length(v - 1) bits;
+// Radix passes over the rank keys of v valid pairs: ranks below v take
+// bit_length(v - 1) bits, plus `extra` (the plane's 2 bits in raster mode);
 // 0 when that needs more than RK_MAX_PASSES digits.
-__host__ __device__ inline uint32_t rank_passes(uint32_t v) {
-    uint32_t bits = 0;
+__host__ __device__ inline uint32_t rank_passes(uint32_t v, uint32_t extra) {
+    uint32_t bits = extra;
     for (uint32_t r = v > 0 ? v - 1 : 0; r; r >>= 1) bits++;
     const uint32_t np = bits ? (bits + RK_BITS - 1) / RK_BITS : 1u;
     return np <= RK_MAX_PASSES ? np : 0u;
+}
+// Raster mode (keys r << 2 | plane, records in raster order) when it needs
+// no more passes than plane-major mode (keys r, records plane by plane):
+// the raster layout is the cheaper one to produce.
+__host__ __device__ inline uint32_t rank_shift_for(uint32_t v) {
+    return rank_passes(v, 2) == rank_passes(v, 0) ? 2u : 0u;
 }
 // Launches RK_MAX_PASSES rank passes (idle beyond *passes), the last of
 // which also holds the fallback over full_bits-bit packed keys (idle unless

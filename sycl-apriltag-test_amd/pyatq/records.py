@@ -20,8 +20,8 @@ DIFF_QUAD = np.dtype([("cluster_start", "<u4"), ("cluster_count", "<u4"), ("idx"
 COUNTS = np.dtype([(n, "<u4") for n in (
     "n_boundary", "n_pairs", "n_clusters", "n_cluster_points", "n_corners", "n_segments",
     "n_quads", "overflow", "oob_reads", "key_label_bits", "pair_filter_off",
-    "n_valid_pairs", "sort_passes")]
-    + [("reserved", "<u4", (3,))])
+    "n_valid_pairs", "sort_passes", "rank_shift")]
+    + [("reserved", "<u4", (2,))])
 
 SIZES = {
     "EXTREMA": 2, "BOUNDARY_POINT": 4, "CLUSTER_POINT": 8, "CLUSTER_EXTENTS": 8,
