@@ -27,6 +27,8 @@ sys.path.insert(0, PKG)
 
 HBM_PEAK_GBS = 8000.0
 W1080, H1080 = 1920, 1080
+COUNT_KEYS = ("n_boundary", "n_valid_pairs", "n_clusters", "n_cluster_points", "n_corners",
+              "n_segments", "n_quads", "sort_passes", "rank_shift")
 METRIC = "Mpix/s & frames/s threshold→quads (1080p, 12MP); batched vs plain; HBM GB/s %"
 
 
@@ -34,10 +36,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def stage_bytes(stage, f, npx, counts, n_shifts):
+def stage_bytes(stage, f, npx, counts):
     """Algorithmic HBM bytes of one stage for one frame (npx = decimated px):
     what the stage must read and write at least, with every intermediate in
-    HBM once."""
+    HBM once. The cluster stage has no kernels of its own (the ranking and
+    the key sort's last pass produce the clusters): None."""
     nbp = int(counts["n_boundary"])
     ncp = int(counts["n_cluster_points"])
     nc = int(counts["n_corners"])
@@ -49,9 +52,9 @@ def stage_bytes(stage, f, npx, counts, n_shifts):
     if stage == "boundary":
         return 4 * npx + 12 * nbp  # labels read + (u64 key, u32 record) written
     if stage == "key_sort":
-        return 24 * nbp * n_shifts  # per pass: key + record read and written
-    if stage == "clusters":
-        return 12 * nbp + 12 * ncp  # grouped records read, cluster point + index written
+        # per pass: key + record read and written; the last pass also writes
+        # each record's cluster point (8 B)
+        return 24 * nbp * sort_passes(npx, counts) + 8 * nbp
     if stage == "angle_fit":
         return 80 * ncp  # point read, sorted point (8 B) and line-fit prefix (64 B) written
     if stage == "fit_lines":
@@ -74,9 +77,13 @@ def stage_traffic(stage):
     return None if st is None else int(st["fetch_bytes"] + st["write_bytes"])
 
 
-def n_sort_passes(npx):
-    """LSD passes of the boundary key sort: 2 label fields + 2 plane bits in
-    digits of at most 11 bits."""
+def sort_passes(npx, counts):
+    """Radix passes of the frame's boundary key sort: the detector's own
+    count over the pair-rank keys, or, when it fell back to the full-key
+    sort (0), 2 label fields + 2 plane bits in 11-bit digits."""
+    n = int(counts["sort_passes"])
+    if n:
+        return n
     lb = 1
     while lb < 31 and (1 << lb) < npx:
         lb += 1
@@ -129,9 +136,8 @@ def bench_12mp(atq, synth, stream, steps=10, tune=True):
     c = det.counts()
     npx12 = 4096 * 3072
     roof12 = {}
-    for st in ("threshold", "ccl", "boundary", "key_sort", "clusters", "angle_fit", "fit_lines",
-               "corners"):
-        b = stage_bytes(st, 1, npx12, c, n_sort_passes(npx12))
+    for st in ("threshold", "ccl", "boundary", "key_sort", "angle_fit", "fit_lines", "corners"):
+        b = stage_bytes(st, 1, npx12, c)
         roof12[st] = {"ms": round(stages[st], 4), "GB/s": round(b / (stages[st] * 1e-3) / 1e9, 1),
                       "frac": round(b / (stages[st] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     out = {
@@ -140,9 +146,7 @@ def bench_12mp(atq, synth, stream, steps=10, tune=True):
         "mpix_per_s": round(steps * npx12 / wall / 1e6, 1),
         "stage_ms": {k: round(v, 4) for k, v in stages.items()},
         "stage_roofline": roof12,
-        "counts": {k: int(c[k]) for k in ("n_boundary", "n_pairs", "n_clusters",
-                                          "n_cluster_points", "n_corners", "n_segments",
-                                          "n_quads")},
+        "counts": {k: int(c[k]) for k in COUNT_KEYS},
     }
     log(f"12MP: {wall / steps * 1e3:.2f} ms/frame; stages {stages}")
     det.close()
@@ -277,7 +281,7 @@ def main():
     st1 = res[1]["stages"]
     dom = max(st1, key=st1.get)
     roof = None
-    alg = stage_bytes(dom, 1, npx, res[1]["counts"], n_sort_passes(npx))
+    alg = stage_bytes(dom, 1, npx, res[1]["counts"])
     traffic = stage_traffic(dom)
     if alg is not None:
         gbs = alg / (st1[dom] * 1e-3) / 1e9
@@ -289,9 +293,8 @@ def main():
                 "frac": None, "traffic": traffic, "stage": dom, "avg_ms": round(st1[dom], 5)}
 
     stage_roof = {}
-    for st in ("threshold", "ccl", "boundary", "key_sort", "clusters", "angle_fit", "fit_lines",
-               "corners"):
-        b = stage_bytes(st, 1, npx, res[1]["counts"], n_sort_passes(npx))
+    for st in ("threshold", "ccl", "boundary", "key_sort", "angle_fit", "fit_lines", "corners"):
+        b = stage_bytes(st, 1, npx, res[1]["counts"])
         stage_roof[st] = {"ms": round(st1[st], 5), "GB/s": round(b / (st1[st] * 1e-3) / 1e9, 1),
                           "frac": round(b / (st1[st] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
@@ -301,9 +304,7 @@ def main():
         "autotuned_shapes_1080p_f1": list(res[1]["shapes"]),
         "stage_ms_1080p_f1": {k: round(v, 5) for k, v in st1.items()},
         "stage_roofline_1080p_f1": stage_roof,
-        "counts_1080p_f1": {k: int(res[1]["counts"][k]) for k in
-                            ("n_boundary", "n_pairs", "n_clusters", "n_cluster_points",
-                             "n_corners", "n_segments", "n_quads")},
+        "counts_1080p_f1": {k: int(res[1]["counts"][k]) for k in COUNT_KEYS},
     }
 
     if 2 in res:

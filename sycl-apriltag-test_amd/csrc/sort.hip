@@ -126,6 +126,16 @@ k_rs_scan(uint32_t *__restrict__ hist, uint32_t nbuckets, const uint32_t *count,
     if (q == 0) totals[d] = total;
 }
 
+// The last rank pass's extras (on: keys hold ranks, key >> shift).
+struct RsFinal {
+    bool on;
+    bool fx;
+    uint32_t shift;
+    const uint64_t *xlat;
+    const float2 *centre;
+    atq_cluster_point *cpts;
+};
+
 // LDS of the scatter body.
 template <int BITS>
 struct RsScatterLds {
@@ -146,7 +156,7 @@ ATQ_DEV void rs_scatter_body(const uint64_t *__restrict__ kin, const uint32_t *_
                              uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
                              uint32_t b, uint32_t e, uint32_t shift,
                              const uint32_t *__restrict__ hist_row,
-                             const uint32_t *__restrict__ totals, const RsDyn *fin,
+                             const uint32_t *__restrict__ totals, const RsFinal fin,
                              RsScatterLds<BITS> &L) {
     constexpr int B = 1 << BITS;
     constexpr int PER = B / RS_THREADS;  // digits per thread in the base scan
@@ -221,13 +231,12 @@ ATQ_DEV void rs_scatter_body(const uint64_t *__restrict__ kin, const uint32_t *_
         // the last pass's per-cluster loads, in flight during the offsets below
         uint64_t xk[RS_ITEMS];
         float2 cc[RS_ITEMS];
-        if (fin) {
-            const uint32_t rsh = *fin->shift;
+        if (fin.on) {
 #pragma unroll
             for (int j = 0; j < RS_ITEMS; j++) {
-                const uint32_t r = dg[j] != 0xFFFFFFFFu ? (uint32_t)(k[j] >> rsh) : 0u;
-                xk[j] = fin->xlat[r];
-                cc[j] = fin->cl.centre[r];
+                const uint32_t r = dg[j] != 0xFFFFFFFFu ? (uint32_t)(k[j] >> fin.shift) : 0u;
+                xk[j] = fin.xlat[r];
+                cc[j] = fin.centre[r];
             }
         }
         __syncthreads();
@@ -246,11 +255,11 @@ ATQ_DEV void rs_scatter_body(const uint64_t *__restrict__ kin, const uint32_t *_
         for (int j = 0; j < RS_ITEMS; j++) {
             if (dg[j] != 0xFFFFFFFFu) {
                 uint32_t pos = wc[wave][dg[j]] + rank[j];
-                if (fin) {
+                if (fin.on) {
                     kout[pos] = (xk[j] << 2) | ((v[j] >> 14) & 3u);  // the record's plane
                     const uint16_t px = (uint16_t)v[j], py = (uint16_t)(v[j] >> 16);
-                    fin->cl.cpts[pos] = atq_cluster_point{
-                        px, py, calc_theta(px, py, cc[j].x, cc[j].y, fin->cl.fx != 0)};
+                    fin.cpts[pos] =
+                        atq_cluster_point{px, py, calc_theta(px, py, cc[j].x, cc[j].y, fin.fx)};
                 } else {
                     kout[pos] = k[j];
                 }
@@ -275,7 +284,7 @@ k_rs_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
     if (blockIdx.x >= G) return;
     const uint32_t b = blockIdx.x * chunk, e = min(n, b + chunk);
     rs_scatter_body<BITS>(kin, vin, kout, vout, b, e, shift, hist + (size_t)blockIdx.x * B,
-                          totals, nullptr, L);
+                          totals, RsFinal{}, L);
 }
 
 // Pair key min<<32|max <-> min<<lb|max.
@@ -522,8 +531,8 @@ ATQ_DEV void rs_full_body(const RsDyn &d, const uint32_t *count, uint32_t cap, u
             totals[dd] = run;
         }
         rs_grid_sync(bar, ++gen * G, error);
-        rs_scatter_body<BITS>(ki, vi, ko, vo, b, e, shift, hist + (size_t)g * B, totals, nullptr,
-                              L);
+        rs_scatter_body<BITS>(ki, vi, ko, vo, b, e, shift, hist + (size_t)g * B, totals,
+                              RsFinal{}, L);
         rs_grid_sync(bar, ++gen * G, error);
     }
     if ((passes & 1u) == 0) {  // the last pass wrote X
@@ -562,8 +571,19 @@ k_rs_scatter_dyn(RsDyn d, const uint32_t *count, uint32_t cap, uint32_t pass,
     rs_chunks(n, &chunk, &G);
     if (blockIdx.x >= G) return;
     const uint32_t b = blockIdx.x * chunk, e = min(n, b + chunk);
+    // the last pass's extras as plain values (taking the address of the
+    // kernel argument would copy it to scratch memory in every lane)
+    RsFinal fin{};
+    if (last) {
+        fin.on = true;
+        fin.shift = *d.shift;
+        fin.xlat = d.xlat;
+        fin.centre = d.cl.centre;
+        fin.cpts = d.cl.cpts;
+        fin.fx = d.cl.fx != 0;
+    }
     rs_scatter_body<RK_BITS>(ki, vi, ko, vo, b, e, pass * RK_BITS, hist + (size_t)blockIdx.x * B,
-                             totals, last ? &d : nullptr, L);
+                             totals, fin, L);
 }
 
 int rank_sort(hipStream_t s, uint32_t full_bits, const RsDyn &d, const uint32_t *count,
