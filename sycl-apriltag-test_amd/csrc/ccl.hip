@@ -108,14 +108,15 @@ ATQ_DEV uint32_t g_find_settled(const uint32_t *parent, uint32_t n) {
 }
 
 // Root of local root n (a FLAG-free word), halving the path on the way:
-// every visited word is lowered to its grandparent. Parents only ever
-// decrease towards the root (min-linking), so atomicMin keeps each word an
-// ancestor in the same tree while unions run concurrently.
+// every visited word is lowered to its grandparent with atomicMin, which
+// keeps it an ancestor in the same tree while other threads halve the same
+// path. Called once every union is done, so the loads may be served by the
+// caches (a stale word is still an ancestor of the same root).
 ATQ_DEV uint32_t g_find_halving(uint32_t *parent, uint32_t n) {
     while (true) {
-        const uint32_t p = g_load(&parent[n]);
+        const uint32_t p = parent[n];
         if (p == n) return n;
-        const uint32_t gp = g_load(&parent[p]);
+        const uint32_t gp = parent[p];
         if (gp == p) return p;
         atomicMin(&parent[n], gp);
         n = gp;
@@ -123,16 +124,19 @@ ATQ_DEV uint32_t g_find_halving(uint32_t *parent, uint32_t n) {
 }
 
 // The roots of a and b, both paths walked (and halved) in lockstep so that
-// their loads are in flight together.
+// their loads are in flight together. The loads may be served by the caches:
+// a stale word is still an ancestor (words only decrease towards the root),
+// so a walk ends at a node that is or was a root, and the union's atomicMin
+// returns the fresh word when that node has been linked meanwhile.
 ATQ_DEV void g_find_pair(uint32_t *parent, uint32_t &a, uint32_t &b) {
     bool ra = false, rb = false;  // root reached
     while (!ra || !rb) {
-        const uint32_t pa = ra ? a : g_load(&parent[a]);
-        const uint32_t pb = rb ? b : g_load(&parent[b]);
+        const uint32_t pa = ra ? a : parent[a];
+        const uint32_t pb = rb ? b : parent[b];
         ra = ra || pa == a;
         rb = rb || pb == b;
-        const uint32_t ga = ra ? a : g_load(&parent[pa]);
-        const uint32_t gb = rb ? b : g_load(&parent[pb]);
+        const uint32_t ga = ra ? a : parent[pa];
+        const uint32_t gb = rb ? b : parent[pb];
         if (!ra) {
             if (ga == pa) {
                 a = pa;
