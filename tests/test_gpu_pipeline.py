@@ -218,6 +218,23 @@ def test_detector_more_valid_pairs_than_ranked(atq, oracle):
     check_against_oracle(g, o)
 
 
+DEGENERATE = [
+    ("flat_64x48", np.full((48, 64), 100, np.uint8)),
+    ("noise_8x8", np.random.default_rng(7).integers(0, 256, (8, 8), dtype=np.uint8)),
+    ("one_square_32x32", np.pad(np.full((12, 12), 30, np.uint8), 10, constant_values=220)),
+]
+
+
+@pytest.mark.parametrize("name,img", DEGENERATE, ids=[x[0] for x in DEGENERATE])
+def test_detector_degenerate_frames(atq, oracle, name, img):
+    """Frames with no or few cluster pairs: the ranking, the one-pass sort
+    of zero or a handful of records and every later stage still agree with
+    the oracle element by element."""
+    o = oracle.pipeline(img, 1)
+    g = run_detector(atq, img, 1)
+    check_against_oracle(g, o)
+
+
 def test_detector_stress_12mp(atq, oracle):
     """The 4096x3072 stress frame (maze, blobs, 200 tags), element by element:
     its longest clusters exceed one work-group's LDS merge (16384 points), so
