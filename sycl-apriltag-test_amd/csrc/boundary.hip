@@ -500,11 +500,11 @@ constexpr uint32_t RANK_PENDING = 0xFFFFFFFEu;  // valid, beyond the rank list
 // Verdicts of the table's slots and the list of valid pairs, in slot order:
 // slot i holds a valid pair when its count is in [24, max_count] and its
 // reduction passes ValidBlobFilter (include/cluster_bounds.h:85-120). One
-// work-group per PV_TILE slots (4 per thread, one 16-byte count load), list
+// work-group per PV_TILE slots (2 per thread, one 8-byte count load), list
 // offsets from a look-back over the work-groups (dispatched in order);
 // counts->n_valid_pairs = the number of valid pairs, the first `cap` listed
 // with their key, slot and record count.
-constexpr int PV_ITEMS = 4;
+constexpr int PV_ITEMS = 2;
 constexpr uint32_t PV_TILE = 256 * PV_ITEMS;
 __global__ void __launch_bounds__(256)
 k_pair_valid(PairTable t, uint32_t w, uint32_t h, uint32_t max_count,
@@ -517,8 +517,8 @@ k_pair_valid(PairTable t, uint32_t w, uint32_t h, uint32_t max_count,
         next_flags[i] = 0;
     const uint32_t slots = t.mask + 1;  // a power of two >= 256
     const uint32_t tile = blockIdx.x, i0 = tile * PV_TILE + threadIdx.x * PV_ITEMS;
-    const uint4 c4 = i0 < slots ? *reinterpret_cast<const uint4 *>(t.cnt + i0) : make_uint4(0, 0, 0, 0);
-    const uint32_t cs[PV_ITEMS] = {c4.x, c4.y, c4.z, c4.w};
+    const uint2 c2 = i0 < slots ? *reinterpret_cast<const uint2 *>(t.cnt + i0) : make_uint2(0, 0);
+    const uint32_t cs[PV_ITEMS] = {c2.x, c2.y};
     uint32_t v[PV_ITEMS], sum = 0;
 #pragma unroll
     for (int k = 0; k < PV_ITEMS; k++) {
