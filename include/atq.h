@@ -399,8 +399,11 @@ enum {
     ATQ_BUF_FITTED = 13,         /* atq_fitted_quad                      */
     ATQ_BUF_QUADS = 14,          /* atq_quad_corners                     */
     ATQ_BUF_COUNTS = 15,         /* atq_counts                           */
-    ATQ_BUF_EXTREMA = 16,        /* atq_extrema (W/4)*(H/4)              */
-    ATQ_BUF_FILTERED_EXTREMA = 17,
+    ATQ_BUF_EXTREMA = 16,        /* atq_extrema (W/4)*(H/4); not written
+                                    by atq_detect_quads (no later stage
+                                    reads the tile extrema: the drop-in
+                                    atq_threshold_image writes them)     */
+    ATQ_BUF_FILTERED_EXTREMA = 17, /* as ATQ_BUF_EXTREMA                 */
     ATQ_BUF_SPARSE_CORNERS = 18, /* atq_corner, one slot per cluster point */
     ATQ_BUF_PAIR_KEYS = 19,      /* u64 ~(min<<32|max) per pair-table slot, 0 = empty */
     ATQ_BUF_PAIR_COUNTS = 20     /* u32 record count per slot (exact up to

@@ -10,21 +10,27 @@
 // it. Components are labelled by their minimum node id (min-linking).
 //
 // gfx950 organisation (three kernels, the reference has five; each takes a
-// table of frames and walks them with a persistent grid):
+// table of frames and walks them with a persistent grid). The scratch array
+// holds a parent word per node (w * h / 2 words; only live local roots and
+// the nodes of tile-border blocks are ever written) and a state word per
+// block; no per-pixel array makes a round trip through HBM but the
+// thresholded input (read twice: tiles, then border probes) and the labels.
 //  1. k_ccl_local: a work-group owns a TBX x TBY tile of blocks (TBY = 8 or
-//     16, chosen per resolution by the autotuner), stages its
-//     pixels (+1 px halo) in LDS, unions the tile-internal edges with an LDS
-//     union-find, counts each local root's pixels, and writes for every node
-//     the global id of its local root (FLAG-tagged for non-roots) and, at
-//     each local root, its pixel count into `sizes`.
+//     16, chosen per resolution by the autotuner), builds every block's
+//     mask and probes from registers, unions the tile-internal edges in an
+//     LDS union-find (lockstep walks), counts each local root's pixels and
+//     publishes a state word per block, the local roots of its border
+//     blocks' nodes, and every live local root (pixels or an edge leaving
+//     the tile) with its count in `sizes`.
 //  2. k_ccl_border: one wave per tile collects the distinct pairs of local
 //     roots its border edges join and merges each once on the global parent
 //     array with the reference's lock-free atomicMin union
 //     (include/union_find.h:50-77), halving paths as it walks.
-//  3. k_ccl_resolve: each block finds the final roots of its two nodes
-//     (path halving), a merged local root adds its pixel count into its
-//     root's size, and the block writes its four label words (white =
-//     0x80000000|root, black = root, 127 -> 0).
+//  3. k_ccl_label: a work-group per tile resolves each of its local roots
+//     that holds pixels once (path halving; a merged root adds its pixel
+//     count into its final root's size) and writes the tile's label words
+//     from the state words (white = 0x80000000|root, black = root,
+//     127 -> 0).
 #include <algorithm>
 
 #include "kernels.hpp"
@@ -177,153 +183,305 @@ ATQ_DEV void g_union(uint32_t *parent, uint32_t a, uint32_t b) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Compat labelling. Node index of the colour-c node of the block at block
+// coordinates (bx, by), i.e. pixels (2bx .. 2bx+1, 2by .. 2by+1):
+//   n = by * w + 2 * bx + c   (< w * h / 2),
+// the reference's node id (its top-left pixel index + c,
+// src/segmentation.cpp:257-259) being gid = n + by * w. Both numberings order
+// the nodes alike, so min-linking on node indices yields the reference's
+// roots, and the parent array needs only w * h / 2 words.
+ATQ_DEV uint32_t node_gid(uint32_t n, uint32_t w) { return n + (n / w) * w; }
+
+constexpr uint32_t CNT_OUT = 0x80000000u;  // the local root has an edge leaving the tile
+constexpr uint32_t ROOT_BITS = 11;         // local node index in a state word
+
+// Scratch words of a frame: the parent array (w * h / 2 words, written only
+// at live local roots and at the tile-border blocks' nodes), then one state
+// word per block (w * h / 4 words): its two nodes' local roots and its
+// pixels' colour codes.
+ATQ_DEV uint32_t *ccl_state(const FrontFrame &F) { return F.parent + (size_t)F.w * F.h / 2; }
+
 template <int TBY>
 struct LocalLds {
-    static constexpr int PR = 2 * TBY + 1;  // pixel rows incl. the halo row above
-    static constexpr int PC = 2 * TBX + 2;  // pixel cols incl. left halo and R column
     static constexpr int NODES = 2 * TBX * TBY;
-    uint8_t px[PR][PC + 2];
-    uint32_t lp[NODES];
-    uint32_t cnt[NODES];
+    uint32_t lp[NODES];   // local parent (local node indices)
+    uint32_t cnt[NODES];  // pixel count of each local root, CNT_OUT if it has an outgoing edge
+    uint8_t part[NODES];  // the node is a run head that takes part in a union pair
+    uint32_t flag[3];     // barrier_or
 };
 
-// One tile of frame F (all 256 threads).
+// Two bytes (columns x, x+1) of row y, or 127s outside the frame.
+ATQ_DEV uint32_t ld_pair(const uint8_t *thr, int x, int y, uint32_t w, uint32_t h) {
+    return (y >= 0 && y < (int)h && x < (int)w)
+               ? (uint32_t)*reinterpret_cast<const uint16_t *>(thr + (size_t)y * w + x)
+               : 0x7F7Fu;
+}
+ATQ_DEV uint32_t ld_byte(const uint8_t *thr, int x, int y, uint32_t w, uint32_t h) {
+    return (y >= 0 && y < (int)h && x >= 0 && x < (int)w) ? (uint32_t)thr[(size_t)y * w + x] : 127u;
+}
+ATQ_DEV uint32_t colour_code(uint32_t v) { return v == C_WHITE ? 1u : (v == C_BLACK ? 2u : 0u); }
+
+// Global node index of local node j of the tile at pixel (x0, y0).
+ATQ_DEV uint32_t tile_node(uint32_t j, int x0, int y0, uint32_t w) {
+    const uint32_t b = j >> 1;
+    return (uint32_t)(y0 / 2 + (int)(b / TBX)) * w + (uint32_t)x0 + 2 * (b % TBX) + (j & 1);
+}
+
+// Work-group barrier that also returns whether any thread's pred was set.
+// __syncthreads_or costs about 600 clocks here (two barriers and a
+// reduction), this one a barrier and an LDS round trip: flag f[s % 3] is set
+// by the threads whose pred holds, read after the barrier, and the flag of
+// step s + 2 (last read before this barrier) is cleared for reuse; s is the
+// work-group-uniform step counter.
+ATQ_DEV bool barrier_or(uint32_t *f, uint32_t &s, bool pred) {
+    if (pred) f[s % 3] = 1;
+    __syncthreads();
+    const bool any = f[s % 3] != 0;
+    if (threadIdx.x == 0) f[(s + 2) % 3] = 0;
+    s++;
+    return any;
+}
+
+// One tile of frame F (32 * TBY threads; thread t owns block (t % 32, t / 32)).
+//  1. Every thread loads its block's pixels and the row above them as byte
+//     pairs straight into registers (the columns left and right of a pair
+//     come from the neighbouring lanes, the tile's halo columns from extra
+//     loads of the edge lanes), then builds the block's neighbour mask and
+//     probes exactly as the reference (src/segmentation.cpp:137-255).
+//  2. A block whose S probes reach its left neighbour in colour c continues
+//     that neighbour's run: one ballot per block row and colour points every
+//     node at the first node of its run.
+//  3. The P, Q, Q, R probes join runs to the row above through an LDS
+//     union-find run in rounds of hooking (atomicMin of the larger root of
+//     each pair onto the smaller) and pointer jumping.
+//  4. Count each local root's pixels, and mark the local roots that own an
+//     edge leaving the tile.
+//  5. Publish: a state word per block (both nodes' local roots and the
+//     pixels' colour codes); the parent word of every node of the tile's
+//     border blocks (its local root) and of every live local root (itself:
+//     a root with pixels or an outgoing edge); sizes[] at live roots.
 template <int TBY>
 ATQ_DEV void ccl_local_tile(LocalLds<TBY> &L, const FrontFrame &F, uint32_t tile, uint32_t tiles_x) {
-    using Lds = LocalLds<TBY>;
+    constexpr int NODES = LocalLds<TBY>::NODES, NT = TBX * TBY;
+    static_assert(TBX == 32 && TBY % 2 == 0 && NODES <= (1 << ROOT_BITS), "a block row is a half-wave");
     const uint8_t *__restrict__ thr = F.thr;
     const uint32_t w = F.w, h = F.h;
     uint32_t *__restrict__ parent = F.parent;
-    const int tid = threadIdx.x;
-    const int x0 = (tile % tiles_x) * (2 * TBX), y0 = (tile / tiles_x) * (2 * TBY);
-    constexpr int PR = Lds::PR, PC = Lds::PC, NODES = Lds::NODES;
-    auto &px = L.px;
-    auto &lp = L.lp;
-    auto &cnt = L.cnt;
+    const int tid = threadIdx.x, lane = tid & 63, bx = tid & 31, by = tid / 32;
+    const int x0 = (int)(tile % tiles_x) * (2 * TBX), y0 = (int)(tile / tiles_x) * (2 * TBY);
+    const int x = x0 + 2 * bx, y = y0 + 2 * by;
+    const bool in_img = x < (int)w && y < (int)h;
 
-    // every load of a thread is issued before the first LDS store
-    constexpr int PER = (PR * PC + 255) / 256;
-    uint8_t vals[PER];
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
-        const int i = tid + 256 * k;
-        const int r = i / PC, c = i % PC;
-        const int y = y0 - 1 + r, x = x0 - 1 + c;
-        vals[k] = (i < PR * PC && y >= 0 && y < (int)h && x >= 0 && x < (int)w)
-                      ? thr[(size_t)y * w + x] : (uint8_t)127;
+    // 1. loads: rows y-1, y, y+1 at columns x, x+1; the tile's halo columns
+    // x0-1 (edge lane 0) and x0+64 (edge lane 31)
+    const uint32_t ya = ld_pair(thr, x, y - 1, w, h);
+    const uint32_t yt = ld_pair(thr, x, y, w, h);
+    const uint32_t yb = ld_pair(thr, x, y + 1, w, h);
+    uint32_t el0 = 127u, el1 = 127u, el2 = 127u, er = 127u;
+    if (bx == 0) {
+        el0 = ld_byte(thr, x0 - 1, y - 1, w, h);
+        el1 = ld_byte(thr, x0 - 1, y, w, h);
+        el2 = ld_byte(thr, x0 - 1, y + 1, w, h);
     }
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
-        const int i = tid + 256 * k;
-        if (i < PR * PC) px[i / PC][i % PC] = vals[k];
-    }
-    for (int i = tid; i < NODES; i += 256) {
-        lp[i] = i;
-        cnt[i] = 0;
-    }
-    __syncthreads();
+    if (bx == TBX - 1) er = ld_byte(thr, x0 + 2 * TBX, y - 1, w, h);
+    L.cnt[2 * tid] = 0;
+    L.cnt[2 * tid + 1] = 0;
+    L.part[2 * tid] = 0;
+    L.part[2 * tid + 1] = 0;
+    if (tid < 3) L.flag[tid] = 0;
 
-    // Tile-internal unions in two steps. (a) Each block row lives in one
-    // half-wave (32 lanes = TBX blocks): a block whose S probes reach its
-    // left neighbour in colour c continues that neighbour's run, so with one
-    // ballot every node points straight at the first node of its run (the
-    // run's minimum id). (b) The P, Q, Q, R probes join runs to the row
-    // above through the LDS union-find; a union that repeats the previous
-    // lane's (run, run above) pair is skipped.
-    static_assert(TBX == 32 && TBY % 8 == 0, "a block row is a half-wave");
-    const int lane = tid & 63;
+    // probes (2 bits each: 0 not tested or 127, 1 white, 2 black), runs;
+    // the left neighbour's column x-1 (rows y-1, y, y+1) and the right
+    // neighbour's column x+2 (row y-1)
+    const uint32_t lpk = __shfl_up(ya | (yt << 16), 1, 32);
+    const uint32_t lb = __shfl_up(yb, 1, 32);
+    const uint32_t rpk = __shfl_down(ya, 1, 32);
+    const uint32_t pP = bx ? (lpk >> 8) & 0xFFu : el0;
+    const uint32_t pS0 = bx ? lpk >> 24 : el1;
+    const uint32_t pS1 = bx ? (lb >> 8) & 0xFFu : el2;
+    const uint32_t pR = bx < TBX - 1 ? rpk & 0xFFu : er;
+    const uint32_t probe[6] = {pP, ya & 0xFFu, ya >> 8, pR, pS0, pS1};
+    uint32_t code = 0;
+    if (in_img) {
+        const uint32_t m = check_mask((uint8_t)(yt & 0xFFu), (uint8_t)(yt >> 8), (uint8_t)(yb & 0xFFu),
+                                      (uint32_t)x, (uint32_t)y, w, h);
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            constexpr uint32_t BIT[6] = {0, 1, 2, 3, 4, 8};
+            if (m & (1u << BIT[k])) code |= colour_code(probe[k]) << (2 * k);
+        }
+    }
+    // edges leaving the tile: P (left or upper tiles), Q (tile above), R
+    // (right or upper tiles), S (left tile)
+    uint32_t out = 0;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const bool cross = k == 0   ? (by == 0 || bx == 0)
+                           : k < 3  ? by == 0
+                           : k == 3 ? (by == 0 || bx == TBX - 1)
+                                    : bx == 0;
+        const uint32_t c = (code >> (2 * k)) & 3u;
+        if (cross && c) out |= 1u << (c - 1);
+    }
     const uint64_t half = lane < 32 ? 0x00000000FFFFFFFFull : 0xFFFFFFFF00000000ull;
     const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
-    uint32_t pv[TBY / 8];  // probe pixels of this thread's blocks: 2 bits per probe
-                           // (0: not tested or 127, 1: white, 2: black)
+    uint32_t head[2];
 #pragma unroll
-    for (int pass = 0; pass < TBY / 8; pass++) {
-        const int b = tid + 256 * pass, bx = b % TBX, by = b / TBX;
-        const int x = x0 + 2 * bx, y = y0 + 2 * by;
-        const bool in_img = x < (int)w && y < (int)h;
-        const int pr = 1 + 2 * by, pc = 1 + 2 * bx;
-        uint32_t m = 0, v2 = 0;
-        if (in_img) {
-            m = check_mask(px[pr][pc], px[pr][pc + 1], px[pr + 1][pc], x, y, w, h);
-#pragma unroll
-            for (int k = 0; k < 6; k++) {
-                const Probe q = PROBES[k];
-                if (!(m & (1u << q.bit))) continue;
-                const uint8_t v = px[pr + q.dy][pc + q.dx];
-                if (v == C_WHITE) v2 |= 1u << (2 * k);
-                else if (v == C_BLACK) v2 |= 2u << (2 * k);
-            }
-        }
-        pv[pass] = v2;
-#pragma unroll
-        for (uint32_t c = 0; c < 2; c++) {
-            // S probes are k = 4, 5; both reach the left neighbour (0, -1)
-            const uint32_t code = c + 1;
-            const bool left = bx > 0 && (((v2 >> 8) & 3u) == code || ((v2 >> 10) & 3u) == code);
-            const uint64_t starts = __ballot(!left) & half & upto;
-            const int run0 = (63 - __clzll((long long)starts)) & 31;
-            lp[2 * (by * TBX + bx) + c] = 2 * (by * TBX + run0) + c;
-        }
+    for (uint32_t c = 0; c < 2; c++) {
+        const uint32_t cc = c + 1;
+        const bool left = bx > 0 && (((code >> 8) & 3u) == cc || ((code >> 10) & 3u) == cc);
+        const uint64_t starts = __ballot(!left) & half & upto;
+        const int run0 = (63 - __clzll((long long)starts)) & 31;
+        head[c] = 2 * (by * TBX + run0) + c;
+        L.lp[2 * tid + c] = head[c];
     }
     __syncthreads();
+
+    // 3. unions with the row above (P, Q, Q, R), in rounds over the run
+    // heads that take part in a pair (the local parent array restricted to
+    // them is a forest of stars at the start of each round): every pair
+    // hooks the larger of its two roots under the smaller with atomicMin
+    // (if that root was hooked meanwhile, the pair of the smaller root and
+    // the word atomicMin found joins in the next round), then pointer
+    // jumping turns the forest into stars again; until no pair is left. Concurrent unions with retries would instead walk the chains
+    // that simultaneous hooks of consecutive rows build, one LDS round trip
+    // per step. Nodes that are not run heads keep pointing at their head.
+    uint32_t A[4], B[4], vm = 0;
 #pragma unroll
-    for (int pass = 0; pass < TBY / 8; pass++) {
-        const int b = tid + 256 * pass, bx = b % TBX, by = b / TBX;
-        const uint32_t v2 = pv[pass];
+    for (int k = 0; k < 4; k++) {
+        const int nbx = bx + (k == 0 ? -1 : (k == 3 ? 1 : 0));
+        const uint32_t cd = (code >> (2 * k)) & 3u;
+        uint32_t a = 0xFFFFFFFFu, b = 0xFFFFFFFFu;
+        if (cd && by > 0 && nbx >= 0 && nbx < TBX) {
+            a = head[cd - 1];
+            b = L.lp[2 * ((by - 1) * TBX + nbx) + cd - 1];
+        }
+        const uint32_t pa = __shfl_up(a, 1, 64), pb = __shfl_up(b, 1, 64);
+        bool dup = (lane & 31) > 0 && pa == a && pb == b;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {  // P, Q, Q, R: the row above
-            const Probe q = PROBES[k];
-            const uint32_t code = (v2 >> (2 * k)) & 3u;
-            const int nbx = bx + q.dbx, nby = by + q.dby;
-            uint32_t ra = 0xFFFFFFFFu, rb = 0xFFFFFFFFu;
-            if (code && nbx >= 0 && nbx < TBX && nby >= 0) {
-                const uint32_t c = code - 1;
-                ra = lp[2 * (by * TBX + bx) + c];
-                rb = lp[2 * (nby * TBX + nbx) + c];
-            }
-            const uint32_t pa = __shfl_up(ra, 1, 64), pb = __shfl_up(rb, 1, 64);
-            const bool dup = (lane & 31) > 0 && pa == ra && pb == rb;
-            if (ra != 0xFFFFFFFFu && !dup) lds_union(lp, ra, rb);
+        for (int q = 0; q < k; q++) dup = dup || (((vm >> q) & 1u) && A[q] == a && B[q] == b);
+        A[k] = a;
+        B[k] = b;
+        if (a != 0xFFFFFFFFu && !dup) {
+            vm |= 1u << k;
+            L.part[a] = 1;
+            L.part[b] = 1;
         }
     }
-    __syncthreads();
-    for (int i = tid; i < NODES; i += 256) lp[i] = lds_find(lp, i);
-    __syncthreads();
-    // pixel counts per local root
-    for (int b = tid; b < TBX * TBY; b += 256) {
-        int bx = b % TBX, by = b / TBX;
-        int x = x0 + 2 * bx, y = y0 + 2 * by;
-        if (x >= (int)w || y >= (int)h) continue;
-        const int pr = 1 + 2 * by, pc = 1 + 2 * bx;
-        uint8_t p4[4] = {px[pr][pc], px[pr][pc + 1], px[pr + 1][pc], px[pr + 1][pc + 1]};
+    const uint32_t n0 = 2 * tid, n1 = 2 * tid + 1;
+    bool jump0 = false, jump1 = false;  // this thread's nodes are heads in a pair
+    uint32_t step = 0;
+    for (bool first = true;; first = false) {
+        uint32_t ra[4], rb[4], lo[4], hi[4], old[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if ((vm >> i) & 1u) {
+                ra[i] = L.lp[A[i]];
+                rb[i] = L.lp[B[i]];
+            }
+        }
+        uint32_t att = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (!((vm >> i) & 1u)) continue;
+            if (ra[i] == rb[i]) {
+                vm &= ~(1u << i);  // joined
+            } else {
+                att |= 1u << i;
+                lo[i] = min(ra[i], rb[i]);
+                hi[i] = max(ra[i], rb[i]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if ((att >> i) & 1u) old[i] = atomicMin(&L.lp[hi[i]], lo[i]);
+        // a hook whose root was hooked meanwhile (old != hi) leaves the
+        // pair (lo, old) to join next round; every other hook is final
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (!((att >> i) & 1u)) continue;
+            if (old[i] == hi[i]) {
+                vm &= ~(1u << i);
+            } else {
+                A[i] = lo[i];
+                B[i] = old[i];
+            }
+        }
+        const bool again = barrier_or(L.flag, step, vm != 0);
+        if (first) {
+            jump0 = L.part[n0] != 0;
+            jump1 = L.part[n1] != 0;
+        }
+        // pointer jumping, unsynchronised: this round's hooks are done, so
+        // every walk ends at its root (a word another thread lowers
+        // meanwhile still names an ancestor); both walks in lockstep
+        uint32_t p0 = jump0 ? L.lp[n0] : n0, p1 = jump1 ? L.lp[n1] : n1;
+        while (true) {
+            const uint32_t g0 = jump0 ? L.lp[p0] : p0, g1 = jump1 ? L.lp[p1] : p1;
+            if (g0 == p0 && g1 == p1) break;
+            p0 = g0;
+            p1 = g1;
+        }
+        if (jump0) L.lp[n0] = p0;
+        if (jump1) L.lp[n1] = p1;
+        __syncthreads();
+        if (!again) break;
+    }
+    // the heads are stars: every node's local root is its head's parent
+    const uint32_t r0 = L.lp[head[0]], r1 = L.lp[head[1]];
+
+    // pixel counts and outgoing edges per local root (the roots of this
+    // block's nodes are r0, r1)
+    {
         uint32_t nw = 0, nb = 0;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            nw += p4[k] == C_WHITE;
-            nb += p4[k] == C_BLACK;
+        for (int j = 0; j < 2; j++) {
+            const uint32_t t = (yt >> (8 * j)) & 0xFFu, u = (yb >> (8 * j)) & 0xFFu;
+            nw += (t == C_WHITE) + (u == C_WHITE);
+            nb += (t == C_BLACK) + (u == C_BLACK);
         }
-        if (nw) atomicAdd(&cnt[lp[2 * b]], nw);
-        if (nb) atomicAdd(&cnt[lp[2 * b + 1]], nb);
+        if (nw) atomicAdd(&L.cnt[r0], nw);
+        if (nb) atomicAdd(&L.cnt[r1], nb);
+        if (out & 1u) atomicOr(&L.cnt[r0], CNT_OUT);
+        if (out & 2u) atomicOr(&L.cnt[r1], CNT_OUT);
     }
     __syncthreads();
-    // publish: parent word per node, size per local root
-    for (int n = tid; n < NODES; n += 256) {
-        int b = n >> 1, c = n & 1;
-        int bx = b % TBX, by = b / TBX;
-        int x = x0 + 2 * bx, y = y0 + 2 * by;
-        if (x >= (int)w || y >= (int)h) continue;
-        uint32_t r = lp[n];
-        int rb = r >> 1;
-        uint32_t gr = (uint32_t)(y0 + 2 * (rb / TBX)) * w + (uint32_t)(x0 + 2 * (rb % TBX)) + (r & 1);
-        uint32_t gn = (uint32_t)y * w + (uint32_t)x + c;
-        parent[gn] = r == (uint32_t)n ? gr : (gr | FLAG);
-        if (r == (uint32_t)n) F.sizes[gn] = cnt[n];
+
+    // 5. publish
+    if (in_img) {
+        uint32_t codes = 0;
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            codes |= colour_code((yt >> (8 * j)) & 0xFFu) << (2 * j);
+            codes |= colour_code((yb >> (8 * j)) & 0xFFu) << (2 * j + 4);
+        }
+        ccl_state(F)[(size_t)(y / 2) * (w / 2) + (uint32_t)(x / 2)] =
+            r0 | (r1 << ROOT_BITS) | (codes << (2 * ROOT_BITS));
+        if (by == 0 || by == TBY - 1 || bx == 0 || bx == TBX - 1) {
+            const uint32_t n = (uint32_t)(y / 2) * w + (uint32_t)x;
+            parent[n] = tile_node(r0, x0, y0, w);
+            parent[n + 1] = tile_node(r1, x0, y0, w);
+        }
     }
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        const int j = 2 * tid + c;
+        const uint32_t cn = L.cnt[j];
+        if ((c ? r1 : r0) != (uint32_t)j || cn == 0) continue;
+        const uint32_t n = tile_node(j, x0, y0, w);
+        parent[n] = n;
+        F.sizes[node_gid(n, w)] = cn & ~CNT_OUT;
+    }
+    // sizes[0] stands for 127 pixels and out-of-image neighbours
+    // (src/find_boundaries.cpp:49-55): node 0 is always a root
+    if (tile == 0 && tid == 0) F.sizes[0] = L.cnt[0] & ~CNT_OUT;
+    (void)NT;
     __syncthreads();  // the LDS is reused by the next tile
 }
 
 template <int TBY>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(TBX * TBY) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_ccl_local(FrontTable tab) {
     __shared__ LocalLds<TBY> L;
     for (uint32_t fi = 0; fi < tab.n; fi++) {
@@ -339,11 +497,12 @@ k_ccl_local(FrontTable tab) {
 
 // One wave per tile; lane k < 62 takes border block k (top row, then the
 // left and right columns) and derives the edges that leave the tile. Each
-// edge is a pair of local roots (a parent word of a non-root points straight
-// at its local root); the wave keeps the distinct pairs in an LDS hash set
-// and only the lane that inserts a pair merges it on the global parent array
-// (lock-free atomicMin union, include/union_find.h:50-77). A tile border is
-// crossed by few components, so a tile issues a handful of global unions.
+// edge is a pair of local roots (the parent word of a border block's node
+// names its local root); the wave keeps the distinct pairs in an LDS hash
+// set and only the lane that inserts a pair merges it on the global parent
+// array (lock-free atomicMin union, include/union_find.h:50-77). A tile
+// border is crossed by few components, so a tile issues a handful of
+// global unions.
 template <int TBY>
 constexpr int border_items() { return TBX + 2 * (TBY - 1); }
 constexpr int BORDER_SLOTS = 512;
@@ -373,9 +532,9 @@ ATQ_DEV void ccl_border_tile(uint64_t *hs, const FrontFrame &F, uint32_t tile, u
     }
     // One round of independent loads: the block's three mask pixels, its two
     // parent words, and for every probe that leaves the tile its pixel and
-    // the two parent words of the block holding it (parent words of local
-    // roots may be lowered by concurrent unions; any value read, a cached
-    // one too, is an ancestor in the same tree, which is all a union needs).
+    // the two parent words of the block holding it (the words name local
+    // roots, which concurrent unions may lower; any value read, a cached one
+    // too, is an ancestor in the same tree, which is all a union needs).
     uint8_t tl = 127, tr = 127, bl = 127;
     uint32_t own[2] = {0, 0}, nb[6][2];
     uint8_t pv[6];
@@ -385,7 +544,7 @@ ATQ_DEV void ccl_border_tile(uint64_t *hs, const FrontFrame &F, uint32_t tile, u
         tl = row[x];
         tr = row[x + 1];
         bl = row[x + w];
-        const uint32_t id = (uint32_t)y * w + (uint32_t)x;
+        const uint32_t id = (uint32_t)(y / 2) * w + (uint32_t)x;
         own[0] = parent[id];
         own[1] = parent[id + 1];
     }
@@ -400,7 +559,7 @@ ATQ_DEV void ccl_border_tile(uint64_t *hs, const FrontFrame &F, uint32_t tile, u
         nb[i][0] = nb[i][1] = 0;
         if (cross[i]) {
             pv[i] = thr[(size_t)py * w + px];
-            const uint32_t nid = (uint32_t)ny * w + (uint32_t)nx;
+            const uint32_t nid = (uint32_t)(ny / 2) * w + (uint32_t)nx;
             nb[i][0] = parent[nid];
             nb[i][1] = parent[nid + 1];
         }
@@ -414,9 +573,8 @@ ATQ_DEV void ccl_border_tile(uint64_t *hs, const FrontFrame &F, uint32_t tile, u
         const uint8_t v = pv[i];
         if (cross[i] && (m & (1u << PROBES[i].bit)) && (v == C_WHITE || v == C_BLACK)) {
             const uint32_t c = v == C_WHITE ? 0 : 1;
-            const uint32_t a = own[c] & ~FLAG, b = nb[i][c] & ~FLAG;
-            ea[i] = min(a, b);
-            eb[i] = max(a, b);
+            ea[i] = min(own[c], nb[i][c]);
+            eb[i] = max(own[c], nb[i][c]);
         }
     }
 #pragma unroll
@@ -458,57 +616,157 @@ k_ccl_border(FrontTable tab) {
     }
 }
 
-// After the border merge, one thread per block: the final root of each of
-// its two nodes (a non-root node's word names its local root; a local root's
-// word is itself or, if the border pass merged it, a node above it) and the
-// four label words (white = 0x80000000 | root, black = root, 127 -> 0). A
-// merged local root halves its path on the way and adds its pixel count into
-// its final root's size; non-root nodes only read, since every block of a
-// component walks the same path. Once a size reaches size_cap further adds
-// are skipped (the detector only asks whether a size reaches
-// min_component_size; the drop-in passes UINT32_MAX for exact sizes).
-// Work items: one 256-block run of a block row.
-ATQ_DEV uint32_t block_runs_x(uint32_t w) { return ((w / 2) + 255) / 256; }
+// Labels of TPW tiles after the border merge (all 256 threads; the tiles
+// are interleaved so that each thread has several independent loads in
+// flight per step). Every local root that holds pixels is resolved once:
+// its final root is found on the parent array (path halving, all of the
+// thread's walks in lockstep), a root merged by the border pass adds its
+// pixel count into the final root's size, and the final root's reference
+// id goes to an LDS table; then every block's four label words are written
+// from its state word (white = 0x80000000 | root, black = root, 127 -> 0),
+// two blocks per thread per row, 16 bytes per store. Once a size reaches
+// size_cap further adds are skipped (the detector only asks whether a size
+// reaches min_component_size; the drop-in passes UINT32_MAX for exact
+// sizes).
+template <int TBY>
+constexpr int label_tiles() { return TBY == 8 ? 2 : 1; }
 
+template <int TBY>
+ATQ_DEV void ccl_label_tiles(uint32_t *table, const FrontFrame &F, uint32_t tile0, uint32_t n_tiles,
+                             uint32_t tiles_x) {
+    constexpr int TPW = label_tiles<TBY>();
+    constexpr int NODES = 2 * TBX * TBY, PAIRS = TBX * TBY / 2;
+    constexpr int PPT = TPW * PAIRS / 256, NPT = TPW * NODES / 256;
+    static_assert(TPW * PAIRS % 256 == 0, "whole pairs per thread");
+    const uint32_t w = F.w, h = F.h, size_cap = F.size_cap;
+    uint32_t *__restrict__ parent = F.parent;
+    uint32_t *__restrict__ sizes = F.sizes;
+    const uint32_t *__restrict__ state = ccl_state(F);
+    const int tid = threadIdx.x;
+    auto origin = [&](int sub, int &x0, int &y0) {
+        const uint32_t t = tile0 + (uint32_t)sub;
+        x0 = (int)(t % tiles_x) * (2 * TBX);
+        y0 = (int)(t / tiles_x) * (2 * TBY);
+    };
+    uint2 st[PPT];
+    bool in[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        const int i = tid + 256 * k, sub = i / PAIRS, pi = i % PAIRS;
+        int x0, y0;
+        origin(sub, x0, y0);
+        const int by = pi / (TBX / 2), bx = 2 * (pi % (TBX / 2));
+        const int x = x0 + 2 * bx, y = y0 + 2 * by;
+        // w is a multiple of 4: both blocks of a pair are in the frame or neither
+        in[k] = tile0 + (uint32_t)sub < n_tiles && x < (int)w && y < (int)h;
+        st[k] = in[k] ? *reinterpret_cast<const uint2 *>(state + (size_t)(y / 2) * (w / 2) + (uint32_t)(x / 2))
+                      : make_uint2(0u, 0u);
+    }
+    for (int j = tid; j < TPW * NODES; j += 256) table[j] = 0;
+    __syncthreads();
+    constexpr uint32_t RM = (1u << ROOT_BITS) - 1;
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        if (!in[k]) continue;
+        uint32_t *tb = table + ((tid + 256 * k) / PAIRS) * NODES;
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const uint32_t s = e ? st[k].y : st[k].x, codes = s >> (2 * ROOT_BITS);
+            // codes: 2 bits per pixel, 01 white, 10 black
+            if (codes & 0x55u) tb[s & RM] = 1;
+            if (codes & 0xAAu) tb[(s >> ROOT_BITS) & RM] = 1;
+        }
+    }
+    __syncthreads();
+    {
+        uint32_t n[NPT], cur[NPT], act = 0;
+#pragma unroll
+        for (int k = 0; k < NPT; k++) {
+            const int j = tid + 256 * k, sub = j / NODES;
+            int x0, y0;
+            origin(sub, x0, y0);
+            n[k] = tile_node((uint32_t)(j % NODES), x0, y0, w);
+            cur[k] = n[k];
+            if (table[j]) act |= 1u << k;
+        }
+        // final roots, path halving (every word lowered stays an ancestor)
+        uint32_t todo = act;
+        while (todo) {
+            uint32_t p[NPT], g[NPT];
+#pragma unroll
+            for (int k = 0; k < NPT; k++)
+                if ((todo >> k) & 1u) p[k] = parent[cur[k]];
+#pragma unroll
+            for (int k = 0; k < NPT; k++)
+                if ((todo >> k) & 1u) g[k] = p[k] == cur[k] ? p[k] : parent[p[k]];
+#pragma unroll
+            for (int k = 0; k < NPT; k++) {
+                if (!((todo >> k) & 1u)) continue;
+                if (p[k] == cur[k]) {
+                    todo &= ~(1u << k);
+                } else if (g[k] == p[k]) {
+                    cur[k] = p[k];
+                    todo &= ~(1u << k);
+                } else {
+                    atomicMin(&parent[cur[k]], g[k]);
+                    cur[k] = g[k];
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NPT; k++) {
+            if (!((act >> k) & 1u)) continue;
+            const uint32_t rg = node_gid(cur[k], w);
+            if (cur[k] != n[k]) {
+                const uint32_t sz = sizes[node_gid(n[k], w)];
+                if (sz && g_load(&sizes[rg]) < size_cap) atomicAdd(&sizes[rg], sz);
+            }
+            table[tid + 256 * k] = rg;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        if (!in[k]) continue;
+        const int i = tid + 256 * k, sub = i / PAIRS, pi = i % PAIRS;
+        const uint32_t *tb = table + sub * NODES;
+        int x0, y0;
+        origin(sub, x0, y0);
+        const int by = pi / (TBX / 2), bx = 2 * (pi % (TBX / 2));
+        const int x = x0 + 2 * bx, y = y0 + 2 * by;
+        uint32_t top[4], bot[4];
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const uint32_t s = e ? st[k].y : st[k].x, codes = s >> (2 * ROOT_BITS);
+            const uint32_t lw = LABEL_PIXEL_MASK | tb[s & RM], lb = tb[(s >> ROOT_BITS) & RM];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t c = (codes >> (2 * j)) & 3u;
+                const uint32_t lab = c == 1 ? lw : (c == 2 ? lb : 0u);
+                if (j < 2) top[2 * e + j] = lab;
+                else bot[2 * e + j - 2] = lab;
+            }
+        }
+        *reinterpret_cast<uint4 *>(F.labels + (size_t)y * w + x) = make_uint4(top[0], top[1], top[2], top[3]);
+        *reinterpret_cast<uint4 *>(F.labels + (size_t)(y + 1) * w + x) = make_uint4(bot[0], bot[1], bot[2], bot[3]);
+    }
+    __syncthreads();  // the table is reused by the next tiles
+}
+
+template <int TBY>
 __global__ void __launch_bounds__(256)
-k_ccl_resolve(FrontTable tab) {
+k_ccl_label(FrontTable tab) {
+    constexpr int TPW = label_tiles<TBY>();
+    __shared__ uint32_t table[TPW * 2 * TBX * TBY];
     for (uint32_t fi = 0; fi < tab.n; fi++) {
         const FrontFrame &F = tab.fr[fi];
         front_clear(F);  // the later stages' tables and counts
-        uint32_t *__restrict__ parent = F.parent;
-        uint32_t *__restrict__ sizes = F.sizes;
-        const uint8_t *__restrict__ thr = F.thr;
-        uint32_t *__restrict__ labels = F.labels;
-        const uint32_t w = F.w, h = F.h, runs_x = block_runs_x(w), size_cap = F.size_cap;
-        for (uint32_t it = blockIdx.x; it < runs_x * (h / 2); it += gridDim.x) {
-            const uint32_t bx = (it % runs_x) * 256 + threadIdx.x, by = it / runs_x;
-            if (2 * bx >= w) continue;
-            const uint32_t id = 2 * by * w + 2 * bx;
-            const uint2 v = *reinterpret_cast<const uint2 *>(parent + id);
-            const uint16_t top = *reinterpret_cast<const uint16_t *>(thr + id);
-            const uint16_t bot = *reinterpret_cast<const uint16_t *>(thr + id + w);
-            uint32_t r[2];
-#pragma unroll
-            for (uint32_t c = 0; c < 2; c++) {
-                const uint32_t n = id + c, word = c ? v.y : v.x;
-                if (word & FLAG) {
-                    r[c] = g_find_settled(parent, word & ~FLAG);  // read-only: many blocks share the path
-                } else if (word == n) {
-                    r[c] = n;
-                } else {
-                    r[c] = g_find_halving(parent, word);
-                    const uint32_t sz = sizes[n];
-                    if (sz && g_load(&sizes[r[c]]) < size_cap) atomicAdd(&sizes[r[c]], sz);
-                }
-            }
-            auto lab = [&](uint32_t p) -> uint32_t {
-                p &= 0xFFu;
-                if (p == C_WHITE) return LABEL_PIXEL_MASK | r[0];
-                if (p == C_BLACK) return r[1];
-                return 0u;
-            };
-            *reinterpret_cast<uint2 *>(labels + id) = make_uint2(lab(top), lab(top >> 8));
-            *reinterpret_cast<uint2 *>(labels + id + w) = make_uint2(lab(bot), lab(bot >> 8));
+        const uint32_t tiles_x = (F.w + 2 * TBX - 1) / (2 * TBX);
+        const uint32_t n_tiles = tiles_x * ((F.h + 2 * TBY - 1) / (2 * TBY));
+        const uint32_t groups = (n_tiles + TPW - 1) / TPW;
+        for (uint32_t it = blockIdx.x; it < xcd_grid(groups); it += gridDim.x) {
+            const uint32_t g = xcd_tile(it, groups);
+            if (g < groups) ccl_label_tiles<TBY>(table, F, g * TPW, n_tiles, tiles_x);
         }
     }
 }
@@ -516,21 +774,19 @@ k_ccl_resolve(FrontTable tab) {
 namespace {
 template <int TBY>
 void ccl_launch(hipStream_t s, const FrontTable &tab) {
-    uint32_t tiles = 0, runs = 0;
+    uint32_t tiles = 0;
     for (uint32_t i = 0; i < tab.n; i++) {
         const uint32_t w = tab.fr[i].w, h = tab.fr[i].h;
-        const uint32_t nt = atq_div_up(w, 2 * TBX) * atq_div_up(h, 2 * TBY);
-        tiles = std::max(tiles, nt);
-        runs = std::max(runs, (uint32_t)atq_div_up(w / 2, 256) * (h / 2));
+        tiles = std::max(tiles, (uint32_t)(atq_div_up(w, 2 * TBX) * atq_div_up(h, 2 * TBY)));
     }
     // persistent grids: the largest frame's items, capped well above what
     // the device holds at once (frames up to 8192 x 8192 loop a few times)
     const uint32_t lg = std::min<uint32_t>(xcd_grid(tiles), 8192u);
     const uint32_t bg = std::min<uint32_t>(atq_div_up(tiles, 4), 4096u);
-    const uint32_t fg = std::min<uint32_t>(runs, 16384u);
-    k_ccl_local<TBY><<<lg ? lg : 8, 256, 0, s>>>(tab);
+    k_ccl_local<TBY><<<lg ? lg : 8, TBX * TBY, 0, s>>>(tab);
     k_ccl_border<TBY><<<bg ? bg : 1, 256, 0, s>>>(tab);
-    k_ccl_resolve<<<fg ? fg : 1, 256, 0, s>>>(tab);
+    const uint32_t labg = std::min<uint32_t>(xcd_grid(atq_div_up(tiles, label_tiles<TBY>())), 8192u);
+    k_ccl_label<TBY><<<labg ? labg : 8, 256, 0, s>>>(tab);
 }
 }  // namespace
 

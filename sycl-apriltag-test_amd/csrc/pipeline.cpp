@@ -181,8 +181,9 @@ FrontFrame front_frame(const atq_detector *d, const uint8_t *gray_in, size_t w_i
     FrontFrame fr{};
     fr.in = gray_in;
     fr.gray = d->gray;
-    fr.ext = d->ext;
-    fr.fil = d->fil;
+    // no later stage reads the tile extrema: the detector does not write them
+    fr.ext = nullptr;
+    fr.fil = nullptr;
     fr.thr = d->thr;
     fr.parent = d->parent;
     fr.labels = d->labels;
