@@ -218,6 +218,25 @@ int atq_image_segmentation(void *stream, const uint8_t *thresholded,
                            uint32_t *sizes, size_t sizes_elem, size_t width,
                            size_t height);
 
+/* Largest frame (width * height decimated pixels) that
+ * atq_image_segmentation_static serves. */
+#define ATQ_SEGMENTATION_STATIC_MAX_PIXELS 4194304u
+
+/* atq_image_segmentation without label_scratch, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). No counterpart in the reference.
+ * The scratch is a 12 MiB array in the library's device memory, one per GPU
+ * the library runs on, held until the process ends; this call allocates
+ * nothing. Results equal atq_image_segmentation's. Frames of more than
+ * ATQ_SEGMENTATION_STATIC_MAX_PIXELS pixels return ATQ_ERR_CAPACITY. Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (on different streams or host threads) race and their results are void;
+ * order them on one stream. */
+int atq_image_segmentation_static(void *stream, const uint8_t *thresholded,
+                                  uint32_t *labels, uint32_t *sizes,
+                                  size_t sizes_elem, size_t width,
+                                  size_t height);
+
 /* Replaces find_boundaries(q, labels, sizes, points, blob_labels, width,
  * height, deps)  include/find_boundaries.h:8-12. Writes the 4 sparse planes
  * of 4*width*height entries exactly as the reference leaves them after its

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 
 #include "kernels.hpp"
 #include "pipeline.hpp"
@@ -11,6 +12,34 @@ using namespace atq;
 namespace {
 
 bool g_ready = false;
+
+// Scratch of atq_image_segmentation_static: the CCL's parent and state words
+// (3/4 word per pixel) for frames up to ATQ_SEGMENTATION_STATIC_MAX_PIXELS.
+// Every word the CCL reads is written earlier in the same call, so nothing
+// needs clearing between calls.
+constexpr size_t STATIC_SCRATCH_WORDS = (size_t)ATQ_SEGMENTATION_STATIC_MAX_PIXELS / 4 * 3;
+__device__ __align__(256) uint32_t scratch[STATIC_SCRATCH_WORDS];
+
+// The array's device address on each device (host code must not pass the
+// symbol itself to a kernel: it names a host-side stand-in).
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+uint32_t *g_scratch[MAX_DEVICES] = {};
+
+// The address of `scratch` on the current device, resolved once per device
+// with that device current.
+int scratch_on_current_device(uint32_t **out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return ATQ_ERR_HIP;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[dev]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return ATQ_ERR_HIP;
+        g_scratch[dev] = static_cast<uint32_t *>(ptr);
+    }
+    *out = g_scratch[dev];
+    return ATQ_OK;
+}
 
 int check_dims(size_t w, size_t h) {
     if (w < 8 || h < 8 || w > 8192 || h > 8192 || (w & 3) || (h & 3)) return ATQ_ERR_BAD_DIMS;
@@ -36,6 +65,8 @@ int atq_init(void) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return ATQ_ERR_HIP;
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return ATQ_ERR_NO_DEVICE;
+    uint32_t *scr = nullptr;  // the static segmentation scratch of this device
+    if (int r = scratch_on_current_device(&scr)) return r;
     g_ready = true;
     return ATQ_OK;
 }
@@ -100,6 +131,19 @@ int atq_image_segmentation(void *stream, const uint8_t *thresholded, uint32_t *l
     if (int r = check_dims(width, height)) return r;
     if (int r = ensure_init()) return r;
     return launch_ccl(S(stream), thresholded, width, height, label_scratch, labels, sizes);
+}
+
+int atq_image_segmentation_static(void *stream, const uint8_t *thresholded, uint32_t *labels,
+                                  uint32_t *sizes, size_t sizes_elem, size_t width,
+                                  size_t height) {
+    (void)sizes_elem;
+    if (!thresholded || !labels || !sizes) return ATQ_ERR_BAD_ARG;
+    if (int r = check_dims(width, height)) return r;
+    if (width * height > ATQ_SEGMENTATION_STATIC_MAX_PIXELS) return ATQ_ERR_CAPACITY;
+    if (int r = ensure_init()) return r;
+    uint32_t *scr = nullptr;
+    if (int r = scratch_on_current_device(&scr)) return r;
+    return launch_ccl(S(stream), thresholded, width, height, scr, labels, sizes);
 }
 
 int atq_find_boundaries(void *stream, const uint32_t *labels, const uint32_t *sizes,

@@ -32,6 +32,7 @@ SIGNATURES = {
     "atq_threshold_image": (C.c_int, [_P, _P, _P, _P, _P, _S, _S]),
     "atq_decimate_threshold": (C.c_int, [_P, _P, _S, _S, C.c_int, _P, _P, _P, _P]),
     "atq_image_segmentation": (C.c_int, [_P, _P, _P, _P, _P, _S, _S, _S]),
+    "atq_image_segmentation_static": (C.c_int, [_P, _P, _P, _P, _S, _S, _S]),
     "atq_find_boundaries": (C.c_int, [_P, _P, _P, _P, _P, _S, _S]),
     "atq_glue_workspace_size": (_S, [_S, _S]),
     "atq_group_boundaries": (C.c_int, [_P, _P, _P, _S, _S, _P, _P, _P, _P, _S]),
