@@ -142,7 +142,12 @@ typedef struct atq_counts {
     uint32_t n_corners;       /* corners with error != 0                   */
     uint32_t n_segments;      /* peak segments E                           */
     uint32_t n_quads;         /* quads surviving process_fitted_quads      */
-    uint32_t overflow;        /* bit flags: capacity exceeded somewhere    */
+    uint32_t overflow;        /* bit flags; non-zero = the frame FAILED and
+                                 its results are void: 1 boundary buffer,
+                                 2 corner buffer, 4 a bounded wait (a
+                                 look-back scan or a grid barrier) gave up.
+                                 Callers must check it before using any
+                                 result of the frame                       */
     uint32_t oob_reads;       /* do_indexing reads past the moment array   */
     uint32_t key_label_bits;  /* lb of the detector's packed pair keys     */
     uint32_t pair_filter_off; /* 1: the detector kept every boundary record

@@ -303,3 +303,49 @@ class RefHeaders:
 
 def ref_headers_available():
     return os.path.exists(os.path.join(_HERE, "_ref", "libref_headers.so"))
+
+
+def line_fp64(m):
+    """A FittedQuad moment (LINE_FIT_POINT record) fitted in exact integer /
+    float64 arithmetic: the centroid (Mx / 2W, My / 2W) and the unit normal
+    = the eigenvector of the smallest eigenvalue of the weighted covariance
+    [[Mxx W - Mx^2, Mxy W - Mx My], [.., Myy W - My^2]] (numpy eigh), the
+    line fit_line approximates in float32 (src/fit_quad.cpp:37-105). The
+    covariance entries are formed with Python integers, so nothing wraps."""
+    W, Mx, My = int(m["W"]), int(m["Mx"]), int(m["My"])
+    cxx = int(m["Mxx"]) * W - Mx * Mx
+    cxy = int(m["Mxy"]) * W - Mx * My
+    cyy = int(m["Myy"]) * W - My * My
+    cov = np.array([[cxx, cxy], [cxy, cyy]], dtype=np.float64)
+    _, vec = np.linalg.eigh(cov)
+    n = vec[:, 0]
+    return np.array([Mx / (2.0 * W), My / (2.0 * W)]), n / np.hypot(n[0], n[1])
+
+
+def quad_corners_fp64(fitted_quad):
+    """float64 twin of process_fitted_quads' corners (src/main.cpp:1052-1096)
+    for one FITTED_QUAD record: corner i is the intersection of lines i and
+    i + 1, each line solved as n . p = n . c. Returns a 4 x 2 array, or None
+    when two consecutive lines are parallel to float64 precision."""
+    lines = [line_fp64(fitted_quad["moments"][i]) for i in range(4)]
+    out = np.empty((4, 2))
+    for i in range(4):
+        (c0, n0), (c1, n1) = lines[i], lines[(i + 1) % 4]
+        A = np.array([n0, n1])
+        if abs(np.linalg.det(A)) < 1e-12:
+            return None
+        out[i] = np.linalg.solve(A, np.array([n0 @ c0, n1 @ c1]))
+    return out
+
+
+def corner_error_fp64(fitted, quads):
+    """Largest |float32 corner - float64 twin| (pixels) over `quads`
+    (QUAD_CORNERS records, blob_index = their fitted quad) and the number of
+    quads compared."""
+    worst = 0.0
+    for q in quads:
+        ref = quad_corners_fp64(fitted[int(q["blob_index"])])
+        if ref is None:
+            continue
+        worst = max(worst, float(np.abs(q["corners"].astype(np.float64) - ref).max()))
+    return worst, len(quads)

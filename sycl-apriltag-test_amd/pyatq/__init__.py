@@ -464,7 +464,12 @@ class Detector:
         return self.read("COUNTS", R.COUNTS, 1)[0]
 
     def quads(self):
-        n = int(self.counts()["n_quads"])
-        return self.read("QUADS", R.QUAD_CORNERS, n)
+        """The last frame's quads; raises AtqError if the frame failed
+        (counts.overflow != 0: a buffer overflowed or a bounded wait of a
+        look-back scan or grid barrier gave up, and the results are void)."""
+        c = self.counts()
+        if int(c["overflow"]):
+            raise AtqError(f"frame failed: atq_counts.overflow = {int(c['overflow']):#x}")
+        return self.read("QUADS", R.QUAD_CORNERS, int(c["n_quads"]))
 
 

@@ -47,7 +47,10 @@ def run_detector(atq, img, f, **kw):
     c = det.counts()
     out = {"counts": c, "dims": det.dims()}
     rd = det.read
-    out["thresholded"] = rd("THRESHOLDED", np.uint8, det.dims()[0] * det.dims()[1])
+    npx = det.dims()[0] * det.dims()[1]
+    out["thresholded"] = rd("THRESHOLDED", np.uint8, npx)
+    out["labels"] = rd("LABELS", np.uint32, npx)
+    out["sizes"] = rd("SIZES", np.uint32, npx)
     nb = int(c["n_boundary"])
     out["points"] = rd("BOUNDARY_POINTS", np.uint32, nb)
     packed = rd("BOUNDARY_KEYS", np.uint64, nb) >> np.uint64(2)  # low 2 bits: plane
@@ -117,6 +120,18 @@ def expected_sort_passes(n_valid, slots, filter_off):
     return max(1, -(-bits // 11))
 
 
+def check_labels_and_sizes(g, o, min_size=25):
+    """Labels bit-exact; the detector's sizes are exact below min_size and
+    only "at least min_size" above it (its size adds stop there)."""
+    np.testing.assert_array_equal(g["labels"], o["labels"].reshape(-1), err_msg="labels")
+    roots = np.unique(np.concatenate([[0], o["labels"].reshape(-1) & 0x7FFFFFFF]))
+    want = o["sizes"].reshape(-1)[roots]
+    got = g["sizes"][roots]
+    small = want < min_size
+    np.testing.assert_array_equal(got[small], want[small], err_msg="sizes below min_size")
+    assert (got[~small] >= min_size).all() and (got[~small] <= want[~small]).all()
+
+
 def check_against_oracle(g, o):
     c = g["counts"]
     assert int(c["overflow"]) == 0
@@ -149,6 +164,7 @@ def check_against_oracle(g, o):
     assert int(c["n_segments"]) == len(o["peak_extents"])
     assert int(c["oob_reads"]) == int(o["oob_reads"])
     np.testing.assert_array_equal(g["thresholded"], o["thresholded"].reshape(-1))
+    check_labels_and_sizes(g, o)
     # grouped records: the reference's stable order, element by element
     np.testing.assert_array_equal(g["keys"], keys)
     np.testing.assert_array_equal(g["points"], points)
@@ -244,7 +260,12 @@ def test_detector_stress_12mp(atq, oracle):
     o = oracle.pipeline(img, 1)
     assert int(o["extents"]["count"].max()) > 16384
     g = run_detector(atq, img, 1)
-    check_against_oracle(g, o)
+    check_against_oracle(g, o)  # labels of the maze included
+    # the corners' error against a float64 twin of the quad stage
+    import pyoracle
+    err, n = pyoracle.corner_error_fp64(g["fitted"], g["quads"])
+    print(f"12 MP: {n} quads, max |float32 - float64| corner {err:.2e} px")
+    assert n > 100 and err < 2e-3
 
 
 def test_every_front_shape_gives_identical_results(atq, oracle):
