@@ -64,11 +64,11 @@ def stage_bytes(stage, f, npx, counts):
     return None
 
 
-def stage_traffic(stage):
-    """HBM bytes per frame of `stage` at the headline workload, from the two
-    counter runs (FETCH_SIZE, WRITE_SIZE, scaled by a device copy) summarised
-    in profiles/pipeline_1080p_f1_hbm_traffic.json; None if absent."""
-    path = os.path.join(ROOT, "profiles", "pipeline_1080p_f1_hbm_traffic.json")
+def stage_traffic(stage, workload="1080p_f1"):
+    """HBM bytes per frame of `stage`, from the two counter runs (FETCH_SIZE,
+    WRITE_SIZE, scaled by a device copy) summarised in
+    profiles/pipeline_<workload>_hbm_traffic.json; None if absent."""
+    path = os.path.join(ROOT, "profiles", f"pipeline_{workload}_hbm_traffic.json")
     try:
         with open(path) as f:
             st = json.load(f)["stages"].get(stage)
@@ -105,6 +105,32 @@ def time_frames(atq, det, devs, w, h, steps, warmup, stream):
     return wall, e0.elapsed_ms(e1) / 1e3
 
 
+def frame_times(atq, det, devs, w, h, stream):
+    """Milliseconds of each frame of devs run once, one HIP event pair per
+    frame on the launch stream (outside the timed region)."""
+    ev = [atq.Event() for _ in range(len(devs) + 1)]
+    ev[0].record(stream.handle)
+    for i, d in enumerate(devs):
+        det.detect(d, w, h, stream.handle)
+        ev[i + 1].record(stream.handle)
+    ev[-1].synchronize()
+    return [ev[i].elapsed_ms(ev[i + 1]) for i in range(len(devs))]
+
+
+def fits_and_quads(det, counts):
+    """The last frame's fitted quads and quads (for the float64 corner check
+    of the CPU leg)."""
+    from pyatq import records as R
+    return (det.read("FITTED", R.FITTED_QUAD, int(counts["n_segments"])),
+            det.read("QUADS", R.QUAD_CORNERS, int(counts["n_quads"])))
+
+
+def median(xs):
+    xs = sorted(xs)
+    n = len(xs)
+    return xs[n // 2] if n % 2 else 0.5 * (xs[n // 2 - 1] + xs[n // 2])
+
+
 def stage_profile(atq, det, devs, w, h, n, stream):
     det.set_timing(True)
     acc = None
@@ -139,7 +165,8 @@ def bench_12mp(atq, synth, stream, steps=10, tune=True):
     for st in ("threshold", "ccl", "boundary", "key_sort", "angle_fit", "fit_lines", "corners"):
         b = stage_bytes(st, 1, npx12, c)
         roof12[st] = {"ms": round(stages[st], 4), "GB/s": round(b / (stages[st] * 1e-3) / 1e9, 1),
-                      "frac": round(b / (stages[st] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                      "frac": round(b / (stages[st] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                      "algorithmic_bytes": int(b), "traffic": stage_traffic(st, "12mp")}
     out = {
         "autotuned_shapes": list(shapes),
         "ms_per_frame": round(wall / steps * 1e3, 3),
@@ -149,8 +176,9 @@ def bench_12mp(atq, synth, stream, steps=10, tune=True):
         "counts": {k: int(c[k]) for k in COUNT_KEYS},
     }
     log(f"12MP: {wall / steps * 1e3:.2f} ms/frame; stages {stages}")
+    fits = fits_and_quads(det, c)
     det.close()
-    return out
+    return out, fits
 
 
 def bench_batch(atq, synth, stream, rounds):
@@ -207,11 +235,13 @@ def bench_batch(atq, synth, stream, rounds):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--frames", type=int, default=100)
-    ap.add_argument("--cpu-frames", type=int, default=4)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-frames", type=int, default=20,
+                    help="frames of the CPU baseline on all allowed host cores")
+    ap.add_argument("--cpu-frames-1t", type=int, default=4,
+                    help="frames of the CPU baseline on one thread")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-12mp", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
@@ -248,7 +278,7 @@ def main():
         dst.free()
     if args.only_12mp:
         print(json.dumps({"stress_12mp": bench_12mp(atq, synth, atq.Stream(), args.steps,
-                                                     not args.no_tune)}))
+                                                     not args.no_tune)[0]}))
         return
     t = time.time()
     frames = [synth.plain_1080p(2000 + i).image for i in range(args.frames)]
@@ -261,11 +291,13 @@ def main():
         det = atq.Detector(W1080, H1080, decimate=f)
         shapes = tuned_shapes(det, devs[0], W1080, H1080, stream, not args.no_tune)
         wall, ev = time_frames(atq, det, devs, W1080, H1080, args.steps, args.warmup, stream)
+        per_frame = frame_times(atq, det, devs, W1080, H1080, stream)
         stages = stage_profile(atq, det, devs, W1080, H1080, 20, stream)
         det.detect(devs[0], W1080, H1080, stream.handle)
         stream.sync()
         counts = det.counts()
-        res[f] = dict(wall=wall, ev=ev, stages=stages, counts=counts, shapes=shapes)
+        res[f] = dict(wall=wall, ev=ev, stages=stages, counts=counts, shapes=shapes,
+                      per_frame=per_frame, fits=fits_and_quads(det, counts))
         log(f"decimate {f}: {args.steps} frames in {wall * 1e3:.1f} ms wall "
             f"({ev * 1e3:.1f} ms events), quads/frame {int(counts['n_quads'])}")
         log("  stages ms: " + ", ".join(f"{k} {v:.4f}" for k, v in stages.items()))
@@ -298,8 +330,14 @@ def main():
         stage_roof[st] = {"ms": round(st1[st], 5), "GB/s": round(b / (st1[st] * 1e-3) / 1e9, 1),
                           "frac": round(b / (st1[st] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
+    pf = res[1]["per_frame"]
     extra = {
         "frames_per_s": round(fps, 2),
+        "frame_ms_median": round(median(pf), 4),
+        "frames_per_s_median": round(1e3 / median(pf), 1),
+        "frame_ms_p90": round(sorted(pf)[int(0.9 * (len(pf) - 1))], 4),
+        "per_frame_timing": f"{len(pf)} frames (seeds 2000-{2000 + len(pf) - 1}) run once each "
+                            "after the timed region, one HIP event pair per frame",
         "event_ms_per_frame": round(res[1]["ev"] / steps * 1e3, 4),
         "autotuned_shapes_1080p_f1": list(res[1]["shapes"]),
         "stage_ms_1080p_f1": {k: round(v, 5) for k, v in st1.items()},
@@ -330,8 +368,19 @@ def main():
             "n_clusters": int(c["n_clusters"]), "n_quads": int(c["n_quads"])}
         log(f"fixed mode: {extra['fixed_mode_1080p_f1']['ms_per_frame']} ms/frame")
 
+    fits = {"1920x1080 seed 2000 f=1": res[1]["fits"]}
+    if 2 in res:
+        fits["1920x1080 seed 2000 f=2"] = res[2]["fits"]
+    small = synth.plain_small()
+    hs, ws = small.image.shape
+    det = atq.Detector(ws, hs, decimate=small.decimate)
+    det.detect(atq.dev_from_np(small.image), ws, hs, stream.handle)
+    stream.sync()
+    fits["640x480 seed 1000 f=2"] = fits_and_quads(det, det.counts())
+    det.close()
     if not args.skip_12mp:
-        extra["stress_12mp"] = bench_12mp(atq, synth, stream, tune=not args.no_tune)
+        extra["stress_12mp"], fits["4096x3072 stress f=1"] = bench_12mp(atq, synth, stream,
+                                                                       tune=not args.no_tune)
 
     if not args.skip_batch:
         extra["batched_16_mixed"] = bench_batch(atq, synth, stream, args.batch_rounds)
@@ -341,27 +390,53 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle
         orc = pyoracle.Oracle()
-        nthreads = min(args.cpu_threads, orc.max_threads())
-        sample = frames[:args.cpu_frames]
-        orc.detect(sample[0], 1, nthreads)  # warm-up
-        t = time.perf_counter()
-        for fr in sample:
-            orc.detect(fr, 1, nthreads)
-        dt = time.perf_counter() - t
-        cpu = {"value": round(len(sample) * npx / dt / 1e6, 3), "unit": "Mpix/s",
+        # the cores this process may run on, within the OpenMP thread limit
+        # the host sets (the GPU box's share of its CPUs)
+        nproc = len(os.sched_getaffinity(0))
+        nthreads = min(nproc, orc.max_threads())
+
+        def per_frame_s(sample, threads):
+            orc.detect(sample[0], 1, threads)  # warm-up
+            ts = []
+            for fr in sample:
+                t = time.perf_counter()
+                orc.detect(fr, 1, threads)
+                ts.append(time.perf_counter() - t)
+            return ts
+
+        n_all = min(args.cpu_frames, len(frames))
+        t_all = per_frame_s(frames[:n_all], nthreads)
+        t_one = per_frame_s(frames[:min(args.cpu_frames_1t, len(frames))], 1)
+        cpu = {"value": round(npx / median(t_all) / 1e6, 3), "unit": "Mpix/s",
                "cores": nthreads, "kind": "port",
-               "sample": f"{len(sample)} of the same 1080p frames (seeds 2000..), decimate 1, "
-                         f"C++ restatement of the reference with OpenMP, {dt:.2f}s"}
+               "sample": f"{n_all} of the same 1080p frames (seeds 2000-{1999 + n_all}), decimate 1, "
+                         f"median per frame; C++ restatement of the reference, OpenMP over "
+                         f"{nthreads} threads (nproc {nproc}), {sum(t_all):.2f}s",
+               "nproc": nproc,
+               "frame_ms_median": round(median(t_all) * 1e3, 2),
+               "one_thread": {"value": round(npx / median(t_one) / 1e6, 3), "unit": "Mpix/s",
+                              "frame_ms_median": round(median(t_one) * 1e3, 2),
+                              "sample": f"{len(t_one)} frames (seeds 2000-{1999 + len(t_one)}), "
+                                        f"{sum(t_one):.2f}s"}}
         log(f"cpu oracle: {cpu}")
+        # the float32 quad corners (bit-exact with the reference
+        # restatement) against a float64 twin of the quad stage
+        err = {}
+        for name, (fitted, quads) in fits.items():
+            e, n = pyoracle.corner_error_fp64(fitted, quads)
+            err[name] = {"quads": n, "max_abs_px": float(f"{e:.3e}")}
+        extra["corner_error_vs_float64"] = err
+        log(f"corner error vs float64: {err}")
 
     line = {
         "metric": METRIC, "value": round(mpix, 2), "unit": "Mpix/s", "n_gpus": 1,
         "steps": steps, "warmup": args.warmup, "ms_per_step": round(wall1 / steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
         "data": "synthetic",
-        "config": {"workload": "plain per-frame threshold->quads, 1920x1080 u8, 30 tags + 200 "
-                               "shapes + 100 lines, blur 1.0 noise 6, 100 seeded frames "
-                               "(2000-2099), decimate 1",
+        "config": {"workload": f"plain per-frame threshold->quads, 1920x1080 u8, 30 tags + 200 "
+                               f"shapes + 100 lines, blur 1.0 noise 6, decimate 1; {steps} timed "
+                               f"frames cycling through seeds 2000-{2000 + min(steps, len(devs)) - 1}"
+                               f" (throughput over the K timed frames)",
                    "global_batch": 1, "frames": args.frames, "parallelism": "single-gpu"},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -202,6 +202,35 @@ constexpr uint32_t ROOT_BITS = 11;         // local node index in a state word
 // pixels' colour codes.
 ATQ_DEV uint32_t *ccl_state(const FrontFrame &F) { return F.parent + (size_t)F.w * F.h / 2; }
 
+// Border entries of the tiles' edge blocks, held in the labels buffer until
+// the last kernel writes the labels (the parent array keeps only live
+// local roots): for every tile row, the entries of its top (side 0) and
+// bottom (side 1) block rows across the whole frame; for every tile column,
+// those of its left (0) and right (1) block columns down the whole frame.
+// 2 * tile_rows * w + 2 * tile_cols * h words, fewer than w * h for any
+// frame of at least 8 x 8 pixels. An entry is two words, the local roots of
+// the block's white and black nodes (NODE_BITS bits) with the 2-bit codes
+// of the block's six probes that leave the tile in their top bits (probes
+// 0-2 in the first word, 3-5 in the second).
+constexpr uint32_t NODE_BITS = 25;  // node indices < w * h / 2 <= 2^25
+constexpr uint32_t NODE_MASK = (1u << NODE_BITS) - 1;
+
+struct BorderMaps {
+    uint32_t *base;
+    uint32_t w, h, tile_rows;
+    ATQ_DEV uint2 *row(uint32_t tr, uint32_t side, uint32_t bxg) const {
+        return reinterpret_cast<uint2 *>(base) + (size_t)(2 * tr + side) * (w / 2) + bxg;
+    }
+    ATQ_DEV uint2 *col(uint32_t tc, uint32_t side, uint32_t byg) const {
+        return reinterpret_cast<uint2 *>(base) + (size_t)tile_rows * w + (size_t)(2 * tc + side) * (h / 2) + byg;
+    }
+};
+
+template <int TBY>
+ATQ_DEV BorderMaps border_maps(const FrontFrame &F) {
+    return BorderMaps{F.labels, F.w, F.h, (F.h + 2 * TBY - 1) / (2 * TBY)};
+}
+
 template <int TBY>
 struct LocalLds {
     static constexpr int NODES = 2 * TBX * TBY;
@@ -314,7 +343,7 @@ ATQ_DEV void ccl_local_tile(LocalLds<TBY> &L, const FrontFrame &F, uint32_t tile
     }
     // edges leaving the tile: P (left or upper tiles), Q (tile above), R
     // (right or upper tiles), S (left tile)
-    uint32_t out = 0;
+    uint32_t out = 0, xcode = 0;
 #pragma unroll
     for (int k = 0; k < 6; k++) {
         const bool cross = k == 0   ? (by == 0 || bx == 0)
@@ -322,7 +351,27 @@ ATQ_DEV void ccl_local_tile(LocalLds<TBY> &L, const FrontFrame &F, uint32_t tile
                            : k == 3 ? (by == 0 || bx == TBX - 1)
                                     : bx == 0;
         const uint32_t c = (code >> (2 * k)) & 3u;
-        if (cross && c) out |= 1u << (c - 1);
+        if (cross && c) {
+            out |= 1u << (c - 1);
+            xcode |= c << (2 * k);
+        }
+    }
+    // what the last steps need of the pixels, in one word: the pixels'
+    // colour codes (8 bits), white and black counts (3 bits each), the
+    // outgoing-edge colours (2 bits) and the codes of the probes that
+    // leave the tile (12 bits)
+    uint32_t info;
+    {
+        uint32_t codes = 0, nw = 0, nb = 0;
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const uint32_t t = (yt >> (8 * j)) & 0xFFu, u = (yb >> (8 * j)) & 0xFFu;
+            codes |= colour_code(t) << (2 * j);
+            codes |= colour_code(u) << (2 * j + 4);
+            nw += (t == C_WHITE) + (u == C_WHITE);
+            nb += (t == C_BLACK) + (u == C_BLACK);
+        }
+        info = codes | (nw << 8) | (nb << 11) | (out << 14) | (xcode << 16);
     }
     const uint64_t half = lane < 32 ? 0x00000000FFFFFFFFull : 0xFFFFFFFF00000000ull;
     const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
@@ -373,7 +422,7 @@ ATQ_DEV void ccl_local_tile(LocalLds<TBY> &L, const FrontFrame &F, uint32_t tile
     bool jump0 = false, jump1 = false;  // this thread's nodes are heads in a pair
     uint32_t step = 0;
     for (bool first = true;; first = false) {
-        uint32_t ra[4], rb[4], lo[4], hi[4], old[4];
+        uint32_t ra[4], rb[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             if ((vm >> i) & 1u) {
@@ -381,31 +430,23 @@ ATQ_DEV void ccl_local_tile(LocalLds<TBY> &L, const FrontFrame &F, uint32_t tile
                 rb[i] = L.lp[B[i]];
             }
         }
-        uint32_t att = 0;
+        // a hook whose root was hooked meanwhile (atomicMin finds another
+        // word than the root) leaves the pair (lower root, that word) to
+        // join next round; every other hook is final
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             if (!((vm >> i) & 1u)) continue;
             if (ra[i] == rb[i]) {
                 vm &= ~(1u << i);  // joined
-            } else {
-                att |= 1u << i;
-                lo[i] = min(ra[i], rb[i]);
-                hi[i] = max(ra[i], rb[i]);
+                continue;
             }
-        }
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-            if ((att >> i) & 1u) old[i] = atomicMin(&L.lp[hi[i]], lo[i]);
-        // a hook whose root was hooked meanwhile (old != hi) leaves the
-        // pair (lo, old) to join next round; every other hook is final
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            if (!((att >> i) & 1u)) continue;
-            if (old[i] == hi[i]) {
+            const uint32_t lo = min(ra[i], rb[i]), hi = max(ra[i], rb[i]);
+            const uint32_t was = atomicMin(&L.lp[hi], lo);
+            if (was == hi) {
                 vm &= ~(1u << i);
             } else {
-                A[i] = lo[i];
-                B[i] = old[i];
+                A[i] = lo;
+                B[i] = was;
             }
         }
         const bool again = barrier_or(L.flag, step, vm != 0);
@@ -434,34 +475,31 @@ ATQ_DEV void ccl_local_tile(LocalLds<TBY> &L, const FrontFrame &F, uint32_t tile
     // pixel counts and outgoing edges per local root (the roots of this
     // block's nodes are r0, r1)
     {
-        uint32_t nw = 0, nb = 0;
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const uint32_t t = (yt >> (8 * j)) & 0xFFu, u = (yb >> (8 * j)) & 0xFFu;
-            nw += (t == C_WHITE) + (u == C_WHITE);
-            nb += (t == C_BLACK) + (u == C_BLACK);
-        }
+        const uint32_t nw = (info >> 8) & 7u, nb = (info >> 11) & 7u;
         if (nw) atomicAdd(&L.cnt[r0], nw);
         if (nb) atomicAdd(&L.cnt[r1], nb);
-        if (out & 1u) atomicOr(&L.cnt[r0], CNT_OUT);
-        if (out & 2u) atomicOr(&L.cnt[r1], CNT_OUT);
+        if (info & (1u << 14)) atomicOr(&L.cnt[r0], CNT_OUT);
+        if (info & (2u << 14)) atomicOr(&L.cnt[r1], CNT_OUT);
     }
     __syncthreads();
 
     // 5. publish
     if (in_img) {
-        uint32_t codes = 0;
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-            codes |= colour_code((yt >> (8 * j)) & 0xFFu) << (2 * j);
-            codes |= colour_code((yb >> (8 * j)) & 0xFFu) << (2 * j + 4);
-        }
+        const uint32_t xcode = info >> 16;
         ccl_state(F)[(size_t)(y / 2) * (w / 2) + (uint32_t)(x / 2)] =
-            r0 | (r1 << ROOT_BITS) | (codes << (2 * ROOT_BITS));
+            r0 | (r1 << ROOT_BITS) | ((info & 0xFFu) << (2 * ROOT_BITS));
         if (by == 0 || by == TBY - 1 || bx == 0 || bx == TBX - 1) {
-            const uint32_t n = (uint32_t)(y / 2) * w + (uint32_t)x;
-            parent[n] = tile_node(r0, x0, y0, w);
-            parent[n + 1] = tile_node(r1, x0, y0, w);
+            // the block's border entry: both nodes' local roots, the codes
+            // of its probes that leave the tile in their top bits
+            const uint2 e = make_uint2(tile_node(r0, x0, y0, w) | ((xcode & 0x3Fu) << NODE_BITS),
+                                       tile_node(r1, x0, y0, w) | ((xcode >> 6) << NODE_BITS));
+            const BorderMaps M = border_maps<TBY>(F);
+            const uint32_t tr = (uint32_t)y0 / (2 * TBY), tc = (uint32_t)x0 / (2 * TBX);
+            const uint32_t bxg = (uint32_t)x / 2, byg = (uint32_t)y / 2;
+            if (by == 0) *M.row(tr, 0, bxg) = e;
+            if (by == TBY - 1) *M.row(tr, 1, bxg) = e;
+            if (bx == 0) *M.col(tc, 0, byg) = e;
+            if (bx == TBX - 1) *M.col(tc, 1, byg) = e;
         }
     }
 #pragma unroll
@@ -496,13 +534,15 @@ k_ccl_local(FrontTable tab) {
 }
 
 // One wave per tile; lane k < 62 takes border block k (top row, then the
-// left and right columns) and derives the edges that leave the tile. Each
-// edge is a pair of local roots (the parent word of a border block's node
-// names its local root); the wave keeps the distinct pairs in an LDS hash
-// set and only the lane that inserts a pair merges it on the global parent
-// array (lock-free atomicMin union, include/union_find.h:50-77). A tile
-// border is crossed by few components, so a tile issues a handful of
-// global unions.
+// left and right columns) and the edges that leave the tile from it, read
+// from the border entries the local pass wrote: its own entry (local roots
+// and the codes of its probes that leave the tile) and the entries of the
+// blocks those probes reach (their local roots), one round of independent
+// loads. Each edge is a pair of local roots; the wave keeps the distinct
+// pairs in an LDS hash set and only the lane that inserts a pair merges it
+// on the global parent array (lock-free atomicMin union,
+// include/union_find.h:50-77). A tile border is crossed by few components,
+// so a tile issues a handful of global unions.
 template <int TBY>
 constexpr int border_items() { return TBX + 2 * (TBY - 1); }
 constexpr int BORDER_SLOTS = 512;
@@ -515,66 +555,51 @@ ATQ_DEV void ccl_border_tile(uint64_t *hs, const FrontFrame &F, uint32_t tile, u
     static_assert(ITEMS <= 64, "one border block per lane");
     constexpr uint32_t NONE = 0xFFFFFFFFu;
     constexpr uint64_t EMPTY = ~0ull;
-    const uint8_t *__restrict__ thr = F.thr;
     const uint32_t w = F.w, h = F.h;
     uint32_t *__restrict__ parent = F.parent;
+    const BorderMaps M = border_maps<TBY>(F);
     const uint32_t k = threadIdx.x & 63;
     bool live = tile < n_tiles && k < (uint32_t)ITEMS;
     int bx = 0, by = 0, x = 0, y = 0;
+    const uint32_t tr = tile / tiles_x, tc = tile % tiles_x;
     if (live) {
         if (k < (uint32_t)TBX) { bx = k; by = 0; }
         else if (k < (uint32_t)(TBX + TBY - 1)) { bx = 0; by = 1 + (k - TBX); }
         else { bx = TBX - 1; by = 1 + (k - TBX - (TBY - 1)); }
-        const int x0 = (tile % tiles_x) * 2 * TBX, y0 = (tile / tiles_x) * 2 * TBY;
-        x = x0 + 2 * bx;
-        y = y0 + 2 * by;
+        x = (int)tc * 2 * TBX + 2 * bx;
+        y = (int)tr * 2 * TBY + 2 * by;
         live = x < (int)w && y < (int)h;
     }
-    // One round of independent loads: the block's three mask pixels, its two
-    // parent words, and for every probe that leaves the tile its pixel and
-    // the two parent words of the block holding it (the words name local
-    // roots, which concurrent unions may lower; any value read, a cached one
-    // too, is an ancestor in the same tree, which is all a union needs).
-    uint8_t tl = 127, tr = 127, bl = 127;
-    uint32_t own[2] = {0, 0}, nb[6][2];
-    uint8_t pv[6];
+    const uint32_t bxg = (uint32_t)x / 2, byg = (uint32_t)y / 2;
+    uint2 own = make_uint2(0u, 0u), nb[6];
     bool cross[6];
-    if (live) {
-        const uint8_t *row = thr + (size_t)y * w;
-        tl = row[x];
-        tr = row[x + 1];
-        bl = row[x + w];
-        const uint32_t id = (uint32_t)(y / 2) * w + (uint32_t)x;
-        own[0] = parent[id];
-        own[1] = parent[id + 1];
-    }
+    if (live) own = by == 0 ? *M.row(tr, 0, bxg) : *M.col(tc, bx == 0 ? 0 : 1, byg);
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         const Probe q = PROBES[i];
         const int nbx = bx + q.dbx, nby = by + q.dby;
-        const int py = y + q.dy, px = x + q.dx, ny = y + 2 * q.dby, nx = x + 2 * q.dbx;
-        cross[i] = live && !(nbx >= 0 && nbx < TBX && nby >= 0) && py >= 0 && py < (int)h &&
-                   px >= 0 && px < (int)w && ny >= 0 && nx >= 0 && nx < (int)w;
-        pv[i] = 127;
-        nb[i][0] = nb[i][1] = 0;
+        const int ny = y + 2 * q.dby, nx = x + 2 * q.dbx;
+        cross[i] = live && !(nbx >= 0 && nbx < TBX && nby >= 0) && ny >= 0 && nx >= 0 && nx < (int)w;
+        nb[i] = make_uint2(0u, 0u);
         if (cross[i]) {
-            pv[i] = thr[(size_t)py * w + px];
-            const uint32_t nid = (uint32_t)(ny / 2) * w + (uint32_t)nx;
-            nb[i][0] = parent[nid];
-            nb[i][1] = parent[nid + 1];
+            // the row above: the bottom row of the tile row above; else the
+            // facing column of the left or right tile
+            if (by == 0 && q.dby < 0) nb[i] = *M.row(tr - 1, 1, (uint32_t)(nx / 2));
+            else nb[i] = *M.col(q.dbx < 0 ? tc - 1 : tc + 1, q.dbx < 0 ? 1 : 0, (uint32_t)(ny / 2));
         }
     }
-    const uint32_t m = live ? check_mask(tl, tr, bl, x, y, w, h) : 0u;
+    const uint32_t codes = (own.x >> NODE_BITS) | ((own.y >> NODE_BITS) << 6);
     uint32_t ea[6], eb[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         ea[i] = NONE;
         eb[i] = NONE;
-        const uint8_t v = pv[i];
-        if (cross[i] && (m & (1u << PROBES[i].bit)) && (v == C_WHITE || v == C_BLACK)) {
-            const uint32_t c = v == C_WHITE ? 0 : 1;
-            ea[i] = min(own[c], nb[i][c]);
-            eb[i] = max(own[c], nb[i][c]);
+        const uint32_t cd = (codes >> (2 * i)) & 3u;  // 1 white, 2 black
+        if (cross[i] && cd) {
+            const uint32_t a = (cd == 1 ? own.x : own.y) & NODE_MASK;
+            const uint32_t b = (cd == 1 ? nb[i].x : nb[i].y) & NODE_MASK;
+            ea[i] = min(a, b);
+            eb[i] = max(a, b);
         }
     }
 #pragma unroll
@@ -594,6 +619,7 @@ ATQ_DEV void ccl_border_tile(uint64_t *hs, const FrontFrame &F, uint32_t tile, u
         }
         if (mine) g_union(parent, ea[i], eb[i]);
     }
+    (void)h;
 }
 
 // Four tiles per work-group round, one per wave.

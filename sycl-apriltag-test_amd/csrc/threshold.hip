@@ -102,6 +102,18 @@ ATQ_DEV void load_strip_row(const FrontFrame &F, bool vec_in, int x, int y, uint
     }
 }
 
+// 0xFF in every byte of v greater than c (0 <= c <= 254, replicated in
+// every byte of cc), 0 elsewhere, four bytes at a time: (v | H) - (c + 1)
+// without H has the high bit of a byte set where its low 7 bits reach
+// c + 1's; where the bytes' high bits differ v's decides instead.
+ATQ_DEV uint32_t bytes_greater(uint32_t v, uint32_t cc) {
+    constexpr uint32_t H = 0x80808080u;
+    const uint32_t c1 = cc + 0x01010101u;
+    const uint32_t t = (v | H) - (c1 & ~H);
+    const uint32_t ge = ((t & ~(v ^ c1)) | (v & (v ^ c1))) & H;
+    return (ge >> 7) * 0xFFu;
+}
+
 // Min and max of the bytes of four words (one 4x4 tile).
 ATQ_DEV void tile_minmax(uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t &mn,
                          uint32_t &mx) {
@@ -118,38 +130,48 @@ ATQ_DEV void tile_minmax(uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_
         }
 }
 
-// Tile extrema of a region and its one-tile halo ring (out-of-image tiles
-// are neutral: min 255, max 0).
+// Tile extrema of a region and its one-tile halo ring, min | max << 8
+// (out-of-image tiles are neutral: min 255, max 0). Rows are padded to an
+// even length so that a thread's 6 columns are 3 aligned words.
 template <int RX, int RY>
-struct ThresholdLds {
-    uint8_t mn[RY + 2][RX + 2], mx[RY + 2][RX + 2];
+struct alignas(16) ThresholdLds {
+    uint16_t e[RY + 2][RX + 2];
 };
 
 // One region of RX x RY tiles of frame F. Each of the 256 threads owns a
 // strip of 4 tiles (4 rows x 16 pixels) and keeps its pixels in registers:
-// it loads them, writes the strip's tile extrema to LDS (threads < the ring
-// size also load one halo tile), and after one barrier dilates its tiles
+// it loads them (threads < the ring size also load one halo tile), writes
+// the strip's tile extrema to LDS, and after one barrier dilates its tiles
 // over their 3x3 neighbourhoods and thresholds and stores its 64 pixels.
 template <int RX, int RY>
-ATQ_DEV void threshold_region(ThresholdLds<RX, RY> &L, const FrontFrame &F, uint32_t region,
-                              uint32_t regions_x, bool vec_in, bool vec_out) {
+struct RegionRegs {
+    uint32_t px[4][4];  // the strip: 4 rows x 4 words
+    uint32_t hp[4];     // the halo tile
+    int tx0, ty0;       // region origin (tiles)
+    bool live;
+};
+
+template <int RX, int RY>
+ATQ_DEV void region_load(RegionRegs<RX, RY> &R, const FrontFrame &F, uint32_t region,
+                         uint32_t regions_x, bool vec_in, bool live) {
     constexpr int SX = RX / 4;
     static_assert(RX % 4 == 0 && SX * RY == 256, "one 4-tile strip per thread");
     constexpr int RING = 2 * (RX + 2) + 2 * RY;
     static_assert(RING <= 256, "one halo tile per thread");
-    const uint32_t w = F.w, h = F.h;
-    const int tw = (int)(w / 4), th = (int)(h / 4);
+    const int tw = (int)(F.w / 4), th = (int)(F.h / 4);
     const int tid = threadIdx.x;
-    const int tx0 = (int)(region % regions_x) * RX, ty0 = (int)(region / regions_x) * RY;
+    R.live = live;
+    R.tx0 = (int)(region % regions_x) * RX;
+    R.ty0 = (int)(region / regions_x) * RY;
     const int sx = tid % SX, sy = tid / SX;
-    const int ty = ty0 + sy, tx = tx0 + 4 * sx;  // the strip's first tile
-    const bool row_in = ty < th;
-
-    // every load of the thread before any use
-    uint32_t px[4][4] = {};
-    if (row_in) {
+    const int ty = R.ty0 + sy, tx = R.tx0 + 4 * sx;  // the strip's first tile
 #pragma unroll
-        for (int r = 0; r < 4; r++) load_strip_row(F, vec_in, 4 * tx, 4 * ty + r, px[r]);
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) R.px[r][q] = 0;
+    if (live && ty < th) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) load_strip_row(F, vec_in, 4 * tx, 4 * ty + r, R.px[r]);
     }
     int hx = 0, hy = 0;  // halo tile (region-relative, -1 .. RX / RY)
     if (tid < RING) {
@@ -158,9 +180,10 @@ ATQ_DEV void threshold_region(ThresholdLds<RX, RY> &L, const FrontFrame &F, uint
         else if (tid < 2 * (RX + 2) + RY) { hx = -1; hy = tid - 2 * (RX + 2); }
         else { hx = RX; hy = tid - 2 * (RX + 2) - RY; }
     }
-    const int htx = tx0 + hx, hty = ty0 + hy;
-    const bool halo_in = tid < RING && htx >= 0 && htx < tw && hty >= 0 && hty < th;
-    uint32_t hp[4] = {0, 0, 0, 0};
+    const int htx = R.tx0 + hx, hty = R.ty0 + hy;
+    const bool halo_in = live && tid < RING && htx >= 0 && htx < tw && hty >= 0 && hty < th;
+#pragma unroll
+    for (int r = 0; r < 4; r++) R.hp[r] = 0;
     if (halo_in) {
         const uint32_t f = F.f;
 #pragma unroll
@@ -169,25 +192,36 @@ ATQ_DEV void threshold_region(ThresholdLds<RX, RY> &L, const FrontFrame &F, uint
             if (F.channels != 1) {
 #pragma unroll
                 for (int j = 0; j < 4; j++)
-                    hp[r] |= grey_at(F, (uint32_t)(4 * htx + j), (uint32_t)(4 * hty + r)) << (8 * j);
+                    R.hp[r] |= grey_at(F, (uint32_t)(4 * htx + j), (uint32_t)(4 * hty + r)) << (8 * j);
             } else if (f == 1) {
-                hp[r] = *reinterpret_cast<const uint32_t *>(row + 4 * htx);
+                R.hp[r] = *reinterpret_cast<const uint32_t *>(row + 4 * htx);
             } else {
 #pragma unroll
-                for (int j = 0; j < 4; j++) hp[r] |= (uint32_t)row[(size_t)(4 * htx + j) * f] << (8 * j);
+                for (int j = 0; j < 4; j++) R.hp[r] |= (uint32_t)row[(size_t)(4 * htx + j) * f] << (8 * j);
             }
         }
     }
+}
 
-    // tile extrema of the strip and of the halo tile
+// Tile extrema of the strip and of the halo tile to LDS (out-of-image
+// tiles are neutral: min 255, max 0); the raw extrema to F.ext if asked.
+template <int RX, int RY>
+ATQ_DEV void region_extrema(ThresholdLds<RX, RY> &L, const RegionRegs<RX, RY> &R,
+                            const FrontFrame &F) {
+    constexpr int SX = RX / 4;
+    constexpr int RING = 2 * (RX + 2) + 2 * RY;
+    const int tw = (int)(F.w / 4), th = (int)(F.h / 4);
+    const int tid = threadIdx.x;
+    const int sx = tid % SX, sy = tid / SX;
+    const int ty = R.ty0 + sy, tx = R.tx0 + 4 * sx;
+    const bool row_in = R.live && ty < th;
     uint32_t smn[4], smx[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         smn[q] = 255;
         smx[q] = 0;
-        if (row_in && tx + q < tw) tile_minmax(px[0][q], px[1][q], px[2][q], px[3][q], smn[q], smx[q]);
-        L.mn[sy + 1][4 * sx + q + 1] = (uint8_t)smn[q];
-        L.mx[sy + 1][4 * sx + q + 1] = (uint8_t)smx[q];
+        if (row_in && tx + q < tw) tile_minmax(R.px[0][q], R.px[1][q], R.px[2][q], R.px[3][q], smn[q], smx[q]);
+        L.e[sy + 1][4 * sx + q + 1] = (uint16_t)(smn[q] | (smx[q] << 8));
     }
     if (F.ext && row_in) {
 #pragma unroll
@@ -196,44 +230,63 @@ ATQ_DEV void threshold_region(ThresholdLds<RX, RY> &L, const FrontFrame &F, uint
                 F.ext[(size_t)ty * tw + tx + q] = atq_extrema{(uint8_t)smn[q], (uint8_t)smx[q]};
     }
     if (tid < RING) {
+        int hx, hy;
+        if (tid < RX + 2) { hx = tid - 1; hy = -1; }
+        else if (tid < 2 * (RX + 2)) { hx = tid - (RX + 2) - 1; hy = RY; }
+        else if (tid < 2 * (RX + 2) + RY) { hx = -1; hy = tid - 2 * (RX + 2); }
+        else { hx = RX; hy = tid - 2 * (RX + 2) - RY; }
+        const int htx = R.tx0 + hx, hty = R.ty0 + hy;
         uint32_t mn = 255, mx = 0;
-        if (halo_in) tile_minmax(hp[0], hp[1], hp[2], hp[3], mn, mx);
-        L.mn[hy + 1][hx + 1] = (uint8_t)mn;
-        L.mx[hy + 1][hx + 1] = (uint8_t)mx;
+        if (R.live && htx >= 0 && htx < tw && hty >= 0 && hty < th)
+            tile_minmax(R.hp[0], R.hp[1], R.hp[2], R.hp[3], mn, mx);
+        L.e[hy + 1][hx + 1] = (uint16_t)(mn | (mx << 8));
     }
-    __syncthreads();
+}
 
-    // 3x3 dilation of the strip's tiles, then the threshold of its pixels:
-    // 127 if max-min < min_diff, else 255 if v > min+(max-min)/2, else 0
+// After the barrier: 3x3 dilation of the strip's tiles, then the threshold
+// of its pixels (127 if max-min < min_diff, else 255 if
+// v > min+(max-min)/2, else 0), stored with the decimated grey when f > 1.
+template <int RX, int RY>
+ATQ_DEV void region_finish(const ThresholdLds<RX, RY> &L, const RegionRegs<RX, RY> &R,
+                           const FrontFrame &F, bool vec_out) {
+    constexpr int SX = RX / 4;
+    const uint32_t w = F.w;
+    const int tw = (int)(F.w / 4), th = (int)(F.h / 4);
+    const int tid = threadIdx.x;
+    const int sx = tid % SX, sy = tid / SX;
+    const int ty = R.ty0 + sy, tx = R.tx0 + 4 * sx;
+    const bool row_in = R.live && ty < th;
+    // the 3 x 6 neighbourhood of the strip's tiles (LDS columns 4 sx ..
+    // 4 sx + 5) as 3 aligned words per row
+    uint32_t nb[3][6];
+#pragma unroll
+    for (int dr = 0; dr < 3; dr++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const uint32_t v = *reinterpret_cast<const uint32_t *>(&L.e[sy + dr][4 * sx + 2 * k]);
+            nb[dr][2 * k] = v & 0xFFFFu;
+            nb[dr][2 * k + 1] = v >> 16;
+        }
     uint32_t out[4][4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        const int c = 4 * sx + q;  // interior tile column; LDS column c + 1
         uint32_t mn = 255, mx = 0;
 #pragma unroll
         for (int dr = 0; dr < 3; dr++)
 #pragma unroll
             for (int dc = 0; dc < 3; dc++) {
-                mn = min(mn, (uint32_t)L.mn[sy + dr][c + dc]);
-                mx = max(mx, (uint32_t)L.mx[sy + dr][c + dc]);
+                mn = min(mn, nb[dr][q + dc] & 0xFFu);
+                mx = max(mx, nb[dr][q + dc] >> 8);
             }
         if (F.fil && row_in && tx + q < tw)
             F.fil[(size_t)ty * tw + tx + q] = atq_extrema{(uint8_t)mn, (uint8_t)mx};
         const bool flat = (int)(mx - mn) < F.min_diff;
         const uint32_t comp = (mn + (mx - mn) / 2) & 0xFFu;
+        // no byte exceeds 255 (comp == 255 only when min == max == 255)
+        const uint32_t cc = comp * 0x01010101u;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            uint32_t o = 0x7F7F7F7Fu;
-            if (!flat) {
-                o = 0;
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const uint32_t v = (px[r][q] >> (8 * j)) & 0xFFu;
-                    o |= (v > comp ? 0xFFu : 0u) << (8 * j);
-                }
-            }
-            out[r][q] = o;
-        }
+        for (int r = 0; r < 4; r++)
+            out[r][q] = flat ? 0x7F7F7F7Fu : (comp < 255 ? bytes_greater(R.px[r][q], cc) : 0u);
     }
     if (row_in && tx < tw) {
         uint8_t *gray_out = F.f > 1 || F.channels != 1 ? F.gray : nullptr;
@@ -243,22 +296,24 @@ ATQ_DEV void threshold_region(ThresholdLds<RX, RY> &L, const FrontFrame &F, uint
             if (vec_out && tx + 4 <= tw) {
                 *reinterpret_cast<uint4 *>(F.thr + o) = make_uint4(out[r][0], out[r][1], out[r][2], out[r][3]);
                 if (gray_out)
-                    *reinterpret_cast<uint4 *>(gray_out + o) = make_uint4(px[r][0], px[r][1], px[r][2], px[r][3]);
+                    *reinterpret_cast<uint4 *>(gray_out + o) =
+                        make_uint4(R.px[r][0], R.px[r][1], R.px[r][2], R.px[r][3]);
             } else {
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     if (tx + q >= tw) break;
                     *reinterpret_cast<uint32_t *>(F.thr + o + 4 * q) = out[r][q];
-                    if (gray_out) *reinterpret_cast<uint32_t *>(gray_out + o + 4 * q) = px[r][q];
+                    if (gray_out) *reinterpret_cast<uint32_t *>(gray_out + o + 4 * q) = R.px[r][q];
                 }
             }
         }
     }
-    __syncthreads();  // the LDS is reused by the next region
 }
 
 // Persistent: the work-groups walk the regions of every frame of the table
-// in turn (XCD-contiguous runs of regions within a frame).
+// in turn (XCD-contiguous runs of regions within a frame). A frame's
+// regions are normally all resident at once, so that all of its loads are
+// in flight together, which HBM's latency-bandwidth product asks for.
 template <int RX, int RY>
 __global__ void __launch_bounds__(256)
 k_threshold(FrontTable tab) {
@@ -273,8 +328,13 @@ k_threshold(FrontTable tab) {
                              (reinterpret_cast<uintptr_t>(F.gray) & 15) == 0;
         for (uint32_t it = blockIdx.x; it < xcd_grid(n_regions); it += gridDim.x) {
             const uint32_t region = xcd_tile(it, n_regions);
-            if (region < n_regions)
-                threshold_region<RX, RY>(L, F, region, regions_x, vec_in, vec_out);
+            if (region >= n_regions) continue;
+            RegionRegs<RX, RY> a;
+            region_load(a, F, region, regions_x, vec_in, true);
+            region_extrema(L, a, F);
+            __syncthreads();
+            region_finish(L, a, F, vec_out);
+            __syncthreads();  // the LDS is reused by the next region
         }
     }
 }
@@ -311,7 +371,8 @@ uint32_t threshold_items(const FrontTable &tab) {
     uint32_t items = 0;
     for (uint32_t i = 0; i < tab.n; i++) {
         const uint32_t rx = (tab.fr[i].w / 4 + RX - 1) / RX, ry = (tab.fr[i].h / 4 + RY - 1) / RY;
-        items = std::max(items, xcd_grid(rx * ry));
+        const uint32_t n = rx * ry;
+        items = std::max(items, xcd_grid(n));
     }
     return items;
 }
