@@ -80,6 +80,60 @@ def collect_candidates(oracle, n_wanted=256, seed0=5000, max_frames=40):
     return frames
 
 
+EDGE_T = np.linspace(0.15, 0.85, 16)
+
+
+def bilinear(img, X, Y):
+    """Bilinear image value and its (X, Y) gradient; pixel (x, y) is the
+    square [x, x+1) x [y, y+1), so its value sits at (x + 0.5, y + 0.5)."""
+    h, w = img.shape
+    u, v = X - 0.5, Y - 0.5
+    x0 = np.clip(np.floor(u).astype(np.int64), 0, w - 2)
+    y0 = np.clip(np.floor(v).astype(np.int64), 0, h - 2)
+    fx, fy = u - x0, v - y0
+    i00, i10, i01, i11 = img[y0, x0], img[y0, x0 + 1], img[y0 + 1, x0], img[y0 + 1, x0 + 1]
+    val = i00 * (1 - fx) * (1 - fy) + i10 * fx * (1 - fy) + i01 * (1 - fx) * fy + i11 * fx * fy
+    dx = (i10 - i00) * (1 - fy) + (i11 - i01) * fy
+    dy = (i01 - i00) * (1 - fx) + (i11 - i10) * fx
+    return val, dx, dy
+
+
+def _edge_samples(c):
+    """Points along the four edges of each quad (n, 4, len(EDGE_T), 2); edge k
+    runs from corner k-1 to corner k."""
+    a = np.roll(c, 1, axis=1)
+    t = EDGE_T[None, None, :, None]
+    return a[:, :, None, :] * (1 - t) + c[:, :, None, :] * t, a
+
+
+def edge_mid_level(img, c):
+    """Per quad the grey level halfway between the two sides of its edges,
+    from samples 1.5 px either side of each edge of the corners c (n, 4, 2)."""
+    p, a = _edge_samples(c)
+    d = c - a
+    n = np.stack([-d[..., 1], d[..., 0]], -1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    off = 1.5 * n[:, :, None, :]
+    hi = bilinear(img, (p + off)[..., 0], (p + off)[..., 1])[0]
+    lo = bilinear(img, (p - off)[..., 0], (p - off)[..., 1])[0]
+    return 0.5 * (hi.mean(axis=(1, 2)) + lo.mean(axis=(1, 2)))
+
+
+def edge_straddle_loss(img, c, mid):
+    """Loss that needs no ground truth: the squared difference between the
+    image along each fitted edge and the quad's mid level (an edge that sits
+    on the intensity step reads the mid level everywhere). Returns the loss
+    and its gradient with respect to the corners (n, 4, 2)."""
+    p, _ = _edge_samples(c)
+    val, dx, dy = bilinear(img, p[..., 0], p[..., 1])
+    r = val - mid[:, None, None]
+    gp = np.stack([2 * r * dx, 2 * r * dy], -1)
+    t = EDGE_T[None, None, :, None]
+    g_end = (gp * t).sum(2)
+    g_start = (gp * (1 - t)).sum(2)
+    return float((r ** 2).sum()), g_end + np.roll(g_start, -1, axis=1)
+
+
 def torch_function():
     """torch.autograd.Function over the C ABI (forward FP32 on the GPU)."""
     import torch
