@@ -714,6 +714,7 @@ void orc_cluster_extents(const void *filt_v, int64_t ncl, void *out_v) {
 
 // Angle sort: src/main.cpp:737-750 (sort by (cluster index, slope); stable here).
 void orc_angle_sort(void *pts_v, uint32_t *idx, int64_t n) {
+    if (n <= 0) return;  // nothing to sort (and no memcpy from an empty vector's data())
     ClusterPoint *pts = reinterpret_cast<ClusterPoint *>(pts_v);
     std::vector<int64_t> order(n);
     std::iota(order.begin(), order.end(), 0);
@@ -1231,9 +1232,17 @@ int64_t orc_fx_do_indexing(const void *ext_v, int64_t E, const void *corners_v, 
 
 // Whole front end (src/main.cpp:322-1023 minus debug dumps and decoding) on
 // one grey frame, decimated by f first. `out` receives up to max_out quads;
-// the return value is the total number of quads.
+// the return value is the total number of quads, or -1 when the decimated
+// frame is below 8x8 or not a multiple of 4 in each dimension: the
+// threshold's last pass indexes the filtered tile extrema as
+// (y / 4) * (w / 4) + x / 4 (src/threshold.cpp:71-78), which reads past them
+// for such dims, so the library rejects them too (ATQ_ERR_BAD_DIMS).
 static int64_t detect_m(const uint8_t *gray_in, int64_t w_in, int64_t h_in, int f, int nthreads,
                         void *out, int64_t max_out, orc_stats *st, bool fx) {
+    {
+        const int64_t wd = f > 1 ? 1 + (w_in - 1) / f : w_in, hd = f > 1 ? 1 + (h_in - 1) / f : h_in;
+        if (w_in < 1 || h_in < 1 || wd < 8 || hd < 8 || (wd & 3) || (hd & 3)) return -1;
+    }
 #ifdef _OPENMP
     int saved = omp_get_max_threads();
     omp_set_num_threads(nthreads_or_default(nthreads));
@@ -1287,7 +1296,7 @@ static int64_t detect_m(const uint8_t *gray_in, int64_t w_in, int64_t h_in, int 
     int64_t oob = do_indexing_m(seg.data(), E, cc.data(), lfp.data(), nfp, cext.data(), fq.data(), fx);
     std::vector<QuadCorners> quads((size_t)std::max<int64_t>(E, 1));
     int64_t nq = orc_process_quads(fq.data(), E, quads.data());
-    if (out && max_out > 0)
+    if (out && max_out > 0 && nq > 0)
         std::memcpy(out, quads.data(), sizeof(QuadCorners) * (size_t)std::min(nq, max_out));
     if (st) {
         st->width = w; st->height = h;

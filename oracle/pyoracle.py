@@ -236,6 +236,9 @@ class Oracle:
         out = np.empty(max_out, R.QUAD_CORNERS)
         st = Stats()
         n = self._f("detect")(_ptr(img), w, h, f, nthreads, _ptr(out), max_out, C.byref(st))
+        if n < 0:
+            raise ValueError(f"{w}x{h} at decimation {f}: the decimated frame must be at least "
+                             "8x8 and a multiple of 4 in each dimension")
         return out[:min(n, max_out)].copy(), st.as_dict()
 
     def max_threads(self):

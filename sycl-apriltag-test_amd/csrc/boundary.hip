@@ -640,7 +640,7 @@ k_pair_rank(PairTable table, const uint64_t *__restrict__ vkeys,
 // plane-0 records first, so a stable sort by r gives the reference's
 // (pair, plane, y, x) order.
 template <bool FX>
-__global__ void __launch_bounds__(256, 7)  // 7 waves per SIMD: no spills
+__global__ void __launch_bounds__(256, 7)  // 7 waves per SIMD: 72 VGPRs, 2-3 spilled to scratch
 k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes, uint32_t w,
              uint32_t h, uint32_t min_size, uint32_t lb, PairTable table,
              const uint32_t *__restrict__ slot_rank, const atq_counts *counts, uint32_t n_tiles,

@@ -74,7 +74,12 @@ BUF = dict(GRAY=0, THRESHOLDED=1, LABELS=2, SIZES=3, BOUNDARY_POINTS=4, BOUNDARY
 
 
 class AtqError(RuntimeError):
-    pass
+    """A non-zero atq status; .code is the ATQ_ERR_* value (0 when the error
+    comes from the binding itself)."""
+
+    def __init__(self, msg, code=0):
+        super().__init__(msg)
+        self.code = code
 
 
 class AtqParams(C.Structure):
@@ -117,7 +122,7 @@ def lib():
 def check(status):
     if status != 0:
         msg = lib().atq_status_string(status).decode()
-        raise AtqError(f"atq status {status}: {msg}")
+        raise AtqError(f"atq status {status}: {msg}", status)
 
 
 # ---- HIP plumbing (the same libamdhip64 instance libatq.so links) ---------

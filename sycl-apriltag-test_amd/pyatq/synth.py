@@ -266,6 +266,22 @@ def diff_frame(seed=5000, n_tags=24):
                       supersample=8)
 
 
+def square_grid(width=4096, height=3072, side=8, pitch=13, margin=3, dark=40, light=200):
+    """Dark side x side squares every pitch pixels on a light background:
+    one valid cluster per square (73,790 at the defaults, more than the
+    reference's u16 cluster ids hold) and a quad for nearly every one."""
+    yy = (np.arange(height) - margin) % pitch < side
+    xx = (np.arange(width) - margin) % pitch < side
+    m = np.outer(yy, xx)
+    m[:margin, :] = False
+    m[:, :margin] = False
+    m[height - pitch:, :] = False
+    m[:, width - pitch:] = False
+    img = np.full((height, width), light, np.uint8)
+    img[m] = dark
+    return Frame(img, 0, 1, [])
+
+
 def decimated_dims(width, height, factor):
     if factor <= 1:
         return width, height

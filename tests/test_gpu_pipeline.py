@@ -268,6 +268,21 @@ def test_detector_stress_12mp(atq, oracle):
     assert n > 100 and err < 2e-3
 
 
+def test_detector_more_valid_clusters_than_u16_ids(atq, oracle):
+    """73,790 valid clusters at 4096x3072: more than the reference's u16
+    cluster ids (65535 is its sentinel) hold, so every cluster index above
+    65,535 goes through the widened u32 ids, element by element against the
+    oracle, labels included."""
+    from pyatq import synth
+    img = synth.square_grid().image
+    o = oracle.pipeline(img, 1)
+    assert len(o["extents"]) > 65535 and len(o["quads"]) > 65535
+    assert int(o["quads"]["blob_index"].max()) > 65535
+    g = run_detector(atq, img, 1)
+    assert int(g["counts"]["overflow"]) == 0
+    check_against_oracle(g, o)
+
+
 def test_every_front_shape_gives_identical_results(atq, oracle):
     """Threshold regions and CCL tiles only split the work differently: the
     thresholded image, the labels and the quads are the same for every shape

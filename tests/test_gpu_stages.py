@@ -92,6 +92,25 @@ def test_bad_dims_rejected(atq):
         atq.threshold_image(t, 10, 10)
     with pytest.raises(atq.AtqError):
         atq.image_segmentation(t, 6, 6)
+    # above the 8192 limit (buffers sized for the dims, so nothing reads past them)
+    for w, h in ((8196, 8), (8, 8196)):
+        big = _up(np.zeros((h, w), np.uint8))
+        with pytest.raises(atq.AtqError) as e:
+            atq.threshold_image(big, w, h)
+        assert e.value.code == 1  # ATQ_ERR_BAD_DIMS
+        with pytest.raises(atq.AtqError) as e:
+            atq.image_segmentation(big, w, h)
+        assert e.value.code == 1
+        with pytest.raises(atq.AtqError) as e:
+            atq.Detector(w, h, decimate=1)
+        assert e.value.code == 1
+    # a frame larger than the detector was created for
+    det = atq.Detector(64, 64, decimate=1)
+    frame = _up(np.zeros((128, 128), np.uint8))
+    with pytest.raises(atq.AtqError) as e:
+        det.detect(frame, 128, 128)
+    assert e.value.code == 2  # ATQ_ERR_CAPACITY
+    det.close()
 
 
 def test_segmentation_repeatable(atq, oracle):

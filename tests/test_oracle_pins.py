@@ -442,3 +442,16 @@ def test_float64_twin_is_exact_on_collinear_sides():
     c = np.array([[100.5, 100.0], [150.0, 104.5], [146.0, 152.0], [98.0, 147.5]])
     got = pyoracle.quad_corners_fp64(_fitted_from_corners(c))
     np.testing.assert_allclose(got, c, atol=1e-9)
+
+
+def test_oracle_widens_cluster_ids_past_u16(oracle):
+    """73,790 valid clusters (one per dark square): the reference's u16
+    cluster index (65535 = sentinel, src/main.cpp:639-651) cannot name them;
+    the oracle's u32 ids give every square its own cluster and quad."""
+    from pyatq import synth
+    img = synth.square_grid().image
+    q, st = oracle.detect(img, 1, max_out=1 << 17)
+    assert st["n_clusters"] == 73790
+    assert len(q) > 65535
+    assert int(q["blob_index"].max()) > 65535
+    assert len(np.unique(q["blob_index"])) == len(q)
