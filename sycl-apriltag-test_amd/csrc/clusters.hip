@@ -206,6 +206,7 @@ struct RunOut {
 // the point's cluster index (every point's key is generated exactly once,
 // by the chunk sort, and a cluster's points keep their positions' range).
 struct AngleKey {
+    static constexpr bool position_low = true;  // low word = index in the cluster
     const atq_cluster_point *in;
     uint32_t *cidx;
     ATQ_DEV uint64_t operator()(uint32_t c, uint32_t start, uint32_t j) const {
@@ -381,10 +382,8 @@ k_fit_reduce(const atq_cluster_point *__restrict__ in, const uint64_t *__restric
         fit_terms(in, start, e0, SEG_CHUNK, src, grey, w, h, p, W, RX, RY, sum, FX);
 #pragma unroll
         for (int q = 0; q < 6; q++) {
-            int64_t x = sum[q];
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-            if (lane == 0) part[q][wave] = x;
+            const int64_t x = wave_inclusive_sum(sum[q]);
+            if (lane == 63) part[q][wave] = x;
         }
         __syncthreads();
         if (threadIdx.x < 6) {

@@ -1,5 +1,6 @@
 // common.hpp - shared device helpers of the gfx950 quad front-end.
 #pragma once
+#include <type_traits>
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -66,16 +67,64 @@ ATQ_DEV uint64_t lanemask_lt() {
     return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
+// DPP lane moves (gfx9-family encodings), every lane of the wave active:
+// row_shr:n (0x110 + n) moves a value n lanes up within each row of 16
+// lanes (lanes with no source in the row read 0); row_bcast:15 (0x142, rows
+// 1 and 3) hands lane 15 of each row to the next row, row_bcast:31 (0x143,
+// rows 2 and 3) hands lane 31 to the upper half; the rows outside the mask
+// read 0. One VALU move each, against a round trip through the LDS
+// crossbar for a lane shuffle.
+template <int CTRL, int ROW_MASK, bool BOUND_ZERO>
+ATQ_DEV uint32_t dpp_move(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROW_MASK, 0xF, BOUND_ZERO);
+}
+template <int CTRL, int ROW_MASK, bool BOUND_ZERO>
+ATQ_DEV uint64_t dpp_move(uint64_t x) {
+    const uint32_t lo = dpp_move<CTRL, ROW_MASK, BOUND_ZERO>((uint32_t)x);
+    const uint32_t hi = dpp_move<CTRL, ROW_MASK, BOUND_ZERO>((uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Inclusive sum over the 64 lanes of a wave in lane order (wrapping
+// arithmetic; 32- or 64-bit integers), every lane active.
+template <typename T>
+ATQ_DEV T wave_inclusive_sum(T x) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit lanes");
+    using U = typename std::conditional<sizeof(T) == 4, uint32_t, uint64_t>::type;
+    U u = (U)x;
+    u += dpp_move<0x111, 0xF, true>(u);
+    u += dpp_move<0x112, 0xF, true>(u);
+    u += dpp_move<0x114, 0xF, true>(u);
+    u += dpp_move<0x118, 0xF, true>(u);
+    u += dpp_move<0x142, 0xA, false>(u);
+    u += dpp_move<0x143, 0xC, false>(u);
+    return (T)u;
+}
+// The same over the first 16 lanes only (lanes 16-63 get their row's sums).
+template <typename T>
+ATQ_DEV T row_inclusive_sum(T x) {
+    using U = typename std::conditional<sizeof(T) == 4, uint32_t, uint64_t>::type;
+    U u = (U)x;
+    u += dpp_move<0x111, 0xF, true>(u);
+    u += dpp_move<0x112, 0xF, true>(u);
+    u += dpp_move<0x114, 0xF, true>(u);
+    u += dpp_move<0x118, 0xF, true>(u);
+    return (T)u;
+}
+// Lane l's value, as a wave-uniform (scalar) value.
+ATQ_DEV uint32_t lane_value(uint32_t x, int l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, l);
+}
+ATQ_DEV uint64_t lane_value(uint64_t x, int l) {
+    return ((uint64_t)lane_value((uint32_t)(x >> 32), l) << 32) | lane_value((uint32_t)x, l);
+}
+ATQ_DEV int64_t lane_value(int64_t x, int l) { return (int64_t)lane_value((uint64_t)x, l); }
+
 // Exclusive prefix of a small per-lane count within the wave; *total gets
-// the wave sum.
+// the wave sum. Every lane active.
 ATQ_DEV uint32_t wave_exclusive_sum(uint32_t v, uint32_t *total) {
-    uint32_t x = v;
-#pragma unroll
-    for (int o = 1; o < WAVE; o <<= 1) {
-        uint32_t y = __shfl_up(x, o, WAVE);
-        if ((int)__lane_id() >= o) x += y;
-    }
-    *total = __shfl(x, WAVE - 1, WAVE);
+    const uint32_t x = wave_inclusive_sum(v);
+    *total = lane_value(x, WAVE - 1);
     return x - v;
 }
 

@@ -19,12 +19,7 @@ template <typename T, int NT = 256>
 ATQ_DEV T block_exclusive_scan(T v, T *lds, T *total) {
     constexpr int NW = NT / 64;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    T x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        T y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
+    const T x = wave_inclusive_sum(v);
     if (lane == 63) lds[wave] = x;
     __syncthreads();
     T before = 0, sum = 0;
@@ -75,12 +70,7 @@ __global__ void __launch_bounds__(1024) k_scan_top(uint32_t *partials, uint32_t 
     __shared__ uint32_t wsum[16];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     uint32_t v = t < (int)ng ? partials[t] : 0u;
-    uint32_t x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
+    const uint32_t x = wave_inclusive_sum(v);
     if (lane == 63) wsum[wave] = x;
     __syncthreads();
     uint32_t before = 0, sum = 0;
@@ -157,10 +147,8 @@ ATQ_DEV uint32_t lb_lookback(uint64_t *flags, uint32_t p, uint32_t tile_sum, uin
         const int first_pre = pre ? __ffsll((long long)pre) - 1 : 64;
         const uint64_t before = first_pre == 64 ? ~0ull : ((1ull << first_pre) - 1);
         if ((zero & before) && ++spins < LB_SPIN_LIMIT) continue;
-        uint32_t c = (int)lane <= first_pre ? (uint32_t)fl : 0u;
-#pragma unroll
-        for (int o2 = 32; o2 > 0; o2 >>= 1) c += __shfl_xor(c, o2, 64);
-        excl += c;
+        const uint32_t c = (int)lane <= first_pre ? (uint32_t)fl : 0u;
+        excl += lane_value(wave_inclusive_sum(c), 63);
         if (first_pre < 64 || spins >= LB_SPIN_LIMIT) break;
         q -= 64;
     }

@@ -6,7 +6,12 @@
 // runs are merged pairwise (merge path: every thread finds where its 4
 // outputs start by binary search on its diagonal and merges them), 10
 // levels for a full chunk. Keys must be distinct (callers put the element's
-// position in the low bits), so the result is a stable sort. A segment that
+// position in the low bits), so the result is a stable sort. A key generator
+// whose low word is exactly the element's index in its segment
+// (KeyGen::position_low) has its chunks sorted on the high words alone, with
+// the position as a 16-bit payload and a stable merge (ties keep position
+// order, which is the order of the low words): half the LDS traffic of the
+// 64-bit merge for the same result. A segment that
 // fits one chunk is finished right there by the caller's Finish functor.
 // Longer segments are written to keys0 and named in a device-side list of
 // long segments. One work-group then merges each SEG_LDS_KEYS block of a
@@ -15,6 +20,8 @@
 // by binary search, alternating keys1 -> keys0 -> keys1 ...;
 // seg_sorted_keys(n, ...) names the buffer a segment of n keys ends in.
 #pragma once
+#include <type_traits>
+
 #include "scan.hpp"
 
 namespace atq {
@@ -64,14 +71,10 @@ ATQ_DEV void seg_block_scan6(const int64_t (&v)[6], int64_t (&incl)[6], int64_t 
                              int64_t *lds) {
     constexpr int NW = SEG_NT / 64;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    static_assert(NW == 16, "the per-wave totals fill one DPP row");
 #pragma unroll
     for (int f = 0; f < 6; f++) {
-        int64_t x = v[f];
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int64_t y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
-        }
+        const int64_t x = wave_inclusive_sum(v[f]);
         incl[f] = x;
         if (lane == 63) lds[f * NW + wave] = x;
     }
@@ -80,12 +83,7 @@ ATQ_DEV void seg_block_scan6(const int64_t (&v)[6], int64_t (&incl)[6], int64_t 
 #pragma unroll
         for (int f = 0; f < 6; f++) {
             const int64_t t = lane < NW ? lds[f * NW + lane] : 0;
-            int64_t x = t;
-#pragma unroll
-            for (int o = 1; o < NW; o <<= 1) {
-                const int64_t y = __shfl_up(x, o, 64);
-                if (lane >= o) x += y;
-            }
+            const int64_t x = row_inclusive_sum(t);
             if (lane < NW) lds[f * NW + lane] = x - t;
             if (lane == NW - 1) lds[6 * NW + f] = x;
         }
@@ -182,6 +180,100 @@ ATQ_DEV void seg_merge_sort(uint64_t *a, uint64_t *b, uint32_t m) {
     }
 }
 
+// KeyGen::position_low, false when absent.
+template <typename T, typename = void>
+struct SegPositionLow : std::false_type {};
+template <typename T>
+struct SegPositionLow<T, std::void_t<decltype(T::position_low)>>
+    : std::integral_constant<bool, T::position_low> {};
+
+// Stable sort of the chunk [b, b + m) of a position-low segment: g holds the
+// keys of chunk positions t + SEG_NT * i (striped; SEG_PAD past m). Sorts
+// the high words with the chunk position (key - b) as payload in the u32/u16
+// buffers K0, K1 (SEG_CHUNK each) and J0, J1, and returns the sorted 64-bit
+// keys of positions e0 + i (e0 = threadIdx.x * SEG_IPT) in v. All SEG_NT
+// threads call it; the buffers are free again once a barrier has passed.
+ATQ_DEV void seg_sort_positions(const uint64_t (&g)[SEG_IPT], uint32_t m, uint32_t b,
+                                uint32_t *K0, uint32_t *K1, uint16_t *J0, uint16_t *J1,
+                                uint64_t (&v)[SEG_IPT]) {
+    const uint32_t t = threadIdx.x, d0 = t * SEG_IPT;
+    uint32_t P = SEG_IPT;
+    while (P < m) P <<= 1;
+#pragma unroll
+    for (int i = 0; i < SEG_IPT; i++) {
+        const uint32_t j = t + SEG_NT * i;
+        K1[j] = j < m ? (uint32_t)(g[i] >> 32) : 0xFFFFFFFFu;  // padding sorts last (stable)
+        J1[j] = (uint16_t)j;
+    }
+    __syncthreads();
+    // four keys per thread in registers, compared with their positions
+    uint64_t r[SEG_IPT];
+#pragma unroll
+    for (int i = 0; i < SEG_IPT; i++) r[i] = ((uint64_t)K1[d0 + i] << 32) | J1[d0 + i];
+    seg_cas(r[0], r[1]);
+    seg_cas(r[2], r[3]);
+    seg_cas(r[0], r[2]);
+    seg_cas(r[1], r[3]);
+    seg_cas(r[1], r[2]);
+    const bool act = d0 < P;
+    if (act)
+#pragma unroll
+        for (int i = 0; i < SEG_IPT; i++) {
+            K0[d0 + i] = (uint32_t)(r[i] >> 32);
+            J0[d0 + i] = (uint16_t)r[i];
+        }
+    __syncthreads();
+    uint32_t *Ks = K0, *Kd = K1;
+    uint16_t *Js = J0, *Jd = J1;
+    for (uint32_t run = SEG_IPT; run < P; run <<= 1) {
+        if (act) {
+            // merge path: i = keys of the left run among the first d outputs;
+            // a left key goes first on a tie
+            const uint32_t base = d0 / (2 * run) * (2 * run), d = d0 - base;
+            uint32_t lo = d > run ? d - run : 0, hi = d < run ? d : run;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (Ks[base + mid] <= Ks[base + run + d - mid - 1]) lo = mid + 1;
+                else hi = mid;
+            }
+            uint32_t i = lo, j = d - lo;
+            uint32_t ak = i < run ? Ks[base + i] : 0xFFFFFFFFu;
+            uint32_t bk = j < run ? Ks[base + run + j] : 0xFFFFFFFFu;
+            uint32_t ok[SEG_IPT];
+            uint16_t oj[SEG_IPT];
+#pragma unroll
+            for (int k = 0; k < SEG_IPT; k++) {
+                const bool ta = j >= run || (i < run && ak <= bk);
+                ok[k] = ta ? ak : bk;
+                oj[k] = Js[ta ? base + i : base + run + j];
+                if (ta) i++;
+                else j++;
+                if (k + 1 < SEG_IPT) {
+                    const bool more = ta ? i < run : j < run;
+                    const uint32_t nk = more ? Ks[ta ? base + i : base + run + j] : 0xFFFFFFFFu;
+                    if (ta) ak = nk;
+                    else bk = nk;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < SEG_IPT; k++) {
+                Kd[d0 + k] = ok[k];
+                Jd[d0 + k] = oj[k];
+            }
+        }
+        __syncthreads();
+        uint32_t *tk = Ks;
+        Ks = Kd;
+        Kd = tk;
+        uint16_t *tj = Js;
+        Js = Jd;
+        Jd = tj;
+    }
+#pragma unroll
+    for (int i = 0; i < SEG_IPT; i++)
+        v[i] = d0 + i < m ? ((uint64_t)Ks[d0 + i] << 32) | (b + Js[d0 + i]) : SEG_PAD;
+}
+
 // KeyGen: uint64_t operator()(uint32_t seg, uint32_t start, uint32_t j) const
 // (key of element j of the segment at `start`). Ext: {start, count}.
 // Finish: void operator()(uint32_t seg, uint32_t start, uint32_t n,
@@ -196,8 +288,12 @@ template <typename KeyGen, typename Finish, typename Ext>
 __global__ void __launch_bounds__(SEG_NT)
 k_seg_chunks(KeyGen gen, Finish fin, const Ext *__restrict__ segs, const uint32_t *n_segs_dev,
              uint32_t cap_segs, uint32_t max_chunks, uint64_t *__restrict__ keys0, SegLong lng) {
+    constexpr bool POS = SegPositionLow<KeyGen>::value;
     __shared__ uint64_t kab[2 * seg_padded(SEG_CHUNK)];
+    static_assert(2 * SEG_CHUNK * (sizeof(uint32_t) + sizeof(uint16_t)) <= sizeof(kab), "");
     uint64_t *const ka = kab, *const kb = kab + seg_padded(SEG_CHUNK);
+    uint32_t *const K = reinterpret_cast<uint32_t *>(kab);
+    uint16_t *const J = reinterpret_cast<uint16_t *>(K + 2 * SEG_CHUNK);
     const uint32_t nseg = min(*n_segs_dev, cap_segs);
     const uint64_t items = (uint64_t)nseg * max_chunks;
     const uint32_t t = threadIdx.x, e0 = t * SEG_IPT;
@@ -207,6 +303,7 @@ k_seg_chunks(KeyGen gen, Finish fin, const Ext *__restrict__ segs, const uint32_
         const uint32_t b = ch * SEG_CHUNK;
         if (b >= n) continue;
         const uint32_t m = min(SEG_CHUNK, n - b);
+        uint64_t v[SEG_IPT];
         {  // every load of a thread in flight before its first LDS store
             uint64_t g[SEG_IPT];
 #pragma unroll
@@ -214,15 +311,18 @@ k_seg_chunks(KeyGen gen, Finish fin, const Ext *__restrict__ segs, const uint32_
                 const uint32_t j = t + SEG_NT * i;
                 g[i] = j < m ? gen(sg, start, b + j) : SEG_PAD;
             }
+            if constexpr (POS) {
+                seg_sort_positions(g, m, b, K, K + SEG_CHUNK, J, J + SEG_CHUNK, v);
+            } else {
 #pragma unroll
-            for (int i = 0; i < SEG_IPT; i++) ka[seg_ix(t + SEG_NT * i)] = g[i];
+                for (int i = 0; i < SEG_IPT; i++) ka[seg_ix(t + SEG_NT * i)] = g[i];
+                __syncthreads();
+                seg_merge_sort(ka, kb, m);
+#pragma unroll
+                for (int i = 0; i < SEG_IPT; i++) v[i] = ka[seg_ix(e0 + i)];
+            }
         }
-        __syncthreads();
-        seg_merge_sort(ka, kb, m);
         if (n <= SEG_CHUNK) {
-            uint64_t v[SEG_IPT];
-#pragma unroll
-            for (int i = 0; i < SEG_IPT; i++) v[i] = ka[seg_ix(e0 + i)];
             fin(sg, start, n, v, kab);
             __syncthreads();
             continue;
@@ -231,26 +331,33 @@ k_seg_chunks(KeyGen gen, Finish fin, const Ext *__restrict__ segs, const uint32_
             const uint32_t slot = atomicAdd(lng.n, 1u);
             if (slot < lng.cap) lng.list[slot] = sg;
         }
-        for (uint32_t j = t; j < m; j += SEG_NT) keys0[start + b + j] = ka[seg_ix(j)];
+        if constexpr (POS) {
+#pragma unroll
+            for (int i = 0; i < SEG_IPT; i++)
+                if (e0 + i < m) keys0[start + b + e0 + i] = v[i];
+        } else {
+            for (uint32_t j = t; j < m; j += SEG_NT) keys0[start + b + j] = ka[seg_ix(j)];
+        }
         __syncthreads();
     }
 }
 
 // Merges the SEG_CHUNK runs of every SEG_LDS_KEYS block of the listed long
-// segments in LDS: kin (runs of SEG_CHUNK) -> kout (runs of SEG_LDS_KEYS,
-// or the whole segment). Each level, every thread finds the start of its
-// outputs on its merge-path diagonal and merges them into registers.
+// segments of more than min_n keys in LDS: kin (runs of SEG_CHUNK) -> kout
+// (runs of SEG_LDS_KEYS, or the whole segment). Each level, every thread
+// finds the start of its outputs on its merge-path diagonal and merges them
+// into registers.
 template <typename Ext>
 __global__ void __launch_bounds__(SEG_NT)
 k_seg_merge_lds(const Ext *__restrict__ segs, SegLong lng, uint32_t max_blocks,
-                const uint64_t *__restrict__ kin, uint64_t *__restrict__ kout) {
+                const uint64_t *__restrict__ kin, uint64_t *__restrict__ kout, uint32_t min_n) {
     __shared__ uint64_t buf[seg_padded(SEG_LDS_KEYS)];
     const uint64_t items = (uint64_t)min(*lng.n, lng.cap) * max_blocks;
     const uint32_t t = threadIdx.x;
     for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
         const Ext e = segs[lng.list[it / max_blocks]];
         const uint32_t b = (uint32_t)(it % max_blocks) * SEG_LDS_KEYS;
-        if (b >= e.count) continue;
+        if (b >= e.count || e.count <= min_n) continue;
         const uint32_t m = min(SEG_LDS_KEYS, e.count - b);
         uint32_t P = SEG_CHUNK;
         while (P < m) P <<= 1;
@@ -288,18 +395,19 @@ k_seg_merge_lds(const Ext *__restrict__ segs, SegLong lng, uint32_t max_blocks,
     }
 }
 
-// One global merge pass over the listed long segments, SEG_MERGE_ITEM keys
-// per item: sorted runs of length `run` pair up into runs of 2*run.
+// One global merge pass over the listed long segments (those of more than
+// min_n keys), SEG_MERGE_ITEM keys per item: sorted runs of length `run`
+// pair up into runs of 2*run.
 template <typename Ext>
 __global__ void __launch_bounds__(SEG_MERGE_NT)
 k_seg_merge(const Ext *__restrict__ segs, SegLong lng, uint32_t max_items, uint32_t run,
-            const uint64_t *__restrict__ kin, uint64_t *__restrict__ kout) {
+            const uint64_t *__restrict__ kin, uint64_t *__restrict__ kout, uint32_t min_n) {
     constexpr int IPT = SEG_MERGE_ITEM / SEG_MERGE_NT;
     const uint64_t items = (uint64_t)min(*lng.n, lng.cap) * max_items;
     for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
         const Ext e = segs[lng.list[it / max_items]];
         const uint32_t n = e.count, b = (uint32_t)(it % max_items) * SEG_MERGE_ITEM;
-        if (n <= run || b >= n) continue;
+        if (n <= run || b >= n || n <= min_n) continue;
         const uint64_t *c = kin + e.start;
         // IPT keys per thread, their binary searches run in lockstep so the
         // loads of one step are in flight together
@@ -350,6 +458,35 @@ k_seg_merge(const Ext *__restrict__ segs, SegLong lng, uint32_t max_items, uint3
     }
 }
 
+// The chunk sort of every segment: segments of at most SEG_CHUNK keys are
+// sorted and finished; longer ones have their chunks sorted into keys0 and
+// are listed in lng (n zero on entry).
+template <typename KeyGen, typename Finish, typename Ext>
+int seg_sort_chunks(hipStream_t s, KeyGen gen, Finish fin, const Ext *segs,
+                    const uint32_t *n_segs_dev, uint32_t cap_segs, uint32_t max_seg,
+                    uint64_t *keys0, SegLong lng) {
+    k_seg_chunks<KeyGen, Finish, Ext><<<1024, SEG_NT, 0, s>>>(gen, fin, segs, n_segs_dev, cap_segs,
+                                                               seg_max_chunks(max_seg), keys0, lng);
+    return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
+}
+
+// The merges of the listed segments of more than min_n keys after
+// seg_sort_chunks (their sorted keys end in seg_sorted_keys(n, keys0, keys1)).
+template <typename Ext>
+int seg_sort_long(hipStream_t s, const Ext *segs, uint32_t max_seg, uint64_t *keys0,
+                  uint64_t *keys1, SegLong lng, uint32_t min_n = 0) {
+    if (max_seg > SEG_CHUNK && max_seg > min_n)
+        k_seg_merge_lds<Ext><<<256, SEG_NT, 0, s>>>(
+            segs, lng, (max_seg + SEG_LDS_KEYS - 1) / SEG_LDS_KEYS, keys0, keys1, min_n);
+    const uint32_t max_items = (max_seg + SEG_MERGE_ITEM - 1) / SEG_MERGE_ITEM;
+    int pass = 0;
+    for (uint32_t run = SEG_LDS_KEYS; run < max_seg; run <<= 1, pass++)
+        k_seg_merge<Ext><<<512, SEG_MERGE_NT, 0, s>>>(segs, lng, max_items, run,
+                                                        (pass & 1) ? keys0 : keys1,
+                                                        (pass & 1) ? keys1 : keys0, min_n);
+    return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
+}
+
 // Sorts every segment; max_seg bounds the segment length (longer segments
 // are left partly sorted: callers bound it by construction). lng.n must be
 // zero on entry; on return the list names the segments longer than
@@ -357,19 +494,9 @@ k_seg_merge(const Ext *__restrict__ segs, SegLong lng, uint32_t max_items, uint3
 template <typename KeyGen, typename Finish, typename Ext>
 int seg_sort(hipStream_t s, KeyGen gen, Finish fin, const Ext *segs, const uint32_t *n_segs_dev,
              uint32_t cap_segs, uint32_t max_seg, uint64_t *keys0, uint64_t *keys1, SegLong lng) {
-    const uint32_t max_chunks = seg_max_chunks(max_seg);
-    k_seg_chunks<KeyGen, Finish, Ext><<<1024, SEG_NT, 0, s>>>(gen, fin, segs, n_segs_dev, cap_segs,
-                                                               max_chunks, keys0, lng);
-    if (max_seg > SEG_CHUNK)
-        k_seg_merge_lds<Ext><<<256, SEG_NT, 0, s>>>(
-            segs, lng, (max_seg + SEG_LDS_KEYS - 1) / SEG_LDS_KEYS, keys0, keys1);
-    const uint32_t max_items = (max_seg + SEG_MERGE_ITEM - 1) / SEG_MERGE_ITEM;
-    int pass = 0;
-    for (uint32_t run = SEG_LDS_KEYS; run < max_seg; run <<= 1, pass++)
-        k_seg_merge<Ext><<<512, SEG_MERGE_NT, 0, s>>>(segs, lng, max_items, run,
-                                                        (pass & 1) ? keys0 : keys1,
-                                                        (pass & 1) ? keys1 : keys0);
-    return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
+    if (int r = seg_sort_chunks(s, gen, fin, segs, n_segs_dev, cap_segs, max_seg, keys0, lng))
+        return r;
+    return seg_sort_long(s, segs, max_seg, keys0, keys1, lng);
 }
 
 }  // namespace atq
