@@ -375,6 +375,72 @@ int atq_linefit_corners_backward(void *stream, const float *image,
                                  const float *grad_corners,
                                  float *grad_weights, float *grad_image);
 
+/* ---- tag payload decode ------------------------------------------------- */
+/* The table-free part of the decode the reference runs on every quad:
+ * decode_quads (src/quad_decoding.cpp:77-128) hands each one to the apriltag
+ * library's quad_decode_task (src/quad_decoding.cpp:46-75), a build-time
+ * dependency (carbotaniuman/apriltag @ a8b2f96, CMakeLists.txt:11-17) absent
+ * from the reference checkout together with its tag36h11 family. What needs
+ * no family table is built here, after apriltag's quad_decode: the
+ * homography from the tag square [-1,1]^2 (corner i at (-1,-1), (1,-1),
+ * (1,1), (-1,1)) to the quad; least-squares planes of the gray level sampled
+ * just outside (white) and just inside (black) the border; the polarity
+ * test; one bilinear sample per bit cell against the mean of the two planes,
+ * sharpened by a 3x3 Laplacian; the code word and its decision margin. The
+ * family's bit layout and code table come from the caller. The image is the
+ * one the quads were found in (the decimated grey image at decimate > 1). */
+#define ATQ_TAG_MAX_BITS 64
+#define ATQ_TAG_MAX_WIDTH 16
+typedef struct atq_tag_layout {
+    int32_t width_at_border;  /* cells across the outer edge of the black
+                                 border, 2 .. ATQ_TAG_MAX_WIDTH            */
+    int32_t total_width;      /* cells across the whole tag, width_at_border
+                                 .. ATQ_TAG_MAX_WIDTH, same parity         */
+    int32_t reversed_border;  /* 1: a white border inside a black ring     */
+    int32_t nbits;            /* 1 .. ATQ_TAG_MAX_BITS                     */
+    float decode_sharpening;  /* apriltag's default is 0.25                */
+    uint8_t bit_x[ATQ_TAG_MAX_BITS]; /* cell of bit i (bit 0 the most      */
+    uint8_t bit_y[ATQ_TAG_MAX_BITS]; /* significant), border coordinates:  */
+                                     /* 0 .. width_at_border - 1 is the    */
+                                     /* black border square                */
+} atq_tag_layout;
+
+typedef struct atq_tag_code {
+    uint64_t code;           /* nbits read in the quad's corner order        */
+    float decision_margin;   /* min(mean white, mean black bit margin);
+                                -1 (and code 0): the polarity test failed    */
+    float homography[9];     /* tag square -> image, row-major, [8] = 1      */
+} atq_tag_code;
+
+/* Decodes quads[0 .. n) (device) in `gray` (device, width x height u8) to
+ * out[0 .. n) (device). n = n_quads, or min(*n_quads_dev, n_quads) when
+ * n_quads_dev (device u32, e.g. the detector's counts) is non-null. layout is
+ * a host pointer, read before the call returns. Stream-ordered. */
+int atq_decode_payloads(void *stream, const uint8_t *gray, size_t width,
+                        size_t height, const atq_quad_corners *quads,
+                        size_t n_quads, const uint32_t *n_quads_dev,
+                        const atq_tag_layout *layout, atq_tag_code *out);
+
+typedef struct atq_tag_match {
+    int32_t id;        /* index in the table; -1: none within max_hamming  */
+    int32_t hamming;   /* bits that differ                                 */
+    int32_t rotation;  /* quarter turns of the read code, each taking the
+                          cell (x, y) to (width_at_border - 1 - y, x)      */
+    int32_t pad_;
+} atq_tag_match;
+
+/* The nearest code of table[0 .. n_table) (device u64, the family's codes in
+ * the layout's bit order) over the four rotations of each codes[i].code,
+ * ties to the lower rotation, then the lower index (the role of apriltag's
+ * quick_decode, which looks the code up in the family's hash of codes within
+ * its Hamming limit). Codes whose polarity test failed match nothing. n and
+ * n_codes_dev as for atq_decode_payloads; the layout's cells must be closed
+ * under the quarter turn (ATQ_ERR_BAD_ARG otherwise). */
+int atq_match_codes(void *stream, const atq_tag_code *codes, size_t n_codes,
+                    const uint32_t *n_codes_dev, const uint64_t *table,
+                    size_t n_table, const atq_tag_layout *layout,
+                    int max_hamming, atq_tag_match *out);
+
 /* ---- whole pipeline ----------------------------------------------------- */
 
 typedef struct atq_detector atq_detector;

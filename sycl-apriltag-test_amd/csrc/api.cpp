@@ -156,6 +156,32 @@ int atq_find_boundaries(void *stream, const uint32_t *labels, const uint32_t *si
                                          blob_labels, 25);
 }
 
+static_assert(sizeof(atq_tag_layout) == 148 && sizeof(atq_tag_code) == 48 &&
+                  sizeof(atq_tag_match) == 16,
+              "decode records as pyatq/records.py lays them out");
+
+int atq_decode_payloads(void *stream, const uint8_t *gray, size_t width, size_t height,
+                        const atq_quad_corners *quads, size_t n_quads, const uint32_t *n_quads_dev,
+                        const atq_tag_layout *layout, atq_tag_code *out) {
+    if (!gray || !layout || ((!quads || !out) && (n_quads || n_quads_dev))) return ATQ_ERR_BAD_ARG;
+    if (width < 1 || height < 1 || width > 0x7FFFFFFFu || height > 0x7FFFFFFFu) return ATQ_ERR_BAD_DIMS;
+    if (n_quads > 0xFFFFFFFFu) return ATQ_ERR_BAD_ARG;
+    if (int r = ensure_init()) return r;
+    return launch_decode_payloads(S(stream), gray, width, height, quads, n_quads, n_quads_dev,
+                                  *layout, out);
+}
+
+int atq_match_codes(void *stream, const atq_tag_code *codes, size_t n_codes,
+                    const uint32_t *n_codes_dev, const uint64_t *table, size_t n_table,
+                    const atq_tag_layout *layout, int max_hamming, atq_tag_match *out) {
+    if (!layout || ((!codes || !out) && (n_codes || n_codes_dev)) || (!table && n_table))
+        return ATQ_ERR_BAD_ARG;
+    if (n_codes > 0xFFFFFFFFu) return ATQ_ERR_BAD_ARG;
+    if (int r = ensure_init()) return r;
+    return launch_match_codes(S(stream), codes, n_codes, n_codes_dev, table, n_table, *layout,
+                              max_hamming, out);
+}
+
 int atq_linefit_corners_forward(void *stream, const float *image, size_t width, size_t height,
                                 const atq_cluster_point *points, size_t n_points,
                                 const float *boundary_weights, const atq_diff_quad *quads,

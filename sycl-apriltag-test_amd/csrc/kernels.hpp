@@ -125,6 +125,14 @@ int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const
 constexpr unsigned BOUNDARY_TILE_PX = 1024;  // pixels per compaction tile
 
 // diff.hip
+// decode.hip
+int launch_decode_payloads(hipStream_t s, const uint8_t *gray, size_t w, size_t h,
+                           const atq_quad_corners *quads, size_t n, const uint32_t *n_dev,
+                           const atq_tag_layout &layout, atq_tag_code *out);
+int launch_match_codes(hipStream_t s, const atq_tag_code *codes, size_t n, const uint32_t *n_dev,
+                       const uint64_t *table, size_t n_table, const atq_tag_layout &layout,
+                       int max_hamming, atq_tag_match *out);
+
 int launch_diff_forward(hipStream_t s, const float *img, size_t w, size_t h,
                         const atq_cluster_point *pts, const float *sw, const atq_diff_quad *quads,
                         size_t nq, float *corners, float *w_out);

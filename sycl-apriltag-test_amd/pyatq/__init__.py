@@ -62,6 +62,8 @@ SIGNATURES = {
     "atq_detector_set_shapes": (C.c_int, [_P, _P]),
     "atq_detector_get_shapes": (C.c_int, [_P, _P]),
     "atq_autotune": (C.c_int, [_P, _P, _P, _S, _S, _P]),
+    "atq_decode_payloads": (C.c_int, [_P, _P, _S, _S, _P, _S, _P, _P, _P]),
+    "atq_match_codes": (C.c_int, [_P, _P, _S, _P, _P, _S, _P, C.c_int, _P]),
     "atq_linefit_corners_forward": (C.c_int, [_P, _P, _S, _S, _P, _S, _P, _P, _S, _P, _P]),
     "atq_linefit_corners_backward": (C.c_int, [_P, _P, _S, _S, _P, _S, _P, _P, _S, _P, _P, _P]),
 }
@@ -384,6 +386,43 @@ def find_boundaries(labels_dev, sizes_dev, w, h):
     keys = dev_empty(8 * 4 * w * h)
     check(lib().atq_find_boundaries(stream(), P(labels_dev), P(sizes_dev), P(pts), P(keys), w, h))
     return pts, keys
+
+
+def tag_layout(d):
+    """atq_tag_layout record from a dict (width_at_border, total_width,
+    reversed_border, bit_x, bit_y, decode_sharpening)."""
+    from . import records as R
+    lay = np.zeros(1, R.TAG_LAYOUT)
+    n = len(d["bit_x"])
+    lay["width_at_border"] = d["width_at_border"]
+    lay["total_width"] = d["total_width"]
+    lay["reversed_border"] = d.get("reversed_border", 0)
+    lay["nbits"] = n
+    lay["decode_sharpening"] = d.get("decode_sharpening", 0.25)
+    lay["bit_x"][0, :n] = d["bit_x"]
+    lay["bit_y"][0, :n] = d["bit_y"]
+    return lay
+
+
+def decode_payloads(gray_dev, w, h, quads_dev, n, layout, n_dev_ptr=None):
+    """atq_decode_payloads -> (device buffer of atq_tag_code, its n slots)."""
+    lay = tag_layout(layout) if isinstance(layout, dict) else layout
+    out = dev_empty(max(n, 1) * 48)
+    nd = C.c_void_p(n_dev_ptr) if n_dev_ptr else C.c_void_p(0)
+    check(lib().atq_decode_payloads(stream(), P(gray_dev), w, h, quads_dev if isinstance(quads_dev, C.c_void_p)
+                                    else P(quads_dev), n, nd,
+                                    C.c_void_p(lay.ctypes.data), P(out)))
+    return out
+
+
+def match_codes(codes_dev, n, table_dev, n_table, layout, max_hamming, n_dev_ptr=None):
+    """atq_match_codes -> device buffer of atq_tag_match."""
+    lay = tag_layout(layout) if isinstance(layout, dict) else layout
+    out = dev_empty(max(n, 1) * 16)
+    nd = C.c_void_p(n_dev_ptr) if n_dev_ptr else C.c_void_p(0)
+    check(lib().atq_match_codes(stream(), P(codes_dev), n, nd, P(table_dev), n_table,
+                                C.c_void_p(lay.ctypes.data), int(max_hamming), P(out)))
+    return out
 
 
 class Detector:

@@ -23,10 +23,18 @@ COUNTS = np.dtype([(n, "<u4") for n in (
     "n_valid_pairs", "sort_passes", "rank_shift")]
     + [("reserved", "<u4", (2,))])
 
+TAG_LAYOUT = np.dtype([("width_at_border", "<i4"), ("total_width", "<i4"),
+                       ("reversed_border", "<i4"), ("nbits", "<i4"),
+                       ("decode_sharpening", "<f4"), ("bit_x", "u1", (64,)),
+                       ("bit_y", "u1", (64,))])
+TAG_CODE = np.dtype([("code", "<u8"), ("decision_margin", "<f4"), ("homography", "<f4", (9,))])
+TAG_MATCH = np.dtype([("id", "<i4"), ("hamming", "<i4"), ("rotation", "<i4"), ("pad_", "<i4")])
+
 SIZES = {
     "EXTREMA": 2, "BOUNDARY_POINT": 4, "CLUSTER_POINT": 8, "CLUSTER_EXTENTS": 8,
     "CLUSTER_BOUNDS": 32, "LINE_FIT_POINT": 64, "PEAK_EXTENTS": 8, "CORNER": 12,
     "FITTED_QUAD": 272, "QUAD_CORNERS": 40, "COUNTS": 64, "DIFF_QUAD": 24,
+    "TAG_LAYOUT": 148, "TAG_CODE": 48, "TAG_MATCH": 16,
 }
 for _name, _size in SIZES.items():
     assert globals()[_name].itemsize == _size, (_name, globals()[_name].itemsize)

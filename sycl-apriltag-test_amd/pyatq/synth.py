@@ -34,6 +34,7 @@ class Frame:
     seed: int
     decimate: int = 1
     corners: list = field(default_factory=list)  # ground truth, per tag 4x2
+    payloads: list = field(default_factory=list)  # per tag its 6x6 cells (row = y), white = 1
 
 
 def _homography(rng, side, cx, cy, tilt_max_deg, focal=1200.0):
@@ -67,7 +68,9 @@ def _tag_cells(rng):
 
 def _render_tag(img, rng, side, tilt_max_deg, black, white, supersample=1, margin=2):
     """Draws one tag into img (float32) in place; returns its 4 image corners
-    (the outer corners of the black square) or None when it does not fit."""
+    (the outer corners of the black square, corner 0 at the payload's top-left
+    cell, then along the payload's rows) and its 6x6 payload, or None when it
+    does not fit."""
     hgt, wid = img.shape
     half = side * 5.0 / 8.0  # half-size of the 10-cell grid
     reach = half * 1.5 + margin
@@ -110,7 +113,8 @@ def _render_tag(img, rng, side, tilt_max_deg, black, white, supersample=1, margi
     region = img[y0:y1, x0:x1]
     region[:] = region * (1.0 - cov) + acc
     sq = side / 2.0
-    return _apply_h(h, np.array([[-sq, -sq], [sq, -sq], [sq, sq], [-sq, sq]]))
+    return (_apply_h(h, np.array([[-sq, -sq], [sq, -sq], [sq, sq], [-sq, sq]])),
+            cells[2:8, 2:8].astype(np.uint8))
 
 
 def _render_clutter(img, rng, n_shapes, n_lines, size_lo=5, size_hi=150):
@@ -213,13 +217,14 @@ def make_frame(seed, width, height, n_tags, side=(40, 120), tilt=30.0, bg=(80, 1
         oy = int(rng.integers(0, height - mh + 1))
         ox = int(rng.integers(0, width - mw + 1))
         img[oy:oy + mh, ox:ox + mw] = mz
-    corners = []
+    corners, payloads = [], []
     for _ in range(n_tags):
         s = rng.uniform(side[0], side[1])
         c = _render_tag(img, rng, s, tilt, 0.0, 255.0, supersample=supersample)
         if c is not None:
-            corners.append(c)
-    return Frame(_finish(img, rng, blur, noise), seed, decimate, corners)
+            corners.append(c[0])
+            payloads.append(c[1])
+    return Frame(_finish(img, rng, blur, noise), seed, decimate, corners, payloads)
 
 
 # ---- the named workloads ---------------------------------------------------
