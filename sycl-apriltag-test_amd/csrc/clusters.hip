@@ -322,27 +322,18 @@ ATQ_DEV void fit_chunk(const atq_cluster_point *in, atq_cluster_point *out, uint
     }
 }
 
-// Clusters of at most SEG_CHUNK points are finished inside the sort, from
-// the sorted keys in registers.
-template <bool FX>
-struct AngleFinish {
-    const atq_cluster_point *in;
-    atq_cluster_point *out;
-    const uint8_t *grey;
-    uint32_t w, h;
-    atq_line_fit_point *lf;
+// Clusters of at most SEG_CHUNK points leave the sort with their sorted keys
+// in `keys` (where seg_sorted_keys puts a cluster of that length); the fit
+// runs in its own kernel, so that the sort keeps few registers and several
+// work-groups share a CU.
+struct AngleStore {
+    uint64_t *keys;
     ATQ_DEV void operator()(uint32_t, uint32_t start, uint32_t n, const uint64_t (&v)[SEG_IPT],
-                            uint64_t *lds) const {
-        uint32_t src[SEG_IPT];
+                            uint64_t *) const {
+        const uint32_t e0 = threadIdx.x * SEG_IPT;
 #pragma unroll
-        for (int i = 0; i < SEG_IPT; i++) src[i] = (uint32_t)v[i];
-        const int64_t zero[6] = {0, 0, 0, 0, 0, 0};
-        int64_t tot[6];
-        // lds: the keys' half is free once the scan's barriers have passed;
-        // the scan's scratch sits at the far end of the other half
-        int64_t *l = reinterpret_cast<int64_t *>(lds);
-        fit_chunk(in, out, start, 0, n, src, grey, w, h, lf, zero, tot,
-                  l + 2 * SEG_CHUNK - (6 * SEG_NT / 64 + 6), l, FX);
+        for (int i = 0; i < SEG_IPT; i++)
+            if (e0 + i < n) keys[start + e0 + i] = v[i];
     }
 };
 
@@ -365,11 +356,12 @@ k_fit_reduce(const atq_cluster_point *__restrict__ in, const uint64_t *__restric
              uint32_t h, int64_t *__restrict__ slot_sums) {
     constexpr int NW = SEG_NT / 64;
     __shared__ int64_t part[6][NW];
-    const uint64_t items = (uint64_t)min(*lng.n, lng.cap) * max_chunks;
+    const uint32_t nl = min(*lng.n, lng.cap);
+    const uint64_t items = (uint64_t)nl * max_chunks;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t e0 = threadIdx.x * SEG_IPT;
-    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
-        const uint32_t c = lng.list[it / max_chunks], k = (uint32_t)(it % max_chunks);
+    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {  // chunk-major
+        const uint32_t c = lng.list[it % nl], k = (uint32_t)(it / nl);
         const uint32_t start = ext[c].start, n = ext[c].count, b = k * SEG_CHUNK;
         if (b + SEG_CHUNK >= n) continue;  // last chunks carry nothing on
         const uint64_t *keys = seg_sorted_keys(n, keys0, keys1) + start;
@@ -403,9 +395,10 @@ ATQ_DEV int64_t uniform64(int64_t v) {
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-// Chunk-parallel form for the listed long clusters (their sorted keys in
-// keys0/keys1): every chunk writes its points and its moments prefix-summed
-// from the sums of the cluster's earlier chunks (slot table, k_fit_reduce).
+// Chunk-parallel form over every cluster (its sorted keys where
+// seg_sorted_keys puts them): every chunk writes its points and its moments
+// prefix-summed from the sums of the cluster's earlier chunks (slot table,
+// k_fit_reduce, for the clusters longer than SEG_CHUNK).
 // SEQ: every cluster (n_clusters_dev) in input order, one work-group per
 // cluster walking its chunks and carrying the sums itself.
 template <bool SEQ, bool FX>
@@ -418,12 +411,12 @@ k_fit_chunks(const atq_cluster_point *__restrict__ in, atq_cluster_point *__rest
              const int64_t *__restrict__ slot_sums) {
     __shared__ int64_t lds[6 * SEG_NT / 64 + 6];
     __shared__ int64_t stage[FIT_STAGE_WORDS];
-    const uint64_t items = SEQ ? (uint64_t)min(*n_clusters_dev, cap_cl)
-                               : (uint64_t)min(*lng.n, lng.cap) * max_chunks;
+    const uint32_t ncl = min(*n_clusters_dev, cap_cl);
+    const uint64_t items = (uint64_t)ncl * (SEQ ? 1 : max_chunks);
     const uint32_t e0 = threadIdx.x * SEG_IPT;
-    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
-        const uint32_t c = SEQ ? (uint32_t)it : lng.list[it / max_chunks];
-        const uint32_t k = SEQ ? 0u : (uint32_t)(it % max_chunks);
+    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {  // chunk-major
+        const uint32_t c = SEQ ? (uint32_t)it : (uint32_t)(it % ncl);
+        const uint32_t k = SEQ ? 0u : (uint32_t)(it / ncl);
         const uint32_t start = ext[c].start, n = ext[c].count, b = k * SEG_CHUNK;
         if (b >= n) continue;
         const uint64_t *keys = SEQ ? nullptr : seg_sorted_keys(n, keys0, keys1) + start;
@@ -505,14 +498,18 @@ int angle_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out
               uint32_t max_cluster, const uint8_t *grey, uint32_t w, uint32_t h,
               atq_line_fit_point *lf, uint64_t *keys0, uint64_t *keys1, int64_t *slot_sums,
               SegLong lng, uint32_t *cidx) {
-    int r = seg_sort(s, AngleKey{in, cidx}, AngleFinish<FX>{in, out, grey, w, h, lf}, ext,
-                     &counts->n_clusters, cap_cl, max_cluster, keys0, keys1, lng);
-    if (r || max_cluster <= SEG_CHUNK) return r;
+    // the sort (every cluster's keys where seg_sorted_keys puts them), then
+    // the fit of every chunk of every cluster
+    int r = seg_sort(s, AngleKey{in, cidx}, AngleStore{keys1}, ext, &counts->n_clusters, cap_cl,
+                     max_cluster, keys0, keys1, lng);
+    if (r) return r;
     const uint32_t max_chunks = seg_max_chunks(max_cluster);
-    if (lf)
+    if (lf && max_cluster > SEG_CHUNK)
         k_fit_reduce<FX><<<512, SEG_NT, 0, s>>>(in, keys0, keys1, ext, lng, max_chunks, grey, w,
                                                 h, slot_sums);
-    k_fit_chunks<false, FX><<<512, SEG_NT, 0, s>>>(in, out, keys0, keys1, ext,
+    // one work-group per chunk slot at most a few rounds deep: the dispatcher
+    // hands the chunks that exist to whichever CU is free
+    k_fit_chunks<false, FX><<<2048, SEG_NT, 0, s>>>(in, out, keys0, keys1, ext,
                                                    &counts->n_clusters, cap_cl, lng, max_chunks,
                                                    grey, w, h, lf, slot_sums);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;

@@ -282,9 +282,10 @@ __global__ void __launch_bounds__(256)
 k_corner_emit(const atq_peak_extents *__restrict__ peaks, SegLong lng, uint32_t max_chunks,
               const uint64_t *__restrict__ keys0, const uint64_t *__restrict__ keys1,
               atq_corner *out) {
-    const uint64_t items = (uint64_t)min(*lng.n, lng.cap) * max_chunks;
-    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
-        const uint32_t e = lng.list[it / max_chunks], ch = (uint32_t)(it % max_chunks);
+    const uint32_t nl = min(*lng.n, lng.cap);
+    const uint64_t items = (uint64_t)nl * max_chunks;
+    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {  // chunk-major
+        const uint32_t e = lng.list[it % nl], ch = (uint32_t)(it / nl);
         const uint32_t cs = peaks[e].start, m = peaks[e].count;
         if (ch * SEG_CHUNK >= m) continue;
         const uint32_t c = out[cs].cluster_index;

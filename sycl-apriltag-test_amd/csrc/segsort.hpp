@@ -280,16 +280,19 @@ ATQ_DEV void seg_sort_positions(const uint64_t (&g)[SEG_IPT], uint32_t m, uint32
 // const uint64_t (&v)[SEG_IPT], uint64_t *lds) const, called by every thread
 // of the work-group with a whole sorted segment of n <= SEG_CHUNK keys in
 // registers (key e0 + i in v[i], e0 = threadIdx.x * SEG_IPT) and lds,
-// 2 * seg_padded(SEG_CHUNK) u64 of scratch: the second half is free on
-// entry, the first half (where the keys were read from) once a barrier has
-// passed. Keys move
+// 2 * seg_padded(SEG_CHUNK) u64 of scratch (none for a position-low
+// KeyGen): the second half is free on entry, the first half (where the keys
+// were read from) once a barrier has passed. Keys move
 // between global memory and LDS coalesced.
 template <typename KeyGen, typename Finish, typename Ext>
 __global__ void __launch_bounds__(SEG_NT)
 k_seg_chunks(KeyGen gen, Finish fin, const Ext *__restrict__ segs, const uint32_t *n_segs_dev,
              uint32_t cap_segs, uint32_t max_chunks, uint64_t *__restrict__ keys0, SegLong lng) {
     constexpr bool POS = SegPositionLow<KeyGen>::value;
-    __shared__ uint64_t kab[2 * seg_padded(SEG_CHUNK)];
+    // the position sort needs its u32/u16 buffers only (48 KiB: three
+    // work-groups per CU); the 64-bit merge and its Finish two padded chunks
+    constexpr uint32_t KAB = POS ? SEG_CHUNK * 3 / 2 : 2 * seg_padded(SEG_CHUNK);
+    __shared__ uint64_t kab[KAB];
     static_assert(2 * SEG_CHUNK * (sizeof(uint32_t) + sizeof(uint16_t)) <= sizeof(kab), "");
     uint64_t *const ka = kab, *const kb = kab + seg_padded(SEG_CHUNK);
     uint32_t *const K = reinterpret_cast<uint32_t *>(kab);
@@ -298,7 +301,9 @@ k_seg_chunks(KeyGen gen, Finish fin, const Ext *__restrict__ segs, const uint32_
     const uint64_t items = (uint64_t)nseg * max_chunks;
     const uint32_t t = threadIdx.x, e0 = t * SEG_IPT;
     for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
-        const uint32_t sg = (uint32_t)(it / max_chunks), ch = (uint32_t)(it % max_chunks);
+        // chunk-major: every segment's first chunk, then every second chunk, ...,
+        // so that the chunks that exist spread evenly over the work-groups
+        const uint32_t sg = (uint32_t)(it % nseg), ch = (uint32_t)(it / nseg);
         const uint32_t start = segs[sg].start, n = segs[sg].count;
         const uint32_t b = ch * SEG_CHUNK;
         if (b >= n) continue;
@@ -352,11 +357,12 @@ __global__ void __launch_bounds__(SEG_NT)
 k_seg_merge_lds(const Ext *__restrict__ segs, SegLong lng, uint32_t max_blocks,
                 const uint64_t *__restrict__ kin, uint64_t *__restrict__ kout, uint32_t min_n) {
     __shared__ uint64_t buf[seg_padded(SEG_LDS_KEYS)];
-    const uint64_t items = (uint64_t)min(*lng.n, lng.cap) * max_blocks;
+    const uint32_t nl = min(*lng.n, lng.cap);
+    const uint64_t items = (uint64_t)nl * max_blocks;
     const uint32_t t = threadIdx.x;
-    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
-        const Ext e = segs[lng.list[it / max_blocks]];
-        const uint32_t b = (uint32_t)(it % max_blocks) * SEG_LDS_KEYS;
+    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {  // block-major
+        const Ext e = segs[lng.list[it % nl]];
+        const uint32_t b = (uint32_t)(it / nl) * SEG_LDS_KEYS;
         if (b >= e.count || e.count <= min_n) continue;
         const uint32_t m = min(SEG_LDS_KEYS, e.count - b);
         uint32_t P = SEG_CHUNK;
@@ -403,10 +409,11 @@ __global__ void __launch_bounds__(SEG_MERGE_NT)
 k_seg_merge(const Ext *__restrict__ segs, SegLong lng, uint32_t max_items, uint32_t run,
             const uint64_t *__restrict__ kin, uint64_t *__restrict__ kout, uint32_t min_n) {
     constexpr int IPT = SEG_MERGE_ITEM / SEG_MERGE_NT;
-    const uint64_t items = (uint64_t)min(*lng.n, lng.cap) * max_items;
-    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
-        const Ext e = segs[lng.list[it / max_items]];
-        const uint32_t n = e.count, b = (uint32_t)(it % max_items) * SEG_MERGE_ITEM;
+    const uint32_t nl = min(*lng.n, lng.cap);
+    const uint64_t items = (uint64_t)nl * max_items;
+    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {  // item-major
+        const Ext e = segs[lng.list[it % nl]];
+        const uint32_t n = e.count, b = (uint32_t)(it / nl) * SEG_MERGE_ITEM;
         if (n <= run || b >= n || n <= min_n) continue;
         const uint64_t *c = kin + e.start;
         // IPT keys per thread, their binary searches run in lockstep so the
