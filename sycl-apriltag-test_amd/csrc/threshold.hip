@@ -114,20 +114,37 @@ ATQ_DEV uint32_t bytes_greater(uint32_t v, uint32_t cc) {
     return (ge >> 7) * 0xFFu;
 }
 
-// Min and max of the bytes of four words (one 4x4 tile).
+// Min and max of the bytes of four words (one 4x4 tile): the even and the
+// odd bytes of each word as two 16-bit lanes, reduced with packed 16-bit
+// min/max (v_pk_min_u16 / v_pk_max_u16) over the 8 lane pairs, then the
+// two lanes.
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+ATQ_DEV u16x2 as_u16x2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
 ATQ_DEV void tile_minmax(uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t &mn,
                          uint32_t &mx) {
-    mn = 255;
-    mx = 0;
     const uint32_t v4[4] = {a, b, c, d};
+    u16x2 lo = as_u16x2(0x00FF00FFu), hi = as_u16x2(0u);
 #pragma unroll
-    for (int k = 0; k < 4; k++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const uint32_t v = (v4[k] >> (8 * j)) & 0xFFu;
-            mn = min(mn, v);
-            mx = max(mx, v);
-        }
+    for (int k = 0; k < 4; k++) {
+        const u16x2 e = as_u16x2(v4[k] & 0x00FF00FFu), o = as_u16x2((v4[k] >> 8) & 0x00FF00FFu);
+        lo = __builtin_elementwise_min(lo, __builtin_elementwise_min(e, o));
+        hi = __builtin_elementwise_max(hi, __builtin_elementwise_max(e, o));
+    }
+    const uint32_t l = __builtin_bit_cast(uint32_t, lo), h = __builtin_bit_cast(uint32_t, hi);
+    mn = min(l & 0xFFFFu, l >> 16);
+    mx = max(h & 0xFFFFu, h >> 16);
+}
+
+// Streaming 16-byte loads and stores (the "nt" cache hint): the input
+// pixels are read once and the thresholded bytes are written once here.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+ATQ_DEV uint4 load_stream(const uint8_t *p) {
+    const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+ATQ_DEV void store_stream(uint8_t *p, uint4 v) {
+    const u32x4 u = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(u, reinterpret_cast<u32x4 *>(p));
 }
 
 // Tile extrema of a region and its one-tile halo ring, min | max << 8
@@ -151,7 +168,11 @@ struct RegionRegs {
     bool live;
 };
 
-template <int RX, int RY>
+// FAST: grey input, no decimation, w a multiple of 16 and every buffer
+// 16-byte aligned, no tile extrema asked for (the host checks this for
+// every frame of the launch): each strip row is one streaming 16-byte load
+// and store, with none of the general path's per-row branches.
+template <int RX, int RY, bool FAST>
 ATQ_DEV void region_load(RegionRegs<RX, RY> &R, const FrontFrame &F, uint32_t region,
                          uint32_t regions_x, bool vec_in, bool live) {
     constexpr int SX = RX / 4;
@@ -169,7 +190,19 @@ ATQ_DEV void region_load(RegionRegs<RX, RY> &R, const FrontFrame &F, uint32_t re
     for (int r = 0; r < 4; r++)
 #pragma unroll
         for (int q = 0; q < 4; q++) R.px[r][q] = 0;
-    if (live && ty < th) {
+    if (FAST) {
+        // w % 16 == 0: a strip that starts in the frame ends in it
+        if (live && ty < th && tx < tw) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const uint4 v = load_stream(F.in + (size_t)(4 * ty + r) * F.w + 4 * tx);
+                R.px[r][0] = v.x;
+                R.px[r][1] = v.y;
+                R.px[r][2] = v.z;
+                R.px[r][3] = v.w;
+            }
+        }
+    } else if (live && ty < th) {
 #pragma unroll
         for (int r = 0; r < 4; r++) load_strip_row(F, vec_in, 4 * tx, 4 * ty + r, R.px[r]);
     }
@@ -184,7 +217,11 @@ ATQ_DEV void region_load(RegionRegs<RX, RY> &R, const FrontFrame &F, uint32_t re
     const bool halo_in = live && tid < RING && htx >= 0 && htx < tw && hty >= 0 && hty < th;
 #pragma unroll
     for (int r = 0; r < 4; r++) R.hp[r] = 0;
-    if (halo_in) {
+    if (FAST && halo_in) {
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            R.hp[r] = *reinterpret_cast<const uint32_t *>(F.in + (size_t)(4 * hty + r) * F.w + 4 * htx);
+    } else if (halo_in) {
         const uint32_t f = F.f;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
@@ -205,7 +242,7 @@ ATQ_DEV void region_load(RegionRegs<RX, RY> &R, const FrontFrame &F, uint32_t re
 
 // Tile extrema of the strip and of the halo tile to LDS (out-of-image
 // tiles are neutral: min 255, max 0); the raw extrema to F.ext if asked.
-template <int RX, int RY>
+template <int RX, int RY, bool FAST>
 ATQ_DEV void region_extrema(ThresholdLds<RX, RY> &L, const RegionRegs<RX, RY> &R,
                             const FrontFrame &F) {
     constexpr int SX = RX / 4;
@@ -223,7 +260,7 @@ ATQ_DEV void region_extrema(ThresholdLds<RX, RY> &L, const RegionRegs<RX, RY> &R
         if (row_in && tx + q < tw) tile_minmax(R.px[0][q], R.px[1][q], R.px[2][q], R.px[3][q], smn[q], smx[q]);
         L.e[sy + 1][4 * sx + q + 1] = (uint16_t)(smn[q] | (smx[q] << 8));
     }
-    if (F.ext && row_in) {
+    if (!FAST && F.ext && row_in) {
 #pragma unroll
         for (int q = 0; q < 4; q++)
             if (tx + q < tw)
@@ -246,7 +283,7 @@ ATQ_DEV void region_extrema(ThresholdLds<RX, RY> &L, const RegionRegs<RX, RY> &R
 // After the barrier: 3x3 dilation of the strip's tiles, then the threshold
 // of its pixels (127 if max-min < min_diff, else 255 if
 // v > min+(max-min)/2, else 0), stored with the decimated grey when f > 1.
-template <int RX, int RY>
+template <int RX, int RY, bool FAST>
 ATQ_DEV void region_finish(const ThresholdLds<RX, RY> &L, const RegionRegs<RX, RY> &R,
                            const FrontFrame &F, bool vec_out) {
     constexpr int SX = RX / 4;
@@ -278,7 +315,7 @@ ATQ_DEV void region_finish(const ThresholdLds<RX, RY> &L, const RegionRegs<RX, R
                 mn = min(mn, nb[dr][q + dc] & 0xFFu);
                 mx = max(mx, nb[dr][q + dc] >> 8);
             }
-        if (F.fil && row_in && tx + q < tw)
+        if (!FAST && F.fil && row_in && tx + q < tw)
             F.fil[(size_t)ty * tw + tx + q] = atq_extrema{(uint8_t)mn, (uint8_t)mx};
         const bool flat = (int)(mx - mn) < F.min_diff;
         const uint32_t comp = (mn + (mx - mn) / 2) & 0xFFu;
@@ -288,7 +325,14 @@ ATQ_DEV void region_finish(const ThresholdLds<RX, RY> &L, const RegionRegs<RX, R
         for (int r = 0; r < 4; r++)
             out[r][q] = flat ? 0x7F7F7F7Fu : (comp < 255 ? bytes_greater(R.px[r][q], cc) : 0u);
     }
-    if (row_in && tx < tw) {
+    if (FAST) {
+        if (row_in && tx < tw) {
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                store_stream(F.thr + (size_t)(4 * ty + r) * w + 4 * tx,
+                             make_uint4(out[r][0], out[r][1], out[r][2], out[r][3]));
+        }
+    } else if (row_in && tx < tw) {
         uint8_t *gray_out = F.f > 1 || F.channels != 1 ? F.gray : nullptr;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
@@ -314,7 +358,7 @@ ATQ_DEV void region_finish(const ThresholdLds<RX, RY> &L, const RegionRegs<RX, R
 // in turn (XCD-contiguous runs of regions within a frame). A frame's
 // regions are normally all resident at once, so that all of its loads are
 // in flight together, which HBM's latency-bandwidth product asks for.
-template <int RX, int RY>
+template <int RX, int RY, bool FAST>
 __global__ void __launch_bounds__(256)
 k_threshold(FrontTable tab) {
     __shared__ ThresholdLds<RX, RY> L;
@@ -330,10 +374,10 @@ k_threshold(FrontTable tab) {
             const uint32_t region = xcd_tile(it, n_regions);
             if (region >= n_regions) continue;
             RegionRegs<RX, RY> a;
-            region_load(a, F, region, regions_x, vec_in, true);
-            region_extrema(L, a, F);
+            region_load<RX, RY, FAST>(a, F, region, regions_x, vec_in, true);
+            region_extrema<RX, RY, FAST>(L, a, F);
             __syncthreads();
-            region_finish(L, a, F, vec_out);
+            region_finish<RX, RY, FAST>(L, a, F, vec_out);
             __syncthreads();  // the LDS is reused by the next region
         }
     }
@@ -377,11 +421,25 @@ uint32_t threshold_items(const FrontTable &tab) {
     return items;
 }
 
+// Every frame of the launch can take the FAST path (region_load).
+bool threshold_fast(const FrontTable &tab) {
+    for (uint32_t i = 0; i < tab.n; i++) {
+        const FrontFrame &F = tab.fr[i];
+        const bool aligned = ((reinterpret_cast<uintptr_t>(F.in) | reinterpret_cast<uintptr_t>(F.thr)) & 15) == 0;
+        if (F.channels != 1 || F.f != 1 || F.w != F.w_in || (F.w & 15) || !aligned || F.ext || F.fil)
+            return false;
+    }
+    return true;
+}
+
 template <int RX, int RY>
 void threshold_launch(hipStream_t s, const FrontTable &tab) {
     // persistent: at most 2048 work-groups walk every frame's regions
     const uint32_t grid = std::min<uint32_t>(threshold_items<RX, RY>(tab), 2048u);
-    k_threshold<RX, RY><<<grid ? grid : 8, 256, 0, s>>>(tab);
+    if (threshold_fast(tab))
+        k_threshold<RX, RY, true><<<grid ? grid : 8, 256, 0, s>>>(tab);
+    else
+        k_threshold<RX, RY, false><<<grid ? grid : 8, 256, 0, s>>>(tab);
 }
 }  // namespace
 
