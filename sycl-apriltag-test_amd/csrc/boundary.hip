@@ -497,6 +497,21 @@ k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
 constexpr uint32_t RANK_NONE = 0xFFFFFFFFu;     // slot_rank: the pair fails the filter
 constexpr uint32_t RANK_PENDING = 0xFFFFFFFEu;  // valid, beyond the rank list
 
+// The ranked pairs' lookup table: open addressing over vmask + 1 words, a
+// word (rank << 2lb) | packed key (packed = min << lb | max), all ones when
+// empty. Usable when a rank below 16384 and the empty word's all-ones rank
+// field fit beside the 2lb key bits (lb <= 24, frames below 2^24 pixels);
+// larger frames look their pairs up in the pair table instead.
+constexpr uint64_t VTAB_EMPTY = ~0ull;
+ATQ_DEV bool vtab_usable(uint32_t lb) { return 2 * lb + 15 <= 64; }
+ATQ_DEV uint32_t vtab_hash(uint64_t packed, uint32_t vbits) {
+    const uint32_t x = ((uint32_t)packed ^ ((uint32_t)(packed >> 32) * 0x9E3779B1u)) * 0x85EBCA77u;
+    return x >> (32 - vbits);
+}
+ATQ_DEV uint64_t packed_pair(uint64_t key, uint32_t lb) {
+    return ((key >> 32) << lb) | (key & 0xFFFFFFFFull);
+}
+
 // Verdicts of the table's slots and the list of valid pairs, in slot order:
 // slot i holds a valid pair when its count is in [24, max_count] and its
 // reduction passes ValidBlobFilter (include/cluster_bounds.h:85-120). One
@@ -510,11 +525,15 @@ __global__ void __launch_bounds__(256)
 k_pair_valid(PairTable t, uint32_t w, uint32_t h, uint32_t max_count,
              uint32_t *__restrict__ slot_rank, uint64_t *__restrict__ vkeys,
              uint32_t *__restrict__ vslots, uint32_t *__restrict__ vcnt, uint32_t cap,
-             atq_counts *counts, uint64_t *flags, uint64_t *next_flags, uint32_t n_flags) {
+             atq_counts *counts, uint64_t *flags, uint64_t *next_flags, uint32_t n_flags,
+             uint64_t *__restrict__ vtab, uint32_t vwords) {
     __shared__ uint32_t lds4[4];
     __shared__ uint32_t s_prefix;
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n_flags; i += gridDim.x * 256)
         next_flags[i] = 0;
+    // the ranked pairs' lookup table, emptied for k_pair_rank
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; 2 * i < vwords; i += gridDim.x * 256)
+        *reinterpret_cast<uint4 *>(vtab + 2 * i) = make_uint4(~0u, ~0u, ~0u, ~0u);
     const uint32_t slots = t.mask + 1;  // a power of two >= 256
     const uint32_t tile = blockIdx.x, i0 = tile * PV_TILE + threadIdx.x * PV_ITEMS;
     const uint2 c2 = i0 < slots ? *reinterpret_cast<const uint2 *>(t.cnt + i0) : make_uint2(0, 0);
@@ -572,7 +591,8 @@ k_pair_rank(PairTable table, const uint64_t *__restrict__ vkeys,
             const uint32_t *__restrict__ vslots, const uint32_t *__restrict__ vcnt,
             atq_counts *counts, uint32_t cap, uint32_t lb, uint32_t *__restrict__ slot_rank,
             uint64_t *__restrict__ rank_keys, atq_cluster_bounds *__restrict__ bounds,
-            atq_cluster_extents *__restrict__ ext, float2 *__restrict__ centre) {
+            atq_cluster_extents *__restrict__ ext, float2 *__restrict__ centre,
+            uint64_t *__restrict__ vtab, uint32_t vmask) {
     __shared__ uint64_t tile[RANK_TILE];
     __shared__ uint32_t tcnt[RANK_TILE];
     __shared__ uint32_t part[RANK_WAVES][64], spart[RANK_WAVES][64];
@@ -616,7 +636,16 @@ k_pair_rank(PairTable table, const uint64_t *__restrict__ vkeys,
         }
         const uint32_t slot = vslots[i], c = vcnt[i];
         slot_rank[slot] = r;
-        rank_keys[r] = ((mine >> 32) << lb) | (mine & 0xFFFFFFFFull);
+        const uint64_t packed = packed_pair(mine, lb);
+        rank_keys[r] = packed;
+        if (vtab_usable(lb)) {
+            // keys are distinct: the first empty word on the probe path
+            const uint64_t word = ((uint64_t)r << (2 * lb)) | packed;
+            for (uint32_t hh = vtab_hash(packed, __popc(vmask));; hh = (hh + 1) & vmask)
+                if (atomicCAS((unsigned long long *)&vtab[hh], (unsigned long long)VTAB_EMPTY,
+                              (unsigned long long)word) == VTAB_EMPTY)
+                    break;
+        }
         atq_cluster_bounds b = table_bounds(table, slot, c);
         b.start = st;
         bounds[r] = b;
@@ -639,39 +668,94 @@ k_pair_rank(PairTable table, const uint64_t *__restrict__ vkeys,
 // counts per plane to tile_counts[plane][tile]; the gather then lays out all
 // plane-0 records first, so a stable sort by r gives the reference's
 // (pair, plane, y, x) order.
+//
+// A thread's records come in runs of one pair (consecutive records with one
+// key); the first record of each run looks its pair's rank up in the ranked
+// pairs' table (one 8-byte load, all of the thread's first probes in
+// flight together; a probe that meets another pair's word walks on), the
+// rest of the run takes it over. With more valid pairs than ranked, or
+// frames too large for the table, the pair table and its slot verdicts
+// answer instead; after a table overflow every record is kept.
 template <bool FX>
-__global__ void __launch_bounds__(256, 7)  // 7 waves per SIMD: 72 VGPRs, 2-3 spilled to scratch
+__global__ void __launch_bounds__(256)
 k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes, uint32_t w,
              uint32_t h, uint32_t min_size, uint32_t lb, PairTable table,
-             const uint32_t *__restrict__ slot_rank, const atq_counts *counts, uint32_t n_tiles,
+             const uint32_t *__restrict__ slot_rank, const uint64_t *__restrict__ vtab,
+             uint32_t vmask, const atq_counts *counts, uint32_t n_tiles,
              uint64_t *__restrict__ tile_keys, atq_boundary_point *__restrict__ tile_points,
              uint32_t *__restrict__ tile_counts) {
     const uint32_t tile = xcd_tile(blockIdx.x, n_tiles);
     if (tile >= n_tiles) return;
-    __shared__ uint64_t lk[TILE_SLOTS];
-    __shared__ uint32_t lc[TILE_SLOTS];  // the pair's rank, RANK_PENDING or RANK_NONE
     __shared__ uint64_t lds8[4];
-    for (int i = threadIdx.x; i < TILE_SLOTS; i += 256) {
-        lk[i] = 0;
-        lc[i] = 0;
-    }
     const bool keep_all = counts->pair_filter_off != 0;
     const uint32_t np = counts->sort_passes;
     const bool raster = np == 0 || counts->rank_shift != 0;  // else plane by plane
-    __syncthreads();
+    const bool ranked = np != 0 && vtab_usable(lb);
     PixelRecords r[BND_PX_PER_THREAD];
     tile_records(labels, sizes, w, h, min_size, tile, r, FX);
-    auto rank_of = [&](uint64_t key) -> uint32_t {
-        // every pair is in the table unless it overflowed (then keep_all)
-        const uint32_t g = global_slot(table, key, false);
-        return g == NO_SLOT ? RANK_NONE : slot_rank[g];
-    };
-    if (!keep_all) {
-        for_each_pair_run(r, [&](uint64_t key, const PairPart &) { tile_slot(lk, key, true); }, FX);
-        __syncthreads();
-        for (int i = threadIdx.x; i < TILE_SLOTS; i += 256)
-            if (lk[i]) lc[i] = rank_of(~lk[i]);
-        __syncthreads();
+    // every key becomes the packed pair key min << lb | max; run heads
+    uint32_t head = 0;
+    {
+        uint64_t prev = ~0ull;
+#pragma unroll
+        for (int k = 0; k < BND_PX_PER_THREAD; k++)
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                if (!((r[k].mask >> p) & 1u)) continue;
+                const uint64_t pk = packed_pair(r[k].k[p], lb);
+                r[k].k[p] = pk;
+                if (pk != prev) head |= 1u << (4 * k + p);
+                prev = pk;
+            }
+    }
+    // rank of every record (RANK_NONE: dropped)
+    uint32_t rank[4 * BND_PX_PER_THREAD];
+    if (keep_all) {
+#pragma unroll
+        for (int i = 0; i < 4 * BND_PX_PER_THREAD; i++) rank[i] = RANK_PENDING;
+    } else if (ranked) {
+        const uint32_t vbits = __popc(vmask);
+        const uint64_t kmask = (1ull << (2 * lb)) - 1;
+        uint32_t hh[4 * BND_PX_PER_THREAD];
+        uint64_t v[4 * BND_PX_PER_THREAD];
+#pragma unroll
+        for (int i = 0; i < 4 * BND_PX_PER_THREAD; i++) {
+            hh[i] = 0;
+            v[i] = VTAB_EMPTY;
+            if ((head >> i) & 1u) {
+                hh[i] = vtab_hash(r[i / 4].k[i % 4], vbits);
+                v[i] = vtab[hh[i]];
+            }
+        }
+        uint32_t cur = RANK_NONE;
+#pragma unroll
+        for (int i = 0; i < 4 * BND_PX_PER_THREAD; i++) {
+            if ((head >> i) & 1u) {
+                const uint64_t pk = r[i / 4].k[i % 4];
+                uint64_t x = v[i];
+                uint32_t at = hh[i];
+                while (x != VTAB_EMPTY && (x & kmask) != pk) {
+                    at = (at + 1) & vmask;
+                    x = vtab[at];
+                }
+                cur = x == VTAB_EMPTY ? RANK_NONE : (uint32_t)(x >> (2 * lb));
+            }
+            rank[i] = cur;
+        }
+    } else {
+        // valid pairs beyond the rank list (or frames too large for the
+        // lookup table): the pair table's slot verdicts
+        uint32_t cur = RANK_NONE;
+#pragma unroll
+        for (int i = 0; i < 4 * BND_PX_PER_THREAD; i++) {
+            if ((head >> i) & 1u) {
+                const uint64_t pk = r[i / 4].k[i % 4];
+                const uint64_t key = ((pk >> lb) << 32) | (pk & ((1ull << lb) - 1));
+                const uint32_t g = global_slot(table, key, false);
+                cur = g == NO_SLOT ? RANK_NONE : slot_rank[g];
+            }
+            rank[i] = cur;
+        }
     }
     // keep: bit 4k+p for record (pixel k, plane p); r[k].k[p] becomes its
     // output key; c: the thread's kept records per plane, 16 bits each
@@ -681,20 +765,12 @@ k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
     for (int k = 0; k < BND_PX_PER_THREAD; k++)
 #pragma unroll
         for (int p = 0; p < 4; p++) {
-            if (!((r[k].mask >> p) & 1u)) continue;
-            const uint64_t key = r[k].k[p];
-            uint32_t rank = RANK_PENDING;
-            if (!keep_all) {
-                const uint32_t sl = tile_slot(lk, key, false);
-                rank = sl != NO_SLOT ? lc[sl] : rank_of(key);
-            }
-            if (rank != RANK_NONE) {
-                keep |= 1u << (4 * k + p);
-                c += 1ull << (16 * p);
-                const uint64_t packed = ((key >> 32) << lb) | (key & 0xFFFFFFFFull);
-                r[k].k[p] = np ? (raster ? ((uint64_t)rank << 2) | (uint64_t)p : (uint64_t)rank)
-                               : (packed << 2) | (uint64_t)p;
-            }
+            const uint32_t rk = rank[4 * k + p];
+            if (!((r[k].mask >> p) & 1u) || rk == RANK_NONE) continue;
+            keep |= 1u << (4 * k + p);
+            c += 1ull << (16 * p);
+            r[k].k[p] = np ? (raster ? ((uint64_t)rk << 2) | (uint64_t)p : (uint64_t)rk)
+                           : (r[k].k[p] << 2) | (uint64_t)p;
         }
     if (raster) c = (c & 0xFFFFu) + ((c >> 16) & 0xFFFFu) + ((c >> 32) & 0xFFFFu) + (c >> 48);
     // the tile's records densely to its region, plane by plane, each plane
@@ -907,7 +983,8 @@ int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const
     const uint32_t slots = table.mask + 1;
     if (!sc.flags || atq_div_up(slots, PV_TILE) + 1 > sc.flag_count ||
         4 * tiles / LB_TILE + 1 > sc.flag_count ||
-        slots < 256 || rk.cap == 0 || rk.cap % 64)
+        slots < 256 || rk.cap == 0 || rk.cap % 64 || !rk.vtab || (rk.vmask & (rk.vmask + 1)) ||
+        rk.vmask + 1 < 2 * rk.cap)
         return ATQ_ERR_BAD_ARG;
     // a few tiles per work-group (merged in LDS before the global table), at
     // most ~12 of them at large frames
@@ -925,21 +1002,21 @@ int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const
         sc.calls++;
         k_pair_valid<<<atq_div_up(slots, PV_TILE), 256, 0, s>>>(
             table, (uint32_t)w, (uint32_t)h, max_count, rk.slot_rank, rk.vkeys, rk.vslots, rk.vcnt,
-            rk.cap, counts, cur, next, sc.flag_count);
+            rk.cap, counts, cur, next, sc.flag_count, rk.vtab, rk.vmask + 1);
     }
     k_pair_rank<<<rk.cap / 64, 64 * RANK_WAVES, 0, s>>>(table, rk.vkeys, rk.vslots, rk.vcnt, counts,
                                                          rk.cap, label_bits, rk.slot_rank,
                                                          rk.rank_keys, out.cl.bounds, out.cl.ext,
-                                                         out.cl.centre);
+                                                         out.cl.centre, rk.vtab, rk.vmask);
     uint32_t *tile_counts = rk.tile_counts;
     if (fx)
         k_bnd_filter<true><<<xcd_grid(tiles), 256, 0, s>>>(
             labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size, label_bits, table,
-            rk.slot_rank, counts, tiles, rk.tile_keys, rk.tile_points, tile_counts);
+            rk.slot_rank, rk.vtab, rk.vmask, counts, tiles, rk.tile_keys, rk.tile_points, tile_counts);
     else
         k_bnd_filter<false><<<xcd_grid(tiles), 256, 0, s>>>(
             labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size, label_bits, table,
-            rk.slot_rank, counts, tiles, rk.tile_keys, rk.tile_points, tile_counts);
+            rk.slot_rank, rk.vtab, rk.vmask, counts, tiles, rk.tile_keys, rk.tile_points, tile_counts);
     uint32_t *tile_off = nullptr;
     if (tiles > GATHER_SUM_TILES) {
         tile_off = tile_counts + 4 * tiles;

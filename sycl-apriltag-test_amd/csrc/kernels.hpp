@@ -92,7 +92,9 @@ struct PairTable {
 };
 // Buffers of the detector's boundary pass: per table slot the verdict /
 // rank of its pair, the listed valid pairs (key min << 32 | max, slot,
-// record count), rank -> packed key (cap pairs at most, a multiple of 64), the per-tile
+// record count), rank -> packed key (cap pairs at most, a multiple of 64),
+// the ranked pairs' lookup table (vmask + 1 >= 2 * cap words, a power of
+// two: (rank << 2lb) | packed key, all ones when empty), the per-tile
 // record regions (4 * w * h entries) and the tiles' counts and offsets (2
 // per 1024-pixel tile).
 struct PairRanks {
@@ -102,6 +104,8 @@ struct PairRanks {
     uint32_t *vcnt;
     uint64_t *rank_keys;
     uint32_t cap;
+    uint64_t *vtab;
+    uint32_t vmask;
     uint64_t *tile_keys;
     atq_boundary_point *tile_points;
     uint32_t *tile_counts;

@@ -81,6 +81,7 @@ struct atq_detector {
     uint64_t *vkeys;       // the frame's valid pairs: key and slot
     uint32_t *vslots, *vcnt;
     uint64_t *rank_keys;   // rank -> packed pair key
+    uint64_t *vtab;        // ranked pairs' lookup table (4 * rank_cap words)
     float2 *centres;       // rank -> the cluster's (cx, cy)
     uint64_t *scan_flags;  // two look-back flag arrays, zeroed before each frame
     uint64_t *pair_keys;   // pair table, zeroed with counts before each frame
