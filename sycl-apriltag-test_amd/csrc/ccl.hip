@@ -539,18 +539,22 @@ k_ccl_local(FrontTable tab) {
 // and the codes of its probes that leave the tile) and the entries of the
 // blocks those probes reach (their local roots), one round of independent
 // loads. Each edge is a pair of local roots; the wave keeps the distinct
-// pairs in an LDS hash set and only the lane that inserts a pair merges it
-// on the global parent array (lock-free atomicMin union,
-// include/union_find.h:50-77). A tile border is crossed by few components,
-// so a tile issues a handful of global unions.
+// pairs in an LDS hash set, the lane that inserts a pair appends it to the
+// wave's edge list, and then the lanes merge the listed pairs one each on
+// the global parent array (lock-free atomicMin union,
+// include/union_find.h:50-77), so that the unions' chains of dependent
+// atomics run side by side rather than one after another in the lanes
+// that found several edges. A tile border is crossed by few components, so
+// a tile issues a handful of global unions.
 template <int TBY>
 constexpr int border_items() { return TBX + 2 * (TBY - 1); }
 constexpr int BORDER_SLOTS = 512;
+constexpr int BORDER_LIST = 256;  // listed edges per tile (more: merged where found)
 
 // The border edges of one tile of frame F (one wave; lane k).
 template <int TBY>
-ATQ_DEV void ccl_border_tile(uint64_t *hs, const FrontFrame &F, uint32_t tile, uint32_t tiles_x,
-                             uint32_t n_tiles) {
+ATQ_DEV void ccl_border_tile(uint64_t *hs, uint64_t *el, uint32_t *en, const FrontFrame &F,
+                             uint32_t tile, uint32_t tiles_x, uint32_t n_tiles) {
     constexpr int ITEMS = border_items<TBY>();
     static_assert(ITEMS <= 64, "one border block per lane");
     constexpr uint32_t NONE = 0xFFFFFFFFu;
@@ -617,7 +621,21 @@ ATQ_DEV void ccl_border_tile(uint64_t *hs, const FrontFrame &F, uint32_t tile, u
                 break;
             }
         }
-        if (mine) g_union(parent, ea[i], eb[i]);
+        if (mine) {
+            const uint32_t at = atomicAdd(en, 1u);
+            if (at < (uint32_t)BORDER_LIST) el[at] = key;
+            else g_union(parent, ea[i], eb[i]);
+        }
+    }
+    // the wave's LDS operations complete in order: every listed edge is in
+    // place once the count is read back
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t n = min(*en, (uint32_t)BORDER_LIST);
+    for (uint32_t j = k; j < n; j += 64) {
+        const uint64_t e = el[j];
+        g_union(parent, (uint32_t)(e >> 32), (uint32_t)e);
     }
     (void)h;
 }
@@ -627,6 +645,8 @@ template <int TBY>
 __global__ void __launch_bounds__(256)
 k_ccl_border(FrontTable tab) {
     __shared__ uint64_t set[4][BORDER_SLOTS];
+    __shared__ uint64_t list[4][BORDER_LIST];
+    __shared__ uint32_t list_n[4];
     const uint32_t wave = threadIdx.x >> 6, k = threadIdx.x & 63;
     uint64_t *hs = set[wave];
     for (uint32_t fi = 0; fi < tab.n; fi++) {
@@ -635,8 +655,9 @@ k_ccl_border(FrontTable tab) {
         const uint32_t n_tiles = tiles_x * ((F.h + 2 * TBY - 1) / (2 * TBY));
         for (uint32_t g = blockIdx.x; g < (n_tiles + 3) / 4; g += gridDim.x) {
             for (int i = k; i < BORDER_SLOTS; i += 64) hs[i] = ~0ull;
+            if (k == 0) list_n[wave] = 0;
             __syncthreads();
-            ccl_border_tile<TBY>(hs, F, 4 * g + wave, tiles_x, n_tiles);
+            ccl_border_tile<TBY>(hs, list[wave], &list_n[wave], F, 4 * g + wave, tiles_x, n_tiles);
             __syncthreads();
         }
     }
