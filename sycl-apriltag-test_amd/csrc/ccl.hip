@@ -736,6 +736,12 @@ ATQ_DEV void ccl_label_tiles(uint32_t *table, const FrontFrame &F, uint32_t tile
             cur[k] = n[k];
             if (table[j]) act |= 1u << k;
         }
+        // the pixel counts of the tile's local roots, loaded alongside the
+        // walks (only a merged root's count is used, and no other
+        // work-group adds into a merged root's count)
+        uint32_t sz[NPT];
+#pragma unroll
+        for (int k = 0; k < NPT; k++) sz[k] = (act >> k) & 1u ? sizes[node_gid(n[k], w)] : 0u;
         // final roots, path halving (every word lowered stays an ancestor)
         uint32_t todo = act;
         while (todo) {
@@ -764,10 +770,7 @@ ATQ_DEV void ccl_label_tiles(uint32_t *table, const FrontFrame &F, uint32_t tile
         for (int k = 0; k < NPT; k++) {
             if (!((act >> k) & 1u)) continue;
             const uint32_t rg = node_gid(cur[k], w);
-            if (cur[k] != n[k]) {
-                const uint32_t sz = sizes[node_gid(n[k], w)];
-                if (sz && g_load(&sizes[rg]) < size_cap) atomicAdd(&sizes[rg], sz);
-            }
+            if (cur[k] != n[k] && sz[k] && g_load(&sizes[rg]) < size_cap) atomicAdd(&sizes[rg], sz[k]);
             table[tid + 256 * k] = rg;
         }
     }
