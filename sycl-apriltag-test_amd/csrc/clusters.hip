@@ -255,6 +255,84 @@ ATQ_DEV void fit_terms(const atq_cluster_point *in, uint32_t start, uint32_t e0,
     }
 }
 
+// The same terms from the points already in angle order (sorted[at + e0 +
+// i]) and their weights (wts, k_fit_terms): one round of coalesced loads.
+ATQ_DEV void fit_terms_sorted(const atq_cluster_point *sorted, const uint16_t *wts, uint32_t at,
+                              uint32_t e0, uint32_t m, uint32_t (&W)[SEG_IPT],
+                              uint32_t (&RX)[SEG_IPT], uint32_t (&RY)[SEG_IPT], int64_t (&sum)[6],
+                              bool fx) {
+    atq_cluster_point p[SEG_IPT];
+    uint32_t wt[SEG_IPT];
+#pragma unroll
+    for (int i = 0; i < SEG_IPT; i++) {
+        const bool in = e0 + i < m;
+        p[i] = in ? sorted[at + e0 + i] : atq_cluster_point{};
+        wt[i] = in ? wts[at + e0 + i] : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < 6; q++) sum[q] = 0;
+#pragma unroll
+    for (int i = 0; i < SEG_IPT; i++) {
+        W[i] = RX[i] = RY[i] = 0;
+        if (e0 + i >= m) continue;
+        W[i] = wt[i];
+        RX[i] = x_with_dx(p[i].packed_x, fx) + 1;
+        RY[i] = y_with_dy(p[i].packed_x, p[i].packed_y, fx) + 1;
+        const atq_line_fit_point f = linefit_from_terms(W[i], RX[i], RY[i]);
+        sum[0] += f.Mx;
+        sum[1] += f.My;
+        sum[2] += f.Mxx;
+        sum[3] += f.Myy;
+        sum[4] += f.Mxy;
+        sum[5] += f.W;
+    }
+}
+
+// Every cluster's points in angle order to `out` and their line-fit weights
+// (include/line_fit_point.h:50-86: 1 + the grey gradient's magnitude) to
+// wts, FIT_SUB positions per work-group item, four per thread in lockstep
+// (sorted keys, then points, then grey neighbourhoods: three rounds of
+// independent loads). Few registers, so many work-groups share a CU and
+// hide the gathers' latency; the prefix-sum kernels then read both arrays
+// coalesced.
+constexpr uint32_t FIT_SUB = 1024;
+template <bool FX>
+__global__ void __launch_bounds__(256)
+k_fit_terms(const atq_cluster_point *__restrict__ in, const uint64_t *__restrict__ keys0,
+            const uint64_t *__restrict__ keys1, const atq_cluster_extents *__restrict__ ext,
+            const uint32_t *n_clusters_dev, uint32_t cap_cl, uint32_t max_subs,
+            const uint8_t *__restrict__ grey, uint32_t w, uint32_t h,
+            atq_cluster_point *__restrict__ out, uint16_t *__restrict__ wts) {
+    const uint32_t ncl = min(*n_clusters_dev, cap_cl);
+    const uint64_t items = (uint64_t)ncl * max_subs;
+    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {  // sub-chunk-major
+        const uint32_t c = (uint32_t)(it % ncl), sb = (uint32_t)(it / ncl);
+        const uint32_t start = ext[c].start, n = ext[c].count, b = sb * FIT_SUB;
+        if (b >= n) continue;
+        const uint64_t *keys = seg_sorted_keys(n, keys0, keys1) + start;
+        uint32_t j[4], src[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            j[i] = b + threadIdx.x + 256 * i;
+            src[i] = j[i] < n ? (uint32_t)keys[j[i]] : 0u;
+        }
+        atq_cluster_point p[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) p[i] = in[start + src[i]];
+        uint32_t g[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) linefit_gather(p[i].packed_x, p[i].packed_y, w, h, grey, g[i], FX);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (j[i] >= n) continue;
+            uint32_t W, rx, ry;
+            linefit_terms_from(p[i].packed_x, p[i].packed_y, w, h, g[i], &W, &rx, &ry, FX);
+            out[start + j[i]] = p[i];
+            wts[start + j[i]] = (uint16_t)W;
+        }
+    }
+}
+
 // One chunk of up to SEG_CHUNK points of a cluster in angle order, chunk
 // positions [base, base + m) (src[i]: the point at chunk position e0 + i,
 // e0 = threadIdx.x * SEG_IPT, for e0 + i < m): writes the points to `out`
@@ -264,13 +342,15 @@ ATQ_DEV void fit_terms(const atq_cluster_point *in, uint32_t start, uint32_t e0,
 // scratch, stage FIT_STAGE_WORDS int64 for the store exchange. Each point
 // keeps only its weight and doubled coordinates in registers; the moments
 // are rebuilt from them when written.
+// PRE: the points are already in `out` in angle order with their weights
+// in wts (k_fit_terms), and src is unused.
+template <bool PRE>
 ATQ_DEV void fit_chunk(const atq_cluster_point *in, atq_cluster_point *out, uint32_t start,
                        uint32_t base, uint32_t m, const uint32_t (&src)[SEG_IPT],
                        const uint8_t *grey, uint32_t w, uint32_t h, atq_line_fit_point *lf,
-                       const int64_t (&carry)[6], int64_t (&tot)[6], int64_t *lds,
-                       int64_t *stage, bool fx) {
+                       const uint16_t *wts, const int64_t (&carry)[6], int64_t (&tot)[6],
+                       int64_t *lds, int64_t *stage, bool fx) {
     const uint32_t t = threadIdx.x, e0 = t * SEG_IPT;
-    atq_cluster_point p[SEG_IPT];
     uint32_t W[SEG_IPT], RX[SEG_IPT], RY[SEG_IPT];
     int64_t sum[6];
     if (!lf) {
@@ -280,11 +360,16 @@ ATQ_DEV void fit_chunk(const atq_cluster_point *in, atq_cluster_point *out, uint
                 if (e0 + i < m) out[start + base + e0 + i] = in[start + src[i]];
         return;
     }
-    fit_terms(in, start, e0, m, src, grey, w, h, p, W, RX, RY, sum, fx);
-    if (out) {
+    if constexpr (PRE) {
+        fit_terms_sorted(out, wts, start + base, e0, m, W, RX, RY, sum, fx);
+    } else {
+        atq_cluster_point p[SEG_IPT];
+        fit_terms(in, start, e0, m, src, grey, w, h, p, W, RX, RY, sum, fx);
+        if (out) {
 #pragma unroll
-        for (int i = 0; i < SEG_IPT; i++)
-            if (e0 + i < m) out[start + base + e0 + i] = p[i];
+            for (int i = 0; i < SEG_IPT; i++)
+                if (e0 + i < m) out[start + base + e0 + i] = p[i];
+        }
     }
     int64_t acc[6];
     seg_block_scan6(sum, acc, tot, lds);
@@ -350,10 +435,9 @@ ATQ_DEV uint32_t fit_slot(uint32_t c, uint32_t start, uint32_t k) {
 // adds).
 template <bool FX>
 __global__ void __launch_bounds__(SEG_NT)
-k_fit_reduce(const atq_cluster_point *__restrict__ in, const uint64_t *__restrict__ keys0,
-             const uint64_t *__restrict__ keys1, const atq_cluster_extents *__restrict__ ext,
-             SegLong lng, uint32_t max_chunks, const uint8_t *__restrict__ grey, uint32_t w,
-             uint32_t h, int64_t *__restrict__ slot_sums) {
+k_fit_reduce(const atq_cluster_point *__restrict__ sorted, const uint16_t *__restrict__ wts,
+             const atq_cluster_extents *__restrict__ ext, SegLong lng, uint32_t max_chunks,
+             int64_t *__restrict__ slot_sums) {
     constexpr int NW = SEG_NT / 64;
     __shared__ int64_t part[6][NW];
     const uint32_t nl = min(*lng.n, lng.cap);
@@ -364,14 +448,9 @@ k_fit_reduce(const atq_cluster_point *__restrict__ in, const uint64_t *__restric
         const uint32_t c = lng.list[it % nl], k = (uint32_t)(it / nl);
         const uint32_t start = ext[c].start, n = ext[c].count, b = k * SEG_CHUNK;
         if (b + SEG_CHUNK >= n) continue;  // last chunks carry nothing on
-        const uint64_t *keys = seg_sorted_keys(n, keys0, keys1) + start;
-        uint32_t src[SEG_IPT];
-#pragma unroll
-        for (int i = 0; i < SEG_IPT; i++) src[i] = (uint32_t)keys[b + e0 + i];
-        atq_cluster_point p[SEG_IPT];
         uint32_t W[SEG_IPT], RX[SEG_IPT], RY[SEG_IPT];
         int64_t sum[6];
-        fit_terms(in, start, e0, SEG_CHUNK, src, grey, w, h, p, W, RX, RY, sum, FX);
+        fit_terms_sorted(sorted, wts, start + b, e0, SEG_CHUNK, W, RX, RY, sum, FX);
 #pragma unroll
         for (int q = 0; q < 6; q++) {
             const int64_t x = wave_inclusive_sum(sum[q]);
@@ -400,15 +479,16 @@ ATQ_DEV int64_t uniform64(int64_t v) {
 // prefix-summed from the sums of the cluster's earlier chunks (slot table,
 // k_fit_reduce, for the clusters longer than SEG_CHUNK).
 // SEQ: every cluster (n_clusters_dev) in input order, one work-group per
-// cluster walking its chunks and carrying the sums itself.
-template <bool SEQ, bool FX>
+// cluster walking its chunks and carrying the sums itself. PRE: the points
+// in angle order and their weights come from k_fit_terms (out, wts).
+template <bool SEQ, bool FX, bool PRE>
 __global__ void __launch_bounds__(SEG_NT)
 k_fit_chunks(const atq_cluster_point *__restrict__ in, atq_cluster_point *__restrict__ out,
              const uint64_t *__restrict__ keys0, const uint64_t *__restrict__ keys1,
              const atq_cluster_extents *__restrict__ ext, const uint32_t *n_clusters_dev,
              uint32_t cap_cl, SegLong lng, uint32_t max_chunks, const uint8_t *__restrict__ grey,
              uint32_t w, uint32_t h, atq_line_fit_point *__restrict__ lf,
-             const int64_t *__restrict__ slot_sums) {
+             const int64_t *__restrict__ slot_sums, const uint16_t *__restrict__ wts) {
     __shared__ int64_t lds[6 * SEG_NT / 64 + 6];
     __shared__ int64_t stage[FIT_STAGE_WORDS];
     const uint32_t ncl = min(*n_clusters_dev, cap_cl);
@@ -430,10 +510,10 @@ k_fit_chunks(const atq_cluster_point *__restrict__ in, atq_cluster_point *__rest
 #pragma unroll
             for (int i = 0; i < SEG_IPT; i++) {
                 const uint32_t j = bb + e0 + i;
-                src[i] = j < n ? (keys ? (uint32_t)keys[j] : j) : 0u;
+                src[i] = !PRE && j < n ? (keys ? (uint32_t)keys[j] : j) : 0u;
             }
-            fit_chunk(in, out, start, bb, min(SEG_CHUNK, n - bb), src, grey, w, h, lf, carry, tot,
-                      lds, stage, FX);
+            fit_chunk<PRE>(in, out, start, bb, min(SEG_CHUNK, n - bb), src, grey, w, h, lf, wts,
+                           carry, tot, lds, stage, FX);
             if (SEQ)
 #pragma unroll
                 for (int q = 0; q < 6; q++) carry[q] += uniform64(tot[q]);
@@ -497,21 +577,33 @@ int angle_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out
               const atq_cluster_extents *ext, const atq_counts *counts, uint32_t cap_cl,
               uint32_t max_cluster, const uint8_t *grey, uint32_t w, uint32_t h,
               atq_line_fit_point *lf, uint64_t *keys0, uint64_t *keys1, int64_t *slot_sums,
-              SegLong lng, uint32_t *cidx) {
+              SegLong lng, uint32_t *cidx, uint16_t *wts) {
     // the sort (every cluster's keys where seg_sorted_keys puts them), then
-    // the fit of every chunk of every cluster
+    // the points in angle order with their weights, then the fit of every
+    // chunk of every cluster
     int r = seg_sort(s, AngleKey{in, cidx}, AngleStore{keys1}, ext, &counts->n_clusters, cap_cl,
                      max_cluster, keys0, keys1, lng);
     if (r) return r;
     const uint32_t max_chunks = seg_max_chunks(max_cluster);
-    if (lf && max_cluster > SEG_CHUNK)
-        k_fit_reduce<FX><<<512, SEG_NT, 0, s>>>(in, keys0, keys1, ext, lng, max_chunks, grey, w,
-                                                h, slot_sums);
+    if (lf && wts) {
+        const uint32_t max_subs = (max_cluster + FIT_SUB - 1) / FIT_SUB;
+        k_fit_terms<FX><<<8192, 256, 0, s>>>(in, keys0, keys1, ext, &counts->n_clusters, cap_cl,
+                                             max_subs ? max_subs : 1, grey, w, h, out, wts);
+        if (max_cluster > SEG_CHUNK)
+            k_fit_reduce<FX><<<512, SEG_NT, 0, s>>>(out, wts, ext, lng, max_chunks, slot_sums);
+    }
     // one work-group per chunk slot at most a few rounds deep: the dispatcher
     // hands the chunks that exist to whichever CU is free
-    k_fit_chunks<false, FX><<<2048, SEG_NT, 0, s>>>(in, out, keys0, keys1, ext,
-                                                   &counts->n_clusters, cap_cl, lng, max_chunks,
-                                                   grey, w, h, lf, slot_sums);
+    if (lf)
+        k_fit_chunks<false, FX, true><<<2048, SEG_NT, 0, s>>>(in, out, keys0, keys1, ext,
+                                                             &counts->n_clusters, cap_cl, lng,
+                                                             max_chunks, grey, w, h, lf, slot_sums,
+                                                             wts);
+    else
+        k_fit_chunks<false, FX, false><<<2048, SEG_NT, 0, s>>>(in, out, keys0, keys1, ext,
+                                                              &counts->n_clusters, cap_cl, lng,
+                                                              max_chunks, grey, w, h, lf,
+                                                              slot_sums, nullptr);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 }  // namespace
@@ -520,19 +612,20 @@ int launch_angle_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_poi
                      const atq_cluster_extents *ext, const atq_counts *counts, uint32_t cap_cl,
                      uint32_t max_cluster, const uint8_t *grey, uint32_t w, uint32_t h,
                      atq_line_fit_point *lf, uint64_t *keys0, uint64_t *keys1, int64_t *slot_sums,
-                     SegLong lng, bool fx, uint32_t *cidx) {
+                     SegLong lng, bool fx, uint32_t *cidx, uint16_t *wts) {
+    if (lf && !wts) return ATQ_ERR_BAD_ARG;
     return fx ? angle_fit<true>(s, in, out, ext, counts, cap_cl, max_cluster, grey, w, h, lf,
-                                keys0, keys1, slot_sums, lng, cidx)
+                                keys0, keys1, slot_sums, lng, cidx, wts)
               : angle_fit<false>(s, in, out, ext, counts, cap_cl, max_cluster, grey, w, h, lf,
-                                 keys0, keys1, slot_sums, lng, cidx);
+                                 keys0, keys1, slot_sums, lng, cidx, wts);
 }
 
 int launch_cluster_fit(hipStream_t s, const atq_cluster_point *in, const atq_cluster_extents *ext,
                        const uint32_t *n_clusters_dev, uint32_t cap_cl, const uint8_t *grey,
                        uint32_t w, uint32_t h, atq_line_fit_point *lf) {
-    k_fit_chunks<true, false><<<1024, SEG_NT, 0, s>>>(in, nullptr, nullptr, nullptr, ext,
+    k_fit_chunks<true, false, false><<<1024, SEG_NT, 0, s>>>(in, nullptr, nullptr, nullptr, ext,
                                                       n_clusters_dev, cap_cl, SegLong{}, 1, grey,
-                                                      w, h, lf, nullptr);
+                                                      w, h, lf, nullptr, nullptr);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 

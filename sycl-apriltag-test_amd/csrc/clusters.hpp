@@ -29,12 +29,14 @@ int launch_assign(hipStream_t s, const atq_boundary_point *pts, const uint32_t *
 // keys0/keys1: two key buffers of cap entries; slot_sums: 6 * (cap_cl + cap
 // / SEG_CHUNK + 2) int64; lng: a long-segment list of seg_long_cap(cap)
 // entries whose count is zero. fx: fixed mode (record decode, weight row).
-// cidx (may be null) receives every point's cluster index.
+// cidx (may be null) receives every point's cluster index. wts: cap
+// entries of scratch for the points' line-fit weights (required with lf).
 int launch_angle_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out,
                      const atq_cluster_extents *ext, const atq_counts *counts, uint32_t cap_cl,
                      uint32_t max_cluster, const uint8_t *grey, uint32_t w, uint32_t h,
                      atq_line_fit_point *lf, uint64_t *keys0, uint64_t *keys1, int64_t *slot_sums,
-                     SegLong lng, bool fx = false, uint32_t *cidx = nullptr);
+                     SegLong lng, bool fx = false, uint32_t *cidx = nullptr,
+                     uint16_t *wts = nullptr);
 // Line-fit prefix sums of points already in angle order (one work-group per
 // cluster).
 int launch_cluster_fit(hipStream_t s, const atq_cluster_point *in, const atq_cluster_extents *ext,

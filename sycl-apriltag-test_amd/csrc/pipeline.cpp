@@ -279,9 +279,11 @@ int run_back(atq_detector *d, hipStream_t s, const uint8_t *gray_in) {
     // a valid cluster holds at most 2 * (2w + 2h) points (valid_blob)
     const uint32_t max_cluster = (uint32_t)std::min<size_t>(c.cap, 4 * (w + h));
     uint64_t *ak = ws.angle_keys();
+    // the line-fit weights use ws.valid, free from the key sort until fit_lines
     TRY(launch_angle_fit(s, ws.cpts_tmp, d->cpts, d->cext, cnt, c.cap_cl, max_cluster, grey,
                          (uint32_t)w, (uint32_t)h, d->lf, ak, ak + c.cap, ws.fit_sums,
-                         ws.long_segs(0, &d->long_n[0], c), fx, d->cidx));
+                         ws.long_segs(0, &d->long_n[0], c), fx, d->cidx,
+                         reinterpret_cast<uint16_t *>(ws.valid)));
     TRY(mark(1 + ATQ_STAGE_ANGLE_FIT));
     TRY(launch_fit_lines(s, d->lf, d->cidx, d->cext, &cnt->n_cluster_points, c.cap, d->sparse,
                          nullptr, reinterpret_cast<float *>(ws.valid), fx));
