@@ -362,9 +362,57 @@ ATQ_DEV void global_add2(const PairTable &t, const uint64_t (&key)[2], const Pai
 
 // Walks a thread's records (4 pixels, planes in order) with a one-entry
 // cache, so runs of one pair reach the LDS table once with their reduction.
+// Compat records all decode as RIGHT half-pixels (dx = 1, dy = 0): the
+// doubled coordinates of the thread's records are xx = 2x + 1 and yy = 2y
+// with one y and x non-decreasing in walking order, so a run's x bounds are
+// its first and last record's, its y bounds yy, its gradient sums (gx, 0)
+// and its moment sum the signed sum of xx (|sum| <= 16 * 2^15, int32);
+// only the count, the signed gradient and moment sums and the last x are
+// updated per record. Fixed-mode records decode every plane's half-pixel
+// and take the general reduction (PairPart::add).
 template <typename F>
 ATQ_DEV void for_each_pair_run(const PixelRecords (&r)[BND_PX_PER_THREAD], F f, bool fx) {
     uint64_t cur = 0;
+    if (!fx) {
+        uint32_t c = 0, x_first = 0, x_last = 0, yy = 0;
+        int32_t gs = 0, ms = 0;
+        auto flush = [&]() {
+            PairPart q;
+            q.c = c;
+            q.nxmin = 0xFFFFu - x_first;
+            q.xmax = x_last;
+            q.nymin = 0xFFFFu - yy;
+            q.ymax = yy;
+            q.gx = gs;
+            q.gy = 0;
+            q.pg = (int64_t)ms;
+            f(cur, q);
+        };
+#pragma unroll
+        for (int k = 0; k < BND_PX_PER_THREAD; k++)
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                if (!((r[k].mask >> p) & 1u)) continue;
+                const atq_boundary_point pt = r[k].p[p];
+                const uint32_t xx = 2u * (pt.packed_x & COORD_VALUE_MASK) + 1u;
+                if (c == 0 || r[k].k[p] != cur) {  // key 0 is a real pair
+                    if (c) flush();
+                    cur = r[k].k[p];
+                    c = 0;
+                    gs = 0;
+                    ms = 0;
+                    x_first = xx;
+                    yy = 2u * (pt.packed_y & COORD_VALUE_MASK);
+                }
+                const bool b2w = black_to_white(pt.packed_y);
+                c++;
+                gs += b2w ? 1 : -1;
+                ms += b2w ? (int32_t)xx : -(int32_t)xx;
+                x_last = xx;
+            }
+        if (c) flush();
+        return;
+    }
     PairPart q;
     q.init();
 #pragma unroll
