@@ -1030,7 +1030,7 @@ int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const
     const uint32_t max_count = (uint32_t)(2 * (2 * w + 2 * h));
     const uint32_t slots = table.mask + 1;
     if (!sc.flags || atq_div_up(slots, PV_TILE) + 1 > sc.flag_count ||
-        4 * tiles / LB_TILE + 1 > sc.flag_count ||
+        4 * tiles / (LB_TILE / 2) + 1 > sc.flag_count ||
         slots < 256 || rk.cap == 0 || rk.cap % 64 || !rk.vtab || (rk.vmask & (rk.vmask + 1)) ||
         rk.vmask + 1 < 2 * rk.cap)
         return ATQ_ERR_BAD_ARG;

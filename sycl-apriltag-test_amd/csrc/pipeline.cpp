@@ -107,9 +107,11 @@ int take_workspace(Workspace &ws, void *workspace, size_t bytes, size_t w, size_
 }
 
 // Look-back flags per scan array: enough tiles for the longest scan (at
-// most cap elements) and for the pair table's slots (512 per tile).
+// most cap elements; scans of at most LB_SMALL_CAP elements take tiles of
+// LB_TILE / 2) and for the pair table's slots (512 per tile).
 uint32_t scan_flag_count(const Caps &c) {
-    return std::max(c.cap / LB_TILE, c.pair_slots / 512) + 2;
+    return std::max({c.cap / LB_TILE, std::min(c.cap, LB_SMALL_CAP) / (LB_TILE / 2),
+                     c.pair_slots / 512}) + 2;
 }
 
 // Valid cluster pairs the detector ranks (more: the full-key sort runs).
