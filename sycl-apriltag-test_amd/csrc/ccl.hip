@@ -323,9 +323,12 @@ ATQ_DEV void ccl_local_tile(LocalLds<TBY> &L, const FrontFrame &F, uint32_t tile
     // probes (2 bits each: 0 not tested or 127, 1 white, 2 black), runs;
     // the left neighbour's column x-1 (rows y-1, y, y+1) and the right
     // neighbour's column x+2 (row y-1)
-    const uint32_t lpk = __shfl_up(ya | (yt << 16), 1, 32);
-    const uint32_t lb = __shfl_up(yb, 1, 32);
-    const uint32_t rpk = __shfl_down(ya, 1, 32);
+    // (lane moves across the whole wave: the edge lanes bx == 0 and
+    // bx == TBX - 1, whose neighbours lie in the next block row or tile, use
+    // their own halo loads instead)
+    const uint32_t lpk = lane_below(ya | (yt << 16));
+    const uint32_t lb = lane_below(yb);
+    const uint32_t rpk = lane_above(ya);
     const uint32_t pP = bx ? (lpk >> 8) & 0xFFu : el0;
     const uint32_t pS0 = bx ? lpk >> 24 : el1;
     const uint32_t pS1 = bx ? (lb >> 8) & 0xFFu : el2;
@@ -406,7 +409,7 @@ ATQ_DEV void ccl_local_tile(LocalLds<TBY> &L, const FrontFrame &F, uint32_t tile
             a = head[cd - 1];
             b = L.lp[2 * ((by - 1) * TBX + nbx) + cd - 1];
         }
-        const uint32_t pa = __shfl_up(a, 1, 64), pb = __shfl_up(b, 1, 64);
+        const uint32_t pa = lane_below(a), pb = lane_below(b);
         bool dup = (lane & 31) > 0 && pa == a && pb == b;
 #pragma unroll
         for (int q = 0; q < k; q++) dup = dup || (((vm >> q) & 1u) && A[q] == a && B[q] == b);
@@ -946,7 +949,7 @@ ATQ_DEV void fx_local_tile(FxLds &L, const FrontFrame &F, uint32_t tile) {
                               fx_px(L, r - 1, c) == v;
             const uint32_t ra = link ? me : 0xFFFFFFFFu;
             const uint32_t rb = link ? L.lp[(r - 1) * FX_TX + c] : 0xFFFFFFFFu;
-            const uint32_t pa = __shfl_up(ra, 1, 64), pb = __shfl_up(rb, 1, 64);
+            const uint32_t pa = lane_below(ra), pb = lane_below(rb);
             const bool dup = lane > 0 && pa == ra && pb == rb;
             if (link && !dup) lds_union(L.lp, ra, rb);
         }

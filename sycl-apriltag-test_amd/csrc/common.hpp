@@ -85,6 +85,12 @@ ATQ_DEV uint64_t dpp_move(uint64_t x) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// The value of the lane one below (wave_shr:1, 0x138) or one above
+// (wave_shl:1, 0x130) across the whole wave, crossing the rows; lane 0
+// (below) and lane 63 (above) read 0. Every lane of the wave active.
+ATQ_DEV uint32_t lane_below(uint32_t x) { return dpp_move<0x138, 0xF, true>(x); }
+ATQ_DEV uint32_t lane_above(uint32_t x) { return dpp_move<0x130, 0xF, true>(x); }
+
 // Inclusive sum over the 64 lanes of a wave in lane order (wrapping
 // arithmetic; 32- or 64-bit integers), every lane active.
 template <typename T>
