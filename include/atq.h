@@ -529,7 +529,7 @@ int atq_detector_set_timing(atq_detector *det, int enable);
 /* No reference counterpart (the reference fixes its work-group shapes).
  * Threshold regions of threshold_tiles_x x threshold_tiles_y 4x4-pixel tiles
  * (64x16, 32x32 or 16x64) and CCL tiles of 32 x ccl_block_rows 2x2-pixel
- * blocks (8 or 16). Results do not depend on the shapes; the default is
+ * blocks (8, 16 or 32). Results do not depend on the shapes; the default is
  * 64x16 and 16. */
 typedef struct atq_shapes {
     int32_t threshold_tiles_x, threshold_tiles_y, ccl_block_rows;

@@ -536,8 +536,9 @@ k_ccl_local(FrontTable tab) {
     }
 }
 
-// One wave per tile; lane k < 62 takes border block k (top row, then the
-// left and right columns) and the edges that leave the tile from it, read
+// One wave per tile; lane k takes border block k (top row, then the left
+// and right columns; and block k + 64 of tiles with more than 64 border
+// blocks) and the edges that leave the tile from it, read
 // from the border entries the local pass wrote: its own entry (local roots
 // and the codes of its probes that leave the tile) and the entries of the
 // blocks those probes reach (their local roots), one round of independent
@@ -559,75 +560,78 @@ template <int TBY>
 ATQ_DEV void ccl_border_tile(uint64_t *hs, uint64_t *el, uint32_t *en, const FrontFrame &F,
                              uint32_t tile, uint32_t tiles_x, uint32_t n_tiles) {
     constexpr int ITEMS = border_items<TBY>();
-    static_assert(ITEMS <= 64, "one border block per lane");
+    static_assert(ITEMS <= 128, "at most two border blocks per lane");
     constexpr uint32_t NONE = 0xFFFFFFFFu;
     constexpr uint64_t EMPTY = ~0ull;
     const uint32_t w = F.w, h = F.h;
     uint32_t *__restrict__ parent = F.parent;
     const BorderMaps M = border_maps<TBY>(F);
-    const uint32_t k = threadIdx.x & 63;
-    bool live = tile < n_tiles && k < (uint32_t)ITEMS;
-    int bx = 0, by = 0, x = 0, y = 0;
+    const uint32_t lane = threadIdx.x & 63;
     const uint32_t tr = tile / tiles_x, tc = tile % tiles_x;
-    if (live) {
-        if (k < (uint32_t)TBX) { bx = k; by = 0; }
-        else if (k < (uint32_t)(TBX + TBY - 1)) { bx = 0; by = 1 + (k - TBX); }
-        else { bx = TBX - 1; by = 1 + (k - TBX - (TBY - 1)); }
-        x = (int)tc * 2 * TBX + 2 * bx;
-        y = (int)tr * 2 * TBY + 2 * by;
-        live = x < (int)w && y < (int)h;
-    }
-    const uint32_t bxg = (uint32_t)x / 2, byg = (uint32_t)y / 2;
-    uint2 own = make_uint2(0u, 0u), nb[6];
-    bool cross[6];
-    if (live) own = by == 0 ? *M.row(tr, 0, bxg) : *M.col(tc, bx == 0 ? 0 : 1, byg);
 #pragma unroll
-    for (int i = 0; i < 6; i++) {
-        const Probe q = PROBES[i];
-        const int nbx = bx + q.dbx, nby = by + q.dby;
-        const int ny = y + 2 * q.dby, nx = x + 2 * q.dbx;
-        cross[i] = live && !(nbx >= 0 && nbx < TBX && nby >= 0) && ny >= 0 && nx >= 0 && nx < (int)w;
-        nb[i] = make_uint2(0u, 0u);
-        if (cross[i]) {
-            // the row above: the bottom row of the tile row above; else the
-            // facing column of the left or right tile
-            if (by == 0 && q.dby < 0) nb[i] = *M.row(tr - 1, 1, (uint32_t)(nx / 2));
-            else nb[i] = *M.col(q.dbx < 0 ? tc - 1 : tc + 1, q.dbx < 0 ? 1 : 0, (uint32_t)(ny / 2));
+    for (uint32_t k = lane; k < (uint32_t)((ITEMS + 63) / 64 * 64); k += 64) {
+        bool live = tile < n_tiles && k < (uint32_t)ITEMS;
+        int bx = 0, by = 0, x = 0, y = 0;
+        if (live) {
+            if (k < (uint32_t)TBX) { bx = k; by = 0; }
+            else if (k < (uint32_t)(TBX + TBY - 1)) { bx = 0; by = 1 + (k - TBX); }
+            else { bx = TBX - 1; by = 1 + (k - TBX - (TBY - 1)); }
+            x = (int)tc * 2 * TBX + 2 * bx;
+            y = (int)tr * 2 * TBY + 2 * by;
+            live = x < (int)w && y < (int)h;
         }
-    }
-    const uint32_t codes = (own.x >> NODE_BITS) | ((own.y >> NODE_BITS) << 6);
-    uint32_t ea[6], eb[6];
+        const uint32_t bxg = (uint32_t)x / 2, byg = (uint32_t)y / 2;
+        uint2 own = make_uint2(0u, 0u), nb[6];
+        bool cross[6];
+        if (live) own = by == 0 ? *M.row(tr, 0, bxg) : *M.col(tc, bx == 0 ? 0 : 1, byg);
 #pragma unroll
-    for (int i = 0; i < 6; i++) {
-        ea[i] = NONE;
-        eb[i] = NONE;
-        const uint32_t cd = (codes >> (2 * i)) & 3u;  // 1 white, 2 black
-        if (cross[i] && cd) {
-            const uint32_t a = (cd == 1 ? own.x : own.y) & NODE_MASK;
-            const uint32_t b = (cd == 1 ? nb[i].x : nb[i].y) & NODE_MASK;
-            ea[i] = min(a, b);
-            eb[i] = max(a, b);
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-        if (ea[i] == NONE || ea[i] == eb[i]) continue;
-        const uint64_t key = ((uint64_t)ea[i] << 32) | eb[i];
-        uint32_t hsl = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 32) & (BORDER_SLOTS - 1);
-        bool mine = true;  // the set is full: merge it here
-        for (int p = 0; p < BORDER_SLOTS; p++, hsl = (hsl + 1) & (BORDER_SLOTS - 1)) {
-            const uint64_t old = atomicCAS((unsigned long long *)&hs[hsl], (unsigned long long)EMPTY,
-                                           (unsigned long long)key);
-            if (old == EMPTY) break;
-            if (old == key) {
-                mine = false;
-                break;
+        for (int i = 0; i < 6; i++) {
+            const Probe q = PROBES[i];
+            const int nbx = bx + q.dbx, nby = by + q.dby;
+            const int ny = y + 2 * q.dby, nx = x + 2 * q.dbx;
+            cross[i] = live && !(nbx >= 0 && nbx < TBX && nby >= 0) && ny >= 0 && nx >= 0 && nx < (int)w;
+            nb[i] = make_uint2(0u, 0u);
+            if (cross[i]) {
+                // the row above: the bottom row of the tile row above; else the
+                // facing column of the left or right tile
+                if (by == 0 && q.dby < 0) nb[i] = *M.row(tr - 1, 1, (uint32_t)(nx / 2));
+                else nb[i] = *M.col(q.dbx < 0 ? tc - 1 : tc + 1, q.dbx < 0 ? 1 : 0, (uint32_t)(ny / 2));
             }
         }
-        if (mine) {
-            const uint32_t at = atomicAdd(en, 1u);
-            if (at < (uint32_t)BORDER_LIST) el[at] = key;
-            else g_union(parent, ea[i], eb[i]);
+        const uint32_t codes = (own.x >> NODE_BITS) | ((own.y >> NODE_BITS) << 6);
+        uint32_t ea[6], eb[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            ea[i] = NONE;
+            eb[i] = NONE;
+            const uint32_t cd = (codes >> (2 * i)) & 3u;  // 1 white, 2 black
+            if (cross[i] && cd) {
+                const uint32_t a = (cd == 1 ? own.x : own.y) & NODE_MASK;
+                const uint32_t b = (cd == 1 ? nb[i].x : nb[i].y) & NODE_MASK;
+                ea[i] = min(a, b);
+                eb[i] = max(a, b);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            if (ea[i] == NONE || ea[i] == eb[i]) continue;
+            const uint64_t key = ((uint64_t)ea[i] << 32) | eb[i];
+            uint32_t hsl = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 32) & (BORDER_SLOTS - 1);
+            bool mine = true;  // the set is full: merge it here
+            for (int p = 0; p < BORDER_SLOTS; p++, hsl = (hsl + 1) & (BORDER_SLOTS - 1)) {
+                const uint64_t old = atomicCAS((unsigned long long *)&hs[hsl], (unsigned long long)EMPTY,
+                                               (unsigned long long)key);
+                if (old == EMPTY) break;
+                if (old == key) {
+                    mine = false;
+                    break;
+                }
+            }
+            if (mine) {
+                const uint32_t at = atomicAdd(en, 1u);
+                if (at < (uint32_t)BORDER_LIST) el[at] = key;
+                else g_union(parent, ea[i], eb[i]);
+            }
         }
     }
     // the wave's LDS operations complete in order: every listed edge is in
@@ -636,7 +640,7 @@ ATQ_DEV void ccl_border_tile(uint64_t *hs, uint64_t *el, uint32_t *en, const Fro
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const uint32_t n = min(*en, (uint32_t)BORDER_LIST);
-    for (uint32_t j = k; j < n; j += 64) {
+    for (uint32_t j = lane; j < n; j += 64) {
         const uint64_t e = el[j];
         g_union(parent, (uint32_t)(e >> 32), (uint32_t)e);
     }
@@ -849,6 +853,7 @@ int launch_ccl_frames(hipStream_t s, const FrontFrame *frames, int n, const Fron
     for (int i = 0; i < n; i++) tab.fr[i] = frames[i];
     tab.n = (uint32_t)n;
     if (sh.ccl_rows == 8) ccl_launch<8>(s, tab);
+    else if (sh.ccl_rows == 32) ccl_launch<32>(s, tab);
     else ccl_launch<16>(s, tab);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }

@@ -395,13 +395,13 @@ int launch_decimate(hipStream_t s, const uint8_t *in, size_t w_in, size_t h_in, 
 bool front_shapes_supported(const FrontShapes &sh) {
     const bool thr = (sh.thr_rx == 64 && sh.thr_ry == 16) || (sh.thr_rx == 32 && sh.thr_ry == 32) ||
                      (sh.thr_rx == 16 && sh.thr_ry == 64);
-    return thr && (sh.ccl_rows == 8 || sh.ccl_rows == 16);
+    return thr && (sh.ccl_rows == 8 || sh.ccl_rows == 16 || sh.ccl_rows == 32);
 }
 
 int front_shape_candidates(FrontShapes *out, int max_out) {
     static const int thr[3][2] = {{64, 16}, {32, 32}, {16, 64}};
     int n = 0;
-    for (int c : {16, 8})
+    for (int c : {16, 8, 32})
         for (const auto &t : thr) {
             if (n < max_out) out[n] = FrontShapes{t[0], t[1], c};
             n++;

@@ -99,7 +99,7 @@ class AtqShapes(C.Structure):
 
 
 # every (threshold_tiles_x, threshold_tiles_y, ccl_block_rows) the library takes
-SHAPES = [(rx, ry, rows) for rows in (16, 8) for rx, ry in ((64, 16), (32, 32), (16, 64))]
+SHAPES = [(rx, ry, rows) for rows in (16, 8, 32) for rx, ry in ((64, 16), (32, 32), (16, 64))]
 
 _lib = None
 
