@@ -312,7 +312,8 @@ ATQ_DEV void global_add(const PairTable &t, uint64_t key, const PairPart &q, uin
 // threadIdx.x + 256, their probes, counts and bounds in lockstep so that the
 // two chains of dependent atomics overlap.
 ATQ_DEV void global_add2(const PairTable &t, const uint64_t (&key)[2], const PairPart (&q)[2],
-                         const bool (&live)[2], uint32_t max_count, atq_counts *counts) {
+                         const bool (&live)[2], uint32_t max_count, atq_counts *counts,
+                         uint32_t (&found)[2]) {
     uint32_t h[2], slot[2] = {NO_SLOT, NO_SLOT};
     bool done[2];
 #pragma unroll
@@ -347,6 +348,7 @@ ATQ_DEV void global_add2(const PairTable &t, const uint64_t (&key)[2], const Pai
     bool add[2];
 #pragma unroll
     for (int j = 0; j < 2; j++) {
+        found[j] = slot[j];
         if (live[j] && slot[j] == NO_SLOT) counts->pair_filter_off = 1u;
         add[j] = slot[j] != NO_SLOT &&
                  __hip_atomic_load(&t.cnt[slot[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= max_count;
@@ -447,15 +449,25 @@ ATQ_DEV void tile_records(const uint32_t *labels, const uint32_t *sizes, uint32_
     for (int k = 0; k < BND_PX_PER_THREAD; k++) quad_pixel_records(q, x0, y, k, 32, r[k], fx);
 }
 
+// Per 4-pixel thread of k_pair_count, its records for the filter (a
+// thread's records nearly always form at most one run of one pair): x =
+// the record slots (bit 4k + p: pixel k, plane p) | their colour bits
+// (packed_y bit 15) << 16; y = the LDS table slot of the thread's run, with
+// SUM_IRREGULAR set when the thread has more than one run or its run missed
+// the LDS table (the filter then derives its records from the labels).
+// The work-group's LDS slot -> pair-table slot map follows at its end.
+constexpr uint32_t SUM_IRREGULAR = 0x80000000u;
+ATQ_DEV uint32_t pc_regions_y(uint32_t h) { return (h + PC_RY - 1) / PC_RY; }
+
 // Each work-group takes a contiguous run of regions (a vertical strip) and
 // merges all their records in its LDS table before touching the global one,
 // so a pair costs one global add per work-group it meets, not per region.
 // FX: fixed mode (record decode and column 0).
 template <bool FX>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))  // 128 VGPRs
 k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes, uint32_t w,
              uint32_t h, uint32_t min_size, PairTable table, uint32_t max_count,
-             atq_counts *counts) {
+             atq_counts *counts, uint2 *__restrict__ sums, uint32_t *__restrict__ wg_map) {
     __shared__ uint64_t lk[TILE_SLOTS];
     __shared__ uint32_t lc[TILE_SLOTS], lb[4][TILE_SLOTS];
     __shared__ unsigned long long lg[TILE_SLOTS], lm[TILE_SLOTS];  // packed gradient, moment
@@ -470,7 +482,7 @@ k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
     // numbered down the columns of regions, so that a work-group's run of
     // regions is a compact vertical strip: a pair then spans few work-groups
     // and reaches the global table from few of them
-    const uint32_t ry_n = (h + PC_RY - 1) / PC_RY;
+    const uint32_t ry_n = pc_regions_y(h);
     const uint32_t regions = (w + PC_RX - 1) / PC_RX * ry_n;
     const uint32_t per = (regions + gridDim.x - 1) / gridDim.x;
     const uint32_t t_end = min(regions, (blockIdx.x + 1) * per);
@@ -502,12 +514,19 @@ k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
             for (int k = 0; k < BND_PX_PER_THREAD; k++)
                 quad_pixel_records(cur, cx0, cy, k, 32, r[k], FX);
         }
+        // the summary's quad index (npx < 2^26) and run state: RUN_NONE, the
+        // first run's LDS slot (NO_SLOT: missed the table), | SUM_IRREGULAR
+        // once a second run follows
+        constexpr uint32_t RUN_NONE = 0xFFFFFFFEu;
+        const uint32_t quad = here ? (cy * w + cx0) / 4 : 0xFFFFFFFFu;
         cur = nxt;
         here = next_here;
         cx0 = nx0;
         cy = ny;
+        uint32_t run = RUN_NONE;
         for_each_pair_run(r, [&](uint64_t key, const PairPart &q) {
             const uint32_t sl = tile_slot(lk, key, true);
+            run = run == RUN_NONE ? sl : run | SUM_IRREGULAR;
             if (sl == NO_SLOT) {
                 global_add(table, key, q, max_count, counts);
                 return;
@@ -520,6 +539,18 @@ k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
             atomicAdd(&lg[sl], grad_pack(q.gx, q.gy));
             atomicAdd(&lm[sl], (unsigned long long)q.pg);
         }, FX);
+        if (quad != 0xFFFFFFFFu) {
+            uint32_t rc = 0;
+#pragma unroll
+            for (int k = 0; k < BND_PX_PER_THREAD; k++) {
+                rc |= r[k].mask << (4 * k);
+#pragma unroll
+                for (int p = 0; p < 4; p++)
+                    if (((r[k].mask >> p) & 1u) && (r[k].p[p].packed_y & COLOR_DIRECTION_MASK))
+                        rc |= 1u << (16 + 4 * k + p);
+            }
+            sums[quad] = make_uint2(rc, run == RUN_NONE ? 0u : run & (SUM_IRREGULAR | (TILE_SLOTS - 1)));
+        }
     }
     __syncthreads();
     static_assert(TILE_SLOTS == 2 * 256, "two table slots per thread");
@@ -539,7 +570,11 @@ k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
         grad_unpack(lg[i], &q[j].gx, &q[j].gy);
         q[j].pg = (int64_t)lm[i];
     }
-    global_add2(table, key, q, live, max_count, counts);
+    uint32_t found[2];
+    global_add2(table, key, q, live, max_count, counts, found);
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+        wg_map[(size_t)blockIdx.x * TILE_SLOTS + threadIdx.x + 256 * j] = live[j] ? found[j] : NO_SLOT;
 }
 
 constexpr uint32_t RANK_NONE = 0xFFFFFFFFu;     // slot_rank: the pair fails the filter
@@ -717,13 +752,17 @@ k_pair_rank(PairTable table, const uint64_t *__restrict__ vkeys,
 // plane-0 records first, so a stable sort by r gives the reference's
 // (pair, plane, y, x) order.
 //
-// A thread's records come in runs of one pair (consecutive records with one
-// key); the first record of each run looks its pair's rank up in the ranked
-// pairs' table (one 8-byte load, all of the thread's first probes in
-// flight together; a probe that meets another pair's word walks on), the
-// rest of the run takes it over. With more valid pairs than ranked, or
-// frames too large for the table, the pair table and its slot verdicts
-// answer instead; after a table overflow every record is kept.
+// A thread whose records form at most one run of one pair (k_pair_count's
+// summary) takes its records from the summary: the run's pair-table slot
+// through the counting work-group's slot map, its verdict or rank from
+// slot_rank, the records rebuilt from the slot and colour masks. Other
+// threads (and every thread after a pair-table overflow) derive their
+// records from the labels: the first record of each run looks its pair's
+// rank up in the ranked pairs' table (one 8-byte probe per run, all of the
+// thread's probes in flight together; a probe that meets another pair's
+// word walks on), or, with more valid pairs than ranked or frames too large
+// for that table, in the pair table and its slot verdicts; after a table
+// overflow every record is kept.
 template <bool FX>
 __global__ void __launch_bounds__(256)
 k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes, uint32_t w,
@@ -731,7 +770,8 @@ k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
              const uint32_t *__restrict__ slot_rank, const uint64_t *__restrict__ vtab,
              uint32_t vmask, const atq_counts *counts, uint32_t n_tiles,
              uint64_t *__restrict__ tile_keys, atq_boundary_point *__restrict__ tile_points,
-             uint32_t *__restrict__ tile_counts) {
+             uint32_t *__restrict__ tile_counts, const uint2 *__restrict__ sums,
+             const uint32_t *__restrict__ wg_map, uint32_t pc_per) {
     const uint32_t tile = xcd_tile(blockIdx.x, n_tiles);
     if (tile >= n_tiles) return;
     __shared__ uint64_t lds8[4];
@@ -739,87 +779,114 @@ k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
     const uint32_t np = counts->sort_passes;
     const bool raster = np == 0 || counts->rank_shift != 0;  // else plane by plane
     const bool ranked = np != 0 && vtab_usable(lb);
+    const size_t n = (size_t)w * h;
+    const size_t p0 = (size_t)tile * BND_TILE + threadIdx.x * BND_PX_PER_THREAD;
+    const bool here = p0 < n;
+    const uint32_t y = here ? (uint32_t)(p0 / w) : 0u, x0 = here ? (uint32_t)(p0 % w) : 0u;
+    const uint2 sm = here && !keep_all ? sums[p0 / 4] : make_uint2(0u, 0u);
+    const bool irr = here && (keep_all || (sm.y & SUM_IRREGULAR) != 0);
+    // a regular thread's run: its rank (or packed key when the full-key
+    // sort runs)
+    const uint32_t rmask = irr ? 0u : sm.x & 0xFFFFu, cmask = sm.x >> 16;
+    uint32_t rank = RANK_NONE;
+    uint64_t rkey = 0;
+    if (rmask) {
+        const uint32_t reg = x0 / PC_RX * pc_regions_y(h) + y / PC_RY;
+        const uint32_t g = wg_map[(size_t)(reg / pc_per) * TILE_SLOTS + (sm.y & (TILE_SLOTS - 1))];
+        if (g != NO_SLOT) {
+            rank = slot_rank[g];
+            if (np == 0 && rank != RANK_NONE) rkey = packed_pair(~table.keys[g], lb);
+        }
+    }
+    const uint32_t kept = rank != RANK_NONE ? rmask : 0u;
+    // keep: bit 4k+p for record (pixel k, plane p); c: the thread's kept
+    // records per plane, 16 bits each
+    uint32_t keep = kept;
+    uint64_t c = 0;
+#pragma unroll
+    for (int p = 0; p < 4; p++) c += (uint64_t)__popc(kept & (0x1111u << p)) << (16 * p);
+    // irregular threads: their records from the labels; r[k].k[p] becomes
+    // each kept record's output key
     PixelRecords r[BND_PX_PER_THREAD];
-    tile_records(labels, sizes, w, h, min_size, tile, r, FX);
-    // every key becomes the packed pair key min << lb | max; run heads
-    uint32_t head = 0;
-    {
-        uint64_t prev = ~0ull;
+    const bool wave_irr = __ballot(irr) != 0;
+    if (wave_irr) {
+#pragma unroll
+        for (int k = 0; k < BND_PX_PER_THREAD; k++) r[k].mask = 0;
+        if (irr) tile_records(labels, sizes, w, h, min_size, tile, r, FX);
+        // every key becomes the packed pair key min << lb | max; run heads
+        uint32_t head = 0;
+        {
+            uint64_t prev = ~0ull;
+#pragma unroll
+            for (int k = 0; k < BND_PX_PER_THREAD; k++)
+#pragma unroll
+                for (int p = 0; p < 4; p++) {
+                    if (!((r[k].mask >> p) & 1u)) continue;
+                    const uint64_t pk = packed_pair(r[k].k[p], lb);
+                    r[k].k[p] = pk;
+                    if (pk != prev) head |= 1u << (4 * k + p);
+                    prev = pk;
+                }
+        }
+        uint32_t rk[4 * BND_PX_PER_THREAD];
+        if (keep_all) {
+#pragma unroll
+            for (int i = 0; i < 4 * BND_PX_PER_THREAD; i++) rk[i] = RANK_PENDING;
+        } else if (ranked) {
+            const uint64_t kmask = (1ull << (2 * lb)) - 1;
+            uint32_t hh[4 * BND_PX_PER_THREAD];
+            uint64_t v[4 * BND_PX_PER_THREAD];
+#pragma unroll
+            for (int i = 0; i < 4 * BND_PX_PER_THREAD; i++) {
+                hh[i] = 0;
+                v[i] = VTAB_EMPTY;
+                if ((head >> i) & 1u) {
+                    hh[i] = vtab_hash(r[i / 4].k[i % 4], __popc(vmask));
+                    v[i] = vtab[hh[i]];
+                }
+            }
+            uint32_t cur = RANK_NONE;
+#pragma unroll
+            for (int i = 0; i < 4 * BND_PX_PER_THREAD; i++) {
+                if ((head >> i) & 1u) {
+                    const uint64_t pk = r[i / 4].k[i % 4];
+                    uint64_t x = v[i];
+                    uint32_t at = hh[i];
+                    while (x != VTAB_EMPTY && (x & kmask) != pk) {
+                        at = (at + 1) & vmask;
+                        x = vtab[at];
+                    }
+                    cur = x == VTAB_EMPTY ? RANK_NONE : (uint32_t)(x >> (2 * lb));
+                }
+                rk[i] = cur;
+            }
+        } else {
+            // valid pairs beyond the rank list (or frames too large for the
+            // lookup table): the pair table's slot verdicts
+            uint32_t cur = RANK_NONE;
+#pragma unroll
+            for (int i = 0; i < 4 * BND_PX_PER_THREAD; i++) {
+                if ((head >> i) & 1u) {
+                    const uint64_t pk = r[i / 4].k[i % 4];
+                    const uint64_t key = ((pk >> lb) << 32) | (pk & ((1ull << lb) - 1));
+                    const uint32_t g = global_slot(table, key, false);
+                    cur = g == NO_SLOT ? RANK_NONE : slot_rank[g];
+                }
+                rk[i] = cur;
+            }
+        }
 #pragma unroll
         for (int k = 0; k < BND_PX_PER_THREAD; k++)
 #pragma unroll
             for (int p = 0; p < 4; p++) {
-                if (!((r[k].mask >> p) & 1u)) continue;
-                const uint64_t pk = packed_pair(r[k].k[p], lb);
-                r[k].k[p] = pk;
-                if (pk != prev) head |= 1u << (4 * k + p);
-                prev = pk;
+                const uint32_t rr = rk[4 * k + p];
+                if (!((r[k].mask >> p) & 1u) || rr == RANK_NONE) continue;
+                keep |= 1u << (4 * k + p);
+                c += 1ull << (16 * p);
+                r[k].k[p] = np ? (raster ? ((uint64_t)rr << 2) | (uint64_t)p : (uint64_t)rr)
+                               : (r[k].k[p] << 2) | (uint64_t)p;
             }
     }
-    // rank of every record (RANK_NONE: dropped)
-    uint32_t rank[4 * BND_PX_PER_THREAD];
-    if (keep_all) {
-#pragma unroll
-        for (int i = 0; i < 4 * BND_PX_PER_THREAD; i++) rank[i] = RANK_PENDING;
-    } else if (ranked) {
-        const uint32_t vbits = __popc(vmask);
-        const uint64_t kmask = (1ull << (2 * lb)) - 1;
-        uint32_t hh[4 * BND_PX_PER_THREAD];
-        uint64_t v[4 * BND_PX_PER_THREAD];
-#pragma unroll
-        for (int i = 0; i < 4 * BND_PX_PER_THREAD; i++) {
-            hh[i] = 0;
-            v[i] = VTAB_EMPTY;
-            if ((head >> i) & 1u) {
-                hh[i] = vtab_hash(r[i / 4].k[i % 4], vbits);
-                v[i] = vtab[hh[i]];
-            }
-        }
-        uint32_t cur = RANK_NONE;
-#pragma unroll
-        for (int i = 0; i < 4 * BND_PX_PER_THREAD; i++) {
-            if ((head >> i) & 1u) {
-                const uint64_t pk = r[i / 4].k[i % 4];
-                uint64_t x = v[i];
-                uint32_t at = hh[i];
-                while (x != VTAB_EMPTY && (x & kmask) != pk) {
-                    at = (at + 1) & vmask;
-                    x = vtab[at];
-                }
-                cur = x == VTAB_EMPTY ? RANK_NONE : (uint32_t)(x >> (2 * lb));
-            }
-            rank[i] = cur;
-        }
-    } else {
-        // valid pairs beyond the rank list (or frames too large for the
-        // lookup table): the pair table's slot verdicts
-        uint32_t cur = RANK_NONE;
-#pragma unroll
-        for (int i = 0; i < 4 * BND_PX_PER_THREAD; i++) {
-            if ((head >> i) & 1u) {
-                const uint64_t pk = r[i / 4].k[i % 4];
-                const uint64_t key = ((pk >> lb) << 32) | (pk & ((1ull << lb) - 1));
-                const uint32_t g = global_slot(table, key, false);
-                cur = g == NO_SLOT ? RANK_NONE : slot_rank[g];
-            }
-            rank[i] = cur;
-        }
-    }
-    // keep: bit 4k+p for record (pixel k, plane p); r[k].k[p] becomes its
-    // output key; c: the thread's kept records per plane, 16 bits each
-    uint32_t keep = 0;
-    uint64_t c = 0;
-#pragma unroll
-    for (int k = 0; k < BND_PX_PER_THREAD; k++)
-#pragma unroll
-        for (int p = 0; p < 4; p++) {
-            const uint32_t rk = rank[4 * k + p];
-            if (!((r[k].mask >> p) & 1u) || rk == RANK_NONE) continue;
-            keep |= 1u << (4 * k + p);
-            c += 1ull << (16 * p);
-            r[k].k[p] = np ? (raster ? ((uint64_t)rk << 2) | (uint64_t)p : (uint64_t)rk)
-                           : (r[k].k[p] << 2) | (uint64_t)p;
-        }
     if (raster) c = (c & 0xFFFFu) + ((c >> 16) & 0xFFFFu) + ((c >> 32) & 0xFFFFu) + (c >> 48);
     // the tile's records densely to its region, plane by plane, each plane
     // in raster order
@@ -833,15 +900,38 @@ k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
         pos[p] = plane_at + ((uint32_t)(pre >> (16 * p)) & 0xFFFFu);
         plane_at += (uint32_t)(tot >> (16 * p)) & 0xFFFFu;
     }
+    if (wave_irr && irr) {
 #pragma unroll
-    for (int k = 0; k < BND_PX_PER_THREAD; k++)
+        for (int k = 0; k < BND_PX_PER_THREAD; k++)
 #pragma unroll
-        for (int p = 0; p < 4; p++) {
-            if (!((keep >> (4 * k + p)) & 1u)) continue;
-            const size_t at = base + (raster ? pos[0]++ : pos[p]++);
-            tile_keys[at] = r[k].k[p];
-            tile_points[at] = r[k].p[p];
+            for (int p = 0; p < 4; p++) {
+                if (!((keep >> (4 * k + p)) & 1u)) continue;
+                const size_t at = base + (raster ? pos[0]++ : pos[p]++);
+                tile_keys[at] = r[k].k[p];
+                tile_points[at] = r[k].p[p];
+            }
+    }
+    // a regular thread's kept records, in slot order (raster: one running
+    // position; plane by plane: the plane's position plus the thread's
+    // earlier records of that plane)
+    uint32_t m = kept, j = 0;
+    while (m) {
+        const uint32_t i = (uint32_t)__builtin_ctz(m);
+        m &= m - 1u;
+        const uint32_t k = i >> 2, pl = i & 3u;
+        uint32_t at;
+        if (raster) {
+            at = pos[0] + j;
+        } else {
+            const uint32_t pb = pl == 0 ? pos[0] : pl == 1 ? pos[1] : pl == 2 ? pos[2] : pos[3];
+            at = pb + (uint32_t)__popc(kept & (0x1111u << pl) & ((1u << i) - 1u));
         }
+        j++;
+        tile_keys[base + at] = np ? (raster ? ((uint64_t)rank << 2) | pl : (uint64_t)rank) : (rkey << 2) | pl;
+        tile_points[base + at] = atq_boundary_point{
+            (uint16_t)((x0 + k) | (pl << 14)),
+            (uint16_t)(((cmask >> i) & 1u) ? (COLOR_DIRECTION_MASK | y) : y)};
+    }
     if (threadIdx.x < 4) tile_counts[threadIdx.x * n_tiles + tile] = (uint32_t)(tot >> (16 * threadIdx.x)) & 0xFFFFu;
 }
 
@@ -1014,6 +1104,21 @@ k_bnd_gather(const uint64_t *__restrict__ tile_keys, const atq_boundary_point *_
         gather_tiles<4>(tile_keys, tile_points, tile_counts, tile_off, n_tiles, counts, keys, points, L);
 }
 
+namespace {
+// k_pair_count's grid: a few regions per work-group (merged in LDS before
+// the global table), at most ~12 of them at large frames; non-decreasing in
+// the frame's dims.
+unsigned pair_count_grid(size_t w, size_t h) {
+    const unsigned regions = atq_div_up(w, PC_RX) * atq_div_up(h, PC_RY);
+    return std::min(regions, std::max(512u, regions / 12));
+}
+}  // namespace
+
+size_t bnd_summary_entries(size_t max_w, size_t max_h) { return max_w * max_h / 4 + 1; }
+size_t bnd_map_words(size_t max_w, size_t max_h) {
+    return (size_t)TILE_SLOTS * pair_count_grid(max_w, max_h);
+}
+
 int launch_find_boundaries_sparse(hipStream_t s, const uint32_t *labels, const uint32_t *sizes,
                                   size_t w, size_t h, atq_boundary_point *points, uint64_t *keys,
                                   int min_size) {
@@ -1032,18 +1137,19 @@ int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const
     if (!sc.flags || atq_div_up(slots, PV_TILE) + 1 > sc.flag_count ||
         4 * tiles / (LB_TILE / 2) + 1 > sc.flag_count ||
         slots < 256 || rk.cap == 0 || rk.cap % 64 || !rk.vtab || (rk.vmask & (rk.vmask + 1)) ||
-        rk.vmask + 1 < 2 * rk.cap)
+        rk.vmask + 1 < 2 * rk.cap || !rk.sums || !rk.wg_map || (w & 3))
         return ATQ_ERR_BAD_ARG;
-    // a few tiles per work-group (merged in LDS before the global table), at
-    // most ~12 of them at large frames
     const unsigned regions = atq_div_up(w, PC_RX) * atq_div_up(h, PC_RY);
-    const unsigned pg = min(regions, std::max(512u, regions / 12));
+    const unsigned pg = pair_count_grid(w, h);
+    const uint32_t pc_per = atq_div_up(regions, pg);
     if (fx)
         k_pair_count<true><<<pg, 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h,
-                                              (uint32_t)min_size, table, max_count, counts);
+                                              (uint32_t)min_size, table, max_count, counts,
+                                              rk.sums, rk.wg_map);
     else
         k_pair_count<false><<<pg, 256, 0, s>>>(labels, sizes, (uint32_t)w, (uint32_t)h,
-                                               (uint32_t)min_size, table, max_count, counts);
+                                               (uint32_t)min_size, table, max_count, counts,
+                                               rk.sums, rk.wg_map);
     {
         uint64_t *cur = sc.flags + (size_t)(sc.calls & 1) * sc.flag_count;
         uint64_t *next = sc.flags + (size_t)((sc.calls + 1) & 1) * sc.flag_count;
@@ -1060,11 +1166,13 @@ int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const
     if (fx)
         k_bnd_filter<true><<<xcd_grid(tiles), 256, 0, s>>>(
             labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size, label_bits, table,
-            rk.slot_rank, rk.vtab, rk.vmask, counts, tiles, rk.tile_keys, rk.tile_points, tile_counts);
+            rk.slot_rank, rk.vtab, rk.vmask, counts, tiles, rk.tile_keys, rk.tile_points, tile_counts,
+            rk.sums, rk.wg_map, pc_per);
     else
         k_bnd_filter<false><<<xcd_grid(tiles), 256, 0, s>>>(
             labels, sizes, (uint32_t)w, (uint32_t)h, (uint32_t)min_size, label_bits, table,
-            rk.slot_rank, rk.vtab, rk.vmask, counts, tiles, rk.tile_keys, rk.tile_points, tile_counts);
+            rk.slot_rank, rk.vtab, rk.vmask, counts, tiles, rk.tile_keys, rk.tile_points, tile_counts,
+            rk.sums, rk.wg_map, pc_per);
     uint32_t *tile_off = nullptr;
     if (tiles > GATHER_SUM_TILES) {
         tile_off = tile_counts + 4 * tiles;

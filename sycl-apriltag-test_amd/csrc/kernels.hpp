@@ -106,6 +106,8 @@ struct PairRanks {
     uint32_t cap;
     uint64_t *vtab;
     uint32_t vmask;
+    uint2 *sums;        // per 4-pixel thread of k_pair_count (bnd_summary_entries)
+    uint32_t *wg_map;   // k_pair_count's LDS slot -> pair-table slot (bnd_map_words)
     uint64_t *tile_keys;
     atq_boundary_point *tile_points;
     uint32_t *tile_counts;
@@ -127,6 +129,9 @@ int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const
                                     const PairRanks &rk, const RsDyn &out, ScanCtx &sc,
                                     bool fx = false);
 constexpr unsigned BOUNDARY_TILE_PX = 1024;  // pixels per compaction tile
+// Sizes of PairRanks::sums and wg_map for frames of up to max_w x max_h.
+size_t bnd_summary_entries(size_t max_w, size_t max_h);
+size_t bnd_map_words(size_t max_w, size_t max_h);
 
 // diff.hip
 // decode.hip

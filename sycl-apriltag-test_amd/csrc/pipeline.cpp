@@ -147,6 +147,8 @@ size_t carve_detector(atq_detector *d, char *base) {
     d->centres = carve_one<float2>(base, off, rank_cap(c));
     d->rank_keys = carve_one<uint64_t>(base, off, rank_cap(c));
     d->vtab = carve_one<uint64_t>(base, off, 4 * (size_t)rank_cap(c));
+    d->bnd_sums = carve_one<uint2>(base, off, bnd_summary_entries(d->max_w, d->max_h));
+    d->bnd_map = carve_one<uint32_t>(base, off, bnd_map_words(d->max_w, d->max_h));
     d->scan_flags = carve_one<uint64_t>(base, off, 2 * (size_t)scan_flag_count(c));
     d->pair_keys = carve_one<uint64_t>(base, off, c.pair_slots);
     d->pair_cnt = carve_one<uint32_t>(base, off, c.pair_slots);
@@ -265,7 +267,7 @@ int run_back(atq_detector *d, hipStream_t s, const uint8_t *gray_in) {
     // free until the fallback's cluster stage (12 of its 32 bytes per entry)
     uint64_t *tile_keys = reinterpret_cast<uint64_t *>(ws.seg_bounds);
     const PairRanks rk{d->slot_rank, d->vkeys, d->vslots, d->vcnt, d->rank_keys, rank_cap(d->caps),
-                       d->vtab, 4 * rank_cap(d->caps) - 1, tile_keys, reinterpret_cast<atq_boundary_point *>(tile_keys + c.cap),
+                       d->vtab, 4 * rank_cap(d->caps) - 1, d->bnd_sums, d->bnd_map, tile_keys, reinterpret_cast<atq_boundary_point *>(tile_keys + c.cap),
                        ws.tiles};
     TRY(launch_find_boundaries_filtered(s, d->labels, d->sizes, w, h, d->params.min_component_size,
                                         (uint32_t)plan.label_bits, table, cnt, rk, io, sc, fx));

@@ -82,6 +82,8 @@ struct atq_detector {
     uint32_t *vslots, *vcnt;
     uint64_t *rank_keys;   // rank -> packed pair key
     uint64_t *vtab;        // ranked pairs' lookup table (4 * rank_cap words)
+    uint2 *bnd_sums;       // the pair count's per-thread record summaries
+    uint32_t *bnd_map;     // the pair count's LDS slot -> pair-table slot maps
     float2 *centres;       // rank -> the cluster's (cx, cy)
     uint64_t *scan_flags;  // two look-back flag arrays, zeroed before each frame
     uint64_t *pair_keys;   // pair table, zeroed with counts before each frame
