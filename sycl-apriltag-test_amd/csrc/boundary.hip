@@ -805,87 +805,51 @@ k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
     uint64_t c = 0;
 #pragma unroll
     for (int p = 0; p < 4; p++) c += (uint64_t)__popc(kept & (0x1111u << p)) << (16 * p);
-    // irregular threads: their records from the labels; r[k].k[p] becomes
-    // each kept record's output key
-    PixelRecords r[BND_PX_PER_THREAD];
-    const bool wave_irr = __ballot(irr) != 0;
-    if (wave_irr) {
-#pragma unroll
-        for (int k = 0; k < BND_PX_PER_THREAD; k++) r[k].mask = 0;
-        if (irr) tile_records(labels, sizes, w, h, min_size, tile, r, FX);
-        // every key becomes the packed pair key min << lb | max; run heads
-        uint32_t head = 0;
-        {
-            uint64_t prev = ~0ull;
-#pragma unroll
-            for (int k = 0; k < BND_PX_PER_THREAD; k++)
-#pragma unroll
-                for (int p = 0; p < 4; p++) {
-                    if (!((r[k].mask >> p) & 1u)) continue;
-                    const uint64_t pk = packed_pair(r[k].k[p], lb);
-                    r[k].k[p] = pk;
-                    if (pk != prev) head |= 1u << (4 * k + p);
-                    prev = pk;
-                }
-        }
-        uint32_t rk[4 * BND_PX_PER_THREAD];
-        if (keep_all) {
-#pragma unroll
-            for (int i = 0; i < 4 * BND_PX_PER_THREAD; i++) rk[i] = RANK_PENDING;
-        } else if (ranked) {
+    // irregular threads: their records from the labels, one pixel's at a
+    // time (counted, then after the scan derived again and written), each
+    // run's verdict looked up once (a one-entry cache); few registers, so
+    // that the regular threads' path keeps the occupancy
+    QuadLabels ql;
+    uint64_t ckey = ~0ull;
+    uint32_t crank = RANK_NONE;
+    auto rank_of_record = [&](uint64_t key) -> uint32_t {
+        if (keep_all) return RANK_PENDING;
+        const uint64_t pk = packed_pair(key, lb);
+        if (pk == ckey) return crank;
+        uint32_t rr = RANK_NONE;
+        if (ranked) {
             const uint64_t kmask = (1ull << (2 * lb)) - 1;
-            uint32_t hh[4 * BND_PX_PER_THREAD];
-            uint64_t v[4 * BND_PX_PER_THREAD];
-#pragma unroll
-            for (int i = 0; i < 4 * BND_PX_PER_THREAD; i++) {
-                hh[i] = 0;
-                v[i] = VTAB_EMPTY;
-                if ((head >> i) & 1u) {
-                    hh[i] = vtab_hash(r[i / 4].k[i % 4], __popc(vmask));
-                    v[i] = vtab[hh[i]];
-                }
+            uint32_t at = vtab_hash(pk, __popc(vmask));
+            uint64_t x = vtab[at];
+            while (x != VTAB_EMPTY && (x & kmask) != pk) {
+                at = (at + 1) & vmask;
+                x = vtab[at];
             }
-            uint32_t cur = RANK_NONE;
-#pragma unroll
-            for (int i = 0; i < 4 * BND_PX_PER_THREAD; i++) {
-                if ((head >> i) & 1u) {
-                    const uint64_t pk = r[i / 4].k[i % 4];
-                    uint64_t x = v[i];
-                    uint32_t at = hh[i];
-                    while (x != VTAB_EMPTY && (x & kmask) != pk) {
-                        at = (at + 1) & vmask;
-                        x = vtab[at];
-                    }
-                    cur = x == VTAB_EMPTY ? RANK_NONE : (uint32_t)(x >> (2 * lb));
-                }
-                rk[i] = cur;
-            }
+            rr = x == VTAB_EMPTY ? RANK_NONE : (uint32_t)(x >> (2 * lb));
         } else {
             // valid pairs beyond the rank list (or frames too large for the
             // lookup table): the pair table's slot verdicts
-            uint32_t cur = RANK_NONE;
-#pragma unroll
-            for (int i = 0; i < 4 * BND_PX_PER_THREAD; i++) {
-                if ((head >> i) & 1u) {
-                    const uint64_t pk = r[i / 4].k[i % 4];
-                    const uint64_t key = ((pk >> lb) << 32) | (pk & ((1ull << lb) - 1));
-                    const uint32_t g = global_slot(table, key, false);
-                    cur = g == NO_SLOT ? RANK_NONE : slot_rank[g];
-                }
-                rk[i] = cur;
-            }
+            const uint32_t g = global_slot(table, key, false);
+            rr = g == NO_SLOT ? RANK_NONE : slot_rank[g];
         }
+        ckey = pk;
+        crank = rr;
+        return rr;
+    };
+    const bool wave_irr = __ballot(irr) != 0;
+    if (wave_irr && irr) {
+        load_quad(labels, sizes, w, h, min_size, x0, y, ql);
 #pragma unroll
-        for (int k = 0; k < BND_PX_PER_THREAD; k++)
+        for (int k = 0; k < BND_PX_PER_THREAD; k++) {
+            PixelRecords pr;
+            quad_pixel_records(ql, x0, y, k, 32, pr, FX);
 #pragma unroll
             for (int p = 0; p < 4; p++) {
-                const uint32_t rr = rk[4 * k + p];
-                if (!((r[k].mask >> p) & 1u) || rr == RANK_NONE) continue;
+                if (!((pr.mask >> p) & 1u) || rank_of_record(pr.k[p]) == RANK_NONE) continue;
                 keep |= 1u << (4 * k + p);
                 c += 1ull << (16 * p);
-                r[k].k[p] = np ? (raster ? ((uint64_t)rr << 2) | (uint64_t)p : (uint64_t)rr)
-                               : (r[k].k[p] << 2) | (uint64_t)p;
             }
+        }
     }
     if (raster) c = (c & 0xFFFFu) + ((c >> 16) & 0xFFFFu) + ((c >> 32) & 0xFFFFu) + (c >> 48);
     // the tile's records densely to its region, plane by plane, each plane
@@ -902,14 +866,19 @@ k_bnd_filter(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
     }
     if (wave_irr && irr) {
 #pragma unroll
-        for (int k = 0; k < BND_PX_PER_THREAD; k++)
+        for (int k = 0; k < BND_PX_PER_THREAD; k++) {
+            PixelRecords pr;
+            quad_pixel_records(ql, x0, y, k, 32, pr, FX);
 #pragma unroll
             for (int p = 0; p < 4; p++) {
                 if (!((keep >> (4 * k + p)) & 1u)) continue;
+                const uint32_t rr = rank_of_record(pr.k[p]);
                 const size_t at = base + (raster ? pos[0]++ : pos[p]++);
-                tile_keys[at] = r[k].k[p];
-                tile_points[at] = r[k].p[p];
+                tile_keys[at] = np ? (raster ? ((uint64_t)rr << 2) | (uint64_t)p : (uint64_t)rr)
+                                   : (packed_pair(pr.k[p], lb) << 2) | (uint64_t)p;
+                tile_points[at] = pr.p[p];
             }
+        }
     }
     // a regular thread's kept records, in slot order (raster: one running
     // position; plane by plane: the plane's position plus the thread's
