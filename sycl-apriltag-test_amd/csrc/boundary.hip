@@ -362,8 +362,10 @@ ATQ_DEV void global_add2(const PairTable &t, const uint64_t (&key)[2], const Pai
         if (add[j] && old[j] + q[j].c <= max_count) global_bounds(t, slot[j], q[j]);
 }
 
-// Walks a thread's records (4 pixels, planes in order) with a one-entry
-// cache, so runs of one pair reach the LDS table once with their reduction.
+// Walks a thread's records (4 pixels, planes in order; fed one record at
+// a time) with a one-entry cache, so runs of one pair reach the LDS table
+// once with their reduction: step() calls f(key, reduction) whenever a run
+// ends, finish() for the last run.
 // Compat records all decode as RIGHT half-pixels (dx = 1, dy = 0): the
 // doubled coordinates of the thread's records are xx = 2x + 1 and yy = 2y
 // with one y and x non-decreasing in walking order, so a run's x bounds are
@@ -372,14 +374,15 @@ ATQ_DEV void global_add2(const PairTable &t, const uint64_t (&key)[2], const Pai
 // only the count, the signed gradient and moment sums and the last x are
 // updated per record. Fixed-mode records decode every plane's half-pixel
 // and take the general reduction (PairPart::add).
-template <typename F>
-ATQ_DEV void for_each_pair_run(const PixelRecords (&r)[BND_PX_PER_THREAD], F f, bool fx) {
+struct RunWalk {
     uint64_t cur = 0;
-    if (!fx) {
-        uint32_t c = 0, x_first = 0, x_last = 0, yy = 0;
-        int32_t gs = 0, ms = 0;
-        auto flush = [&]() {
-            PairPart q;
+    uint32_t c = 0, x_first = 0, x_last = 0, yy = 0;
+    int32_t gs = 0, ms = 0;
+    PairPart q;  // fixed mode
+    template <typename F>
+    ATQ_DEV void flush(F &f, bool fx) {
+        if (!c) return;
+        if (!fx) {
             q.c = c;
             q.nxmin = 0xFFFFu - x_first;
             q.xmax = x_last;
@@ -388,49 +391,36 @@ ATQ_DEV void for_each_pair_run(const PixelRecords (&r)[BND_PX_PER_THREAD], F f, 
             q.gx = gs;
             q.gy = 0;
             q.pg = (int64_t)ms;
-            f(cur, q);
-        };
-#pragma unroll
-        for (int k = 0; k < BND_PX_PER_THREAD; k++)
-#pragma unroll
-            for (int p = 0; p < 4; p++) {
-                if (!((r[k].mask >> p) & 1u)) continue;
-                const atq_boundary_point pt = r[k].p[p];
-                const uint32_t xx = 2u * (pt.packed_x & COORD_VALUE_MASK) + 1u;
-                if (c == 0 || r[k].k[p] != cur) {  // key 0 is a real pair
-                    if (c) flush();
-                    cur = r[k].k[p];
-                    c = 0;
-                    gs = 0;
-                    ms = 0;
-                    x_first = xx;
-                    yy = 2u * (pt.packed_y & COORD_VALUE_MASK);
-                }
-                const bool b2w = black_to_white(pt.packed_y);
-                c++;
-                gs += b2w ? 1 : -1;
-                ms += b2w ? (int32_t)xx : -(int32_t)xx;
-                x_last = xx;
-            }
-        if (c) flush();
-        return;
-    }
-    PairPart q;
-    q.init();
-#pragma unroll
-    for (int k = 0; k < BND_PX_PER_THREAD; k++)
-#pragma unroll
-        for (int p = 0; p < 4; p++) {
-            if (!((r[k].mask >> p) & 1u)) continue;
-            if (r[k].k[p] != cur) {
-                if (q.c) f(cur, q);
-                cur = r[k].k[p];
-                q.init();
-            }
-            q.add(r[k].p[p], fx);
         }
-    if (q.c) f(cur, q);
-}
+        f(cur, q);
+    }
+    template <typename F>
+    ATQ_DEV void step(uint64_t key, atq_boundary_point pt, F &f, bool fx) {
+        const uint32_t xx = 2u * (pt.packed_x & COORD_VALUE_MASK) + 1u;
+        if (c == 0 || key != cur) {  // key 0 is a real pair
+            flush(f, fx);
+            cur = key;
+            c = 0;
+            gs = 0;
+            ms = 0;
+            x_first = xx;
+            yy = 2u * (pt.packed_y & COORD_VALUE_MASK);
+            if (fx) q.init();
+        }
+        if (fx) {
+            q.add(pt, true);
+            c++;
+        } else {
+            const bool b2w = black_to_white(pt.packed_y);
+            c++;
+            gs += b2w ? 1 : -1;
+            ms += b2w ? (int32_t)xx : -(int32_t)xx;
+            x_last = xx;
+        }
+    }
+    template <typename F>
+    ATQ_DEV void finish(F &f, bool fx) { flush(f, fx); }
+};
 
 // Records of the 4 pixels of this thread in tile `tile` (w is a multiple
 // of 4, so the 4 pixels share a row).
@@ -464,7 +454,7 @@ ATQ_DEV uint32_t pc_regions_y(uint32_t h) { return (h + PC_RY - 1) / PC_RY; }
 // so a pair costs one global add per work-group it meets, not per region.
 // FX: fixed mode (record decode and column 0).
 template <bool FX>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))  // 128 VGPRs
+__global__ void __launch_bounds__(256)
 k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ sizes, uint32_t w,
              uint32_t h, uint32_t min_size, PairTable table, uint32_t max_count,
              atq_counts *counts, uint2 *__restrict__ sums, uint32_t *__restrict__ wg_map) {
@@ -491,40 +481,21 @@ k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
         y = reg % ry_n * PC_RY + threadIdx.x / (PC_RX / 4);
         return x0 < w && y < h;
     };
-    // software-pipelined over the regions: the next region's labels load
-    // while this region's sizes are in flight
-    QuadLabels cur;
-    const uint32_t t0 = blockIdx.x * per;
-    uint32_t cx0, cy;
-    bool here = t0 < t_end && origin(t0, cx0, cy);
-    if (here) load_labels(labels, w, h, cx0, cy, cur);
-    for (uint32_t reg = t0; reg < t_end; reg++) {
-        uint32_t sz[12];
-        if (here) load_sizes(sizes, cur, sz);
-        QuadLabels nxt;
-        uint32_t nx0 = 0, ny = 0;
-        const bool next_here = reg + 1 < t_end && origin(reg + 1, nx0, ny);
-        if (next_here) load_labels(labels, w, h, nx0, ny, nxt);
-        PixelRecords r[BND_PX_PER_THREAD];
-#pragma unroll
-        for (int k = 0; k < BND_PX_PER_THREAD; k++) r[k].mask = 0;
-        if (here) {
-            size_bits(cur, sz, min_size);
-#pragma unroll
-            for (int k = 0; k < BND_PX_PER_THREAD; k++)
-                quad_pixel_records(cur, cx0, cy, k, 32, r[k], FX);
-        }
-        // the summary's quad index (npx < 2^26) and run state: RUN_NONE, the
-        // first run's LDS slot (NO_SLOT: missed the table), | SUM_IRREGULAR
-        // once a second run follows
+    // a region's records are derived one pixel at a time and walked in
+    // order as pair runs (RunWalk), so that only one pixel's records are
+    // live: few registers, many work-groups per CU to hide the label and
+    // size loads
+    for (uint32_t reg = blockIdx.x * per; reg < t_end; reg++) {
+        uint32_t x0, y;
+        const bool here = origin(reg, x0, y);
+        QuadLabels ql;
+        if (here) load_quad(labels, sizes, w, h, min_size, x0, y, ql);
+        // run state: RUN_NONE, the first run's LDS slot (NO_SLOT: missed the
+        // table), | SUM_IRREGULAR once a second run follows
         constexpr uint32_t RUN_NONE = 0xFFFFFFFEu;
-        const uint32_t quad = here ? (cy * w + cx0) / 4 : 0xFFFFFFFFu;
-        cur = nxt;
-        here = next_here;
-        cx0 = nx0;
-        cy = ny;
-        uint32_t run = RUN_NONE;
-        for_each_pair_run(r, [&](uint64_t key, const PairPart &q) {
+        uint32_t run = RUN_NONE, rc = 0;
+        RunWalk rw;
+        auto flush = [&](uint64_t key, const PairPart &q) {
             const uint32_t sl = tile_slot(lk, key, true);
             run = run == RUN_NONE ? sl : run | SUM_IRREGULAR;
             if (sl == NO_SLOT) {
@@ -538,18 +509,23 @@ k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
             atomicMax(&lb[3][sl], q.ymax);
             atomicAdd(&lg[sl], grad_pack(q.gx, q.gy));
             atomicAdd(&lm[sl], (unsigned long long)q.pg);
-        }, FX);
-        if (quad != 0xFFFFFFFFu) {
-            uint32_t rc = 0;
+        };
+        if (here) {
 #pragma unroll
             for (int k = 0; k < BND_PX_PER_THREAD; k++) {
-                rc |= r[k].mask << (4 * k);
+                PixelRecords pr;
+                quad_pixel_records(ql, x0, y, k, 32, pr, FX);
+                rc |= pr.mask << (4 * k);
 #pragma unroll
-                for (int p = 0; p < 4; p++)
-                    if (((r[k].mask >> p) & 1u) && (r[k].p[p].packed_y & COLOR_DIRECTION_MASK))
-                        rc |= 1u << (16 + 4 * k + p);
+                for (int p = 0; p < 4; p++) {
+                    if (!((pr.mask >> p) & 1u)) continue;
+                    if (pr.p[p].packed_y & COLOR_DIRECTION_MASK) rc |= 1u << (16 + 4 * k + p);
+                    rw.step(pr.k[p], pr.p[p], flush, FX);
+                }
             }
-            sums[quad] = make_uint2(rc, run == RUN_NONE ? 0u : run & (SUM_IRREGULAR | (TILE_SLOTS - 1)));
+            rw.finish(flush, FX);
+            sums[(y * w + x0) / 4] =
+                make_uint2(rc, run == RUN_NONE ? 0u : run & (SUM_IRREGULAR | (TILE_SLOTS - 1)));
         }
     }
     __syncthreads();
@@ -1075,11 +1051,12 @@ k_bnd_gather(const uint64_t *__restrict__ tile_keys, const atq_boundary_point *_
 
 namespace {
 // k_pair_count's grid: a few regions per work-group (merged in LDS before
-// the global table), at most ~12 of them at large frames; non-decreasing in
-// the frame's dims.
+// the global table), at most ~7 of them at large frames, so that the grid
+// is resident at once (7 work-groups of 72 VGPRs and 22 KiB of LDS share a
+// CU); non-decreasing in the frame's dims.
 unsigned pair_count_grid(size_t w, size_t h) {
     const unsigned regions = atq_div_up(w, PC_RX) * atq_div_up(h, PC_RY);
-    return std::min(regions, std::max(512u, regions / 12));
+    return std::min(regions, std::max(512u, regions / 7));
 }
 }  // namespace
 
