@@ -1053,10 +1053,12 @@ namespace {
 // k_pair_count's grid: a few regions per work-group (merged in LDS before
 // the global table), at most ~7 of them at large frames, so that the grid
 // is resident at once (7 work-groups of 72 VGPRs and 22 KiB of LDS share a
-// CU); non-decreasing in the frame's dims.
+// CU), and at least 1024 work-groups (about 2 regions each at 1080p: fewer
+// leave the CUs waiting on each region's loads, more multiply the global
+// table's flushes); non-decreasing in the frame's dims.
 unsigned pair_count_grid(size_t w, size_t h) {
     const unsigned regions = atq_div_up(w, PC_RX) * atq_div_up(h, PC_RY);
-    return std::min(regions, std::max(512u, regions / 7));
+    return std::min(regions, std::max(1024u, regions / 7));
 }
 }  // namespace
 
