@@ -445,7 +445,8 @@ ATQ_DEV void tile_records(const uint32_t *labels, const uint32_t *sizes, uint32_
 // (packed_y bit 15) << 16; y = the LDS table slot of the thread's run, with
 // SUM_IRREGULAR set when the thread has more than one run or its run missed
 // the LDS table (the filter then derives its records from the labels).
-// The work-group's LDS slot -> pair-table slot map follows at its end.
+// The work-group's LDS slot -> pair-table slot map follows at its end, for
+// its occupied slots (the only ones a summary names).
 constexpr uint32_t SUM_IRREGULAR = 0x80000000u;
 ATQ_DEV uint32_t pc_regions_y(uint32_t h) { return (h + PC_RY - 1) / PC_RY; }
 
@@ -548,9 +549,10 @@ k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
     }
     uint32_t found[2];
     global_add2(table, key, q, live, max_count, counts, found);
+    // only occupied slots: a summary names the slot of a run it recorded
 #pragma unroll
     for (int j = 0; j < 2; j++)
-        wg_map[(size_t)blockIdx.x * TILE_SLOTS + threadIdx.x + 256 * j] = live[j] ? found[j] : NO_SLOT;
+        if (live[j]) wg_map[(size_t)blockIdx.x * TILE_SLOTS + threadIdx.x + 256 * j] = found[j];
 }
 
 constexpr uint32_t RANK_NONE = 0xFFFFFFFFu;     // slot_rank: the pair fails the filter
