@@ -218,9 +218,16 @@ struct AngleKey {
 // Prefix records staged per round of FIT_STAGE positions: the owners put
 // their moments in LDS, then consecutive lanes write consecutive 16-byte
 // quarters of the 64-byte records (a thread's own four records lie 256 bytes
-// from its neighbours', which leaves every store instruction scattered).
+// from its neighbours', which leaves every store instruction scattered). A
+// thread's four staged records (24 int64) take FIT_STAGE_THREAD = 25 slots:
+// at a stride of 24 the owners' stores hit the same few LDS banks (48 words
+// apart), at 25 (50 words) they spread over the banks.
 constexpr uint32_t FIT_STAGE = 1024;
-constexpr uint32_t FIT_STAGE_WORDS = 6 * FIT_STAGE;
+constexpr uint32_t FIT_STAGE_THREAD = 6 * SEG_IPT + 1;
+constexpr uint32_t FIT_STAGE_WORDS = FIT_STAGE / SEG_IPT * FIT_STAGE_THREAD;
+ATQ_DEV uint32_t fit_stage_ix(uint32_t pos) {
+    return FIT_STAGE_THREAD * (pos / SEG_IPT) + 6 * (pos % SEG_IPT);
+}
 
 // Loads the points of chunk positions e0 + i < m (src[i]: their positions in
 // the cluster) and their line-fit terms: weight W and doubled coordinates
@@ -378,7 +385,7 @@ ATQ_DEV void fit_chunk(const atq_cluster_point *in, atq_cluster_point *out, uint
     int64_t *lfw = reinterpret_cast<int64_t *>(lf + start + base);
     for (uint32_t r = 0; r * FIT_STAGE < m; r++) {
         if (e0 / FIT_STAGE == r) {
-            int64_t *d = stage + 6 * (e0 - r * FIT_STAGE);
+            int64_t *d = stage + fit_stage_ix(e0 - r * FIT_STAGE);
 #pragma unroll
             for (int i = 0; i < SEG_IPT; i++) {
                 const atq_line_fit_point f = linefit_from_terms(W[i], RX[i], RY[i]);
@@ -400,7 +407,10 @@ ATQ_DEV void fit_chunk(const atq_cluster_point *in, atq_cluster_point *out, uint
             const uint32_t piece = t + SEG_NT * k, rc = piece >> 2, qt = piece & 3;
             if (rc >= nr) continue;
             longlong2 v = {0, 0};
-            if (qt < 3) v = longlong2{stage[6 * rc + 2 * qt], stage[6 * rc + 2 * qt + 1]};
+            if (qt < 3) {
+                const uint32_t sx = fit_stage_ix(rc) + 2 * qt;
+                v = longlong2{stage[sx], stage[sx + 1]};
+            }
             *reinterpret_cast<longlong2 *>(dst + 8 * rc + 2 * qt) = v;
         }
         __syncthreads();
