@@ -379,20 +379,25 @@ struct RunWalk {
     uint32_t c = 0, x_first = 0, x_last = 0, yy = 0;
     int32_t gs = 0, ms = 0;
     PairPart q;  // fixed mode
+    // the open run's reduction (zero, the identity of every field, when
+    // there is none)
+    ATQ_DEV PairPart part(bool fx) const {
+        PairPart r;
+        r.init();
+        if (!c) return r;
+        if (fx) return q;
+        r.c = c;
+        r.nxmin = 0xFFFFu - x_first;
+        r.xmax = x_last;
+        r.nymin = 0xFFFFu - yy;
+        r.ymax = yy;
+        r.gx = gs;
+        r.pg = (int64_t)ms;
+        return r;
+    }
     template <typename F>
     ATQ_DEV void flush(F &f, bool fx) {
-        if (!c) return;
-        if (!fx) {
-            q.c = c;
-            q.nxmin = 0xFFFFu - x_first;
-            q.xmax = x_last;
-            q.nymin = 0xFFFFu - yy;
-            q.ymax = yy;
-            q.gx = gs;
-            q.gy = 0;
-            q.pg = (int64_t)ms;
-        }
-        f(cur, q);
+        if (c) f(cur, part(fx));
     }
     template <typename F>
     ATQ_DEV void step(uint64_t key, atq_boundary_point pt, F &f, bool fx) {
@@ -418,9 +423,22 @@ struct RunWalk {
             x_last = xx;
         }
     }
-    template <typename F>
-    ATQ_DEV void finish(F &f, bool fx) { flush(f, fx); }
 };
+
+// The wave's sum of every lane's part (maxima for the bounds), in scalars.
+// Every lane active.
+ATQ_DEV PairPart wave_part_total(const PairPart &q) {
+    PairPart t;
+    t.c = lane_value(wave_inclusive_sum(q.c), WAVE - 1);
+    t.nxmin = lane_value(wave_max_to_last(q.nxmin), WAVE - 1);
+    t.xmax = lane_value(wave_max_to_last(q.xmax), WAVE - 1);
+    t.nymin = lane_value(wave_max_to_last(q.nymin), WAVE - 1);
+    t.ymax = lane_value(wave_max_to_last(q.ymax), WAVE - 1);
+    t.gx = (int32_t)lane_value(wave_inclusive_sum((uint32_t)q.gx), WAVE - 1);
+    t.gy = (int32_t)lane_value(wave_inclusive_sum((uint32_t)q.gy), WAVE - 1);
+    t.pg = lane_value(wave_inclusive_sum(q.pg), WAVE - 1);
+    return t;
+}
 
 // Records of the 4 pixels of this thread in tile `tile` (w is a multiple
 // of 4, so the 4 pixels share a row).
@@ -524,10 +542,35 @@ k_pair_count(const uint32_t *__restrict__ labels, const uint32_t *__restrict__ s
                     rw.step(pr.k[p], pr.p[p], flush, FX);
                 }
             }
-            rw.finish(flush, FX);
+        }
+        // the last runs, every lane of the wave here: when they all belong
+        // to one pair (a region inside one pair's boundary, the common
+        // case), the wave reduces them and one lane adds the total, instead
+        // of 64 lanes queueing their LDS atomics on the same words
+        const bool has = rw.c != 0;
+        const uint64_t hb = __builtin_amdgcn_ballot_w64(has);
+        if (hb) {
+            const int l0 = __builtin_ctzll(hb);
+            const uint64_t k0 = lane_value(rw.cur, l0);
+            if (__popcll(hb) > 1 && __builtin_amdgcn_ballot_w64(has && rw.cur != k0) == 0) {
+                const PairPart tq = wave_part_total(rw.part(FX));
+                uint32_t sl = 0;
+                if ((int)(threadIdx.x & (WAVE - 1)) == l0) {
+                    const uint32_t save = run;
+                    run = RUN_NONE;
+                    flush(k0, tq);
+                    sl = run;
+                    run = save;
+                }
+                sl = lane_value(sl, l0);
+                if (has) run = run == RUN_NONE ? sl : run | SUM_IRREGULAR;
+            } else {
+                rw.flush(flush, FX);
+            }
+        }
+        if (here)
             sums[(y * w + x0) / 4] =
                 make_uint2(rc, run == RUN_NONE ? 0u : run & (SUM_IRREGULAR | (TILE_SLOTS - 1)));
-        }
     }
     __syncthreads();
     static_assert(TILE_SLOTS == 2 * 256, "two table slots per thread");

@@ -106,6 +106,17 @@ ATQ_DEV T wave_inclusive_sum(T x) {
     u += dpp_move<0x143, 0xC, false>(u);
     return (T)u;
 }
+// Maximum over the 64 lanes of a wave (unsigned), in lane 63 (the lanes
+// below hold partial maxima); every lane active.
+ATQ_DEV uint32_t wave_max_to_last(uint32_t u) {
+    u = max(u, dpp_move<0x111, 0xF, true>(u));
+    u = max(u, dpp_move<0x112, 0xF, true>(u));
+    u = max(u, dpp_move<0x114, 0xF, true>(u));
+    u = max(u, dpp_move<0x118, 0xF, true>(u));
+    u = max(u, dpp_move<0x142, 0xA, false>(u));
+    u = max(u, dpp_move<0x143, 0xC, false>(u));
+    return u;
+}
 // The same over the first 16 lanes only (lanes 16-63 get their row's sums).
 template <typename T>
 ATQ_DEV T row_inclusive_sum(T x) {
