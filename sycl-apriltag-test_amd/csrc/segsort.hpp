@@ -378,13 +378,17 @@ k_seg_merge_lds(const Ext *__restrict__ segs, SegLong lng, uint32_t max_blocks,
         for (uint32_t k = 0; k < SEG_LDS_IPT; k++)
             if (k < per) buf[seg_ix(t + SEG_NT * k)] = v[k];
         __syncthreads();
+        // threads whose outputs all lie past the segment's keys leave their
+        // pads in place: a merge of pads yields the same pads
+        const bool live = d0 < m;
         for (uint32_t run = SEG_CHUNK; run < P; run <<= 1) {
             const uint32_t base = d0 / (2 * run) * (2 * run);
-            seg_merge_at(buf, base, run, d0 - base, per, v);
+            if (live) seg_merge_at(buf, base, run, d0 - base, per, v);
             __syncthreads();
+            if (live)
 #pragma unroll
-            for (uint32_t k = 0; k < SEG_LDS_IPT; k++)
-                if (k < per) buf[seg_ix(d0 + k)] = v[k];
+                for (uint32_t k = 0; k < SEG_LDS_IPT; k++)
+                    if (k < per) buf[seg_ix(d0 + k)] = v[k];
             __syncthreads();
         }
 #pragma unroll
