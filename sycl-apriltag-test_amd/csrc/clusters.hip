@@ -295,21 +295,32 @@ ATQ_DEV void fit_terms_sorted(const atq_cluster_point *sorted, const uint16_t *w
     }
 }
 
+// Slot of (cluster c, sub-chunk q) in the sub-chunk sum table: c + (start +
+// q * FIT_SUB) / FIT_SUB is strictly increasing over the sub-chunks of
+// successive clusters, so the table needs n_clusters + n_points / FIT_SUB
+// + 1 slots.
+ATQ_DEV uint32_t fit_sub_slot(uint32_t c, uint32_t start, uint32_t q) {
+    return c + (start + q * FIT_SUB) / FIT_SUB;
+}
+
 // Every cluster's points in angle order to `out` and their line-fit weights
 // (include/line_fit_point.h:50-86: 1 + the grey gradient's magnitude) to
 // wts, FIT_SUB positions per work-group item, four per thread in lockstep
 // (sorted keys, then points, then grey neighbourhoods: three rounds of
 // independent loads). Few registers, so many work-groups share a CU and
 // hide the gathers' latency; the prefix-sum kernels then read both arrays
-// coalesced.
-constexpr uint32_t FIT_SUB = 1024;
+// coalesced. With sub_sums, a long cluster's sub-chunks (all but those of
+// its last SEG_CHUNK chunk) also leave their six moment sums there: the
+// carries that the chunk-parallel prefix sums add.
 template <bool FX>
 __global__ void __launch_bounds__(256)
 k_fit_terms(const atq_cluster_point *__restrict__ in, const uint64_t *__restrict__ keys0,
             const uint64_t *__restrict__ keys1, const atq_cluster_extents *__restrict__ ext,
             const uint32_t *n_clusters_dev, uint32_t cap_cl, uint32_t max_subs,
             const uint8_t *__restrict__ grey, uint32_t w, uint32_t h,
-            atq_cluster_point *__restrict__ out, uint16_t *__restrict__ wts) {
+            atq_cluster_point *__restrict__ out, uint16_t *__restrict__ wts,
+            int64_t *__restrict__ sub_sums) {
+    __shared__ int64_t part[6][4];
     const uint32_t ncl = min(*n_clusters_dev, cap_cl);
     const uint64_t items = (uint64_t)ncl * max_subs;
     for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {  // sub-chunk-major
@@ -329,13 +340,39 @@ k_fit_terms(const atq_cluster_point *__restrict__ in, const uint64_t *__restrict
         uint32_t g[4][4];
 #pragma unroll
         for (int i = 0; i < 4; i++) linefit_gather(p[i].packed_x, p[i].packed_y, w, h, grey, g[i], FX);
+        uint32_t W[4], rx[4], ry[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
+            W[i] = rx[i] = ry[i] = 0;
             if (j[i] >= n) continue;
-            uint32_t W, rx, ry;
-            linefit_terms_from(p[i].packed_x, p[i].packed_y, w, h, g[i], &W, &rx, &ry, FX);
+            linefit_terms_from(p[i].packed_x, p[i].packed_y, w, h, g[i], &W[i], &rx[i], &ry[i], FX);
             out[start + j[i]] = p[i];
-            wts[start + j[i]] = (uint16_t)W;
+            wts[start + j[i]] = (uint16_t)W[i];
+        }
+        if (sub_sums && (b / SEG_CHUNK + 1) * SEG_CHUNK < n) {  // uniform in the work-group
+            int64_t sum[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const atq_line_fit_point f = linefit_from_terms(W[i], rx[i], ry[i]);
+                sum[0] += f.Mx;
+                sum[1] += f.My;
+                sum[2] += f.Mxx;
+                sum[3] += f.Myy;
+                sum[4] += f.Mxy;
+                sum[5] += f.W;
+            }
+            const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                const int64_t x = wave_inclusive_sum(sum[q]);
+                if (lane == 63) part[q][wave] = x;
+            }
+            __syncthreads();
+            if (threadIdx.x < 6)
+                sub_sums[6 * (size_t)fit_sub_slot(c, start, sb) + threadIdx.x] =
+                    part[threadIdx.x][0] + part[threadIdx.x][1] + part[threadIdx.x][2] +
+                    part[threadIdx.x][3];
+            __syncthreads();
         }
     }
 }
@@ -432,51 +469,6 @@ struct AngleStore {
     }
 };
 
-// Slot of (cluster c, chunk k) in the chunk-sum table: c + (start + k *
-// SEG_CHUNK) / SEG_CHUNK is strictly increasing over the chunks of
-// successive clusters, so the table needs n_clusters + n_points / SEG_CHUNK
-// + 1 slots.
-ATQ_DEV uint32_t fit_slot(uint32_t c, uint32_t start, uint32_t k) {
-    return c + (start + k * SEG_CHUNK) / SEG_CHUNK;
-}
-
-// Sums of the line-fit moments of every chunk but the last of the listed
-// long clusters, into the slot table (the carries the chunk-parallel fit
-// adds).
-template <bool FX>
-__global__ void __launch_bounds__(SEG_NT)
-k_fit_reduce(const atq_cluster_point *__restrict__ sorted, const uint16_t *__restrict__ wts,
-             const atq_cluster_extents *__restrict__ ext, SegLong lng, uint32_t max_chunks,
-             int64_t *__restrict__ slot_sums) {
-    constexpr int NW = SEG_NT / 64;
-    __shared__ int64_t part[6][NW];
-    const uint32_t nl = min(*lng.n, lng.cap);
-    const uint64_t items = (uint64_t)nl * max_chunks;
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t e0 = threadIdx.x * SEG_IPT;
-    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {  // chunk-major
-        const uint32_t c = lng.list[it % nl], k = (uint32_t)(it / nl);
-        const uint32_t start = ext[c].start, n = ext[c].count, b = k * SEG_CHUNK;
-        if (b + SEG_CHUNK >= n) continue;  // last chunks carry nothing on
-        uint32_t W[SEG_IPT], RX[SEG_IPT], RY[SEG_IPT];
-        int64_t sum[6];
-        fit_terms_sorted(sorted, wts, start + b, e0, SEG_CHUNK, W, RX, RY, sum, FX);
-#pragma unroll
-        for (int q = 0; q < 6; q++) {
-            const int64_t x = wave_inclusive_sum(sum[q]);
-            if (lane == 63) part[q][wave] = x;
-        }
-        __syncthreads();
-        if (threadIdx.x < 6) {
-            int64_t x = 0;
-#pragma unroll
-            for (int v = 0; v < NW; v++) x += part[threadIdx.x][v];
-            slot_sums[6 * (size_t)fit_slot(c, start, k) + threadIdx.x] = x;
-        }
-        __syncthreads();
-    }
-}
-
 // A work-group-uniform value kept in scalar registers.
 ATQ_DEV int64_t uniform64(int64_t v) {
     const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
@@ -486,8 +478,8 @@ ATQ_DEV int64_t uniform64(int64_t v) {
 
 // Chunk-parallel form over every cluster (its sorted keys where
 // seg_sorted_keys puts them): every chunk writes its points and its moments
-// prefix-summed from the sums of the cluster's earlier chunks (slot table,
-// k_fit_reduce, for the clusters longer than SEG_CHUNK).
+// prefix-summed from the sums of the cluster's earlier chunks (the
+// sub-chunk sums k_fit_terms leaves for the clusters longer than SEG_CHUNK).
 // SEQ: every cluster (n_clusters_dev) in input order, one work-group per
 // cluster walking its chunks and carrying the sums itself. PRE: the points
 // in angle order and their weights come from k_fit_terms (out, wts).
@@ -511,10 +503,28 @@ k_fit_chunks(const atq_cluster_point *__restrict__ in, atq_cluster_point *__rest
         if (b >= n) continue;
         const uint64_t *keys = SEQ ? nullptr : seg_sorted_keys(n, keys0, keys1) + start;
         int64_t carry[6] = {0, 0, 0, 0, 0, 0}, tot[6];
-        if (!SEQ && lf)
-            for (uint32_t q = 0; q < k; q++)
+        if (!SEQ && lf && k) {
+            // the sums of the 4k sub-chunks before this chunk: one round of
+            // loads spread over the threads (thread t < 1020 keeps field
+            // t % 6), then per field through LDS (stage is free here)
+            constexpr uint32_t SPAN = SEG_NT / 6 * 6;
+            const uint32_t t = threadIdx.x, nv = 6 * k * (SEG_CHUNK / FIT_SUB);
+            int64_t part = 0;
+            if (t < SPAN)
+                for (uint32_t x = t; x < nv; x += SPAN)
+                    part += slot_sums[6 * (size_t)fit_sub_slot(c, start, x / 6) + x % 6];
+            if (t < SPAN) stage[t] = part;
+            __syncthreads();
+            if (t < 6) {
+                int64_t a = 0;
+                for (uint32_t q = t; q < min(nv, SPAN); q += 6) a += stage[q];
+                stage[SPAN + t] = a;
+            }
+            __syncthreads();
 #pragma unroll
-                for (int f = 0; f < 6; f++) carry[f] += slot_sums[6 * (size_t)fit_slot(c, start, q) + f];
+            for (int f = 0; f < 6; f++) carry[f] = stage[SPAN + f];
+            __syncthreads();
+        }
         for (uint32_t bb = b; bb < (SEQ ? n : b + 1); bb += SEG_CHUNK) {
             uint32_t src[SEG_IPT];
 #pragma unroll
@@ -598,9 +608,8 @@ int angle_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out
     if (lf && wts) {
         const uint32_t max_subs = (max_cluster + FIT_SUB - 1) / FIT_SUB;
         k_fit_terms<FX><<<8192, 256, 0, s>>>(in, keys0, keys1, ext, &counts->n_clusters, cap_cl,
-                                             max_subs ? max_subs : 1, grey, w, h, out, wts);
-        if (max_cluster > SEG_CHUNK)
-            k_fit_reduce<FX><<<512, SEG_NT, 0, s>>>(out, wts, ext, lng, max_chunks, slot_sums);
+                                             max_subs ? max_subs : 1, grey, w, h, out, wts,
+                                             max_cluster > SEG_CHUNK ? slot_sums : nullptr);
     }
     // one work-group per chunk slot at most a few rounds deep: the dispatcher
     // hands the chunks that exist to whichever CU is free

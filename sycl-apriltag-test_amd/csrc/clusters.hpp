@@ -27,10 +27,11 @@ int launch_assign(hipStream_t s, const atq_boundary_point *pts, const uint32_t *
 // in that order to `out` and their line-fit prefix sums to `lf` (may be
 // null; src/main.cpp:768-801). max_cluster bounds the cluster length.
 // keys0/keys1: two key buffers of cap entries; slot_sums: 6 * (cap_cl + cap
-// / SEG_CHUNK + 2) int64; lng: a long-segment list of seg_long_cap(cap)
+// / FIT_SUB + 2) int64; lng: a long-segment list of seg_long_cap(cap)
 // entries whose count is zero. fx: fixed mode (record decode, weight row).
 // cidx (may be null) receives every point's cluster index. wts: cap
 // entries of scratch for the points' line-fit weights (required with lf).
+constexpr uint32_t FIT_SUB = 1024;  // the line fit's sub-chunks (a quarter of SEG_CHUNK)
 int launch_angle_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out,
                      const atq_cluster_extents *ext, const atq_counts *counts, uint32_t cap_cl,
                      uint32_t max_cluster, const uint8_t *grey, uint32_t w, uint32_t h,

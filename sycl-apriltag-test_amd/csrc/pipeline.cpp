@@ -61,7 +61,7 @@ size_t Workspace::carve(char *base, const Caps &c) {
     seg_cluster = carve_one<uint32_t>(base, off, c.cap);
     seg_bounds = carve_one<atq_cluster_bounds>(base, off, c.cap);
     cpts_tmp = carve_one<atq_cluster_point>(base, off, c.cap);
-    fit_sums = carve_one<int64_t>(base, off, 6 * ((size_t)c.cap_cl + c.cap / SEG_CHUNK + 2));
+    fit_sums = carve_one<int64_t>(base, off, 6 * ((size_t)c.cap_cl + c.cap / FIT_SUB + 2));
     long_list = carve_one<uint32_t>(base, off, 2 * (size_t)seg_long_cap(c.cap));
     tiles = carve_one<uint32_t>(base, off, 8 * ((size_t)c.npx / BOUNDARY_TILE_PX + 1));
     qtmp = carve_one<atq_quad_corners>(base, off, c.cap_cl);

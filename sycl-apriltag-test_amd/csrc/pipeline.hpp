@@ -32,7 +32,7 @@ struct Workspace {
     uint32_t *seg_id, *seg_start, *valid, *seg_cluster;
     atq_cluster_bounds *seg_bounds;
     atq_cluster_point *cpts_tmp;
-    int64_t *fit_sums;  // per (cluster, chunk) line-fit sums of long clusters
+    int64_t *fit_sums;  // per (cluster, sub-chunk) line-fit sums of long clusters
     uint32_t *long_list;  // long segments of the angle and corner sorts (2 lists)
     uint32_t *tiles;
     atq_quad_corners *qtmp;
