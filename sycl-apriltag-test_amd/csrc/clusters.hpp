@@ -64,11 +64,13 @@ int launch_zero_u32(hipStream_t s, uint32_t *p, const uint32_t *n_dev, uint32_t 
                     uint32_t set_idx = 0xFFFFFFFFu, uint32_t set_val = 0);
 // Corners compacted and grouped into per-cluster peak segments, each in
 // (error descending, index) order; scratch: 2 * cap u64; lng as for
-// launch_angle_fit.
+// launch_angle_fit. top_only: only each segment's min(10, n) first peaks in
+// that order (all that do_indexing reads), the others after them in the
+// compaction's order but for the ones those ten displaced.
 int launch_corner_segments(hipStream_t s, const atq_corner *sparse, const uint32_t *n_sparse_dev,
                            uint32_t cap, atq_counts *counts, uint32_t cap_cl, uint32_t max_seg,
                            atq_peak_extents *peaks, atq_corner *compacted, uint64_t *scratch,
-                           SegLong lng, ScanCtx &sc);
+                           SegLong lng, ScanCtx &sc, bool top_only = false);
 int launch_do_indexing(hipStream_t s, const atq_peak_extents *peaks, const uint32_t *n_seg_dev,
                        uint32_t cap, const atq_corner *corners, const atq_line_fit_point *lf,
                        const uint32_t *n_points_dev, uint32_t n_points_host,

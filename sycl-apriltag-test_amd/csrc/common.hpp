@@ -137,6 +137,29 @@ ATQ_DEV uint64_t lane_value(uint64_t x, int l) {
 }
 ATQ_DEV int64_t lane_value(int64_t x, int l) { return (int64_t)lane_value((uint64_t)x, l); }
 
+// One step of wave_min_u64: the minimum of x and the DPP-moved x (all ones
+// where the move finds no source lane).
+template <int CTRL, int ROW_MASK>
+ATQ_DEV uint64_t dpp_min_step(uint64_t x) {
+    const uint32_t lo =
+        (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)(uint32_t)x, CTRL, ROW_MASK, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)(uint32_t)(x >> 32), CTRL,
+                                                              ROW_MASK, 0xF, false);
+    const uint64_t y = ((uint64_t)hi << 32) | lo;
+    return y < x ? y : x;
+}
+// Minimum over the 64 lanes of a wave (unsigned 64-bit), in every lane;
+// every lane active.
+ATQ_DEV uint64_t wave_min_u64(uint64_t v) {
+    v = dpp_min_step<0x111, 0xF>(v);
+    v = dpp_min_step<0x112, 0xF>(v);
+    v = dpp_min_step<0x114, 0xF>(v);
+    v = dpp_min_step<0x118, 0xF>(v);
+    v = dpp_min_step<0x142, 0xA>(v);
+    v = dpp_min_step<0x143, 0xC>(v);
+    return lane_value(v, WAVE - 1);
+}
+
 // Exclusive prefix of a small per-lane count within the wave; *total gets
 // the wave sum. Every lane active.
 ATQ_DEV uint32_t wave_exclusive_sum(uint32_t v, uint32_t *total) {

@@ -479,16 +479,148 @@ int launch_fit_lines(hipStream_t s, const atq_line_fit_point *lf, const uint32_t
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
+// The detector's peak order. do_indexing (src/fit_quad.cpp:445-519) reads
+// only a segment's min(10, n) highest-error peaks, so the detector does not
+// sort whole segments: it puts those peaks first, in the reference's order
+// (CornerKey ascending), moves the peaks they displace from the first
+// min(10, n) places into the places they left (both in position order) and
+// leaves every other peak where the compaction put it. One work-group per
+// segment (which also closes the segment's extent, as k_peak_close does):
+// each thread keeps the ten smallest keys of its positions (t, t + 256, ...,
+// loaded four at a time), each wave draws its ten smallest from its lanes'
+// lists, and wave 0 ranks the four waves' forty and plans the moves.
+constexpr int CORNER_TOP = 10;
+__global__ void __launch_bounds__(256)
+k_corner_top(atq_peak_extents *__restrict__ peaks, const atq_counts *counts, uint32_t cap,
+             atq_corner *__restrict__ c) {
+    constexpr int NW = 256 / 64;
+    __shared__ uint64_t wk[NW * CORNER_TOP];
+    __shared__ uint32_t wp[NW * CORNER_TOP];
+    __shared__ uint32_t tp[CORNER_TOP];
+    __shared__ uint32_t mv_src[2 * CORNER_TOP], mv_dst[2 * CORNER_TOP];
+    __shared__ uint32_t n_mv;
+    const uint32_t E = min(counts->n_segments, cap), nc = counts->n_corners;
+    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    for (uint32_t e = blockIdx.x; e < E; e += gridDim.x) {
+        const uint32_t start = peaks[e].start;
+        const uint32_t n = (e + 1 < E ? peaks[e + 1].start : nc) - start;
+        if (t == 0) peaks[e].count = n;
+        const uint32_t m = min(n, (uint32_t)CORNER_TOP);
+        uint64_t bk[CORNER_TOP];
+        uint32_t bp[CORNER_TOP];
+#pragma unroll
+        for (int k = 0; k < CORNER_TOP; k++) {
+            bk[k] = SEG_PAD;
+            bp[k] = 0;
+        }
+        for (uint32_t b0 = 0; b0 < n; b0 += 4 * 256) {
+            atq_corner cr[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {  // the four loads in flight together
+                const uint32_t i = b0 + t + 256 * q;
+                cr[q] = i < n ? c[start + i] : atq_corner{};
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t i = b0 + t + 256 * q;
+                uint64_t key = i < n ? ((uint64_t)~float_key(cr[q].error) << 32) |
+                                           cr[q].line_fit_point_index
+                                     : SEG_PAD;
+                uint32_t pos = i;
+#pragma unroll
+                for (int k = 0; k < CORNER_TOP; k++) {  // the key sinks to its place
+                    const bool lt = key < bk[k];
+                    const uint64_t ck = lt ? bk[k] : key;
+                    const uint32_t cp = lt ? bp[k] : pos;
+                    bk[k] = lt ? key : bk[k];
+                    bp[k] = lt ? pos : bp[k];
+                    key = ck;
+                    pos = cp;
+                }
+            }
+        }
+        // the wave's ten smallest: each round takes the smallest head
+        for (int r = 0; r < CORNER_TOP; r++) {
+            const uint64_t mk = wave_min_u64(bk[0]);
+            const uint64_t who = __builtin_amdgcn_ballot_w64(bk[0] == mk);
+            const int l = (int)__builtin_ctzll(who);
+            const uint32_t p = lane_value(bp[0], l);
+            if (lane == 0) {
+                wk[wave * CORNER_TOP + r] = mk;
+                wp[wave * CORNER_TOP + r] = p;
+            }
+            if ((int)lane == l) {
+#pragma unroll
+                for (int k = 0; k + 1 < CORNER_TOP; k++) {
+                    bk[k] = bk[k + 1];
+                    bp[k] = bp[k + 1];
+                }
+                bk[CORNER_TOP - 1] = SEG_PAD;
+            }
+        }
+        __syncthreads();
+        if (wave == 0) {
+            // rank of each of the forty (keys are distinct; pads rank >= m)
+            if (lane < NW * CORNER_TOP) {
+                const uint64_t k0 = wk[lane];
+                uint32_t rank = 0;
+#pragma unroll
+                for (int j = 0; j < NW * CORNER_TOP; j++) rank += wk[j] < k0 ? 1u : 0u;
+                if (rank < m) tp[rank] = wp[lane];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            // moves: the ten to the front (lane k); a front place q that
+            // holds none of the ten is displaced (its index among those, j)
+            // into the j-th smallest of the places the ten left behind
+            uint32_t mine = 0, in_top = 0;
+            bool left = false;
+            uint32_t left_rank = 0;
+            if (lane < m) {
+                mine = tp[lane];
+                left = mine >= m;
+#pragma unroll
+                for (int k = 0; k < CORNER_TOP; k++) {
+                    if ((uint32_t)k >= m) break;
+                    const uint32_t o = tp[k];
+                    in_top |= o == lane ? 1u : 0u;
+                    left_rank += (o >= m && o < mine) ? 1u : 0u;
+                }
+                mv_src[lane] = mine;
+                mv_dst[lane] = lane;
+            }
+            const uint64_t disp = __builtin_amdgcn_ballot_w64(lane < m && !in_top);
+            const uint64_t below = (1ull << lane) - 1;
+            if (lane < m && !in_top) mv_src[m + __popcll(disp & below)] = lane;
+            if (lane < m && left) mv_dst[m + left_rank] = mine;
+            if (lane == 0) n_mv = m + (uint32_t)__popcll(disp);
+        }
+        __syncthreads();
+        // every source read before any place is overwritten
+        const uint32_t nm = n_mv;
+        atq_corner v{};
+        if (t < nm) v = c[start + mv_src[t]];
+        __syncthreads();
+        if (t < nm) c[start + mv_dst[t]] = v;
+        __syncthreads();
+    }
+}
+
 int launch_corner_segments(hipStream_t s, const atq_corner *sparse, const uint32_t *n_sparse_dev,
                            uint32_t cap, atq_counts *counts, uint32_t cap_cl, uint32_t max_seg,
                            atq_peak_extents *peaks, atq_corner *compacted, uint64_t *scratch,
-                           SegLong lng, ScanCtx &sc) {
+                           SegLong lng, ScanCtx &sc, bool top_only) {
     int r = device_scan(s, n_sparse_dev, cap, CornerFlag{sparse}, CornerOut{sparse, compacted},
                         sc, &counts->n_corners);
     if (r) return r;
     r = device_scan(s, &counts->n_corners, cap, SegHead{compacted}, SegStart{peaks}, sc,
                     &counts->n_segments);
     if (r) return r;
+    if (top_only) {  // closes the extents itself
+        k_corner_top<<<min(cap_cl, 2048u), 256, 0, s>>>(peaks, counts, cap_cl, compacted);
+        return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
+    }
     k_peak_close<<<min(64u, atq_div_up(cap_cl, 256) + 1), 256, 0, s>>>(counts, cap_cl, peaks);
     r = seg_sort(s, CornerKey{compacted}, CornerFinish{compacted}, peaks, &counts->n_segments,
                  cap_cl, max_seg, scratch, scratch + cap, lng);

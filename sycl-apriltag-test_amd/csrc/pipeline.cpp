@@ -294,7 +294,7 @@ int run_back(atq_detector *d, hipStream_t s, const uint8_t *gray_in) {
     TRY(mark(1 + ATQ_STAGE_FIT_LINES));
     TRY(launch_corner_segments(s, d->sparse, &cnt->n_cluster_points, c.cap, cnt, c.cap_cl,
                                max_cluster, d->peaks, d->corners, ws.angle_keys(),
-                               ws.long_segs(1, &d->long_n[1], c), sc));
+                               ws.long_segs(1, &d->long_n[1], c), sc, true));
     TRY(mark(1 + ATQ_STAGE_CORNERS));
     TRY(launch_do_indexing(s, d->peaks, &cnt->n_segments, c.cap_cl, d->corners, d->lf,
                            &cnt->n_cluster_points, 0, d->cext, d->fitted, &cnt->oob_reads, fx));
