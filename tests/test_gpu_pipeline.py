@@ -304,6 +304,19 @@ def test_detector_more_valid_clusters_than_u16_ids(atq, oracle):
     check_against_oracle(g, o)
 
 
+def test_detector_two_rank_passes(atq, oracle):
+    """About 7,000 valid clusters at 1280x960: more than one 11-bit radix pass
+    of the pair ranks covers (2048), fewer than the detector ranks at this
+    table size (8192), so the rank sort takes its second pass."""
+    from pyatq import synth
+    img = synth.square_grid(1280, 960).image
+    o = oracle.pipeline(img, 1)
+    assert 2048 < len(o["bounds"]) <= 8192
+    g = run_detector(atq, img, 1)
+    assert int(g["counts"]["sort_passes"]) == 2
+    check_against_oracle(g, o)
+
+
 def test_every_front_shape_gives_identical_results(atq, oracle):
     """Threshold regions and CCL tiles only split the work differently: the
     thresholded image, the labels and the quads are the same for every shape
