@@ -545,8 +545,63 @@ ATQ_DEV void rs_full_body(const RsDyn &d, const uint32_t *count, uint32_t cap, u
     rs_fallback_clusters(d, n, G, bar, gen, error, L.lds4, wpart);
 }
 
-// The last launched rank pass also hosts the full-key fallback (its
-// work-groups 0 .. RS_FULL_GROUPS-1), full_passes 11-bit digits.
+// The last pass's extras as plain values (taking the address of the
+// kernel argument would copy it to scratch memory in every lane).
+ATQ_DEV RsFinal rs_final(const RsDyn &d, bool last) {
+    RsFinal fin{};
+    if (last) {
+        fin.on = true;
+        fin.shift = *d.shift;
+        fin.xlat = d.xlat;
+        fin.centre = d.cl.centre;
+        fin.cpts = d.cl.cpts;
+        fin.fx = d.cl.fx != 0;
+    }
+    return fin;
+}
+
+// One rank pass by the G resident work-groups of a launch: chunk
+// histograms, column scans and the scatter, separated by grid barriers
+// (the second pass of frames whose ranks need two digits, so that frames
+// needing one pay for a single idle launch, not three).
+ATQ_DEV void rs_rank_pass_body(const RsDyn &d, const uint32_t *count, uint32_t cap, uint32_t pass,
+                               uint32_t G, uint32_t *hist, uint32_t *totals, uint32_t *bar,
+                               uint32_t *error, RsScatterLds<RK_BITS> &L) {
+    constexpr int B = 1 << RK_BITS;
+    const uint64_t *ki;
+    const uint32_t *vi;
+    uint64_t *ko;
+    uint32_t *vo;
+    bool last;
+    if (!rs_dyn_io(d, pass, &ki, &vi, &ko, &vo, &last)) return;
+    const uint32_t n = min(*count, cap), g = blockIdx.x;
+    uint32_t chunk = (n + G - 1) / G;
+    chunk = (chunk + RS_ROUND - 1) / RS_ROUND * RS_ROUND;
+    const uint32_t b = min(n, g * chunk), e = min(n, b + chunk);
+    const uint32_t shift = pass * RK_BITS;
+    uint32_t gen = 0;
+    rs_hist_body<RK_BITS>(ki, b, e, shift, hist + (size_t)g * B, L.wc[0]);
+    rs_grid_sync(bar, ++gen * G, error);
+    for (uint32_t dd = g * RS_THREADS + threadIdx.x; dd < (uint32_t)B; dd += G * RS_THREADS) {
+        uint32_t run = 0;
+        for (uint32_t q = 0; q < G; q++) {
+            const uint32_t v = hist[(size_t)q * B + dd];
+            hist[(size_t)q * B + dd] = run;
+            run += v;
+        }
+        totals[dd] = run;
+    }
+    rs_grid_sync(bar, ++gen * G, error);
+    rs_scatter_body<RK_BITS>(ki, vi, ko, vo, b, e, shift, hist + (size_t)g * B, totals,
+                             rs_final(d, last), L);
+}
+
+// The last launched rank pass (pass 1) also hosts the full-key fallback and
+// the second rank pass: its work-groups 0 .. RS_FULL_GROUPS-1 run the
+// fallback's full_passes 11-bit digits when the ranks are not used, or the
+// second rank pass when the ranks need two digits (rs_rank_pass_body);
+// otherwise every work-group exits at once. Launched for pass 0 it is that
+// pass's scatter.
 __global__ void __launch_bounds__(RS_THREADS)
 k_rs_scatter_dyn(RsDyn d, const uint32_t *count, uint32_t cap, uint32_t pass,
                  uint32_t *__restrict__ hist, uint32_t *__restrict__ totals, uint32_t full_passes,
@@ -560,6 +615,11 @@ k_rs_scatter_dyn(RsDyn d, const uint32_t *count, uint32_t cap, uint32_t pass,
                                   error, L, wpart);
         return;
     }
+    if (pass > 0) {
+        if (blockIdx.x < RS_FULL_GROUPS)
+            rs_rank_pass_body(d, count, cap, pass, RS_FULL_GROUPS, hist, totals, bar, error, L);
+        return;
+    }
     const uint64_t *ki;
     const uint32_t *vi;
     uint64_t *ko;
@@ -571,32 +631,22 @@ k_rs_scatter_dyn(RsDyn d, const uint32_t *count, uint32_t cap, uint32_t pass,
     rs_chunks(n, &chunk, &G);
     if (blockIdx.x >= G) return;
     const uint32_t b = blockIdx.x * chunk, e = min(n, b + chunk);
-    // the last pass's extras as plain values (taking the address of the
-    // kernel argument would copy it to scratch memory in every lane)
-    RsFinal fin{};
-    if (last) {
-        fin.on = true;
-        fin.shift = *d.shift;
-        fin.xlat = d.xlat;
-        fin.centre = d.cl.centre;
-        fin.cpts = d.cl.cpts;
-        fin.fx = d.cl.fx != 0;
-    }
     rs_scatter_body<RK_BITS>(ki, vi, ko, vo, b, e, pass * RK_BITS, hist + (size_t)blockIdx.x * B,
-                             totals, fin, L);
+                             totals, rs_final(d, last), L);
 }
 
 int rank_sort(hipStream_t s, uint32_t full_bits, const RsDyn &d, const uint32_t *count,
               uint32_t cap, uint32_t *hist, uint32_t *totals, uint32_t *bar, uint32_t *error) {
     constexpr uint32_t B = 1u << RK_BITS;
     const uint32_t full_passes = (full_bits + RK_BITS - 1) / RK_BITS;
-    for (uint32_t pass = 0; pass < RK_MAX_PASSES; pass++) {
-        const bool last = pass + 1 == RK_MAX_PASSES;
-        k_rs_hist_dyn<<<RS_GROUPS, RS_THREADS, 0, s>>>(d, count, cap, pass, hist);
-        k_rs_scan<<<B / 64, 64 * RS_SCAN_WAVES, 0, s>>>(hist, B, count, cap, totals, d.passes, pass);
-        k_rs_scatter_dyn<<<RS_GROUPS, RS_THREADS, 0, s>>>(d, count, cap, pass, hist, totals,
-                                                          last ? full_passes : 0u, bar, error);
-    }
+    static_assert(RK_MAX_PASSES == 2, "pass 1 runs inside the last launch");
+    k_rs_hist_dyn<<<RS_GROUPS, RS_THREADS, 0, s>>>(d, count, cap, 0, hist);
+    k_rs_scan<<<B / 64, 64 * RS_SCAN_WAVES, 0, s>>>(hist, B, count, cap, totals, d.passes, 0);
+    k_rs_scatter_dyn<<<RS_GROUPS, RS_THREADS, 0, s>>>(d, count, cap, 0, hist, totals, 0u, bar,
+                                                      error);
+    // pass 1 (and the fallback): RS_FULL_GROUPS work-groups, grid barriers
+    k_rs_scatter_dyn<<<RS_FULL_GROUPS, RS_THREADS, 0, s>>>(d, count, cap, 1, hist, totals,
+                                                           full_passes, bar, error);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
