@@ -76,8 +76,10 @@ int launch_do_indexing(hipStream_t s, const atq_peak_extents *peaks, const uint3
                        const uint32_t *n_points_dev, uint32_t n_points_host,
                        const atq_cluster_extents *ext, atq_fitted_quad *fitted,
                        uint32_t *oob_counter, bool fx = false);
+// process_fitted_quads (minus decoding) and the passing quads' compaction in
+// one look-back scan pass; tmp: cap entries of scratch.
 int launch_quads(hipStream_t s, const atq_fitted_quad *fitted, const uint32_t *n_fitted_dev,
-                 uint32_t cap, atq_quad_corners *tmp, uint32_t *flags, atq_quad_corners *out,
+                 uint32_t cap, atq_quad_corners *tmp, atq_quad_corners *out,
                  uint32_t *n_out_dev, ScanCtx &sc, int min_tag_width);
 
 }  // namespace atq

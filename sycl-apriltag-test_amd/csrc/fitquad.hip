@@ -129,9 +129,13 @@ ATQ_DEV void combo_decode(uint32_t num, uint32_t out[4]) {
     }
 }
 
+// The quads' compaction scan's flag: fitted quad e's corners to tmp[e]
+// (read back by QuadOut in the same thread) and whether it passes.
 struct QuadFlag {
-    const uint32_t *flags;
-    ATQ_DEV uint32_t operator()(uint32_t e) const { return flags[e]; }
+    const atq_fitted_quad *fitted;
+    atq_quad_corners *tmp;
+    int min_tag_width;
+    ATQ_DEV uint32_t operator()(uint32_t e) const;
 };
 struct QuadOut {
     const atq_quad_corners *tmp;
@@ -393,69 +397,69 @@ k_do_indexing(const atq_peak_extents *__restrict__ peaks, const uint32_t *n_seg_
     }
 }
 
-// process_fitted_quads (src/main.cpp:1035-1162) minus decoding.
-__global__ void __launch_bounds__(256)
-k_quads(const atq_fitted_quad *__restrict__ fitted, const uint32_t *n_dev, uint32_t cap,
-        atq_quad_corners *__restrict__ tmp, uint32_t *__restrict__ flags, int min_tag_width) {
-    const uint32_t E = n_dev ? min(*n_dev, cap) : cap;
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
-        const atq_fitted_quad &q = fitted[e];
-        uint32_t ok = 0;
-        atq_quad_corners c;
-        c.reversed_border = 0;
-        c.pad_[0] = c.pad_[1] = c.pad_[2] = 0;
-        c.blob_index = e;
-        for (int i = 0; i < 4; i++) c.corners[i][0] = c.corners[i][1] = 0.0f;
-        if (q.indices[0] | q.indices[1] | q.indices[2] | q.indices[3]) {
-            ok = 1;
-            float lines[4][4];
-            for (int i = 0; i < 4; i++) {
-                float err, mse;
-                fit_line(q.moments[i], q.num_in_moments[i], lines[i], &err, &mse);
-            }
-            for (int i = 0; i < 4 && ok; i++) {
-                const int i1 = (i + 1) & 3;
-                double A00 = lines[i][3], A01 = -lines[i1][3];
-                double A10 = -lines[i][2], A11 = lines[i1][2];
-                double B0 = -lines[i][0] + lines[i1][0];
-                double B1 = -lines[i][1] + lines[i1][1];
-                double det = A00 * A11 - A10 * A01;
-                double W00 = A11 / det, W01 = -A01 / det;
-                if (fabs(det) < 0.001) { ok = 0; break; }
-                double L0 = W00 * B0 + W01 * B1;
-                c.corners[i][0] = (float)((double)lines[i][0] + L0 * A00);
-                c.corners[i][1] = (float)((double)lines[i][1] + L0 * A10);
-            }
-            if (ok) {
-                float area = 0.0f, len[3], pp;
-                for (int i = 0; i < 3; i++) {
-                    int a = i, b = (i + 1) % 3;
-                    len[i] = hypot_f(c.corners[b][0] - c.corners[a][0], c.corners[b][1] - c.corners[a][1]);
-                }
-                pp = (len[0] + len[1] + len[2]) / 2;
-                area += sqrtf(pp * (pp - len[0]) * (pp - len[1]) * (pp - len[2]));
-                const int idxs[4] = {2, 3, 0, 2};
-                for (int i = 0; i < 3; i++) {
-                    int a = idxs[i], b = idxs[i + 1];
-                    len[i] = hypot_f(c.corners[b][0] - c.corners[a][0], c.corners[b][1] - c.corners[a][1]);
-                }
-                pp = (len[0] + len[1] + len[2]) / 2;
-                area += sqrtf(pp * (pp - len[0]) * (pp - len[1]) * (pp - len[2]));
-                if ((double)area < 0.95 * min_tag_width * min_tag_width) ok = 0;
-            }
-            for (int i = 0; i < 4 && ok; i++) {
-                int i0 = i, i1 = (i + 1) & 3, i2 = (i + 2) & 3;
-                float dx1 = c.corners[i1][0] - c.corners[i0][0];
-                float dy1 = c.corners[i1][1] - c.corners[i0][1];
-                float dx2 = c.corners[i2][0] - c.corners[i1][0];
-                float dy2 = c.corners[i2][1] - c.corners[i1][1];
-                float cosd = (dx1 * dx2 + dy1 * dy2) / sqrtf((dx1 * dx1 + dy1 * dy1) * (dx2 * dx2 + dy2 * dy2));
-                if ((double)fabsf(cosd) > COS10_D || dx1 * dy2 < dy1 * dx2) ok = 0;
-            }
+// process_fitted_quads (src/main.cpp:1035-1162) minus decoding, for fitted
+// quad e: its corners to *out, returns whether it passes.
+ATQ_DEV uint32_t quad_of(const atq_fitted_quad &q, uint32_t e, int min_tag_width,
+                         atq_quad_corners *out) {
+    uint32_t ok = 0;
+    atq_quad_corners c;
+    c.reversed_border = 0;
+    c.pad_[0] = c.pad_[1] = c.pad_[2] = 0;
+    c.blob_index = e;
+    for (int i = 0; i < 4; i++) c.corners[i][0] = c.corners[i][1] = 0.0f;
+    if (q.indices[0] | q.indices[1] | q.indices[2] | q.indices[3]) {
+        ok = 1;
+        float lines[4][4];
+        for (int i = 0; i < 4; i++) {
+            float err, mse;
+            fit_line(q.moments[i], q.num_in_moments[i], lines[i], &err, &mse);
         }
-        tmp[e] = c;
-        flags[e] = ok;
+        for (int i = 0; i < 4 && ok; i++) {
+            const int i1 = (i + 1) & 3;
+            double A00 = lines[i][3], A01 = -lines[i1][3];
+            double A10 = -lines[i][2], A11 = lines[i1][2];
+            double B0 = -lines[i][0] + lines[i1][0];
+            double B1 = -lines[i][1] + lines[i1][1];
+            double det = A00 * A11 - A10 * A01;
+            double W00 = A11 / det, W01 = -A01 / det;
+            if (fabs(det) < 0.001) { ok = 0; break; }
+            double L0 = W00 * B0 + W01 * B1;
+            c.corners[i][0] = (float)((double)lines[i][0] + L0 * A00);
+            c.corners[i][1] = (float)((double)lines[i][1] + L0 * A10);
+        }
+        if (ok) {
+            float area = 0.0f, len[3], pp;
+            for (int i = 0; i < 3; i++) {
+                int a = i, b = (i + 1) % 3;
+                len[i] = hypot_f(c.corners[b][0] - c.corners[a][0], c.corners[b][1] - c.corners[a][1]);
+            }
+            pp = (len[0] + len[1] + len[2]) / 2;
+            area += sqrtf(pp * (pp - len[0]) * (pp - len[1]) * (pp - len[2]));
+            const int idxs[4] = {2, 3, 0, 2};
+            for (int i = 0; i < 3; i++) {
+                int a = idxs[i], b = idxs[i + 1];
+                len[i] = hypot_f(c.corners[b][0] - c.corners[a][0], c.corners[b][1] - c.corners[a][1]);
+            }
+            pp = (len[0] + len[1] + len[2]) / 2;
+            area += sqrtf(pp * (pp - len[0]) * (pp - len[1]) * (pp - len[2]));
+            if ((double)area < 0.95 * min_tag_width * min_tag_width) ok = 0;
+        }
+        for (int i = 0; i < 4 && ok; i++) {
+            int i0 = i, i1 = (i + 1) & 3, i2 = (i + 2) & 3;
+            float dx1 = c.corners[i1][0] - c.corners[i0][0];
+            float dy1 = c.corners[i1][1] - c.corners[i0][1];
+            float dx2 = c.corners[i2][0] - c.corners[i1][0];
+            float dy2 = c.corners[i2][1] - c.corners[i1][1];
+            float cosd = (dx1 * dx2 + dy1 * dy2) / sqrtf((dx1 * dx1 + dy1 * dy1) * (dx2 * dx2 + dy2 * dy2));
+            if ((double)fabsf(cosd) > COS10_D || dx1 * dy2 < dy1 * dx2) ok = 0;
+        }
     }
+    *out = c;
+    return ok;
+}
+
+ATQ_DEV uint32_t QuadFlag::operator()(uint32_t e) const {
+    return quad_of(fitted[e], e, min_tag_width, &tmp[e]);
 }
 
 int launch_set_u32(hipStream_t s, uint32_t *p, uint32_t v) {
@@ -641,12 +645,11 @@ int launch_do_indexing(hipStream_t s, const atq_peak_extents *peaks, const uint3
 }
 
 int launch_quads(hipStream_t s, const atq_fitted_quad *fitted, const uint32_t *n_fitted_dev,
-                 uint32_t cap, atq_quad_corners *tmp, uint32_t *flags, atq_quad_corners *out,
+                 uint32_t cap, atq_quad_corners *tmp, atq_quad_corners *out,
                  uint32_t *n_out_dev, ScanCtx &sc, int min_tag_width) {
-    k_quads<<<min(256u, atq_div_up(cap, 256) + 1), 256, 0, s>>>(fitted, n_fitted_dev, cap, tmp,
-                                                                  flags, min_tag_width);
-    return device_scan(s, n_fitted_dev, cap, QuadFlag{flags}, QuadOut{tmp, out}, sc,
-                       n_out_dev);
+    return device_scan<QuadFlag, QuadOut, true>(s, n_fitted_dev, cap,
+                                                QuadFlag{fitted, tmp, min_tag_width},
+                                                QuadOut{tmp, out}, sc, n_out_dev);
 }
 
 }  // namespace atq

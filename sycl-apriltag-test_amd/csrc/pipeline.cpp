@@ -65,7 +65,6 @@ size_t Workspace::carve(char *base, const Caps &c) {
     long_list = carve_one<uint32_t>(base, off, 2 * (size_t)seg_long_cap(c.cap));
     tiles = carve_one<uint32_t>(base, off, 8 * ((size_t)c.npx / BOUNDARY_TILE_PX + 1));
     qtmp = carve_one<atq_quad_corners>(base, off, c.cap_cl);
-    qflags = carve_one<uint32_t>(base, off, c.cap_cl);
     scan = carve_one<uint32_t>(base, off, SCAN_GROUPS + 1);
     hist = carve_one<uint32_t>(base, off, (size_t)RS_MAX_DIGITS * RS_GROUPS);
     digit_totals = carve_one<uint32_t>(base, off, RS_MAX_DIGITS);
@@ -299,7 +298,7 @@ int run_back(atq_detector *d, hipStream_t s, const uint8_t *gray_in) {
     TRY(launch_do_indexing(s, d->peaks, &cnt->n_segments, c.cap_cl, d->corners, d->lf,
                            &cnt->n_cluster_points, 0, d->cext, d->fitted, &cnt->oob_reads, fx));
     TRY(mark(1 + ATQ_STAGE_INDEXING));
-    TRY(launch_quads(s, d->fitted, &cnt->n_segments, c.cap_cl, ws.qtmp, ws.qflags, d->quads,
+    TRY(launch_quads(s, d->fitted, &cnt->n_segments, c.cap_cl, ws.qtmp, d->quads,
                      &cnt->n_quads, sc, d->params.min_tag_width));
     TRY(mark(1 + ATQ_STAGE_QUADS));
     return ATQ_OK;
@@ -482,7 +481,7 @@ int atq_quads_from_fits(void *stream, const atq_fitted_quad *fitted, size_t n_fi
     sc.scratch = ws.scan;
     uint32_t *n_dev = &ws.counts->reserved[1];
     TRY(launch_set_u32(s, n_dev, (uint32_t)n_fitted));
-    return launch_quads(s, fitted, n_dev, c.cap_cl, ws.qtmp, ws.qflags, out, n_quads_dev, sc,
+    return launch_quads(s, fitted, n_dev, c.cap_cl, ws.qtmp, out, n_quads_dev, sc,
                         8);
 }
 

@@ -36,7 +36,6 @@ struct Workspace {
     uint32_t *long_list;  // long segments of the angle and corner sorts (2 lists)
     uint32_t *tiles;
     atq_quad_corners *qtmp;
-    uint32_t *qflags;
     uint32_t *scan;
     uint32_t *hist, *digit_totals;
     atq_counts *counts;

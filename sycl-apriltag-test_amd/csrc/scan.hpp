@@ -218,17 +218,24 @@ struct ScanCtx {
 // Scans min(*count, count_cap) elements (count == nullptr: exactly
 // count_cap): o(i, exclusive prefix, value) for every element; *total
 // receives the sum (device memory).
-template <typename F, typename O>
+// ONE_ITEM: one element per thread in the look-back form, for flag
+// functors that do a whole item's work (the elements' latencies then
+// overlap across the threads instead of adding up in each).
+template <typename F, typename O, bool ONE_ITEM = false>
 int device_scan(hipStream_t s, const uint32_t *count, uint32_t count_cap, F f, O o, ScanCtx &sc,
                 uint32_t *total) {
     if (sc.flags) {
         const bool small = count_cap <= LB_SMALL_CAP;
-        const uint32_t tiles = count_cap / (small ? LB_TILE / 2 : LB_TILE) + 1;
+        const uint32_t tiles =
+            count_cap / (ONE_ITEM ? LB_THREADS : small ? LB_TILE / 2 : LB_TILE) + 1;
         if (tiles > sc.flag_count) return ATQ_ERR_CAPACITY;
         uint64_t *cur = sc.flags + (size_t)(sc.calls & 1) * sc.flag_count;
         uint64_t *next = sc.flags + (size_t)((sc.calls + 1) & 1) * sc.flag_count;
         sc.calls++;
-        if (small)
+        if (ONE_ITEM)
+            k_scan_lookback<F, O, 1><<<tiles, LB_THREADS, 0, s>>>(
+                count, count_cap, f, o, cur, next, sc.flag_count, total, sc.error);
+        else if (small)
             k_scan_lookback<F, O, LB_ITEMS / 2><<<tiles, LB_THREADS, 0, s>>>(
                 count, count_cap, f, o, cur, next, sc.flag_count, total, sc.error);
         else
