@@ -611,12 +611,85 @@ k_corner_top(atq_peak_extents *__restrict__ peaks, const atq_counts *counts, uin
     }
 }
 
+// The corners (error != 0) of the sparse slots in point order, the form of
+// the compaction scan that reads coalesced: work-group p takes slots
+// [p T, (p + 1) T), T = 256 ITEMS, thread t the slots t, t + 256, ...;
+// offsets from per-wave ballots in (item, wave, lane) order and a decoupled
+// look-back over the work-groups (dispatched in order). *n_out = the count.
+template <int ITEMS>
+__global__ void __launch_bounds__(256)
+k_corner_compact(const atq_corner *__restrict__ sparse, const uint32_t *n_dev, uint32_t cap,
+                 atq_corner *__restrict__ compacted, uint32_t *n_out, uint64_t *flags,
+                 uint64_t *next_flags, uint32_t n_flags, uint32_t *error) {
+    constexpr int NW = 256 / 64;
+    __shared__ uint32_t cnt[ITEMS][NW];
+    __shared__ uint32_t s_prefix;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n_flags; i += gridDim.x * 256)
+        next_flags[i] = 0;
+    const uint32_t N = min(*n_dev, cap);
+    constexpr uint32_t T = 256 * ITEMS;
+    const uint32_t p = blockIdx.x, b = p * T;
+    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (b >= N) {
+        if (p == 0 && t == 0) *n_out = 0;
+        return;
+    }
+    atq_corner c[ITEMS];
+    uint64_t bal[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {  // every load in flight together
+        const uint32_t x = b + t + 256 * k;
+        c[k] = x < N ? sparse[x] : atq_corner{0, 0, 0.0f};
+    }
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        bal[k] = __builtin_amdgcn_ballot_w64(c[k].error != 0.0f);
+        if (lane == 0) cnt[k][wave] = (uint32_t)__popcll(bal[k]);
+    }
+    __syncthreads();
+    uint32_t tile_sum = 0;
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++)
+#pragma unroll
+        for (int w = 0; w < NW; w++) tile_sum += cnt[k][w];
+    if (t < 64) {
+        const uint32_t excl = lb_lookback(flags, p, tile_sum, error);
+        if (t == 0) s_prefix = excl;
+    }
+    __syncthreads();
+    const uint32_t prefix = s_prefix;
+    if (t == 0 && b + T >= N) *n_out = prefix + tile_sum;
+    uint32_t before = prefix;
+    const uint64_t below = (1ull << lane) - 1;
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        uint32_t at = before;
+#pragma unroll
+        for (int w = 0; w < NW; w++) at += (uint32_t)w < wave ? cnt[k][w] : 0u;
+        if (c[k].error != 0.0f) compacted[at + (uint32_t)__popcll(bal[k] & below)] = c[k];
+#pragma unroll
+        for (int w = 0; w < NW; w++) before += cnt[k][w];
+    }
+}
+
 int launch_corner_segments(hipStream_t s, const atq_corner *sparse, const uint32_t *n_sparse_dev,
                            uint32_t cap, atq_counts *counts, uint32_t cap_cl, uint32_t max_seg,
                            atq_peak_extents *peaks, atq_corner *compacted, uint64_t *scratch,
                            SegLong lng, ScanCtx &sc, bool top_only) {
-    int r = device_scan(s, n_sparse_dev, cap, CornerFlag{sparse}, CornerOut{sparse, compacted},
-                        sc, &counts->n_corners);
+    int r = ATQ_OK;
+    constexpr int CC_ITEMS = 8;
+    const uint32_t tiles = cap / (256 * CC_ITEMS) + 1;
+    if (sc.flags && tiles <= sc.flag_count) {
+        uint64_t *cur = sc.flags + (size_t)(sc.calls & 1) * sc.flag_count;
+        uint64_t *next = sc.flags + (size_t)((sc.calls + 1) & 1) * sc.flag_count;
+        sc.calls++;
+        k_corner_compact<CC_ITEMS><<<tiles, 256, 0, s>>>(sparse, n_sparse_dev, cap, compacted,
+                                                         &counts->n_corners, cur, next,
+                                                         sc.flag_count, sc.error);
+    } else {
+        r = device_scan(s, n_sparse_dev, cap, CornerFlag{sparse}, CornerOut{sparse, compacted}, sc,
+                        &counts->n_corners);
+    }
     if (r) return r;
     r = device_scan(s, &counts->n_corners, cap, SegHead{compacted}, SegStart{peaks}, sc,
                     &counts->n_segments);
