@@ -167,8 +167,8 @@ k_boundaries_sparse(const uint32_t *__restrict__ labels, const uint32_t *__restr
 //     no more adds (the pairs that span the frame would otherwise queue on
 //     one address); such a pair fails the filter, and every pair at or below
 //     the limit is reduced exactly.
-//  2. a look-back scan over the table's slots applies ValidBlobFilter to
-//     every pair's reduction and lists the valid pairs (slot order);
+//  2. k_pair_valid applies ValidBlobFilter to every slot's reduction and
+//     lists the valid pairs (each work-group its run, by one atomic add);
 //     k_pair_rank gives each listed pair its rank r among them (by key).
 //  3. k_bnd_filter: each tile looks its pairs up and writes the records of
 //     the valid ones, keyed (r << 2 | plane), in raster order and plane
@@ -616,20 +616,23 @@ ATQ_DEV uint64_t packed_pair(uint64_t key, uint32_t lb) {
     return ((key >> 32) << lb) | (key & 0xFFFFFFFFull);
 }
 
-// Verdicts of the table's slots and the list of valid pairs, in slot order:
-// slot i holds a valid pair when its count is in [24, max_count] and its
-// reduction passes ValidBlobFilter (include/cluster_bounds.h:85-120). One
-// work-group per PV_TILE slots (2 per thread, one 8-byte count load), list
-// offsets from a look-back over the work-groups (dispatched in order);
-// counts->n_valid_pairs = the number of valid pairs, the first `cap` listed
-// with their key, slot and record count.
+// Verdicts of the table's slots and the list of valid pairs: slot i holds a
+// valid pair when its count is in [24, max_count] and its reduction passes
+// ValidBlobFilter (include/cluster_bounds.h:85-120). One work-group per
+// PV_TILE slots (2 per thread, one 8-byte count load); each work-group
+// takes its run of the list with one atomic add on counts->n_valid_pairs
+// (zero before the frame), so the list is in no particular order: the
+// ranking counts smaller keys, which does not depend on it. When more than
+// `cap` pairs are valid the ranks are not used (the full-key sort runs), so
+// which ones the list holds does not matter. n_valid_pairs ends as the
+// number of valid pairs.
 constexpr int PV_ITEMS = 2;
 constexpr uint32_t PV_TILE = 256 * PV_ITEMS;
 __global__ void __launch_bounds__(256)
 k_pair_valid(PairTable t, uint32_t w, uint32_t h, uint32_t max_count,
              uint32_t *__restrict__ slot_rank, uint64_t *__restrict__ vkeys,
              uint32_t *__restrict__ vslots, uint32_t *__restrict__ vcnt, uint32_t cap,
-             atq_counts *counts, uint64_t *flags, uint64_t *next_flags, uint32_t n_flags,
+             atq_counts *counts, uint64_t *next_flags, uint32_t n_flags,
              uint64_t *__restrict__ vtab, uint32_t vwords) {
     __shared__ uint32_t lds4[4];
     __shared__ uint32_t s_prefix;
@@ -651,13 +654,9 @@ k_pair_valid(PairTable t, uint32_t w, uint32_t h, uint32_t max_count,
     }
     uint32_t tot;
     const uint32_t ex = block_exclusive_scan<uint32_t>(sum, lds4, &tot);
-    if (threadIdx.x < 64) {
-        const uint32_t excl = lb_lookback(flags, tile, tot, &counts->overflow);
-        if (threadIdx.x == 0) s_prefix = excl;
-    }
+    if (threadIdx.x == 0) s_prefix = tot ? atomicAdd(&counts->n_valid_pairs, tot) : 0u;
     __syncthreads();
     const uint32_t prefix = s_prefix;
-    if (threadIdx.x == 0 && (tile + 1) * PV_TILE >= slots) counts->n_valid_pairs = prefix + tot;
     if (i0 >= slots) return;
     uint32_t run = prefix + ex;
 #pragma unroll
@@ -1144,12 +1143,13 @@ int launch_find_boundaries_filtered(hipStream_t s, const uint32_t *labels, const
                                                (uint32_t)min_size, table, max_count, counts,
                                                rk.sums, rk.wg_map);
     {
-        uint64_t *cur = sc.flags + (size_t)(sc.calls & 1) * sc.flag_count;
+        // no look-back here, but the scans' flag arrays keep alternating:
+        // this kernel zeroes the array the next scan uses
         uint64_t *next = sc.flags + (size_t)((sc.calls + 1) & 1) * sc.flag_count;
         sc.calls++;
         k_pair_valid<<<atq_div_up(slots, PV_TILE), 256, 0, s>>>(
             table, (uint32_t)w, (uint32_t)h, max_count, rk.slot_rank, rk.vkeys, rk.vslots, rk.vcnt,
-            rk.cap, counts, cur, next, sc.flag_count, rk.vtab, rk.vmask + 1);
+            rk.cap, counts, next, sc.flag_count, rk.vtab, rk.vmask + 1);
     }
     k_pair_rank<<<rk.cap / 64, 64 * RANK_WAVES, 0, s>>>(table, rk.vkeys, rk.vslots, rk.vcnt, counts,
                                                          rk.cap, label_bits, rk.slot_rank,
