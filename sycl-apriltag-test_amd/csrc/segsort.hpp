@@ -406,66 +406,98 @@ k_seg_merge_lds(const Ext *__restrict__ segs, SegLong lng, uint32_t max_blocks,
 }
 
 // One global merge pass over the listed long segments (those of more than
-// min_n keys), SEG_MERGE_ITEM keys per item: sorted runs of length `run`
-// pair up into runs of 2*run.
+// min_n keys): sorted runs of length `run` pair up into runs of 2*run.
+// Each item makes SEG_MERGE_ITEM consecutive outputs of a pair of runs A, B:
+// the merge-path splits of its first and last diagonal (the number of A keys
+// among the outputs before it) come from a 128-way parallel search each (a
+// few rounds of independent loads instead of a chain of dependent ones),
+// then the item's slices of A and B (together SEG_MERGE_ITEM keys) are
+// loaded into LDS coalesced, merged there four outputs per thread and
+// written out contiguously. Keys are distinct.
+// The split of diagonal d of A (na keys) and B (nb keys): the first i in
+// [max(0, d - nb), min(d, na)) with !(A[i] < B[d - i - 1]), else the upper
+// end. 128 threads (q = 0..127) probe evenly spaced i per round, so a range
+// of r + 1 candidates takes 1 + ceil(log128(r + 1)) rounds; every thread of
+// the work-group runs `rounds` rounds (two searches side by side, slot 0
+// and 1, each over two waves), so the barriers match.
+ATQ_DEV uint32_t seg_split_search(const uint64_t *A, uint32_t na, const uint64_t *B, uint32_t nb,
+                                  uint32_t d, uint32_t q, int rounds, uint32_t *cnt_lds,
+                                  int slot) {
+    uint32_t lo = d > nb ? d - nb : 0, hi = min(d, na);
+    for (int r = 0; r < rounds; r++) {
+        const uint32_t step = (hi - lo + 127) / 128;
+        const uint32_t i = lo + q * step;
+        const bool pred = hi > lo && i < hi && A[i] < B[d - i - 1];
+        const uint64_t bal = __builtin_amdgcn_ballot_w64(pred);
+        if ((threadIdx.x & 63) == 0) cnt_lds[slot * 2 + ((threadIdx.x >> 6) & 1)] = (uint32_t)__popcll(bal);
+        __syncthreads();
+        const uint32_t c = cnt_lds[slot * 2] + cnt_lds[slot * 2 + 1];
+        __syncthreads();
+        if (hi > lo) {
+            const uint32_t nlo = c ? lo + (c - 1) * step + 1 : lo;
+            hi = min(lo + c * step, hi);
+            lo = nlo;
+        }
+    }
+    return lo;
+}
+
 template <typename Ext>
 __global__ void __launch_bounds__(SEG_MERGE_NT)
 k_seg_merge(const Ext *__restrict__ segs, SegLong lng, uint32_t max_items, uint32_t run,
             const uint64_t *__restrict__ kin, uint64_t *__restrict__ kout, uint32_t min_n) {
-    constexpr int IPT = SEG_MERGE_ITEM / SEG_MERGE_NT;
+    static_assert(SEG_MERGE_NT == 256 && SEG_MERGE_ITEM == 4 * SEG_MERGE_NT, "");
+    __shared__ uint64_t buf[SEG_MERGE_ITEM];
+    __shared__ uint32_t cnt_lds[4];
+    __shared__ uint32_t split[2];
     const uint32_t nl = min(*lng.n, lng.cap);
     const uint64_t items = (uint64_t)nl * max_items;
+    const uint32_t t = threadIdx.x, half = t >> 7, q = t & 127;
     for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {  // item-major
         const Ext e = segs[lng.list[it % nl]];
-        const uint32_t n = e.count, b = (uint32_t)(it / nl) * SEG_MERGE_ITEM;
-        if (n <= run || b >= n || n <= min_n) continue;
+        const uint32_t n = e.count, ob = (uint32_t)(it / nl) * SEG_MERGE_ITEM;
+        if (n <= run || ob >= n || n <= min_n) continue;
         const uint64_t *c = kin + e.start;
-        // IPT keys per thread, their binary searches run in lockstep so the
-        // loads of one step are in flight together
-        uint32_t lo[IPT], len[IPT], j[IPT];
-        const uint64_t *arr[IPT];
-        uint64_t key[IPT];
-#pragma unroll
-        for (int k = 0; k < IPT; k++) {
-            j[k] = b + threadIdx.x + SEG_MERGE_NT * k;
-            key[k] = j[k] < n ? c[j[k]] : 0;
-        }
-#pragma unroll
-        for (int k = 0; k < IPT; k++) {
-            const uint32_t base = j[k] / (2 * run) * (2 * run);
-            lo[k] = 0;
-            len[k] = 0;
-            arr[k] = c;
-            if (j[k] >= n) continue;
-            if (j[k] - base < run) {  // left run: count the right run's smaller keys
-                const uint32_t rb = min(base + run, n), re = min(base + 2 * run, n);
-                arr[k] = c + rb;
-                len[k] = re - rb;
-            } else {                  // right run: count the left run's smaller keys
-                arr[k] = c + base;
-                len[k] = run;
+        const uint32_t base = ob / (2 * run) * (2 * run);
+        const uint32_t na = min(run, n - base), nb = n - base > run ? min(run, n - base - run) : 0u;
+        const uint64_t *A = c + base, *B = c + base + run;
+        const uint32_t d0 = ob - base, d1 = min(d0 + SEG_MERGE_ITEM, na + nb);
+        // the two splits side by side, one per half of the work-group
+        int rounds = 1;
+        for (uint32_t x = run + 1; x > 128; x = (x + 127) / 128) rounds++;
+        const uint32_t s = seg_split_search(A, na, B, nb, half ? d1 : d0, q, rounds, cnt_lds, half);
+        if (q == 0) split[half] = s;
+        __syncthreads();
+        const uint32_t i0 = split[0], i1 = split[1], j0 = d0 - i0, j1 = d1 - i1;
+        const uint32_t ma = i1 - i0, mb = j1 - j0;
+        for (uint32_t k = t; k < ma + mb; k += SEG_MERGE_NT)
+            buf[k] = k < ma ? A[i0 + k] : B[j0 + (k - ma)];
+        __syncthreads();
+        // four outputs per thread: merge path inside the item's slices
+        const uint32_t o = 4 * t;
+        if (o < ma + mb) {
+            uint32_t lo = o > mb ? o - mb : 0, hi = min(o, ma);
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (buf[mid] < buf[ma + o - mid - 1]) lo = mid + 1;
+                else hi = mid;
             }
-        }
-        for (uint32_t r = run; r; r >>= 1) {  // log2(run) + 1 halvings end every search
+            uint32_t ia = lo, ib = o - lo;
+            uint64_t out[4];
 #pragma unroll
-            for (int k = 0; k < IPT; k++) {
-                if (!len[k]) continue;
-                const uint32_t half = len[k] >> 1;
-                if (arr[k][lo[k] + half] < key[k]) {
-                    lo[k] += half + 1;
-                    len[k] -= half + 1;
-                } else {
-                    len[k] = half;
-                }
+            for (int k = 0; k < 4; k++) {
+                const uint64_t av = ia < ma ? buf[ia] : SEG_PAD;
+                const uint64_t bv = ib < mb ? buf[ma + ib] : SEG_PAD;
+                const bool ta = av < bv;
+                out[k] = ta ? av : bv;
+                ia += ta ? 1u : 0u;
+                ib += ta ? 0u : 1u;
             }
-        }
 #pragma unroll
-        for (int k = 0; k < IPT; k++) {
-            if (j[k] >= n) continue;
-            const uint32_t base = j[k] / (2 * run) * (2 * run);
-            const uint32_t pos = j[k] - base < run ? j[k] + lo[k] : j[k] - run + lo[k];
-            kout[e.start + pos] = key[k];
+            for (int k = 0; k < 4; k++)
+                if (o + k < ma + mb) kout[e.start + base + d0 + o + k] = out[k];
         }
+        __syncthreads();
     }
 }
 
