@@ -319,7 +319,7 @@ k_fit_terms(const atq_cluster_point *__restrict__ in, const uint64_t *__restrict
             const uint32_t *n_clusters_dev, uint32_t cap_cl, uint32_t max_subs,
             const uint8_t *__restrict__ grey, uint32_t w, uint32_t h,
             atq_cluster_point *__restrict__ out, uint16_t *__restrict__ wts,
-            int64_t *__restrict__ sub_sums) {
+            int64_t *__restrict__ sub_sums, uint32_t lds) {
     __shared__ int64_t part[6][4];
     const uint32_t ncl = min(*n_clusters_dev, cap_cl);
     const uint64_t items = (uint64_t)ncl * max_subs;
@@ -327,7 +327,7 @@ k_fit_terms(const atq_cluster_point *__restrict__ in, const uint64_t *__restrict
         const uint32_t c = (uint32_t)(it % ncl), sb = (uint32_t)(it / ncl);
         const uint32_t start = ext[c].start, n = ext[c].count, b = sb * FIT_SUB;
         if (b >= n) continue;
-        const uint64_t *keys = seg_sorted_keys(n, keys0, keys1) + start;
+        const uint64_t *keys = seg_sorted_keys(n, keys0, keys1, lds != 0) + start;
         uint32_t j[4], src[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
@@ -490,7 +490,8 @@ k_fit_chunks(const atq_cluster_point *__restrict__ in, atq_cluster_point *__rest
              const atq_cluster_extents *__restrict__ ext, const uint32_t *n_clusters_dev,
              uint32_t cap_cl, SegLong lng, uint32_t max_chunks, const uint8_t *__restrict__ grey,
              uint32_t w, uint32_t h, atq_line_fit_point *__restrict__ lf,
-             const int64_t *__restrict__ slot_sums, const uint16_t *__restrict__ wts) {
+             const int64_t *__restrict__ slot_sums, const uint16_t *__restrict__ wts,
+             uint32_t lds_merge) {
     __shared__ int64_t lds[6 * SEG_NT / 64 + 6];
     __shared__ int64_t stage[FIT_STAGE_WORDS];
     const uint32_t ncl = min(*n_clusters_dev, cap_cl);
@@ -501,7 +502,8 @@ k_fit_chunks(const atq_cluster_point *__restrict__ in, atq_cluster_point *__rest
         const uint32_t k = SEQ ? 0u : (uint32_t)(it / ncl);
         const uint32_t start = ext[c].start, n = ext[c].count, b = k * SEG_CHUNK;
         if (b >= n) continue;
-        const uint64_t *keys = SEQ ? nullptr : seg_sorted_keys(n, keys0, keys1) + start;
+        const uint64_t *keys =
+            SEQ ? nullptr : seg_sorted_keys(n, keys0, keys1, lds_merge != 0) + start;
         int64_t carry[6] = {0, 0, 0, 0, 0, 0}, tot[6];
         if (!SEQ && lf && k) {
             // the sums of the 4k sub-chunks before this chunk: one round of
@@ -605,11 +607,12 @@ int angle_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out
                      max_cluster, keys0, keys1, lng);
     if (r) return r;
     const uint32_t max_chunks = seg_max_chunks(max_cluster);
+    const uint32_t lds = seg_lds_merge(max_cluster) ? 1u : 0u;
     if (lf && wts) {
         const uint32_t max_subs = (max_cluster + FIT_SUB - 1) / FIT_SUB;
         k_fit_terms<FX><<<8192, 256, 0, s>>>(in, keys0, keys1, ext, &counts->n_clusters, cap_cl,
                                              max_subs ? max_subs : 1, grey, w, h, out, wts,
-                                             max_cluster > SEG_CHUNK ? slot_sums : nullptr);
+                                             max_cluster > SEG_CHUNK ? slot_sums : nullptr, lds);
     }
     // one work-group per chunk slot at most a few rounds deep: the dispatcher
     // hands the chunks that exist to whichever CU is free
@@ -617,12 +620,12 @@ int angle_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out
         k_fit_chunks<false, FX, true><<<2048, SEG_NT, 0, s>>>(in, out, keys0, keys1, ext,
                                                              &counts->n_clusters, cap_cl, lng,
                                                              max_chunks, grey, w, h, lf, slot_sums,
-                                                             wts);
+                                                             wts, lds);
     else
         k_fit_chunks<false, FX, false><<<2048, SEG_NT, 0, s>>>(in, out, keys0, keys1, ext,
                                                               &counts->n_clusters, cap_cl, lng,
                                                               max_chunks, grey, w, h, lf,
-                                                              slot_sums, nullptr);
+                                                              slot_sums, nullptr, lds);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 }  // namespace
@@ -644,7 +647,7 @@ int launch_cluster_fit(hipStream_t s, const atq_cluster_point *in, const atq_clu
                        uint32_t w, uint32_t h, atq_line_fit_point *lf) {
     k_fit_chunks<true, false, false><<<1024, SEG_NT, 0, s>>>(in, nullptr, nullptr, nullptr, ext,
                                                       n_clusters_dev, cap_cl, SegLong{}, 1, grey,
-                                                      w, h, lf, nullptr, nullptr);
+                                                      w, h, lf, nullptr, nullptr, 0u);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 

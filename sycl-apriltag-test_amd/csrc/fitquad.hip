@@ -285,7 +285,7 @@ struct CornerFinish {
 __global__ void __launch_bounds__(256)
 k_corner_emit(const atq_peak_extents *__restrict__ peaks, SegLong lng, uint32_t max_chunks,
               const uint64_t *__restrict__ keys0, const uint64_t *__restrict__ keys1,
-              atq_corner *out) {
+              atq_corner *out, uint32_t lds) {
     const uint32_t nl = min(*lng.n, lng.cap);
     const uint64_t items = (uint64_t)nl * max_chunks;
     for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {  // chunk-major
@@ -293,7 +293,7 @@ k_corner_emit(const atq_peak_extents *__restrict__ peaks, SegLong lng, uint32_t 
         const uint32_t cs = peaks[e].start, m = peaks[e].count;
         if (ch * SEG_CHUNK >= m) continue;
         const uint32_t c = out[cs].cluster_index;
-        const uint64_t *k = seg_sorted_keys(m, keys0, keys1) + cs;
+        const uint64_t *k = seg_sorted_keys(m, keys0, keys1, lds != 0) + cs;
         const uint32_t end = min(m, (ch + 1) * SEG_CHUNK);
         constexpr int PER = SEG_CHUNK / 256;
         uint64_t kk[PER];
@@ -703,7 +703,7 @@ int launch_corner_segments(hipStream_t s, const atq_corner *sparse, const uint32
                  cap_cl, max_seg, scratch, scratch + cap, lng);
     if (r || max_seg <= SEG_CHUNK) return r;
     k_corner_emit<<<256, 256, 0, s>>>(peaks, lng, seg_max_chunks(max_seg), scratch, scratch + cap,
-                                      compacted);
+                                      compacted, seg_lds_merge(max_seg) ? 1u : 0u);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 

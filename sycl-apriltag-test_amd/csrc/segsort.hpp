@@ -36,17 +36,31 @@ constexpr uint32_t SEG_LDS_IPT = SEG_LDS_KEYS / SEG_NT;
 constexpr uint32_t SEG_MERGE_NT = 256;    // global passes: keys per work-group item
 constexpr uint32_t SEG_MERGE_ITEM = 4 * SEG_MERGE_NT;
 
-// Global merge passes a segment of n > SEG_CHUNK keys takes.
-ATQ_DEV uint32_t seg_global_passes(uint32_t n) {
+// Whether the long segments of a sort bounded by max_seg are merged in LDS
+// first (SEG_CHUNK runs into SEG_LDS_KEYS blocks, one work-group each) or
+// by global merge passes from SEG_CHUNK on: the LDS merge pays when many
+// long segments share the device (frames whose clusters can pass
+// SEG_LDS_KEYS points), the global passes when few do (they spread one
+// segment over many work-groups).
+__host__ __device__ inline bool seg_lds_merge(uint32_t max_seg) { return max_seg > SEG_LDS_KEYS; }
+
+// Global merge passes a segment of n > SEG_CHUNK keys takes after the LDS
+// merge (runs of SEG_LDS_KEYS up) or without it (runs of SEG_CHUNK up).
+ATQ_DEV uint32_t seg_global_passes(uint32_t n, bool lds) {
     uint32_t q = 0;
-    for (uint32_t r = SEG_LDS_KEYS; r < n; r <<= 1) q++;
+    for (uint32_t r = lds ? SEG_LDS_KEYS : SEG_CHUNK; r < n; r <<= 1) q++;
     return q;
 }
 
-// Where the sorted keys of a long segment (n > SEG_CHUNK) end.
+// Where the sorted keys of a segment of n keys end (lds = seg_lds_merge of
+// the sort's bound): keys1 when it fits one chunk (the Finish functor
+// writes them there); otherwise after an odd number of hops keys0 -> keys1
+// -> keys0 ... keys1, else keys0 (the LDS merge is one hop).
 template <typename T>
-ATQ_DEV T *seg_sorted_keys(uint32_t n, T *keys0, T *keys1) {
-    return (seg_global_passes(n) & 1) ? keys0 : keys1;
+ATQ_DEV T *seg_sorted_keys(uint32_t n, T *keys0, T *keys1, bool lds) {
+    if (n <= SEG_CHUNK) return keys1;
+    const uint32_t hops = seg_global_passes(n, lds) + (lds ? 1u : 0u);
+    return (hops & 1) ? keys1 : keys0;
 }
 
 inline uint32_t seg_max_chunks(uint32_t max_seg) {
@@ -514,19 +528,22 @@ int seg_sort_chunks(hipStream_t s, KeyGen gen, Finish fin, const Ext *segs,
 }
 
 // The merges of the listed segments of more than min_n keys after
-// seg_sort_chunks (their sorted keys end in seg_sorted_keys(n, keys0, keys1)).
+// seg_sort_chunks (their sorted keys end in seg_sorted_keys(n, keys0, keys1,
+// seg_lds_merge(max_seg))).
 template <typename Ext>
 int seg_sort_long(hipStream_t s, const Ext *segs, uint32_t max_seg, uint64_t *keys0,
                   uint64_t *keys1, SegLong lng, uint32_t min_n = 0) {
-    if (max_seg > SEG_CHUNK && max_seg > min_n)
+    const bool lds = seg_lds_merge(max_seg);
+    if (lds && max_seg > min_n)
         k_seg_merge_lds<Ext><<<256, SEG_NT, 0, s>>>(
             segs, lng, (max_seg + SEG_LDS_KEYS - 1) / SEG_LDS_KEYS, keys0, keys1, min_n);
+    // hop h reads keys0 when h is even (the chunks, or the LDS merge's input)
     const uint32_t max_items = (max_seg + SEG_MERGE_ITEM - 1) / SEG_MERGE_ITEM;
-    int pass = 0;
-    for (uint32_t run = SEG_LDS_KEYS; run < max_seg; run <<= 1, pass++)
-        k_seg_merge<Ext><<<512, SEG_MERGE_NT, 0, s>>>(segs, lng, max_items, run,
-                                                        (pass & 1) ? keys0 : keys1,
-                                                        (pass & 1) ? keys1 : keys0, min_n);
+    uint32_t hop = lds ? 1u : 0u;
+    for (uint32_t run = lds ? SEG_LDS_KEYS : SEG_CHUNK; run < max_seg; run <<= 1, hop++)
+        k_seg_merge<Ext><<<1024, SEG_MERGE_NT, 0, s>>>(segs, lng, max_items, run,
+                                                         (hop & 1) ? keys1 : keys0,
+                                                         (hop & 1) ? keys0 : keys1, min_n);
     return hipGetLastError() == hipSuccess ? ATQ_OK : ATQ_ERR_HIP;
 }
 
