@@ -599,15 +599,16 @@ int angle_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out
               const atq_cluster_extents *ext, const atq_counts *counts, uint32_t cap_cl,
               uint32_t max_cluster, const uint8_t *grey, uint32_t w, uint32_t h,
               atq_line_fit_point *lf, uint64_t *keys0, uint64_t *keys1, int64_t *slot_sums,
-              SegLong lng, uint32_t *cidx, uint16_t *wts) {
+              SegLong lng, uint32_t *cidx, uint16_t *wts, bool throughput) {
     // the sort (every cluster's keys where seg_sorted_keys puts them), then
     // the points in angle order with their weights, then the fit of every
     // chunk of every cluster
+    const bool lds_merge = throughput || seg_lds_merge(max_cluster);
     int r = seg_sort(s, AngleKey{in, cidx}, AngleStore{keys1}, ext, &counts->n_clusters, cap_cl,
-                     max_cluster, keys0, keys1, lng);
+                     max_cluster, keys0, keys1, lng, lds_merge);
     if (r) return r;
     const uint32_t max_chunks = seg_max_chunks(max_cluster);
-    const uint32_t lds = seg_lds_merge(max_cluster) ? 1u : 0u;
+    const uint32_t lds = lds_merge ? 1u : 0u;
     if (lf && wts) {
         const uint32_t max_subs = (max_cluster + FIT_SUB - 1) / FIT_SUB;
         k_fit_terms<FX><<<8192, 256, 0, s>>>(in, keys0, keys1, ext, &counts->n_clusters, cap_cl,
@@ -634,12 +635,12 @@ int launch_angle_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_poi
                      const atq_cluster_extents *ext, const atq_counts *counts, uint32_t cap_cl,
                      uint32_t max_cluster, const uint8_t *grey, uint32_t w, uint32_t h,
                      atq_line_fit_point *lf, uint64_t *keys0, uint64_t *keys1, int64_t *slot_sums,
-                     SegLong lng, bool fx, uint32_t *cidx, uint16_t *wts) {
+                     SegLong lng, bool fx, uint32_t *cidx, uint16_t *wts, bool throughput) {
     if (lf && !wts) return ATQ_ERR_BAD_ARG;
     return fx ? angle_fit<true>(s, in, out, ext, counts, cap_cl, max_cluster, grey, w, h, lf,
-                                keys0, keys1, slot_sums, lng, cidx, wts)
+                                keys0, keys1, slot_sums, lng, cidx, wts, throughput)
               : angle_fit<false>(s, in, out, ext, counts, cap_cl, max_cluster, grey, w, h, lf,
-                                 keys0, keys1, slot_sums, lng, cidx, wts);
+                                 keys0, keys1, slot_sums, lng, cidx, wts, throughput);
 }
 
 int launch_cluster_fit(hipStream_t s, const atq_cluster_point *in, const atq_cluster_extents *ext,

@@ -31,13 +31,15 @@ int launch_assign(hipStream_t s, const atq_boundary_point *pts, const uint32_t *
 // entries whose count is zero. fx: fixed mode (record decode, weight row).
 // cidx (may be null) receives every point's cluster index. wts: cap
 // entries of scratch for the points' line-fit weights (required with lf).
+// throughput: the frame shares the device with others (a batch), so the
+// long clusters merge in LDS whatever their bound (seg_lds_merge).
 constexpr uint32_t FIT_SUB = 1024;  // the line fit's sub-chunks (a quarter of SEG_CHUNK)
 int launch_angle_fit(hipStream_t s, const atq_cluster_point *in, atq_cluster_point *out,
                      const atq_cluster_extents *ext, const atq_counts *counts, uint32_t cap_cl,
                      uint32_t max_cluster, const uint8_t *grey, uint32_t w, uint32_t h,
                      atq_line_fit_point *lf, uint64_t *keys0, uint64_t *keys1, int64_t *slot_sums,
                      SegLong lng, bool fx = false, uint32_t *cidx = nullptr,
-                     uint16_t *wts = nullptr);
+                     uint16_t *wts = nullptr, bool throughput = false);
 // Line-fit prefix sums of points already in angle order (one work-group per
 // cluster).
 int launch_cluster_fit(hipStream_t s, const atq_cluster_point *in, const atq_cluster_extents *ext,

@@ -700,7 +700,7 @@ int launch_corner_segments(hipStream_t s, const atq_corner *sparse, const uint32
     }
     k_peak_close<<<min(64u, atq_div_up(cap_cl, 256) + 1), 256, 0, s>>>(counts, cap_cl, peaks);
     r = seg_sort(s, CornerKey{compacted}, CornerFinish{compacted}, peaks, &counts->n_segments,
-                 cap_cl, max_seg, scratch, scratch + cap, lng);
+                 cap_cl, max_seg, scratch, scratch + cap, lng, seg_lds_merge(max_seg));
     if (r || max_seg <= SEG_CHUNK) return r;
     k_corner_emit<<<256, 256, 0, s>>>(peaks, lng, seg_max_chunks(max_seg), scratch, scratch + cap,
                                       compacted, seg_lds_merge(max_seg) ? 1u : 0u);

@@ -286,7 +286,7 @@ int run_back(atq_detector *d, hipStream_t s, const uint8_t *gray_in) {
     TRY(launch_angle_fit(s, ws.cpts_tmp, d->cpts, d->cext, cnt, c.cap_cl, max_cluster, grey,
                          (uint32_t)w, (uint32_t)h, d->lf, ak, ak + c.cap, ws.fit_sums,
                          ws.long_segs(0, &d->long_n[0], c), fx, d->cidx,
-                         reinterpret_cast<uint16_t *>(ws.valid)));
+                         reinterpret_cast<uint16_t *>(ws.valid), d->in_batch));
     TRY(mark(1 + ATQ_STAGE_ANGLE_FIT));
     TRY(launch_fit_lines(s, d->lf, d->cidx, d->cext, &cnt->n_cluster_points, c.cap, d->sparse,
                          nullptr, reinterpret_cast<float *>(ws.valid), fx));
@@ -840,6 +840,7 @@ int atq_batch_create(size_t n_frames, const size_t *widths_in, const size_t *hei
         atq_params p{decimate ? decimate[i] : 1, 5, 25, 8, 0};
         atq_detector *d = nullptr;
         rc = atq_detector_create(widths_in[i], heights_in[i], &p, &d);
+        if (d) d->in_batch = true;
         b->dets.push_back(d);
         uint8_t *in = nullptr;
         if (rc == ATQ_OK && hipMalloc(&in, widths_in[i] * heights_in[i]) != hipSuccess) rc = ATQ_ERR_HIP;

@@ -98,5 +98,9 @@ struct atq_detector {
     int channels;  // of the last frame's input: 1 grey, 3 RGB, 4 RGBA
     bool sorted_in_alt;
     bool timing;
+    // a batch's frame: several frames share the device, so the long
+    // clusters' angle sort keeps the work-efficient LDS merge (one read and
+    // write of each cluster) over the latency-saving global passes
+    bool in_batch = false;
     hipEvent_t ev[ATQ_N_STAGES + 1];
 };

@@ -40,8 +40,8 @@ constexpr uint32_t SEG_MERGE_ITEM = 4 * SEG_MERGE_NT;
 // first (SEG_CHUNK runs into SEG_LDS_KEYS blocks, one work-group each) or
 // by global merge passes from SEG_CHUNK on: the LDS merge pays when many
 // long segments share the device (frames whose clusters can pass
-// SEG_LDS_KEYS points), the global passes when few do (they spread one
-// segment over many work-groups).
+// SEG_LDS_KEYS points, or several frames at once), the global passes when
+// few do (they spread one segment over many work-groups, for latency).
 __host__ __device__ inline bool seg_lds_merge(uint32_t max_seg) { return max_seg > SEG_LDS_KEYS; }
 
 // Global merge passes a segment of n > SEG_CHUNK keys takes after the LDS
@@ -528,12 +528,11 @@ int seg_sort_chunks(hipStream_t s, KeyGen gen, Finish fin, const Ext *segs,
 }
 
 // The merges of the listed segments of more than min_n keys after
-// seg_sort_chunks (their sorted keys end in seg_sorted_keys(n, keys0, keys1,
-// seg_lds_merge(max_seg))).
+// seg_sort_chunks, in LDS first when lds (their sorted keys end in
+// seg_sorted_keys(n, keys0, keys1, lds)).
 template <typename Ext>
 int seg_sort_long(hipStream_t s, const Ext *segs, uint32_t max_seg, uint64_t *keys0,
-                  uint64_t *keys1, SegLong lng, uint32_t min_n = 0) {
-    const bool lds = seg_lds_merge(max_seg);
+                  uint64_t *keys1, SegLong lng, bool lds, uint32_t min_n = 0) {
     if (lds && max_seg > min_n)
         k_seg_merge_lds<Ext><<<256, SEG_NT, 0, s>>>(
             segs, lng, (max_seg + SEG_LDS_KEYS - 1) / SEG_LDS_KEYS, keys0, keys1, min_n);
@@ -553,10 +552,11 @@ int seg_sort_long(hipStream_t s, const Ext *segs, uint32_t max_seg, uint64_t *ke
 // SEG_CHUNK.
 template <typename KeyGen, typename Finish, typename Ext>
 int seg_sort(hipStream_t s, KeyGen gen, Finish fin, const Ext *segs, const uint32_t *n_segs_dev,
-             uint32_t cap_segs, uint32_t max_seg, uint64_t *keys0, uint64_t *keys1, SegLong lng) {
+             uint32_t cap_segs, uint32_t max_seg, uint64_t *keys0, uint64_t *keys1, SegLong lng,
+             bool lds) {
     if (int r = seg_sort_chunks(s, gen, fin, segs, n_segs_dev, cap_segs, max_seg, keys0, lng))
         return r;
-    return seg_sort_long(s, segs, max_seg, keys0, keys1, lng);
+    return seg_sort_long(s, segs, max_seg, keys0, keys1, lng, lds);
 }
 
 }  // namespace atq
