@@ -194,6 +194,18 @@ ATQ_DEV void seg_merge_sort(uint64_t *a, uint64_t *b, uint32_t m) {
     }
 }
 
+// Orders a wave's LDS accesses before and after it (a wave's LDS operations
+// complete in issue order, so data one lane wrote is there for the others).
+ATQ_DEV void seg_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Positions one wave holds in the blocked layout of the chunk sorts: merges
+// of runs up to half of it read only what the same wave wrote.
+constexpr uint32_t SEG_WAVE_KEYS = 64 * SEG_IPT;
+
 // KeyGen::position_low, false when absent.
 template <typename T, typename = void>
 struct SegPositionLow : std::false_type {};
@@ -236,7 +248,7 @@ ATQ_DEV void seg_sort_positions(const uint64_t (&g)[SEG_IPT], uint32_t m, uint32
             K0[d0 + i] = (uint32_t)(r[i] >> 32);
             J0[d0 + i] = (uint16_t)r[i];
         }
-    __syncthreads();
+    seg_wave_sync();  // the merges of small runs stay within the wave
     uint32_t *Ks = K0, *Kd = K1;
     uint16_t *Js = J0, *Jd = J1;
     for (uint32_t run = SEG_IPT; run < P; run <<= 1) {
@@ -275,7 +287,8 @@ ATQ_DEV void seg_sort_positions(const uint64_t (&g)[SEG_IPT], uint32_t m, uint32
                 Jd[d0 + k] = oj[k];
             }
         }
-        __syncthreads();
+        if (4 * run <= SEG_WAVE_KEYS) seg_wave_sync();  // the next merge is the wave's own
+        else __syncthreads();
         uint32_t *tk = Ks;
         Ks = Kd;
         Kd = tk;
